@@ -1,0 +1,144 @@
+"""Benchmark of the PBF particle substep on one MI355X.
+
+Workload: Large PBF -- 4,194,304 fluid particles, uniform random fill of a 4 x 2 x 2 m box at the
+tight-pack rest number density (r = 8.6 mm, h = 28.7 mm = cell size), zero velocity, static box
+walls, 64 neighbor slots, dt = 1/120 s, 4 substeps x 4 solver iterations, FP32.  A step is one
+cn_step (one frame = 4 substeps); the metric counts particle-substeps.
+
+Prints ONE JSON line with value (particle-substeps/s), roofline (dominant kernel, algorithmic
+bytes over its HIP-event-timed average launch), substep_roofline (whole substep against the HBM
+peak) and cpu_baseline (the CPU oracle on a 262,144-particle slice of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cressim-neo_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 4
+ITERS = 4
+CAP = 64
+
+
+def algorithmic_bytes(kbar: float) -> dict:
+    """Compulsory HBM bytes per particle (per launch for the kernels, per substep for the total).
+
+    Per particle and substep (I = 4 iterations): predict 80, key build 24, 8-bit LSD sort of
+    (key, index) 68, cell table 8, neighbor list 4 + 4K, per iteration density
+    32 + 4K + 16 and delta 48 + 4K + 16 and apply/clamp 48, velocity update 96 and boundary
+    projection 48: 968 + 36K bytes.  The fused neighbor+density kernel moves 32 (own position and
+    delta) + 4K (list write) + 4 (count) + 16 (constraint) = 52 + 4K bytes per particle; the fused
+    delta+apply+clamp kernel moves 48 + 4K + 16 + 48 = 112 + 4K.
+    """
+    return {"substep": 968.0 + 36.0 * kbar, "fluid_density": 52.0 + 4.0 * kbar,
+            "fluid_delta": 112.0 + 4.0 * kbar}
+
+
+def cpu_baseline(n_sample: int, seed: int = 2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cn_oracle  # CPU oracle: the reported baseline only
+    import cnhip
+    from cnhip import scenes
+    sc = scenes.large_pbf(n_sample, seed=seed, box=scenes.scaled_box_for(n_sample))
+    p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
+    osc = cn_oracle.OracleScene(sc)
+    t0 = time.perf_counter()
+    cn_oracle.step(osc, p)
+    el = time.perf_counter() - t0
+    return {"value": n_sample * SUBSTEPS / el, "unit": "particle-substeps/s", "cores": cn_oracle.threads(),
+            "kind": "port",
+            "sample": f"1 step (4 substeps x 4 iterations) of a {n_sample}-particle random fill at the same "
+                      f"density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4_194_304)
+    ap.add_argument("--key-mode", choices=["morton", "hash"], default="morton")
+    ap.add_argument("--cpu-sample", type=int, default=262_144)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import cnhip
+    from cnhip import scenes
+
+    sc = scenes.large_pbf(args.n)
+    km = cnhip.CN_KEY_MORTON if args.key_mode == "morton" else cnhip.CN_KEY_HASH
+    p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP,
+                             key_mode=km)
+    s = cnhip.Solver(0)
+    s.reserve(sc.n, CAP)
+    s.upload(sc)
+    print(f"bench: {sc.n} particles uploaded; warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s.step(p)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(p)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    meta = s.meta()
+    kbar = meta.fluid_neighbor_total / sc.n
+    value = sc.n * SUBSTEPS * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+    print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}",
+          file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timing on the context stream, in separate steps
+    s.profile(True)
+    for _ in range(args.profile_steps):
+        s.step(p)
+    prof = s.profile_read()
+    s.profile(False)
+    s.close()
+    bytes_pp = algorithmic_bytes(kbar)
+    avg = {k: (ms / max(cnt, 1)) for k, (cnt, ms) in prof.items()}
+    dom = max(("fluid_density", "fluid_delta"), key=lambda k: prof.get(k, (0, 0.0))[1])
+    achieved = bytes_pp[dom] * sc.n / (avg[dom] * 1e-3) / 1e9
+    sub_ms = ms_per_step / SUBSTEPS
+    sub_ach = bytes_pp["substep"] * sc.n / (sub_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_fluid_density_large_pbf_4m.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "particle-substeps/s at 4M PBF (%HBM roofline); env-steps/s + p99 at 512 scenes",
+        "value": value, "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "large_pbf_4194304_random_fill_4x2x2m", "particles": sc.n,
+                   "substeps_per_step": SUBSTEPS, "iterations": ITERS, "neighbor_cap": CAP,
+                   "key_mode": args.key_mode, "mean_neighbors": round(kbar, 3),
+                   "max_neighbors": meta.max_fluid_neighbor_count},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_particle": bytes_pp[dom], "avg_launch_ms": avg[dom]},
+        "substep_roofline": {"bound": "hbm", "achieved": sub_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": sub_ach / HBM_PEAK_GBS, "bytes_per_particle_substep": bytes_pp["substep"],
+                             "ms_per_substep": sub_ms},
+        "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+    }
+    if not args.no_cpu:
+        print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,228 @@
+"""Python front of libcn_hip.so, the gfx950 particle substep behind the CRESSim-Neo solver seam.
+
+``Solver`` mirrors the shape of the reference's ``PhysicsSolver`` (include/physics/physics_solver.h
+:150-205): construct with a descriptor, upload a world, ``step()``, read back.  ``Scene`` mirrors the
+unified particle pool (``ParticleSoAHost``, include/physics/physics_types.h:858) plus the tables the
+solver reads.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .abi import (CN_KEY_HASH, CN_KEY_MORTON, CN_PARTICLE_KIND_FLUID, CN_PARTICLE_KIND_SOLID,
+                  CnError, CnLibraryError, load_library)
+
+__all__ = ["Scene", "Solver", "default_params", "fluid_material", "CnError", "CnLibraryError",
+           "CN_KEY_HASH", "CN_KEY_MORTON", "CN_PARTICLE_KIND_FLUID", "CN_PARTICLE_KIND_SOLID",
+           "load_library"]
+
+FLUID_MATERIAL_DTYPE = np.dtype([(n, np.float32) for n, _ in abi.FluidMaterial._fields_])
+COLLIDER_DTYPE = np.dtype([("center", np.float32, 3), ("half_extents", np.float32, 3),
+                           ("material", np.float32, 4), ("environment", np.uint32),
+                           ("layer", np.uint32), ("mask", np.uint32), ("enabled", np.uint32),
+                           ("reserved", np.uint32, 2)])
+assert COLLIDER_DTYPE.itemsize == C.sizeof(abi.BoxCollider) == 64
+assert FLUID_MATERIAL_DTYPE.itemsize == C.sizeof(abi.FluidMaterial) == 48
+
+
+@dataclass
+class Scene:
+    """Unified particle pool and scene tables (host arrays, particle-index order)."""
+    pos: np.ndarray                     # (n, 4) float32 xyz + inverse mass
+    vel: np.ndarray                     # (n, 4) float32
+    radii: np.ndarray                   # (n,) float32
+    kinds: np.ndarray                   # (n,) uint32
+    fluid_mat: np.ndarray               # (n,) uint32
+    part_mat: np.ndarray                # (n,) uint32
+    env: np.ndarray                     # (n,) uint32
+    phase: np.ndarray                   # (n,) uint32
+    layer: np.ndarray                   # (n,) uint32
+    mask: np.ndarray                    # (n,) uint32
+    fluid_mats: np.ndarray              # structured FLUID_MATERIAL_DTYPE
+    part_mats: np.ndarray               # (m, 4) float32
+    colliders: np.ndarray               # structured COLLIDER_DTYPE
+    name: str = "scene"
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    def meta(self) -> np.ndarray:
+        return np.ascontiguousarray(np.stack([self.env, self.phase, self.layer, self.mask], 1)
+                                    .astype(np.uint32))
+
+    def copy(self) -> "Scene":
+        import copy
+        return copy.deepcopy(self)
+
+
+def default_params(**overrides) -> abi.StepParams:
+    """PhysicsSolverDesc defaults (include/physics/physics_solver.h:21-143) as cn_step_params."""
+    lib = load_library()
+    p = abi.StepParams()
+    lib.cn_default_step_params(C.byref(p))
+    for k, v in overrides.items():
+        if k == "gravity":
+            for i, g in enumerate(v):
+                p.gravity[i] = g
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def fluid_material(radius: float, viscosity=0.01, cohesion=0.0, surface_tension=0.0,
+                   vorticity_confinement=0.0, gravity_scale=1.0, cfl_coefficient=1.0) -> np.ndarray:
+    """toFluidSolverMaterial (src/physics/physics_world.cpp:233-263) through the library."""
+    lib = load_library()
+    d = abi.FluidMaterialDesc(viscosity, cohesion, surface_tension, vorticity_confinement,
+                              gravity_scale, cfl_coefficient)
+    out = abi.FluidMaterial()
+    st = lib.cn_fluid_material_from_desc(C.byref(d), C.c_float(radius), C.byref(out))
+    if st != abi.CN_OK:
+        raise CnError(st, "cn_fluid_material_from_desc failed")
+    arr = np.zeros(1, FLUID_MATERIAL_DTYPE)
+    C.memmove(arr.ctypes.data, C.byref(out), C.sizeof(out))
+    return arr
+
+
+class Solver:
+    """One HIP context with one resident scene: PhysicsSolver-shaped driver over the C ABI."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self._ctx = C.c_void_p()
+        desc = abi.DeviceDesc(device, 0)
+        st = self.lib.cn_ctx_create(C.byref(desc), C.byref(self._ctx))
+        if st != abi.CN_OK:
+            raise CnError(st, f"cn_ctx_create failed on device {device}")
+        self.scene: Scene | None = None
+        self._keep = []
+
+    # -- helpers
+    def _check(self, st: int, what: str):
+        if st != abi.CN_OK:
+            msg = self.lib.cn_last_error(self._ctx)
+            raise CnError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._ctx:
+            self.lib.cn_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- scene
+    def reserve(self, n: int, max_fluid_neighbors: int = 128):
+        self._check(self.lib.cn_scene_reserve(self._ctx, n, max_fluid_neighbors), "cn_scene_reserve")
+
+    def upload(self, sc: Scene):
+        self.scene = sc
+        arrs = dict(
+            pos=np.ascontiguousarray(sc.pos, np.float32), vel=np.ascontiguousarray(sc.vel, np.float32),
+            radii=np.ascontiguousarray(sc.radii, np.float32),
+            kinds=np.ascontiguousarray(sc.kinds, np.uint32),
+            fm=np.ascontiguousarray(sc.fluid_mat, np.uint32),
+            pm=np.ascontiguousarray(sc.part_mat, np.uint32),
+            env=np.ascontiguousarray(sc.env, np.uint32), ph=np.ascontiguousarray(sc.phase, np.uint32),
+            ly=np.ascontiguousarray(sc.layer, np.uint32), mk=np.ascontiguousarray(sc.mask, np.uint32),
+            fmats=np.ascontiguousarray(sc.fluid_mats), pmats=np.ascontiguousarray(sc.part_mats, np.float32),
+            cols=np.ascontiguousarray(sc.colliders))
+        p = abi.ptr
+        view = abi.ParticleSoaView(sc.n, p(arrs["pos"]), p(arrs["vel"]), p(arrs["radii"]), p(arrs["kinds"]),
+                                   p(arrs["fm"]), p(arrs["pm"]), p(arrs["env"]), p(arrs["ph"]),
+                                   p(arrs["ly"]), p(arrs["mk"]))
+        tables = abi.SceneTables(len(arrs["fmats"]), p(arrs["fmats"]), len(arrs["pmats"]), p(arrs["pmats"]),
+                                 len(arrs["cols"]), p(arrs["cols"]))
+        self._check(self.lib.cn_scene_upload(self._ctx, C.byref(view), C.byref(tables), None), "cn_scene_upload")
+
+    def step(self, params: abi.StepParams):
+        self._check(self.lib.cn_step(self._ctx, C.byref(params)), "cn_step")
+
+    def synchronize(self):
+        self._check(self.lib.cn_synchronize(self._ctx), "cn_synchronize")
+
+    def readback(self):
+        n = self.scene.n
+        pos = np.zeros((n, 4), np.float32)
+        prev = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        self._check(self.lib.cn_readback(self._ctx, pos.ctypes.data, prev.ctypes.data, vel.ctypes.data, n),
+                    "cn_readback")
+        return pos, prev, vel
+
+    def meta(self, check: bool = False) -> abi.Meta:
+        m = abi.Meta()
+        st = self.lib.cn_validate_meta(self._ctx, C.byref(m))
+        if check:
+            self._check(st, "cn_validate_meta")
+        elif st not in (abi.CN_OK, abi.CN_ERR_CAPACITY):
+            self._check(st, "cn_validate_meta")
+        return m
+
+    def device_views(self) -> abi.DeviceViews:
+        v = abi.DeviceViews()
+        self._check(self.lib.cn_device_views_get(self._ctx, C.byref(v)), "cn_device_views_get")
+        return v
+
+    # -- per-pass entries
+    def pass_keys(self, pos: np.ndarray, cell: float):
+        n = pos.shape[0]
+        pos = np.ascontiguousarray(pos, np.float32)
+        keys = np.zeros(n, np.uint32)
+        cells = np.zeros((n, 3), np.int32)
+        self._check(self.lib.cn_pass_broadphase_keys(self._ctx, n, pos.ctypes.data, C.c_float(cell),
+                                                     keys.ctypes.data, cells.ctypes.data), "keys")
+        return keys, cells
+
+    def pass_sort(self, keys: np.ndarray, idx: np.ndarray):
+        k = np.ascontiguousarray(keys, np.uint32).copy()
+        v = np.ascontiguousarray(idx, np.uint32).copy()
+        self._check(self.lib.cn_pass_sort_pairs(self._ctx, len(k), k.ctypes.data, v.ctypes.data), "sort")
+        return k, v
+
+    def pass_cell_ranges(self, sorted_keys: np.ndarray, capacity: int):
+        k = np.ascontiguousarray(sorted_keys, np.uint32)
+        out = np.zeros((capacity, 4), np.uint32)
+        self._check(self.lib.cn_pass_cell_ranges(self._ctx, len(k), k.ctypes.data, capacity, out.ctypes.data),
+                    "cell ranges")
+        return out
+
+    def pass_substep_broadphase(self, params: abi.StepParams):
+        n = self.scene.n
+        cap = params.max_fluid_neighbors
+        keys = np.zeros(n, np.uint32)
+        ids = np.zeros(n, np.uint32)
+        counts = np.zeros(n, np.uint32)
+        lists = np.zeros((n, max(cap, 1)), np.uint32)
+        self._check(self.lib.cn_pass_substep_broadphase(self._ctx, C.byref(params), keys.ctypes.data,
+                                                        ids.ctypes.data, counts.ctypes.data,
+                                                        lists.ctypes.data), "substep broadphase")
+        return keys, ids, counts, lists
+
+    # -- profiling
+    def profile(self, enable: bool):
+        self._check(self.lib.cn_profile_enable(self._ctx, 1 if enable else 0), "profile")
+
+    def profile_read(self) -> dict:
+        s = abi.KernelStats()
+        self._check(self.lib.cn_profile_read(self._ctx, C.byref(s)), "profile read")
+        out = {}
+        for k in range(s.used):
+            name = bytes(s.name[k]).split(b"\0")[0].decode()
+            out[name] = (int(s.launches[k]), float(s.total_ms[k]))
+        return out

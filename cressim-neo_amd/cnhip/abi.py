@@ -1,0 +1,173 @@
+"""ctypes mirror of include/cn_hip.h and loader for libcn_hip.so.
+
+The library is looked up next to this package (``cressim-neo_amd/libcn_hip.so``).  A missing or
+unloadable library raises ``CnLibraryError``: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcn_hip.so")
+
+CN_OK = 0
+CN_ERR_INVALID_ARGUMENT = 1
+CN_ERR_OUT_OF_MEMORY = 2
+CN_ERR_DEVICE = 3
+CN_ERR_NOT_READY = 4
+CN_ERR_CAPACITY = 5
+CN_ERR_UNSUPPORTED = 6
+
+CN_PARTICLE_KIND_SOLID = 0
+CN_PARTICLE_KIND_FLUID = 1
+CN_KEY_MORTON = 0
+CN_KEY_HASH = 1
+CN_SOLVER_CONFIG_FLOATS = 40
+CN_KERNEL_SLOTS = 16
+
+
+class CnLibraryError(RuntimeError):
+    pass
+
+
+class CnError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"cn status {status}: {message}")
+        self.status = status
+
+
+class FluidMaterial(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "rest_density", "inv_rest_density", "smoothing_radius", "density_constraint_scale",
+        "viscosity", "cohesion", "cohesion1", "cohesion2", "surface_tension",
+        "vorticity_confinement", "gravity_scale", "cfl_radius")]
+
+
+class FluidMaterialDesc(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "viscosity", "cohesion", "surface_tension", "vorticity_confinement", "gravity_scale",
+        "cfl_coefficient")]
+
+
+class BoxCollider(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("half_extents", C.c_float * 3),
+                ("material", C.c_float * 4), ("environment", C.c_uint32),
+                ("layer", C.c_uint32), ("mask", C.c_uint32), ("enabled", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class ParticleSoaView(C.Structure):
+    _fields_ = [("count", C.c_uint32),
+                ("positions_inv_mass", C.c_void_p), ("velocities", C.c_void_p),
+                ("radii", C.c_void_p), ("kinds", C.c_void_p),
+                ("fluid_material_indices", C.c_void_p),
+                ("particle_material_indices", C.c_void_p),
+                ("environment_indices", C.c_void_p), ("phases", C.c_void_p),
+                ("collision_layers", C.c_void_p), ("collision_masks", C.c_void_p)]
+
+
+class SceneTables(C.Structure):
+    _fields_ = [("fluid_material_count", C.c_uint32), ("fluid_materials", C.c_void_p),
+                ("particle_material_count", C.c_uint32), ("particle_materials", C.c_void_p),
+                ("collider_count", C.c_uint32), ("colliders", C.c_void_p)]
+
+
+class ConstraintView(C.Structure):
+    _fields_ = [("edge_count", C.c_uint32), ("edges", C.c_void_p),
+                ("bend_count", C.c_uint32), ("bends", C.c_void_p),
+                ("dihedral_count", C.c_uint32), ("dihedrals", C.c_void_p),
+                ("tet_count", C.c_uint32), ("tets", C.c_void_p)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("substeps", C.c_uint32), ("default_iterations", C.c_uint32),
+                ("fluid_iterations", C.c_uint32), ("soft_internal_iterations", C.c_uint32),
+                ("soft_contact_iterations", C.c_uint32), ("gravity", C.c_float * 4),
+                ("solver_config", C.c_float * CN_SOLVER_CONFIG_FLOATS),
+                ("cell_size", C.c_float), ("max_fluid_neighbors", C.c_uint32),
+                ("key_mode", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Meta(C.Structure):
+    _fields_ = [("max_fluid_neighbor_count", C.c_uint32),
+                ("fluid_neighbor_truncations", C.c_uint32),
+                ("max_cell_occupancy", C.c_uint32), ("verlet_rebuilds", C.c_uint32),
+                ("fluid_neighbor_total", C.c_uint64), ("last_particle_count", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+
+class DeviceViews(C.Structure):
+    _fields_ = [("positions_inv_mass", C.c_void_p), ("velocities", C.c_void_p),
+                ("count", C.c_uint32), ("stream", C.c_void_p)]
+
+
+class DeviceDesc(C.Structure):
+    _fields_ = [("device_ordinal", C.c_int), ("flags", C.c_uint32)]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("name", (C.c_char * 32) * CN_KERNEL_SLOTS),
+                ("launches", C.c_uint32 * CN_KERNEL_SLOTS),
+                ("total_ms", C.c_double * CN_KERNEL_SLOTS), ("used", C.c_uint32)]
+
+
+# every symbol include/cn_hip.h declares, with its ctypes signature
+_P = C.c_void_p
+_U = C.c_uint32
+SIGNATURES = {
+    "cn_abi_version": (_U, []),
+    "cn_ctx_create": (C.c_int, [C.POINTER(DeviceDesc), C.POINTER(_P)]),
+    "cn_ctx_destroy": (None, [_P]),
+    "cn_last_error": (C.c_char_p, [_P]),
+    "cn_default_step_params": (None, [C.POINTER(StepParams)]),
+    "cn_fluid_material_from_desc": (C.c_int, [C.POINTER(FluidMaterialDesc), C.c_float,
+                                               C.POINTER(FluidMaterial)]),
+    "cn_particle_grid_cell_size": (C.c_float, [_U, _P, _P, _U, _P]),
+    "cn_scene_reserve": (C.c_int, [_P, _U, _U]),
+    "cn_scene_upload": (C.c_int, [_P, C.POINTER(ParticleSoaView), C.POINTER(SceneTables),
+                                  C.POINTER(ConstraintView)]),
+    "cn_step": (C.c_int, [_P, C.POINTER(StepParams)]),
+    "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
+    "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),
+    "cn_validate_meta": (C.c_int, [_P, C.POINTER(Meta)]),
+    "cn_synchronize": (C.c_int, [_P]),
+    "cn_pass_broadphase_keys": (C.c_int, [_P, _U, _P, C.c_float, _P, _P]),
+    "cn_pass_sort_pairs": (C.c_int, [_P, _U, _P, _P]),
+    "cn_pass_cell_ranges": (C.c_int, [_P, _U, _P, _U, _P]),
+    "cn_pass_substep_broadphase": (C.c_int, [_P, C.POINTER(StepParams), _P, _P, _P, _P]),
+    "cn_profile_enable": (C.c_int, [_P, C.c_int]),
+    "cn_profile_read": (C.c_int, [_P, C.POINTER(KernelStats)]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> C.CDLL:
+    """Load libcn_hip.so and bind every C-ABI symbol; raises CnLibraryError if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise CnLibraryError(f"{p} not found: build the HIP library first (__graft_entry__.build())")
+    try:
+        lib = C.CDLL(p)
+    except OSError as e:  # pragma: no cover - depends on the ROCm runtime being present
+        raise CnLibraryError(f"cannot load {p}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed across the C ABI must be contiguous"
+    return a.ctypes.data

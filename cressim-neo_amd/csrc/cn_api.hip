@@ -1,0 +1,803 @@
+// cn_api.hip -- host side of libcn_hip.so: context, scene state, the substep driver and the C ABI.
+//
+// The driver follows PhysicsSolver::step (src/physics/physics_solver.cpp:336-1401) for the particle
+// path: per substep predict + broadphase, then the position-solve iterations, then the velocity
+// update and the fluid boundary velocity projection.  Every pass is a kernel on the context's
+// stream; nothing here synchronizes except readback, validation and profiling reads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cn_internal.h"
+
+using namespace cn;
+
+struct cn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    SceneBuffers s;
+    uint32_t capacity = 0;       // particle capacity of the allocation
+    uint32_t cap_alloc = 0;      // neighbor slots per particle of the allocation
+    bool uploaded = false;
+    float cell_size = 0.0f;      // derived on upload like the reference world
+    uint32_t morton_bits = 10;
+    uint32_t bucket_bits_alloc = 4;  // bucket index bits the allocation holds
+    // profiling
+    bool profiling = false;
+    struct Ev {
+        int slot;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    cn_kernel_stats stats{};
+    cn_meta last_meta{};
+};
+
+namespace {
+
+cn_status fail(cn_ctx *ctx, cn_status st, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return st;
+}
+
+#define CN_HIP(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), CN_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count)
+{
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+uint32_t ceil_log2(uint64_t v)
+{
+    uint32_t b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
+
+void free_scene(SceneBuffers &s)
+{
+    for (int k = 0; k < 2; ++k) {
+        dfree(s.pos[k]);
+        dfree(s.vel[k]);
+        dfree(s.cls[k]);
+        dfree(s.id[k]);
+        dfree(s.kv[k]);
+        dfree(s.dacc[k]);
+    }
+    dfree(s.prev);
+    dfree(s.cellc);
+    dfree(s.bnd);
+    dfree(s.sort_scratch);
+    dfree(s.elem_run);
+    dfree(s.run_key);
+    dfree(s.run_start);
+    dfree(s.run_info);
+    dfree(s.bucket);
+    dfree(s.block_runs);
+    dfree(s.run_count);
+    dfree(s.perm);
+    dfree(s.nc);
+    dfree(s.nbr);
+    dfree(s.nbr_cnt);
+    dfree(s.out_pos);
+    dfree(s.out_vel);
+    dfree(s.out_prev);
+    dfree(s.classes);
+    dfree(s.fluid_mats);
+    dfree(s.part_mats);
+    dfree(s.colliders);
+    dfree(s.meta);
+    s = SceneBuffers{};
+}
+
+// host restatement of the static-collider visit order: enabled colliders, Morton codes of their
+// broad-phase AABB centres in the static extent (physics_rigid_morton_codes.cs.hlsl:8-50), stable
+// order, walked right child first (physics_particle_emit_fluid_boundary_candidates.cs.hlsl:158-225)
+uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+std::vector<Collider> visit_ordered_colliders(const cn_box_collider *c, uint32_t nc)
+{
+    struct E {
+        uint32_t idx;
+        float mn[3], mx[3];
+        uint32_t code;
+    };
+    std::vector<E> es;
+    for (uint32_t i = 0; i < nc; ++i) {
+        if (!c[i].enabled) continue;
+        E e;
+        e.idx = i;
+        for (int a = 0; a < 3; ++a) {
+            e.mn[a] = (c[i].center[a] - c[i].half_extents[a]) - kBroadPhaseMargin;
+            e.mx[a] = (c[i].center[a] + c[i].half_extents[a]) + kBroadPhaseMargin;
+        }
+        es.push_back(e);
+    }
+    float gmn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+    float gmx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+    for (const E &e : es)
+        for (int a = 0; a < 3; ++a) {
+            gmn[a] = std::min(gmn[a], e.mn[a]);
+            gmx[a] = std::max(gmx[a], e.mx[a]);
+        }
+    for (E &e : es) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) {
+            const float size = std::max(gmx[a] - gmn[a], 1.0e-5f);
+            const float center = 0.5f * (e.mn[a] + e.mx[a]);
+            float m = (center - gmn[a]) / size;
+            m = std::min(std::max(m, 0.0f), 1.0f);
+            float s = m * 1024.0f;
+            s = std::min(std::max(s, 0.0f), 1023.0f);
+            q[a] = static_cast<uint32_t>(s);
+        }
+        e.code = expand10(q[0]) * 4u + expand10(q[1]) * 2u + expand10(q[2]);
+    }
+    std::stable_sort(es.begin(), es.end(), [](const E &x, const E &y) { return x.code < y.code; });
+    std::vector<Collider> out;
+    for (auto it = es.rbegin(); it != es.rend(); ++it) {
+        const cn_box_collider &b = c[it->idx];
+        Collider k{};
+        for (int a = 0; a < 3; ++a) {
+            k.c[a] = b.center[a];
+            k.he[a] = b.half_extents[a];
+            k.aabb_min[a] = it->mn[a];
+            k.aabb_max[a] = it->mx[a];
+        }
+        for (int a = 0; a < 4; ++a) k.material[a] = b.material[a];
+        k.env = b.environment;
+        k.layer = b.layer;
+        k.mask = b.mask;
+        k.enabled = b.enabled;
+        k.index = it->idx;
+        out.push_back(k);
+    }
+    return out;
+}
+
+uint32_t resolve_iters(uint32_t v, uint32_t d) { return v > 0 ? v : d; }
+
+struct ProfScope {
+    cn_ctx *ctx;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(cn_ctx *c, const char *name) : ctx(c), slot(-1)
+    {
+        if (!ctx->profiling) return;
+        cn_kernel_stats &st = ctx->stats;
+        for (uint32_t k = 0; k < st.used; ++k)
+            if (std::strncmp(st.name[k], name, 31) == 0) slot = (int)k;
+        if (slot < 0 && st.used < CN_KERNEL_SLOTS) {
+            slot = (int)st.used++;
+            std::snprintf(st.name[slot], 32, "%s", name);
+        }
+        if (slot < 0) return;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, ctx->stream);
+    }
+    ~ProfScope()
+    {
+        if (slot < 0 || !a) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->events.push_back({slot, a, b});
+    }
+};
+
+SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
+{
+    SubstepConsts c{};
+    c.dt = dt;
+    const float cell = p->cell_size > 0.0f ? p->cell_size : ctx->cell_size;
+    c.inv_cell = 1.0f / std::max(cell, 1.0e-4f);
+    c.gravity[0] = p->gravity[0];
+    c.gravity[1] = p->gravity[1];
+    c.gravity[2] = p->gravity[2];
+    c.n = ctx->s.n;
+    c.cap = p->max_fluid_neighbors;
+    c.ncolliders = ctx->s.ncolliders;
+    c.key_mode = p->key_mode;
+    c.morton_bits = ctx->morton_bits;
+    const uint32_t key_bits = p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
+    c.bucket_shift = key_bits - ctx->s.bucket_bits;
+    c.uniform_class = ctx->s.nclasses == 1 ? 1u : 0u;
+    std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
+    return c;
+}
+
+uint32_t key_bits_of(const cn_ctx *ctx, const cn_step_params *p)
+{
+    return p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
+}
+
+// predict + keys + sort + runs + reorder for one substep; returns the sorted (key, old slot) pairs
+uint2 *broadphase(cn_ctx *ctx, const SubstepConsts &c, uint32_t key_bits)
+{
+    SceneBuffers &s = ctx->s;
+    {
+        ProfScope ps(ctx, "keys");
+        launch_keys(s, c, ctx->stream);
+    }
+    uint2 *sorted;
+    {
+        ProfScope ps(ctx, "sort");
+        sorted = radix_sort_pairs(s.kv[0], s.kv[1], c.n, key_bits, s.sort_scratch, ctx->stream);
+    }
+    {
+        ProfScope ps(ctx, "runs");
+        launch_runs(s, c, sorted, key_bits, ctx->stream);
+    }
+    {
+        ProfScope ps(ctx, "reorder");
+        launch_reorder(s, c, ctx->stream);
+    }
+    return sorted;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t cn_abi_version(void) { return CN_ABI_VERSION; }
+
+cn_status cn_ctx_create(const cn_device_desc *desc, cn_ctx **out)
+{
+    if (!out) return CN_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    cn_ctx *ctx = new cn_ctx();
+    ctx->device = desc ? desc->device_ordinal : 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= ctx->device) {
+        delete ctx;
+        return CN_ERR_DEVICE;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return CN_ERR_DEVICE;
+    }
+    *out = ctx;
+    return CN_OK;
+}
+
+void cn_ctx_destroy(cn_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    free_scene(ctx->s);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *cn_last_error(const cn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void cn_default_step_params(cn_step_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // PhysicsSolverDesc defaults, include/physics/physics_solver.h:21-143, in makeSolverConfig
+    // order (src/physics/physics_solver.cpp:100-129)
+    p->dt = 1.0f / 60.0f;
+    p->substeps = 1;
+    p->default_iterations = 20;
+    p->gravity[1] = -9.81f;
+    const float cfg[CN_SOLVER_CONFIG_FLOATS] = {1.0e-3f, 4.0f,    0.95f,  0.05f,  0.5f,  2.0f,  0.0f,  1.0e-5f,
+                                                0.25f,   0.90f,   0.10f,  1.0f,   0.01f, 0.10f, 0.8f,  0.5f,
+                                                0.95f,   0.7f,    0.05f,  0.02f,  0.12f, 1.0e-5f, 5.0e-5f, 1.0e-5f,
+                                                0.2f,    0.0f,    0.0f,   0.0f,   0.0f,  0.0f,  0.0f,  0.0f,
+                                                0.7f,    1.0f,    0.12f,  0.08f,  0.0f,  0.0f,  0.0f,  0.0f};
+    std::memcpy(p->solver_config, cfg, sizeof(cfg));
+    p->max_fluid_neighbors = 128;  // kDefaultFluidMaxNeighborhood, src/physics/rigid_body_common.h:304
+    p->key_mode = CN_KEY_MORTON;
+}
+
+// toFluidSolverMaterial and its helpers, src/physics/physics_world.cpp:99-263
+cn_status cn_fluid_material_from_desc(const cn_fluid_material_desc *desc, float r, cn_fluid_material *out)
+{
+    if (!desc || !out) return CN_ERR_INVALID_ARGUMENT;
+    const float pi = 3.14159265358979323846f;
+    const float rest = std::max(2.0f * r, 1.0e-4f);
+    const float h = std::max((2.0f / 0.6f) * r, rest);
+    auto w = [&](float d) {
+        const float omq = 1.0f - d / h;
+        const float k = 15.0f / (pi * h * h * h);
+        return k * omq * omq;
+    };
+    auto dw = [&](float d) {
+        const float omq = 1.0f - d / h;
+        const float k = 15.0f / (pi * h * h * h);
+        return -k * 2.0f * omq / h;
+    };
+    // tightPack3D: hexagonal-ish pack of spacing `rest` inside radius h, at most 2048 samples
+    const int dim = static_cast<int>(std::ceil(h / rest));
+    const float row = std::sqrt(0.75f) * rest;
+    float rho = 0.0f, dcs = 0.0f, a = 0.0f, b = 0.0f;
+    uint32_t count = 0;
+    bool full = false;
+    for (int z = -dim; z <= dim && !full; ++z)
+        for (int y = -dim; y <= dim && !full; ++y)
+            for (int x = -dim; x <= dim; ++x) {
+                const float sx = static_cast<float>(x) * rest + (((y + z) & 1) != 0 ? rest * 0.5f : 0.0f);
+                const float sy = static_cast<float>(y) * row;
+                const float sz = static_cast<float>(z) * row;
+                const float m2 = sx * sx + sy * sy + sz * sz;
+                if (m2 == 0.0f || !(std::sqrt(m2) <= h)) continue;
+                if (count == 2048u) {
+                    full = true;
+                    break;
+                }
+                ++count;
+                const float dist = std::sqrt(sx * sx + sy * sy + sz * sz);
+                rho += w(dist);
+                const float der = dw(dist);
+                dcs += der * der;
+                if (sy <= 0.0f && dist > 0.0f) {
+                    const float ct = sy / dist;
+                    a += der * ct;
+                    b -= dist * ct;
+                }
+            }
+    if (rho <= 1.0e-6f) rho = 1.0f;
+    if (dcs <= 1.0e-6f) dcs = 1.0f;
+    const float sts = (std::fabs(b) > 1.0e-6f && std::fabs(a) > 1.0e-6f) ? std::fabs(a / b) : 1.0f;
+    const float inv = 1.0f / rho;
+    const float rr = std::min(std::max(rest / h, 1.0e-3f), 0.999f);
+    const float rr2 = rr * rr;
+    out->rest_density = rho;
+    out->inv_rest_density = inv;
+    out->smoothing_radius = h;
+    out->density_constraint_scale = 1.0f / std::max(dcs, 1.0e-6f);
+    out->viscosity = desc->viscosity * inv;
+    out->cohesion = desc->cohesion * h;
+    out->cohesion1 = -(1.0f + rr) / rr2;
+    out->cohesion2 = (rr2 + rr + 1.0f) / rr2;
+    out->surface_tension = (desc->surface_tension * inv) / sts;
+    out->vorticity_confinement = desc->vorticity_confinement * inv * h;
+    out->gravity_scale = desc->gravity_scale;
+    out->cfl_radius = desc->cfl_coefficient * h;
+    return CN_OK;
+}
+
+float cn_particle_grid_cell_size(uint32_t n, const float *radii, const uint32_t *fmi, uint32_t nm,
+                                 const cn_fluid_material *fm)
+{
+    float g = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        g = std::max(g, radii[i] * 2.0f);
+        if (fmi && fmi[i] != 0xffffffffu && fmi[i] < nm) g = std::max(g, fm[fmi[i]].smoothing_radius);
+    }
+    return std::max(g, 1.0e-4f);
+}
+
+cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (n <= ctx->capacity && cap <= ctx->cap_alloc && ctx->s.pos[0]) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_scene(ctx->s);
+    ctx->uploaded = false;
+    SceneBuffers &s = ctx->s;
+    const size_t N = n ? n : 1;
+    for (int k = 0; k < 2; ++k) {
+        CN_HIP(ctx, dalloc(&s.pos[k], N));
+        CN_HIP(ctx, dalloc(&s.vel[k], N));
+        CN_HIP(ctx, dalloc(&s.cls[k], N));
+        CN_HIP(ctx, dalloc(&s.id[k], N));
+        CN_HIP(ctx, dalloc(&s.kv[k], N));
+        CN_HIP(ctx, dalloc(&s.dacc[k], N));
+    }
+    CN_HIP(ctx, dalloc(&s.prev, N));
+    CN_HIP(ctx, dalloc(&s.cellc, N));
+    CN_HIP(ctx, dalloc(&s.bnd, N));
+    CN_HIP(ctx, dalloc(&s.sort_scratch, sort_scratch_words((uint32_t)N)));
+    CN_HIP(ctx, dalloc(&s.elem_run, N));
+    CN_HIP(ctx, dalloc(&s.run_key, N));
+    CN_HIP(ctx, dalloc(&s.run_start, N + 1));
+    CN_HIP(ctx, dalloc(&s.run_info, N));
+    const uint32_t bb = std::max<uint32_t>(4u, ceil_log2(N) > 1 ? ceil_log2(N) - 1 : 1);
+    s.bucket_bits = std::min<uint32_t>(bb, 30u);
+    ctx->bucket_bits_alloc = s.bucket_bits;
+    CN_HIP(ctx, dalloc(&s.bucket, (size_t(1) << s.bucket_bits) + 1));
+    CN_HIP(ctx, dalloc(&s.block_runs, (N + 1023) / 1024 + 1));
+    CN_HIP(ctx, dalloc(&s.run_count, 1));
+    CN_HIP(ctx, dalloc(&s.perm, N));
+    CN_HIP(ctx, dalloc(&s.nc, N));
+    CN_HIP(ctx, dalloc(&s.nbr, N * (size_t)(cap ? cap : 1)));
+    CN_HIP(ctx, dalloc(&s.nbr_cnt, N));
+    CN_HIP(ctx, dalloc(&s.out_pos, N));
+    CN_HIP(ctx, dalloc(&s.out_vel, N));
+    CN_HIP(ctx, dalloc(&s.out_prev, N));
+    CN_HIP(ctx, dalloc(&s.meta, 8));
+    CN_HIP(ctx, hipMemsetAsync(s.meta, 0, 8 * sizeof(uint32_t), ctx->stream));
+    CN_HIP(ctx, hipMemsetAsync(s.dacc[0], 0, N * sizeof(float4), ctx->stream));
+    CN_HIP(ctx, hipMemsetAsync(s.dacc[1], 0, N * sizeof(float4), ctx->stream));
+    ctx->capacity = n;
+    ctx->cap_alloc = cap;
+    s.cap = cap;
+    return CN_OK;
+}
+
+cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_scene_tables *tb,
+                          const cn_constraint_view *cv)
+{
+    if (!ctx || !pv || !tb) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    const uint32_t n = pv->count;
+    if (n == 0) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "empty particle pool");
+    if (!pv->positions_inv_mass || !pv->velocities || !pv->radii || !pv->kinds || !pv->particle_material_indices)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "missing particle arrays");
+    if (cv && (cv->edge_count || cv->bend_count || cv->dihedral_count || cv->tet_count))
+        return fail(ctx, CN_ERR_UNSUPPORTED, "soft constraints are not supported by this build");
+    if (tb->collider_count > kMaxColliders)
+        return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+    if (n > ctx->capacity || !ctx->s.pos[0]) {
+        const cn_status st = cn_scene_reserve(ctx, n, std::max<uint32_t>(ctx->cap_alloc, 128u));
+        if (st != CN_OK) return st;
+    }
+    SceneBuffers &s = ctx->s;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s.n = n;
+    s.cur = 0;
+
+    // particle classes
+    std::map<std::array<uint32_t, 8>, uint32_t> cmap;
+    std::vector<ParticleClass> classes;
+    std::vector<uint32_t> cls(n);
+    std::array<uint32_t, 8> last{};
+    uint32_t last_id = 0xffffffffu;
+    for (uint32_t i = 0; i < n; ++i) {
+        ParticleClass pc{};
+        pc.radius = pv->radii[i];
+        pc.kind = pv->kinds[i];
+        pc.fluid_mat = pv->fluid_material_indices ? pv->fluid_material_indices[i] : 0xffffffffu;
+        pc.part_mat = pv->particle_material_indices[i];
+        pc.env = pv->environment_indices ? pv->environment_indices[i] : 0u;
+        pc.phase = pv->phases ? pv->phases[i] : 0u;
+        pc.layer = pv->collision_layers ? pv->collision_layers[i] : 1u;
+        pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
+        if (pc.kind == CN_PARTICLE_KIND_FLUID && pc.fluid_mat >= tb->fluid_material_count)
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid particle with an invalid fluid material index");
+        if (pc.part_mat >= tb->particle_material_count)
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "invalid particle material index");
+        std::array<uint32_t, 8> key;
+        std::memcpy(key.data(), &pc, sizeof(pc));
+        if (last_id != 0xffffffffu && key == last) {
+            cls[i] = last_id;
+            continue;
+        }
+        auto it = cmap.find(key);
+        uint32_t id;
+        if (it == cmap.end()) {
+            id = (uint32_t)classes.size();
+            cmap.emplace(key, id);
+            classes.push_back(pc);
+        } else {
+            id = it->second;
+        }
+        cls[i] = id;
+        last = key;
+        last_id = id;
+    }
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+
+    dfree(s.classes);
+    dfree(s.fluid_mats);
+    dfree(s.part_mats);
+    dfree(s.colliders);
+    s.classes = nullptr;
+    s.fluid_mats = nullptr;
+    s.part_mats = nullptr;
+    s.colliders = nullptr;
+    s.nclasses = (uint32_t)classes.size();
+    CN_HIP(ctx, dalloc(&s.classes, classes.size()));
+    CN_HIP(ctx, hipMemcpy(s.classes, classes.data(), classes.size() * sizeof(ParticleClass), hipMemcpyHostToDevice));
+    CN_HIP(ctx, dalloc(&s.fluid_mats, tb->fluid_material_count));
+    if (tb->fluid_material_count)
+        CN_HIP(ctx, hipMemcpy(s.fluid_mats, tb->fluid_materials, tb->fluid_material_count * sizeof(cn_fluid_material),
+                              hipMemcpyHostToDevice));
+    CN_HIP(ctx, dalloc(&s.part_mats, tb->particle_material_count));
+    CN_HIP(ctx, hipMemcpy(s.part_mats, tb->particle_materials, tb->particle_material_count * sizeof(float4),
+                          hipMemcpyHostToDevice));
+    const std::vector<Collider> col = visit_ordered_colliders(tb->colliders, tb->collider_count);
+    s.ncolliders = (uint32_t)col.size();
+    CN_HIP(ctx, dalloc(&s.colliders, col.size()));
+    if (!col.empty())
+        CN_HIP(ctx, hipMemcpy(s.colliders, col.data(), col.size() * sizeof(Collider), hipMemcpyHostToDevice));
+
+    CN_HIP(ctx, hipMemcpy(s.pos[0], pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.vel[0], pv->velocities, n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.cls[0], cls.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.id[0], ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_pos, pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_vel, pv->velocities, n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_prev, pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
+
+    // grid cell size (recomputeParticleGridCellSize) and Morton resolution from the scene extent
+    std::vector<cn_fluid_material> fmh(tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
+    ctx->cell_size = cn_particle_grid_cell_size(n, pv->radii, pv->fluid_material_indices, tb->fluid_material_count,
+                                                fmh.data());
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pv->positions_inv_mass[4 * i + a]);
+            hi[a] = std::max(hi[a], pv->positions_inv_mass[4 * i + a]);
+        }
+    for (const Collider &c : col)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], c.aabb_min[a]);
+            hi[a] = std::max(hi[a], c.aabb_max[a]);
+        }
+    double span = 1.0;
+    for (int a = 0; a < 3; ++a) span = std::max(span, (double)(hi[a] - lo[a]) / (double)ctx->cell_size + 3.0);
+    ctx->morton_bits = std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)std::ceil(span))));
+    const uint32_t kb = 3u * ctx->morton_bits;
+    const uint32_t bb = std::max<uint32_t>(4u, ceil_log2(n) > 1 ? ceil_log2(n) - 1 : 1);
+    // bucket bits cannot exceed the key bits of either key mode; the allocation fits the largest
+    s.bucket_bits = std::min<uint32_t>(std::min<uint32_t>(bb, kb), ctx->bucket_bits_alloc);
+    ctx->uploaded = true;
+    return CN_OK;
+}
+
+cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
+{
+    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step before cn_scene_upload");
+    if (p->key_mode != CN_KEY_MORTON && p->key_mode != CN_KEY_HASH)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "unknown key_mode");
+    if (p->max_fluid_neighbors > ctx->cap_alloc)
+        return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    SceneBuffers &s = ctx->s;
+    const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
+    const uint32_t def = std::max<uint32_t>(p->default_iterations, 1u);
+    const uint32_t fluid_iters = resolve_iters(p->fluid_iterations, def);
+    const float dt = p->dt / static_cast<float>(substeps);
+    const SubstepConsts c = make_consts(ctx, p, dt);
+    const uint32_t key_bits = key_bits_of(ctx, p);
+    CN_HIP(ctx, hipMemsetAsync(s.meta, 0, 8 * sizeof(uint32_t), ctx->stream));
+    for (uint32_t sub = 0; sub < substeps; ++sub) {
+        broadphase(ctx, c, key_bits);
+        int din = 0;
+        for (uint32_t it = 0; it < fluid_iters; ++it) {
+            {
+                ProfScope ps(ctx, "fluid_density");
+                launch_fluid_density(s, c, it, din, ctx->stream);
+            }
+            {
+                ProfScope ps(ctx, "fluid_delta");
+                launch_fluid_delta(s, c, it, din, ctx->stream);
+            }
+            din = 1 - din;
+        }
+        {
+            ProfScope ps(ctx, "finalize");
+            launch_finalize(s, c, din, ctx->stream);
+        }
+    }
+    {
+        ProfScope ps(ctx, "export");
+        launch_export(s, ctx->stream);
+    }
+    CN_HIP(ctx, hipGetLastError());
+    return CN_OK;
+}
+
+cn_status cn_readback(cn_ctx *ctx, float *pos, float *prev, float *vel, uint32_t count)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (count > ctx->s.n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "count exceeds particle count");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (pos) CN_HIP(ctx, hipMemcpy(pos, ctx->s.out_pos, count * sizeof(float4), hipMemcpyDeviceToHost));
+    if (prev) CN_HIP(ctx, hipMemcpy(prev, ctx->s.out_prev, count * sizeof(float4), hipMemcpyDeviceToHost));
+    if (vel) CN_HIP(ctx, hipMemcpy(vel, ctx->s.out_vel, count * sizeof(float4), hipMemcpyDeviceToHost));
+    return CN_OK;
+}
+
+cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out)
+{
+    if (!ctx || !out) return CN_ERR_INVALID_ARGUMENT;
+    out->positions_inv_mass = ctx->s.out_pos;
+    out->velocities = ctx->s.out_vel;
+    out->count = ctx->s.n;
+    out->stream = ctx->stream;
+    return CN_OK;
+}
+
+cn_status cn_synchronize(cn_ctx *ctx)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CN_OK;
+}
+
+cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t m[8];
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CN_HIP(ctx, hipMemcpy(m, ctx->s.meta, sizeof(m), hipMemcpyDeviceToHost));
+    cn_meta r{};
+    r.max_fluid_neighbor_count = m[0];
+    r.fluid_neighbor_truncations = m[1];
+    uint64_t tot;
+    std::memcpy(&tot, &m[2], sizeof(tot));
+    r.fluid_neighbor_total = tot;
+    r.last_particle_count = ctx->s.n;
+    if (out) *out = r;
+    if (r.fluid_neighbor_truncations)
+        return fail(ctx, CN_ERR_CAPACITY, "fluid neighbor list truncated at max_fluid_neighbors");
+    return CN_OK;
+}
+
+cn_status cn_pass_broadphase_keys(cn_ctx *ctx, uint32_t n, const float *pos, float cell, uint32_t *keys,
+                                  int32_t *cells)
+{
+    if (!ctx || !pos || !keys || !cells) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    float4 *dp = nullptr;
+    uint32_t *dk = nullptr;
+    int32_t *dc = nullptr;
+    CN_HIP(ctx, dalloc(&dp, n));
+    CN_HIP(ctx, dalloc(&dk, n));
+    CN_HIP(ctx, dalloc(&dc, 3ull * n));
+    CN_HIP(ctx, hipMemcpy(dp, pos, n * sizeof(float4), hipMemcpyHostToDevice));
+    launch_ref_keys(dp, n, 1.0f / std::max(cell, 1.0e-4f), dk, dc, ctx->stream);
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CN_HIP(ctx, hipMemcpy(keys, dk, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    CN_HIP(ctx, hipMemcpy(cells, dc, 3ull * n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    dfree(dp);
+    dfree(dk);
+    dfree(dc);
+    return CN_OK;
+}
+
+cn_status cn_pass_sort_pairs(cn_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *indices)
+{
+    if (!ctx || !keys || !indices) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (n <= 1) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<uint2> h(n);
+    for (uint32_t i = 0; i < n; ++i) h[i] = make_uint2(keys[i], indices[i]);
+    uint2 *a = nullptr, *b = nullptr;
+    uint32_t *scr = nullptr;
+    CN_HIP(ctx, dalloc(&a, n));
+    CN_HIP(ctx, dalloc(&b, n));
+    CN_HIP(ctx, dalloc(&scr, sort_scratch_words(n)));
+    CN_HIP(ctx, hipMemcpy(a, h.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
+    uint2 *res = radix_sort_pairs(a, b, n, 32u, scr, ctx->stream);
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CN_HIP(ctx, hipMemcpy(h.data(), res, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+        keys[i] = h[i].x;
+        indices[i] = h[i].y;
+    }
+    dfree(a);
+    dfree(b);
+    dfree(scr);
+    return CN_OK;
+}
+
+cn_status cn_pass_cell_ranges(cn_ctx *ctx, uint32_t n, const uint32_t *sorted_keys, uint32_t capacity,
+                              uint32_t *ranges)
+{
+    if (!ctx || !ranges || (n && !sorted_keys)) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (capacity == 0) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t *dk = nullptr, *dr = nullptr;
+    CN_HIP(ctx, dalloc(&dk, n));
+    CN_HIP(ctx, dalloc(&dr, 4ull * capacity + n));
+    if (n) CN_HIP(ctx, hipMemcpy(dk, sorted_keys, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    launch_ref_cell_ranges(dk, n, capacity, dr, ctx->stream);
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CN_HIP(ctx, hipMemcpy(ranges, dr, 4ull * capacity * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    dfree(dk);
+    dfree(dr);
+    return CN_OK;
+}
+
+cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint32_t *sorted_keys,
+                                     uint32_t *sorted_ids, uint32_t *counts, uint32_t *lists)
+{
+    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (p->max_fluid_neighbors > ctx->cap_alloc)
+        return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    SceneBuffers &s = ctx->s;
+    const uint32_t n = s.n;
+    const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
+    const SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
+    const int old_cur = s.cur;
+    uint2 *sorted = broadphase(ctx, c, key_bits_of(ctx, p));
+    launch_fluid_density(s, c, 0, 0, ctx->stream);
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<uint2> kv(n);
+    std::vector<uint32_t> id(n), cnt(n);
+    CN_HIP(ctx, hipMemcpy(kv.data(), sorted, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    CN_HIP(ctx, hipMemcpy(id.data(), s.id[s.cur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    CN_HIP(ctx, hipMemcpy(cnt.data(), s.nbr_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t cap = p->max_fluid_neighbors;
+    if (sorted_keys)
+        for (uint32_t t = 0; t < n; ++t) sorted_keys[t] = kv[t].x;
+    if (sorted_ids) std::memcpy(sorted_ids, id.data(), n * sizeof(uint32_t));
+    if (counts || lists) {
+        std::vector<uint32_t> nb((size_t)n * (cap ? cap : 1));
+        if (cap) CN_HIP(ctx, hipMemcpy(nb.data(), s.nbr, (size_t)n * cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < n; ++t) {
+            const uint32_t d = id[t];
+            if (counts) counts[d] = cnt[t];
+            if (lists)
+                for (uint32_t k = 0; k < cnt[t] && k < cap; ++k) lists[(size_t)d * cap + k] = id[nb[(size_t)k * n + t]];
+        }
+    }
+    s.cur = old_cur;
+    return CN_OK;
+}
+
+cn_status cn_profile_enable(cn_ctx *ctx, int enable)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    ctx->profiling = enable != 0;
+    return CN_OK;
+}
+
+cn_status cn_profile_read(cn_ctx *ctx, cn_kernel_stats *out)
+{
+    if (!ctx || !out) return CN_ERR_INVALID_ARGUMENT;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &e : ctx->events) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            ctx->stats.total_ms[e.slot] += ms;
+            ctx->stats.launches[e.slot] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    ctx->events.clear();
+    *out = ctx->stats;
+    return CN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// cn_device.h -- device-side types and per-element math shared by the gfx950 kernels.
+//
+// Every formula here restates one HLSL helper of the reference and keeps its operation order
+// (the library is built with -ffp-contract=off, and divisions / square roots are the correctly
+// rounded IEEE operations: '/' lowers to the div_scale/div_fmas/div_fixup sequence and
+// __builtin_sqrtf to the corrected v_sqrt sequence, whereas sqrtf()/__fsqrt_rn lower to the
+// 1-ulp v_sqrt_f32 on gfx950), so a kernel and the CPU oracle produce the same float32 bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cn_hip.h"
+
+namespace cn {
+
+constexpr float kEps = 1.0e-6f;              // kEpsilon, physics_base.hlsli:7
+constexpr uint32_t kInvalid = 0xffffffffu;   // kInvalidIndex, physics_base.hlsli:6
+constexpr float kKernelPi = 3.14159265359f;  // physics_fluid_common.hlsli:26
+constexpr float kBroadPhaseMargin = 0.05f;   // physics_shape_common.hlsli:10
+constexpr uint32_t kMaxColliders = 32;       // colliders a particle's candidate bitmask can name
+constexpr int kBlock = 256;                  // threads per workgroup for per-particle kernels
+
+// GpuPhysicsSolverConfig float indices (physics_solver_config.hlsli:19-51)
+constexpr int kCfgContactSlop = 0;
+constexpr int kCfgPosFrictionMin = 7;
+constexpr int kCfgSoftRelax = 24;
+constexpr int kCfgFluidSolve = 32;
+constexpr int kCfgFluidRelax = 33;
+constexpr int kCfgFluidBndDensity = 34;
+constexpr int kCfgFluidBndDelta = 35;
+constexpr int kCfgFluidMaxUnder = 36;
+
+// Static per-particle attributes, deduplicated on upload; each particle stores a class index.
+struct ParticleClass {
+    float radius;
+    uint32_t kind;
+    uint32_t fluid_mat;
+    uint32_t part_mat;
+    uint32_t env;
+    uint32_t phase;
+    uint32_t layer;
+    uint32_t mask;
+};
+
+// A static box collider in LBVH visit order, with its broad-phase AABB precomputed.
+struct Collider {
+    float c[3];
+    float he[3];
+    float aabb_min[3];
+    float aabb_max[3];
+    float material[4];
+    uint32_t env, layer, mask, enabled;
+    uint32_t index;  // collider index in the caller's table
+    uint32_t pad[3];
+};
+
+// Solver-wide scalars of one substep.
+struct SubstepConsts {
+    float dt;
+    float inv_cell;       // 1 / max(cell, 1e-4)
+    float gravity[3];
+    uint32_t n;
+    uint32_t cap;         // neighbor slots per particle
+    uint32_t ncolliders;  // colliders in visit order
+    uint32_t key_mode;
+    uint32_t morton_bits; // bits per axis of the Morton key
+    uint32_t bucket_shift;
+    uint32_t uniform_class;  // 1: every particle has class 0
+    float cfg[CN_SOLVER_CONFIG_FLOATS];
+};
+
+__device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ float fmaxd(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fmind(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fsat(float a) { return fmind(fmaxd(a, 0.0f), 1.0f); }
+__device__ __forceinline__ float fclamp(float x, float lo, float hi) { return fmind(fmaxd(x, lo), hi); }
+
+// dot(a, a) then sqrt: HLSL length(), evaluated as ((x*x + y*y) + z*z).
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return __fadd_rn(__fadd_rn(__fmul_rn(ax, bx), __fmul_rn(ay, by)), __fmul_rn(az, bz));
+}
+__device__ __forceinline__ float len3(float x, float y, float z) { return fsqrt(dot3(x, y, z, x, y, z)); }
+
+// ComputeParticleGridCellKey, physics_particle_grid.hlsli:4-10
+__device__ __forceinline__ uint32_t cell_hash(int gx, int gy, int gz)
+{
+    uint32_t s = (uint32_t)gx * 73856093u;
+    s ^= (uint32_t)gy * 19349663u;
+    s ^= (uint32_t)gz * 83492791u;
+    return s;
+}
+
+// 3-D Morton interleave of the low `bits` bits of each wrapped cell coordinate.
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t cell_morton(int gx, int gy, int gz, uint32_t bits)
+{
+    const uint32_t m = (1u << bits) - 1u;
+    return (spread3((uint32_t)gx & m) << 2) | (spread3((uint32_t)gy & m) << 1) | spread3((uint32_t)gz & m);
+}
+__device__ __forceinline__ uint32_t cell_key(int gx, int gy, int gz, uint32_t mode, uint32_t bits)
+{
+    return mode == CN_KEY_HASH ? cell_hash(gx, gy, gz) : cell_morton(gx, gy, gz, bits);
+}
+
+// FluidSpikyKernel / FluidSpikyKernelDerivative / FluidCohesionKernel,
+// physics_fluid_common.hlsli:17-65
+__device__ __forceinline__ float spiky_w(float d, float h)
+{
+    if (d >= h) return 0.0f;
+    const float omq = __fsub_rn(1.0f, fdiv(d, h));
+    const float h3 = __fmul_rn(__fmul_rn(h, h), h);
+    const float k = fdiv(15.0f, __fmul_rn(kKernelPi, h3));
+    return __fmul_rn(__fmul_rn(k, omq), omq);
+}
+__device__ __forceinline__ float spiky_dw(float d, float h)
+{
+    if (d <= kEps || d >= h) return 0.0f;
+    const float omq = __fsub_rn(1.0f, fdiv(d, h));
+    const float h3 = __fmul_rn(__fmul_rn(h, h), h);
+    const float k = fdiv(15.0f, __fmul_rn(kKernelPi, h3));
+    return fdiv(__fmul_rn(__fmul_rn(-2.0f, k), omq), h);
+}
+__device__ __forceinline__ float cohesion_kernel(float d, float h, float c1, float c2)
+{
+    if (d <= kEps || d >= h) return 0.0f;
+    const float q = fdiv(d, h);
+    const float a = __fmul_rn(__fmul_rn(__fmul_rn(c1, q), q), q);
+    const float b = __fmul_rn(__fmul_rn(c2, q), q);
+    return __fsub_rn(__fadd_rn(a, b), 1.0f);
+}
+
+// ComputeFluidBoundaryContactInfo for an identity-oriented box,
+// physics_fluid_boundary_common.hlsli:10-66 with ClosestFaceNormalLocal
+// (physics_shape_common.hlsli:100-108).
+__device__ __forceinline__ bool box_contact(const Collider &c, float px, float py, float pz,
+                                            float &sd, float &nx, float &ny, float &nz)
+{
+    nx = 0.0f; ny = 1.0f; nz = 0.0f; sd = 0.0f;
+    if (!c.enabled) return false;
+    const float lx = __fsub_rn(px, c.c[0]), ly = __fsub_rn(py, c.c[1]), lz = __fsub_rn(pz, c.c[2]);
+    const float dx = __fsub_rn(lx, fclamp(lx, -c.he[0], c.he[0]));
+    const float dy = __fsub_rn(ly, fclamp(ly, -c.he[1], c.he[1]));
+    const float dz = __fsub_rn(lz, fclamp(lz, -c.he[2], c.he[2]));
+    const float dl = len3(dx, dy, dz);
+    if (dl > kEps) {
+        nx = fdiv(dx, dl); ny = fdiv(dy, dl); nz = fdiv(dz, dl);
+        sd = dl;
+    } else {
+        const float ex = __fsub_rn(c.he[0], fabsf(lx));
+        const float ey = __fsub_rn(c.he[1], fabsf(ly));
+        const float ez = __fsub_rn(c.he[2], fabsf(lz));
+        nx = 0.0f; ny = 0.0f; nz = 0.0f;
+        if (ex <= ey && ex <= ez) nx = lx >= 0.0f ? 1.0f : -1.0f;
+        else if (ey <= ez) ny = ly >= 0.0f ? 1.0f : -1.0f;
+        else nz = lz >= 0.0f ? 1.0f : -1.0f;
+        sd = -fmind(ex, fmind(ey, ez));
+    }
+    return true;
+}
+
+// ComputeFluidBoundaryGhostDelta, physics_fluid_boundary_common.hlsli:90-110
+__device__ __forceinline__ bool ghost_delta(const Collider &c, float px, float py, float pz, float h,
+                                            float &gx, float &gy, float &gz)
+{
+    float sd, nx, ny, nz;
+    gx = gy = gz = 0.0f;
+    if (!box_contact(c, px, py, pz, sd, nx, ny, nz)) return false;
+    if (sd < 0.0f || sd >= h) return false;
+    const float t = __fmul_rn(2.0f, sd);
+    gx = __fmul_rn(nx, t); gy = __fmul_rn(ny, t); gz = __fmul_rn(nz, t);
+    return true;
+}
+
+// XCD-aware tile index: consecutive logical tiles land on the same XCD (and its L2), since the
+// dispatcher deals workgroups to the 8 XCDs round-robin.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nblocks)
+{
+    const uint32_t per = nblocks / 8u;
+    if (per == 0u) return bid;
+    const uint32_t full = per * 8u;
+    if (bid >= full) return bid;
+    return (bid % 8u) * per + bid / 8u;
+}
+
+}  // namespace cn

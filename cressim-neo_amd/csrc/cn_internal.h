@@ -1,0 +1,76 @@
+// cn_internal.h -- declarations shared by the library's translation units (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cn_device.h"
+
+namespace cn {
+
+// ---- radix sort (cn_sort.hip) ----
+size_t sort_scratch_words(uint32_t n);
+uint2 *radix_sort_pairs(uint2 *buf_a, uint2 *buf_b, uint32_t n, uint32_t key_bits, uint32_t *scratch,
+                        hipStream_t stream);
+
+// Device pointers of one resident scene (single scene or one slot pool).
+struct SceneBuffers {
+    uint32_t n = 0;
+    // particle state in slot order, double-buffered across the per-substep reorder
+    float4 *pos[2] = {nullptr, nullptr};
+    float4 *vel[2] = {nullptr, nullptr};
+    uint32_t *cls[2] = {nullptr, nullptr};
+    uint32_t *id[2] = {nullptr, nullptr};
+    int cur = 0;
+    float4 *prev = nullptr;
+    int4 *cellc = nullptr;          // cell coordinates of each slot (w unused)
+    uint32_t *bnd = nullptr;        // boundary-candidate bitmask over colliders in visit order
+    // broadphase
+    uint2 *kv[2] = {nullptr, nullptr};
+    uint32_t *sort_scratch = nullptr;
+    uint32_t *elem_run = nullptr;
+    uint32_t *run_key = nullptr;
+    uint32_t *run_start = nullptr;  // n + 1
+    int4 *run_info = nullptr;       // (cx, cy, cz, pure)
+    uint32_t *bucket = nullptr;     // (1 << bucket_bits) + 1
+    uint32_t bucket_bits = 0;
+    uint32_t *block_runs = nullptr; // per 1024-element block, then scanned
+    uint32_t *run_count = nullptr;  // device scalar
+    uint32_t *perm = nullptr;
+    // fluid solve
+    float4 *dacc[2] = {nullptr, nullptr};
+    float4 *nc = nullptr;           // surface normal * tension, density constraint
+    uint32_t *nbr = nullptr;        // neighbor slots, k-major: nbr[k * n + i]
+    uint32_t *nbr_cnt = nullptr;
+    uint32_t cap = 0;
+    // index-order exports
+    float4 *out_pos = nullptr;
+    float4 *out_vel = nullptr;
+    float4 *out_prev = nullptr;
+    // tables
+    ParticleClass *classes = nullptr;
+    uint32_t nclasses = 0;
+    cn_fluid_material *fluid_mats = nullptr;
+    float4 *part_mats = nullptr;
+    Collider *colliders = nullptr;
+    uint32_t ncolliders = 0;
+    // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total
+    uint32_t *meta = nullptr;
+};
+
+// ---- broadphase + fluid launches (cn_kernels.hip) ----
+void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, uint32_t key_bits, hipStream_t st);
+void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
+void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
+void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
+void launch_export(const SceneBuffers &s, hipStream_t st);
+
+// per-pass entries in the reference's data order (cn_passes.hip)
+void launch_ref_keys(const float4 *pos, uint32_t n, float inv_cell, uint32_t *keys, int32_t *cells,
+                     hipStream_t st);
+void launch_ref_cell_ranges(const uint32_t *sorted_keys, uint32_t n, uint32_t capacity, uint32_t *ranges,
+                            hipStream_t st);
+
+}  // namespace cn

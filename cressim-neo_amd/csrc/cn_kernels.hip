@@ -1,0 +1,777 @@
+// cn_kernels.hip -- broadphase and PBF fluid kernels of the particle substep (gfx950).
+//
+// Data layout: particle state lives in "slot" order, the cell-sorted order of the previous
+// substep (slot t holds particle id[t]).  Every substep re-sorts the slots by cell key and by
+// particle index inside a cell, then gathers the state into the new order, so a cell's particles
+// are contiguous and the 27-cell neighbor walk reads short contiguous runs from L2 instead of
+// scattered 16-byte records.  The neighbor list of a particle is, exactly as in the reference,
+// the 27 cells in dz,dy,dx order with each cell's particles in ascending particle index
+// (physics_fluid_build_neighbors.cs.hlsl:67-141), so results do not depend on the slot order.
+#include <hip/hip_runtime.h>
+
+#include "cn_internal.h"
+
+namespace cn {
+
+static inline uint32_t grid1(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+// physics_soft_predict.cs.hlsl:12-47
+__device__ __forceinline__ void predict(float4 &p, float4 &v, float4 &prev, float gs, const SubstepConsts &c)
+{
+    prev = make_float4(p.x, p.y, p.z, 0.0f);
+    if (p.w <= kEps) {
+        v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float dts = c.dt * gs;
+    v.x = v.x + c.gravity[0] * dts;
+    v.y = v.y + c.gravity[1] * dts;
+    v.z = v.z + c.gravity[2] * dts;
+    v.w = 0.0f;
+    p.x = p.x + v.x * c.dt;
+    p.y = p.y + v.y * c.dt;
+    p.z = p.z + v.z * c.dt;
+}
+
+__device__ __forceinline__ float gravity_scale(const ParticleClass &pc, const cn_fluid_material *fm)
+{
+    return pc.kind == CN_PARTICLE_KIND_FLUID ? fm[pc.fluid_mat].gravity_scale : 1.0f;
+}
+
+// physics_particle_build_broad_phase_entries.cs.hlsl:19-24
+__device__ __forceinline__ int3 cell_of(float x, float y, float z, float inv_cell)
+{
+    return make_int3((int)floorf(x * inv_cell), (int)floorf(y * inv_cell), (int)floorf(z * inv_cell));
+}
+
+// ---------------------------------------------------------------------------------------------
+// keys: predict (without storing) and emit (cell key, slot) pairs
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_keys(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                 const uint32_t *__restrict__ cls,
+                                                 const ParticleClass *__restrict__ classes,
+                                                 const cn_fluid_material *__restrict__ fm, SubstepConsts c,
+                                                 uint2 *__restrict__ kv)
+{
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= c.n) return;
+    float4 p = pos[s], v = vel[s], pv;
+    const ParticleClass pc = classes[c.uniform_class ? 0u : cls[s]];
+    predict(p, v, pv, gravity_scale(pc, fm), c);
+    const int3 g = cell_of(p.x, p.y, p.z, c.inv_cell);
+    kv[s] = make_uint2(cell_key(g.x, g.y, g.z, c.key_mode, c.morton_bits), s);
+}
+
+void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_keys, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.cls[s.cur],
+                       s.classes, s.fluid_mats, c, s.kv[0]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// runs of equal keys: flags -> block counts -> scan -> run table, bucket index, rank fix-up
+// ---------------------------------------------------------------------------------------------
+constexpr int kRunItems = 4;
+constexpr int kRunTile = kBlock * kRunItems;
+
+__device__ __forceinline__ bool run_flag(const uint2 *kv, uint32_t i, uint32_t n)
+{
+    return i < n && (i == 0 || kv[i].x != kv[i - 1].x);
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t &total)
+{
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBlock; off <<= 1) {
+        const uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    total = sh[kBlock - 1];
+    const uint32_t ex = sh[t] - v;
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(kBlock) void k_run_count(const uint2 *__restrict__ kv, uint32_t n,
+                                                      uint32_t *__restrict__ block_runs)
+{
+    __shared__ uint32_t sh[kBlock];
+    const uint32_t base = blockIdx.x * kRunTile + threadIdx.x * kRunItems;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) cnt += run_flag(kv, base + k, n) ? 1u : 0u;
+    uint32_t total;
+    block_exclusive_scan(cnt, sh, total);
+    if (threadIdx.x == 0) block_runs[blockIdx.x] = total;
+}
+
+// single workgroup: exclusive scan of the per-block run counts; total -> *run_count
+__global__ __launch_bounds__(1024) void k_scan_block_runs(uint32_t *__restrict__ v, uint32_t nb,
+                                                          uint32_t *__restrict__ run_count)
+{
+    __shared__ uint32_t sh[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b0 = t * per;
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < nb) local += v[b0 + k];
+    sh[t] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = sh[t] - local;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < nb) {
+            const uint32_t x = v[b0 + k];
+            v[b0 + k] = run;
+            run += x;
+        }
+    if (t == 1023) *run_count = sh[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_run_emit(const uint2 *__restrict__ kv, uint32_t n,
+                                                     const uint32_t *__restrict__ block_off,
+                                                     uint32_t *__restrict__ elem_run, uint32_t *__restrict__ run_key,
+                                                     uint32_t *__restrict__ run_start)
+{
+    __shared__ uint32_t sh[kBlock];
+    const uint32_t base = blockIdx.x * kRunTile + threadIdx.x * kRunItems;
+    bool f[kRunItems];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) {
+        f[k] = run_flag(kv, base + k, n);
+        cnt += f[k] ? 1u : 0u;
+    }
+    uint32_t total;
+    uint32_t incl = block_off[blockIdx.x] + block_exclusive_scan(cnt, sh, total);
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) {
+        const uint32_t i = base + k;
+        if (i >= n) break;
+        if (f[k]) {
+            run_key[incl] = kv[i].x;
+            run_start[incl] = i;
+            ++incl;
+        }
+        elem_run[i] = incl - 1u;
+        if (i == n - 1) run_start[incl] = n;
+    }
+}
+
+// bucket[b] = first run whose key >> shift is >= b (b in [0, nbuckets]); lookups scan
+// runs [bucket[b], bucket[b + 1]).
+__global__ __launch_bounds__(kBlock) void k_bucket_fill(const uint32_t *__restrict__ run_key,
+                                                        const uint32_t *__restrict__ run_count, uint32_t shift,
+                                                        uint32_t nbuckets, uint32_t *__restrict__ bucket)
+{
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b > nbuckets) return;
+    const uint32_t R = *run_count;
+    uint32_t lo = 0, hi = R;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if ((run_key[mid] >> shift) < b) lo = mid + 1u;
+        else hi = mid;
+    }
+    bucket[b] = lo;
+}
+
+// Stable order inside a run is by particle index: rank = #run members with a smaller id.
+__global__ __launch_bounds__(kBlock) void k_rank_fix(const uint2 *__restrict__ kv, uint32_t n,
+                                                     const uint32_t *__restrict__ elem_run,
+                                                     const uint32_t *__restrict__ run_start,
+                                                     const uint32_t *__restrict__ id_old, uint32_t *__restrict__ perm)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = elem_run[i];
+    const uint32_t s = run_start[r], e = run_start[r + 1];
+    const uint32_t o = kv[i].y;
+    if (e - s == 1u) {
+        perm[i] = o;
+        return;
+    }
+    const uint32_t me = id_old[o];
+    uint32_t rank = 0;
+    for (uint32_t q = s; q < e; ++q) rank += id_old[kv[q].y] < me ? 1u : 0u;
+    perm[s + rank] = o;
+}
+
+void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, uint32_t key_bits, hipStream_t st)
+{
+    const uint32_t n = c.n;
+    const uint32_t nb = (n + kRunTile - 1) / kRunTile;
+    hipLaunchKernelGGL(k_run_count, dim3(nb), dim3(kBlock), 0, st, sorted, n, s.block_runs);
+    hipLaunchKernelGGL(k_scan_block_runs, dim3(1), dim3(1024), 0, st, s.block_runs, nb, s.run_count);
+    hipLaunchKernelGGL(k_run_emit, dim3(nb), dim3(kBlock), 0, st, sorted, n, s.block_runs, s.elem_run, s.run_key,
+                       s.run_start);
+    const uint32_t nbuckets = 1u << s.bucket_bits;
+    const uint32_t shift = key_bits - s.bucket_bits;
+    hipLaunchKernelGGL(k_bucket_fill, dim3((nbuckets + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s.run_key,
+                       s.run_count, shift, nbuckets, s.bucket);
+    hipLaunchKernelGGL(k_rank_fix, dim3(grid1(n)), dim3(kBlock), 0, st, sorted, n, s.elem_run, s.run_start,
+                       s.id[s.cur], s.perm);
+}
+
+// ---------------------------------------------------------------------------------------------
+// reorder: gather state into the new slot order, predict, cell coordinates, boundary candidates
+// ---------------------------------------------------------------------------------------------
+
+// Candidate colliders of a fluid particle (physics_particle_count_fluid_boundary_candidates.cs.hlsl
+// :20-28 query radius, :43-85 filters), as a bitmask over the colliders in LBVH visit order.
+__device__ __forceinline__ uint32_t boundary_mask(const ParticleClass &pc, const cn_fluid_material *fm,
+                                                  const Collider *__restrict__ col, uint32_t ncol, float x,
+                                                  float y, float z)
+{
+    if (pc.kind != CN_PARTICLE_KIND_FLUID || ncol == 0) return 0u;
+    const float r = fmaxd(pc.radius, 0.0f);
+    const cn_fluid_material m = fm[pc.fluid_mat];
+    const float support = fmaxd(m.smoothing_radius, r);
+    const float reuse = fmaxd(fmaxd(m.cfl_radius, r), support);
+    const float qr = support + reuse;
+    const float x0 = x - qr, y0 = y - qr, z0 = z - qr;
+    const float x1 = x + qr, y1 = y + qr, z1 = z + qr;
+    uint32_t mask = 0;
+    for (uint32_t k = 0; k < ncol; ++k) {
+        const Collider &cc = col[k];
+        const bool ov = x0 <= cc.aabb_max[0] && x1 >= cc.aabb_min[0] && y0 <= cc.aabb_max[1] &&
+                        y1 >= cc.aabb_min[1] && z0 <= cc.aabb_max[2] && z1 >= cc.aabb_min[2];
+        if (ov && cc.enabled && cc.env == pc.env && (pc.mask & cc.layer) != 0u && (cc.mask & pc.layer) != 0u)
+            mask |= 1u << k;
+    }
+    return mask;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reorder(
+    const uint32_t *__restrict__ perm, const float4 *__restrict__ pos_o, const float4 *__restrict__ vel_o,
+    const uint32_t *__restrict__ cls_o, const uint32_t *__restrict__ id_o, float4 *__restrict__ pos_n,
+    float4 *__restrict__ vel_n, uint32_t *__restrict__ cls_n, uint32_t *__restrict__ id_n,
+    float4 *__restrict__ prev, int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
+    const ParticleClass *__restrict__ classes, const cn_fluid_material *__restrict__ fm,
+    const Collider *__restrict__ col, SubstepConsts c)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= c.n) return;
+    const uint32_t o = perm[t];
+    float4 p = pos_o[o], v = vel_o[o], pv;
+    const uint32_t k = cls_o[o];
+    const ParticleClass pc = classes[c.uniform_class ? 0u : k];
+    predict(p, v, pv, gravity_scale(pc, fm), c);
+    pos_n[t] = p;
+    vel_n[t] = v;
+    prev[t] = pv;
+    cls_n[t] = k;
+    id_n[t] = id_o[o];
+    const int3 g = cell_of(p.x, p.y, p.z, c.inv_cell);
+    cellc[t] = make_int4(g.x, g.y, g.z, 0);
+    bnd[t] = boundary_mask(pc, fm, col, c.ncolliders, p.x, p.y, p.z);
+}
+
+// (cx, cy, cz, pure): pure runs hold a single cell; mixed runs (hash collisions or wrapped
+// Morton coordinates) are filtered per particle like the reference (:94-98).
+__global__ __launch_bounds__(kBlock) void k_run_info(const uint32_t *__restrict__ run_start,
+                                                     const uint32_t *__restrict__ run_count,
+                                                     const int4 *__restrict__ cellc, int4 *__restrict__ run_info)
+{
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= *run_count) return;
+    const uint32_t s = run_start[r], e = run_start[r + 1];
+    const int4 c0 = cellc[s];
+    int pure = 1;
+    for (uint32_t q = s + 1; q < e; ++q) {
+        const int4 cq = cellc[q];
+        if (cq.x != c0.x || cq.y != c0.y || cq.z != c0.z) {
+            pure = 0;
+            break;
+        }
+    }
+    run_info[r] = make_int4(c0.x, c0.y, c0.z, pure);
+}
+
+void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const int o = s.cur, nw = 1 - s.cur;
+    hipLaunchKernelGGL(k_reorder, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.perm, s.pos[o], s.vel[o], s.cls[o],
+                       s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev, s.cellc, s.bnd, s.classes,
+                       s.fluid_mats, s.colliders, c);
+    hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
+                       s.run_info);
+    s.cur = nw;
+}
+
+// ---------------------------------------------------------------------------------------------
+// fluid: neighbor list + density constraint (one kernel), delta + apply + clamp (one kernel)
+// ---------------------------------------------------------------------------------------------
+struct FluidArgs {
+    const float4 *pos;
+    const uint32_t *cls;
+    const int4 *cellc;
+    const uint32_t *bnd;
+    const float4 *din;   // accumulated delta of the previous iteration
+    float4 *dout;
+    float4 *nc;
+    uint32_t *nbr;
+    uint32_t *nbr_cnt;
+    const uint32_t *run_key;
+    const uint32_t *run_start;
+    const int4 *run_info;
+    const uint32_t *bucket;
+    uint32_t bucket_shift;
+    const ParticleClass *classes;
+    const cn_fluid_material *fm;
+    const Collider *col;
+    uint32_t *meta;
+    uint32_t it;
+};
+
+__device__ __forceinline__ int find_run(uint32_t key, const FluidArgs &a)
+{
+    const uint32_t b = key >> a.bucket_shift;
+    uint32_t r = a.bucket[b];
+    const uint32_t e = a.bucket[b + 1];
+    while (r < e && a.run_key[r] < key) ++r;
+    return (r < e && a.run_key[r] == key) ? (int)r : -1;
+}
+
+__device__ __forceinline__ bool neighbor_ok(const ParticleClass &self, const ParticleClass &o)
+{
+    // ShouldProcessFluidNeighbor, physics_fluid_neighbor_build.hlsli:48-70 (index check done by caller)
+    return o.kind == CN_PARTICLE_KIND_FLUID && o.env == self.env && (self.mask & o.layer) != 0u &&
+           (o.mask & self.layer) != 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// physics_fluid_build_neighbors.cs.hlsl:21-147 fused with
+// physics_fluid_compute_density_lambda.cs.hlsl:26-115: the density sum runs over the listed
+// neighbors in list order, which is the order they are found in.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_density(FluidArgs a, SubstepConsts c)
+{
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    uint32_t required = 0, trunc = 0, listed = 0;
+    if (i < c.n) {
+        const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+        const float4 p = a.pos[i];
+        if (pc.kind != CN_PARTICLE_KIND_FLUID || p.w <= kEps) {
+            a.nc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.nbr_cnt[i] = 0;
+        } else {
+            const bool use_d = a.it > 0;
+            const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float sx = p.x + di.x, sy = p.y + di.y, sz = p.z + di.z;
+            const cn_fluid_material m = a.fm[pc.fluid_mat];
+            const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
+            const float rho0 = fmaxd(m.rest_density, 1.0f);
+            const float st = fmaxd(m.surface_tension, 0.0f);
+            const bool tension = st > kEps;
+            const bool self_ok = !UNIFORM || neighbor_ok(pc, pc);
+            const int4 ci = a.cellc[i];
+            float density = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
+            uint32_t cnt = 0;
+            if (c.cap > 0 && self_ok) {
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int tx = ci.x + dx, ty = ci.y + dy, tz = ci.z + dz;
+                            // a cell whose hash is the table's empty marker is never found
+                            // (physics_fluid_build_neighbors.cs.hlsl:82-85)
+                            if (cell_hash(tx, ty, tz) == kInvalid) continue;
+                            const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
+                            if (r < 0) continue;
+                            const int4 info = a.run_info[r];
+                            const bool pure = info.w != 0;
+                            if (pure && (info.x != tx || info.y != ty || info.z != tz)) continue;
+                            const uint32_t qe = a.run_start[r + 1];
+                            for (uint32_t q = a.run_start[r]; q < qe; ++q) {
+                                if (!pure) {
+                                    const int4 cq = a.cellc[q];
+                                    if (cq.x != tx || cq.y != ty || cq.z != tz) continue;
+                                }
+                                if (q == i) continue;
+                                if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) continue;
+                                const float4 pq = a.pos[q];
+                                if (pq.w <= kEps) continue;
+                                const float4 dq = use_d ? a.din[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                                const float ddx = sx - (pq.x + dq.x);
+                                const float ddy = sy - (pq.y + dq.y);
+                                const float ddz = sz - (pq.z + dq.z);
+                                const float dist = len3(ddx, ddy, ddz);
+                                if (dist >= h) continue;
+                                ++required;
+                                if (cnt >= c.cap) continue;
+                                a.nbr[(size_t)cnt * c.n + i] = q;
+                                ++cnt;
+                                density = density + spiky_w(dist, h);
+                                if (tension) {
+                                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                                    if (!(dist <= kEps || dist >= h)) {
+                                        const float dw = spiky_dw(dist, h);
+                                        gx = dw * fdiv(ddx, dist);
+                                        gy = dw * fdiv(ddy, dist);
+                                        gz = dw * fdiv(ddz, dist);
+                                    }
+                                    snx = snx + gx;
+                                    sny = sny + gy;
+                                    snz = snz + gz;
+                                }
+                            }
+                        }
+            }
+            // AccumulateFluidBoundaryDensityFromCandidates, physics_fluid_boundary_common.hlsli:112-138
+            const uint32_t bm = a.bnd[i];
+            const float bscale = c.cfg[kCfgFluidBndDensity];
+            for (uint32_t k = 0; k < c.ncolliders; ++k) {
+                if (!((bm >> k) & 1u)) continue;
+                float gx, gy, gz;
+                if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
+                const float gd = len3(gx, gy, gz);
+                if (gd >= h) continue;
+                density = density + spiky_w(gd, h) * bscale;
+            }
+            const float C = fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) *
+                            m.density_constraint_scale;
+            a.nc[i] = tension ? make_float4(snx * st, sny * st, snz * st, C) : make_float4(0.0f, 0.0f, 0.0f, C);
+            a.nbr_cnt[i] = cnt;
+            trunc = required > c.cap ? 1u : 0u;
+            listed = cnt;
+        }
+    }
+    const uint32_t wmax = wave_max(required);
+    const uint32_t wtr = wave_sum(trunc);
+    const uint32_t wls = wave_sum(listed);
+    if ((threadIdx.x & 63u) == 0) {
+        if (wmax) atomicMax(&a.meta[0], wmax);
+        if (wtr) atomicAdd(&a.meta[1], wtr);
+        if (a.it == 0 && wls) atomicAdd((unsigned long long *)&a.meta[2], (unsigned long long)wls);
+    }
+}
+
+// physics_fluid_compute_delta_positions.cs.hlsl:27-157, then
+// physics_fluid_apply_delta_positions.cs.hlsl:11-31 and physics_fluid_clamp_boundary.cs.hlsl:23-80
+// on the same particle (both are per-particle and read only the particle's own delta).
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_delta(FluidArgs a, SubstepConsts c)
+{
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= c.n) return;
+    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    if (pc.kind != CN_PARTICLE_KIND_FLUID) {
+        a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 p = a.pos[i];
+    const bool use_d = a.it > 0;
+    const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
+    if (p.w > kEps) {
+        const float sx = p.x + di.x, sy = p.y + di.y, sz = p.z + di.z;
+        const float4 nci = a.nc[i];
+        const float Ci = nci.w;
+        const cn_fluid_material m = a.fm[pc.fluid_mat];
+        const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
+        const float cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
+        const float viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
+        const float st = fmaxd(m.surface_tension, 0.0f);
+        const float cfl = fmaxd(m.cfl_radius, 0.0f);
+        const bool tension = st > kEps;
+        const float snx = tension ? nci.x : 0.0f, sny = tension ? nci.y : 0.0f, snz = tension ? nci.z : 0.0f;
+        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f, wsum = 0.0f;
+        const uint32_t cnt = a.nbr_cnt[i];
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t q = a.nbr[(size_t)k * c.n + i];
+            const float4 pq = a.pos[q];
+            if (pq.w <= kEps) continue;
+            const float4 dq = use_d ? a.din[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float ddx = sx - (pq.x + dq.x);
+            const float ddy = sy - (pq.y + dq.y);
+            const float ddz = sz - (pq.z + dq.z);
+            const float dist = len3(ddx, ddy, ddz);
+            if (dist >= h || dist <= kEps) continue;
+            const float4 ncq = a.nc[q];
+            const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
+            const float der = spiky_dw(dist, h);
+            const float sc = -0.5f * (Ci + ncq.w) * der;
+            float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
+            const bool same = UNIFORM || pc.fluid_mat == a.classes[a.cls[q]].fluid_mat;
+            if (same) {
+                const float rx = di.x - dq.x, ry = di.y - dq.y, rz = di.z - dq.z;
+                if (cohesion > kEps) {
+                    const float ck = cohesion * cohesion_kernel(dist, h, m.cohesion1, m.cohesion2);
+                    pdx = pdx - nx * ck;
+                    pdy = pdy - ny * ck;
+                    pdz = pdz - nz * ck;
+                }
+                if (viscosity > kEps) {
+                    const float vs = 1.0f - fdiv(1.0f, 1.0f + viscosity * spiky_w(dist, h));
+                    pdx = pdx - vs * rx;
+                    pdy = pdy - vs * ry;
+                    pdz = pdz - vs * rz;
+                }
+                if (cfl > kEps) {
+                    const float proj = dot3(rx, ry, rz, nx, ny, nz);
+                    if (proj < -cfl) {
+                        const float pcf = proj + cfl;
+                        pdx = pdx - nx * pcf * 0.5f;
+                        pdy = pdy - ny * pcf * 0.5f;
+                        pdz = pdz - nz * pcf * 0.5f;
+                    }
+                }
+                if (tension) {
+                    pdx = pdx - c.dt * (snx - ncq.x);
+                    pdy = pdy - c.dt * (sny - ncq.y);
+                    pdz = pdz - c.dt * (snz - ncq.z);
+                }
+            }
+            dpx = dpx + pdx;
+            dpy = dpy + pdy;
+            dpz = dpz + pdz;
+            wsum = wsum + 1.0f;
+        }
+        // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
+        float bx = 0.0f, by = 0.0f, bz = 0.0f;
+        const float scn = fmaxd(Ci, 0.0f);
+        const uint32_t bm = a.bnd[i];
+        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+            if (!((bm >> k) & 1u)) continue;
+            float gx, gy, gz;
+            if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
+            const float gd = len3(gx, gy, gz);
+            if (gd <= kEps || gd >= h) continue;
+            const float f = -scn * spiky_dw(gd, h);
+            bx = bx + f * fdiv(gx, gd);
+            by = by + f * fdiv(gy, gd);
+            bz = bz + f * fdiv(gz, gd);
+        }
+        const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
+        ox = dpx * solve + bx * bdel;
+        oy = dpy * solve + by * bdel;
+        oz = dpz * solve + bz * bdel;
+        ow = wsum;
+    }
+    // apply
+    const float w = fmaxd(ow * c.cfg[kCfgFluidRelax], 1.0f);
+    const float ax = di.x + fdiv(ox, w), ay = di.y + fdiv(oy, w), az = di.z + fdiv(oz, w);
+    // clamp
+    const float r = fmaxd(pc.radius, 0.0f);
+    float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
+    const uint32_t bm = a.bnd[i];
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        bool changed = false;
+        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+            if (!((bm >> k) & 1u)) continue;
+            float sd, nx, ny, nz;
+            if (!box_contact(a.col[k], cx, cy, cz, sd, nx, ny, nz)) continue;
+            if (sd >= r) continue;
+            const float push = r - sd;
+            cx = cx + nx * push;
+            cy = cy + ny * push;
+            cz = cz + nz * push;
+            changed = true;
+        }
+        if (!changed) break;
+    }
+    a.dout[i] = make_float4(cx - p.x, cy - p.y, cz - p.z, 0.0f);
+}
+
+static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din)
+{
+    FluidArgs a;
+    const int cur = s.cur;
+    a.pos = s.pos[cur];
+    a.cls = s.cls[cur];
+    a.cellc = s.cellc;
+    a.bnd = s.bnd;
+    a.din = s.dacc[din];
+    a.dout = s.dacc[1 - din];
+    a.nc = s.nc;
+    a.nbr = s.nbr;
+    a.nbr_cnt = s.nbr_cnt;
+    a.run_key = s.run_key;
+    a.run_start = s.run_start;
+    a.run_info = s.run_info;
+    a.bucket = s.bucket;
+    a.bucket_shift = c.bucket_shift;
+    a.classes = s.classes;
+    a.fm = s.fluid_mats;
+    a.col = s.colliders;
+    a.meta = s.meta;
+    a.it = it;
+    return a;
+}
+
+void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, it, din);
+    if (c.uniform_class)
+        hipLaunchKernelGGL(k_fluid_density<true>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
+    else
+        hipLaunchKernelGGL(k_fluid_density<false>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
+}
+
+void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, it, din);
+    if (c.uniform_class)
+        hipLaunchKernelGGL(k_fluid_delta<true>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
+    else
+        hipLaunchKernelGGL(k_fluid_delta<false>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
+}
+
+// ---------------------------------------------------------------------------------------------
+// finalize: physics_particle_update_velocities.cs.hlsl:13-44 then
+// physics_fluid_project_boundary_velocities.cs.hlsl:26-118 on the same particle
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                     const float4 *__restrict__ prev, const uint32_t *__restrict__ cls,
+                                                     const float4 *__restrict__ dacc, const uint32_t *__restrict__ bnd,
+                                                     const ParticleClass *__restrict__ classes,
+                                                     const float4 *__restrict__ part_mats,
+                                                     const Collider *__restrict__ col, SubstepConsts c)
+{
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= c.n) return;
+    const ParticleClass pc = classes[c.uniform_class ? 0u : cls[i]];
+    float4 p = pos[i];
+    float vx, vy, vz;
+    const float dt = c.dt;
+    const bool fluid = pc.kind == CN_PARTICLE_KIND_FLUID;
+    const float4 pm = part_mats[pc.part_mat];
+    if (p.w <= kEps || dt <= kEps) {
+        vx = vy = vz = 0.0f;
+    } else {
+        const float4 pv = prev[i];
+        if (fluid) {
+            const float4 d = dacc[i];
+            p.x = p.x + d.x;
+            p.y = p.y + d.y;
+            p.z = p.z + d.z;
+            pos[i] = p;
+        }
+        const float damping = fmaxd(pm.z, 0.0f);
+        const float scale = fmaxd(1.0f - damping * dt, 0.0f);
+        vx = fdiv(p.x - pv.x, dt) * scale;
+        vy = fdiv(p.y - pv.y, dt) * scale;
+        vz = fdiv(p.z - pv.z, dt) * scale;
+    }
+    if (fluid && c.ncolliders > 0) {
+        const float r = fmaxd(pc.radius, 0.0f);
+        const float slop = c.cfg[kCfgContactSlop];
+        const float damping = fmaxd(pm.z, 0.0f);
+        float best = 3.402823466e+38f, bnx = 0.0f, bny = 1.0f, bnz = 0.0f, bmf = 0.0f, bms = 0.0f;
+        bool found = false;
+        const uint32_t bm = bnd[i];
+        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+            if (!((bm >> k) & 1u)) continue;
+            float sd, nx, ny, nz;
+            if (!box_contact(col[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
+            const float cdist = r + 2.0f * slop;
+            if (sd > cdist) continue;
+            if (sd < best) {
+                best = sd;
+                bnx = nx;
+                bny = ny;
+                bnz = nz;
+                bmf = fsqrt(fmaxd(0.0f, pm.x) * fmaxd(0.0f, col[k].material[0]));
+                bms = fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, col[k].material[3]));
+                found = true;
+            }
+        }
+        if (found) {
+            const float pen = fmaxd(r - best, 0.0f);
+            const float cdist = fmaxd(r + 2.0f * slop, 1.0e-5f);
+            const float cw = fsat(fdiv(cdist - best, cdist));
+            float rx = vx, ry = vy, rz = vz;
+            const float nv = dot3(rx, ry, rz, bnx, bny, bnz);
+            if (nv < 0.0f) {
+                rx = rx - bnx * nv;
+                ry = ry - bny * nv;
+                rz = rz - bnz * nv;
+            }
+            const float rn = dot3(rx, ry, rz, bnx, bny, bnz);
+            float tx = rx - bnx * rn, ty = ry - bny * rn, tz = rz - bnz * rn;
+            if (dt > kEps) {
+                // ComputePositionFrictionDelta, physics_rigid_solver_shared.hlsli:50-67
+                const float ex = tx * dt, ey = ty * dt, ez = tz * dt;
+                const float penf = fmaxd(pen, slop);
+                const float td = len3(ex, ey, ez);
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                if (!(td <= c.cfg[kCfgPosFrictionMin] || penf <= kEps)) {
+                    const float sl = fsat(bms) * penf;
+                    if (td <= sl) {
+                        fx = ex;
+                        fy = ey;
+                        fz = ez;
+                    } else {
+                        const float ks = fmind(fdiv(fsat(bmf) * penf, td), 1.0f);
+                        fx = ex * ks;
+                        fy = ey * ks;
+                        fz = ez * ks;
+                    }
+                }
+                tx = tx - fdiv(fx, dt);
+                ty = ty - fdiv(fy, dt);
+                tz = tz - fdiv(fz, dt);
+            }
+            const float tds = fmaxd(1.0f - damping * cw * dt, 0.0f);
+            tx = tx * tds;
+            ty = ty * tds;
+            tz = tz * tds;
+            const float rn2 = fmaxd(dot3(rx, ry, rz, bnx, bny, bnz), 0.0f);
+            vx = bnx * rn2 + tx;
+            vy = bny * rn2 + ty;
+            vz = bnz * rn2 + tz;
+        }
+    }
+    vel[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
+                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, c);
+}
+
+// slot order -> particle-index order
+__global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                   const float4 *__restrict__ prev, const uint32_t *__restrict__ id,
+                                                   uint32_t n, float4 *__restrict__ opos, float4 *__restrict__ ovel,
+                                                   float4 *__restrict__ oprev)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t d = id[t];
+    opos[d] = pos[t];
+    ovel[d] = vel[t];
+    oprev[d] = prev[t];
+}
+
+void launch_export(const SceneBuffers &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
+                       s.id[s.cur], s.n, s.out_pos, s.out_vel, s.out_prev);
+}
+
+}  // namespace cn

@@ -1,0 +1,280 @@
+/*
+ * cn_hip.h -- C ABI of the MI355X (gfx950) particle-substep library libcn_hip.so.
+ *
+ * The library replaces the particle/fluid/soft compute passes that CRESSim-Neo issues through
+ * PhysicsPassDispatcher (src/physics/physics_pass_dispatcher.h:24-342) and the HLSL kernels they
+ * bind (assets/shaders/physics/{particle,fluid,soft,shared}/...).  The substep loop that
+ * PhysicsSolver::step runs (src/physics/physics_solver.cpp:336-1401) is cn_step().
+ *
+ * Conventions
+ *   - Every function returns cn_status; CN_OK == 0.  The message of the last failure on a
+ *     context is cn_last_error().  No exception crosses this boundary.
+ *   - All structs are plain data.  Host pointers passed in are borrowed for the call only.
+ *   - One context owns one HIP stream and all device buffers of its scene(s).  A context is not
+ *     re-entrant; different contexts may be used from different threads.
+ *   - Particle arrays are indexed by the caller's particle index (the reference's pool index).
+ *     float4 arrays are 16-byte (x, y, z, w) records.
+ */
+#ifndef CN_HIP_H
+#define CN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CN_ABI_VERSION 1u
+
+typedef enum cn_status {
+    CN_OK = 0,
+    CN_ERR_INVALID_ARGUMENT = 1,
+    CN_ERR_OUT_OF_MEMORY = 2,
+    CN_ERR_DEVICE = 3,
+    CN_ERR_NOT_READY = 4,
+    CN_ERR_CAPACITY = 5,   /* a capacity (neighbor cap, candidate capacity, slot budget) overflowed */
+    CN_ERR_UNSUPPORTED = 6
+} cn_status;
+
+/* Particle kinds (assets/shaders/include/physics/particle/physics_particle_types.hlsli:9-10). */
+#define CN_PARTICLE_KIND_SOLID 0u
+#define CN_PARTICLE_KIND_FLUID 1u
+
+/* Broadphase key used to order particles inside a substep.
+ *   CN_KEY_HASH    the reference cell hash (physics_particle_grid.hlsli:4-10), 32-bit keys.
+ *   CN_KEY_MORTON  Morton code of the wrapped cell coordinate (10 bits per axis).
+ * Both produce bit-identical neighbor lists and solver results (a cell's particles are visited in
+ * ascending particle index and the 27 cells in dz,dy,dx order either way); Morton keeps
+ * spatially adjacent cells adjacent in memory. */
+#define CN_KEY_MORTON 0u
+#define CN_KEY_HASH 1u
+
+/* GpuFluidMaterial (physics_particle_types.hlsli:115-129), 48 bytes. */
+typedef struct cn_fluid_material {
+    float rest_density;
+    float inv_rest_density;
+    float smoothing_radius;
+    float density_constraint_scale;
+    float viscosity;
+    float cohesion;
+    float cohesion1;
+    float cohesion2;
+    float surface_tension;
+    float vorticity_confinement;
+    float gravity_scale;
+    float cfl_radius;
+} cn_fluid_material;
+
+/* FluidMaterialDesc (include/physics/physics_types.h:333-343) without the contact block. */
+typedef struct cn_fluid_material_desc {
+    float viscosity;
+    float cohesion;
+    float surface_tension;
+    float vorticity_confinement;
+    float gravity_scale;
+    float cfl_coefficient;
+} cn_fluid_material_desc;
+
+/* Static axis-aligned box collider.  It is the reference's static rigid body with identity
+ * orientation, unit scale and a box collider at local position 0 (GpuColliderGeometryData,
+ * physics_fluid_boundary_common.hlsli:10-66 with an identity quaternion).  material is the
+ * collider material (friction, restitution, 0, staticFriction) of physics_world.cpp:85-88. */
+typedef struct cn_box_collider {
+    float center[3];
+    float half_extents[3];
+    float material[4];
+    uint32_t environment;
+    uint32_t layer;
+    uint32_t mask;
+    uint32_t enabled;
+    uint32_t reserved[2];
+} cn_box_collider;
+
+/* Host view of the unified particle pool (ParticleSoAHost, include/physics/physics_types.h:858). */
+typedef struct cn_particle_soa_view {
+    uint32_t count;
+    const float *positions_inv_mass;    /* float4[count] */
+    const float *velocities;            /* float4[count] */
+    const float *radii;                 /* float[count] */
+    const uint32_t *kinds;              /* CN_PARTICLE_KIND_* */
+    const uint32_t *fluid_material_indices;
+    const uint32_t *particle_material_indices;
+    const uint32_t *environment_indices;
+    const uint32_t *phases;
+    const uint32_t *collision_layers;
+    const uint32_t *collision_masks;
+} cn_particle_soa_view;
+
+/* Scene tables uploaded beside the particles. */
+typedef struct cn_scene_tables {
+    uint32_t fluid_material_count;
+    const cn_fluid_material *fluid_materials;
+    uint32_t particle_material_count;
+    const float *particle_materials;    /* float4 (friction, restitution, damping, staticFriction) */
+    uint32_t collider_count;
+    const cn_box_collider *colliders;
+} cn_scene_tables;
+
+/* Soft/cloth constraints (physics_particle_types.hlsli:147-205 layouts). */
+typedef struct cn_soft_edge {
+    uint32_t a, b;
+    float rest_length;
+    float compliance;
+    uint32_t enabled;
+    uint32_t reserved[3];
+} cn_soft_edge;
+
+typedef struct cn_soft_bend {
+    uint32_t p0, p1, p2;
+    float rest_angle;
+    float compliance;
+    uint32_t reserved[3];
+} cn_soft_bend;
+
+typedef struct cn_cloth_dihedral {
+    uint32_t e0, e1, o0, o1;
+    float rest_angle;
+    float compliance;
+    uint32_t reserved[2];
+} cn_cloth_dihedral;
+
+typedef struct cn_soft_tet {
+    uint32_t p[4];
+    float rest_volume;
+    float compliance;
+    uint32_t reserved[2];
+} cn_soft_tet;
+
+typedef struct cn_constraint_view {
+    uint32_t edge_count;
+    const cn_soft_edge *edges;
+    uint32_t bend_count;
+    const cn_soft_bend *bends;
+    uint32_t dihedral_count;
+    const cn_cloth_dihedral *dihedrals;
+    uint32_t tet_count;
+    const cn_soft_tet *tets;
+} cn_constraint_view;
+
+/* GpuPhysicsSolverConfig (assets/shaders/include/physics/physics_solver_config.hlsli:5-51),
+ * 10 float4 records: contact0, contact1, rigid0, rigid1, joints0, joints1, soft0, soft1,
+ * fluid0, fluid1.  cn_default_step_params() fills the PhysicsSolverDesc defaults. */
+#define CN_SOLVER_CONFIG_FLOATS 40
+
+typedef struct cn_step_params {
+    float dt;                        /* frame delta; substep dt = dt / substeps */
+    uint32_t substeps;
+    uint32_t default_iterations;
+    uint32_t fluid_iterations;       /* 0 -> default_iterations */
+    uint32_t soft_internal_iterations;
+    uint32_t soft_contact_iterations;
+    float gravity[4];
+    float solver_config[CN_SOLVER_CONFIG_FLOATS];
+    float cell_size;                 /* particle grid cell size; 0 -> derived like the reference */
+    uint32_t max_fluid_neighbors;    /* neighbor slots per fluid particle (reference: 128) */
+    uint32_t key_mode;               /* CN_KEY_MORTON or CN_KEY_HASH */
+    uint32_t flags;                  /* reserved, 0 */
+} cn_step_params;
+
+/* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
+ * physics_solver.cpp:1403-1492), plus the fluid-neighbor truncation the reference lacks. */
+typedef struct cn_meta {
+    uint32_t max_fluid_neighbor_count;    /* largest required (untruncated) in-radius count */
+    uint32_t fluid_neighbor_truncations;  /* particles whose list hit max_fluid_neighbors */
+    uint32_t max_cell_occupancy;          /* longest run of equal keys in the last sort */
+    uint32_t verlet_rebuilds;             /* reserved */
+    uint64_t fluid_neighbor_total;        /* sum of listed neighbors over the last substep's
+                                             first iteration (for the mean K) */
+    uint32_t last_particle_count;
+    uint32_t reserved[5];
+} cn_meta;
+
+/* Raw device pointers of the resident scene (PhysicsGpuSceneView analog). */
+typedef struct cn_device_views {
+    void *positions_inv_mass;   /* float4[count], particle-index order after cn_step */
+    void *velocities;           /* float4[count] */
+    uint32_t count;
+    void *stream;               /* hipStream_t */
+} cn_device_views;
+
+typedef struct cn_ctx cn_ctx;
+
+typedef struct cn_device_desc {
+    int device_ordinal;
+    uint32_t flags;             /* reserved */
+} cn_device_desc;
+
+/* -------- context -------- */
+uint32_t cn_abi_version(void);
+cn_status cn_ctx_create(const cn_device_desc *desc, cn_ctx **out_ctx);
+void cn_ctx_destroy(cn_ctx *ctx);
+const char *cn_last_error(const cn_ctx *ctx);
+
+/* -------- host authoring helpers (no device work) -------- */
+/* PhysicsSolverDesc defaults (include/physics/physics_solver.h:21-143). */
+void cn_default_step_params(cn_step_params *params);
+/* toFluidSolverMaterial (src/physics/physics_world.cpp:154-263). */
+cn_status cn_fluid_material_from_desc(const cn_fluid_material_desc *desc, float particle_radius,
+                                      cn_fluid_material *out);
+/* recomputeParticleGridCellSize (src/physics/physics_world.cpp:6829-6846). */
+float cn_particle_grid_cell_size(uint32_t count, const float *radii,
+                                 const uint32_t *fluid_material_indices,
+                                 uint32_t fluid_material_count,
+                                 const cn_fluid_material *fluid_materials);
+
+/* -------- scene state (PhysicsSceneGpuState analog) -------- */
+/* ensureCapacity (physics_scene_gpu_state.cpp:305). */
+cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t particle_capacity, uint32_t max_fluid_neighbors);
+/* uploadWorldState / uploadParticles / uploadSoftTopology (:2348, :3163, :3255). */
+cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *particles,
+                          const cn_scene_tables *tables, const cn_constraint_view *constraints);
+/* PhysicsSolver::step (physics_solver.cpp:336-1401) for the particle path; asynchronous on the
+ * context stream. */
+cn_status cn_step(cn_ctx *ctx, const cn_step_params *params);
+/* Blocking readback (physics_scene_gpu_state.cpp:4082-4142). Any pointer may be NULL. */
+cn_status cn_readback(cn_ctx *ctx, float *positions_inv_mass, float *previous_positions,
+                      float *velocities, uint32_t count);
+cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
+/* validateGpuMetaBlocking analog; returns CN_ERR_CAPACITY if a capacity overflowed. */
+cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out);
+cn_status cn_synchronize(cn_ctx *ctx);
+
+/* -------- per-pass entries (host in/out; used by parity tests) -------- */
+/* buildParticleBroadPhaseEntries + buildParticleBroadPhaseKeys
+ * (physics_particle_build_broad_phase_{entries,keys}.cs.hlsl). */
+cn_status cn_pass_broadphase_keys(cn_ctx *ctx, uint32_t count, const float *positions_inv_mass,
+                                  float cell_size, uint32_t *keys, int32_t *cells_xyz);
+/* sortParticleBroadPhase (physics_pass_dispatcher.cpp:3963-4063): stable ascending sort of
+ * (key, index) pairs, in place. */
+cn_status cn_pass_sort_pairs(cn_ctx *ctx, uint32_t count, uint32_t *keys, uint32_t *indices);
+/* clear/mark/build ParticleCellRanges (physics_particle_*_cell_range*.cs.hlsl); ranges holds
+ * capacity uint4 records {key, start, end, 0}, unused {0xffffffff, 0, 0, 0}. */
+cn_status cn_pass_cell_ranges(cn_ctx *ctx, uint32_t count, const uint32_t *sorted_keys,
+                              uint32_t capacity, uint32_t *ranges);
+
+/* One substep's broadphase and first neighbor build on the uploaded scene, without committing
+ * state: predict, keys, sort, cell runs, then the iteration-0 neighbor lists
+ * (physics_solver.cpp:580-633 and :876).  sorted_keys/sorted_ids (count entries) are the sorted
+ * cell keys and the particle index at each sorted position; counts (count) and lists
+ * (count * max_fluid_neighbors, neighbor particle indices) are in particle-index order.
+ * Any output may be NULL. */
+cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *params, uint32_t *sorted_keys,
+                                     uint32_t *sorted_ids, uint32_t *counts, uint32_t *lists);
+
+/* -------- profiling (HIP events around the dominant kernels of cn_step) -------- */
+#define CN_KERNEL_SLOTS 16
+typedef struct cn_kernel_stats {
+    char name[CN_KERNEL_SLOTS][32];
+    uint32_t launches[CN_KERNEL_SLOTS];
+    double total_ms[CN_KERNEL_SLOTS];
+    uint32_t used;
+} cn_kernel_stats;
+cn_status cn_profile_enable(cn_ctx *ctx, int enable);
+cn_status cn_profile_read(cn_ctx *ctx, cn_kernel_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CN_HIP_H */

@@ -1,0 +1,1170 @@
+/*
+ * cn_oracle.c -- CPU restatement of CRESSim-Neo's particle substep.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity oracle and the CPU baseline.  The product library never
+ * links, loads or calls this file.  See cn_oracle.h for the conventions.  Each function names the
+ * reference file:line it restates (paths relative to the reference repository root).
+ *
+ * Build: gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math (oracle/Makefile).
+ */
+#include "cn_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EPS 1.0e-6f             /* kEpsilon, assets/shaders/include/physics/core/physics_base.hlsli:7 */
+#define INVALID 0xffffffffu     /* kInvalidIndex, physics_base.hlsli:6 */
+#define KERNEL_PI 3.14159265359f /* literal of physics_fluid_common.hlsli:26 */
+#define BROADPHASE_MARGIN 0.05f  /* kBroadPhaseMargin, physics_shape_common.hlsli:10 */
+
+/* solver config indices (physics_solver_config.hlsli:19-51) */
+#define CFG_CONTACT_SLOP 0
+#define CFG_POS_FRICTION_MIN 7
+#define CFG_SOFT_RELAX 24
+#define CFG_FLUID_SOLVE 32
+#define CFG_FLUID_RELAX 33
+#define CFG_FLUID_BND_DENSITY 34
+#define CFG_FLUID_BND_DELTA 35
+#define CFG_FLUID_MAX_UNDER 36
+
+static float maxf(float a, float b) { return a > b ? a : b; }
+static float minf(float a, float b) { return a < b ? a : b; }
+static float saturatef(float a) { return minf(maxf(a, 0.0f), 1.0f); }
+static float dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static float len3(const float *a) { return sqrtf(dot3(a, a)); }
+
+int cno_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void cno_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0)
+        omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Host authoring
+ * ------------------------------------------------------------------------------------------ */
+
+/* PhysicsSolverDesc defaults, include/physics/physics_solver.h:21-143, packed by makeSolverConfig
+ * (src/physics/physics_solver.cpp:100-129). */
+void cno_default_params(cn_step_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->dt = 1.0f / 60.0f;
+    p->substeps = 1;
+    p->default_iterations = 20;
+    p->gravity[0] = 0.0f;
+    p->gravity[1] = -9.81f;
+    p->gravity[2] = 0.0f;
+    const float cfg[CN_SOLVER_CONFIG_FLOATS] = {
+        1.0e-3f, 4.0f, 0.95f, 0.05f,        /* contact0 */
+        0.5f, 2.0f, 0.0f, 1.0e-5f,          /* contact1 */
+        0.25f, 0.90f, 0.10f, 1.0f,          /* rigid0 */
+        0.01f, 0.10f, 0.8f, 0.5f,           /* rigid1 */
+        0.95f, 0.7f, 0.05f, 0.02f,          /* joints0 */
+        0.12f, 1.0e-5f, 5.0e-5f, 1.0e-5f,   /* joints1 */
+        0.2f, 0.0f, 0.0f, 0.0f,             /* soft0 */
+        0.0f, 0.0f, 0.0f, 0.0f,             /* soft1 */
+        0.7f, 1.0f, 0.12f, 0.08f,           /* fluid0 */
+        0.0f, 0.0f, 0.0f, 0.0f};            /* fluid1 */
+    memcpy(p->solver_config, cfg, sizeof(cfg));
+    p->max_fluid_neighbors = 128;  /* kDefaultFluidMaxNeighborhood, rigid_body_common.h:304 */
+    p->key_mode = CN_KEY_HASH;
+}
+
+/* fluidSpikyWeight / fluidSpikyWeightDerivative, src/physics/physics_world.cpp:99-112 */
+static const float kHostPi = 3.14159265358979323846f;
+static float host_spiky_w(float d, float h)
+{
+    const float omq = 1.0f - d / h;
+    const float k = 15.0f / (kHostPi * h * h * h);
+    return k * omq * omq;
+}
+static float host_spiky_dw(float d, float h)
+{
+    const float omq = 1.0f - d / h;
+    const float k = 15.0f / (kHostPi * h * h * h);
+    return -k * 2.0f * omq / h;
+}
+
+/* tightPack3D + calculateFluidSolveScales, src/physics/physics_world.cpp:140-231 */
+static void fluid_solve_scales(float rest_distance, float support, float *rest_density,
+                               float *density_scale, float *surface_scale)
+{
+    const int dim = (int)ceilf(support / rest_distance);
+    const float row_scale = sqrtf(0.75f) * rest_distance;
+    float rho = 0.0f, dcs = 0.0f, a = 0.0f, b = 0.0f;
+    uint32_t count = 0;
+    for (int z = -dim; z <= dim; ++z)
+        for (int y = -dim; y <= dim; ++y)
+            for (int x = -dim; x <= dim; ++x) {
+                const float sx =
+                    (float)x * rest_distance + ((((y + z) & 1) != 0) ? rest_distance * 0.5f : 0.0f);
+                const float sy = (float)y * row_scale;
+                const float sz = (float)z * row_scale;
+                const float m2 = sx * sx + sy * sy + sz * sz;
+                if (m2 == 0.0f)
+                    continue;
+                if (!(sqrtf(m2) <= support))
+                    continue;
+                if (count == 2048u)
+                    goto done;
+                ++count;
+                const float dist = sqrtf(sx * sx + sy * sy + sz * sz);
+                rho += host_spiky_w(dist, support);
+                const float der = host_spiky_dw(dist, support);
+                dcs += der * der;
+                if (sy <= 0.0f && dist > 0.0f) {
+                    const float ct = sy / dist;
+                    a += der * ct;
+                    b -= dist * ct;
+                }
+            }
+done:
+    if (rho <= 1.0e-6f)
+        rho = 1.0f;
+    if (dcs <= 1.0e-6f)
+        dcs = 1.0f;
+    *rest_density = rho;
+    *density_scale = dcs;
+    *surface_scale = (fabsf(b) > 1.0e-6f && fabsf(a) > 1.0e-6f) ? fabsf(a / b) : 1.0f;
+}
+
+/* toFluidSolverMaterial, src/physics/physics_world.cpp:233-263 */
+void cno_fluid_material(const cn_fluid_material_desc *desc, float radius, cn_fluid_material *out)
+{
+    const float rest_distance = maxf(2.0f * radius, 1.0e-4f);
+    const float h = maxf((2.0f / 0.6f) * radius, rest_distance);
+    float rho, dcs, sts;
+    fluid_solve_scales(rest_distance, h, &rho, &dcs, &sts);
+    const float inv_rho = 1.0f / rho;
+    float rr = rest_distance / h;
+    rr = rr < 1.0e-3f ? 1.0e-3f : (rr > 0.999f ? 0.999f : rr);
+    const float rr2 = rr * rr;
+    out->rest_density = rho;
+    out->inv_rest_density = inv_rho;
+    out->smoothing_radius = h;
+    out->density_constraint_scale = 1.0f / maxf(dcs, 1.0e-6f);
+    out->viscosity = desc->viscosity * inv_rho;
+    out->cohesion = desc->cohesion * h;
+    out->cohesion1 = -(1.0f + rr) / rr2;
+    out->cohesion2 = (rr2 + rr + 1.0f) / rr2;
+    out->surface_tension = (desc->surface_tension * inv_rho) / sts;
+    out->vorticity_confinement = desc->vorticity_confinement * inv_rho * h;
+    out->gravity_scale = desc->gravity_scale;
+    out->cfl_radius = desc->cfl_coefficient * h;
+}
+
+/* recomputeParticleGridCellSize, src/physics/physics_world.cpp:6829-6846 */
+float cno_grid_cell_size(uint32_t n, const float *radii, const uint32_t *fluid_mat, uint32_t n_mats,
+                         const cn_fluid_material *mats)
+{
+    float g = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        g = maxf(g, radii[i] * 2.0f);
+        if (fluid_mat && fluid_mat[i] != INVALID && fluid_mat[i] < n_mats)
+            g = maxf(g, mats[fluid_mat[i]].smoothing_radius);
+    }
+    return maxf(g, 1.0e-4f);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Predict: assets/shaders/physics/soft/solver/physics_soft_predict.cs.hlsl:12-47
+ * ------------------------------------------------------------------------------------------ */
+void cno_predict(cno_scene *s, float dt, const float *g)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *p = s->pos + 4 * i;
+        float *pv = s->prev + 4 * i;
+        float *v = s->vel + 4 * i;
+        pv[0] = p[0];
+        pv[1] = p[1];
+        pv[2] = p[2];
+        pv[3] = 0.0f;
+        if (p[3] <= EPS) {
+            v[0] = v[1] = v[2] = v[3] = 0.0f;
+            continue;
+        }
+        float gs = 1.0f;
+        if (s->kinds[i] == CN_PARTICLE_KIND_FLUID)
+            gs = s->fluid_mats[s->fluid_mat[i]].gravity_scale;
+        const float dts = dt * gs;
+        const float vx = v[0] + g[0] * dts;
+        const float vy = v[1] + g[1] * dts;
+        const float vz = v[2] + g[2] * dts;
+        v[0] = vx;
+        v[1] = vy;
+        v[2] = vz;
+        v[3] = 0.0f;
+        p[0] = p[0] + vx * dt;
+        p[1] = p[1] + vy * dt;
+        p[2] = p[2] + vz * dt;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Broadphase
+ * ------------------------------------------------------------------------------------------ */
+
+/* ComputeParticleGridCellKey, assets/shaders/include/physics/particle/physics_particle_grid.hlsli:4-10 */
+static uint32_t cell_hash(int32_t gx, int32_t gy, int32_t gz)
+{
+    uint32_t seed = (uint32_t)gx * 73856093u;
+    seed ^= (uint32_t)gy * 19349663u;
+    seed ^= (uint32_t)gz * 83492791u;
+    return seed;
+}
+
+/* physics_particle_build_broad_phase_entries.cs.hlsl:9-35 + ..._keys.cs.hlsl:8-23 */
+void cno_broadphase_entries(uint32_t n, const float *pos, float cell_size, uint32_t *keys,
+                            int32_t *cells)
+{
+    const float safe = maxf(cell_size, 1.0e-4f);
+    const float inv = 1.0f / safe;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const int32_t gx = (int32_t)floorf(pos[4 * i + 0] * inv);
+        const int32_t gy = (int32_t)floorf(pos[4 * i + 1] * inv);
+        const int32_t gz = (int32_t)floorf(pos[4 * i + 2] * inv);
+        cells[3 * i + 0] = gx;
+        cells[3 * i + 1] = gy;
+        cells[3 * i + 2] = gz;
+        keys[i] = cell_hash(gx, gy, gz);
+    }
+}
+
+/* Exclusive scan, the result of dispatchExclusiveScanWithCpuCount
+ * (src/physics/physics_pass_dispatcher.cpp:445-602, physics_scan_block.cs.hlsl:10-52,
+ * physics_scan_add_offsets.cs.hlsl:7-22): out[i] = sum(in[0..i-1]) modulo 2^32. */
+static void exclusive_scan(uint32_t n, const uint32_t *in, uint32_t *out)
+{
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = acc;
+        acc += in[i];
+    }
+}
+
+/* dispatchSoftRadixSortPass, src/physics/physics_pass_dispatcher.cpp:3963-4063, with
+ * physics_radix_classify.cs.hlsl:7-16, physics_radix_finalize.cs.hlsl:8-24 and
+ * physics_radix_scatter.cs.hlsl:10-23: 32 one-bit split passes over (key, index) pairs. */
+void cno_radix_sort_1bit(uint32_t n, uint32_t *keys, uint32_t *indices)
+{
+    if (n == 0)
+        return;
+    uint32_t *k2 = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *i2 = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *flags = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *offs = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *ki = keys, *ii = indices, *ko = k2, *io = i2;
+    for (uint32_t bit = 0; bit < 32u; ++bit) {
+#pragma omp parallel for schedule(static)
+        for (int64_t t = 0; t < (int64_t)n; ++t)
+            flags[t] = (ki[t] >> bit) & 1u;
+        exclusive_scan(n, flags, offs);
+        const uint32_t total_ones = offs[n - 1] + flags[n - 1];
+        const uint32_t total_zeros = n - total_ones;
+#pragma omp parallel for schedule(static)
+        for (int64_t t = 0; t < (int64_t)n; ++t) {
+            const uint32_t dst =
+                flags[t] == 0u ? (uint32_t)t - offs[t] : total_zeros + offs[t];
+            ko[dst] = ki[t];
+            io[dst] = ii[t];
+        }
+        uint32_t *tk = ki; ki = ko; ko = tk;
+        uint32_t *ti = ii; ii = io; io = ti;
+    }
+    /* 32 passes: the result ends in the original buffers */
+    free(k2);
+    free(i2);
+    free(flags);
+    free(offs);
+}
+
+/* nextPowerOfTwo, src/physics/physics_solver.cpp:22-36 */
+uint32_t cno_next_pow2(uint32_t v)
+{
+    if (v <= 1u)
+        return 1u;
+    --v;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1u;
+}
+
+/* clearParticleCellRanges / markParticleCellRangeStarts / scan / buildParticleCellRanges:
+ * physics_particle_clear_cell_ranges.cs.hlsl:6-21, ..._mark_cell_range_starts.cs.hlsl:7-28,
+ * ..._build_cell_ranges.cs.hlsl:10-58; driver physics_pass_dispatcher.cpp:989-1085. */
+void cno_cell_ranges(uint32_t n, const uint32_t *sk, uint32_t capacity, uint32_t *ranges)
+{
+    for (uint32_t r = 0; r < capacity; ++r) {
+        ranges[4 * r + 0] = INVALID;
+        ranges[4 * r + 1] = 0;
+        ranges[4 * r + 2] = 0;
+        ranges[4 * r + 3] = 0;
+    }
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (i != 0 && sk[i - 1] == sk[i])
+            continue;
+        uint32_t end = i + 1;
+        while (end < n && sk[end] == sk[i])
+            ++end;
+        if (run < capacity) {
+            ranges[4 * run + 0] = sk[i];
+            ranges[4 * run + 1] = i;
+            ranges[4 * run + 2] = end;
+            ranges[4 * run + 3] = 0;
+        }
+        ++run;
+    }
+}
+
+/* FindParticleCellRange, assets/shaders/include/physics/fluid/physics_fluid_neighbor_build.hlsli:6-46.
+ * Returns the record at the lower bound; the caller treats key == 0xffffffff as missing. */
+int cno_find_cell_range(const uint32_t *ranges, uint32_t capacity, uint32_t key, uint32_t *start,
+                        uint32_t *end)
+{
+    if (capacity == 0)
+        return 0;
+    uint32_t lo = 0, hi = capacity;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (ranges[4 * mid] < key)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    if (lo < capacity && ranges[4 * lo] == key) {
+        if (ranges[4 * lo] == INVALID)
+            return 0;
+        *start = ranges[4 * lo + 1];
+        *end = ranges[4 * lo + 2];
+        return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Static box boundary
+ * ------------------------------------------------------------------------------------------ */
+
+/* ExpandBits / Morton3D, assets/shaders/physics/rigid/broad_phase/physics_rigid_morton_codes.cs.hlsl:8-26 */
+static uint32_t expand_bits(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+static float clampf(float x, float lo, float hi) { return minf(maxf(x, lo), hi); }
+static uint32_t morton3d(float x, float y, float z)
+{
+    x = clampf(x * 1024.0f, 0.0f, 1023.0f);
+    y = clampf(y * 1024.0f, 0.0f, 1023.0f);
+    z = clampf(z * 1024.0f, 0.0f, 1023.0f);
+    return expand_bits((uint32_t)x) * 4u + expand_bits((uint32_t)y) * 2u + expand_bits((uint32_t)z);
+}
+
+/* World AABB of a static identity-oriented box collider with the broad-phase margin:
+ * physics_rigid_update_world_aabbs.cs.hlsl:51-91 with ComputeBodyAabb
+ * (physics_shape_common.hlsli:49-75). */
+static void collider_aabb(const cn_box_collider *c, float *mn, float *mx)
+{
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = (c->center[a] - c->half_extents[a]) - BROADPHASE_MARGIN;
+        mx[a] = (c->center[a] + c->half_extents[a]) + BROADPHASE_MARGIN;
+    }
+}
+
+/* Visit order of the static LBVH traversal used by the fluid-boundary candidate passes
+ * (physics_particle_count_fluid_boundary_candidates.cs.hlsl:30-86: stack, right child popped
+ * first).  The static tree is a Karras LBVH over the Morton codes of the enabled static
+ * colliders (physics_pass_dispatcher.cpp:4163-4256, physics_rigid_morton_codes.cs.hlsl,
+ * physics_rigid_bvh_hierarchy.cs.hlsl), whose in-order leaf sequence is the stable Morton order;
+ * a right-first depth-first walk therefore meets the leaves in reverse stable Morton order. */
+void cno_static_collider_order(uint32_t nc, const cn_box_collider *c, uint32_t *order)
+{
+    uint32_t ne = 0;
+    uint32_t *elem = (uint32_t *)malloc(sizeof(uint32_t) * (nc ? nc : 1));
+    for (uint32_t i = 0; i < nc; ++i)
+        if (c[i].enabled)
+            elem[ne++] = i;
+    float gmin[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+    float gmax[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+    for (uint32_t e = 0; e < ne; ++e) {
+        float mn[3], mx[3];
+        collider_aabb(&c[elem[e]], mn, mx);
+        for (int a = 0; a < 3; ++a) {
+            gmin[a] = minf(gmin[a], mn[a]);
+            gmax[a] = maxf(gmax[a], mx[a]);
+        }
+    }
+    uint32_t *codes = (uint32_t *)malloc(sizeof(uint32_t) * (ne ? ne : 1));
+    uint32_t *eidx = (uint32_t *)malloc(sizeof(uint32_t) * (ne ? ne : 1));
+    for (uint32_t e = 0; e < ne; ++e) {
+        float mn[3], mx[3], m[3];
+        collider_aabb(&c[elem[e]], mn, mx);
+        for (int a = 0; a < 3; ++a) {
+            const float size = maxf(gmax[a] - gmin[a], 1.0e-5f);
+            const float center = 0.5f * (mn[a] + mx[a]);
+            m[a] = saturatef((center - gmin[a]) / size);
+        }
+        codes[e] = morton3d(m[0], m[1], m[2]);
+        eidx[e] = e;
+    }
+    cno_radix_sort_1bit(ne, codes, eidx);
+    for (uint32_t k = 0; k < ne; ++k)
+        order[k] = elem[eidx[ne - 1 - k]];
+    for (uint32_t k = ne; k < nc; ++k)
+        order[k] = INVALID;
+    free(codes);
+    free(eidx);
+    free(elem);
+}
+
+/* ComputeFluidBoundaryContactInfo for a box with identity orientation,
+ * assets/shaders/include/physics/fluid/physics_fluid_boundary_common.hlsli:10-66 and
+ * ClosestFaceNormalLocal (physics_shape_common.hlsli:100-108). */
+static int box_contact(const cn_box_collider *c, const float *p, float *sd, float *nrm)
+{
+    nrm[0] = 0.0f;
+    nrm[1] = 1.0f;
+    nrm[2] = 0.0f;
+    *sd = 0.0f;
+    if (!c->enabled)
+        return 0;
+    float local[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        local[a] = p[a] - c->center[a];
+        const float cl = clampf(local[a], -c->half_extents[a], c->half_extents[a]);
+        d[a] = local[a] - cl;
+    }
+    const float dl = len3(d);
+    if (dl > EPS) {
+        nrm[0] = d[0] / dl;
+        nrm[1] = d[1] / dl;
+        nrm[2] = d[2] / dl;
+        *sd = dl;
+    } else {
+        float dist[3];
+        for (int a = 0; a < 3; ++a)
+            dist[a] = c->half_extents[a] - fabsf(local[a]);
+        nrm[0] = nrm[1] = nrm[2] = 0.0f;
+        if (dist[0] <= dist[1] && dist[0] <= dist[2])
+            nrm[0] = local[0] >= 0.0f ? 1.0f : -1.0f;
+        else if (dist[1] <= dist[2])
+            nrm[1] = local[1] >= 0.0f ? 1.0f : -1.0f;
+        else
+            nrm[2] = local[2] >= 0.0f ? 1.0f : -1.0f;
+        *sd = -minf(dist[0], minf(dist[1], dist[2]));
+    }
+    return 1;
+}
+
+/* Count + scan + emit of buildFluidBoundaryCandidatePairs (physics_pass_dispatcher.cpp:1305-1407,
+ * physics_particle_{count,emit,finalize}_fluid_boundary_candidates.cs.hlsl).  ranges[2i] =
+ * offset, ranges[2i+1] = count, written for every particle (the reference writes a range only
+ * when count > 0; a stale range only names colliders that lie beyond the 2h query box and so
+ * contributes nothing).  Returns the required pair count. */
+uint32_t cno_fluid_boundary_candidates(const cno_scene *s, const uint32_t *order, uint32_t *ranges,
+                                       uint32_t *pairs, uint32_t capacity)
+{
+    const uint32_t n = s->n;
+    uint32_t *counts = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint32_t *offs = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint32_t nvis = 0;
+    while (nvis < s->n_colliders && order[nvis] != INVALID)
+        ++nvis;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            exclusive_scan(n, counts, offs);
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            if (s->kinds[i] != CN_PARTICLE_KIND_FLUID) {
+                if (pass == 0)
+                    counts[i] = 0;
+                else {
+                    ranges[2 * i] = offs[i];
+                    ranges[2 * i + 1] = 0;
+                }
+                continue;
+            }
+            const float *p = s->pos + 4 * i;
+            const uint32_t *md = s->meta + 4 * i;
+            const float r = maxf(s->radii[i], 0.0f);
+            const cn_fluid_material *m = &s->fluid_mats[s->fluid_mat[i]];
+            const float support = maxf(m->smoothing_radius, r);
+            const float reuse = maxf(maxf(m->cfl_radius, r), support);
+            const float qr = support + reuse;
+            float qmin[3], qmax[3];
+            for (int a = 0; a < 3; ++a) {
+                qmin[a] = p[a] - qr;
+                qmax[a] = p[a] + qr;
+            }
+            uint32_t cnt = 0, w = pass ? offs[i] : 0;
+            for (uint32_t k = 0; k < nvis; ++k) {
+                const cn_box_collider *c = &s->colliders[order[k]];
+                float mn[3], mx[3];
+                collider_aabb(c, mn, mx);
+                const int overlap = qmin[0] <= mx[0] && qmax[0] >= mn[0] && qmin[1] <= mx[1] &&
+                                    qmax[1] >= mn[1] && qmin[2] <= mx[2] && qmax[2] >= mn[2];
+                if (!overlap)
+                    continue;
+                if (!(c->enabled && c->environment == md[0] && (md[3] & c->layer) != 0u &&
+                      (c->mask & md[2]) != 0u))
+                    continue;
+                if (pass == 1) {
+                    if (w >= capacity)
+                        break;
+                    pairs[w++] = order[k];
+                }
+                ++cnt;
+            }
+            if (pass == 0)
+                counts[i] = cnt;
+            else {
+                ranges[2 * i] = offs[i];
+                ranges[2 * i + 1] = counts[i];
+            }
+        }
+    }
+    const uint32_t required = n ? offs[n - 1] + counts[n - 1] : 0;
+    free(counts);
+    free(offs);
+    return required;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Fluid passes
+ * ------------------------------------------------------------------------------------------ */
+
+/* FluidSpikyKernel / FluidSpikyKernelDerivative / FluidCohesionKernel,
+ * assets/shaders/include/physics/fluid/physics_fluid_common.hlsli:17-65 */
+static float spiky_w(float d, float h)
+{
+    if (d >= h)
+        return 0.0f;
+    const float omq = 1.0f - d / h;
+    const float h3 = h * h * h;
+    const float k = 15.0f / (KERNEL_PI * h3);
+    return k * omq * omq;
+}
+static float spiky_dw(float d, float h)
+{
+    if (d <= EPS || d >= h)
+        return 0.0f;
+    const float omq = 1.0f - d / h;
+    const float h3 = h * h * h;
+    const float k = 15.0f / (KERNEL_PI * h3);
+    return -2.0f * k * omq / h;
+}
+static float cohesion_kernel(float d, float h, float c1, float c2)
+{
+    if (d <= EPS || d >= h)
+        return 0.0f;
+    const float q = d / h;
+    return c1 * q * q * q + c2 * q * q - 1.0f;
+}
+
+/* ShouldProcessFluidNeighbor, physics_fluid_neighbor_build.hlsli:48-70 */
+static int should_process(const cno_scene *s, uint32_t i, uint32_t env, uint32_t layer,
+                          uint32_t mask, uint32_t j)
+{
+    if (j == i)
+        return 0;
+    if (s->kinds[j] != CN_PARTICLE_KIND_FLUID)
+        return 0;
+    const uint32_t *mj = s->meta + 4 * j;
+    if (mj[0] != env)
+        return 0;
+    return (mask & mj[2]) != 0u && (mj[3] & layer) != 0u;
+}
+
+/* physics_fluid_build_neighbors.cs.hlsl:21-147.  required[i] counts in-radius neighbors before
+ * the slot budget (the reference truncates silently, :143-146). */
+void cno_fluid_neighbors(const cno_scene *s, float cell_size, const int32_t *cells,
+                         const uint32_t *sk, const uint32_t *si, const uint32_t *tbl,
+                         uint32_t capacity, const float *dacc, uint32_t it, uint32_t cap,
+                         uint32_t *counts, uint32_t *list, uint32_t *required)
+{
+    (void)cell_size;
+    (void)sk;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const float *pi = s->pos + 4 * i;
+        if (pi[3] <= EPS)
+            continue;
+        if (cap == 0) {
+            counts[i] = 0;
+            if (required)
+                required[i] = 0;
+            continue;
+        }
+        float sp[3];
+        for (int a = 0; a < 3; ++a)
+            sp[a] = pi[a] + (it > 0 ? dacc[4 * i + a] : 0.0f);
+        const float h = maxf(s->fluid_mats[s->fluid_mat[i]].smoothing_radius, 1.0e-4f);
+        const uint32_t env = s->meta[4 * i], layer = s->meta[4 * i + 2], mask = s->meta[4 * i + 3];
+        const int32_t *ci = cells + 3 * i;
+        uint32_t cnt = 0, req = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int32_t tx = ci[0] + dx, ty = ci[1] + dy, tz = ci[2] + dz;
+                    uint32_t start, end;
+                    if (!cno_find_cell_range(tbl, capacity, cell_hash(tx, ty, tz), &start, &end))
+                        continue;
+                    for (uint32_t q = start; q < end; ++q) {
+                        const uint32_t j = si[q];
+                        const int32_t *cj = cells + 3 * j;
+                        if (cj[0] != tx || cj[1] != ty || cj[2] != tz)
+                            continue;
+                        if (!should_process(s, i, env, layer, mask, j))
+                            continue;
+                        const float *pj = s->pos + 4 * j;
+                        if (pj[3] <= EPS)
+                            continue;
+                        float d[3];
+                        for (int a = 0; a < 3; ++a)
+                            d[a] = sp[a] - (pj[a] + (it > 0 ? dacc[4 * j + a] : 0.0f));
+                        if (len3(d) >= h)
+                            continue;
+                        ++req;
+                        if (cnt >= cap)
+                            continue;
+                        list[(uint64_t)i * cap + cnt] = j;
+                        ++cnt;
+                    }
+                }
+        counts[i] = cnt;
+        if (required)
+            required[i] = req;
+    }
+}
+
+/* ComputeFluidBoundaryGhostDelta, physics_fluid_boundary_common.hlsli:90-110 */
+static int ghost_delta(const cno_scene *s, uint32_t collider, const float *p, float h, float *g)
+{
+    float sd, n[3];
+    g[0] = g[1] = g[2] = 0.0f;
+    if (!box_contact(&s->colliders[collider], p, &sd, n))
+        return 0;
+    if (sd < 0.0f || sd >= h)
+        return 0;
+    const float t = 2.0f * sd;
+    g[0] = n[0] * t;
+    g[1] = n[1] * t;
+    g[2] = n[2] * t;
+    return 1;
+}
+
+/* physics_fluid_compute_density_lambda.cs.hlsl:26-115 with
+ * AccumulateFluidBoundaryDensityFromCandidates (physics_fluid_boundary_common.hlsli:112-138) */
+void cno_fluid_density(const cno_scene *s, const float *cfg, const float *dacc, uint32_t it,
+                       uint32_t cap, const uint32_t *counts, const uint32_t *list,
+                       const uint32_t *br, const uint32_t *bp, float *nc)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *o = nc + 4 * i;
+        const float *pi = s->pos + 4 * i;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID || pi[3] <= EPS) {
+            o[0] = o[1] = o[2] = o[3] = 0.0f;
+            continue;
+        }
+        float sp[3];
+        for (int a = 0; a < 3; ++a)
+            sp[a] = pi[a] + (it > 0 ? dacc[4 * i + a] : 0.0f);
+        const cn_fluid_material *m = &s->fluid_mats[s->fluid_mat[i]];
+        const float rho0 = maxf(m->rest_density, 1.0f);
+        const float dcs = m->density_constraint_scale;
+        const float h = maxf(m->smoothing_radius, 1.0e-4f);
+        const float st = maxf(m->surface_tension, 0.0f);
+        float density = 0.0f, sn[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < counts[i]; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            const float *pj = s->pos + 4 * j;
+            float d[3];
+            for (int a = 0; a < 3; ++a)
+                d[a] = sp[a] - (pj[a] + (it > 0 ? dacc[4 * j + a] : 0.0f));
+            const float dist = len3(d);
+            if (dist >= h)
+                continue;
+            density += spiky_w(dist, h);
+            if (st > EPS) {
+                /* FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53 */
+                float gr[3] = {0.0f, 0.0f, 0.0f};
+                if (!(dist <= EPS || dist >= h)) {
+                    const float dw = spiky_dw(dist, h);
+                    for (int a = 0; a < 3; ++a)
+                        gr[a] = dw * (d[a] / dist);
+                }
+                for (int a = 0; a < 3; ++a)
+                    sn[a] += gr[a];
+            }
+        }
+        for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
+            float g[3];
+            if (!ghost_delta(s, bp[br[2 * i] + k], sp, h, g))
+                continue;
+            const float dist = len3(g);
+            if (dist >= h)
+                continue;
+            density += spiky_w(dist, h) * cfg[CFG_FLUID_BND_DENSITY];
+        }
+        const float C =
+            maxf(density - rho0, -rho0 * maxf(cfg[CFG_FLUID_MAX_UNDER], 0.0f)) * dcs;
+        if (st > EPS) {
+            o[0] = sn[0] * st;
+            o[1] = sn[1] * st;
+            o[2] = sn[2] * st;
+        } else {
+            o[0] = o[1] = o[2] = 0.0f;
+        }
+        o[3] = C;
+    }
+}
+
+/* physics_fluid_compute_delta_positions.cs.hlsl:27-157 with
+ * ComputeFluidBoundaryDeltaFromCandidates (physics_fluid_boundary_common.hlsli:140-167) */
+void cno_fluid_delta(const cno_scene *s, const float *cfg, float dt, const float *dacc,
+                     uint32_t it, uint32_t cap, const uint32_t *counts, const uint32_t *list,
+                     const uint32_t *br, const uint32_t *bp, const float *nc, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *o = out + 4 * i;
+        const float *pi = s->pos + 4 * i;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID || pi[3] <= EPS) {
+            o[0] = o[1] = o[2] = o[3] = 0.0f;
+            continue;
+        }
+        float sd_[3], sp[3];
+        for (int a = 0; a < 3; ++a) {
+            sd_[a] = it > 0 ? dacc[4 * i + a] : 0.0f;
+            sp[a] = pi[a] + sd_[a];
+        }
+        const float Ci = nc[4 * i + 3];
+        const uint32_t mi = s->fluid_mat[i];
+        const cn_fluid_material *m = &s->fluid_mats[mi];
+        const float h = maxf(m->smoothing_radius, 1.0e-4f);
+        const float cohesion = dt * maxf(m->cohesion, 0.0f);
+        const float viscosity = dt * maxf(m->viscosity, 0.0f);
+        const float st = maxf(m->surface_tension, 0.0f);
+        const float cfl = maxf(m->cfl_radius, 0.0f);
+        float dp[3] = {0.0f, 0.0f, 0.0f}, wsum = 0.0f;
+        float sn[3] = {0.0f, 0.0f, 0.0f};
+        if (st > EPS)
+            for (int a = 0; a < 3; ++a)
+                sn[a] = nc[4 * i + a];
+        for (uint32_t k = 0; k < counts[i]; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            const float *pj = s->pos + 4 * j;
+            if (pj[3] <= EPS)
+                continue;
+            float nd[3], d[3];
+            for (int a = 0; a < 3; ++a) {
+                nd[a] = it > 0 ? dacc[4 * j + a] : 0.0f;
+                d[a] = sp[a] - (pj[a] + nd[a]);
+            }
+            const float dist = len3(d);
+            if (dist >= h || dist <= EPS)
+                continue;
+            const float Cj = nc[4 * j + 3];
+            float nij[3], pd[3];
+            for (int a = 0; a < 3; ++a)
+                nij[a] = d[a] / dist;
+            const float der = spiky_dw(dist, h);
+            const float sc = -0.5f * (Ci + Cj) * der;
+            for (int a = 0; a < 3; ++a)
+                pd[a] = sc * nij[a];
+            if (mi == s->fluid_mat[j]) {
+                float rel[3];
+                for (int a = 0; a < 3; ++a)
+                    rel[a] = sd_[a] - nd[a];
+                if (cohesion > EPS) {
+                    const float ck = cohesion * cohesion_kernel(dist, h, m->cohesion1, m->cohesion2);
+                    for (int a = 0; a < 3; ++a)
+                        pd[a] -= nij[a] * ck;
+                }
+                if (viscosity > EPS) {
+                    const float vs = 1.0f - (1.0f / (1.0f + viscosity * spiky_w(dist, h)));
+                    for (int a = 0; a < 3; ++a)
+                        pd[a] -= vs * rel[a];
+                }
+                if (cfl > EPS) {
+                    const float proj = dot3(rel, nij);
+                    if (proj < -cfl) {
+                        const float pc = proj + cfl;
+                        for (int a = 0; a < 3; ++a)
+                            pd[a] -= nij[a] * pc * 0.5f;
+                    }
+                }
+                if (st > EPS) {
+                    for (int a = 0; a < 3; ++a)
+                        pd[a] -= dt * (sn[a] - nc[4 * j + a]);
+                }
+            }
+            for (int a = 0; a < 3; ++a)
+                dp[a] += pd[a];
+            wsum += 1.0f;
+        }
+        float bd[3] = {0.0f, 0.0f, 0.0f};
+        const float sc = maxf(Ci, 0.0f);
+        for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
+            float g[3];
+            if (!ghost_delta(s, bp[br[2 * i] + k], sp, h, g))
+                continue;
+            const float dist = len3(g);
+            if (dist <= EPS || dist >= h)
+                continue;
+            const float der = spiky_dw(dist, h);
+            const float f = -sc * der;
+            for (int a = 0; a < 3; ++a)
+                bd[a] += f * (g[a] / dist);
+        }
+        for (int a = 0; a < 3; ++a)
+            o[a] = dp[a] * cfg[CFG_FLUID_SOLVE] + bd[a] * cfg[CFG_FLUID_BND_DELTA];
+        o[3] = wsum;
+    }
+}
+
+/* physics_fluid_apply_delta_positions.cs.hlsl:11-31 */
+void cno_fluid_apply(const cno_scene *s, const float *cfg, uint32_t it, const float *din,
+                     float *dacc)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const float *d = din + 4 * i;
+        const float w = maxf(d[3] * cfg[CFG_FLUID_RELAX], 1.0f);
+        float *acc = dacc + 4 * i;
+        for (int a = 0; a < 3; ++a)
+            acc[a] = (it > 0 ? acc[a] : 0.0f) + d[a] / w;
+        acc[3] = 0.0f;
+    }
+}
+
+/* physics_fluid_clamp_boundary.cs.hlsl:23-80 */
+void cno_fluid_clamp(const cno_scene *s, const float *din, const uint32_t *br, const uint32_t *bp,
+                     float *dout)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const float *b = s->pos + 4 * i;
+        const float r = maxf(s->radii[i], 0.0f);
+        float cur[3];
+        for (int a = 0; a < 3; ++a)
+            cur[a] = b[a] + din[4 * i + a];
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            int changed = 0;
+            for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
+                float sd, n[3];
+                if (!box_contact(&s->colliders[bp[br[2 * i] + k]], cur, &sd, n))
+                    continue;
+                if (sd >= r)
+                    continue;
+                const float push = r - sd;
+                for (int a = 0; a < 3; ++a)
+                    cur[a] += n[a] * push;
+                changed = 1;
+            }
+            if (!changed)
+                break;
+        }
+        for (int a = 0; a < 3; ++a)
+            dout[4 * i + a] = cur[a] - b[a];
+        dout[4 * i + 3] = 0.0f;
+    }
+}
+
+/* physics_particle_update_velocities.cs.hlsl:13-44 */
+void cno_update_velocities(cno_scene *s, float dt, float *dacc)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *p = s->pos + 4 * i;
+        float *v = s->vel + 4 * i;
+        if (p[3] <= EPS || dt <= EPS) {
+            v[0] = v[1] = v[2] = v[3] = 0.0f;
+            continue;
+        }
+        const float *pv = s->prev + 4 * i;
+        float x[3] = {p[0], p[1], p[2]};
+        if (s->kinds[i] == CN_PARTICLE_KIND_FLUID) {
+            for (int a = 0; a < 3; ++a)
+                x[a] = x[a] + dacc[4 * i + a];
+            p[0] = x[0];
+            p[1] = x[1];
+            p[2] = x[2];
+            dacc[4 * i] = dacc[4 * i + 1] = dacc[4 * i + 2] = dacc[4 * i + 3] = 0.0f;
+        }
+        const float damping = maxf(s->part_mats[4 * s->part_mat[i] + 2], 0.0f);
+        const float scale = maxf(1.0f - damping * dt, 0.0f);
+        for (int a = 0; a < 3; ++a)
+            v[a] = ((x[a] - pv[a]) / dt) * scale;
+        v[3] = 0.0f;
+    }
+}
+
+/* ComputePositionFrictionDelta, physics_rigid_solver_shared.hlsli:50-67 */
+static void friction_delta(const float *cfg, const float *disp, float pen, float kin, float stat,
+                           float *out)
+{
+    const float td = len3(disp);
+    out[0] = out[1] = out[2] = 0.0f;
+    if (td <= cfg[CFG_POS_FRICTION_MIN] || pen <= EPS)
+        return;
+    const float sl = saturatef(stat) * pen;
+    if (td <= sl) {
+        out[0] = disp[0];
+        out[1] = disp[1];
+        out[2] = disp[2];
+        return;
+    }
+    const float ks = minf(saturatef(kin) * pen / td, 1.0f);
+    for (int a = 0; a < 3; ++a)
+        out[a] = disp[a] * ks;
+}
+
+/* physics_fluid_project_boundary_velocities.cs.hlsl:26-118 with CombineContactMaterial and
+ * ProjectOntoContactTangent (physics_rigid_solver_shared.hlsli:37-48) */
+void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt, const uint32_t *br,
+                                     const uint32_t *bp)
+{
+    const float slop = cfg[CFG_CONTACT_SLOP];
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const float *p = s->pos + 4 * i;
+        const float r = maxf(s->radii[i], 0.0f);
+        float *vv = s->vel + 4 * i;
+        float v[3] = {vv[0], vv[1], vv[2]};
+        const float *pm = s->part_mats + 4 * s->part_mat[i];
+        const float damping = maxf(pm[2], 0.0f);
+        float best = 3.402823466e+38f, bn[3] = {0.0f, 1.0f, 0.0f}, bm[3] = {0.0f, 0.0f, 0.0f};
+        int found = 0;
+        for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
+            const uint32_t ci = bp[br[2 * i] + k];
+            float sd, n[3];
+            if (!box_contact(&s->colliders[ci], p, &sd, n))
+                continue;
+            const float cd = r + 2.0f * slop;
+            if (sd > cd)
+                continue;
+            if (sd < best) {
+                best = sd;
+                bn[0] = n[0];
+                bn[1] = n[1];
+                bn[2] = n[2];
+                const float *cm = s->colliders[ci].material;
+                bm[0] = sqrtf(maxf(0.0f, pm[0]) * maxf(0.0f, cm[0]));
+                bm[1] = maxf(maxf(0.0f, pm[1]), maxf(0.0f, cm[1]));
+                bm[2] = sqrtf(maxf(0.0f, pm[3]) * maxf(0.0f, cm[3]));
+                found = 1;
+            }
+        }
+        if (!found)
+            continue;
+        const float pen = maxf(r - best, 0.0f);
+        const float cd = maxf(r + 2.0f * slop, 1.0e-5f);
+        const float cw = saturatef((cd - best) / cd);
+        float rv[3] = {v[0], v[1], v[2]};
+        const float nv = dot3(rv, bn);
+        if (nv < 0.0f)
+            for (int a = 0; a < 3; ++a)
+                rv[a] -= bn[a] * nv;
+        float tv[3];
+        const float rn = dot3(rv, bn);
+        for (int a = 0; a < 3; ++a)
+            tv[a] = rv[a] - bn[a] * rn;
+        if (dt > EPS) {
+            float disp[3], fd[3];
+            for (int a = 0; a < 3; ++a)
+                disp[a] = tv[a] * dt;
+            friction_delta(cfg, disp, maxf(pen, slop), bm[0], bm[2], fd);
+            for (int a = 0; a < 3; ++a)
+                tv[a] -= fd[a] / dt;
+        }
+        const float tds = maxf(1.0f - damping * cw * dt, 0.0f);
+        for (int a = 0; a < 3; ++a)
+            tv[a] *= tds;
+        const float rn2 = maxf(dot3(rv, bn), 0.0f);
+        for (int a = 0; a < 3; ++a)
+            vv[a] = bn[a] * rn2 + tv[a];
+        vv[3] = 0.0f;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * PhysicsSolver::step for the particle path, src/physics/physics_solver.cpp:336-1401
+ * ------------------------------------------------------------------------------------------ */
+static uint32_t resolve_iters(uint32_t v, uint32_t d) { return v > 0 ? v : d; }
+
+int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
+{
+    const uint32_t n = s->n;
+    const uint32_t substeps = prm->substeps > 0 ? prm->substeps : 1u;
+    const uint32_t def = prm->default_iterations > 0 ? prm->default_iterations : 1u;
+    const uint32_t fluid_iters = resolve_iters(prm->fluid_iterations, def);
+    const uint32_t soft_iters = resolve_iters(prm->soft_internal_iterations, def);
+    const uint32_t contact_iters = resolve_iters(prm->soft_contact_iterations, def);
+    uint32_t max_iters = fluid_iters;
+    if (soft_iters > max_iters)
+        max_iters = soft_iters;
+    if (contact_iters > max_iters)
+        max_iters = contact_iters;
+    if (def > max_iters)
+        max_iters = def; /* rigid joint / rigid contact iterations resolve to the default */
+    const float dt = prm->dt / (float)substeps;
+    const float *cfg = prm->solver_config;
+    const uint32_t cap = prm->max_fluid_neighbors;
+    float cell = prm->cell_size;
+    if (cell <= 0.0f)
+        cell = cno_grid_cell_size(n, s->radii, s->fluid_mat, s->n_fluid_mats, s->fluid_mats);
+
+    uint32_t fluid_count = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        fluid_count += s->kinds[i] == CN_PARTICLE_KIND_FLUID;
+    const int has_fluid = fluid_count > 0 && n > 0;
+    const int has_bnd = has_fluid && s->n_colliders > 0;
+
+    const uint32_t capacity = cno_next_pow2(n * 2u > 1u ? n * 2u : 1u);
+    uint32_t *keys = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    int32_t *cells = (int32_t *)malloc(sizeof(int32_t) * 3 * (n ? n : 1));
+    uint32_t *tbl = (uint32_t *)malloc(sizeof(uint32_t) * 4 * capacity);
+    uint32_t *counts = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint32_t *req = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint32_t *list = (uint32_t *)malloc(sizeof(uint32_t) * ((uint64_t)(n ? n : 1) * (cap ? cap : 1)));
+    float *dacc = (float *)calloc(4 * (size_t)(n ? n : 1), sizeof(float));
+    float *dtmp = (float *)calloc(4 * (size_t)(n ? n : 1), sizeof(float));
+    float *nc = (float *)calloc(4 * (size_t)(n ? n : 1), sizeof(float));
+    uint32_t *br = (uint32_t *)calloc(2 * (size_t)(n ? n : 1), sizeof(uint32_t));
+    const uint32_t bcap = 32u * n;  /* fluidBoundaryCandidatePairCapacity, physics_scene_gpu_state.cpp:728-729 */
+    uint32_t *bp = (uint32_t *)calloc(bcap ? bcap : 1, sizeof(uint32_t));
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_colliders ? s->n_colliders : 1));
+    if (!keys || !idx || !cells || !tbl || !counts || !req || !list || !dacc || !dtmp || !nc ||
+        !br || !bp || !order)
+        return -1;
+    cno_static_collider_order(s->n_colliders, s->colliders, order);
+
+    cno_stats st;
+    memset(&st, 0, sizeof(st));
+    st.fluid_count = fluid_count;
+    const int has_bp_work = (contact_iters > 0 && n > 1) || has_fluid || s->n_colliders > 0;
+
+    for (uint32_t sub = 0; sub < substeps; ++sub) {
+        cno_predict(s, dt, prm->gravity);
+        if (has_bp_work) {
+            cno_broadphase_entries(n, s->pos, cell, keys, cells);
+            for (uint32_t i = 0; i < n; ++i)
+                idx[i] = i;
+            cno_radix_sort_1bit(n, keys, idx);
+            cno_cell_ranges(n, keys, capacity, tbl);
+        }
+        if (has_bnd)
+            cno_fluid_boundary_candidates(s, order, br, bp, bcap);
+        for (uint32_t it = 0; it < max_iters; ++it) {
+            if (has_fluid && it < fluid_iters) {
+                cno_fluid_neighbors(s, cell, cells, keys, idx, tbl, capacity, dacc, it, cap,
+                                    counts, list, req);
+                if (it == 0) {
+                    uint64_t tot = 0;
+                    for (uint32_t i = 0; i < n; ++i)
+                        if (s->kinds[i] == CN_PARTICLE_KIND_FLUID && s->pos[4 * i + 3] > EPS)
+                            tot += counts[i];
+                    st.neighbor_total_first_iter = tot;
+                }
+                for (uint32_t i = 0; i < n; ++i)
+                    if (s->kinds[i] == CN_PARTICLE_KIND_FLUID && s->pos[4 * i + 3] > EPS) {
+                        if (req[i] > st.max_required_neighbors)
+                            st.max_required_neighbors = req[i];
+                        if (req[i] > cap)
+                            ++st.truncations;
+                    }
+                cno_fluid_density(s, cfg, dacc, it, cap, counts, list, br, bp, nc);
+                cno_fluid_delta(s, cfg, dt, dacc, it, cap, counts, list, br, bp, nc, dtmp);
+                cno_fluid_apply(s, cfg, it, dtmp, dacc);
+                cno_fluid_clamp(s, dacc, br, bp, dacc);
+            }
+        }
+        cno_update_velocities(s, dt, dacc);
+        if (has_fluid) {
+            cno_project_boundary_velocities(s, cfg, dt, br, bp);
+            /* post-update neighbor rebuild (physics_solver.cpp:1262); it feeds only vorticity
+             * and render anisotropy, so it changes no particle state */
+            cno_fluid_neighbors(s, cell, cells, keys, idx, tbl, capacity, dacc,
+                                max_iters > 0 ? max_iters - 1 : 0, cap, counts, list, NULL);
+        }
+    }
+    /* density-constraint residual of the last density pass of the step */
+    if (has_fluid) {
+        double mx = 0.0, sum = 0.0;
+        uint32_t cntf = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            if (s->kinds[i] == CN_PARTICLE_KIND_FLUID && s->pos[4 * i + 3] > EPS) {
+                const double c = nc[4 * i + 3];
+                if (fabs(c) > mx)
+                    mx = fabs(c);
+                sum += c;
+                ++cntf;
+            }
+        st.max_abs_constraint_last = mx;
+        st.mean_constraint_last = cntf ? sum / cntf : 0.0;
+    }
+    if (stats)
+        *stats = st;
+    free(keys);
+    free(idx);
+    free(cells);
+    free(tbl);
+    free(counts);
+    free(req);
+    free(list);
+    free(dacc);
+    free(dtmp);
+    free(nc);
+    free(br);
+    free(bp);
+    free(order);
+    return 0;
+}
