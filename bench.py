@@ -91,7 +91,7 @@ def main():
     s.synchronize()
     el = time.perf_counter() - t0
     meta = s.meta()
-    kbar = meta.fluid_neighbor_total / sc.n
+    kbar = meta.fluid_neighbor_total / (sc.n * SUBSTEPS)
     value = sc.n * SUBSTEPS * args.steps / el
     ms_per_step = el * 1e3 / args.steps
     print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}",

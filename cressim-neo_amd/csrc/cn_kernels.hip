@@ -196,14 +196,14 @@ __global__ __launch_bounds__(kBlock) void k_rank_fix(const uint2 *__restrict__ k
     if (i >= n) return;
     const uint32_t r = elem_run[i];
     const uint32_t s = run_start[r], e = run_start[r + 1];
-    const uint32_t o = kv[i].y;
+    const uint32_t o = min(kv[i].y, n - 1u);
     if (e - s == 1u) {
         perm[i] = o;
         return;
     }
     const uint32_t me = id_old[o];
     uint32_t rank = 0;
-    for (uint32_t q = s; q < e; ++q) rank += id_old[kv[q].y] < me ? 1u : 0u;
+    for (uint32_t q = s; q < e; ++q) rank += id_old[min(kv[q].y, n - 1u)] < me ? 1u : 0u;
     perm[s + rank] = o;
 }
 
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= c.n) return;
-    const uint32_t o = perm[t];
+    const uint32_t o = min(perm[t], c.n - 1u);  // perm is a permutation; the clamp only bounds reads
     float4 p = pos_o[o], v = vel_o[o], pv;
     const uint32_t k = cls_o[o];
     const ParticleClass pc = classes[c.uniform_class ? 0u : k];
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ po
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    const uint32_t d = id[t];
+    const uint32_t d = min(id[t], n - 1u);
     opos[d] = pos[t];
     ovel[d] = vel[t];
     oprev[d] = prev[t];

@@ -184,8 +184,9 @@ typedef struct cn_meta {
     uint32_t fluid_neighbor_truncations;  /* particles whose list hit max_fluid_neighbors */
     uint32_t max_cell_occupancy;          /* longest run of equal keys in the last sort */
     uint32_t verlet_rebuilds;             /* reserved */
-    uint64_t fluid_neighbor_total;        /* sum of listed neighbors over the last substep's
-                                             first iteration (for the mean K) */
+    uint64_t fluid_neighbor_total;        /* listed neighbors at iteration 0, summed over the
+                                             substeps of the last cn_step (mean K = total /
+                                             (particles * substeps)) */
     uint32_t last_particle_count;
     uint32_t reserved[5];
 } cn_meta;

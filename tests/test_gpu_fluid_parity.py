@@ -176,7 +176,7 @@ def test_large_pbf_4m_properties(solver):
     pos, prev, vel = solver.readback()
     m = solver.meta()
     assert m.fluid_neighbor_truncations == 0 and m.max_fluid_neighbor_count <= 64
-    kbar = m.fluid_neighbor_total / sc.n
+    kbar = m.fluid_neighbor_total / (sc.n * p.substeps)
     assert 18 < kbar < 34, kbar
     assert np.isfinite(pos).all() and np.isfinite(vel).all()
     r = sc.radii[0]
