@@ -94,8 +94,11 @@ def main():
     kbar = meta.fluid_neighbor_total / (sc.n * SUBSTEPS)
     value = sc.n * SUBSTEPS * args.steps / el
     ms_per_step = el * 1e3 / args.steps
-    print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}",
-          file=sys.stderr, flush=True)
+    import struct
+    maxd = struct.unpack("f", struct.pack("I", meta.max_displacement2_bits))[0] ** 0.5
+    print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}, "
+          f"re-walks {meta.verlet_rebuilds}, LDS tiles {meta.lds_tiles}, run-table tiles {meta.run_table_tiles}, "
+          f"max displacement {maxd:.3e}", file=sys.stderr, flush=True)
 
     # per-kernel HIP-event timing on the context stream, in separate steps
     s.profile(True)

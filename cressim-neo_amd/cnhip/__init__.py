@@ -47,6 +47,17 @@ class Scene:
     colliders: np.ndarray               # structured COLLIDER_DTYPE
     name: str = "scene"
     info: dict = field(default_factory=dict)
+    edges: np.ndarray | None = None      # authoring.EDGE_DTYPE
+    bends: np.ndarray | None = None      # authoring.BEND_DTYPE
+    dihedrals: np.ndarray | None = None  # authoring.DIHEDRAL_DTYPE
+    tets: np.ndarray | None = None       # authoring.TET_DTYPE
+
+    def constraint_arrays(self):
+        from . import authoring as au
+        def arr(a, dt):
+            return np.ascontiguousarray(a if a is not None else np.zeros(0, dt))
+        return (arr(self.edges, au.EDGE_DTYPE), arr(self.bends, au.BEND_DTYPE),
+                arr(self.dihedrals, au.DIHEDRAL_DTYPE), arr(self.tets, au.TET_DTYPE))
 
     @property
     def n(self) -> int:
@@ -148,7 +159,11 @@ class Solver:
                                    p(arrs["ly"]), p(arrs["mk"]))
         tables = abi.SceneTables(len(arrs["fmats"]), p(arrs["fmats"]), len(arrs["pmats"]), p(arrs["pmats"]),
                                  len(arrs["cols"]), p(arrs["cols"]))
-        self._check(self.lib.cn_scene_upload(self._ctx, C.byref(view), C.byref(tables), None), "cn_scene_upload")
+        e, b, d, t = sc.constraint_arrays()
+        cons = abi.ConstraintView(len(e), p(e) if len(e) else None, len(b), p(b) if len(b) else None,
+                                  len(d), p(d) if len(d) else None, len(t), p(t) if len(t) else None)
+        self._check(self.lib.cn_scene_upload(self._ctx, C.byref(view), C.byref(tables), C.byref(cons)),
+                    "cn_scene_upload")
 
     def step(self, params: abi.StepParams):
         self._check(self.lib.cn_step(self._ctx, C.byref(params)), "cn_step")

@@ -93,6 +93,95 @@ def large_pbf(n: int = 4_194_304, seed: int = 2, box=(4.0, 2.0, 2.0)) -> Scene:
     return _fluid_scene(p, r, cols, f"large_pbf_{n}", dict(seed=seed, box=tuple(box), radius=r))
 
 
+SOFT_CONTACT = (0.0, 0.0, 0.0, -1.0)
+
+
+def _solid_part(pos_xyz, radius, inv_mass, part_mat=0, env=0):
+    n = len(pos_xyz)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = pos_xyz
+    pos[:, 3] = np.float32(inv_mass)
+    return dict(pos=pos, radii=np.full(n, radius, np.float32), kinds=np.zeros(n, np.uint32),
+                fluid_mat=np.full(n, 0xFFFFFFFF, np.uint32), part_mat=np.full(n, part_mat, np.uint32),
+                env=np.full(n, env, np.uint32))
+
+
+def compose(parts, fluid_mats, colliders, name, info=None) -> Scene:
+    """Concatenate pool parts (soft bodies, then cloths, then fluids -- the reference pool order of
+    src/physics/physics_world.cpp:4930-5387) with their constraints shifted to pool indices."""
+    from . import authoring as au
+    n = sum(len(p["pos"]) for p in parts)
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("pos", "radii", "kinds", "fluid_mat", "part_mat", "env")}
+    cons = {"edges": [], "bends": [], "dihedrals": [], "tets": []}
+    off = 0
+    for p in parts:
+        for k in cons:
+            if k in p and p[k] is not None and len(p[k]):
+                a = p[k].copy()
+                for f in {"edges": ("a", "b"), "bends": ("p0", "p1", "p2"),
+                          "dihedrals": ("e0", "e1", "o0", "o1"), "tets": ("p",)}[k]:
+                    a[f] += off
+                cons[k].append(a)
+        off += len(p["pos"])
+    dts = {"edges": au.EDGE_DTYPE, "bends": au.BEND_DTYPE, "dihedrals": au.DIHEDRAL_DTYPE, "tets": au.TET_DTYPE}
+    c = {k: (np.concatenate(v) if v else np.zeros(0, dts[k])) for k, v in cons.items()}
+    return Scene(pos=cat["pos"], vel=np.zeros((n, 4), np.float32), radii=cat["radii"], kinds=cat["kinds"],
+                 fluid_mat=cat["fluid_mat"], part_mat=cat["part_mat"], env=cat["env"],
+                 phase=np.zeros(n, np.uint32), layer=np.ones(n, np.uint32), mask=np.full(n, 0xFFFFFFFF, np.uint32),
+                 fluid_mats=fluid_mats, part_mats=np.array([SOFT_CONTACT], np.float32), colliders=colliders,
+                 name=name, info=info or {}, edges=c["edges"], bends=c["bends"], dihedrals=c["dihedrals"],
+                 tets=c["tets"])
+
+
+def soft_block_part(res=8, spacing=0.05, radius=None, translation=(1.0, 0.6, 1.0), mass=0.1, compliance=1e-6):
+    from . import authoring as au
+    size = (res - 1) * spacing
+    pos, tets = au.tet_grid((size, size, size), resolution=(res, res, res), translation=translation)
+    edges, tc = au.tet_constraints(pos, tets, 0, compliance, compliance)
+    part = _solid_part(pos, radius if radius is not None else 0.5 * spacing, 1.0 / mass)
+    part.update(edges=edges, tets=tc)
+    return part
+
+
+def cloth_part(n=32, spacing=0.02, radius=None, translation=(0.5, 1.5, 0.5), mass=0.01, compliance=1e-6):
+    from . import authoring as au
+    pos, tris = au.cloth_grid(n, n, spacing, translation)
+    edges, dih = au.cloth_constraints(pos, tris, 0, compliance, compliance)
+    part = _solid_part(pos, radius if radius is not None else 0.5 * spacing, 1.0 / mass)
+    part.update(edges=edges, dihedrals=dih)
+    return part
+
+
+def fluid_part(side=16, spacing=0.05, radius=0.03, origin=(0.0, 0.0, 0.0), seed=0, jitter=0.0, fluid_mat=0):
+    g = (np.arange(side, dtype=np.float32) + np.float32(0.5)) * np.float32(spacing)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32) + np.asarray(origin, np.float32)
+    if jitter:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        p += rng.uniform(-jitter, jitter, size=p.shape).astype(np.float32)
+    n = len(p)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = p
+    pos[:, 3] = np.float32(1.0 / (1000.0 * spacing ** 3))
+    return dict(pos=pos, radii=np.full(n, radius, np.float32), kinds=np.ones(n, np.uint32),
+                fluid_mat=np.full(n, fluid_mat, np.uint32), part_mat=np.zeros(n, np.uint32),
+                env=np.zeros(n, np.uint32))
+
+
+def mixed(fluid_side=64, tet_res=32, cloth_n=256, seed=1) -> Scene:
+    """Mixed single scene: a fluid block (fluid_side^3 particles, r = 0.01 m), a tet soft block from a
+    tet_res^3 vertex grid (5 tets per cube) and a cloth_n^2 cloth, compliance 1e-6, in a 4 x 2 x 2 m
+    container (pool order: soft body, cloth, fluid)."""
+    r = 0.01
+    soft = soft_block_part(tet_res, 0.02, r, translation=(3.2, 0.4, 1.0))
+    cloth = cloth_part(cloth_n, 1.9 / max(cloth_n - 1, 1), r * 0.5, translation=(0.05, 1.6, 0.05))
+    fluid = fluid_part(fluid_side, 0.02, r, origin=(0.05, 0.02, 0.05), seed=seed, jitter=0.002)
+    fm = fluid_material(r)
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    return compose([soft, cloth, fluid], fm, cols, f"mixed_{fluid_side}_{tet_res}_{cloth_n}",
+                   dict(fluid=fluid_side ** 3, soft=tet_res ** 3, cloth=cloth_n ** 2))
+
+
 def scaled_box_for(n: int, n_ref: int = 4_194_304, box_ref=(4.0, 2.0, 2.0)):
     """Box with the same aspect and number density as box_ref holds for n_ref particles."""
     s = (n / n_ref) ** (1.0 / 3.0)

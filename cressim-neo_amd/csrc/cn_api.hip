@@ -27,9 +27,12 @@ struct cn_ctx {
     uint32_t capacity = 0;       // particle capacity of the allocation
     uint32_t cap_alloc = 0;      // neighbor slots per particle of the allocation
     bool uploaded = false;
+    bool stepped = false;        // a cn_step ran since the upload (slot order differs from index order)
     float cell_size = 0.0f;      // derived on upload like the reference world
     uint32_t morton_bits = 10;
     uint32_t bucket_bits_alloc = 4;  // bucket index bits the allocation holds
+    float max_h = 0.0f;              // largest fluid smoothing radius of the scene
+    bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
     // profiling
     bool profiling = false;
     struct Ev {
@@ -98,6 +101,11 @@ void free_scene(SceneBuffers &s)
     dfree(s.run_count);
     dfree(s.perm);
     dfree(s.nc);
+    dfree(s.curpos);
+    dfree(s.cs);
+    dfree(s.blk_first_run);
+    dfree(s.nblk);
+    dfree(s.iter_state);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
     dfree(s.out_pos);
@@ -108,7 +116,66 @@ void free_scene(SceneBuffers &s)
     dfree(s.part_mats);
     dfree(s.colliders);
     dfree(s.meta);
+    dfree(s.slot_of);
+    dfree(s.edges);
+    dfree(s.bends);
+    dfree(s.dihedrals);
+    dfree(s.tets);
+    dfree(s.lam);
+    dfree(s.corr);
+    dfree(s.inc);
     s = SceneBuffers{};
+}
+
+void free_constraints(SceneBuffers &s)
+{
+    dfree(s.edges);
+    dfree(s.bends);
+    dfree(s.dihedrals);
+    dfree(s.tets);
+    dfree(s.lam);
+    dfree(s.corr);
+    dfree(s.inc);
+    s.edges = nullptr;
+    s.bends = nullptr;
+    s.dihedrals = nullptr;
+    s.tets = nullptr;
+    s.lam = nullptr;
+    s.corr = nullptr;
+    s.inc = nullptr;
+    s.ne = s.nb = s.nd = s.nt = 0;
+}
+
+// Particle -> incident constraints, ascending constraint order (physics_scene_gpu_state.cpp:3273-3360);
+// appends start[n + 1] then ref[] to out and returns the two offsets.
+template <class Fn>
+void build_incidence(uint32_t n, uint32_t groups, const uint32_t *counts, const uint32_t *arity, const uint32_t *offset,
+                     Fn parts, std::vector<uint32_t> &out, size_t &start_off, size_t &ref_off)
+{
+    std::vector<uint32_t> cnt(n + 1, 0);
+    for (uint32_t g = 0; g < groups; ++g)
+        for (uint32_t c = 0; c < counts[g]; ++c)
+            for (uint32_t k = 0; k < arity[g]; ++k) {
+                const uint32_t p = parts(g, c, k);
+                if (p < n) ++cnt[p];
+            }
+    start_off = out.size();
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        out.push_back(acc);
+        acc += cnt[i];
+    }
+    out.push_back(acc);
+    ref_off = out.size();
+    out.resize(out.size() + acc);
+    std::vector<uint32_t> cur(n);
+    for (uint32_t i = 0; i < n; ++i) cur[i] = out[start_off + i];
+    for (uint32_t g = 0; g < groups; ++g)
+        for (uint32_t c = 0; c < counts[g]; ++c)
+            for (uint32_t k = 0; k < arity[g]; ++k) {
+                const uint32_t p = parts(g, c, k);
+                if (p < n) out[ref_off + cur[p]++] = ((c + offset[g]) << 2) | k;
+            }
 }
 
 // host restatement of the static-collider visit order: enabled colliders, Morton codes of their
@@ -229,6 +296,13 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     const uint32_t key_bits = p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
     c.bucket_shift = key_bits - ctx->s.bucket_bits;
     c.uniform_class = ctx->s.nclasses == 1 ? 1u : 0u;
+    c.list_cap = ctx->cap_alloc;
+    // superset lists: radius h + skin; they stay exact while every particle has moved less than
+    // skin / 2 (minus a rounding margin) since the walk
+    c.skin = 0.1f * ctx->max_h;
+    const float budget = 0.5f * c.skin - 1.0e-5f * ctx->max_h;
+    c.fallback_d2 = budget > 0.0f ? budget * budget : 0.0f;
+    c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
     return c;
 }
@@ -258,6 +332,7 @@ uint2 *broadphase(cn_ctx *ctx, const SubstepConsts &c, uint32_t key_bits)
     {
         ProfScope ps(ctx, "reorder");
         launch_reorder(s, c, ctx->stream);
+        if (c.key_mode == CN_KEY_MORTON) launch_blocks(s, c, ctx->stream);
     }
     return sorted;
 }
@@ -435,6 +510,12 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.run_count, 1));
     CN_HIP(ctx, dalloc(&s.perm, N));
     CN_HIP(ctx, dalloc(&s.nc, N));
+    CN_HIP(ctx, dalloc(&s.curpos, N));
+    CN_HIP(ctx, dalloc(&s.cs, N));
+    CN_HIP(ctx, dalloc(&s.blk_first_run, N + 1));
+    CN_HIP(ctx, dalloc(&s.nblk, 1));
+    CN_HIP(ctx, dalloc(&s.iter_state, 3 * kIterStateWords));
+    CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, 3 * kIterStateWords * sizeof(uint32_t), ctx->stream));
     CN_HIP(ctx, dalloc(&s.nbr, N * (size_t)(cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.nbr_cnt, N));
     CN_HIP(ctx, dalloc(&s.out_pos, N));
@@ -458,8 +539,30 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     if (n == 0) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "empty particle pool");
     if (!pv->positions_inv_mass || !pv->velocities || !pv->radii || !pv->kinds || !pv->particle_material_indices)
         return fail(ctx, CN_ERR_INVALID_ARGUMENT, "missing particle arrays");
-    if (cv && (cv->edge_count || cv->bend_count || cv->dihedral_count || cv->tet_count))
-        return fail(ctx, CN_ERR_UNSUPPORTED, "soft constraints are not supported by this build");
+    if (cv) {
+        if ((cv->edge_count && !cv->edges) || (cv->bend_count && !cv->bends) ||
+            (cv->dihedral_count && !cv->dihedrals) || (cv->tet_count && !cv->tets))
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "constraint count without array");
+        for (uint32_t e = 0; e < cv->edge_count; ++e)
+            if (cv->edges[e].a >= n || cv->edges[e].b >= n)
+                return fail(ctx, CN_ERR_INVALID_ARGUMENT, "edge particle index out of range");
+        for (uint32_t b = 0; b < cv->bend_count; ++b)
+            if (cv->bends[b].p0 >= n || cv->bends[b].p1 >= n || cv->bends[b].p2 >= n)
+                return fail(ctx, CN_ERR_INVALID_ARGUMENT, "bend particle index out of range");
+        for (uint32_t d = 0; d < cv->dihedral_count; ++d) {
+            const cn_cloth_dihedral &D = cv->dihedrals[d];
+            if (D.e0 >= n || D.e1 >= n || D.o0 >= n || D.o1 >= n)
+                return fail(ctx, CN_ERR_INVALID_ARGUMENT, "dihedral particle index out of range");
+        }
+        for (uint32_t t = 0; t < cv->tet_count; ++t)
+            for (int k = 0; k < 4; ++k)
+                if (cv->tets[t].p[k] >= n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "tet particle index out of range");
+        const uint64_t refs = 2ull * cv->edge_count + 3ull * cv->bend_count + 4ull * cv->dihedral_count +
+                              4ull * cv->tet_count;
+        if (cv->edge_count >= (1u << 30) || cv->bend_count + (uint64_t)cv->dihedral_count >= (1u << 30) ||
+            cv->tet_count >= (1u << 30) || refs >= (1ull << 32))
+            return fail(ctx, CN_ERR_UNSUPPORTED, "too many constraints");
+    }
     if (tb->collider_count > kMaxColliders)
         return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
     if (n > ctx->capacity || !ctx->s.pos[0]) {
@@ -513,6 +616,8 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     }
     std::vector<uint32_t> ids(n);
     for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+    ctx->has_fluid = false;
+    for (uint32_t i = 0; i < n && !ctx->has_fluid; ++i) ctx->has_fluid = pv->kinds[i] == CN_PARTICLE_KIND_FLUID;
 
     dfree(s.classes);
     dfree(s.fluid_mats);
@@ -546,10 +651,66 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     CN_HIP(ctx, hipMemcpy(s.out_vel, pv->velocities, n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.out_prev, pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
 
+    // soft constraints, their incidence lists and the particle -> slot map
+    free_constraints(s);
+    if (cv && (cv->edge_count || cv->bend_count || cv->dihedral_count || cv->tet_count)) {
+        s.ne = cv->edge_count;
+        s.nb = cv->bend_count;
+        s.nd = cv->dihedral_count;
+        s.nt = cv->tet_count;
+        CN_HIP(ctx, dalloc(&s.edges, s.ne));
+        CN_HIP(ctx, dalloc(&s.bends, s.nb));
+        CN_HIP(ctx, dalloc(&s.dihedrals, s.nd));
+        CN_HIP(ctx, dalloc(&s.tets, s.nt));
+        if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, cv->edges, s.ne * sizeof(cn_soft_edge), hipMemcpyHostToDevice));
+        if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, cv->bends, s.nb * sizeof(cn_soft_bend), hipMemcpyHostToDevice));
+        if (s.nd)
+            CN_HIP(ctx, hipMemcpy(s.dihedrals, cv->dihedrals, s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
+        if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, cv->tets, s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
+        const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
+        const size_t ncorr = 2ull * s.ne + 4ull * (s.nb + s.nd) + 4ull * s.nt;
+        CN_HIP(ctx, dalloc(&s.lam, nlam));
+        CN_HIP(ctx, dalloc(&s.corr, ncorr));
+        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
+        std::vector<uint32_t> inc;
+        {
+            const uint32_t cnt[1] = {s.ne}, ar[1] = {2}, off[1] = {0};
+            build_incidence(n, 1, cnt, ar, off,
+                            [&](uint32_t, uint32_t c, uint32_t k) { return k == 0 ? cv->edges[c].a : cv->edges[c].b; },
+                            inc, s.inc_off[0], s.inc_off[1]);
+        }
+        {
+            const uint32_t cnt[2] = {s.nb, s.nd}, ar[2] = {3, 4}, off[2] = {0, s.nb};
+            build_incidence(n, 2, cnt, ar, off,
+                            [&](uint32_t g, uint32_t c, uint32_t k) {
+                                if (g == 0) {
+                                    const cn_soft_bend &B = cv->bends[c];
+                                    return k == 0 ? B.p0 : (k == 1 ? B.p1 : B.p2);
+                                }
+                                const cn_cloth_dihedral &D = cv->dihedrals[c];
+                                return k == 0 ? D.e0 : (k == 1 ? D.e1 : (k == 2 ? D.o0 : D.o1));
+                            },
+                            inc, s.inc_off[2], s.inc_off[3]);
+        }
+        {
+            const uint32_t cnt[1] = {s.nt}, ar[1] = {4}, off[1] = {0};
+            build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return cv->tets[c].p[k]; },
+                            inc, s.inc_off[4], s.inc_off[5]);
+        }
+        CN_HIP(ctx, dalloc(&s.inc, inc.size()));
+        CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        dfree(s.slot_of);
+        s.slot_of = nullptr;
+        CN_HIP(ctx, dalloc(&s.slot_of, std::max<uint32_t>(ctx->capacity, n)));
+        CN_HIP(ctx, hipMemcpy(s.slot_of, ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+
     // grid cell size (recomputeParticleGridCellSize) and Morton resolution from the scene extent
     std::vector<cn_fluid_material> fmh(tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
     ctx->cell_size = cn_particle_grid_cell_size(n, pv->radii, pv->fluid_material_indices, tb->fluid_material_count,
                                                 fmh.data());
+    ctx->max_h = 1.0e-4f;
+    for (const cn_fluid_material &m : fmh) ctx->max_h = std::max(ctx->max_h, m.smoothing_radius);
     float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
     for (uint32_t i = 0; i < n; ++i)
         for (int a = 0; a < 3; ++a) {
@@ -569,6 +730,7 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     // bucket bits cannot exceed the key bits of either key mode; the allocation fits the largest
     s.bucket_bits = std::min<uint32_t>(std::min<uint32_t>(bb, kb), ctx->bucket_bits_alloc);
     ctx->uploaded = true;
+    ctx->stepped = false;
     return CN_OK;
 }
 
@@ -589,19 +751,65 @@ cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
     const SubstepConsts c = make_consts(ctx, p, dt);
     const uint32_t key_bits = key_bits_of(ctx, p);
     CN_HIP(ctx, hipMemsetAsync(s.meta, 0, 8 * sizeof(uint32_t), ctx->stream));
+    const uint32_t soft_iters = resolve_iters(p->soft_internal_iterations, def);
+    const uint32_t contact_iters = resolve_iters(p->soft_contact_iterations, def);
+    const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
+    const bool has_soft = s.ne + s.nb + s.nd + s.nt > 0;
+    SoftArgs sa{};
+    if (has_soft) {
+        sa.slot_of = s.slot_of;
+        sa.n = s.n;
+        sa.dt = dt;
+        sa.relax = p->solver_config[kCfgSoftRelax];
+        sa.edges = s.edges;
+        sa.ne = s.ne;
+        sa.bends = s.bends;
+        sa.nb = s.nb;
+        sa.dih = s.dihedrals;
+        sa.nd = s.nd;
+        sa.tets = s.tets;
+        sa.nt = s.nt;
+        sa.lam_e = s.lam;
+        sa.lam_b = s.lam + s.ne;
+        sa.lam_t = s.lam + s.ne + s.nb + s.nd;
+        sa.corr_e = s.corr;
+        sa.corr_b = s.corr + 2ull * s.ne;
+        sa.corr_t = s.corr + 2ull * s.ne + 4ull * (s.nb + s.nd);
+        sa.inc_e_start = s.inc + s.inc_off[0];
+        sa.inc_e_ref = s.inc + s.inc_off[1];
+        sa.inc_b_start = s.inc + s.inc_off[2];
+        sa.inc_b_ref = s.inc + s.inc_off[3];
+        sa.inc_t_start = s.inc + s.inc_off[4];
+        sa.inc_t_ref = s.inc + s.inc_off[5];
+    }
     for (uint32_t sub = 0; sub < substeps; ++sub) {
         broadphase(ctx, c, key_bits);
+        if (has_soft) {
+            sa.pos = s.pos[s.cur];
+            sa.id = s.id[s.cur];
+            launch_soft_clear(sa, ctx->stream);
+        }
         int din = 0;
-        for (uint32_t it = 0; it < fluid_iters; ++it) {
-            {
-                ProfScope ps(ctx, "fluid_density");
-                launch_fluid_density(s, c, it, din, ctx->stream);
+        for (uint32_t it = 0; it < max_iters; ++it) {
+            if (ctx->has_fluid && it < fluid_iters) {
+                {
+                    ProfScope ps(ctx, it == 0 ? "fluid_walk" : "fluid_walk_check");
+                    launch_fluid_walk(s, c, it, din, ctx->stream);
+                }
+                if (it > 0) {
+                    ProfScope ps(ctx, "fluid_filter");
+                    launch_fluid_filter(s, c, it, din, ctx->stream);
+                }
+                {
+                    ProfScope ps(ctx, "fluid_delta");
+                    launch_fluid_delta(s, c, it, din, ctx->stream);
+                }
+                din = 1 - din;
             }
-            {
-                ProfScope ps(ctx, "fluid_delta");
-                launch_fluid_delta(s, c, it, din, ctx->stream);
+            if (has_soft && it < soft_iters) {
+                ProfScope ps(ctx, "soft");
+                launch_soft_iteration(sa, ctx->stream);
             }
-            din = 1 - din;
         }
         {
             ProfScope ps(ctx, "finalize");
@@ -610,8 +818,9 @@ cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
     }
     {
         ProfScope ps(ctx, "export");
-        launch_export(s, ctx->stream);
+        launch_export(s, true, false, ctx->stream);
     }
+    ctx->stepped = true;
     CN_HIP(ctx, hipGetLastError());
     return CN_OK;
 }
@@ -622,6 +831,7 @@ cn_status cn_readback(cn_ctx *ctx, float *pos, float *prev, float *vel, uint32_t
     if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
     if (count > ctx->s.n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "count exceeds particle count");
     CN_HIP(ctx, hipSetDevice(ctx->device));
+    if (prev && ctx->stepped) launch_export(ctx->s, false, true, ctx->stream);
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (pos) CN_HIP(ctx, hipMemcpy(pos, ctx->s.out_pos, count * sizeof(float4), hipMemcpyDeviceToHost));
     if (prev) CN_HIP(ctx, hipMemcpy(prev, ctx->s.out_prev, count * sizeof(float4), hipMemcpyDeviceToHost));
@@ -662,6 +872,10 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     std::memcpy(&tot, &m[2], sizeof(tot));
     r.fluid_neighbor_total = tot;
     r.last_particle_count = ctx->s.n;
+    r.verlet_rebuilds = m[4];
+    r.lds_tiles = m[5];
+    r.run_table_tiles = m[6];
+    r.max_displacement2_bits = m[7];
     if (out) *out = r;
     if (r.fluid_neighbor_truncations)
         return fail(ctx, CN_ERR_CAPACITY, "fluid neighbor list truncated at max_fluid_neighbors");
@@ -746,7 +960,10 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
     SceneBuffers &s = ctx->s;
     const uint32_t n = s.n;
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
-    const SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
+    SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
+    // list radius h and list slots = cap: the walk stores exactly the reference's neighbor lists
+    c.skin = 0.0f;
+    c.list_cap = c.cap;
     const int old_cur = s.cur;
     uint2 *sorted = broadphase(ctx, c, key_bits_of(ctx, p));
     launch_fluid_density(s, c, 0, 0, ctx->stream);
@@ -765,9 +982,10 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
         if (cap) CN_HIP(ctx, hipMemcpy(nb.data(), s.nbr, (size_t)n * cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (uint32_t t = 0; t < n; ++t) {
             const uint32_t d = id[t];
-            if (counts) counts[d] = cnt[t];
+            const uint32_t ct = cnt[t] & 0x7fffffffu;
+            if (counts) counts[d] = ct;
             if (lists)
-                for (uint32_t k = 0; k < cnt[t] && k < cap; ++k) lists[(size_t)d * cap + k] = id[nb[(size_t)k * n + t]];
+                for (uint32_t k = 0; k < ct && k < cap; ++k) lists[(size_t)d * cap + k] = id[nb[(size_t)k * n + t]];
         }
     }
     s.cur = old_cur;

@@ -67,6 +67,10 @@ struct SubstepConsts {
     uint32_t morton_bits; // bits per axis of the Morton key
     uint32_t bucket_shift;
     uint32_t uniform_class;  // 1: every particle has class 0
+    uint32_t list_cap;       // neighbor slots per particle of the allocation (superset lists)
+    float skin;              // superset radius = h + skin
+    float fallback_d2;       // superset lists stay valid while max |delta|^2 < fallback_d2
+    float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];
 };
 
@@ -180,6 +184,99 @@ __device__ __forceinline__ bool ghost_delta(const Collider &c, float px, float p
     const float t = __fmul_rn(2.0f, sd);
     gx = __fmul_rn(nx, t); gy = __fmul_rn(ny, t); gz = __fmul_rn(nz, t);
     return true;
+}
+
+// Single-precision atan / atan2 / asin / acos after the Cephes Math Library (S. L. Moshier): the
+// same range reductions and polynomial coefficients as its atanf.c / asinf.c, one rounding per
+// operation.  Used by the bend (acos) and dihedral (atan2) constraints so the kernels and the
+// CPU oracle agree bit for bit.
+constexpr float kCephesPi = 3.141592653589793238f;
+constexpr float kCephesPio2 = 1.5707963267948966192f;
+constexpr float kCephesPio4 = 0.7853981633974483096f;
+
+__device__ __forceinline__ float cephes_atanf(float xx)
+{
+    float x = xx, y;
+    const bool neg = x < 0.0f;
+    if (neg) x = -x;
+    if (x > 2.414213562373095f) {
+        y = kCephesPio2;
+        x = -(1.0f / x);
+    } else if (x > 0.4142135623730950f) {
+        y = kCephesPio4;
+        x = (x - 1.0f) / (x + 1.0f);
+    } else {
+        y = 0.0f;
+    }
+    const float z = x * x;
+    const float p = (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) *
+                        z * x + x;
+    y = y + p;
+    return neg ? -y : y;
+}
+
+__device__ __forceinline__ float cephes_atan2f(float y, float x)
+{
+    int code = x < 0.0f ? 2 : 0;
+    if (y < 0.0f) code |= 1;
+    if (x == 0.0f) {
+        if (code & 1) return -kCephesPio2;
+        if (y == 0.0f) return 0.0f;
+        return kCephesPio2;
+    }
+    if (y == 0.0f) return (code & 2) ? kCephesPi : 0.0f;
+    const float w = code == 2 ? kCephesPi : (code == 3 ? -kCephesPi : 0.0f);
+    return w + cephes_atanf(y / x);
+}
+
+__device__ __forceinline__ float cephes_asinf(float xx)
+{
+    const float a = fabsf(xx);
+    if (a > 1.0f) return __builtin_nanf("");
+    if (a < 1.0e-4f) return xx;
+    float x, z;
+    bool flag = false;
+    if (a > 0.5f) {
+        z = 0.5f * (1.0f - a);
+        x = fsqrt(z);
+        flag = true;
+    } else {
+        x = a;
+        z = x * x;
+    }
+    z = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+         1.6666752422e-1f) * z * x + x;
+    if (flag) {
+        z = z + z;
+        z = kCephesPio2 - z;
+    }
+    return xx < 0.0f ? -z : z;
+}
+
+__device__ __forceinline__ float cephes_acosf(float x)
+{
+    if (x < -1.0f || x > 1.0f) return __builtin_nanf("");
+    if (x < -0.5f) return kCephesPi - 2.0f * cephes_asinf(fsqrt(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * cephes_asinf(fsqrt(0.5f * (1.0f - x)));
+    return kCephesPio2 - cephes_asinf(x);
+}
+
+// Exclusive scan over a kBlock-thread workgroup (sh holds kBlock words); total receives the sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t &total)
+{
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kBlock; off <<= 1) {
+        const uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    total = sh[kBlock - 1];
+    const uint32_t ex = sh[t] - v;
+    __syncthreads();
+    return ex;
 }
 
 // XCD-aware tile index: consecutive logical tiles land on the same XCD (and its L2), since the

@@ -8,6 +8,8 @@
 
 namespace cn {
 
+constexpr uint32_t kIterStateWords = 64;  // iterations tracked per substep (more fall back to walks)
+
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
 uint2 *radix_sort_pairs(uint2 *buf_a, uint2 *buf_b, uint32_t n, uint32_t key_bits, uint32_t *scratch,
@@ -39,7 +41,12 @@ struct SceneBuffers {
     uint32_t *perm = nullptr;
     // fluid solve
     float4 *dacc[2] = {nullptr, nullptr};
-    float4 *nc = nullptr;           // surface normal * tension, density constraint
+    float4 *curpos = nullptr;       // (x* + delta, inverse mass): neighbor positions of the walk
+    float4 *cs = nullptr;           // (x* + delta, density constraint): neighbor records of the delta pass
+    float4 *nc = nullptr;           // surface normal * tension (only with surface tension)
+    uint32_t *blk_first_run = nullptr;  // first run of each Morton 4x4x4 block, then the run count
+    uint32_t *nblk = nullptr;
+    uint32_t *iter_state = nullptr; // [work counters | max |delta|^2 bits | fallback flags] per iteration
     uint32_t *nbr = nullptr;        // neighbor slots, k-major: nbr[k * n + i]
     uint32_t *nbr_cnt = nullptr;
     uint32_t cap = 0;
@@ -54,18 +61,61 @@ struct SceneBuffers {
     float4 *part_mats = nullptr;
     Collider *colliders = nullptr;
     uint32_t ncolliders = 0;
-    // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total
+    // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
+    // [4] iterations that had to walk the cells again
     uint32_t *meta = nullptr;
+    // soft constraints
+    uint32_t *slot_of = nullptr;
+    cn_soft_edge *edges = nullptr;
+    cn_soft_bend *bends = nullptr;
+    cn_cloth_dihedral *dihedrals = nullptr;
+    cn_soft_tet *tets = nullptr;
+    uint32_t ne = 0, nb = 0, nd = 0, nt = 0;
+    float *lam = nullptr;           // edges | bends + dihedrals | tets
+    float4 *corr = nullptr;         // edges (2 each) | bends + dihedrals (4 each) | tets (4 each)
+    uint32_t *inc = nullptr;        // three CSR incidence structures back to back
+    size_t inc_off[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // ---- broadphase + fluid launches (cn_kernels.hip) ----
 void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, uint32_t key_bits, hipStream_t st);
 void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
+void launch_fluid_walk(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
+void launch_fluid_filter(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
-void launch_export(const SceneBuffers &s, hipStream_t st);
+void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
+
+// ---- XPBD soft constraints (cn_soft.hip) ----
+struct SoftArgs {
+    float4 *pos;               // slot order
+    const uint32_t *slot_of;   // particle index -> slot
+    const uint32_t *id;        // slot -> particle index
+    uint32_t n;
+    float dt, relax;
+    const cn_soft_edge *edges;
+    uint32_t ne;
+    float *lam_e;
+    float4 *corr_e;            // 2 per edge
+    const cn_soft_bend *bends;
+    uint32_t nb;
+    const cn_cloth_dihedral *dih;
+    uint32_t nd;
+    float *lam_b;              // bends then dihedrals
+    float4 *corr_b;            // 4 per bend / dihedral
+    const cn_soft_tet *tets;
+    uint32_t nt;
+    float *lam_t;
+    float4 *corr_t;            // 4 per tet
+    const uint32_t *inc_e_start, *inc_e_ref;  // incidence: ref = (constraint << 2) | slot
+    const uint32_t *inc_b_start, *inc_b_ref;
+    const uint32_t *inc_t_start, *inc_t_ref;
+};
+void launch_soft_clear(const SoftArgs &a, hipStream_t st);
+void launch_soft_iteration(const SoftArgs &a, hipStream_t st);
 
 // per-pass entries in the reference's data order (cn_passes.hip)
 void launch_ref_keys(const float4 *pos, uint32_t n, float inv_cell, uint32_t *keys, int32_t *cells,

@@ -79,23 +79,6 @@ __device__ __forceinline__ bool run_flag(const uint2 *kv, uint32_t i, uint32_t n
     return i < n && (i == 0 || kv[i].x != kv[i - 1].x);
 }
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t &total)
-{
-    const uint32_t t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < kBlock; off <<= 1) {
-        const uint32_t add = t >= off ? sh[t - off] : 0u;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
-    }
-    total = sh[kBlock - 1];
-    const uint32_t ex = sh[t] - v;
-    __syncthreads();
-    return ex;
-}
-
 __global__ __launch_bounds__(kBlock) void k_run_count(const uint2 *__restrict__ kv, uint32_t n,
                                                       uint32_t *__restrict__ block_runs)
 {
@@ -256,9 +239,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t *__restrict__ perm, const float4 *__restrict__ pos_o, const float4 *__restrict__ vel_o,
     const uint32_t *__restrict__ cls_o, const uint32_t *__restrict__ id_o, float4 *__restrict__ pos_n,
     float4 *__restrict__ vel_n, uint32_t *__restrict__ cls_n, uint32_t *__restrict__ id_n,
-    float4 *__restrict__ prev, int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
-    const ParticleClass *__restrict__ classes, const cn_fluid_material *__restrict__ fm,
-    const Collider *__restrict__ col, SubstepConsts c)
+    float4 *__restrict__ prev, float4 *__restrict__ cur, int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
+    uint32_t *__restrict__ slot_of, const ParticleClass *__restrict__ classes,
+    const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, SubstepConsts c)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= c.n) return;
@@ -268,10 +251,13 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const ParticleClass pc = classes[c.uniform_class ? 0u : k];
     predict(p, v, pv, gravity_scale(pc, fm), c);
     pos_n[t] = p;
+    cur[t] = p;  // solver position x* + delta with delta = 0 at the first iteration
     vel_n[t] = v;
     prev[t] = pv;
     cls_n[t] = k;
-    id_n[t] = id_o[o];
+    const uint32_t pid = id_o[o];
+    id_n[t] = pid;
+    if (slot_of) slot_of[min(pid, c.n - 1u)] = t;
     const int3 g = cell_of(p.x, p.y, p.z, c.inv_cell);
     cellc[t] = make_int4(g.x, g.y, g.z, 0);
     bnd[t] = boundary_mask(pc, fm, col, c.ncolliders, p.x, p.y, p.z);
@@ -302,342 +288,77 @@ void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
     const int o = s.cur, nw = 1 - s.cur;
     hipLaunchKernelGGL(k_reorder, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.perm, s.pos[o], s.vel[o], s.cls[o],
-                       s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev, s.cellc, s.bnd, s.classes,
-                       s.fluid_mats, s.colliders, c);
+                       s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev, s.curpos, s.cellc, s.bnd,
+                       s.ne + s.nb + s.nd + s.nt ? s.slot_of : nullptr, s.classes, s.fluid_mats, s.colliders,
+                       c);
     hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
                        s.run_info);
     s.cur = nw;
 }
 
 // ---------------------------------------------------------------------------------------------
-// fluid: neighbor list + density constraint (one kernel), delta + apply + clamp (one kernel)
+// Morton blocks: runs grouped by key >> 6 (aligned 4x4x4 cells), the tiles of the LDS walk
 // ---------------------------------------------------------------------------------------------
-struct FluidArgs {
-    const float4 *pos;
-    const uint32_t *cls;
-    const int4 *cellc;
-    const uint32_t *bnd;
-    const float4 *din;   // accumulated delta of the previous iteration
-    float4 *dout;
-    float4 *nc;
-    uint32_t *nbr;
-    uint32_t *nbr_cnt;
-    const uint32_t *run_key;
-    const uint32_t *run_start;
-    const int4 *run_info;
-    const uint32_t *bucket;
-    uint32_t bucket_shift;
-    const ParticleClass *classes;
-    const cn_fluid_material *fm;
-    const Collider *col;
-    uint32_t *meta;
-    uint32_t it;
-};
-
-__device__ __forceinline__ int find_run(uint32_t key, const FluidArgs &a)
+__device__ __forceinline__ bool block_flag(const uint32_t *run_key, uint32_t r, uint32_t R)
 {
-    const uint32_t b = key >> a.bucket_shift;
-    uint32_t r = a.bucket[b];
-    const uint32_t e = a.bucket[b + 1];
-    while (r < e && a.run_key[r] < key) ++r;
-    return (r < e && a.run_key[r] == key) ? (int)r : -1;
+    return r < R && (r == 0 || (run_key[r] >> 6) != (run_key[r - 1] >> 6));
 }
 
-__device__ __forceinline__ bool neighbor_ok(const ParticleClass &self, const ParticleClass &o)
+__global__ __launch_bounds__(kBlock) void k_blk_count(const uint32_t *__restrict__ run_key,
+                                                      const uint32_t *__restrict__ run_count,
+                                                      uint32_t *__restrict__ block_cnt)
 {
-    // ShouldProcessFluidNeighbor, physics_fluid_neighbor_build.hlsli:48-70 (index check done by caller)
-    return o.kind == CN_PARTICLE_KIND_FLUID && o.env == self.env && (self.mask & o.layer) != 0u &&
-           (o.mask & self.layer) != 0u;
+    __shared__ uint32_t sh[kBlock];
+    const uint32_t R = *run_count;
+    const uint32_t base = blockIdx.x * kRunTile + threadIdx.x * kRunItems;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) cnt += block_flag(run_key, base + k, R) ? 1u : 0u;
+    uint32_t total;
+    block_exclusive_scan(cnt, sh, total);
+    if (threadIdx.x == 0) block_cnt[blockIdx.x] = total;
 }
 
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+__global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict__ run_key,
+                                                     const uint32_t *__restrict__ run_count,
+                                                     const uint32_t *__restrict__ block_off,
+                                                     uint32_t *__restrict__ blk_first_run,
+                                                     uint32_t *__restrict__ nblk, uint32_t *__restrict__ work,
+                                                     uint32_t nwork)
 {
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t o = __shfl_xor(v, off);
-        v = o > v ? o : v;
+    __shared__ uint32_t sh[kBlock];
+    const uint32_t R = *run_count;
+    const uint32_t base = blockIdx.x * kRunTile + threadIdx.x * kRunItems;
+    bool f[kRunItems];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) {
+        f[k] = block_flag(run_key, base + k, R);
+        cnt += f[k] ? 1u : 0u;
     }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-// physics_fluid_build_neighbors.cs.hlsl:21-147 fused with
-// physics_fluid_compute_density_lambda.cs.hlsl:26-115: the density sum runs over the listed
-// neighbors in list order, which is the order they are found in.
-template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_density(FluidArgs a, SubstepConsts c)
-{
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    uint32_t required = 0, trunc = 0, listed = 0;
-    if (i < c.n) {
-        const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-        const float4 p = a.pos[i];
-        if (pc.kind != CN_PARTICLE_KIND_FLUID || p.w <= kEps) {
-            a.nc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            a.nbr_cnt[i] = 0;
-        } else {
-            const bool use_d = a.it > 0;
-            const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const float sx = p.x + di.x, sy = p.y + di.y, sz = p.z + di.z;
-            const cn_fluid_material m = a.fm[pc.fluid_mat];
-            const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
-            const float rho0 = fmaxd(m.rest_density, 1.0f);
-            const float st = fmaxd(m.surface_tension, 0.0f);
-            const bool tension = st > kEps;
-            const bool self_ok = !UNIFORM || neighbor_ok(pc, pc);
-            const int4 ci = a.cellc[i];
-            float density = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
-            uint32_t cnt = 0;
-            if (c.cap > 0 && self_ok) {
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int tx = ci.x + dx, ty = ci.y + dy, tz = ci.z + dz;
-                            // a cell whose hash is the table's empty marker is never found
-                            // (physics_fluid_build_neighbors.cs.hlsl:82-85)
-                            if (cell_hash(tx, ty, tz) == kInvalid) continue;
-                            const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
-                            if (r < 0) continue;
-                            const int4 info = a.run_info[r];
-                            const bool pure = info.w != 0;
-                            if (pure && (info.x != tx || info.y != ty || info.z != tz)) continue;
-                            const uint32_t qe = a.run_start[r + 1];
-                            for (uint32_t q = a.run_start[r]; q < qe; ++q) {
-                                if (!pure) {
-                                    const int4 cq = a.cellc[q];
-                                    if (cq.x != tx || cq.y != ty || cq.z != tz) continue;
-                                }
-                                if (q == i) continue;
-                                if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) continue;
-                                const float4 pq = a.pos[q];
-                                if (pq.w <= kEps) continue;
-                                const float4 dq = use_d ? a.din[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                                const float ddx = sx - (pq.x + dq.x);
-                                const float ddy = sy - (pq.y + dq.y);
-                                const float ddz = sz - (pq.z + dq.z);
-                                const float dist = len3(ddx, ddy, ddz);
-                                if (dist >= h) continue;
-                                ++required;
-                                if (cnt >= c.cap) continue;
-                                a.nbr[(size_t)cnt * c.n + i] = q;
-                                ++cnt;
-                                density = density + spiky_w(dist, h);
-                                if (tension) {
-                                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-                                    if (!(dist <= kEps || dist >= h)) {
-                                        const float dw = spiky_dw(dist, h);
-                                        gx = dw * fdiv(ddx, dist);
-                                        gy = dw * fdiv(ddy, dist);
-                                        gz = dw * fdiv(ddz, dist);
-                                    }
-                                    snx = snx + gx;
-                                    sny = sny + gy;
-                                    snz = snz + gz;
-                                }
-                            }
-                        }
-            }
-            // AccumulateFluidBoundaryDensityFromCandidates, physics_fluid_boundary_common.hlsli:112-138
-            const uint32_t bm = a.bnd[i];
-            const float bscale = c.cfg[kCfgFluidBndDensity];
-            for (uint32_t k = 0; k < c.ncolliders; ++k) {
-                if (!((bm >> k) & 1u)) continue;
-                float gx, gy, gz;
-                if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
-                const float gd = len3(gx, gy, gz);
-                if (gd >= h) continue;
-                density = density + spiky_w(gd, h) * bscale;
-            }
-            const float C = fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) *
-                            m.density_constraint_scale;
-            a.nc[i] = tension ? make_float4(snx * st, sny * st, snz * st, C) : make_float4(0.0f, 0.0f, 0.0f, C);
-            a.nbr_cnt[i] = cnt;
-            trunc = required > c.cap ? 1u : 0u;
-            listed = cnt;
+    uint32_t total;
+    uint32_t idx = block_off[blockIdx.x] + block_exclusive_scan(cnt, sh, total);
+#pragma unroll
+    for (int k = 0; k < kRunItems; ++k) {
+        const uint32_t r = base + k;
+        if (r >= R) break;
+        if (f[k]) blk_first_run[idx++] = r;
+        if (r == R - 1) {
+            blk_first_run[idx] = R;
+            *nblk = idx;
         }
     }
-    const uint32_t wmax = wave_max(required);
-    const uint32_t wtr = wave_sum(trunc);
-    const uint32_t wls = wave_sum(listed);
-    if ((threadIdx.x & 63u) == 0) {
-        if (wmax) atomicMax(&a.meta[0], wmax);
-        if (wtr) atomicAdd(&a.meta[1], wtr);
-        if (a.it == 0 && wls) atomicAdd((unsigned long long *)&a.meta[2], (unsigned long long)wls);
-    }
+    if (blockIdx.x == 0)
+        for (uint32_t w = threadIdx.x; w < nwork; w += kBlock) work[w] = 0;
 }
 
-// physics_fluid_compute_delta_positions.cs.hlsl:27-157, then
-// physics_fluid_apply_delta_positions.cs.hlsl:11-31 and physics_fluid_clamp_boundary.cs.hlsl:23-80
-// on the same particle (both are per-particle and read only the particle's own delta).
-template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_delta(FluidArgs a, SubstepConsts c)
+void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (i >= c.n) return;
-    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-    if (pc.kind != CN_PARTICLE_KIND_FLUID) {
-        a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
-    const float4 p = a.pos[i];
-    const bool use_d = a.it > 0;
-    const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
-    if (p.w > kEps) {
-        const float sx = p.x + di.x, sy = p.y + di.y, sz = p.z + di.z;
-        const float4 nci = a.nc[i];
-        const float Ci = nci.w;
-        const cn_fluid_material m = a.fm[pc.fluid_mat];
-        const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
-        const float cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
-        const float viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
-        const float st = fmaxd(m.surface_tension, 0.0f);
-        const float cfl = fmaxd(m.cfl_radius, 0.0f);
-        const bool tension = st > kEps;
-        const float snx = tension ? nci.x : 0.0f, sny = tension ? nci.y : 0.0f, snz = tension ? nci.z : 0.0f;
-        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f, wsum = 0.0f;
-        const uint32_t cnt = a.nbr_cnt[i];
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t q = a.nbr[(size_t)k * c.n + i];
-            const float4 pq = a.pos[q];
-            if (pq.w <= kEps) continue;
-            const float4 dq = use_d ? a.din[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const float ddx = sx - (pq.x + dq.x);
-            const float ddy = sy - (pq.y + dq.y);
-            const float ddz = sz - (pq.z + dq.z);
-            const float dist = len3(ddx, ddy, ddz);
-            if (dist >= h || dist <= kEps) continue;
-            const float4 ncq = a.nc[q];
-            const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
-            const float der = spiky_dw(dist, h);
-            const float sc = -0.5f * (Ci + ncq.w) * der;
-            float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
-            const bool same = UNIFORM || pc.fluid_mat == a.classes[a.cls[q]].fluid_mat;
-            if (same) {
-                const float rx = di.x - dq.x, ry = di.y - dq.y, rz = di.z - dq.z;
-                if (cohesion > kEps) {
-                    const float ck = cohesion * cohesion_kernel(dist, h, m.cohesion1, m.cohesion2);
-                    pdx = pdx - nx * ck;
-                    pdy = pdy - ny * ck;
-                    pdz = pdz - nz * ck;
-                }
-                if (viscosity > kEps) {
-                    const float vs = 1.0f - fdiv(1.0f, 1.0f + viscosity * spiky_w(dist, h));
-                    pdx = pdx - vs * rx;
-                    pdy = pdy - vs * ry;
-                    pdz = pdz - vs * rz;
-                }
-                if (cfl > kEps) {
-                    const float proj = dot3(rx, ry, rz, nx, ny, nz);
-                    if (proj < -cfl) {
-                        const float pcf = proj + cfl;
-                        pdx = pdx - nx * pcf * 0.5f;
-                        pdy = pdy - ny * pcf * 0.5f;
-                        pdz = pdz - nz * pcf * 0.5f;
-                    }
-                }
-                if (tension) {
-                    pdx = pdx - c.dt * (snx - ncq.x);
-                    pdy = pdy - c.dt * (sny - ncq.y);
-                    pdz = pdz - c.dt * (snz - ncq.z);
-                }
-            }
-            dpx = dpx + pdx;
-            dpy = dpy + pdy;
-            dpz = dpz + pdz;
-            wsum = wsum + 1.0f;
-        }
-        // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
-        float bx = 0.0f, by = 0.0f, bz = 0.0f;
-        const float scn = fmaxd(Ci, 0.0f);
-        const uint32_t bm = a.bnd[i];
-        for (uint32_t k = 0; k < c.ncolliders; ++k) {
-            if (!((bm >> k) & 1u)) continue;
-            float gx, gy, gz;
-            if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
-            const float gd = len3(gx, gy, gz);
-            if (gd <= kEps || gd >= h) continue;
-            const float f = -scn * spiky_dw(gd, h);
-            bx = bx + f * fdiv(gx, gd);
-            by = by + f * fdiv(gy, gd);
-            bz = bz + f * fdiv(gz, gd);
-        }
-        const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
-        ox = dpx * solve + bx * bdel;
-        oy = dpy * solve + by * bdel;
-        oz = dpz * solve + bz * bdel;
-        ow = wsum;
-    }
-    // apply
-    const float w = fmaxd(ow * c.cfg[kCfgFluidRelax], 1.0f);
-    const float ax = di.x + fdiv(ox, w), ay = di.y + fdiv(oy, w), az = di.z + fdiv(oz, w);
-    // clamp
-    const float r = fmaxd(pc.radius, 0.0f);
-    float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
-    const uint32_t bm = a.bnd[i];
-    for (int sweep = 0; sweep < 2; ++sweep) {
-        bool changed = false;
-        for (uint32_t k = 0; k < c.ncolliders; ++k) {
-            if (!((bm >> k) & 1u)) continue;
-            float sd, nx, ny, nz;
-            if (!box_contact(a.col[k], cx, cy, cz, sd, nx, ny, nz)) continue;
-            if (sd >= r) continue;
-            const float push = r - sd;
-            cx = cx + nx * push;
-            cy = cy + ny * push;
-            cz = cz + nz * push;
-            changed = true;
-        }
-        if (!changed) break;
-    }
-    a.dout[i] = make_float4(cx - p.x, cy - p.y, cz - p.z, 0.0f);
-}
-
-static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din)
-{
-    FluidArgs a;
-    const int cur = s.cur;
-    a.pos = s.pos[cur];
-    a.cls = s.cls[cur];
-    a.cellc = s.cellc;
-    a.bnd = s.bnd;
-    a.din = s.dacc[din];
-    a.dout = s.dacc[1 - din];
-    a.nc = s.nc;
-    a.nbr = s.nbr;
-    a.nbr_cnt = s.nbr_cnt;
-    a.run_key = s.run_key;
-    a.run_start = s.run_start;
-    a.run_info = s.run_info;
-    a.bucket = s.bucket;
-    a.bucket_shift = c.bucket_shift;
-    a.classes = s.classes;
-    a.fm = s.fluid_mats;
-    a.col = s.colliders;
-    a.meta = s.meta;
-    a.it = it;
-    return a;
-}
-
-void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
-{
-    const FluidArgs a = fluid_args(s, c, it, din);
-    if (c.uniform_class)
-        hipLaunchKernelGGL(k_fluid_density<true>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
-    else
-        hipLaunchKernelGGL(k_fluid_density<false>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
-}
-
-void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
-{
-    const FluidArgs a = fluid_args(s, c, it, din);
-    if (c.uniform_class)
-        hipLaunchKernelGGL(k_fluid_delta<true>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
-    else
-        hipLaunchKernelGGL(k_fluid_delta<false>, dim3(grid1(c.n)), dim3(kBlock), 0, st, a, c);
+    const uint32_t nb = (c.n + kRunTile - 1) / kRunTile;
+    hipLaunchKernelGGL(k_blk_count, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs);
+    hipLaunchKernelGGL(k_scan_block_runs, dim3(1), dim3(1024), 0, st, s.block_runs, nb, s.nblk);
+    hipLaunchKernelGGL(k_blk_emit, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs,
+                       s.blk_first_run, s.nblk, s.iter_state, kIterStateWords);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -763,15 +484,16 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ po
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     const uint32_t d = min(id[t], n - 1u);
-    opos[d] = pos[t];
-    ovel[d] = vel[t];
-    oprev[d] = prev[t];
+    if (opos) opos[d] = pos[t];
+    if (ovel) ovel[d] = vel[t];
+    if (oprev) oprev[d] = prev[t];
 }
 
-void launch_export(const SceneBuffers &s, hipStream_t st)
+void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st)
 {
     hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
-                       s.id[s.cur], s.n, s.out_pos, s.out_vel, s.out_prev);
+                       s.id[s.cur], s.n, state ? s.out_pos : nullptr, state ? s.out_vel : nullptr,
+                       prev ? s.out_prev : nullptr);
 }
 
 }  // namespace cn

@@ -1,0 +1,286 @@
+// cn_soft.hip -- XPBD soft-body and cloth constraints on gfx950.
+//
+// Reference passes (assets/shaders/physics/soft/solver/): physics_soft_solve_edge_constraints.cs.hlsl
+// :10-74, physics_soft_solve_bend_constraints.cs.hlsl:20-86,
+// physics_cloth_solve_dihedral_constraints.cs.hlsl:20-95, physics_soft_solve_tet_constraints.cs.hlsl
+// :15-87, physics_soft_apply_{edge,bend,tet}_corrections.cs.hlsl and
+// physics_clear_soft_constraint_state.cs.hlsl:14-54.  Each solve is Jacobi over its constraint type:
+// one thread per constraint reads the current positions, updates its lambda and writes one
+// correction record per vertex; the apply pass then sums, per particle, the records of its incident
+// constraints in ascending constraint order (the incidence lists of
+// physics_scene_gpu_state.cpp:3273-3360) and adds the sum to the position.  Constraints refer to
+// particle indices; positions live in slot order, reached through slot_of[particle index].
+#include <hip/hip_runtime.h>
+
+#include "cn_internal.h"
+
+namespace cn {
+
+static inline uint32_t grid_n(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+__device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.pos[a.slot_of[particle]]; }
+
+__device__ __forceinline__ void cross3d(float ax, float ay, float az, float bx, float by, float bz, float &ox, float &oy,
+                                        float &oz)
+{
+    ox = ay * bz - az * by;
+    oy = az * bx - ax * bz;
+    oz = ax * by - ay * bx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_soft_clear(SoftArgs a)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < a.ne) a.lam_e[i] = 0.0f;
+    if (i < a.nb + a.nd) a.lam_b[i] = 0.0f;
+    if (i < a.nt) a.lam_t[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
+{
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
+    const cn_soft_edge E = a.edges[e];
+    float4 *out = a.corr_e + 2 * (size_t)e;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (E.enabled == 0u) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
+    const float4 pa = ldp(a, E.a), pb = ldp(a, E.b);
+    const float wa = pa.w, wb = pb.w, wsum = wa + wb;
+    if (wsum <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
+    const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
+    const float l2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (l2 <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
+    const float len = fsqrt(l2);
+    const float gax = fdiv(-dx, len), gay = fdiv(-dy, len), gaz = fdiv(-dz, len);
+    const float gbx = -gax, gby = -gay, gbz = -gaz;
+    const float alpha = fdiv(fmaxd(E.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    const float C = len - E.rest_length;
+    const float lambda = a.lam_e[e];
+    const float den = wsum + alpha;
+    if (den <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
+    const float dl = fdiv(-(C + alpha * lambda), den);
+    a.lam_e[e] = lambda + dl;
+    const float fa = wa * dl, fb = wb * dl;
+    out[0] = make_float4(fa * gax * a.relax, fa * gay * a.relax, fa * gaz * a.relax, 0.0f);
+    out[1] = make_float4(fb * gbx * a.relax, fb * gby * a.relax, fb * gbz * a.relax, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
+{
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= a.nb) return;
+    const cn_soft_bend B = a.bends[b];
+    float4 *out = a.corr_b + 4 * (size_t)b;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 p0 = ldp(a, B.p0), p1 = ldp(a, B.p1), p2 = ldp(a, B.p2);
+    const float w0 = p0.w, w1 = p1.w, w2 = p2.w;
+    bool zero = w0 + w1 + w2 <= kEps;
+    float g0x = 0, g0y = 0, g0z = 0, g1x = 0, g1y = 0, g1z = 0, g2x = 0, g2y = 0, g2z = 0, dl = 0;
+    if (!zero) {
+        const float e0x = p0.x - p1.x, e0y = p0.y - p1.y, e0z = p0.z - p1.z;
+        const float e1x = p2.x - p1.x, e1y = p2.y - p1.y, e1z = p2.z - p1.z;
+        const float l0s = dot3(e0x, e0y, e0z, e0x, e0y, e0z), l1s = dot3(e1x, e1y, e1z, e1x, e1y, e1z);
+        zero = l0s <= kEps || l1s <= kEps;
+        if (!zero) {
+            const float l0 = fsqrt(l0s), l1 = fsqrt(l1s);
+            const float d0x = fdiv(e0x, l0), d0y = fdiv(e0y, l0), d0z = fdiv(e0z, l0);
+            const float d1x = fdiv(e1x, l1), d1y = fdiv(e1y, l1), d1z = fdiv(e1z, l1);
+            const float ct = fclamp(dot3(d0x, d0y, d0z, d1x, d1y, d1z), -1.0f + 1.0e-4f, 1.0f - 1.0e-4f);
+            const float st = fsqrt(fmaxd(1.0f - ct * ct, 1.0e-8f));
+            const float theta = cephes_acosf(ct);
+            const float q0 = fmaxd(l0 * st, kEps), q2 = fmaxd(l1 * st, kEps);
+            g0x = fdiv(-(d1x - ct * d0x), q0);
+            g0y = fdiv(-(d1y - ct * d0y), q0);
+            g0z = fdiv(-(d1z - ct * d0z), q0);
+            g2x = fdiv(-(d0x - ct * d1x), q2);
+            g2y = fdiv(-(d0y - ct * d1y), q2);
+            g2z = fdiv(-(d0z - ct * d1z), q2);
+            g1x = -(g0x + g2x);
+            g1y = -(g0y + g2y);
+            g1z = -(g0z + g2z);
+            const float den = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
+                              w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z);
+            const float alpha = fdiv(fmaxd(B.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+            zero = den + alpha <= kEps;
+            if (!zero) {
+                const float C = theta - B.rest_angle;
+                const float lambda = a.lam_b[b];
+                dl = fdiv(-(C + alpha * lambda), den + alpha);
+                a.lam_b[b] = lambda + dl;
+            }
+        }
+    }
+    if (zero) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
+    const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, r = a.relax;
+    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
+    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
+    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
+    out[3] = z4;
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
+{
+    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    if (d >= a.nd) return;
+    const uint32_t ci = a.nb + d;
+    const cn_cloth_dihedral D = a.dih[d];
+    float4 *out = a.corr_b + 4 * (size_t)ci;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 e0m = ldp(a, D.e0), e1m = ldp(a, D.e1), o0m = ldp(a, D.o0), o1m = ldp(a, D.o1);
+    // p0, p1 opposite vertices; p2, p3 edge vertices
+    const float ex = e1m.x - e0m.x, ey = e1m.y - e0m.y, ez = e1m.z - e0m.z;
+    const float el2 = dot3(ex, ey, ez, ex, ey, ez);
+    float n0x, n0y, n0z, n1x, n1y, n1z;
+    cross3d(e0m.x - o0m.x, e0m.y - o0m.y, e0m.z - o0m.z, e1m.x - o0m.x, e1m.y - o0m.y, e1m.z - o0m.z, n0x, n0y, n0z);
+    cross3d(e1m.x - o1m.x, e1m.y - o1m.y, e1m.z - o1m.z, e0m.x - o1m.x, e0m.y - o1m.y, e0m.z - o1m.z, n1x, n1y, n1z);
+    const float n0s = dot3(n0x, n0y, n0z, n0x, n0y, n0z), n1s = dot3(n1x, n1y, n1z, n1x, n1y, n1z);
+    if (el2 <= kEps || n0s <= kEps || n1s <= kEps) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
+    const float el = fsqrt(el2);
+    const float iel = fdiv(1.0f, el);
+    const float s0x = fdiv(n0x, n0s), s0y = fdiv(n0y, n0s), s0z = fdiv(n0z, n0s);
+    const float s1x = fdiv(n1x, n1s), s1y = fdiv(n1y, n1s), s1z = fdiv(n1z, n1s);
+    const float f00 = dot3(o0m.x - e1m.x, o0m.y - e1m.y, o0m.z - e1m.z, ex, ey, ez) * iel;
+    const float f01 = dot3(o1m.x - e1m.x, o1m.y - e1m.y, o1m.z - e1m.z, ex, ey, ez) * iel;
+    const float f10 = dot3(e0m.x - o0m.x, e0m.y - o0m.y, e0m.z - o0m.z, ex, ey, ez) * iel;
+    const float f11 = dot3(e0m.x - o1m.x, e0m.y - o1m.y, e0m.z - o1m.z, ex, ey, ez) * iel;
+    const float g0x = -(f00 * s0x + f01 * s1x), g0y = -(f00 * s0y + f01 * s1y), g0z = -(f00 * s0z + f01 * s1z);
+    const float g1x = -(f10 * s0x + f11 * s1x), g1y = -(f10 * s0y + f11 * s1y), g1z = -(f10 * s0z + f11 * s1z);
+    const float g2x = -(el * s0x), g2y = -(el * s0y), g2z = -(el * s0z);
+    const float g3x = -(el * s1x), g3y = -(el * s1y), g3z = -(el * s1z);
+    const float r0 = fsqrt(n0s), r1 = fsqrt(n1s);
+    const float u0x = fdiv(n0x, r0), u0y = fdiv(n0y, r0), u0z = fdiv(n0z, r0);
+    const float u1x = fdiv(n1x, r1), u1y = fdiv(n1y, r1), u1z = fdiv(n1z, r1);
+    float cx, cy, cz;
+    cross3d(u0x, u0y, u0z, u1x, u1y, u1z, cx, cy, cz);
+    const float theta = cephes_atan2f(dot3(ex * iel, ey * iel, ez * iel, cx, cy, cz),
+                                      fclamp(dot3(u0x, u0y, u0z, u1x, u1y, u1z), -1.0f, 1.0f));
+    const float two_pi = 6.283185307179586f, pi = 3.141592653589793f;
+    float C = theta - D.rest_angle;
+    C = C - two_pi * floorf(fdiv(C + pi, two_pi));
+    const float w0 = e0m.w, w1 = e1m.w, w2 = o0m.w, w3 = o1m.w;
+    const float den = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
+                      w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
+    const float alpha = fdiv(fmaxd(D.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    if (den + alpha <= kEps) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
+    const float lambda = a.lam_b[ci];
+    const float dl = fdiv(-(C + alpha * lambda), den + alpha);
+    a.lam_b[ci] = lambda + dl;
+    const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
+    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
+    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
+    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
+    out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.nt) return;
+    const cn_soft_tet T = a.tets[t];
+    const float4 p0 = ldp(a, T.p[0]), p1 = ldp(a, T.p[1]), p2 = ldp(a, T.p[2]), p3 = ldp(a, T.p[3]);
+    const float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
+    if ((w0 + w1 + w2 + w3) <= kEps) return;  // the record keeps its previous value
+    float cx, cy, cz;
+    cross3d(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, cx, cy, cz);
+    const float sv = fdiv(dot3(cx, cy, cz, p3.x - p0.x, p3.y - p0.y, p3.z - p0.z), 6.0f);
+    const float sgn = sv >= 0.0f ? 1.0f : -1.0f;
+    const float C = fabsf(sv) - fmaxd(T.rest_volume, 0.0f);
+    const float f = fdiv(sgn, 6.0f);
+    float g0x, g0y, g0z, g1x, g1y, g1z, g2x, g2y, g2z, g3x, g3y, g3z;
+    cross3d(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z, p2.x - p1.x, p2.y - p1.y, p2.z - p1.z, g0x, g0y, g0z);
+    cross3d(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, p3.x - p0.x, p3.y - p0.y, p3.z - p0.z, g1x, g1y, g1z);
+    cross3d(p3.x - p0.x, p3.y - p0.y, p3.z - p0.z, p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, g2x, g2y, g2z);
+    cross3d(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, g3x, g3y, g3z);
+    g0x = g0x * f; g0y = g0y * f; g0z = g0z * f;
+    g1x = g1x * f; g1y = g1y * f; g1z = g1z * f;
+    g2x = g2x * f; g2y = g2y * f; g2z = g2z * f;
+    g3x = g3x * f; g3y = g3y * f; g3z = g3z * f;
+    const float sg = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
+                     w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
+    const float alpha = fdiv(fmaxd(T.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    const float lambda = a.lam_t[t];
+    const float den = sg + alpha;
+    if (den <= kEps) return;
+    const float dl = fdiv(-(C + alpha * lambda), den);
+    a.lam_t[t] = lambda + dl;
+    const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
+    float4 *out = a.corr_t + 4 * (size_t)t;
+    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
+    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
+    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
+    out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
+}
+
+// Per slot: the particle's incident records in list order, summed, then added to the position.
+__global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
+                                                              const uint32_t *__restrict__ ref,
+                                                              const float4 *__restrict__ rec, uint32_t per)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    float4 p = a.pos[t];
+    if (p.w <= kEps) return;
+    const uint32_t pid = a.id[t];
+    const uint32_t k0 = start[pid], k1 = start[pid + 1];
+    if (k0 == k1) return;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t r = ref[k];
+        const float4 c = rec[(size_t)(r >> 2) * per + (r & 3u)];
+        tx = tx + c.x;
+        ty = ty + c.y;
+        tz = tz + c.z;
+    }
+    p.x = p.x + tx;
+    p.y = p.y + ty;
+    p.z = p.z + tz;
+    a.pos[t] = p;
+}
+
+void launch_soft_clear(const SoftArgs &a, hipStream_t st)
+{
+    const uint32_t m = std::max(std::max(a.ne, a.nb + a.nd), a.nt);
+    if (m) hipLaunchKernelGGL(k_soft_clear, dim3(grid_n(m)), dim3(kBlock), 0, st, a);
+}
+
+void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
+{
+    // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
+    if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start, a.inc_e_ref,
+                       (const float4 *)a.corr_e, 2u);
+    if (a.nb) hipLaunchKernelGGL(k_solve_bends, dim3(grid_n(a.nb)), dim3(kBlock), 0, st, a);
+    if (a.nd) hipLaunchKernelGGL(k_solve_dihedrals, dim3(grid_n(a.nd)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start, a.inc_b_ref,
+                       (const float4 *)a.corr_b, 4u);
+    if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start, a.inc_t_ref,
+                       (const float4 *)a.corr_t, 4u);
+}
+
+}  // namespace cn

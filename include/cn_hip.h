@@ -188,7 +188,10 @@ typedef struct cn_meta {
                                              substeps of the last cn_step (mean K = total /
                                              (particles * substeps)) */
     uint32_t last_particle_count;
-    uint32_t reserved[5];
+    uint32_t lds_tiles;                   /* neighbor-walk tiles staged in LDS */
+    uint32_t run_table_tiles;             /* neighbor-walk tiles walked through the run table */
+    uint32_t max_displacement2_bits;      /* float bits of the largest (|delta| + slack)^2 */
+    uint32_t reserved[2];
 } cn_meta;
 
 /* Raw device pointers of the resident scene (PhysicsGpuSceneView analog). */

@@ -1035,6 +1035,456 @@ void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt, c
 }
 
 /* ------------------------------------------------------------------------------------------
+ * XPBD soft constraints
+ * ------------------------------------------------------------------------------------------ */
+
+/* Cephes single-precision atanf / atan2f / asinf / acosf (S. L. Moshier, Cephes Math Library),
+ * written out with one rounding per operation. */
+#define CEPHES_PIF 3.141592653589793238f
+#define CEPHES_PIO2F 1.5707963267948966192f
+#define CEPHES_PIO4F 0.7853981633974483096f
+static float cephes_atanf(float xx)
+{
+    float x = xx, y, z;
+    int neg = 0;
+    if (x < 0.0f) {
+        neg = 1;
+        x = -x;
+    }
+    if (x > 2.414213562373095f) {
+        y = CEPHES_PIO2F;
+        x = -(1.0f / x);
+    } else if (x > 0.4142135623730950f) {
+        y = CEPHES_PIO4F;
+        x = (x - 1.0f) / (x + 1.0f);
+    } else {
+        y = 0.0f;
+    }
+    z = x * x;
+    const float p = (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z -
+                     3.33329491539e-1f) * z * x + x;
+    y = y + p;
+    return neg ? -y : y;
+}
+
+float cno_atan2f(float y, float x)
+{
+    int code = 0;
+    if (x < 0.0f)
+        code = 2;
+    if (y < 0.0f)
+        code |= 1;
+    if (x == 0.0f) {
+        if (code & 1)
+            return -CEPHES_PIO2F;
+        if (y == 0.0f)
+            return 0.0f;
+        return CEPHES_PIO2F;
+    }
+    if (y == 0.0f)
+        return (code & 2) ? CEPHES_PIF : 0.0f;
+    float w = 0.0f;
+    if (code == 2)
+        w = CEPHES_PIF;
+    else if (code == 3)
+        w = -CEPHES_PIF;
+    return w + cephes_atanf(y / x);
+}
+
+static float cephes_asinf(float xx)
+{
+    float a = fabsf(xx), x, z;
+    if (a > 1.0f)
+        return NAN;
+    if (a < 1.0e-4f)
+        return xx;
+    int flag = 0;
+    if (a > 0.5f) {
+        z = 0.5f * (1.0f - a);
+        x = sqrtf(z);
+        flag = 1;
+    } else {
+        x = a;
+        z = x * x;
+    }
+    z = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+         1.6666752422e-1f) * z * x + x;
+    if (flag) {
+        z = z + z;
+        z = CEPHES_PIO2F - z;
+    }
+    return xx < 0.0f ? -z : z;
+}
+
+float cno_acosf(float x)
+{
+    if (x < -1.0f || x > 1.0f)
+        return NAN;
+    if (x < -0.5f)
+        return CEPHES_PIF - 2.0f * cephes_asinf(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f)
+        return 2.0f * cephes_asinf(sqrtf(0.5f * (1.0f - x)));
+    return CEPHES_PIO2F - cephes_asinf(x);
+}
+
+static void cross3(const float *a, const float *b, float *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void sub3(const float *a, const float *b, float *o)
+{
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+
+/* Particle -> incident constraint lists (physics_scene_gpu_state.cpp:3273-3360): per particle the
+ * references in ascending constraint index; ref = (constraint << 2) | slot.  Bend corrections are
+ * indexed bends first, then dihedrals at softBendCount + dihedralIndex. */
+typedef struct csr {
+    uint32_t *start;
+    uint32_t *ref;
+} csr;
+
+static void csr_build(uint32_t n, uint32_t nc, uint32_t arity, const uint32_t *(*get)(const void *, uint32_t),
+                      const void *arr, uint32_t index_offset, csr *out, uint32_t *count)
+{
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t *p = get(arr, c);
+        for (uint32_t k = 0; k < arity; ++k)
+            if (p[k] < n)
+                ++count[p[k]];
+    }
+    (void)index_offset;
+    (void)out;
+}
+
+static const uint32_t *edge_parts(const void *a, uint32_t c) { return &((const cn_soft_edge *)a)[c].a; }
+static const uint32_t *bend_parts(const void *a, uint32_t c) { return &((const cn_soft_bend *)a)[c].p0; }
+static const uint32_t *dih_parts(const void *a, uint32_t c) { return &((const cn_cloth_dihedral *)a)[c].e0; }
+static const uint32_t *tet_parts(const void *a, uint32_t c) { return ((const cn_soft_tet *)a)[c].p; }
+
+static void csr_fill(uint32_t n, uint32_t nc, uint32_t arity, const uint32_t *(*get)(const void *, uint32_t),
+                     const void *arr, uint32_t index_offset, csr *out, uint32_t *cursor)
+{
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t *p = get(arr, c);
+        for (uint32_t k = 0; k < arity; ++k)
+            if (p[k] < n)
+                out->ref[cursor[p[k]]++] = ((c + index_offset) << 2) | k;
+    }
+}
+
+static void csr_make(uint32_t n, csr *out, uint32_t groups, const uint32_t *ncs, const uint32_t *arities,
+                     const uint32_t *(**gets)(const void *, uint32_t), const void **arrs, const uint32_t *offsets)
+{
+    uint32_t *count = (uint32_t *)calloc(n + 1, sizeof(uint32_t));
+    for (uint32_t g = 0; g < groups; ++g)
+        csr_build(n, ncs[g], arities[g], gets[g], arrs[g], offsets[g], out, count);
+    out->start = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        out->start[i] = acc;
+        acc += count[i];
+    }
+    out->start[n] = acc;
+    out->ref = (uint32_t *)malloc(sizeof(uint32_t) * (acc ? acc : 1));
+    for (uint32_t i = 0; i < n; ++i)
+        count[i] = out->start[i];
+    for (uint32_t g = 0; g < groups; ++g)
+        csr_fill(n, ncs[g], arities[g], gets[g], arrs[g], offsets[g], out, count);
+    free(count);
+}
+
+static void csr_free(csr *c)
+{
+    free(c->start);
+    free(c->ref);
+}
+
+static float pos_w(const cno_scene *s, uint32_t i) { return s->pos[4 * i + 3]; }
+
+/* physics_soft_solve_edge_constraints.cs.hlsl:10-74 */
+static void solve_edges(cno_scene *s, const float *cfg, float dt, float *lam)
+{
+    const float relax = cfg[CFG_SOFT_RELAX];
+#pragma omp parallel for schedule(static)
+    for (int64_t ee = 0; ee < (int64_t)s->n_edges; ++ee) {
+        const uint32_t e = (uint32_t)ee;
+        const cn_soft_edge *E = &s->edges[e];
+        float *out = s->edge_corr + 8 * e;
+        memset(out, 0, 8 * sizeof(float));
+        if (E->enabled == 0u)
+            continue;
+        const float *pa = s->pos + 4 * E->a, *pb = s->pos + 4 * E->b;
+        const float wa = pa[3], wb = pb[3], wsum = wa + wb;
+        if (wsum <= EPS)
+            continue;
+        float d[3];
+        sub3(pb, pa, d);
+        const float l2 = dot3(d, d);
+        if (l2 <= EPS)
+            continue;
+        const float len = sqrtf(l2);
+        float ga[3], gb[3];
+        for (int k = 0; k < 3; ++k) {
+            ga[k] = -d[k] / len;
+            gb[k] = -ga[k];
+        }
+        const float comp = maxf(E->compliance, 0.0f);
+        const float alpha = comp / maxf(dt * dt, EPS);
+        const float C = len - E->rest_length;
+        const float lambda = lam[e];
+        const float den = wsum + alpha;
+        if (den <= EPS)
+            continue;
+        const float dl = -(C + alpha * lambda) / den;
+        lam[e] = lambda + dl;
+        for (int k = 0; k < 3; ++k) {
+            out[k] = wa * dl * ga[k] * relax;
+            out[4 + k] = wb * dl * gb[k] * relax;
+        }
+    }
+}
+
+static void store_zero16(float *o) { memset(o, 0, 16 * sizeof(float)); }
+
+/* physics_soft_solve_bend_constraints.cs.hlsl:20-86 */
+static void solve_bends(cno_scene *s, const float *cfg, float dt, float *lam)
+{
+    const float relax = cfg[CFG_SOFT_RELAX];
+#pragma omp parallel for schedule(static)
+    for (int64_t bb = 0; bb < (int64_t)s->n_bends; ++bb) {
+        const uint32_t b = (uint32_t)bb;
+        const cn_soft_bend *B = &s->bends[b];
+        float *out = s->bend_corr + 16 * b;
+        const float *p0 = s->pos + 4 * B->p0, *p1 = s->pos + 4 * B->p1, *p2 = s->pos + 4 * B->p2;
+        const float w0 = p0[3], w1 = p1[3], w2 = p2[3];
+        if (w0 + w1 + w2 <= EPS) {
+            store_zero16(out);
+            continue;
+        }
+        float e0[3], e1[3];
+        sub3(p0, p1, e0);
+        sub3(p2, p1, e1);
+        const float l0s = dot3(e0, e0), l1s = dot3(e1, e1);
+        if (l0s <= EPS || l1s <= EPS) {
+            store_zero16(out);
+            continue;
+        }
+        const float l0 = sqrtf(l0s), l1 = sqrtf(l1s);
+        float d0[3], d1[3];
+        for (int k = 0; k < 3; ++k) {
+            d0[k] = e0[k] / l0;
+            d1[k] = e1[k] / l1;
+        }
+        const float ct = clampf(dot3(d0, d1), -1.0f + 1.0e-4f, 1.0f - 1.0e-4f);
+        const float st2 = maxf(1.0f - ct * ct, 1.0e-8f);
+        const float st = sqrtf(st2);
+        const float theta = cno_acosf(ct);
+        const float q0 = maxf(l0 * st, EPS), q2 = maxf(l1 * st, EPS);
+        float g0[3], g1[3], g2[3];
+        for (int k = 0; k < 3; ++k) {
+            g0[k] = -(d1[k] - ct * d0[k]) / q0;
+            g2[k] = -(d0[k] - ct * d1[k]) / q2;
+            g1[k] = -(g0[k] + g2[k]);
+        }
+        const float den = w0 * dot3(g0, g0) + w1 * dot3(g1, g1) + w2 * dot3(g2, g2);
+        const float alpha = maxf(B->compliance, 0.0f) / maxf(dt * dt, EPS);
+        if (den + alpha <= EPS) {
+            store_zero16(out);
+            continue;
+        }
+        const float C = theta - B->rest_angle;
+        const float lambda = lam[b];
+        const float dl = -(C + alpha * lambda) / (den + alpha);
+        lam[b] = lambda + dl;
+        for (int k = 0; k < 3; ++k) {
+            out[k] = w0 * dl * g0[k] * relax;
+            out[4 + k] = w1 * dl * g1[k] * relax;
+            out[8 + k] = w2 * dl * g2[k] * relax;
+            out[12 + k] = 0.0f;
+        }
+        out[3] = out[7] = out[11] = out[15] = 0.0f;
+    }
+}
+
+/* physics_cloth_solve_dihedral_constraints.cs.hlsl:20-95 */
+static void solve_dihedrals(cno_scene *s, const float *cfg, float dt, float *lam)
+{
+    const float relax = cfg[CFG_SOFT_RELAX];
+    const float two_pi = 6.283185307179586f, pi = 3.141592653589793f;
+#pragma omp parallel for schedule(static)
+    for (int64_t dd = 0; dd < (int64_t)s->n_dihedrals; ++dd) {
+        const uint32_t d = (uint32_t)dd;
+        const uint32_t ci = s->n_bends + d;
+        const cn_cloth_dihedral *D = &s->dihedrals[d];
+        float *out = s->bend_corr + 16 * ci;
+        const float *e0m = s->pos + 4 * D->e0, *e1m = s->pos + 4 * D->e1;
+        const float *o0m = s->pos + 4 * D->o0, *o1m = s->pos + 4 * D->o1;
+        const float *p0 = o0m, *p1 = o1m, *p2 = e0m, *p3 = e1m;
+        float edge[3], a[3], b[3], rn0[3], rn1[3];
+        sub3(p3, p2, edge);
+        const float el2 = dot3(edge, edge);
+        sub3(p2, p0, a);
+        sub3(p3, p0, b);
+        cross3(a, b, rn0);
+        sub3(p3, p1, a);
+        sub3(p2, p1, b);
+        cross3(a, b, rn1);
+        const float n0s = dot3(rn0, rn0), n1s = dot3(rn1, rn1);
+        if (el2 <= EPS || n0s <= EPS || n1s <= EPS) {
+            store_zero16(out);
+            continue;
+        }
+        const float el = sqrtf(el2);
+        const float iel = 1.0f / el;
+        float sn0[3], sn1[3], dop0[3], dop1[3], de0[3], de1[3];
+        for (int k = 0; k < 3; ++k) {
+            sn0[k] = rn0[k] / n0s;
+            sn1[k] = rn1[k] / n1s;
+            dop0[k] = el * sn0[k];
+            dop1[k] = el * sn1[k];
+        }
+        float t0[3], t1[3];
+        sub3(p0, p3, t0);
+        sub3(p1, p3, t1);
+        const float f00 = dot3(t0, edge) * iel, f01 = dot3(t1, edge) * iel;
+        sub3(p2, p0, t0);
+        sub3(p2, p1, t1);
+        const float f10 = dot3(t0, edge) * iel, f11 = dot3(t1, edge) * iel;
+        for (int k = 0; k < 3; ++k) {
+            de0[k] = f00 * sn0[k] + f01 * sn1[k];
+            de1[k] = f10 * sn0[k] + f11 * sn1[k];
+        }
+        float g0[3], g1[3], g2[3], g3[3];
+        for (int k = 0; k < 3; ++k) {
+            g0[k] = -de0[k];
+            g1[k] = -de1[k];
+            g2[k] = -dop0[k];
+            g3[k] = -dop1[k];
+        }
+        const float r0 = sqrtf(n0s), r1 = sqrtf(n1s);
+        float n0[3], n1[3], eu[3], cr[3];
+        for (int k = 0; k < 3; ++k) {
+            n0[k] = rn0[k] / r0;
+            n1[k] = rn1[k] / r1;
+            eu[k] = edge[k] * iel;
+        }
+        cross3(n0, n1, cr);
+        const float theta = cno_atan2f(dot3(eu, cr), clampf(dot3(n0, n1), -1.0f, 1.0f));
+        float C = theta - D->rest_angle;
+        C = C - two_pi * floorf((C + pi) / two_pi);
+        const float w0 = e0m[3], w1 = e1m[3], w2 = o0m[3], w3 = o1m[3];
+        const float den = w0 * dot3(g0, g0) + w1 * dot3(g1, g1) + w2 * dot3(g2, g2) + w3 * dot3(g3, g3);
+        const float alpha = maxf(D->compliance, 0.0f) / maxf(dt * dt, EPS);
+        if (den + alpha <= EPS) {
+            store_zero16(out);
+            continue;
+        }
+        const float lambda = lam[ci];
+        const float dl = -(C + alpha * lambda) / (den + alpha);
+        lam[ci] = lambda + dl;
+        for (int k = 0; k < 3; ++k) {
+            out[k] = w0 * dl * g0[k] * relax;
+            out[4 + k] = w1 * dl * g1[k] * relax;
+            out[8 + k] = w2 * dl * g2[k] * relax;
+            out[12 + k] = w3 * dl * g3[k] * relax;
+        }
+        out[3] = out[7] = out[11] = out[15] = 0.0f;
+    }
+}
+
+/* physics_soft_solve_tet_constraints.cs.hlsl:15-87; an early return leaves the record as it was */
+static void solve_tets(cno_scene *s, const float *cfg, float dt, float *lam)
+{
+    const float relax = cfg[CFG_SOFT_RELAX];
+#pragma omp parallel for schedule(static)
+    for (int64_t tt = 0; tt < (int64_t)s->n_tets; ++tt) {
+        const uint32_t t = (uint32_t)tt;
+        const cn_soft_tet *T = &s->tets[t];
+        const float *p0 = s->pos + 4 * T->p[0], *p1 = s->pos + 4 * T->p[1];
+        const float *p2 = s->pos + 4 * T->p[2], *p3 = s->pos + 4 * T->p[3];
+        const float w0 = p0[3], w1 = p1[3], w2 = p2[3], w3 = p3[3];
+        if ((w0 + w1 + w2 + w3) <= EPS)
+            continue;
+        float a[3], b[3], c[3], cr[3];
+        sub3(p1, p0, a);
+        sub3(p2, p0, b);
+        sub3(p3, p0, c);
+        cross3(a, b, cr);
+        const float sv = dot3(cr, c) / 6.0f;
+        const float sgn = sv >= 0.0f ? 1.0f : -1.0f;
+        const float vol = fabsf(sv);
+        const float rest = maxf(T->rest_volume, 0.0f);
+        const float C = vol - rest;
+        const float f = sgn / 6.0f;
+        float g0[3], g1[3], g2[3], g3[3], u[3], v[3];
+        sub3(p3, p1, u);
+        sub3(p2, p1, v);
+        cross3(u, v, g0);
+        sub3(p2, p0, u);
+        sub3(p3, p0, v);
+        cross3(u, v, g1);
+        sub3(p3, p0, u);
+        sub3(p1, p0, v);
+        cross3(u, v, g2);
+        sub3(p1, p0, u);
+        sub3(p2, p0, v);
+        cross3(u, v, g3);
+        for (int k = 0; k < 3; ++k) {
+            g0[k] = g0[k] * f;
+            g1[k] = g1[k] * f;
+            g2[k] = g2[k] * f;
+            g3[k] = g3[k] * f;
+        }
+        const float sg = w0 * dot3(g0, g0) + w1 * dot3(g1, g1) + w2 * dot3(g2, g2) + w3 * dot3(g3, g3);
+        const float comp = maxf(T->compliance, 0.0f);
+        const float alpha = comp / maxf(dt * dt, EPS);
+        const float lambda = lam[t];
+        const float den = sg + alpha;
+        if (den <= EPS)
+            continue;
+        const float dl = -(C + alpha * lambda) / den;
+        lam[t] = lambda + dl;
+        float *out = s->tet_corr + 16 * t;
+        for (int k = 0; k < 3; ++k) {
+            out[k] = w0 * dl * g0[k] * relax;
+            out[4 + k] = w1 * dl * g1[k] * relax;
+            out[8 + k] = w2 * dl * g2[k] * relax;
+            out[12 + k] = w3 * dl * g3[k] * relax;
+        }
+        out[3] = out[7] = out[11] = out[15] = 0.0f;
+    }
+}
+
+/* physics_soft_apply_{edge,bend,tet}_corrections.cs.hlsl: per particle, the incident records in
+ * list order summed onto the position (records of `stride` floats, slot k at 4 k). */
+static void apply_corrections(cno_scene *s, const csr *c, const float *rec, uint32_t stride)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *p = s->pos + 4 * i;
+        if (p[3] <= EPS)
+            continue;
+        float t[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = c->start[i]; k < c->start[i + 1]; ++k) {
+            const uint32_t ref = c->ref[k];
+            const float *r = rec + (size_t)(ref >> 2) * stride + 4 * (ref & 3u);
+            t[0] += r[0];
+            t[1] += r[1];
+            t[2] += r[2];
+        }
+        p[0] = p[0] + t[0];
+        p[1] = p[1] + t[1];
+        p[2] = p[2] + t[2];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
  * PhysicsSolver::step for the particle path, src/physics/physics_solver.cpp:336-1401
  * ------------------------------------------------------------------------------------------ */
 static uint32_t resolve_iters(uint32_t v, uint32_t d) { return v > 0 ? v : d; }
@@ -1087,6 +1537,27 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         return -1;
     cno_static_collider_order(s->n_colliders, s->colliders, order);
 
+    const int has_soft = n > 0 && (s->n_edges || s->n_bends || s->n_dihedrals || s->n_tets);
+    csr inc_e = {0, 0}, inc_b = {0, 0}, inc_t = {0, 0};
+    float *lam_e = NULL, *lam_b = NULL, *lam_t = NULL;
+    if (has_soft) {
+        const uint32_t *(*ge[1])(const void *, uint32_t) = {edge_parts};
+        const void *ae[1] = {s->edges};
+        const uint32_t ne[1] = {s->n_edges}, ar_e[1] = {2}, of_e[1] = {0};
+        csr_make(n, &inc_e, 1, ne, ar_e, ge, ae, of_e);
+        const uint32_t *(*gb[2])(const void *, uint32_t) = {bend_parts, dih_parts};
+        const void *ab[2] = {s->bends, s->dihedrals};
+        const uint32_t nb[2] = {s->n_bends, s->n_dihedrals}, ar_b[2] = {3, 4}, of_b[2] = {0, s->n_bends};
+        csr_make(n, &inc_b, 2, nb, ar_b, gb, ab, of_b);
+        const uint32_t *(*gt[1])(const void *, uint32_t) = {tet_parts};
+        const void *at[1] = {s->tets};
+        const uint32_t nt[1] = {s->n_tets}, ar_t[1] = {4}, of_t[1] = {0};
+        csr_make(n, &inc_t, 1, nt, ar_t, gt, at, of_t);
+        lam_e = (float *)calloc(s->n_edges + 1, sizeof(float));
+        lam_b = (float *)calloc(s->n_bends + s->n_dihedrals + 1, sizeof(float));
+        lam_t = (float *)calloc(s->n_tets + 1, sizeof(float));
+    }
+
     cno_stats st;
     memset(&st, 0, sizeof(st));
     st.fluid_count = fluid_count;
@@ -1103,6 +1574,12 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         }
         if (has_bnd)
             cno_fluid_boundary_candidates(s, order, br, bp, bcap);
+        if (has_soft) {
+            /* clearSoftConstraintState, physics_clear_soft_constraint_state.cs.hlsl:14-54 */
+            memset(lam_e, 0, sizeof(float) * (s->n_edges + 1));
+            memset(lam_b, 0, sizeof(float) * (s->n_bends + s->n_dihedrals + 1));
+            memset(lam_t, 0, sizeof(float) * (s->n_tets + 1));
+        }
         for (uint32_t it = 0; it < max_iters; ++it) {
             if (has_fluid && it < fluid_iters) {
                 cno_fluid_neighbors(s, cell, cells, keys, idx, tbl, capacity, dacc, it, cap,
@@ -1125,6 +1602,16 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
                 cno_fluid_delta(s, cfg, dt, dacc, it, cap, counts, list, br, bp, nc, dtmp);
                 cno_fluid_apply(s, cfg, it, dtmp, dacc);
                 cno_fluid_clamp(s, dacc, br, bp, dacc);
+            }
+            if (has_soft && it < soft_iters) {
+                /* physics_solver.cpp:915-1019: edges, then bends + dihedrals, then tets */
+                solve_edges(s, cfg, dt, lam_e);
+                apply_corrections(s, &inc_e, s->edge_corr, 8);
+                solve_bends(s, cfg, dt, lam_b);
+                solve_dihedrals(s, cfg, dt, lam_b);
+                apply_corrections(s, &inc_b, s->bend_corr, 16);
+                solve_tets(s, cfg, dt, lam_t);
+                apply_corrections(s, &inc_t, s->tet_corr, 16);
             }
         }
         cno_update_velocities(s, dt, dacc);
@@ -1166,5 +1653,13 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     free(br);
     free(bp);
     free(order);
+    if (has_soft) {
+        csr_free(&inc_e);
+        csr_free(&inc_b);
+        csr_free(&inc_t);
+        free(lam_e);
+        free(lam_b);
+        free(lam_t);
+    }
     return 0;
 }

@@ -46,6 +46,11 @@ typedef struct cno_scene {
     const cn_cloth_dihedral *dihedrals;
     uint32_t n_tets;
     const cn_soft_tet *tets;
+    /* per-constraint correction records; they persist across substeps and steps because the tet
+     * solve leaves a record untouched when it returns early (caller allocates, zero-initialised) */
+    float *edge_corr;           /* 8 floats per edge (two float4) */
+    float *bend_corr;           /* 16 floats per bend + dihedral (four float4) */
+    float *tet_corr;            /* 16 floats per tet */
 } cno_scene;
 
 typedef struct cno_stats {
@@ -104,6 +109,12 @@ void cno_update_velocities(cno_scene *s, float dt, float *delta_acc);
 void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt,
                                      const uint32_t *bnd_ranges, const uint32_t *bnd_pairs);
 void cno_predict(cno_scene *s, float dt, const float *gravity);
+
+/* ---- XPBD soft constraints ---- */
+/* Cephes single-precision atan2f / acosf (S. L. Moshier), evaluated without contraction; the GPU
+ * kernels evaluate the same sequence. */
+float cno_atan2f(float y, float x);
+float cno_acosf(float x);
 
 /* ---- full step: PhysicsSolver::step for the particle path ---- */
 int cno_step(cno_scene *s, const cn_step_params *params, cno_stats *stats);

@@ -25,7 +25,8 @@ class CnoScene(C.Structure):
                 ("n_edges", C.c_uint32), ("edges", C.c_void_p),
                 ("n_bends", C.c_uint32), ("bends", C.c_void_p),
                 ("n_dihedrals", C.c_uint32), ("dihedrals", C.c_void_p),
-                ("n_tets", C.c_uint32), ("tets", C.c_void_p)]
+                ("n_tets", C.c_uint32), ("tets", C.c_void_p),
+                ("edge_corr", C.c_void_p), ("bend_corr", C.c_void_p), ("tet_corr", C.c_void_p)]
 
 
 class CnoStats(C.Structure):
@@ -91,13 +92,22 @@ class OracleScene:
         self.part_mats = np.ascontiguousarray(sc.part_mats, np.float32)
         self.colliders = np.ascontiguousarray(sc.colliders)
         n = self.pos.shape[0]
+        self.edges, self.bends, self.dihedrals, self.tets = sc.constraint_arrays()
+        self.edge_corr = np.zeros((max(len(self.edges), 1), 8), np.float32)
+        self.bend_corr = np.zeros((max(len(self.bends) + len(self.dihedrals), 1), 16), np.float32)
+        self.tet_corr = np.zeros((max(len(self.tets), 1), 16), np.float32)
+
+        def p(a):
+            return a.ctypes.data if len(a) else None
         self.c = CnoScene(n, self.pos.ctypes.data, self.prev.ctypes.data, self.vel.ctypes.data,
                           self.radii.ctypes.data, self.kinds.ctypes.data, self.fluid_mat.ctypes.data,
                           self.part_mat.ctypes.data, self.meta.ctypes.data,
                           len(self.fluid_mats), self.fluid_mats.ctypes.data,
                           len(self.part_mats), self.part_mats.ctypes.data,
-                          len(self.colliders), self.colliders.ctypes.data if len(self.colliders) else None,
-                          0, None, 0, None, 0, None, 0, None)
+                          len(self.colliders), p(self.colliders),
+                          len(self.edges), p(self.edges), len(self.bends), p(self.bends),
+                          len(self.dihedrals), p(self.dihedrals), len(self.tets), p(self.tets),
+                          self.edge_corr.ctypes.data, self.bend_corr.ctypes.data, self.tet_corr.ctypes.data)
 
     @property
     def n(self):
