@@ -97,7 +97,7 @@ def main():
     import struct
     maxd = struct.unpack("f", struct.pack("I", meta.max_displacement2_bits))[0] ** 0.5
     print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}, "
-          f"re-walks {meta.verlet_rebuilds}, LDS tiles {meta.lds_tiles}, run-table tiles {meta.run_table_tiles}, "
+          f"LDS blocks {meta.lds_tiles}, run-table blocks {meta.run_table_tiles}, "
           f"max displacement {maxd:.3e}", file=sys.stderr, flush=True)
 
     # per-kernel HIP-event timing on the context stream, in separate steps

@@ -33,6 +33,8 @@ struct cn_ctx {
     uint32_t bucket_bits_alloc = 4;  // bucket index bits the allocation holds
     float max_h = 0.0f;              // largest fluid smoothing radius of the scene
     bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
+    bool has_tension = false;        // any fluid material with surface tension
+    uint32_t tile_max = 2048;        // halo particles a fluid-pass workgroup may stage in LDS
     // profiling
     bool profiling = false;
     struct Ev {
@@ -105,7 +107,11 @@ void free_scene(SceneBuffers &s)
     dfree(s.cs);
     dfree(s.blk_first_run);
     dfree(s.nblk);
+    dfree(s.blk_halo);
     dfree(s.iter_state);
+    dfree(s.fluid_t2);
+    dfree(s.list);
+    dfree(s.lcnt);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
     dfree(s.out_pos);
@@ -252,6 +258,18 @@ std::vector<Collider> visit_ordered_colliders(const cn_box_collider *c, uint32_t
 
 uint32_t resolve_iters(uint32_t v, uint32_t d) { return v > 0 ? v : d; }
 
+// Smallest float T with sqrt(T) >= h under the correctly rounded IEEE square root (the host's
+// std::sqrt and the kernels' __builtin_sqrtf agree), so that for every float d2,
+// sqrt(d2) < h  <=>  d2 < T.  The rounded root is monotone, hence the set below h is an interval.
+float sqrt_threshold(float h)
+{
+    if (!(h == h)) return h;
+    float x = h * h;
+    while (x > 0.0f && std::sqrt(x) >= h) x = std::nextafter(x, 0.0f);
+    while (!(std::sqrt(x) >= h)) x = std::nextafter(x, HUGE_VALF);
+    return x;
+}
+
 struct ProfScope {
     cn_ctx *ctx;
     int slot;
@@ -296,12 +314,9 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     const uint32_t key_bits = p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
     c.bucket_shift = key_bits - ctx->s.bucket_bits;
     c.uniform_class = ctx->s.nclasses == 1 ? 1u : 0u;
-    c.list_cap = ctx->cap_alloc;
-    // superset lists: radius h + skin; they stay exact while every particle has moved less than
-    // skin / 2 (minus a rounding margin) since the walk
-    c.skin = 0.1f * ctx->max_h;
-    const float budget = 0.5f * c.skin - 1.0e-5f * ctx->max_h;
-    c.fallback_d2 = budget > 0.0f ? budget * budget : 0.0f;
+    c.tension = ctx->has_tension ? 1u : 0u;
+    c.tile_max = ctx->tile_max;
+    c.blk_cap = ctx->s.blk_cap;
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
     return c;
@@ -332,7 +347,11 @@ uint2 *broadphase(cn_ctx *ctx, const SubstepConsts &c, uint32_t key_bits)
     {
         ProfScope ps(ctx, "reorder");
         launch_reorder(s, c, ctx->stream);
-        if (c.key_mode == CN_KEY_MORTON) launch_blocks(s, c, ctx->stream);
+    }
+    if (c.key_mode == CN_KEY_MORTON) {
+        ProfScope ps(ctx, "blocks");
+        launch_blocks(s, c, ctx->stream);
+        if (ctx->has_fluid) launch_blk_halo(s, c, ctx->stream);
     }
     return sorted;
 }
@@ -514,10 +533,14 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.cs, N));
     CN_HIP(ctx, dalloc(&s.blk_first_run, N + 1));
     CN_HIP(ctx, dalloc(&s.nblk, 1));
-    CN_HIP(ctx, dalloc(&s.iter_state, 3 * kIterStateWords));
-    CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, 3 * kIterStateWords * sizeof(uint32_t), ctx->stream));
-    CN_HIP(ctx, dalloc(&s.nbr, N * (size_t)(cap ? cap : 1)));
-    CN_HIP(ctx, dalloc(&s.nbr_cnt, N));
+    // halo descriptors for up to one cell block per 32 particles; sparser scenes walk the rest of
+    // their blocks through the run table
+    s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
+    CN_HIP(ctx, dalloc(&s.blk_halo, (size_t)s.blk_cap * kHaloStride));
+    CN_HIP(ctx, dalloc(&s.list, N * (size_t)(cap ? cap : 1)));
+    CN_HIP(ctx, dalloc(&s.lcnt, N));
+    CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
+    CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, kStateWords * sizeof(uint32_t), ctx->stream));
     CN_HIP(ctx, dalloc(&s.out_pos, N));
     CN_HIP(ctx, dalloc(&s.out_vel, N));
     CN_HIP(ctx, dalloc(&s.out_prev, N));
@@ -710,7 +733,18 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     ctx->cell_size = cn_particle_grid_cell_size(n, pv->radii, pv->fluid_material_indices, tb->fluid_material_count,
                                                 fmh.data());
     ctx->max_h = 1.0e-4f;
-    for (const cn_fluid_material &m : fmh) ctx->max_h = std::max(ctx->max_h, m.smoothing_radius);
+    ctx->has_tension = false;
+    std::vector<float> t2(fmh.size());
+    for (size_t k = 0; k < fmh.size(); ++k) {
+        const cn_fluid_material &m = fmh[k];
+        ctx->max_h = std::max(ctx->max_h, m.smoothing_radius);
+        ctx->has_tension |= (m.surface_tension > 0.0f ? m.surface_tension : 0.0f) > kEps;
+        t2[k] = sqrt_threshold(m.smoothing_radius > 1.0e-4f ? m.smoothing_radius : 1.0e-4f);
+    }
+    dfree(s.fluid_t2);
+    s.fluid_t2 = nullptr;
+    CN_HIP(ctx, dalloc(&s.fluid_t2, t2.size()));
+    if (!t2.empty()) CN_HIP(ctx, hipMemcpy(s.fluid_t2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
     float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
     for (uint32_t i = 0; i < n; ++i)
         for (int a = 0; a < 3; ++a) {
@@ -793,12 +827,8 @@ cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
         for (uint32_t it = 0; it < max_iters; ++it) {
             if (ctx->has_fluid && it < fluid_iters) {
                 {
-                    ProfScope ps(ctx, it == 0 ? "fluid_walk" : "fluid_walk_check");
-                    launch_fluid_walk(s, c, it, din, ctx->stream);
-                }
-                if (it > 0) {
-                    ProfScope ps(ctx, "fluid_filter");
-                    launch_fluid_filter(s, c, it, din, ctx->stream);
+                    ProfScope ps(ctx, "fluid_density");
+                    launch_fluid_density(s, c, it, din, ctx->stream);
                 }
                 {
                     ProfScope ps(ctx, "fluid_delta");
@@ -872,7 +902,7 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     std::memcpy(&tot, &m[2], sizeof(tot));
     r.fluid_neighbor_total = tot;
     r.last_particle_count = ctx->s.n;
-    r.verlet_rebuilds = m[4];
+    r.reserved0 = 0;
     r.lds_tiles = m[5];
     r.run_table_tiles = m[6];
     r.max_displacement2_bits = m[7];
@@ -960,13 +990,21 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
     SceneBuffers &s = ctx->s;
     const uint32_t n = s.n;
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
-    SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
-    // list radius h and list slots = cap: the walk stores exactly the reference's neighbor lists
-    c.skin = 0.0f;
-    c.list_cap = c.cap;
+    const SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
+    const size_t slots = (size_t)n * std::max<uint32_t>(c.cap, 1u);
+    if (s.nbr_alloc < slots) {
+        dfree(s.nbr);
+        dfree(s.nbr_cnt);
+        s.nbr = nullptr;
+        s.nbr_cnt = nullptr;
+        s.nbr_alloc = 0;
+        CN_HIP(ctx, dalloc(&s.nbr, slots));
+        CN_HIP(ctx, dalloc(&s.nbr_cnt, std::max<size_t>(n, ctx->capacity)));
+        s.nbr_alloc = slots;
+    }
     const int old_cur = s.cur;
     uint2 *sorted = broadphase(ctx, c, key_bits_of(ctx, p));
-    launch_fluid_density(s, c, 0, 0, ctx->stream);
+    launch_fluid_lists(s, c, ctx->stream);
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<uint2> kv(n);
     std::vector<uint32_t> id(n), cnt(n);
@@ -982,7 +1020,7 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
         if (cap) CN_HIP(ctx, hipMemcpy(nb.data(), s.nbr, (size_t)n * cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (uint32_t t = 0; t < n; ++t) {
             const uint32_t d = id[t];
-            const uint32_t ct = cnt[t] & 0x7fffffffu;
+            const uint32_t ct = cnt[t];
             if (counts) counts[d] = ct;
             if (lists)
                 for (uint32_t k = 0; k < ct && k < cap; ++k) lists[(size_t)d * cap + k] = id[nb[(size_t)k * n + t]];

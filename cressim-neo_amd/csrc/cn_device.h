@@ -67,9 +67,9 @@ struct SubstepConsts {
     uint32_t morton_bits; // bits per axis of the Morton key
     uint32_t bucket_shift;
     uint32_t uniform_class;  // 1: every particle has class 0
-    uint32_t list_cap;       // neighbor slots per particle of the allocation (superset lists)
-    float skin;              // superset radius = h + skin
-    float fallback_d2;       // superset lists stay valid while max |delta|^2 < fallback_d2
+    uint32_t tension;        // 1: some fluid material has surface tension
+    uint32_t tile_max;       // halo particles a fluid-pass workgroup may stage in LDS
+    uint32_t blk_cap;        // cell blocks with a stored halo descriptor
     float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];
 };
@@ -119,22 +119,27 @@ __device__ __forceinline__ uint32_t cell_key(int gx, int gy, int gz, uint32_t mo
 
 // FluidSpikyKernel / FluidSpikyKernelDerivative / FluidCohesionKernel,
 // physics_fluid_common.hlsli:17-65
-__device__ __forceinline__ float spiky_w(float d, float h)
+// The kernels' normalisation 15 / (pi h^3) depends on h only: callers that evaluate many
+// neighbors of one particle compute it once (spiky_coef) and pass it to the _k forms.
+__device__ __forceinline__ float spiky_coef(float h)
+{
+    const float h3 = __fmul_rn(__fmul_rn(h, h), h);
+    return fdiv(15.0f, __fmul_rn(kKernelPi, h3));
+}
+__device__ __forceinline__ float spiky_w_k(float d, float h, float k)
 {
     if (d >= h) return 0.0f;
     const float omq = __fsub_rn(1.0f, fdiv(d, h));
-    const float h3 = __fmul_rn(__fmul_rn(h, h), h);
-    const float k = fdiv(15.0f, __fmul_rn(kKernelPi, h3));
     return __fmul_rn(__fmul_rn(k, omq), omq);
 }
-__device__ __forceinline__ float spiky_dw(float d, float h)
+__device__ __forceinline__ float spiky_dw_k(float d, float h, float k)
 {
     if (d <= kEps || d >= h) return 0.0f;
     const float omq = __fsub_rn(1.0f, fdiv(d, h));
-    const float h3 = __fmul_rn(__fmul_rn(h, h), h);
-    const float k = fdiv(15.0f, __fmul_rn(kKernelPi, h3));
     return fdiv(__fmul_rn(__fmul_rn(-2.0f, k), omq), h);
 }
+__device__ __forceinline__ float spiky_w(float d, float h) { return spiky_w_k(d, h, spiky_coef(h)); }
+__device__ __forceinline__ float spiky_dw(float d, float h) { return spiky_dw_k(d, h, spiky_coef(h)); }
 __device__ __forceinline__ float cohesion_kernel(float d, float h, float c1, float c2)
 {
     if (d <= kEps || d >= h) return 0.0f;

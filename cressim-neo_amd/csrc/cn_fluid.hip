@@ -1,21 +1,26 @@
-// cn_fluid.hip -- PBF fluid iterations on gfx950: neighbor walk + density constraint, neighbor
-// filter + density constraint, and delta + apply + clamp.
+// cn_fluid.hip -- PBF fluid iterations on gfx950: the density-constraint pass and the
+// delta + apply + clamp pass, both tiled over aligned 4x4x4-cell Morton blocks whose 6x6x6-cell
+// halo is staged in LDS, and the exact neighbor-list pass of the per-pass entry.
 //
 // Reference passes (assets/shaders/physics/fluid/solver/): physics_fluid_build_neighbors.cs.hlsl
 // :21-147, physics_fluid_compute_density_lambda.cs.hlsl:26-115,
 // physics_fluid_compute_delta_positions.cs.hlsl:27-157, physics_fluid_apply_delta_positions.cs.hlsl
-// :11-31, physics_fluid_clamp_boundary.cs.hlsl:23-80.  The reference rebuilds each fluid particle's
-// neighbor list every iteration: the fluid particles within h of its current position x* + delta,
-// found by walking the 27 cells around its broadphase cell in dz,dy,dx order, each cell's particles
-// in ascending particle index, keeping the first max_fluid_neighbors.
+// :11-31, physics_fluid_clamp_boundary.cs.hlsl:23-80.  Every iteration the reference builds each
+// fluid particle's neighbor list -- the fluid particles within h of its current position x* + delta,
+// found by walking the 27 cells around its broadphase cell in dz, dy, dx order, each cell's
+// particles in ascending slot, keeping the first max_fluid_neighbors -- and its density and delta
+// passes iterate that list.
 //
-// Here the first iteration of a substep walks the cells once and stores a superset list (radius
-// h + skin, same walk order) and every later iteration filters that list by the current distance
-// < h, which yields the same neighbors in the same order as long as no particle has moved more than
-// skin / 2 since the walk.  The delta pass tracks the largest displacement; when the bound could be
-// crossed the next iteration walks the cells again (exact lists), so results never depend on the
-// skin.  The walk itself is tiled: a workgroup takes one aligned 4x4x4 block of cells (one Morton
-// block), stages the current positions of its 6x6x6-cell halo in LDS and walks from there.
+// Here a workgroup takes one aligned 4x4x4-cell block and stages its 6x6x6-cell halo in LDS, laid
+// out cell by cell with x fastest, so the three dx cells of one (dz, dy) row are one contiguous LDS
+// range and the 27-cell walk is nine range loops.  The density pass walks them with a loop body
+// that only tests and compacts: the list it writes holds 16-bit halo offsets, not slots.  The sums
+// then run over the compacted list, where every lane of a wave has work, and so does the delta
+// pass, which stages the density records of the same halo and reads the list in the same offsets.
+// The in-radius test sqrt(d2) < h is evaluated as d2 < T(h), T being the smallest float whose
+// correctly rounded square root is >= h (the rounded square root is monotone), so the root is taken
+// for listed neighbors only.  Blocks that cannot be staged (impure cells, a halo over the LDS
+// capacity) and hash-key scenes walk the run table and sum in the walk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,22 +29,24 @@
 
 namespace cn {
 
-constexpr uint32_t kExact = 0x80000000u;  // nbr_cnt flag: the stored list is exact, not a superset
-constexpr int kHalo = 216;                // 6 x 6 x 6 cells around a 4 x 4 x 4 block
-constexpr int kTileMax = 2560;            // halo particles a workgroup stages in LDS
+// Record position of particles that no neighbor walk may list (non-fluid or zero inverse mass):
+// every squared distance to it is +inf, which fails d2 < T.
+constexpr float kFarAway = __builtin_huge_valf();
 
 struct FluidArgs {
-    const float4 *pos;
+    const float4 *pos;        // x* (slot order)
     const uint32_t *cls;
     const int4 *cellc;
     const uint32_t *bnd;
-    const float4 *din;
+    const float4 *din;        // accumulated delta of the previous iteration
     float4 *dout;
-    float4 *cur;
-    float4 *cs;
-    float4 *nc;
+    float4 *cur;              // (x* + delta, inverse mass) at the start of the iteration
+    float4 *cs;               // density-pass records: (x* + delta, C)
+    float4 *nc;               // surface normal * tension, C
     uint32_t *nbr;
     uint32_t *nbr_cnt;
+    uint16_t *list;           // listed neighbors as halo offsets, k-major (list[k * n + i]), LDS blocks
+    uint32_t *lcnt;           // listed neighbor count
     const uint32_t *run_key;
     const uint32_t *run_start;
     const int4 *run_info;
@@ -47,26 +54,16 @@ struct FluidArgs {
     uint32_t bucket_shift;
     const uint32_t *blk_first_run;
     const uint32_t *nblk;
+    uint32_t *blk_halo;
     uint32_t *iter_state;
     const ParticleClass *classes;
     const cn_fluid_material *fm;
+    const float *t2;          // per fluid material: d2 < t2 <=> sqrt(d2) < h
     const Collider *col;
     uint32_t *meta;
     uint32_t it;
+    uint32_t tile;            // halo particles this launch can stage in LDS
 };
-
-__device__ __forceinline__ uint32_t *work_counter(const FluidArgs &a) { return a.iter_state + min(a.it, kIterStateWords - 1); }
-__device__ __forceinline__ uint32_t *max_disp(const FluidArgs &a, uint32_t it) { return a.iter_state + kIterStateWords + it; }
-__device__ __forceinline__ uint32_t *walk_flag(const FluidArgs &a, uint32_t it) { return a.iter_state + 2 * kIterStateWords + it; }
-
-// True when iteration a.it must build its lists by walking the cells: the first iteration, after a
-// fallback earlier in the substep, or when the displacement bound of the superset lists is spent.
-__device__ __forceinline__ bool need_walk(const FluidArgs &a, const SubstepConsts &c)
-{
-    if (a.it == 0 || a.it >= kIterStateWords) return true;
-    if (*walk_flag(a, a.it - 1) != 0u) return true;
-    return !(__uint_as_float(*max_disp(a, a.it - 1)) < c.fallback_d2);
-}
 
 __device__ __forceinline__ int find_run(uint32_t key, const FluidArgs &a)
 {
@@ -98,64 +95,21 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// One particle's neighbor walk / filter state.
-struct Walk {
-    uint32_t i;
-    float sx, sy, sz;   // x* + delta of the particle
-    float h, rl;        // smoothing radius, list radius (h or h + skin)
-    uint32_t lcap;      // list slots (cap for exact lists, the allocation for supersets)
-    bool tension;
-    bool acc;           // accumulate density and counts (false on the list-only second pass)
-    ParticleClass self;
-    float density, snx, sny, snz;
-    uint32_t lcnt, inr, req;
-};
+// ---------------------------------------------------------------------------------------------
+// candidate walks, in the reference's order: f(k) per candidate until f returns false
+// ---------------------------------------------------------------------------------------------
 
-// One candidate of the walk, in walk order.  The density sum runs over the first `cap` in-radius
-// neighbors in that order, exactly the reference's list.
-template <bool UNIFORM>
-__device__ __forceinline__ void visit(Walk &w, uint32_t q, const float4 pj, uint32_t qcls, bool store,
-                                      const FluidArgs &a, const SubstepConsts &c)
-{
-    if (q == w.i) return;
-    if (!UNIFORM && !neighbor_ok(w.self, a.classes[qcls])) return;
-    if (pj.w <= kEps) return;
-    const float dx = w.sx - pj.x, dy = w.sy - pj.y, dz = w.sz - pj.z;
-    const float dist = len3(dx, dy, dz);
-    if (store && dist < w.rl) {
-        if (w.lcnt < w.lcap) a.nbr[(size_t)w.lcnt * c.n + w.i] = q;
-        ++w.lcnt;
-    }
-    if (!w.acc || dist >= w.h) return;
-    ++w.req;
-    if (w.inr >= c.cap) return;
-    ++w.inr;
-    w.density = w.density + spiky_w(dist, w.h);
-    if (w.tension) {
-        // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53
-        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        if (!(dist <= kEps || dist >= w.h)) {
-            const float dw = spiky_dw(dist, w.h);
-            gx = dw * fdiv(dx, dist);
-            gy = dw * fdiv(dy, dist);
-            gz = dw * fdiv(dz, dist);
-        }
-        w.snx = w.snx + gx;
-        w.sny = w.sny + gy;
-        w.snz = w.snz + gz;
-    }
-}
-
-// 27-cell walk through the run table (hash keys, fallback tiles, single exact re-walks).
-template <bool UNIFORM>
-__device__ void walk_global(Walk &w, int4 ci, bool store, const FluidArgs &a, const SubstepConsts &c)
+// The 27 cells through the run table (hash keys, blocks without a staged halo).  A cell whose hash
+// equals the empty-table marker is never found (physics_fluid_build_neighbors.cs.hlsl:81-85 with
+// FindParticleCellRange); a run shared by several cells (hash collision, Morton wrap) is filtered
+// by cell coordinates.
+template <class F>
+__device__ __forceinline__ void walk_runs(int4 ci, const FluidArgs &a, const SubstepConsts &c, F &&f)
 {
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
                 const int tx = ci.x + dx, ty = ci.y + dy, tz = ci.z + dz;
-                // a cell whose hash equals the empty-table marker is never found
-                // (physics_fluid_build_neighbors.cs.hlsl:81-85 with FindParticleCellRange)
                 if (cell_hash(tx, ty, tz) == kInvalid) continue;
                 const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
                 if (r < 0) continue;
@@ -168,97 +122,73 @@ __device__ void walk_global(Walk &w, int4 ci, bool store, const FluidArgs &a, co
                         const int4 cq = a.cellc[q];
                         if (cq.x != tx || cq.y != ty || cq.z != tz) continue;
                     }
-                    visit<UNIFORM>(w, q, a.cur[q], UNIFORM ? 0u : a.cls[q], store, a, c);
+                    if (!f(q)) return;
                 }
             }
 }
 
-struct MetaAcc {
-    uint32_t req = 0, trunc = 0, listed = 0;
+// The same walk over a staged halo (hc = the particle's halo cell): nine LDS ranges.
+template <class F>
+__device__ __forceinline__ void walk_rows(const uint32_t *hoff, int hc, F &&f)
+{
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int h0 = hc - 1 + 6 * dy + 36 * dz;
+            const uint32_t f1 = hoff[h0 + 3];
+            for (uint32_t k = hoff[h0]; k < f1; ++k)
+                if (!f(k)) return;
+        }
+}
+
+// f(list entry) for the cnt entries of one particle's k-major list, index loads issued four ahead.
+template <class F>
+__device__ __forceinline__ void for_list(const uint16_t *li, uint32_t cnt, uint32_t n, F &&f)
+{
+    for (uint32_t k = 0; k < cnt; k += 4) {
+        uint32_t e[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) e[u] = k + u < cnt ? li[(size_t)(k + u) * n] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (k + u < cnt) f(e[u]);
+    }
+}
+
+// Halo cell holding staged particle f: the last cell whose offset is <= f.
+__device__ __forceinline__ uint32_t halo_cell_of(const uint32_t *hoff, uint32_t f)
+{
+    uint32_t lo = 0, hi = kHalo;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hoff[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Block queue: the blocks are split into kXcds contiguous chunks, one per die, so neighboring
+// blocks (which share halo cells) run under the same L2; a workgroup drains its die's chunk, then
+// helps the others.  Called by one thread; returns kInvalid when every chunk is drained.
+struct BlockQueue {
+    uint32_t die, tries;
+    __device__ explicit BlockQueue(uint32_t wg) : die(wg % kXcds), tries(0) {}
+    __device__ uint32_t next(uint32_t *q, uint32_t nblk)
+    {
+        while (tries < kXcds) {
+            const uint32_t b0 = (uint32_t)(((uint64_t)nblk * die) / kXcds);
+            const uint32_t b1 = (uint32_t)(((uint64_t)nblk * (die + 1)) / kXcds);
+            const uint32_t t = atomicAdd(q + die, 1u);
+            if (t < b1 - b0) return b0 + t;
+            die = (die + 1) % kXcds;
+            ++tries;
+        }
+        return kInvalid;
+    }
 };
 
-__device__ __forceinline__ void begin(Walk &w, uint32_t i, const ParticleClass &pc, const cn_fluid_material &m,
-                                      const float4 s, bool exact, const SubstepConsts &c)
-{
-    w.i = i;
-    w.sx = s.x;
-    w.sy = s.y;
-    w.sz = s.z;
-    w.h = fmaxd(m.smoothing_radius, 1.0e-4f);
-    w.rl = exact ? w.h : w.h + c.skin;
-    w.lcap = exact ? c.cap : c.list_cap;
-    w.tension = fmaxd(m.surface_tension, 0.0f) > kEps;
-    w.acc = true;
-    w.self = pc;
-    w.density = w.snx = w.sny = w.snz = 0.0f;
-    w.lcnt = w.inr = w.req = 0;
-}
-
-// Boundary ghost density and the density constraint
-// (physics_fluid_compute_density_lambda.cs.hlsl:97-114, physics_fluid_boundary_common.hlsli:112-138).
-__device__ __forceinline__ void finish(const Walk &w, const cn_fluid_material &m, uint32_t list_word, bool write_list,
-                                       MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
-{
-    float density = w.density;
-    const uint32_t bm = a.bnd[w.i];
-    const float bscale = c.cfg[kCfgFluidBndDensity];
-    for (uint32_t k = 0; k < c.ncolliders; ++k) {
-        if (!((bm >> k) & 1u)) continue;
-        float gx, gy, gz;
-        if (!ghost_delta(a.col[k], w.sx, w.sy, w.sz, w.h, gx, gy, gz)) continue;
-        const float gd = len3(gx, gy, gz);
-        if (gd >= w.h) continue;
-        density = density + spiky_w(gd, w.h) * bscale;
-    }
-    const float rho0 = fmaxd(m.rest_density, 1.0f);
-    const float C = fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) * m.density_constraint_scale;
-    a.cs[w.i] = make_float4(w.sx, w.sy, w.sz, C);
-    if (w.tension) {
-        const float st = fmaxd(m.surface_tension, 0.0f);
-        a.nc[w.i] = make_float4(w.snx * st, w.sny * st, w.snz * st, C);
-    }
-    if (write_list) a.nbr_cnt[w.i] = list_word;
-    ma.req = max(ma.req, w.req);
-    ma.trunc += w.req > c.cap ? 1u : 0u;
-    ma.listed += w.inr;
-}
-
-__device__ __forceinline__ void inactive(uint32_t i, const FluidArgs &a)
-{
-    const float4 s = a.cur[i];
-    a.cs[i] = make_float4(s.x, s.y, s.z, 0.0f);
-    a.nbr_cnt[i] = kExact;
-}
-
-// Walk one particle through the run table and store its list (superset on the first iteration).
-template <bool UNIFORM>
-__device__ void walk_particle_global(uint32_t i, bool exact, MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
-{
-    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-    const float4 s = a.cur[i];
-    if (pc.kind != CN_PARTICLE_KIND_FLUID || s.w <= kEps) {
-        inactive(i, a);
-        return;
-    }
-    const cn_fluid_material m = a.fm[pc.fluid_mat];
-    Walk w;
-    begin(w, i, pc, m, s, exact, c);
-    const bool self_ok = !UNIFORM || neighbor_ok(pc, pc);
-    const int4 ci = a.cellc[i];
-    if (c.cap > 0 && self_ok) {
-        walk_global<UNIFORM>(w, ci, true, a, c);
-        if (w.lcnt > w.lcap) {  // superset overflow: a second, list-only pass stores the exact list
-            w.rl = w.h;
-            w.lcap = c.cap;
-            w.lcnt = 0;
-            w.acc = false;
-            walk_global<UNIFORM>(w, ci, true, a, c);
-            exact = true;
-        }
-    }
-    const uint32_t word = min(w.lcnt, w.lcap) | (exact ? kExact : 0u);
-    finish(w, m, word, true, ma, a, c);
-}
+struct MetaAcc {
+    uint32_t req = 0, trunc = 0, listed = 0, lds = 0, runs = 0;
+};
 
 __device__ __forceinline__ void flush_meta(const MetaAcc &ma, const FluidArgs &a)
 {
@@ -270,71 +200,335 @@ __device__ __forceinline__ void flush_meta(const MetaAcc &ma, const FluidArgs &a
         if (wtr) atomicAdd(&a.meta[1], wtr);
         if (a.it == 0 && wls) atomicAdd((unsigned long long *)&a.meta[2], (unsigned long long)wls);
     }
-}
-
-__device__ __forceinline__ void walk_bookkeeping(bool walk, const FluidArgs &a)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.it < kIterStateWords) {
-        *max_disp(a, a.it) = 0u;
-        *walk_flag(a, a.it) = (a.it > 0 && walk) ? 1u : 0u;
-        if (a.it > 0 && walk) atomicAdd(&a.meta[4], 1u);
+    if (threadIdx.x == 0) {
+        if (ma.lds) atomicAdd(&a.meta[5], ma.lds);
+        if (ma.runs) atomicAdd(&a.meta[6], ma.runs);
     }
 }
 
-// Thread-per-particle walk (hash keys).
-template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_walk_global(FluidArgs a, SubstepConsts c)
+// ---------------------------------------------------------------------------------------------
+// density pass: physics_fluid_build_neighbors.cs.hlsl:88-131 (the list filter) with
+// physics_fluid_compute_density_lambda.cs.hlsl:26-115 (density, surface normal, constraint)
+// ---------------------------------------------------------------------------------------------
+struct DensitySum {
+    float density = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
+    uint32_t inr = 0, req = 0;
+};
+
+struct DensitySelf {
+    ParticleClass pc;
+    cn_fluid_material m;
+    float4 s;
+    float h, t2, kw;
+    bool tension;
+};
+
+// Returns false (and writes the unreachable record) for particles that take no part in the solve.
+__device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const float4 s, uint32_t ic,
+                                              const FluidArgs &a)
 {
-    const bool walk = need_walk(a, c);
-    walk_bookkeeping(walk, a);
-    if (!walk) return;
-    MetaAcc ma;
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (i < c.n) walk_particle_global<UNIFORM>(i, a.it > 0, ma, a, c);
-    flush_meta(ma, a);
+    x.pc = a.classes[ic];
+    x.s = s;
+    if (x.pc.kind != CN_PARTICLE_KIND_FLUID || s.w <= kEps) {
+        a.cs[i] = make_float4(kFarAway, kFarAway, kFarAway, 0.0f);
+        return false;
+    }
+    x.m = a.fm[x.pc.fluid_mat];
+    x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
+    x.t2 = a.t2[x.pc.fluid_mat];
+    x.kw = spiky_coef(x.h);
+    x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
+    return true;
 }
 
-// Tiled walk (Morton keys): persistent workgroups take Morton blocks from a work counter.
-template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_walk_tiled(FluidArgs a, SubstepConsts c)
+__device__ __forceinline__ bool density_walks(const DensitySelf &x, bool uniform, const SubstepConsts &c)
 {
-    __shared__ float4 lcur[kTileMax];
-    __shared__ uint32_t lcls[UNIFORM ? 1 : kTileMax];
-    __shared__ uint32_t hstart[kHalo];
-    __shared__ uint32_t hoff[kHalo + 1];
-    __shared__ uint32_t scan[kBlock];
-    __shared__ uint32_t s_blk;
-    __shared__ int s_base[3];
+    return c.cap > 0 && (!uniform || neighbor_ok(x.pc, x.pc));
+}
 
-    const bool walk = need_walk(a, c);
-    walk_bookkeeping(walk, a);
-    if (!walk) return;
-    const bool exact = a.it > 0;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t nblk = *a.nblk;
-    MetaAcc ma;
-    for (;;) {
-        if (tid == 0) s_blk = atomicAdd(work_counter(a), 1u);
-        __syncthreads();
-        const uint32_t b = s_blk;
-        if (b >= nblk) break;
-        const uint32_t r0 = a.blk_first_run[b], r1 = a.blk_first_run[b + 1];
-        const uint32_t p0 = a.run_start[r0], p1 = a.run_start[r1];
-        if (tid == 0) {
-            const int4 c0 = a.cellc[p0];
-            s_base[0] = c0.x & ~3;
-            s_base[1] = c0.y & ~3;
-            s_base[2] = c0.z & ~3;
+// The list filter: candidate pj (class qcls) is a neighbor.
+template <bool UNIFORM>
+__device__ __forceinline__ bool density_test(const DensitySelf &x, const float4 pj, uint32_t qcls, const FluidArgs &a)
+{
+    if (!UNIFORM && !neighbor_ok(x.pc, a.classes[qcls])) return false;
+    if (pj.w <= kEps) return false;
+    const float dx = x.s.x - pj.x, dy = x.s.y - pj.y, dz = x.s.z - pj.z;
+    return dot3(dx, dy, dz, dx, dy, dz) < x.t2;
+}
+
+// Density and surface-normal terms of one listed neighbor.
+__device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x, const float4 pj)
+{
+    const float dx = x.s.x - pj.x, dy = x.s.y - pj.y, dz = x.s.z - pj.z;
+    const float dist = fsqrt(dot3(dx, dy, dz, dx, dy, dz));
+    d.density = d.density + spiky_w_k(dist, x.h, x.kw);
+    if (x.tension) {
+        // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if (!(dist <= kEps || dist >= x.h)) {
+            const float dw = spiky_dw_k(dist, x.h, x.kw);
+            gx = dw * fdiv(dx, dist);
+            gy = dw * fdiv(dy, dist);
+            gz = dw * fdiv(dz, dist);
         }
-        __syncthreads();
-        const int bx = s_base[0], by = s_base[1], bz = s_base[2];
+        d.snx = d.snx + gx;
+        d.sny = d.sny + gy;
+        d.snz = d.snz + gz;
+    }
+}
+
+// Boundary ghost density (physics_fluid_boundary_common.hlsli:112-138), the constraint and the record.
+__device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, const DensitySelf &x, MetaAcc &ma,
+                                            const FluidArgs &a, const SubstepConsts &c)
+{
+    float density = d.density;
+    const uint32_t bm = a.bnd[i];
+    const float bscale = c.cfg[kCfgFluidBndDensity];
+    for (uint32_t k = 0; k < c.ncolliders; ++k) {
+        if (!((bm >> k) & 1u)) continue;
+        float gx, gy, gz;
+        if (!ghost_delta(a.col[k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) continue;
+        const float gd = len3(gx, gy, gz);
+        if (gd >= x.h) continue;
+        density = density + spiky_w_k(gd, x.h, x.kw) * bscale;
+    }
+    const float rho0 = fmaxd(x.m.rest_density, 1.0f);
+    const float C =
+        fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) * x.m.density_constraint_scale;
+    a.cs[i] = make_float4(x.s.x, x.s.y, x.s.z, C);
+    if (x.tension) {
+        const float st = fmaxd(x.m.surface_tension, 0.0f);
+        a.nc[i] = make_float4(d.snx * st, d.sny * st, d.snz * st, C);
+    }
+    ma.req = max(ma.req, d.req);
+    ma.trunc += d.req > c.cap ? 1u : 0u;
+    ma.listed += d.inr;
+}
+
+// Run-table walk with the sums taken in the walk.
+template <bool UNIFORM>
+__device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
+{
+    DensitySelf x;
+    if (!density_begin(x, i, a.cur[i], UNIFORM ? 0u : a.cls[i], a)) return;
+    DensitySum d;
+    if (density_walks(x, UNIFORM, c))
+        walk_runs(a.cellc[i], a, c, [&](uint32_t q) {
+            if (q == i) return true;
+            const float4 pj = a.cur[q];
+            if (!density_test<UNIFORM>(x, pj, UNIFORM ? 0u : a.cls[q], a)) return true;
+            ++d.req;
+            if (d.inr < c.cap) {
+                ++d.inr;
+                density_add(d, x, pj);
+            }
+            return true;
+        });
+    density_end(i, d, x, ma, a, c);
+}
+
+// ---------------------------------------------------------------------------------------------
+// delta pass: physics_fluid_compute_delta_positions.cs.hlsl:27-157, then
+// physics_fluid_apply_delta_positions.cs.hlsl:11-31 and physics_fluid_clamp_boundary.cs.hlsl:23-80
+// on the same particle.  Returns (|delta| + slack)^2 of the particle.
+// walk(v) calls v(k) for every neighbor record k in list order (LISTED: the walk yields exactly the
+// listed neighbors; otherwise it yields every candidate but the particle's own record, and the list
+// filter is applied here); rcs / rdin / rnc / rcls address the records (LDS or global).
+// ---------------------------------------------------------------------------------------------
+template <bool UNIFORM, bool LISTED, class Walk>
+__device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Walk &&walk, const float4 *rcs,
+                                                const float4 *rdin, const float4 *rnc, const uint32_t *rcls,
+                                                const FluidArgs &a, const SubstepConsts &c)
+{
+    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    if (pc.kind != CN_PARTICLE_KIND_FLUID) {
+        a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return 0.0f;
+    }
+    const float4 p = a.pos[i];
+    const bool use_d = a.it > 0;
+    const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
+    if (p.w > kEps) {
+        const float4 csi = rcs[own];
+        const float sx = csi.x, sy = csi.y, sz = csi.z, Ci = csi.w;
+        const cn_fluid_material m = a.fm[pc.fluid_mat];
+        const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
+        const float t2 = a.t2[pc.fluid_mat];
+        const float kw = spiky_coef(h);
+        const float cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
+        const float viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
+        const float st = fmaxd(m.surface_tension, 0.0f);
+        const float cfl = fmaxd(m.cfl_radius, 0.0f);
+        const bool tension = st > kEps;
+        // relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can only
+        // act when |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl
+        bool cfl_on = false;
+        if (use_d && cfl > kEps) {
+            const float md2 = a.it - 1 < kIterStateWords ? __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1])
+                                                         : 3.4e38f;
+            cfl_on = !(4.0f * md2 * c.cfl_guard < cfl * cfl);
+        }
+        const bool visc_on = use_d && viscosity > kEps;
+        const bool need_rel = cfl_on || visc_on;
+        const float4 nci = tension ? rnc[own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f, wsum = 0.0f;
+        uint32_t inr = 0;
+        if (LISTED || (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))))
+            walk([&](uint32_t k) {
+                if (!LISTED && !UNIFORM && !neighbor_ok(pc, a.classes[rcls[k]])) return true;
+                const float4 cq = rcs[k];
+                const float ddx = sx - cq.x, ddy = sy - cq.y, ddz = sz - cq.z;
+                const float d2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
+                if (!LISTED) {
+                    if (!(d2 < t2)) return true;
+                    if (inr >= c.cap) return false;
+                    ++inr;
+                }
+                const float dist = fsqrt(d2);
+                if (dist <= kEps) return true;
+                const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
+                const float der = spiky_dw_k(dist, h, kw);
+                const float sc = -0.5f * (Ci + cq.w) * der;
+                float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
+                const bool same = UNIFORM || pc.fluid_mat == a.classes[rcls[k]].fluid_mat;
+                if (same) {
+                    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+                    if (need_rel) {
+                        const float4 dq = rdin[k];
+                        rx = di.x - dq.x;
+                        ry = di.y - dq.y;
+                        rz = di.z - dq.z;
+                    }
+                    if (cohesion > kEps) {
+                        const float ck = cohesion * cohesion_kernel(dist, h, m.cohesion1, m.cohesion2);
+                        pdx = pdx - nx * ck;
+                        pdy = pdy - ny * ck;
+                        pdz = pdz - nz * ck;
+                    }
+                    if (visc_on) {
+                        const float vs = 1.0f - fdiv(1.0f, 1.0f + viscosity * spiky_w_k(dist, h, kw));
+                        pdx = pdx - vs * rx;
+                        pdy = pdy - vs * ry;
+                        pdz = pdz - vs * rz;
+                    }
+                    if (cfl_on) {
+                        const float proj = dot3(rx, ry, rz, nx, ny, nz);
+                        if (proj < -cfl) {
+                            const float pcf = proj + cfl;
+                            pdx = pdx - nx * pcf * 0.5f;
+                            pdy = pdy - ny * pcf * 0.5f;
+                            pdz = pdz - nz * pcf * 0.5f;
+                        }
+                    }
+                    if (tension) {
+                        const float4 ncq = rnc[k];
+                        pdx = pdx - c.dt * (nci.x - ncq.x);
+                        pdy = pdy - c.dt * (nci.y - ncq.y);
+                        pdz = pdz - c.dt * (nci.z - ncq.z);
+                    }
+                }
+                dpx = dpx + pdx;
+                dpy = dpy + pdy;
+                dpz = dpz + pdz;
+                wsum = wsum + 1.0f;
+                return true;
+            });
+        // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
+        float bx = 0.0f, by = 0.0f, bz = 0.0f;
+        const float scn = fmaxd(Ci, 0.0f);
+        const uint32_t bm = a.bnd[i];
+        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+            if (!((bm >> k) & 1u)) continue;
+            float gx, gy, gz;
+            if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
+            const float gd = len3(gx, gy, gz);
+            if (gd <= kEps || gd >= h) continue;
+            const float f = -scn * spiky_dw_k(gd, h, kw);
+            bx = bx + f * fdiv(gx, gd);
+            by = by + f * fdiv(gy, gd);
+            bz = bz + f * fdiv(gz, gd);
+        }
+        const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
+        ox = dpx * solve + bx * bdel;
+        oy = dpy * solve + by * bdel;
+        oz = dpz * solve + bz * bdel;
+        ow = wsum;
+    }
+    // apply
+    const float wgt = fmaxd(ow * c.cfg[kCfgFluidRelax], 1.0f);
+    const float ax = di.x + fdiv(ox, wgt), ay = di.y + fdiv(oy, wgt), az = di.z + fdiv(oz, wgt);
+    // clamp
+    const float r = fmaxd(pc.radius, 0.0f);
+    float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
+    const uint32_t bm = a.bnd[i];
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        bool changed = false;
+        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+            if (!((bm >> k) & 1u)) continue;
+            float sd, nx, ny, nz;
+            if (!box_contact(a.col[k], cx, cy, cz, sd, nx, ny, nz)) continue;
+            if (sd >= r) continue;
+            const float push = r - sd;
+            cx = cx + nx * push;
+            cy = cy + ny * push;
+            cz = cz + nz * push;
+            changed = true;
+        }
+        if (!changed) break;
+    }
+    const float ex = cx - p.x, ey = cy - p.y, ez = cz - p.z;
+    a.dout[i] = make_float4(ex, ey, ez, 0.0f);
+    const float nx = p.x + ex, ny = p.y + ey, nz = p.z + ez;
+    a.cur[i] = make_float4(nx, ny, nz, p.w);
+    // |delta| plus a few ulps of the position: bounds the next iteration's relDelta for the CFL skip
+    const float slack = fmaxd(fmaxd(fabsf(nx), fabsf(ny)), fabsf(nz)) * 4.76837158e-7f + 1.0e-30f;
+    const float dl = len3(ex, ey, ez) + slack;
+    return dl * dl;
+}
+
+template <bool UNIFORM>
+__device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, const SubstepConsts &c)
+{
+    const int4 ci = a.cellc[i];
+    return delta_particle<UNIFORM, false>(
+        i, i, [&](auto &&v) { walk_runs(ci, a, c, [&](uint32_t q) { return q == i || v(q); }); }, a.cs, a.din, a.nc,
+        a.cls, a, c);
+}
+
+__device__ __forceinline__ void flush_disp(float d2, const FluidArgs &a)
+{
+    if (a.it >= kIterStateWords) return;
+    const uint32_t wm = wave_max(__float_as_uint(d2));
+    if ((threadIdx.x & 63u) == 0 && wm) {
+        atomicMax(a.iter_state + kStateMaxDisp + a.it, wm);
+        atomicMax(&a.meta[7], wm);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-block halo descriptors, once per substep
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts c)
+{
+    __shared__ uint32_t scan[kBlock];
+    const uint32_t nb = min(*a.nblk, c.blk_cap);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t p0 = a.run_start[a.blk_first_run[b]], p1 = a.run_start[a.blk_first_run[b + 1]];
+        const int4 c0 = a.cellc[p0];
+        const int bx = c0.x & ~3, by = c0.y & ~3, bz = c0.z & ~3;
+        // particles of a wrapped Morton block that lie outside the aligned block, particles whose own
+        // cell hashes to the empty-table marker (their cell has no halo range), or halo cells shared
+        // with other cells leave the block to the run-table walk
         bool bad = false;
         for (uint32_t i = p0 + tid; i < p1; i += kBlock) {
             const int4 ci = a.cellc[i];
             bad |= (uint32_t)(ci.x - bx) > 3u || (uint32_t)(ci.y - by) > 3u || (uint32_t)(ci.z - bz) > 3u;
+            bad |= cell_hash(ci.x, ci.y, ci.z) == kInvalid;
         }
         uint32_t cnt = 0, st = 0;
-        if (tid < (uint32_t)kHalo) {
+        if (tid < kHalo) {
             const int tx = bx - 1 + (int)(tid % 6u), ty = by - 1 + (int)((tid / 6u) % 6u), tz = bz - 1 + (int)(tid / 36u);
             if (cell_hash(tx, ty, tz) != kInvalid) {
                 const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
@@ -347,265 +541,235 @@ __global__ __launch_bounds__(kBlock) void k_fluid_walk_tiled(FluidArgs a, Subste
                     }
                 }
             }
-            hstart[tid] = st;
         }
         const int any_bad = __syncthreads_or(bad ? 1 : 0);
         uint32_t total;
         const uint32_t off = block_exclusive_scan(cnt, scan, total);
-        if (tid < (uint32_t)kHalo) hoff[tid] = off;
-        if (tid == 0) hoff[kHalo] = total;
-        __syncthreads();
-        const bool lds = !any_bad && total <= (uint32_t)kTileMax;
-        if (tid == 0) atomicAdd(&a.meta[lds ? 5 : 6], 1u);
-        if (lds) {
+        uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
+        if (tid < kHalo) {
+            hd[kHdStart + tid] = st;
+            hd[kHdOff + tid] = off;
+        }
+        if (tid == 0) {
+            hd[kHdOff + kHalo] = total;
+            hd[kHdFlags] = any_bad ? 0u : 1u;
+            hd[kHdP0] = p0;
+            hd[kHdP1] = p1;
+            hd[kHdBase] = (uint32_t)bx;
+            hd[kHdBase + 1] = (uint32_t)by;
+            hd[kHdBase + 2] = (uint32_t)bz;
+        }
+    }
+}
+
+// Common head of the tiled passes: claim a block, decide LDS or run table, load the halo table.
+struct TileState {
+    uint32_t blk;
+    uint32_t p0, p1;
+    bool lds;
+    int bx, by, bz;
+};
+
+__device__ __forceinline__ bool tile_begin(BlockQueue &bq, uint32_t *queue, uint32_t nblk, uint32_t *s_blk,
+                                           uint32_t *hoff, uint32_t *hstart, TileState &ts, const FluidArgs &a,
+                                           const SubstepConsts &c)
+{
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) *s_blk = bq.next(queue, nblk);
+    __syncthreads();
+    const uint32_t b = *s_blk;
+    if (b == kInvalid) return false;
+    ts.blk = b;
+    ts.lds = false;
+    if (b < c.blk_cap) {
+        const uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
+        ts.p0 = hd[kHdP0];
+        ts.p1 = hd[kHdP1];
+        ts.lds = hd[kHdFlags] != 0u && hd[kHdOff + kHalo] <= a.tile;
+        if (ts.lds) {
+            for (uint32_t t = tid; t < kHalo; t += kBlock) {
+                hstart[t] = hd[kHdStart + t];
+                hoff[t] = hd[kHdOff + t];
+            }
+            if (tid == 0) hoff[kHalo] = hd[kHdOff + kHalo];
+            ts.bx = (int)hd[kHdBase];
+            ts.by = (int)hd[kHdBase + 1];
+            ts.bz = (int)hd[kHdBase + 2];
+        }
+    } else {
+        ts.p0 = a.run_start[a.blk_first_run[b]];
+        ts.p1 = a.run_start[a.blk_first_run[b + 1]];
+    }
+    __syncthreads();
+    return true;
+}
+
+__device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
+{
+    return (ci.x - ts.bx + 1) + 6 * (ci.y - ts.by + 1) + 36 * (ci.z - ts.bz + 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------------
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, SubstepConsts c)
+{
+    extern __shared__ float4 dyn[];
+    __shared__ uint32_t hoff[kHalo + 1];
+    __shared__ uint32_t hstart[kHalo];
+    __shared__ uint32_t s_blk;
+    float4 *lpos = dyn;
+    uint32_t *lcls = reinterpret_cast<uint32_t *>(dyn + a.tile);
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
+    const uint32_t nblk = *a.nblk;
+    uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
+    BlockQueue bq(blockIdx.x);
+    MetaAcc ma;
+    TileState ts;
+    while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
+        if (ts.lds) {
+            const uint32_t total = hoff[kHalo];
             for (uint32_t f = tid; f < total; f += kBlock) {
-                uint32_t lo = 0, hi = kHalo;  // last h with hoff[h] <= f
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (hoff[mid] <= f) lo = mid;
-                    else hi = mid;
-                }
-                const uint32_t q = hstart[lo] + (f - hoff[lo]);
-                lcur[f] = a.cur[q];
+                const uint32_t hc = halo_cell_of(hoff, f);
+                const uint32_t q = hstart[hc] + (f - hoff[hc]);
+                lpos[f] = a.cur[q];
                 if (!UNIFORM) lcls[f] = a.cls[q];
             }
-        }
-        __syncthreads();
-        for (uint32_t i = p0 + tid; i < p1; i += kBlock) {
-            if (!lds) {
-                walk_particle_global<UNIFORM>(i, exact, ma, a, c);
-                continue;
-            }
-            const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-            const float4 s = a.cur[i];
-            if (pc.kind != CN_PARTICLE_KIND_FLUID || s.w <= kEps) {
-                inactive(i, a);
-                continue;
-            }
-            const cn_fluid_material m = a.fm[pc.fluid_mat];
-            Walk w;
-            begin(w, i, pc, m, s, exact, c);
-            const bool self_ok = !UNIFORM || neighbor_ok(pc, pc);
-            const int4 ci = a.cellc[i];
-            bool ex = exact;
-            if (c.cap > 0 && self_ok) {
-                const int rx = ci.x - bx + 1, ry = ci.y - by + 1, rz = ci.z - bz + 1;
-                for (int pass = 0; pass < 2; ++pass) {
-                    for (int dz = -1; dz <= 1; ++dz)
-                        for (int dy = -1; dy <= 1; ++dy)
-                            for (int dx = -1; dx <= 1; ++dx) {
-                                const int hh = (rx + dx) + 6 * (ry + dy) + 36 * (rz + dz);
-                                const uint32_t f0 = hoff[hh], f1 = hoff[hh + 1];
-                                const uint32_t qb = hstart[hh] - f0;
-                                for (uint32_t f = f0; f < f1; ++f)
-                                    visit<UNIFORM>(w, qb + f, lcur[f], UNIFORM ? 0u : lcls[f], true, a, c);
+            __syncthreads();
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) {
+                const int hc = halo_cell(ts, a.cellc[i]);
+                const uint32_t fs = hoff[hc] + (i - hstart[hc]);
+                DensitySelf x;
+                uint32_t cnt = 0;
+                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a)) {
+                    DensitySum d;
+                    uint16_t *li = a.list + i;
+                    if (density_walks(x, UNIFORM, c)) {
+                        // the walk only compacts the listed neighbors' halo offsets (a cheap loop
+                        // body for every candidate); the sums then run over the list
+                        walk_rows(hoff, hc, [&](uint32_t f) {
+                            if (f != fs && density_test<UNIFORM>(x, lpos[f], UNIFORM ? 0u : lcls[f], a)) {
+                                if (d.req < c.cap) li[(size_t)d.req * c.n] = (uint16_t)f;
+                                ++d.req;
                             }
-                    if (w.lcnt <= w.lcap) break;
-                    // superset overflow: a second, list-only pass stores the exact list
-                    w.rl = w.h;
-                    w.lcap = c.cap;
-                    w.lcnt = 0;
-                    w.acc = false;
-                    ex = true;
+                            return true;
+                        });
+                        d.inr = min(d.req, c.cap);
+                        for_list(li, d.inr, c.n, [&](uint32_t f) { density_add(d, x, lpos[f]); });
+                    }
+                    cnt = d.inr;
+                    density_end(i, d, x, ma, a, c);
                 }
+                a.lcnt[i] = cnt;
             }
-            const uint32_t word = min(w.lcnt, w.lcap) | (ex ? kExact : 0u);
-            finish(w, m, word, true, ma, a, c);
+            if (tid == 0) ++ma.lds;
+        } else {
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) density_global<UNIFORM>(i, ma, a, c);
+            if (tid == 0) ++ma.runs;
         }
         __syncthreads();
     }
     flush_meta(ma, a);
 }
 
-// Later iterations: filter the stored superset list by the current distance.
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_filter(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kBlock) void k_fluid_delta_tiled(FluidArgs a, SubstepConsts c)
 {
-    if (need_walk(a, c)) return;
+    extern __shared__ float4 dyn[];
+    __shared__ uint32_t hoff[kHalo + 1];
+    __shared__ uint32_t hstart[kHalo];
+    __shared__ uint32_t s_blk;
+    const bool use_d = a.it > 0;
+    const bool tension = c.tension != 0u;
+    float4 *lcs = dyn;
+    float4 *ldin = lcs + a.tile;
+    float4 *lnc = ldin + (use_d ? a.tile : 0u);
+    uint32_t *lcls = reinterpret_cast<uint32_t *>(lnc + (tension ? a.tile : 0u));
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nblk = *a.nblk;
+    uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds;
+    BlockQueue bq(blockIdx.x);
+    TileState ts;
+    float d2max = 0.0f;
+    while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
+        if (ts.lds) {
+            const uint32_t total = hoff[kHalo];
+            for (uint32_t f = tid; f < total; f += kBlock) {
+                const uint32_t hc = halo_cell_of(hoff, f);
+                const uint32_t q = hstart[hc] + (f - hoff[hc]);
+                lcs[f] = a.cs[q];
+                if (use_d) ldin[f] = a.din[q];
+                if (tension) lnc[f] = a.nc[q];
+                if (!UNIFORM) lcls[f] = a.cls[q];
+            }
+            __syncthreads();
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) {
+                const int hc = halo_cell(ts, a.cellc[i]);
+                const uint32_t fs = hoff[hc] + (i - hstart[hc]);
+                const uint16_t *li = a.list + i;
+                const float d2 = delta_particle<UNIFORM, true>(
+                    i, fs, [&](auto &&v) { for_list(li, a.lcnt[i], c.n, v); }, lcs, ldin, lnc, lcls, a, c);
+                d2max = fmaxd(d2max, d2);
+            }
+        } else {
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) d2max = fmaxd(d2max, delta_global<UNIFORM>(i, a, c));
+        }
+        __syncthreads();
+    }
+    flush_disp(d2max, a);
+}
+
+// Thread per particle through the run table: hash keys, and iterations past the tracked ones.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_density_global(FluidArgs a, SubstepConsts c)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.it < kIterStateWords) a.iter_state[kStateMaxDisp + a.it] = 0u;
     MetaAcc ma;
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (i < c.n) {
-        const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-        const float4 s = a.cur[i];
-        if (pc.kind != CN_PARTICLE_KIND_FLUID || s.w <= kEps) {
-            inactive(i, a);
-        } else {
-            const uint32_t word = a.nbr_cnt[i];
-            if (word & kExact) {
-                walk_particle_global<UNIFORM>(i, true, ma, a, c);
-            } else {
-                const cn_fluid_material m = a.fm[pc.fluid_mat];
-                Walk w;
-                begin(w, i, pc, m, s, true, c);
-                for (uint32_t k = 0; k < word; ++k) {
-                    const uint32_t q = a.nbr[(size_t)k * c.n + i];
-                    visit<true>(w, q, a.cur[q], 0u, false, a, c);
-                }
-                finish(w, m, word, false, ma, a, c);
-            }
-        }
-    }
+    if (i < c.n) density_global<UNIFORM>(i, ma, a, c);
     flush_meta(ma, a);
 }
 
-// physics_fluid_compute_delta_positions.cs.hlsl:27-157 over the stored list (in-radius entries in
-// list order up to the slot budget), then physics_fluid_apply_delta_positions.cs.hlsl:11-31 and
-// physics_fluid_clamp_boundary.cs.hlsl:23-80 on the same particle.
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_delta(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kBlock) void k_fluid_delta_global(FluidArgs a, SubstepConsts c)
 {
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     float d2 = 0.0f;
-    if (i < c.n) {
-        const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
-        if (pc.kind != CN_PARTICLE_KIND_FLUID) {
-            a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else {
-            const float4 p = a.pos[i];
-            const bool use_d = a.it > 0;
-            const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
-            if (p.w > kEps) {
-                const float4 csi = a.cs[i];
-                const float sx = csi.x, sy = csi.y, sz = csi.z, Ci = csi.w;
-                const cn_fluid_material m = a.fm[pc.fluid_mat];
-                const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
-                const float cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
-                const float viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
-                const float st = fmaxd(m.surface_tension, 0.0f);
-                const float cfl = fmaxd(m.cfl_radius, 0.0f);
-                const bool tension = st > kEps;
-                // relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can
-                // only act when |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl
-                bool cfl_on = false;
-                if (use_d && cfl > kEps) {
-                    const float md2 = a.it - 1 < kIterStateWords ? __uint_as_float(*max_disp(a, a.it - 1)) : 3.4e38f;
-                    cfl_on = !(4.0f * md2 * c.cfl_guard < cfl * cfl);
-                }
-                const bool visc_on = use_d && viscosity > kEps;
-                const bool need_rel = cfl_on || visc_on;
-                float4 nci = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (tension) nci = a.nc[i];
-                float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f, wsum = 0.0f;
-                const uint32_t cnt = a.nbr_cnt[i] & ~kExact;
-                uint32_t inr = 0;
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const uint32_t q = a.nbr[(size_t)k * c.n + i];
-                    const float4 cq = a.cs[q];
-                    const float ddx = sx - cq.x, ddy = sy - cq.y, ddz = sz - cq.z;
-                    const float dist = len3(ddx, ddy, ddz);
-                    if (dist >= h) continue;
-                    if (inr >= c.cap) break;
-                    ++inr;
-                    if (dist <= kEps) continue;
-                    const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
-                    const float der = spiky_dw(dist, h);
-                    const float sc = -0.5f * (Ci + cq.w) * der;
-                    float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
-                    const bool same = UNIFORM || pc.fluid_mat == a.classes[a.cls[q]].fluid_mat;
-                    if (same) {
-                        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-                        if (need_rel) {
-                            const float4 dq = a.din[q];
-                            rx = di.x - dq.x;
-                            ry = di.y - dq.y;
-                            rz = di.z - dq.z;
-                        }
-                        if (cohesion > kEps) {
-                            const float ck = cohesion * cohesion_kernel(dist, h, m.cohesion1, m.cohesion2);
-                            pdx = pdx - nx * ck;
-                            pdy = pdy - ny * ck;
-                            pdz = pdz - nz * ck;
-                        }
-                        if (visc_on) {
-                            const float vs = 1.0f - fdiv(1.0f, 1.0f + viscosity * spiky_w(dist, h));
-                            pdx = pdx - vs * rx;
-                            pdy = pdy - vs * ry;
-                            pdz = pdz - vs * rz;
-                        }
-                        if (cfl_on) {
-                            const float proj = dot3(rx, ry, rz, nx, ny, nz);
-                            if (proj < -cfl) {
-                                const float pcf = proj + cfl;
-                                pdx = pdx - nx * pcf * 0.5f;
-                                pdy = pdy - ny * pcf * 0.5f;
-                                pdz = pdz - nz * pcf * 0.5f;
-                            }
-                        }
-                        if (tension) {
-                            const float4 ncq = a.nc[q];
-                            pdx = pdx - c.dt * (nci.x - ncq.x);
-                            pdy = pdy - c.dt * (nci.y - ncq.y);
-                            pdz = pdz - c.dt * (nci.z - ncq.z);
-                        }
-                    }
-                    dpx = dpx + pdx;
-                    dpy = dpy + pdy;
-                    dpz = dpz + pdz;
-                    wsum = wsum + 1.0f;
-                }
-                // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
-                float bx = 0.0f, by = 0.0f, bz = 0.0f;
-                const float scn = fmaxd(Ci, 0.0f);
-                const uint32_t bm = a.bnd[i];
-                for (uint32_t k = 0; k < c.ncolliders; ++k) {
-                    if (!((bm >> k) & 1u)) continue;
-                    float gx, gy, gz;
-                    if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
-                    const float gd = len3(gx, gy, gz);
-                    if (gd <= kEps || gd >= h) continue;
-                    const float f = -scn * spiky_dw(gd, h);
-                    bx = bx + f * fdiv(gx, gd);
-                    by = by + f * fdiv(gy, gd);
-                    bz = bz + f * fdiv(gz, gd);
-                }
-                const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
-                ox = dpx * solve + bx * bdel;
-                oy = dpy * solve + by * bdel;
-                oz = dpz * solve + bz * bdel;
-                ow = wsum;
-            }
-            // apply
-            const float wgt = fmaxd(ow * c.cfg[kCfgFluidRelax], 1.0f);
-            const float ax = di.x + fdiv(ox, wgt), ay = di.y + fdiv(oy, wgt), az = di.z + fdiv(oz, wgt);
-            // clamp
-            const float r = fmaxd(pc.radius, 0.0f);
-            float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
-            const uint32_t bm = a.bnd[i];
-            for (int sweep = 0; sweep < 2; ++sweep) {
-                bool changed = false;
-                for (uint32_t k = 0; k < c.ncolliders; ++k) {
-                    if (!((bm >> k) & 1u)) continue;
-                    float sd, nx, ny, nz;
-                    if (!box_contact(a.col[k], cx, cy, cz, sd, nx, ny, nz)) continue;
-                    if (sd >= r) continue;
-                    const float push = r - sd;
-                    cx = cx + nx * push;
-                    cy = cy + ny * push;
-                    cz = cz + nz * push;
-                    changed = true;
-                }
-                if (!changed) break;
-            }
-            const float ex = cx - p.x, ey = cy - p.y, ez = cz - p.z;
-            a.dout[i] = make_float4(ex, ey, ez, 0.0f);
-            const float nx = p.x + ex, ny = p.y + ey, nz = p.z + ez;
-            a.cur[i] = make_float4(nx, ny, nz, p.w);
-            // displacement bound of the superset lists: |delta| plus a few ulps of the position
-            const float slack = fmaxd(fmaxd(fabsf(nx), fabsf(ny)), fabsf(nz)) * 4.76837158e-7f + 1.0e-30f;
-            const float dl = len3(ex, ey, ez) + slack;
-            d2 = dl * dl;
-        }
-    }
-    if (a.it < kIterStateWords) {
-        const uint32_t wm = wave_max(__float_as_uint(d2));
-        if ((threadIdx.x & 63u) == 0 && wm) {
-            atomicMax(max_disp(a, a.it), wm);
-            atomicMax(&a.meta[7], wm);
-        }
-    }
+    if (i < c.n) d2 = delta_global<UNIFORM>(i, a, c);
+    flush_disp(d2, a);
 }
 
+// Exact neighbor lists of iteration 0 (the per-pass entry): the first cap in-radius neighbors in
+// walk order, k-major, and their count.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_lists(FluidArgs a, SubstepConsts c)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.n) return;
+    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    const float4 s = a.cur[i];
+    uint32_t cnt = 0;
+    if (pc.kind == CN_PARTICLE_KIND_FLUID && s.w > kEps && c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))) {
+        const float t2 = a.t2[pc.fluid_mat];
+        walk_runs(a.cellc[i], a, c, [&](uint32_t q) {
+            if (q == i) return true;
+            if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
+            const float4 pj = a.cur[q];
+            if (pj.w <= kEps) return true;
+            const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
+            if (!(dot3(dx, dy, dz, dx, dy, dz) < t2)) return true;
+            a.nbr[(size_t)cnt * c.n + i] = q;
+            return ++cnt < c.cap;
+        });
+    }
+    a.nbr_cnt[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------------------------
 static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din)
 {
     FluidArgs a;
@@ -621,6 +785,8 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.nc = s.nc;
     a.nbr = s.nbr;
     a.nbr_cnt = s.nbr_cnt;
+    a.list = s.list;
+    a.lcnt = s.lcnt;
     a.run_key = s.run_key;
     a.run_start = s.run_start;
     a.run_info = s.run_info;
@@ -628,60 +794,88 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.bucket_shift = c.bucket_shift;
     a.blk_first_run = s.blk_first_run;
     a.nblk = s.nblk;
+    a.blk_halo = s.blk_halo;
     a.iter_state = s.iter_state;
     a.classes = s.classes;
     a.fm = s.fluid_mats;
+    a.t2 = s.fluid_t2;
     a.col = s.colliders;
     a.meta = s.meta;
     a.it = it;
+    a.tile = 0;
     return a;
 }
 
 static uint32_t grid_of(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
+// LDS budget of one tiled workgroup: the dynamic tile plus the static halo table stays under 64 KiB
+constexpr uint32_t kDynLdsBytes = 62u * 1024u;
+
+static bool tiled(const SubstepConsts &c, uint32_t it) { return c.key_mode == CN_KEY_MORTON && it < kIterStateWords; }
+
+// Halo capacity shared by the density and delta passes: a block uses the LDS path in both or in
+// neither (the delta pass reads the lists the density pass wrote in halo offsets).
+static uint32_t tile_of(const SubstepConsts &c)
+{
+    const uint32_t widest = 32u + (c.tension ? 16u : 0u) + (c.uniform_class ? 0u : 4u);
+    return std::min<uint32_t>(c.tile_max, kDynLdsBytes / widest);
+}
+
+// persistent grid: blocks in flight on all dies, each pulling cell blocks from the queues
+static uint32_t persistent_grid(uint32_t n) { return std::min<uint32_t>(2048u, std::max<uint32_t>(kXcds, (n + 63u) / 64u)); }
+
+void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, 0, 0);
+    const uint32_t g = std::min<uint32_t>(std::max<uint32_t>(c.blk_cap, 1u), 4096u);
+    hipLaunchKernelGGL(k_blk_halo, dim3(g), dim3(kBlock), 0, st, a, c);
+}
+
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
 {
-    launch_fluid_walk(s, c, it, din, st);
-    launch_fluid_filter(s, c, it, din, st);
-}
-
-void launch_fluid_filter(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
-{
-    const FluidArgs a = fluid_args(s, c, it, din);
-    if (it > 0) {
+    FluidArgs a = fluid_args(s, c, it, din);
+    if (tiled(c, it)) {
+        const uint32_t per = 16u + (c.uniform_class ? 0u : 4u);
+        a.tile = tile_of(c);
+        const size_t lds = (size_t)a.tile * per;
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_filter<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_filter<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
-    }
-}
-
-void launch_fluid_walk(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
-{
-    const FluidArgs a = fluid_args(s, c, it, din);
-    const bool tiled = c.key_mode == CN_KEY_MORTON && it + 1 < kIterStateWords;
-    if (tiled) {
-        // persistent grid: enough workgroups to fill the chip, each pulling Morton blocks
-        const uint32_t g = std::min<uint32_t>(grid_of(c.n), 256u * 6u);
-        if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_walk_tiled<true>, dim3(g), dim3(kBlock), 0, st, a, c);
-        else
-            hipLaunchKernelGGL(k_fluid_walk_tiled<false>, dim3(g), dim3(kBlock), 0, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
     } else {
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_walk_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_walk_global<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_global<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
     }
 }
 
 void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
 {
-    const FluidArgs a = fluid_args(s, c, it, din);
+    FluidArgs a = fluid_args(s, c, it, din);
+    if (tiled(c, it)) {
+        const uint32_t per = 16u + (it > 0 ? 16u : 0u) + (c.tension ? 16u : 0u) + (c.uniform_class ? 0u : 4u);
+        a.tile = tile_of(c);
+        const size_t lds = (size_t)a.tile * per;
+        if (c.uniform_class)
+            hipLaunchKernelGGL(k_fluid_delta_tiled<true>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+        else
+            hipLaunchKernelGGL(k_fluid_delta_tiled<false>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+    } else {
+        if (c.uniform_class)
+            hipLaunchKernelGGL(k_fluid_delta_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+        else
+            hipLaunchKernelGGL(k_fluid_delta_global<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+    }
+}
+
+void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, 0, 0);
     if (c.uniform_class)
-        hipLaunchKernelGGL(k_fluid_delta<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+        hipLaunchKernelGGL(k_fluid_lists<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
     else
-        hipLaunchKernelGGL(k_fluid_delta<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+        hipLaunchKernelGGL(k_fluid_lists<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
 }
 
 }  // namespace cn

@@ -8,7 +8,25 @@
 
 namespace cn {
 
-constexpr uint32_t kIterStateWords = 64;  // iterations tracked per substep (more fall back to walks)
+constexpr uint32_t kIterStateWords = 64;  // iterations with tiled fluid passes (later ones walk the run table)
+constexpr uint32_t kXcds = 8;             // accelerator dies; each has its own L2 and block-queue counter
+// iter_state words: block-queue counters of the density and delta passes (kXcds per iteration),
+// then the largest |delta|^2 bits of each iteration
+constexpr uint32_t kStateDensity = 0;
+constexpr uint32_t kStateDelta = kIterStateWords * kXcds;
+constexpr uint32_t kStateMaxDisp = 2 * kIterStateWords * kXcds;
+constexpr uint32_t kStateWords = kStateMaxDisp + kIterStateWords;
+// per-block halo descriptor: start slot and exclusive particle offset of each of the 6x6x6 halo
+// cells (x fastest), the halo particle total, a purity flag, the block's particle range and the
+// cell coordinates of its low corner
+constexpr uint32_t kHalo = 216;
+constexpr uint32_t kHdStart = 0;
+constexpr uint32_t kHdOff = kHalo;              // kHalo + 1 offsets, the last one the total
+constexpr uint32_t kHdFlags = 2 * kHalo + 1;
+constexpr uint32_t kHdP0 = kHdFlags + 1;
+constexpr uint32_t kHdP1 = kHdFlags + 2;
+constexpr uint32_t kHdBase = kHdFlags + 3;      // 3 words
+constexpr uint32_t kHaloStride = 448;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -46,9 +64,15 @@ struct SceneBuffers {
     float4 *nc = nullptr;           // surface normal * tension (only with surface tension)
     uint32_t *blk_first_run = nullptr;  // first run of each Morton 4x4x4 block, then the run count
     uint32_t *nblk = nullptr;
-    uint32_t *iter_state = nullptr; // [work counters | max |delta|^2 bits | fallback flags] per iteration
-    uint32_t *nbr = nullptr;        // neighbor slots, k-major: nbr[k * n + i]
+    uint32_t *blk_halo = nullptr;   // kHaloStride words per block, for the first blk_cap blocks
+    uint32_t blk_cap = 0;
+    uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
+    float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
+    uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, k-major: list[k * n + i]
+    uint32_t *lcnt = nullptr;       // fluid neighbor count per slot
+    uint32_t *nbr = nullptr;        // neighbor lists of the pass entry, k-major: nbr[k * n + i]
     uint32_t *nbr_cnt = nullptr;
+    size_t nbr_alloc = 0;
     uint32_t cap = 0;
     // index-order exports
     float4 *out_pos = nullptr;
@@ -62,7 +86,7 @@ struct SceneBuffers {
     Collider *colliders = nullptr;
     uint32_t ncolliders = 0;
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
-    // [4] iterations that had to walk the cells again
+    // [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
     uint32_t *meta = nullptr;
     // soft constraints
     uint32_t *slot_of = nullptr;
@@ -82,10 +106,10 @@ void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, uint32_t key_bits, hipStream_t st);
 void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
-void launch_fluid_walk(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
-void launch_fluid_filter(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
+void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
 

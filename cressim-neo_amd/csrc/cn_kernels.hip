@@ -358,7 +358,7 @@ void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
     hipLaunchKernelGGL(k_blk_count, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs);
     hipLaunchKernelGGL(k_scan_block_runs, dim3(1), dim3(1024), 0, st, s.block_runs, nb, s.nblk);
     hipLaunchKernelGGL(k_blk_emit, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs,
-                       s.blk_first_run, s.nblk, s.iter_state, kIterStateWords);
+                       s.blk_first_run, s.nblk, s.iter_state, kStateMaxDisp);
 }
 
 // ---------------------------------------------------------------------------------------------

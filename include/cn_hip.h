@@ -183,13 +183,13 @@ typedef struct cn_meta {
     uint32_t max_fluid_neighbor_count;    /* largest required (untruncated) in-radius count */
     uint32_t fluid_neighbor_truncations;  /* particles whose list hit max_fluid_neighbors */
     uint32_t max_cell_occupancy;          /* longest run of equal keys in the last sort */
-    uint32_t verlet_rebuilds;             /* reserved */
+    uint32_t reserved0;
     uint64_t fluid_neighbor_total;        /* listed neighbors at iteration 0, summed over the
                                              substeps of the last cn_step (mean K = total /
                                              (particles * substeps)) */
     uint32_t last_particle_count;
-    uint32_t lds_tiles;                   /* neighbor-walk tiles staged in LDS */
-    uint32_t run_table_tiles;             /* neighbor-walk tiles walked through the run table */
+    uint32_t lds_tiles;                   /* density-pass cell blocks whose halo was staged in LDS */
+    uint32_t run_table_tiles;             /* density-pass cell blocks walked through the run table */
     uint32_t max_displacement2_bits;      /* float bits of the largest (|delta| + slack)^2 */
     uint32_t reserved[2];
 } cn_meta;
