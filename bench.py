@@ -55,6 +55,19 @@ def cpu_baseline(n_sample: int, seed: int = 2):
                       f"density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
 
 
+def copy_probe(mib: int):
+    """A plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters give the
+    factor between those counters and bytes moved."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +78,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design in DESIGN.md, not a measured path.", file=sys.stderr)

@@ -140,20 +140,6 @@ __device__ __forceinline__ void walk_rows(const uint32_t *hoff, int hc, F &&f)
         }
 }
 
-// f(list entry) for the cnt entries of one particle's k-major list, index loads issued four ahead.
-template <class F>
-__device__ __forceinline__ void for_list(const uint16_t *li, uint32_t cnt, uint32_t n, F &&f)
-{
-    for (uint32_t k = 0; k < cnt; k += 4) {
-        uint32_t e[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = k + u < cnt ? li[(size_t)(k + u) * n] : 0u;
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (k + u < cnt) f(e[u]);
-    }
-}
-
 // Halo cell holding staged particle f: the last cell whose offset is <= f.
 __device__ __forceinline__ uint32_t halo_cell_of(const uint32_t *hoff, uint32_t f)
 {
@@ -257,23 +243,34 @@ __device__ __forceinline__ bool density_test(const DensitySelf &x, const float4 
 }
 
 // Density and surface-normal terms of one listed neighbor.
-__device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x, const float4 pj)
+struct DensityTerm {
+    float w, gx, gy, gz;
+};
+
+__device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const float4 pj)
 {
+    DensityTerm t;
     const float dx = x.s.x - pj.x, dy = x.s.y - pj.y, dz = x.s.z - pj.z;
     const float dist = fsqrt(dot3(dx, dy, dz, dx, dy, dz));
-    d.density = d.density + spiky_w_k(dist, x.h, x.kw);
-    if (x.tension) {
+    t.w = spiky_w_k(dist, x.h, x.kw);
+    t.gx = t.gy = t.gz = 0.0f;
+    if (x.tension && !(dist <= kEps || dist >= x.h)) {
         // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53
-        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        if (!(dist <= kEps || dist >= x.h)) {
-            const float dw = spiky_dw_k(dist, x.h, x.kw);
-            gx = dw * fdiv(dx, dist);
-            gy = dw * fdiv(dy, dist);
-            gz = dw * fdiv(dz, dist);
-        }
-        d.snx = d.snx + gx;
-        d.sny = d.sny + gy;
-        d.snz = d.snz + gz;
+        const float dw = spiky_dw_k(dist, x.h, x.kw);
+        t.gx = dw * fdiv(dx, dist);
+        t.gy = dw * fdiv(dy, dist);
+        t.gz = dw * fdiv(dz, dist);
+    }
+    return t;
+}
+
+__device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x, const DensityTerm &t)
+{
+    d.density = d.density + t.w;
+    if (x.tension) {
+        d.snx = d.snx + t.gx;
+        d.sny = d.sny + t.gy;
+        d.snz = d.snz + t.gz;
     }
 }
 
@@ -320,7 +317,7 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
             ++d.req;
             if (d.inr < c.cap) {
                 ++d.inr;
-                density_add(d, x, pj);
+                density_add(d, x, density_term(x, pj));
             }
             return true;
         });
@@ -330,15 +327,93 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
 // ---------------------------------------------------------------------------------------------
 // delta pass: physics_fluid_compute_delta_positions.cs.hlsl:27-157, then
 // physics_fluid_apply_delta_positions.cs.hlsl:11-31 and physics_fluid_clamp_boundary.cs.hlsl:23-80
-// on the same particle.  Returns (|delta| + slack)^2 of the particle.
-// walk(v) calls v(k) for every neighbor record k in list order (LISTED: the walk yields exactly the
-// listed neighbors; otherwise it yields every candidate but the particle's own record, and the list
-// filter is applied here); rcs / rdin / rnc / rcls address the records (LDS or global).
+// on the same particle.
 // ---------------------------------------------------------------------------------------------
-template <bool UNIFORM, bool LISTED, class Walk>
-__device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Walk &&walk, const float4 *rcs,
-                                                const float4 *rdin, const float4 *rnc, const uint32_t *rcls,
-                                                const FluidArgs &a, const SubstepConsts &c)
+struct DeltaSelf {
+    ParticleClass pc;
+    cn_fluid_material m;
+    float4 di, nci;
+    float sx, sy, sz, Ci, h, t2, kw, cohesion, viscosity, cfl, dt;
+    bool tension, visc_on, cfl_on, need_rel;
+};
+
+struct DeltaTerm {
+    float x, y, z;
+    bool live;  // the neighbor is not coincident (dist > eps): it adds a term and a weight
+};
+
+// One listed neighbor's term (physics_fluid_compute_delta_positions.cs.hlsl:60-140).
+template <bool UNIFORM>
+__device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4 cq, const float4 dq, const float4 ncq,
+                                                uint32_t qcls, const FluidArgs &a)
+{
+    DeltaTerm t;
+    const float ddx = x.sx - cq.x, ddy = x.sy - cq.y, ddz = x.sz - cq.z;
+    const float dist = fsqrt(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
+    t.live = !(dist <= kEps);
+    const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
+    const float der = spiky_dw_k(dist, x.h, x.kw);
+    const float sc = -0.5f * (x.Ci + cq.w) * der;
+    float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
+    const bool same = UNIFORM || x.pc.fluid_mat == a.classes[qcls].fluid_mat;
+    if (same) {
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+        if (x.need_rel) {
+            rx = x.di.x - dq.x;
+            ry = x.di.y - dq.y;
+            rz = x.di.z - dq.z;
+        }
+        if (x.cohesion > kEps) {
+            const float ck = x.cohesion * cohesion_kernel(dist, x.h, x.m.cohesion1, x.m.cohesion2);
+            pdx = pdx - nx * ck;
+            pdy = pdy - ny * ck;
+            pdz = pdz - nz * ck;
+        }
+        if (x.visc_on) {
+            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_k(dist, x.h, x.kw));
+            pdx = pdx - vs * rx;
+            pdy = pdy - vs * ry;
+            pdz = pdz - vs * rz;
+        }
+        if (x.cfl_on) {
+            const float proj = dot3(rx, ry, rz, nx, ny, nz);
+            if (proj < -x.cfl) {
+                const float pcf = proj + x.cfl;
+                pdx = pdx - nx * pcf * 0.5f;
+                pdy = pdy - ny * pcf * 0.5f;
+                pdz = pdz - nz * pcf * 0.5f;
+            }
+        }
+        if (x.tension) {
+            pdx = pdx - x.dt * (x.nci.x - ncq.x);
+            pdy = pdy - x.dt * (x.nci.y - ncq.y);
+            pdz = pdz - x.dt * (x.nci.z - ncq.z);
+        }
+    }
+    t.x = pdx;
+    t.y = pdy;
+    t.z = pdz;
+    return t;
+}
+
+struct DeltaSum {
+    float x = 0.0f, y = 0.0f, z = 0.0f, w = 0.0f;
+};
+
+__device__ __forceinline__ void delta_add(DeltaSum &d, const DeltaTerm &t)
+{
+    if (!t.live) return;
+    d.x = d.x + t.x;
+    d.y = d.y + t.y;
+    d.z = d.z + t.z;
+    d.w = d.w + 1.0f;
+}
+
+// sum(x, d) adds the neighbor terms in list order.  own addresses the particle's record in
+// rcs / rnc (LDS offset or slot).  Returns (|delta| + slack)^2 of the particle.
+template <bool UNIFORM, class Sum>
+__device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&sum, const float4 *rcs,
+                                                const float4 *rnc, const FluidArgs &a, const SubstepConsts &c)
 {
     const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
     if (pc.kind != CN_PARTICLE_KIND_FLUID) {
@@ -350,110 +425,56 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Walk &
     const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
     if (p.w > kEps) {
+        DeltaSelf x;
+        x.pc = pc;
+        x.di = di;
         const float4 csi = rcs[own];
-        const float sx = csi.x, sy = csi.y, sz = csi.z, Ci = csi.w;
-        const cn_fluid_material m = a.fm[pc.fluid_mat];
-        const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
-        const float t2 = a.t2[pc.fluid_mat];
-        const float kw = spiky_coef(h);
-        const float cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
-        const float viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
-        const float st = fmaxd(m.surface_tension, 0.0f);
-        const float cfl = fmaxd(m.cfl_radius, 0.0f);
-        const bool tension = st > kEps;
+        x.sx = csi.x;
+        x.sy = csi.y;
+        x.sz = csi.z;
+        x.Ci = csi.w;
+        x.m = a.fm[pc.fluid_mat];
+        x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
+        x.t2 = a.t2[pc.fluid_mat];
+        x.kw = spiky_coef(x.h);
+        x.dt = c.dt;
+        x.cohesion = c.dt * fmaxd(x.m.cohesion, 0.0f);
+        x.viscosity = c.dt * fmaxd(x.m.viscosity, 0.0f);
+        x.cfl = fmaxd(x.m.cfl_radius, 0.0f);
+        x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
         // relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can only
         // act when |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl
-        bool cfl_on = false;
-        if (use_d && cfl > kEps) {
+        x.cfl_on = false;
+        if (use_d && x.cfl > kEps) {
             const float md2 = a.it - 1 < kIterStateWords ? __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1])
                                                          : 3.4e38f;
-            cfl_on = !(4.0f * md2 * c.cfl_guard < cfl * cfl);
+            x.cfl_on = !(4.0f * md2 * c.cfl_guard < x.cfl * x.cfl);
         }
-        const bool visc_on = use_d && viscosity > kEps;
-        const bool need_rel = cfl_on || visc_on;
-        const float4 nci = tension ? rnc[own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f, wsum = 0.0f;
-        uint32_t inr = 0;
-        if (LISTED || (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))))
-            walk([&](uint32_t k) {
-                if (!LISTED && !UNIFORM && !neighbor_ok(pc, a.classes[rcls[k]])) return true;
-                const float4 cq = rcs[k];
-                const float ddx = sx - cq.x, ddy = sy - cq.y, ddz = sz - cq.z;
-                const float d2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
-                if (!LISTED) {
-                    if (!(d2 < t2)) return true;
-                    if (inr >= c.cap) return false;
-                    ++inr;
-                }
-                const float dist = fsqrt(d2);
-                if (dist <= kEps) return true;
-                const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
-                const float der = spiky_dw_k(dist, h, kw);
-                const float sc = -0.5f * (Ci + cq.w) * der;
-                float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
-                const bool same = UNIFORM || pc.fluid_mat == a.classes[rcls[k]].fluid_mat;
-                if (same) {
-                    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-                    if (need_rel) {
-                        const float4 dq = rdin[k];
-                        rx = di.x - dq.x;
-                        ry = di.y - dq.y;
-                        rz = di.z - dq.z;
-                    }
-                    if (cohesion > kEps) {
-                        const float ck = cohesion * cohesion_kernel(dist, h, m.cohesion1, m.cohesion2);
-                        pdx = pdx - nx * ck;
-                        pdy = pdy - ny * ck;
-                        pdz = pdz - nz * ck;
-                    }
-                    if (visc_on) {
-                        const float vs = 1.0f - fdiv(1.0f, 1.0f + viscosity * spiky_w_k(dist, h, kw));
-                        pdx = pdx - vs * rx;
-                        pdy = pdy - vs * ry;
-                        pdz = pdz - vs * rz;
-                    }
-                    if (cfl_on) {
-                        const float proj = dot3(rx, ry, rz, nx, ny, nz);
-                        if (proj < -cfl) {
-                            const float pcf = proj + cfl;
-                            pdx = pdx - nx * pcf * 0.5f;
-                            pdy = pdy - ny * pcf * 0.5f;
-                            pdz = pdz - nz * pcf * 0.5f;
-                        }
-                    }
-                    if (tension) {
-                        const float4 ncq = rnc[k];
-                        pdx = pdx - c.dt * (nci.x - ncq.x);
-                        pdy = pdy - c.dt * (nci.y - ncq.y);
-                        pdz = pdz - c.dt * (nci.z - ncq.z);
-                    }
-                }
-                dpx = dpx + pdx;
-                dpy = dpy + pdy;
-                dpz = dpz + pdz;
-                wsum = wsum + 1.0f;
-                return true;
-            });
+        x.visc_on = use_d && x.viscosity > kEps;
+        x.need_rel = x.cfl_on || x.visc_on;
+        x.nci = x.tension ? rnc[own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        DeltaSum d;
+        if (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))) sum(x, d);
         // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
         float bx = 0.0f, by = 0.0f, bz = 0.0f;
-        const float scn = fmaxd(Ci, 0.0f);
+        const float scn = fmaxd(x.Ci, 0.0f);
         const uint32_t bm = a.bnd[i];
         for (uint32_t k = 0; k < c.ncolliders; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float gx, gy, gz;
-            if (!ghost_delta(a.col[k], sx, sy, sz, h, gx, gy, gz)) continue;
+            if (!ghost_delta(a.col[k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) continue;
             const float gd = len3(gx, gy, gz);
-            if (gd <= kEps || gd >= h) continue;
-            const float f = -scn * spiky_dw_k(gd, h, kw);
+            if (gd <= kEps || gd >= x.h) continue;
+            const float f = -scn * spiky_dw_k(gd, x.h, x.kw);
             bx = bx + f * fdiv(gx, gd);
             by = by + f * fdiv(gy, gd);
             bz = bz + f * fdiv(gz, gd);
         }
         const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
-        ox = dpx * solve + bx * bdel;
-        oy = dpy * solve + by * bdel;
-        oz = dpz * solve + bz * bdel;
-        ow = wsum;
+        ox = d.x * solve + bx * bdel;
+        oy = d.y * solve + by * bdel;
+        oz = d.z * solve + bz * bdel;
+        ow = d.w;
     }
     // apply
     const float wgt = fmaxd(ow * c.cfg[kCfgFluidRelax], 1.0f);
@@ -487,13 +508,31 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Walk &
     return dl * dl;
 }
 
+// Run-table walk: the list filter and the cap applied in the walk.
 template <bool UNIFORM>
 __device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, const SubstepConsts &c)
 {
     const int4 ci = a.cellc[i];
-    return delta_particle<UNIFORM, false>(
-        i, i, [&](auto &&v) { walk_runs(ci, a, c, [&](uint32_t q) { return q == i || v(q); }); }, a.cs, a.din, a.nc,
-        a.cls, a, c);
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return delta_particle<UNIFORM>(
+        i, i,
+        [&](const DeltaSelf &x, DeltaSum &d) {
+            uint32_t inr = 0;
+            walk_runs(ci, a, c, [&](uint32_t q) {
+                if (q == i) return true;
+                const uint32_t qc = UNIFORM ? 0u : a.cls[q];
+                if (!UNIFORM && !neighbor_ok(x.pc, a.classes[qc])) return true;
+                const float4 cq = a.cs[q];
+                const float ddx = x.sx - cq.x, ddy = x.sy - cq.y, ddz = x.sz - cq.z;
+                if (!(dot3(ddx, ddy, ddz, ddx, ddy, ddz) < x.t2)) return true;
+                if (inr >= c.cap) return false;
+                ++inr;
+                delta_add(d, delta_term<UNIFORM>(x, cq, x.need_rel ? a.din[q] : zero, x.tension ? a.nc[q] : zero, qc,
+                                                 a));
+                return true;
+            });
+        },
+        a.cs, a.nc, a, c);
 }
 
 __device__ __forceinline__ void flush_disp(float d2, const FluidArgs &a)
@@ -562,6 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
     }
 }
 
+constexpr int kTileBlock = 512;  // threads of a tiled fluid workgroup: one pass over a block's particles
+
 // Common head of the tiled passes: claim a block, decide LDS or run table, load the halo table.
 struct TileState {
     uint32_t blk;
@@ -587,7 +628,7 @@ __device__ __forceinline__ bool tile_begin(BlockQueue &bq, uint32_t *queue, uint
         ts.p1 = hd[kHdP1];
         ts.lds = hd[kHdFlags] != 0u && hd[kHdOff + kHalo] <= a.tile;
         if (ts.lds) {
-            for (uint32_t t = tid; t < kHalo; t += kBlock) {
+            for (uint32_t t = tid; t < kHalo; t += blockDim.x) {
                 hstart[t] = hd[kHdStart + t];
                 hoff[t] = hd[kHdOff + t];
             }
@@ -610,10 +651,71 @@ __device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
 }
 
 // ---------------------------------------------------------------------------------------------
+// LDS-path loops
+// ---------------------------------------------------------------------------------------------
+
+// The density walk over the staged halo: tests every candidate in walk order and compacts the
+// listed ones (their halo offsets) into the particle's k-major list, four candidates per step with
+// their LDS loads issued first.  Returns the untruncated neighbor count.  Staged particles with
+// zero inverse mass carry an unreachable position, so the distance test also rejects them.
+template <bool UNIFORM>
+__device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
+                                              const uint32_t *lcls, const DensitySelf &x, uint16_t *li,
+                                              const FluidArgs &a, const SubstepConsts &c)
+{
+    uint32_t req = 0;
+    const size_t n = c.n;
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int h0 = hc - 1 + 6 * (r % 3 - 1) + 36 * (r / 3 - 1);
+        const uint32_t f1 = hoff[h0 + 3];
+        for (uint32_t k = hoff[h0]; k < f1; k += 4) {
+            float4 pj[4];
+            uint32_t qc[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t f = min(k + u, f1 - 1);
+                pj[u] = lpos[f];
+                qc[u] = UNIFORM ? 0u : lcls[f];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t f = k + u;
+                const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
+                bool ok = (f < f1) & (f != fs) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2);
+                if (!UNIFORM) ok = ok && neighbor_ok(x.pc, a.classes[qc[u]]);
+                if (ok) {
+                    if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
+                    ++req;
+                }
+            }
+        }
+    }
+    return req;
+}
+
+// Density sums over a compacted list, four entries per step (loads first, sums in list order).
+__device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, size_t n, const float4 *lpos,
+                                             const DensitySelf &x, DensitySum &d)
+{
+    for (uint32_t k = 0; k < cnt; k += 4) {
+        uint32_t e[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) e[u] = li[(size_t)min(k + u, cnt - 1) * n];
+        DensityTerm t[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) t[u] = density_term(x, lpos[e[u]]);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (k + u < cnt) density_add(d, x, t[u]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
@@ -631,14 +733,16 @@ __global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, Sub
     while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
-            for (uint32_t f = tid; f < total; f += kBlock) {
+            for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
-                lpos[f] = a.cur[q];
+                float4 v = a.cur[q];
+                if (!(v.w > kEps)) v.x = v.y = v.z = kFarAway;
+                lpos[f] = v;
                 if (!UNIFORM) lcls[f] = a.cls[q];
             }
             __syncthreads();
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) {
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
@@ -647,17 +751,9 @@ __global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, Sub
                     DensitySum d;
                     uint16_t *li = a.list + i;
                     if (density_walks(x, UNIFORM, c)) {
-                        // the walk only compacts the listed neighbors' halo offsets (a cheap loop
-                        // body for every candidate); the sums then run over the list
-                        walk_rows(hoff, hc, [&](uint32_t f) {
-                            if (f != fs && density_test<UNIFORM>(x, lpos[f], UNIFORM ? 0u : lcls[f], a)) {
-                                if (d.req < c.cap) li[(size_t)d.req * c.n] = (uint16_t)f;
-                                ++d.req;
-                            }
-                            return true;
-                        });
+                        d.req = scan_rows<UNIFORM>(hoff, hc, fs, lpos, lcls, x, li, a, c);
                         d.inr = min(d.req, c.cap);
-                        for_list(li, d.inr, c.n, [&](uint32_t f) { density_add(d, x, lpos[f]); });
+                        density_list(li, d.inr, c.n, lpos, x, d);
                     }
                     cnt = d.inr;
                     density_end(i, d, x, ma, a, c);
@@ -666,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, Sub
             }
             if (tid == 0) ++ma.lds;
         } else {
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) density_global<UNIFORM>(i, ma, a, c);
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) density_global<UNIFORM>(i, ma, a, c);
             if (tid == 0) ++ma.runs;
         }
         __syncthreads();
@@ -675,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_fluid_density_tiled(FluidArgs a, Sub
 }
 
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void k_fluid_delta_tiled(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
@@ -692,11 +788,12 @@ __global__ __launch_bounds__(kBlock) void k_fluid_delta_tiled(FluidArgs a, Subst
     uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds;
     BlockQueue bq(blockIdx.x);
     TileState ts;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float d2max = 0.0f;
     while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
-            for (uint32_t f = tid; f < total; f += kBlock) {
+            for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 lcs[f] = a.cs[q];
@@ -705,16 +802,36 @@ __global__ __launch_bounds__(kBlock) void k_fluid_delta_tiled(FluidArgs a, Subst
                 if (!UNIFORM) lcls[f] = a.cls[q];
             }
             __syncthreads();
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) {
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 const uint16_t *li = a.list + i;
-                const float d2 = delta_particle<UNIFORM, true>(
-                    i, fs, [&](auto &&v) { for_list(li, a.lcnt[i], c.n, v); }, lcs, ldin, lnc, lcls, a, c);
+                const float d2 = delta_particle<UNIFORM>(
+                    i, fs,
+                    [&](const DeltaSelf &x, DeltaSum &d) {
+                        // the listed neighbors, four per step: loads first, terms independent,
+                        // sums in list order
+                        const uint32_t cnt = a.lcnt[i];
+                        for (uint32_t k = 0; k < cnt; k += 4) {
+                            uint32_t e[4];
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; ++u) e[u] = li[(size_t)min(k + u, cnt - 1) * c.n];
+                            DeltaTerm t[4];
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; ++u)
+                                t[u] = delta_term<UNIFORM>(x, lcs[e[u]], x.need_rel ? ldin[e[u]] : zero,
+                                                           x.tension ? lnc[e[u]] : zero, UNIFORM ? 0u : lcls[e[u]], a);
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; ++u)
+                                if (k + u < cnt) delta_add(d, t[u]);
+                        }
+                    },
+                    lcs, lnc, a, c);
                 d2max = fmaxd(d2max, d2);
             }
         } else {
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kBlock) d2max = fmaxd(d2max, delta_global<UNIFORM>(i, a, c));
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock)
+                d2max = fmaxd(d2max, delta_global<UNIFORM>(i, a, c));
         }
         __syncthreads();
     }
@@ -822,7 +939,7 @@ static uint32_t tile_of(const SubstepConsts &c)
 }
 
 // persistent grid: blocks in flight on all dies, each pulling cell blocks from the queues
-static uint32_t persistent_grid(uint32_t n) { return std::min<uint32_t>(2048u, std::max<uint32_t>(kXcds, (n + 63u) / 64u)); }
+static uint32_t persistent_grid(uint32_t n) { return std::min<uint32_t>(1024u, std::max<uint32_t>(kXcds, (n + 127u) / 128u)); }
 
 void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
@@ -839,9 +956,9 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
         a.tile = tile_of(c);
         const size_t lds = (size_t)a.tile * per;
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
     } else {
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_density_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
@@ -858,9 +975,9 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
         a.tile = tile_of(c);
         const size_t lds = (size_t)a.tile * per;
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_delta_tiled<true>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_delta_tiled<true>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_delta_tiled<false>, dim3(persistent_grid(c.n)), dim3(kBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_delta_tiled<false>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
     } else {
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_delta_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
