@@ -34,7 +34,7 @@ struct cn_ctx {
     float max_h = 0.0f;              // largest fluid smoothing radius of the scene
     bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
     bool has_tension = false;        // any fluid material with surface tension
-    uint32_t tile_max = 2048;        // halo particles a fluid-pass workgroup may stage in LDS
+    uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
     // profiling
     bool profiling = false;
     struct Ev {
@@ -108,6 +108,7 @@ void free_scene(SceneBuffers &s)
     dfree(s.blk_first_run);
     dfree(s.nblk);
     dfree(s.blk_halo);
+    dfree(s.wide_blk);
     dfree(s.iter_state);
     dfree(s.fluid_t2);
     dfree(s.list);
@@ -317,6 +318,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.tension = ctx->has_tension ? 1u : 0u;
     c.tile_max = ctx->tile_max;
     c.blk_cap = ctx->s.blk_cap;
+    fluid_tiles(c);
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
     return c;
@@ -537,6 +539,7 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     // their blocks through the run table
     s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
     CN_HIP(ctx, dalloc(&s.blk_halo, (size_t)s.blk_cap * kHaloStride));
+    CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
     CN_HIP(ctx, dalloc(&s.list, N * (size_t)(cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));

@@ -68,7 +68,10 @@ struct SubstepConsts {
     uint32_t bucket_shift;
     uint32_t uniform_class;  // 1: every particle has class 0
     uint32_t tension;        // 1: some fluid material has surface tension
-    uint32_t tile_max;       // halo particles a fluid-pass workgroup may stage in LDS
+    uint32_t tile_max;       // halo particles a fluid-pass workgroup may stage in LDS (upper bound)
+    uint32_t tile_dens;      // halo capacity of the density pass
+    uint32_t tile_delta;     // halo capacity of the delta pass from the second iteration on
+    uint32_t tile_wide;      // halo capacity of the wide delta pass (previous delta read from HBM)
     uint32_t blk_cap;        // cell blocks with a stored halo descriptor
     float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];

@@ -55,6 +55,7 @@ struct FluidArgs {
     const uint32_t *blk_first_run;
     const uint32_t *nblk;
     uint32_t *blk_halo;
+    uint32_t *wide_blk;
     uint32_t *iter_state;
     const ParticleClass *classes;
     const cn_fluid_material *fm;
@@ -590,6 +591,9 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
             hd[kHdOff + tid] = off;
         }
         if (tid == 0) {
+            // blocks the delta pass stages only in the wide tile get their own launch
+            if (!any_bad && total > c.tile_delta && total <= c.tile_wide)
+                a.wide_blk[atomicAdd(a.iter_state + kStateWideCount, 1u)] = b;
             hd[kHdOff + kHalo] = total;
             hd[kHdFlags] = any_bad ? 0u : 1u;
             hd[kHdP0] = p0;
@@ -603,30 +607,37 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
 
 constexpr int kTileBlock = 512;  // threads of a tiled fluid workgroup: one pass over a block's particles
 
-// Common head of the tiled passes: claim a block, decide LDS or run table, load the halo table.
+// Common head of the tiled passes: claim a block (next() runs on thread 0), decide LDS or run
+// table, load the halo table.  skip_to is a halo size above the LDS capacity up to which a block is
+// left to the wide launch (0: none).
 struct TileState {
     uint32_t blk;
     uint32_t p0, p1;
-    bool lds;
+    bool lds, skip;
     int bx, by, bz;
 };
 
-__device__ __forceinline__ bool tile_begin(BlockQueue &bq, uint32_t *queue, uint32_t nblk, uint32_t *s_blk,
-                                           uint32_t *hoff, uint32_t *hstart, TileState &ts, const FluidArgs &a,
+template <class Next>
+__device__ __forceinline__ bool tile_begin(Next &&next, uint32_t *s_blk, uint32_t *hoff, uint32_t *hstart,
+                                           TileState &ts, uint32_t skip_to, const FluidArgs &a,
                                            const SubstepConsts &c)
 {
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) *s_blk = bq.next(queue, nblk);
+    if (tid == 0) *s_blk = next();
     __syncthreads();
     const uint32_t b = *s_blk;
     if (b == kInvalid) return false;
     ts.blk = b;
     ts.lds = false;
+    ts.skip = false;
     if (b < c.blk_cap) {
         const uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
         ts.p0 = hd[kHdP0];
         ts.p1 = hd[kHdP1];
-        ts.lds = hd[kHdFlags] != 0u && hd[kHdOff + kHalo] <= a.tile;
+        const uint32_t total = hd[kHdOff + kHalo];
+        const bool pure = hd[kHdFlags] != 0u;
+        ts.lds = pure && total <= a.tile;
+        ts.skip = pure && !ts.lds && total <= skip_to;
         if (ts.lds) {
             for (uint32_t t = tid; t < kHalo; t += blockDim.x) {
                 hstart[t] = hd[kHdStart + t];
@@ -730,7 +741,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     BlockQueue bq(blockIdx.x);
     MetaAcc ma;
     TileState ts;
-    while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
+    while (tile_begin([&] { return bq.next(queue, nblk); }, &s_blk, hoff, hstart, ts, 0u, a, c)) {
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             for (uint32_t f = tid; f < total; f += kTileBlock) {
@@ -770,7 +781,10 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     flush_meta(ma, a);
 }
 
-template <bool UNIFORM>
+// WIDE = false: every block, the previous delta staged in LDS beside the records; blocks whose halo
+// only fits without it are skipped and left to the WIDE = true launch, which takes the listed wide
+// blocks and reads the previous delta from HBM through a staged slot map.
+template <bool UNIFORM, bool WIDE>
 __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
@@ -781,23 +795,36 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
     const bool tension = c.tension != 0u;
     float4 *lcs = dyn;
     float4 *ldin = lcs + a.tile;
-    float4 *lnc = ldin + (use_d ? a.tile : 0u);
+    float4 *lnc = ldin + (use_d && !WIDE ? a.tile : 0u);
     uint32_t *lcls = reinterpret_cast<uint32_t *>(lnc + (tension ? a.tile : 0u));
+    uint32_t *lslot = lcls + (UNIFORM ? 0u : a.tile);
     const uint32_t tid = threadIdx.x;
     const uint32_t nblk = *a.nblk;
     uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds;
+    uint32_t *wqueue = a.iter_state + kStateWideQueue + a.it;
+    const uint32_t nwide = WIDE ? a.iter_state[kStateWideCount] : 0u;
     BlockQueue bq(blockIdx.x);
     TileState ts;
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float d2max = 0.0f;
-    while (tile_begin(bq, queue, nblk, &s_blk, hoff, hstart, ts, a, c)) {
+    auto next = [&]() -> uint32_t {
+        if (!WIDE) return bq.next(queue, nblk);
+        const uint32_t t = atomicAdd(wqueue, 1u);
+        return t < nwide ? a.wide_blk[t] : kInvalid;
+    };
+    while (tile_begin(next, &s_blk, hoff, hstart, ts, WIDE || !use_d ? 0u : c.tile_wide, a, c)) {
+        if (ts.skip) {
+            __syncthreads();
+            continue;
+        }
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 lcs[f] = a.cs[q];
-                if (use_d) ldin[f] = a.din[q];
+                if (!WIDE && use_d) ldin[f] = a.din[q];
+                if (WIDE) lslot[f] = q;
                 if (tension) lnc[f] = a.nc[q];
                 if (!UNIFORM) lcls[f] = a.cls[q];
             }
@@ -816,11 +843,15 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                             uint32_t e[4];
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u) e[u] = li[(size_t)min(k + u, cnt - 1) * c.n];
+                            float4 dq[4];
+#pragma unroll
+                            for (uint32_t u = 0; u < 4; ++u)
+                                dq[u] = !x.need_rel ? zero : (WIDE ? a.din[lslot[e[u]]] : ldin[e[u]]);
                             DeltaTerm t[4];
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u)
-                                t[u] = delta_term<UNIFORM>(x, lcs[e[u]], x.need_rel ? ldin[e[u]] : zero,
-                                                           x.tension ? lnc[e[u]] : zero, UNIFORM ? 0u : lcls[e[u]], a);
+                                t[u] = delta_term<UNIFORM>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
+                                                           UNIFORM ? 0u : lcls[e[u]], a);
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u)
                                 if (k + u < cnt) delta_add(d, t[u]);
@@ -912,6 +943,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.blk_first_run = s.blk_first_run;
     a.nblk = s.nblk;
     a.blk_halo = s.blk_halo;
+    a.wide_blk = s.wide_blk;
     a.iter_state = s.iter_state;
     a.classes = s.classes;
     a.fm = s.fluid_mats;
@@ -930,16 +962,20 @@ constexpr uint32_t kDynLdsBytes = 62u * 1024u;
 
 static bool tiled(const SubstepConsts &c, uint32_t it) { return c.key_mode == CN_KEY_MORTON && it < kIterStateWords; }
 
-// Halo capacity shared by the density and delta passes: a block uses the LDS path in both or in
-// neither (the delta pass reads the lists the density pass wrote in halo offsets).
-static uint32_t tile_of(const SubstepConsts &c)
-{
-    const uint32_t widest = 32u + (c.tension ? 16u : 0u) + (c.uniform_class ? 0u : 4u);
-    return std::min<uint32_t>(c.tile_max, kDynLdsBytes / widest);
-}
-
 // persistent grid: blocks in flight on all dies, each pulling cell blocks from the queues
 static uint32_t persistent_grid(uint32_t n) { return std::min<uint32_t>(1024u, std::max<uint32_t>(kXcds, (n + 127u) / 128u)); }
+
+// Halo capacities: the density pass stages positions (and classes), the delta pass records,
+// previous deltas, normals and classes, the wide delta pass the same without previous deltas but
+// with a slot map.  density >= wide >= delta, so every block the delta passes stage was staged (and
+// listed in halo offsets) by the density pass.
+void fluid_tiles(SubstepConsts &c)
+{
+    const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
+    c.tile_dens = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (16u + cls));
+    c.tile_delta = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (32u + nc + cls));
+    c.tile_wide = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (20u + nc + cls));
+}
 
 void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
@@ -953,7 +989,7 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
     FluidArgs a = fluid_args(s, c, it, din);
     if (tiled(c, it)) {
         const uint32_t per = 16u + (c.uniform_class ? 0u : 4u);
-        a.tile = tile_of(c);
+        a.tile = c.tile_dens;
         const size_t lds = (size_t)a.tile * per;
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
@@ -971,13 +1007,24 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
 {
     FluidArgs a = fluid_args(s, c, it, din);
     if (tiled(c, it)) {
-        const uint32_t per = 16u + (it > 0 ? 16u : 0u) + (c.tension ? 16u : 0u) + (c.uniform_class ? 0u : 4u);
-        a.tile = tile_of(c);
-        const size_t lds = (size_t)a.tile * per;
+        const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
+        // first iteration: no previous delta, so the delta pass stages as much as the wide one
+        a.tile = it > 0 ? c.tile_delta : c.tile_wide;
+        size_t lds = (size_t)a.tile * (16u + (it > 0 ? 16u : 0u) + nc + cls);
+        const dim3 grid(persistent_grid(c.n)), block(kTileBlock);
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_delta_tiled<true>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
+            hipLaunchKernelGGL((k_fluid_delta_tiled<true, false>), grid, block, lds, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_delta_tiled<false>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
+            hipLaunchKernelGGL((k_fluid_delta_tiled<false, false>), grid, block, lds, st, a, c);
+        if (it > 0 && c.tile_wide > c.tile_delta) {
+            a.tile = c.tile_wide;
+            lds = (size_t)a.tile * (20u + nc + cls);
+            const dim3 wgrid(std::min<uint32_t>(256u, persistent_grid(c.n)));
+            if (c.uniform_class)
+                hipLaunchKernelGGL((k_fluid_delta_tiled<true, true>), wgrid, block, lds, st, a, c);
+            else
+                hipLaunchKernelGGL((k_fluid_delta_tiled<false, true>), wgrid, block, lds, st, a, c);
+        }
     } else {
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_delta_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);

@@ -11,10 +11,13 @@ namespace cn {
 constexpr uint32_t kIterStateWords = 64;  // iterations with tiled fluid passes (later ones walk the run table)
 constexpr uint32_t kXcds = 8;             // accelerator dies; each has its own L2 and block-queue counter
 // iter_state words: block-queue counters of the density and delta passes (kXcds per iteration),
-// then the largest |delta|^2 bits of each iteration
+// the queue counter of the wide-tile delta pass per iteration and the wide-block count (all
+// cleared once per substep), then the largest |delta|^2 bits of each iteration
 constexpr uint32_t kStateDensity = 0;
 constexpr uint32_t kStateDelta = kIterStateWords * kXcds;
-constexpr uint32_t kStateMaxDisp = 2 * kIterStateWords * kXcds;
+constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds;
+constexpr uint32_t kStateWideCount = kStateWideQueue + kIterStateWords;
+constexpr uint32_t kStateMaxDisp = kStateWideCount + 1;
 constexpr uint32_t kStateWords = kStateMaxDisp + kIterStateWords;
 // per-block halo descriptor: start slot and exclusive particle offset of each of the 6x6x6 halo
 // cells (x fastest), the halo particle total, a purity flag, the block's particle range and the
@@ -66,6 +69,7 @@ struct SceneBuffers {
     uint32_t *nblk = nullptr;
     uint32_t *blk_halo = nullptr;   // kHaloStride words per block, for the first blk_cap blocks
     uint32_t blk_cap = 0;
+    uint32_t *wide_blk = nullptr;   // blocks whose halo fits the wide delta tile only (blk_cap)
     uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
     float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
     uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, k-major: list[k * n + i]
@@ -106,6 +110,7 @@ void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, uint32_t key_bits, hipStream_t st);
 void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void fluid_tiles(SubstepConsts &c);
 void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
