@@ -55,6 +55,68 @@ def cpu_baseline(n_sample: int, seed: int = 2):
                       f"density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
 
 
+def batch_bench(count: int, steps: int, warmup: int, loop_steps: int):
+    """The batched workload: `count` independent scenes (sizes log-uniform in [2k, 64k], 40 % PBF,
+    30 % tet soft bodies, 30 % cloth, seed 3) stepped together as one pool with a graph-captured
+    step, against looping the plain path (one context per scene, each stepped in turn)."""
+    import numpy as np
+    import cnhip
+    from cnhip import scenes
+    t0 = time.perf_counter()
+    batch = scenes.batched(count)
+    gen_s = time.perf_counter() - t0
+    total = sum(sc.n for sc in batch)
+    p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
+    print(f"bench: batch of {count} scenes, {total} particles (built in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    pool = cnhip.Solver(0)
+    pool.reserve(total, CAP)
+    pool.upload_batch(batch)
+    pool.graphs(True)
+    for _ in range(warmup):
+        pool.step(p)
+    pool.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        pool.step(p)
+        pool.synchronize()
+        lat.append(time.perf_counter() - t)
+    el = time.perf_counter() - t0
+    trunc = pool.meta().fluid_neighbor_truncations
+    pool.close()
+    print(f"bench: batched {steps} steps in {el:.3f} s; looping the plain path next", file=sys.stderr, flush=True)
+    ctxs = []
+    for sc in batch:
+        s = cnhip.Solver(0)
+        s.reserve(sc.n, CAP)
+        s.upload(sc)
+        ctxs.append(s)
+    for s in ctxs:
+        s.step(p)
+    for s in ctxs:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for s in ctxs:
+            s.step(p)
+    for s in ctxs:
+        s.synchronize()
+    el_loop = time.perf_counter() - t0
+    for s in ctxs:
+        s.close()
+    lat_ms = np.array(lat) * 1e3
+    env = count * steps / el
+    env_loop = count * loop_steps / el_loop
+    return {"workload": f"{count} scenes, log-uniform 2k-64k particles, 40% PBF / 30% tet / 30% cloth, seed 3",
+            "scenes": count, "particles": total, "steps": steps, "warmup": warmup,
+            "env_steps_per_s": env, "ms_per_step": el * 1e3 / steps,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "particle_substeps_per_s": total * SUBSTEPS * steps / el,
+            "loop_plain_env_steps_per_s": env_loop, "loop_plain_steps": loop_steps,
+            "speedup_vs_loop_plain": env / env_loop, "graph": True, "neighbor_truncations": trunc}
+
+
 def copy_probe(mib: int):
     """A plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters give the
     factor between those counters and bytes moved."""
@@ -78,6 +140,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--batch-scenes", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -155,6 +221,8 @@ def main():
                              "ms_per_substep": sub_ms},
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
     }
+    if not args.no_batch:
+        out["batch"] = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps)
     if not args.no_cpu:
         print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -141,8 +141,9 @@ class Solver:
     def reserve(self, n: int, max_fluid_neighbors: int = 128):
         self._check(self.lib.cn_scene_reserve(self._ctx, n, max_fluid_neighbors), "cn_scene_reserve")
 
-    def upload(self, sc: Scene):
-        self.scene = sc
+    @staticmethod
+    def _views(sc: Scene):
+        """C views of a scene's arrays; the returned arrays must outlive the call that uses the views."""
         arrs = dict(
             pos=np.ascontiguousarray(sc.pos, np.float32), vel=np.ascontiguousarray(sc.vel, np.float32),
             radii=np.ascontiguousarray(sc.radii, np.float32),
@@ -158,21 +159,55 @@ class Solver:
                                    p(arrs["fm"]), p(arrs["pm"]), p(arrs["env"]), p(arrs["ph"]),
                                    p(arrs["ly"]), p(arrs["mk"]))
         tables = abi.SceneTables(len(arrs["fmats"]), p(arrs["fmats"]), len(arrs["pmats"]), p(arrs["pmats"]),
-                                 len(arrs["cols"]), p(arrs["cols"]))
+                                 len(arrs["cols"]), p(arrs["cols"]) if len(arrs["cols"]) else None)
         e, b, d, t = sc.constraint_arrays()
         cons = abi.ConstraintView(len(e), p(e) if len(e) else None, len(b), p(b) if len(b) else None,
                                   len(d), p(d) if len(d) else None, len(t), p(t) if len(t) else None)
+        return view, tables, cons, (arrs, e, b, d, t)
+
+    def upload(self, sc: Scene):
+        self.scene = sc
+        self.scenes = [sc]
+        self.n = sc.n
+        view, tables, cons, keep = self._views(sc)
         self._check(self.lib.cn_scene_upload(self._ctx, C.byref(view), C.byref(tables), C.byref(cons)),
                     "cn_scene_upload")
 
+    def upload_batch(self, scenes):
+        """Upload independent scenes into one pool (cn_batch_upload); step() then steps them all."""
+        self.scene = None
+        self.scenes = list(scenes)
+        views = [self._views(sc) for sc in self.scenes]
+        descs = (abi.SceneDesc * len(views))()
+        for k, (v, t, c, _) in enumerate(views):
+            descs[k] = abi.SceneDesc(C.pointer(v), C.pointer(t), C.pointer(c))
+        self.n = sum(sc.n for sc in self.scenes)
+        self._check(self.lib.cn_batch_upload(self._ctx, len(views), descs), "cn_batch_upload")
+        self.ranges = []
+        for k in range(len(views)):
+            first, count = C.c_uint32(), C.c_uint32()
+            self._check(self.lib.cn_batch_scene_range(self._ctx, k, C.byref(first), C.byref(count)),
+                        "cn_batch_scene_range")
+            self.ranges.append((first.value, count.value))
+
+    def readback_scene(self, k: int):
+        """(pos, prev, vel) of scene k of the pool."""
+        pos, prev, vel = self.readback()
+        f, c = self.ranges[k]
+        return pos[f:f + c], prev[f:f + c], vel[f:f + c]
+
     def step(self, params: abi.StepParams):
         self._check(self.lib.cn_step(self._ctx, C.byref(params)), "cn_step")
+
+    def graphs(self, enable: bool):
+        """Capture cn_step into a HIP graph and replay it (cn_graph_enable)."""
+        self._check(self.lib.cn_graph_enable(self._ctx, 1 if enable else 0), "cn_graph_enable")
 
     def synchronize(self):
         self._check(self.lib.cn_synchronize(self._ctx), "cn_synchronize")
 
     def readback(self):
-        n = self.scene.n
+        n = self.n
         pos = np.zeros((n, 4), np.float32)
         prev = np.zeros((n, 4), np.float32)
         vel = np.zeros((n, 4), np.float32)
@@ -218,7 +253,7 @@ class Solver:
         return out
 
     def pass_substep_broadphase(self, params: abi.StepParams):
-        n = self.scene.n
+        n = self.n
         cap = params.max_fluid_neighbors
         keys = np.zeros(n, np.uint32)
         ids = np.zeros(n, np.uint32)

@@ -82,6 +82,11 @@ class ConstraintView(C.Structure):
                 ("tet_count", C.c_uint32), ("tets", C.c_void_p)]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [("particles", C.POINTER(ParticleSoaView)), ("tables", C.POINTER(SceneTables)),
+                ("constraints", C.POINTER(ConstraintView))]
+
+
 class StepParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("substeps", C.c_uint32), ("default_iterations", C.c_uint32),
                 ("fluid_iterations", C.c_uint32), ("soft_internal_iterations", C.c_uint32),
@@ -131,6 +136,9 @@ SIGNATURES = {
     "cn_scene_upload": (C.c_int, [_P, C.POINTER(ParticleSoaView), C.POINTER(SceneTables),
                                   C.POINTER(ConstraintView)]),
     "cn_step": (C.c_int, [_P, C.POINTER(StepParams)]),
+    "cn_batch_upload": (C.c_int, [_P, _U, C.POINTER(SceneDesc)]),
+    "cn_batch_scene_range": (C.c_int, [_P, _U, C.POINTER(_U), C.POINTER(_U)]),
+    "cn_graph_enable": (C.c_int, [_P, C.c_int]),
     "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
     "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),
     "cn_validate_meta": (C.c_int, [_P, C.POINTER(Meta)]),

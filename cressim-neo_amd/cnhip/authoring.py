@@ -56,21 +56,19 @@ def tet_grid(size=(1.0, 1.0, 1.0), spacing=0.5, translation=(0.0, 0.0, 0.0), res
     def flat(x, y, z):
         return (x * ry + y) * rz + z
 
-    tets = []
-    for x in range(rx - 1):
-        for y in range(ry - 1):
-            for z in range(rz - 1):
-                p0, p1 = flat(x, y, z), flat(x, y, z + 1)
-                p3, p2 = flat(x + 1, y, z), flat(x + 1, y, z + 1)
-                p7, p6 = flat(x + 1, y + 1, z), flat(x + 1, y + 1, z + 1)
-                p4, p5 = flat(x, y + 1, z), flat(x, y + 1, z + 1)
-                if (x + y + z) & 1:
-                    tets += [(p2, p1, p6, p3), (p6, p3, p4, p7), (p4, p1, p6, p5), (p3, p1, p4, p0),
-                             (p6, p1, p4, p3)]
-                else:
-                    tets += [(p0, p2, p5, p1), (p7, p2, p0, p3), (p5, p2, p7, p6), (p7, p0, p5, p4),
-                             (p0, p2, p7, p5)]
-    return pos, np.asarray(tets, np.uint32).reshape(-1, 4)
+    # cubes in x, y, z order (z fastest), five tets each, the split alternating with (x + y + z) parity
+    x, y, z = (v.ravel().astype(np.int64) for v in
+               np.meshgrid(np.arange(rx - 1), np.arange(ry - 1), np.arange(rz - 1), indexing="ij"))
+    p0, p1 = flat(x, y, z), flat(x, y, z + 1)
+    p3, p2 = flat(x + 1, y, z), flat(x + 1, y, z + 1)
+    p7, p6 = flat(x + 1, y + 1, z), flat(x + 1, y + 1, z + 1)
+    p4, p5 = flat(x, y + 1, z), flat(x, y + 1, z + 1)
+    odd = np.stack([np.stack(t, -1) for t in ((p2, p1, p6, p3), (p6, p3, p4, p7), (p4, p1, p6, p5),
+                                               (p3, p1, p4, p0), (p6, p1, p4, p3))], 1)
+    even = np.stack([np.stack(t, -1) for t in ((p0, p2, p5, p1), (p7, p2, p0, p3), (p5, p2, p7, p6),
+                                                (p7, p0, p5, p4), (p0, p2, p7, p5))], 1)
+    tets = np.where(((x + y + z) & 1).astype(bool)[:, None, None], odd, even)
+    return pos, tets.reshape(-1, 4).astype(np.uint32)
 
 
 def tet_constraints(pos, tets, offset=0, edge_compliance=0.0, volume_compliance=0.0):

@@ -186,3 +186,37 @@ def scaled_box_for(n: int, n_ref: int = 4_194_304, box_ref=(4.0, 2.0, 2.0)):
     """Box with the same aspect and number density as box_ref holds for n_ref particles."""
     s = (n / n_ref) ** (1.0 / 3.0)
     return tuple(float(b * s) for b in box_ref)
+
+
+def batch_sizes(count: int = 512, seed: int = 3, lo: int = 2000, hi: int = 64000):
+    """Particle counts log-uniform in [lo, hi] and scene kinds (0 PBF 40 %, 1 tet soft body 30 %,
+    2 cloth 30 %) of a batch of independent scenes, from one seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+    kind = rng.choice(3, size=count, p=[0.4, 0.3, 0.3])
+    return n, kind
+
+
+def batch_scene(n: int, kind: int, seed: int = 0) -> Scene:
+    """One scene of the batched workload with about n particles in its own 4 x 2 x 2 m container:
+    a jittered PBF block (spacing 0.05 m, r = 0.03 m, the dam-break geometry), a tet soft block
+    (spacing 0.04 m) or a cloth grid spanning 1.9 m, compliance 1e-6."""
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    fm = fluid_material(0.03)
+    if kind == 0:
+        side = max(2, int(round(n ** (1.0 / 3.0))))
+        part = fluid_part(side, 0.05, 0.03, origin=(0.05, 0.02, 0.05), seed=seed, jitter=0.005)
+    elif kind == 1:
+        res = max(2, int(round(n ** (1.0 / 3.0))))
+        part = soft_block_part(res, 0.04, 0.02, translation=(1.0, 0.2, 0.2))
+    else:
+        m = max(2, int(round(n ** 0.5)))
+        sp = 1.9 / (m - 1)
+        part = cloth_part(m, sp, 0.5 * sp, translation=(0.05, 1.2, 0.05))
+    return compose([part], fm, cols, f"batch_{('pbf', 'tet', 'cloth')[kind]}_{n}", dict(n=int(n), kind=int(kind)))
+
+
+def batched(count: int = 512, seed: int = 3, lo: int = 2000, hi: int = 64000):
+    """The batched workload: count independent scenes sized and typed by batch_sizes()."""
+    n, kind = batch_sizes(count, seed, lo, hi)
+    return [batch_scene(int(n[k]), int(kind[k]), seed=k) for k in range(count)]

@@ -35,6 +35,20 @@ struct cn_ctx {
     bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
     bool has_tension = false;        // any fluid material with surface tension
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
+    // scene pool
+    uint32_t nscenes = 0;
+    uint32_t scene_bits = 0;         // key bits above the Morton code holding the scene index
+    std::vector<uint32_t> scene_first;  // first particle of each scene, then the total
+    std::vector<float> scene_cell;      // derived grid cell size of each scene
+    bool graph_valid = false;         // the cached graphs refer to the current buffers
+    bool graphs_enabled = false;
+    struct Graph {
+        cn_step_params params;
+        int cur0, cur1;                  // slot-order parity before and after the step
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
     // profiling
     bool profiling = false;
     struct Ev {
@@ -122,6 +136,7 @@ void free_scene(SceneBuffers &s)
     dfree(s.fluid_mats);
     dfree(s.part_mats);
     dfree(s.colliders);
+    dfree(s.scenes);
     dfree(s.meta);
     dfree(s.slot_of);
     dfree(s.edges);
@@ -132,6 +147,15 @@ void free_scene(SceneBuffers &s)
     dfree(s.corr);
     dfree(s.inc);
     s = SceneBuffers{};
+}
+
+void clear_graphs(cn_ctx *ctx)
+{
+    for (auto &g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    ctx->graphs.clear();
 }
 
 void free_constraints(SceneBuffers &s)
@@ -298,22 +322,25 @@ struct ProfScope {
     }
 };
 
+uint32_t key_bits_of(const cn_ctx *ctx, const cn_step_params *p)
+{
+    return p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits + ctx->scene_bits;
+}
+
 SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
 {
     SubstepConsts c{};
     c.dt = dt;
-    const float cell = p->cell_size > 0.0f ? p->cell_size : ctx->cell_size;
-    c.inv_cell = 1.0f / std::max(cell, 1.0e-4f);
+    // 0: each scene's derived cell size (SceneInfo); > 0: the caller's cell size for every scene
+    c.inv_cell = p->cell_size > 0.0f ? 1.0f / std::max(p->cell_size, 1.0e-4f) : 0.0f;
     c.gravity[0] = p->gravity[0];
     c.gravity[1] = p->gravity[1];
     c.gravity[2] = p->gravity[2];
     c.n = ctx->s.n;
     c.cap = p->max_fluid_neighbors;
-    c.ncolliders = ctx->s.ncolliders;
     c.key_mode = p->key_mode;
     c.morton_bits = ctx->morton_bits;
-    const uint32_t key_bits = p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
-    c.bucket_shift = key_bits - ctx->s.bucket_bits;
+    c.bucket_shift = key_bits_of(ctx, p) - ctx->s.bucket_bits;
     c.uniform_class = ctx->s.nclasses == 1 ? 1u : 0u;
     c.tension = ctx->has_tension ? 1u : 0u;
     c.tile_max = ctx->tile_max;
@@ -324,10 +351,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     return c;
 }
 
-uint32_t key_bits_of(const cn_ctx *ctx, const cn_step_params *p)
-{
-    return p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits;
-}
+
 
 // predict + keys + sort + runs + reorder for one substep; returns the sorted (key, old slot) pairs
 uint2 *broadphase(cn_ctx *ctx, const SubstepConsts &c, uint32_t key_bits)
@@ -392,6 +416,7 @@ void cn_ctx_destroy(cn_ctx *ctx)
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
+    clear_graphs(ctx);
     free_scene(ctx->s);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -505,6 +530,7 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_scene(ctx->s);
     ctx->uploaded = false;
+    ctx->graph_valid = false;
     SceneBuffers &s = ctx->s;
     const size_t N = n ? n : 1;
     for (int k = 0; k < 2; ++k) {
@@ -557,10 +583,16 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     return CN_OK;
 }
 
-cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_scene_tables *tb,
-                          const cn_constraint_view *cv)
+}  // extern "C"
+
+namespace {
+
+cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
 {
-    if (!ctx || !pv || !tb) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    const cn_particle_soa_view *pv = d.particles;
+    const cn_scene_tables *tb = d.tables;
+    const cn_constraint_view *cv = d.constraints;
+    if (!pv || !tb) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
     const uint32_t n = pv->count;
     if (n == 0) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "empty particle pool");
     if (!pv->positions_inv_mass || !pv->velocities || !pv->radii || !pv->kinds || !pv->particle_material_indices)
@@ -575,22 +607,50 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
         for (uint32_t b = 0; b < cv->bend_count; ++b)
             if (cv->bends[b].p0 >= n || cv->bends[b].p1 >= n || cv->bends[b].p2 >= n)
                 return fail(ctx, CN_ERR_INVALID_ARGUMENT, "bend particle index out of range");
-        for (uint32_t d = 0; d < cv->dihedral_count; ++d) {
-            const cn_cloth_dihedral &D = cv->dihedrals[d];
+        for (uint32_t q = 0; q < cv->dihedral_count; ++q) {
+            const cn_cloth_dihedral &D = cv->dihedrals[q];
             if (D.e0 >= n || D.e1 >= n || D.o0 >= n || D.o1 >= n)
                 return fail(ctx, CN_ERR_INVALID_ARGUMENT, "dihedral particle index out of range");
         }
         for (uint32_t t = 0; t < cv->tet_count; ++t)
             for (int k = 0; k < 4; ++k)
                 if (cv->tets[t].p[k] >= n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "tet particle index out of range");
-        const uint64_t refs = 2ull * cv->edge_count + 3ull * cv->bend_count + 4ull * cv->dihedral_count +
-                              4ull * cv->tet_count;
-        if (cv->edge_count >= (1u << 30) || cv->bend_count + (uint64_t)cv->dihedral_count >= (1u << 30) ||
-            cv->tet_count >= (1u << 30) || refs >= (1ull << 32))
-            return fail(ctx, CN_ERR_UNSUPPORTED, "too many constraints");
     }
-    if (tb->collider_count > kMaxColliders)
-        return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+    if (tb->collider_count > kMaxColliders) return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t kind = pv->kinds[i];
+        const uint32_t fm = pv->fluid_material_indices ? pv->fluid_material_indices[i] : 0xffffffffu;
+        if (kind == CN_PARTICLE_KIND_FLUID && fm >= tb->fluid_material_count)
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid particle with an invalid fluid material index");
+        if (pv->particle_material_indices[i] >= tb->particle_material_count)
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "invalid particle material index");
+    }
+    return CN_OK;
+}
+
+// Uploads a pool of independent scenes: particles, classes, materials and constraints are
+// concatenated with index offsets, colliders are visit-ordered per scene, and each scene gets its
+// own cell size and key range.  One scene is the plain path.
+cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
+{
+    if (!ctx || !descs || count == 0) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument or no scene");
+    uint64_t total = 0, refs = 0, ne = 0, nb = 0, nd = 0, nt = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const cn_status st = check_scene(ctx, descs[k]);
+        if (st != CN_OK) return st;
+        total += descs[k].particles->count;
+        if (const cn_constraint_view *cv = descs[k].constraints) {
+            ne += cv->edge_count;
+            nb += cv->bend_count;
+            nd += cv->dihedral_count;
+            nt += cv->tet_count;
+        }
+    }
+    refs = 2 * ne + 3 * nb + 4 * nd + 4 * nt;
+    if (total >= (1ull << 31)) return fail(ctx, CN_ERR_UNSUPPORTED, "more than 2^31 particles");
+    if (ne >= (1u << 30) || nb + nd >= (1u << 30) || nt >= (1u << 30) || refs >= (1ull << 32))
+        return fail(ctx, CN_ERR_UNSUPPORTED, "too many constraints");
+    const uint32_t n = (uint32_t)total;
     if (n > ctx->capacity || !ctx->s.pos[0]) {
         const cn_status st = cn_scene_reserve(ctx, n, std::max<uint32_t>(ctx->cap_alloc, 128u));
         if (st != CN_OK) return st;
@@ -601,140 +661,160 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
     s.n = n;
     s.cur = 0;
 
-    // particle classes
-    std::map<std::array<uint32_t, 8>, uint32_t> cmap;
+    // per-scene tables, concatenated
+    std::vector<float> pos(4ull * n), vel(4ull * n);
+    std::vector<cn_fluid_material> fmh;
+    std::vector<float> pmh;
+    std::vector<Collider> col;
+    std::vector<SceneInfo> sinfo(count);
+    std::vector<cn_soft_edge> edges;
+    std::vector<cn_soft_bend> bends;
+    std::vector<cn_cloth_dihedral> dih;
+    std::vector<cn_soft_tet> tets;
+    std::map<std::array<uint32_t, 9>, uint32_t> cmap;
     std::vector<ParticleClass> classes;
     std::vector<uint32_t> cls(n);
-    std::array<uint32_t, 8> last{};
-    uint32_t last_id = 0xffffffffu;
-    for (uint32_t i = 0; i < n; ++i) {
-        ParticleClass pc{};
-        pc.radius = pv->radii[i];
-        pc.kind = pv->kinds[i];
-        pc.fluid_mat = pv->fluid_material_indices ? pv->fluid_material_indices[i] : 0xffffffffu;
-        pc.part_mat = pv->particle_material_indices[i];
-        pc.env = pv->environment_indices ? pv->environment_indices[i] : 0u;
-        pc.phase = pv->phases ? pv->phases[i] : 0u;
-        pc.layer = pv->collision_layers ? pv->collision_layers[i] : 1u;
-        pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
-        if (pc.kind == CN_PARTICLE_KIND_FLUID && pc.fluid_mat >= tb->fluid_material_count)
-            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid particle with an invalid fluid material index");
-        if (pc.part_mat >= tb->particle_material_count)
-            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "invalid particle material index");
-        std::array<uint32_t, 8> key;
-        std::memcpy(key.data(), &pc, sizeof(pc));
-        if (last_id != 0xffffffffu && key == last) {
-            cls[i] = last_id;
-            continue;
-        }
-        auto it = cmap.find(key);
-        uint32_t id;
-        if (it == cmap.end()) {
-            id = (uint32_t)classes.size();
-            cmap.emplace(key, id);
-            classes.push_back(pc);
-        } else {
-            id = it->second;
-        }
-        cls[i] = id;
-        last = key;
-        last_id = id;
-    }
-    std::vector<uint32_t> ids(n);
-    for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+    ctx->scene_first.assign(count + 1, 0);
+    ctx->scene_cell.assign(count, 0.0f);
     ctx->has_fluid = false;
-    for (uint32_t i = 0; i < n && !ctx->has_fluid; ++i) ctx->has_fluid = pv->kinds[i] == CN_PARTICLE_KIND_FLUID;
+    uint32_t base = 0;
+    double max_span = 1.0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const cn_particle_soa_view *pv = descs[k].particles;
+        const cn_scene_tables *tb = descs[k].tables;
+        const cn_constraint_view *cv = descs[k].constraints;
+        const uint32_t m = pv->count;
+        const uint32_t fm0 = (uint32_t)fmh.size(), pm0 = (uint32_t)(pmh.size() / 4);
+        fmh.insert(fmh.end(), tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
+        pmh.insert(pmh.end(), tb->particle_materials, tb->particle_materials + 4 * tb->particle_material_count);
+        std::memcpy(pos.data() + 4ull * base, pv->positions_inv_mass, 4ull * m * sizeof(float));
+        std::memcpy(vel.data() + 4ull * base, pv->velocities, 4ull * m * sizeof(float));
+        std::array<uint32_t, 9> last{};
+        uint32_t last_id = 0xffffffffu;
+        for (uint32_t i = 0; i < m; ++i) {
+            ParticleClass pc{};
+            pc.radius = pv->radii[i];
+            pc.kind = pv->kinds[i];
+            const uint32_t fmi = pv->fluid_material_indices ? pv->fluid_material_indices[i] : 0xffffffffu;
+            pc.fluid_mat = fmi < tb->fluid_material_count ? fm0 + fmi : 0xffffffffu;
+            pc.part_mat = pm0 + pv->particle_material_indices[i];
+            pc.env = pv->environment_indices ? pv->environment_indices[i] : 0u;
+            pc.phase = pv->phases ? pv->phases[i] : 0u;
+            pc.layer = pv->collision_layers ? pv->collision_layers[i] : 1u;
+            pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
+            pc.scene = k;
+            ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
+            std::array<uint32_t, 9> key;
+            std::memcpy(key.data(), &pc, sizeof(pc));
+            if (last_id != 0xffffffffu && key == last) {
+                cls[base + i] = last_id;
+                continue;
+            }
+            auto it = cmap.find(key);
+            uint32_t id;
+            if (it == cmap.end()) {
+                id = (uint32_t)classes.size();
+                cmap.emplace(key, id);
+                classes.push_back(pc);
+            } else {
+                id = it->second;
+            }
+            cls[base + i] = id;
+            last = key;
+            last_id = id;
+        }
+        // static colliders in the scene's own LBVH visit order
+        const std::vector<Collider> sc = visit_ordered_colliders(tb->colliders, tb->collider_count);
+        sinfo[k].col0 = (uint32_t)col.size();
+        sinfo[k].ncol = (uint32_t)sc.size();
+        col.insert(col.end(), sc.begin(), sc.end());
+        // constraints, shifted to pool indices
+        if (cv) {
+            for (uint32_t e = 0; e < cv->edge_count; ++e) {
+                cn_soft_edge E = cv->edges[e];
+                E.a += base;
+                E.b += base;
+                edges.push_back(E);
+            }
+            for (uint32_t q = 0; q < cv->bend_count; ++q) {
+                cn_soft_bend B = cv->bends[q];
+                B.p0 += base;
+                B.p1 += base;
+                B.p2 += base;
+                bends.push_back(B);
+            }
+            for (uint32_t q = 0; q < cv->dihedral_count; ++q) {
+                cn_cloth_dihedral D = cv->dihedrals[q];
+                D.e0 += base;
+                D.e1 += base;
+                D.o0 += base;
+                D.o1 += base;
+                dih.push_back(D);
+            }
+            for (uint32_t t = 0; t < cv->tet_count; ++t) {
+                cn_soft_tet T = cv->tets[t];
+                for (int j = 0; j < 4; ++j) T.p[j] += base;
+                tets.push_back(T);
+            }
+        }
+        // grid cell size of the scene (recomputeParticleGridCellSize) and its extent in cells
+        const float cell = cn_particle_grid_cell_size(m, pv->radii, pv->fluid_material_indices,
+                                                      tb->fluid_material_count, tb->fluid_materials);
+        ctx->scene_cell[k] = cell;
+        sinfo[k].inv_cell = 1.0f / std::max(cell, 1.0e-4f);
+        float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (uint32_t i = 0; i < m; ++i)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], pv->positions_inv_mass[4 * i + a]);
+                hi[a] = std::max(hi[a], pv->positions_inv_mass[4 * i + a]);
+            }
+        for (const Collider &c : sc)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], c.aabb_min[a]);
+                hi[a] = std::max(hi[a], c.aabb_max[a]);
+            }
+        for (int a = 0; a < 3; ++a) max_span = std::max(max_span, (double)(hi[a] - lo[a]) / (double)cell + 3.0);
+        ctx->scene_first[k] = base;
+        base += m;
+    }
+    ctx->scene_first[count] = base;
+    ctx->cell_size = ctx->scene_cell[0];
+    // key layout: [scene index | 3 x morton_bits]; coordinates beyond 2^morton_bits cells wrap and
+    // their runs are filtered per particle
+    ctx->scene_bits = count > 1 ? ceil_log2(count) : 0u;
+    const uint32_t room = (32u - ctx->scene_bits) / 3u;
+    ctx->morton_bits = std::min<uint32_t>(std::min<uint32_t>(10u, room),
+                                          std::max<uint32_t>(1u, ceil_log2((uint64_t)std::ceil(max_span))));
+    for (uint32_t k = 0; k < count; ++k) sinfo[k].key_prefix = ctx->scene_bits ? k << (3u * ctx->morton_bits) : 0u;
+    ctx->nscenes = count;
 
     dfree(s.classes);
     dfree(s.fluid_mats);
     dfree(s.part_mats);
     dfree(s.colliders);
+    dfree(s.scenes);
+    dfree(s.fluid_t2);
     s.classes = nullptr;
     s.fluid_mats = nullptr;
     s.part_mats = nullptr;
     s.colliders = nullptr;
+    s.scenes = nullptr;
+    s.fluid_t2 = nullptr;
     s.nclasses = (uint32_t)classes.size();
     CN_HIP(ctx, dalloc(&s.classes, classes.size()));
     CN_HIP(ctx, hipMemcpy(s.classes, classes.data(), classes.size() * sizeof(ParticleClass), hipMemcpyHostToDevice));
-    CN_HIP(ctx, dalloc(&s.fluid_mats, tb->fluid_material_count));
-    if (tb->fluid_material_count)
-        CN_HIP(ctx, hipMemcpy(s.fluid_mats, tb->fluid_materials, tb->fluid_material_count * sizeof(cn_fluid_material),
-                              hipMemcpyHostToDevice));
-    CN_HIP(ctx, dalloc(&s.part_mats, tb->particle_material_count));
-    CN_HIP(ctx, hipMemcpy(s.part_mats, tb->particle_materials, tb->particle_material_count * sizeof(float4),
-                          hipMemcpyHostToDevice));
-    const std::vector<Collider> col = visit_ordered_colliders(tb->colliders, tb->collider_count);
+    CN_HIP(ctx, dalloc(&s.fluid_mats, fmh.size()));
+    if (!fmh.empty())
+        CN_HIP(ctx, hipMemcpy(s.fluid_mats, fmh.data(), fmh.size() * sizeof(cn_fluid_material), hipMemcpyHostToDevice));
+    CN_HIP(ctx, dalloc(&s.part_mats, pmh.size() / 4));
+    if (!pmh.empty()) CN_HIP(ctx, hipMemcpy(s.part_mats, pmh.data(), pmh.size() * sizeof(float), hipMemcpyHostToDevice));
     s.ncolliders = (uint32_t)col.size();
     CN_HIP(ctx, dalloc(&s.colliders, col.size()));
     if (!col.empty())
         CN_HIP(ctx, hipMemcpy(s.colliders, col.data(), col.size() * sizeof(Collider), hipMemcpyHostToDevice));
-
-    CN_HIP(ctx, hipMemcpy(s.pos[0], pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.vel[0], pv->velocities, n * sizeof(float4), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.cls[0], cls.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.id[0], ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.out_pos, pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.out_vel, pv->velocities, n * sizeof(float4), hipMemcpyHostToDevice));
-    CN_HIP(ctx, hipMemcpy(s.out_prev, pv->positions_inv_mass, n * sizeof(float4), hipMemcpyHostToDevice));
-
-    // soft constraints, their incidence lists and the particle -> slot map
-    free_constraints(s);
-    if (cv && (cv->edge_count || cv->bend_count || cv->dihedral_count || cv->tet_count)) {
-        s.ne = cv->edge_count;
-        s.nb = cv->bend_count;
-        s.nd = cv->dihedral_count;
-        s.nt = cv->tet_count;
-        CN_HIP(ctx, dalloc(&s.edges, s.ne));
-        CN_HIP(ctx, dalloc(&s.bends, s.nb));
-        CN_HIP(ctx, dalloc(&s.dihedrals, s.nd));
-        CN_HIP(ctx, dalloc(&s.tets, s.nt));
-        if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, cv->edges, s.ne * sizeof(cn_soft_edge), hipMemcpyHostToDevice));
-        if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, cv->bends, s.nb * sizeof(cn_soft_bend), hipMemcpyHostToDevice));
-        if (s.nd)
-            CN_HIP(ctx, hipMemcpy(s.dihedrals, cv->dihedrals, s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
-        if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, cv->tets, s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
-        const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
-        const size_t ncorr = 2ull * s.ne + 4ull * (s.nb + s.nd) + 4ull * s.nt;
-        CN_HIP(ctx, dalloc(&s.lam, nlam));
-        CN_HIP(ctx, dalloc(&s.corr, ncorr));
-        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
-        std::vector<uint32_t> inc;
-        {
-            const uint32_t cnt[1] = {s.ne}, ar[1] = {2}, off[1] = {0};
-            build_incidence(n, 1, cnt, ar, off,
-                            [&](uint32_t, uint32_t c, uint32_t k) { return k == 0 ? cv->edges[c].a : cv->edges[c].b; },
-                            inc, s.inc_off[0], s.inc_off[1]);
-        }
-        {
-            const uint32_t cnt[2] = {s.nb, s.nd}, ar[2] = {3, 4}, off[2] = {0, s.nb};
-            build_incidence(n, 2, cnt, ar, off,
-                            [&](uint32_t g, uint32_t c, uint32_t k) {
-                                if (g == 0) {
-                                    const cn_soft_bend &B = cv->bends[c];
-                                    return k == 0 ? B.p0 : (k == 1 ? B.p1 : B.p2);
-                                }
-                                const cn_cloth_dihedral &D = cv->dihedrals[c];
-                                return k == 0 ? D.e0 : (k == 1 ? D.e1 : (k == 2 ? D.o0 : D.o1));
-                            },
-                            inc, s.inc_off[2], s.inc_off[3]);
-        }
-        {
-            const uint32_t cnt[1] = {s.nt}, ar[1] = {4}, off[1] = {0};
-            build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return cv->tets[c].p[k]; },
-                            inc, s.inc_off[4], s.inc_off[5]);
-        }
-        CN_HIP(ctx, dalloc(&s.inc, inc.size()));
-        CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-        dfree(s.slot_of);
-        s.slot_of = nullptr;
-        CN_HIP(ctx, dalloc(&s.slot_of, std::max<uint32_t>(ctx->capacity, n)));
-        CN_HIP(ctx, hipMemcpy(s.slot_of, ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
-
-    // grid cell size (recomputeParticleGridCellSize) and Morton resolution from the scene extent
-    std::vector<cn_fluid_material> fmh(tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
-    ctx->cell_size = cn_particle_grid_cell_size(n, pv->radii, pv->fluid_material_indices, tb->fluid_material_count,
-                                                fmh.data());
+    s.nscenes = count;
+    CN_HIP(ctx, dalloc(&s.scenes, sinfo.size()));
+    CN_HIP(ctx, hipMemcpy(s.scenes, sinfo.data(), sinfo.size() * sizeof(SceneInfo), hipMemcpyHostToDevice));
     ctx->max_h = 1.0e-4f;
     ctx->has_tension = false;
     std::vector<float> t2(fmh.size());
@@ -744,42 +824,113 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_
         ctx->has_tension |= (m.surface_tension > 0.0f ? m.surface_tension : 0.0f) > kEps;
         t2[k] = sqrt_threshold(m.smoothing_radius > 1.0e-4f ? m.smoothing_radius : 1.0e-4f);
     }
-    dfree(s.fluid_t2);
-    s.fluid_t2 = nullptr;
     CN_HIP(ctx, dalloc(&s.fluid_t2, t2.size()));
     if (!t2.empty()) CN_HIP(ctx, hipMemcpy(s.fluid_t2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
-    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-    for (uint32_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], pv->positions_inv_mass[4 * i + a]);
-            hi[a] = std::max(hi[a], pv->positions_inv_mass[4 * i + a]);
+
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+    CN_HIP(ctx, hipMemcpy(s.pos[0], pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.vel[0], vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.cls[0], cls.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.id[0], ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_pos, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.out_prev, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+
+    // soft constraints, their incidence lists and the particle -> slot map
+    free_constraints(s);
+    if (!edges.empty() || !bends.empty() || !dih.empty() || !tets.empty()) {
+        s.ne = (uint32_t)edges.size();
+        s.nb = (uint32_t)bends.size();
+        s.nd = (uint32_t)dih.size();
+        s.nt = (uint32_t)tets.size();
+        CN_HIP(ctx, dalloc(&s.edges, s.ne));
+        CN_HIP(ctx, dalloc(&s.bends, s.nb));
+        CN_HIP(ctx, dalloc(&s.dihedrals, s.nd));
+        CN_HIP(ctx, dalloc(&s.tets, s.nt));
+        if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, edges.data(), s.ne * sizeof(cn_soft_edge), hipMemcpyHostToDevice));
+        if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, bends.data(), s.nb * sizeof(cn_soft_bend), hipMemcpyHostToDevice));
+        if (s.nd)
+            CN_HIP(ctx, hipMemcpy(s.dihedrals, dih.data(), s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
+        if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, tets.data(), s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
+        const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
+        const size_t ncorr = 2ull * s.ne + 4ull * (s.nb + s.nd) + 4ull * s.nt;
+        CN_HIP(ctx, dalloc(&s.lam, nlam));
+        CN_HIP(ctx, dalloc(&s.corr, ncorr));
+        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
+        std::vector<uint32_t> inc;
+        {
+            const uint32_t cnt[1] = {s.ne}, ar[1] = {2}, off[1] = {0};
+            build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return k == 0 ? edges[c].a : edges[c].b; },
+                            inc, s.inc_off[0], s.inc_off[1]);
         }
-    for (const Collider &c : col)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], c.aabb_min[a]);
-            hi[a] = std::max(hi[a], c.aabb_max[a]);
+        {
+            const uint32_t cnt[2] = {s.nb, s.nd}, ar[2] = {3, 4}, off[2] = {0, s.nb};
+            build_incidence(n, 2, cnt, ar, off,
+                            [&](uint32_t g, uint32_t c, uint32_t k) {
+                                if (g == 0) {
+                                    const cn_soft_bend &B = bends[c];
+                                    return k == 0 ? B.p0 : (k == 1 ? B.p1 : B.p2);
+                                }
+                                const cn_cloth_dihedral &D = dih[c];
+                                return k == 0 ? D.e0 : (k == 1 ? D.e1 : (k == 2 ? D.o0 : D.o1));
+                            },
+                            inc, s.inc_off[2], s.inc_off[3]);
         }
-    double span = 1.0;
-    for (int a = 0; a < 3; ++a) span = std::max(span, (double)(hi[a] - lo[a]) / (double)ctx->cell_size + 3.0);
-    ctx->morton_bits = std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)std::ceil(span))));
-    const uint32_t kb = 3u * ctx->morton_bits;
+        {
+            const uint32_t cnt[1] = {s.nt}, ar[1] = {4}, off[1] = {0};
+            build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return tets[c].p[k]; }, inc,
+                            s.inc_off[4], s.inc_off[5]);
+        }
+        CN_HIP(ctx, dalloc(&s.inc, inc.size()));
+        CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        dfree(s.slot_of);
+        s.slot_of = nullptr;
+        CN_HIP(ctx, dalloc(&s.slot_of, std::max<uint32_t>(ctx->capacity, n)));
+        CN_HIP(ctx, hipMemcpy(s.slot_of, ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+
+    const uint32_t kb = 3u * ctx->morton_bits + ctx->scene_bits;
     const uint32_t bb = std::max<uint32_t>(4u, ceil_log2(n) > 1 ? ceil_log2(n) - 1 : 1);
     // bucket bits cannot exceed the key bits of either key mode; the allocation fits the largest
     s.bucket_bits = std::min<uint32_t>(std::min<uint32_t>(bb, kb), ctx->bucket_bits_alloc);
     ctx->uploaded = true;
     ctx->stepped = false;
+    ctx->graph_valid = false;
     return CN_OK;
 }
 
-cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
+}  // namespace
+
+extern "C" {
+
+cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *pv, const cn_scene_tables *tb,
+                          const cn_constraint_view *cv)
 {
-    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
-    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step before cn_scene_upload");
-    if (p->key_mode != CN_KEY_MORTON && p->key_mode != CN_KEY_HASH)
-        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "unknown key_mode");
-    if (p->max_fluid_neighbors > ctx->cap_alloc)
-        return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
-    CN_HIP(ctx, hipSetDevice(ctx->device));
+    const cn_scene_desc d{pv, tb, cv};
+    return upload_pool(ctx, 1, &d);
+}
+
+cn_status cn_batch_upload(cn_ctx *ctx, uint32_t scene_count, const cn_scene_desc *scenes)
+{
+    return upload_pool(ctx, scene_count, scenes);
+}
+
+cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *first, uint32_t *count)
+{
+    if (!ctx || !ctx->uploaded || scene >= ctx->nscenes) return CN_ERR_INVALID_ARGUMENT;
+    if (first) *first = ctx->scene_first[scene];
+    if (count) *count = ctx->scene_first[scene + 1] - ctx->scene_first[scene];
+    return CN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Enqueues one cn_step (every substep) on the context stream; advances the slot-order parity.
+void record_step(cn_ctx *ctx, const cn_step_params *p)
+{
     SceneBuffers &s = ctx->s;
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
     const uint32_t def = std::max<uint32_t>(p->default_iterations, 1u);
@@ -787,7 +938,7 @@ cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
     const float dt = p->dt / static_cast<float>(substeps);
     const SubstepConsts c = make_consts(ctx, p, dt);
     const uint32_t key_bits = key_bits_of(ctx, p);
-    CN_HIP(ctx, hipMemsetAsync(s.meta, 0, 8 * sizeof(uint32_t), ctx->stream));
+    launch_fill(s.meta, 8, 0u, ctx->stream);
     const uint32_t soft_iters = resolve_iters(p->soft_internal_iterations, def);
     const uint32_t contact_iters = resolve_iters(p->soft_contact_iterations, def);
     const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
@@ -853,8 +1004,70 @@ cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
         ProfScope ps(ctx, "export");
         launch_export(s, true, false, ctx->stream);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
+{
+    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step before cn_scene_upload");
+    if (p->key_mode != CN_KEY_MORTON && p->key_mode != CN_KEY_HASH)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "unknown key_mode");
+    if (p->key_mode == CN_KEY_HASH && ctx->nscenes > 1)
+        return fail(ctx, CN_ERR_UNSUPPORTED, "hash keys hold one scene; a scene pool needs CN_KEY_MORTON");
+    if (p->max_fluid_neighbors > ctx->cap_alloc)
+        return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->graphs_enabled || ctx->profiling) {
+        record_step(ctx, p);
+        ctx->stepped = true;
+        CN_HIP(ctx, hipGetLastError());
+        return CN_OK;
+    }
+    // graph path: one captured graph per (parameters, starting slot-order parity), replayed
+    if (!ctx->graph_valid) {
+        clear_graphs(ctx);
+        ctx->graph_valid = true;
+    }
+    const int cur0 = ctx->s.cur;
+    for (auto &g : ctx->graphs)
+        if (g.cur0 == cur0 && std::memcmp(&g.params, p, sizeof(*p)) == 0) {
+            CN_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+            ctx->s.cur = g.cur1;
+            ctx->stepped = true;
+            return CN_OK;
+        }
+    cn_ctx::Graph g{};
+    g.params = *p;
+    g.cur0 = cur0;
+    CN_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    record_step(ctx, p);
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g.graph);
+    if (ce != hipSuccess) {
+        ctx->s.cur = cur0;
+        return fail(ctx, CN_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+    }
+    g.cur1 = ctx->s.cur;
+    const hipError_t ie = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g.graph);
+        ctx->s.cur = cur0;
+        return fail(ctx, CN_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+    }
+    if (ctx->graphs.size() >= 8) clear_graphs(ctx);
+    ctx->graphs.push_back(g);
+    CN_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
     ctx->stepped = true;
-    CN_HIP(ctx, hipGetLastError());
+    return CN_OK;
+}
+
+cn_status cn_graph_enable(cn_ctx *ctx, int enable)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    ctx->graphs_enabled = enable != 0;
     return CN_OK;
 }
 

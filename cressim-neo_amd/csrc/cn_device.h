@@ -41,6 +41,17 @@ struct ParticleClass {
     uint32_t phase;
     uint32_t layer;
     uint32_t mask;
+    uint32_t scene;  // scene of the pool the particle belongs to
+};
+
+// One scene of the pool: its cell size, its key range and its static colliders.  Scenes never
+// interact: the scene index occupies the key bits above the Morton code, so no cell, run or
+// block holds particles of two scenes.
+struct SceneInfo {
+    float inv_cell;       // 1 / max(cell size, 1e-4) of the scene
+    uint32_t key_prefix;  // scene index << (3 * morton_bits)
+    uint32_t col0;        // first collider of the scene in the visit-ordered collider array
+    uint32_t ncol;        // colliders of the scene (at most kMaxColliders)
 };
 
 // A static box collider in LBVH visit order, with its broad-phase AABB precomputed.
@@ -58,11 +69,10 @@ struct Collider {
 // Solver-wide scalars of one substep.
 struct SubstepConsts {
     float dt;
-    float inv_cell;       // 1 / max(cell, 1e-4)
+    float inv_cell;       // > 0: 1 / max(cell, 1e-4) for every scene; 0: each scene's own
     float gravity[3];
     uint32_t n;
     uint32_t cap;         // neighbor slots per particle
-    uint32_t ncolliders;  // colliders in visit order
     uint32_t key_mode;
     uint32_t morton_bits; // bits per axis of the Morton key
     uint32_t bucket_shift;
@@ -115,9 +125,10 @@ __device__ __forceinline__ uint32_t cell_morton(int gx, int gy, int gz, uint32_t
     const uint32_t m = (1u << bits) - 1u;
     return (spread3((uint32_t)gx & m) << 2) | (spread3((uint32_t)gy & m) << 1) | spread3((uint32_t)gz & m);
 }
-__device__ __forceinline__ uint32_t cell_key(int gx, int gy, int gz, uint32_t mode, uint32_t bits)
+// Key of a cell of the scene whose key range starts at prefix (0 in hash mode, which holds one scene).
+__device__ __forceinline__ uint32_t cell_key(int gx, int gy, int gz, uint32_t prefix, uint32_t mode, uint32_t bits)
 {
-    return mode == CN_KEY_HASH ? cell_hash(gx, gy, gz) : cell_morton(gx, gy, gz, bits);
+    return mode == CN_KEY_HASH ? cell_hash(gx, gy, gz) : (prefix | cell_morton(gx, gy, gz, bits));
 }
 
 // FluidSpikyKernel / FluidSpikyKernelDerivative / FluidCohesionKernel,

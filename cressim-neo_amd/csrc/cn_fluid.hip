@@ -61,6 +61,7 @@ struct FluidArgs {
     const cn_fluid_material *fm;
     const float *t2;          // per fluid material: d2 < t2 <=> sqrt(d2) < h
     const Collider *col;
+    const SceneInfo *scenes;
     uint32_t *meta;
     uint32_t it;
     uint32_t tile;            // halo particles this launch can stage in LDS
@@ -112,7 +113,7 @@ __device__ __forceinline__ void walk_runs(int4 ci, const FluidArgs &a, const Sub
             for (int dx = -1; dx <= 1; ++dx) {
                 const int tx = ci.x + dx, ty = ci.y + dy, tz = ci.z + dz;
                 if (cell_hash(tx, ty, tz) == kInvalid) continue;
-                const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
+                const int r = find_run(cell_key(tx, ty, tz, (uint32_t)ci.w, c.key_mode, c.morton_bits), a);
                 if (r < 0) continue;
                 const int4 info = a.run_info[r];
                 const bool pure = info.w != 0;
@@ -282,10 +283,11 @@ __device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, con
     float density = d.density;
     const uint32_t bm = a.bnd[i];
     const float bscale = c.cfg[kCfgFluidBndDensity];
-    for (uint32_t k = 0; k < c.ncolliders; ++k) {
+    const SceneInfo sc = a.scenes[x.pc.scene];
+    for (uint32_t k = 0; k < sc.ncol; ++k) {
         if (!((bm >> k) & 1u)) continue;
         float gx, gy, gz;
-        if (!ghost_delta(a.col[k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) continue;
+        if (!ghost_delta(a.col[sc.col0 + k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) continue;
         const float gd = len3(gx, gy, gz);
         if (gd >= x.h) continue;
         density = density + spiky_w_k(gd, x.h, x.kw) * bscale;
@@ -460,10 +462,11 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         float bx = 0.0f, by = 0.0f, bz = 0.0f;
         const float scn = fmaxd(x.Ci, 0.0f);
         const uint32_t bm = a.bnd[i];
-        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+        const SceneInfo sc = a.scenes[pc.scene];
+        for (uint32_t k = 0; k < sc.ncol; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float gx, gy, gz;
-            if (!ghost_delta(a.col[k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) continue;
+            if (!ghost_delta(a.col[sc.col0 + k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) continue;
             const float gd = len3(gx, gy, gz);
             if (gd <= kEps || gd >= x.h) continue;
             const float f = -scn * spiky_dw_k(gd, x.h, x.kw);
@@ -484,12 +487,13 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
     const float r = fmaxd(pc.radius, 0.0f);
     float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
     const uint32_t bm = a.bnd[i];
+    const SceneInfo scb = a.scenes[pc.scene];
     for (int sweep = 0; sweep < 2; ++sweep) {
         bool changed = false;
-        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+        for (uint32_t k = 0; k < scb.ncol; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float sd, nx, ny, nz;
-            if (!box_contact(a.col[k], cx, cy, cz, sd, nx, ny, nz)) continue;
+            if (!box_contact(a.col[scb.col0 + k], cx, cy, cz, sd, nx, ny, nz)) continue;
             if (sd >= r) continue;
             const float push = r - sd;
             cx = cx + nx * push;
@@ -571,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
         if (tid < kHalo) {
             const int tx = bx - 1 + (int)(tid % 6u), ty = by - 1 + (int)((tid / 6u) % 6u), tz = bz - 1 + (int)(tid / 36u);
             if (cell_hash(tx, ty, tz) != kInvalid) {
-                const int r = find_run(cell_key(tx, ty, tz, c.key_mode, c.morton_bits), a);
+                const int r = find_run(cell_key(tx, ty, tz, (uint32_t)c0.w, c.key_mode, c.morton_bits), a);
                 if (r >= 0) {
                     const int4 info = a.run_info[r];
                     if (info.w == 0) bad = true;
@@ -949,6 +953,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.fm = s.fluid_mats;
     a.t2 = s.fluid_t2;
     a.col = s.colliders;
+    a.scenes = s.scenes;
     a.meta = s.meta;
     a.it = it;
     a.tile = 0;

@@ -89,6 +89,8 @@ struct SceneBuffers {
     float4 *part_mats = nullptr;
     Collider *colliders = nullptr;
     uint32_t ncolliders = 0;
+    SceneInfo *scenes = nullptr;    // one record per scene of the pool
+    uint32_t nscenes = 0;
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
     // [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
     uint32_t *meta = nullptr;
@@ -117,6 +119,7 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
 void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
+void launch_fill(uint32_t *p, uint32_t words, uint32_t value, hipStream_t st);
 
 // ---- XPBD soft constraints (cn_soft.hip) ----
 struct SoftArgs {

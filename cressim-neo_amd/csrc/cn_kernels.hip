@@ -47,10 +47,16 @@ __device__ __forceinline__ int3 cell_of(float x, float y, float z, float inv_cel
 // ---------------------------------------------------------------------------------------------
 // keys: predict (without storing) and emit (cell key, slot) pairs
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float scene_inv_cell(const SceneInfo &sc, const SubstepConsts &c)
+{
+    return c.inv_cell > 0.0f ? c.inv_cell : sc.inv_cell;
+}
+
 __global__ __launch_bounds__(kBlock) void k_keys(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                                                  const uint32_t *__restrict__ cls,
                                                  const ParticleClass *__restrict__ classes,
-                                                 const cn_fluid_material *__restrict__ fm, SubstepConsts c,
+                                                 const cn_fluid_material *__restrict__ fm,
+                                                 const SceneInfo *__restrict__ scenes, SubstepConsts c,
                                                  uint2 *__restrict__ kv)
 {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -58,14 +64,15 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float4 *__restrict__ pos,
     float4 p = pos[s], v = vel[s], pv;
     const ParticleClass pc = classes[c.uniform_class ? 0u : cls[s]];
     predict(p, v, pv, gravity_scale(pc, fm), c);
-    const int3 g = cell_of(p.x, p.y, p.z, c.inv_cell);
-    kv[s] = make_uint2(cell_key(g.x, g.y, g.z, c.key_mode, c.morton_bits), s);
+    const SceneInfo sc = scenes[pc.scene];
+    const int3 g = cell_of(p.x, p.y, p.z, scene_inv_cell(sc, c));
+    kv[s] = make_uint2(cell_key(g.x, g.y, g.z, sc.key_prefix, c.key_mode, c.morton_bits), s);
 }
 
 void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
     hipLaunchKernelGGL(k_keys, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.cls[s.cur],
-                       s.classes, s.fluid_mats, c, s.kv[0]);
+                       s.classes, s.fluid_mats, s.scenes, c, s.kv[0]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -241,7 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     float4 *__restrict__ vel_n, uint32_t *__restrict__ cls_n, uint32_t *__restrict__ id_n,
     float4 *__restrict__ prev, float4 *__restrict__ cur, int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
     uint32_t *__restrict__ slot_of, const ParticleClass *__restrict__ classes,
-    const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, SubstepConsts c)
+    const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, const SceneInfo *__restrict__ scenes,
+    SubstepConsts c)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= c.n) return;
@@ -258,9 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t pid = id_o[o];
     id_n[t] = pid;
     if (slot_of) slot_of[min(pid, c.n - 1u)] = t;
-    const int3 g = cell_of(p.x, p.y, p.z, c.inv_cell);
-    cellc[t] = make_int4(g.x, g.y, g.z, 0);
-    bnd[t] = boundary_mask(pc, fm, col, c.ncolliders, p.x, p.y, p.z);
+    const SceneInfo sc = scenes[pc.scene];
+    const int3 g = cell_of(p.x, p.y, p.z, scene_inv_cell(sc, c));
+    // w: the scene's key prefix, from which neighbor-cell keys are formed
+    cellc[t] = make_int4(g.x, g.y, g.z, (int)sc.key_prefix);
+    bnd[t] = boundary_mask(pc, fm, col + sc.col0, sc.ncol, p.x, p.y, p.z);
 }
 
 // (cx, cy, cz, pure): pure runs hold a single cell; mixed runs (hash collisions or wrapped
@@ -290,7 +300,7 @@ void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
     hipLaunchKernelGGL(k_reorder, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.perm, s.pos[o], s.vel[o], s.cls[o],
                        s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev, s.curpos, s.cellc, s.bnd,
                        s.ne + s.nb + s.nd + s.nt ? s.slot_of : nullptr, s.classes, s.fluid_mats, s.colliders,
-                       c);
+                       s.scenes, c);
     hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
                        s.run_info);
     s.cur = nw;
@@ -370,7 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
                                                      const float4 *__restrict__ dacc, const uint32_t *__restrict__ bnd,
                                                      const ParticleClass *__restrict__ classes,
                                                      const float4 *__restrict__ part_mats,
-                                                     const Collider *__restrict__ col, SubstepConsts c)
+                                                     const Collider *__restrict__ col,
+                                                     const SceneInfo *__restrict__ scenes, SubstepConsts c)
 {
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= c.n) return;
@@ -397,17 +408,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
         vy = fdiv(p.y - pv.y, dt) * scale;
         vz = fdiv(p.z - pv.z, dt) * scale;
     }
-    if (fluid && c.ncolliders > 0) {
+    const SceneInfo sc = scenes[pc.scene];
+    if (fluid && sc.ncol > 0) {
         const float r = fmaxd(pc.radius, 0.0f);
         const float slop = c.cfg[kCfgContactSlop];
         const float damping = fmaxd(pm.z, 0.0f);
         float best = 3.402823466e+38f, bnx = 0.0f, bny = 1.0f, bnz = 0.0f, bmf = 0.0f, bms = 0.0f;
         bool found = false;
         const uint32_t bm = bnd[i];
-        for (uint32_t k = 0; k < c.ncolliders; ++k) {
+        const Collider *scol = col + sc.col0;
+        for (uint32_t k = 0; k < sc.ncol; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float sd, nx, ny, nz;
-            if (!box_contact(col[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
+            if (!box_contact(scol[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
             const float cdist = r + 2.0f * slop;
             if (sd > cdist) continue;
             if (sd < best) {
@@ -415,8 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
                 bnx = nx;
                 bny = ny;
                 bnz = nz;
-                bmf = fsqrt(fmaxd(0.0f, pm.x) * fmaxd(0.0f, col[k].material[0]));
-                bms = fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, col[k].material[3]));
+                bmf = fsqrt(fmaxd(0.0f, pm.x) * fmaxd(0.0f, scol[k].material[0]));
+                bms = fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, scol[k].material[3]));
                 found = true;
             }
         }
@@ -472,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finalize, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
-                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, c);
+                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, s.scenes, c);
 }
 
 // slot order -> particle-index order
@@ -494,6 +507,18 @@ void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st)
     hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
                        s.id[s.cur], s.n, state ? s.out_pos : nullptr, state ? s.out_vel : nullptr,
                        prev ? s.out_prev : nullptr);
+}
+
+// word fill as a kernel (stays a kernel node when the step is captured into a graph)
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t *__restrict__ p, uint32_t words, uint32_t value)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < words) p[i] = value;
+}
+
+void launch_fill(uint32_t *p, uint32_t words, uint32_t value, hipStream_t st)
+{
+    if (words) hipLaunchKernelGGL(k_fill, dim3(grid1(words)), dim3(kBlock), 0, st, p, words, value);
 }
 
 }  // namespace cn

@@ -236,6 +236,28 @@ cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *particles,
 /* PhysicsSolver::step (physics_solver.cpp:336-1401) for the particle path; asynchronous on the
  * context stream. */
 cn_status cn_step(cn_ctx *ctx, const cn_step_params *params);
+/* -------- scene pool (batched path) --------
+ * One independent scene: its particle view, tables and constraints (indices local to the scene). */
+typedef struct cn_scene_desc {
+    const cn_particle_soa_view *particles;
+    const cn_scene_tables *tables;
+    const cn_constraint_view *constraints;   /* may be NULL */
+} cn_scene_desc;
+/* Uploads scene_count independent scenes into one pool (no reference counterpart: the reference
+ * steps one world per solver; its batched RL environments loop worlds or co-locate them by
+ * environment index, docs/source/guides/batched-environments.md).  Particles of scene k occupy
+ * pool indices [first_k, first_k + count_k) in upload order; every scene keeps its own cell size,
+ * static colliders and key range, so each scene's results equal those of uploading it alone.
+ * cn_step then steps every scene with one launch per pass; cn_readback and cn_device_views_get
+ * address the whole pool.  Requires CN_KEY_MORTON when scene_count > 1. */
+cn_status cn_batch_upload(cn_ctx *ctx, uint32_t scene_count, const cn_scene_desc *scenes);
+/* Pool index range of one uploaded scene. */
+cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *first, uint32_t *count);
+/* enable != 0: cn_step captures its kernel sequence into a HIP graph on the first call with given
+ * parameters and replays it afterwards (no reference counterpart: the reference records Vulkan
+ * command lists per step).  Uploads and reserves drop the cached graphs.  Off by default. */
+cn_status cn_graph_enable(cn_ctx *ctx, int enable);
+
 /* Blocking readback (physics_scene_gpu_state.cpp:4082-4142). Any pointer may be NULL. */
 cn_status cn_readback(cn_ctx *ctx, float *positions_inv_mass, float *previous_positions,
                       float *velocities, uint32_t count);
