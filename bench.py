@@ -117,6 +117,64 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int):
             "speedup_vs_loop_plain": env / env_loop, "graph": True, "neighbor_truncations": trunc}
 
 
+def serve_bench(sessions: int, rates, duration: float, shards: int, loop_requests: int):
+    """Served workload: `sessions` scene sessions sized as the batched workload (seed 3), all
+    admitted under a 200 GB budget, Poisson step requests at each rate for `duration` seconds,
+    continuous batching over `shards` pools; beside it, the plain path serving requests one at a
+    time (one context per session, stepped and synchronized per request)."""
+    import numpy as np
+    import cnhip
+    from cnhip import scenes, serving
+    t0 = time.perf_counter()
+    ses = scenes.batched(sessions)
+    total = sum(sc.n for sc in ses)
+    p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
+    print(f"bench: {sessions} sessions, {total} particles (built in {time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    srv = serving.Server(ses, p, budget_bytes=200 << 30, shards=shards, graphs=True)
+    out = {"workload": f"{sessions} sessions sized as the batched workload, Poisson arrivals, 200 GB budget",
+           "sessions": sessions, "admitted": len(srv.admitted), "deferred": len(srv.deferred),
+           "resident_bytes_estimate": srv.bytes_used, "particles": total, "shards": len(srv.shards),
+           "rates": {}}
+    for r in rates:
+        arr = serving.poisson_arrivals(r, duration, srv.admitted, seed=4)
+        srv.ticks, srv.tick_sizes = 0, []
+        done = srv.serve(arr)
+        st = serving.latency_stats(done, srv.elapsed)
+        st.update(offered_req_per_s=r, duration_s=duration, ticks=srv.ticks,
+                  mean_tick_sessions=float(np.mean(srv.tick_sizes)) if srv.tick_sizes else 0.0)
+        out["rates"][str(int(r))] = st
+        print(f"bench: serve {r:.0f} req/s -> {st['env_steps_per_s']:.0f} env-steps/s, p50 {st['p50_ms']:.1f} ms, "
+              f"p99 {st['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+    srv.close()
+    # the plain path, one request at a time
+    sub = ses[:min(len(ses), 256)]
+    ctxs = []
+    for sc in sub:
+        s = cnhip.Solver(0)
+        s.reserve(sc.n, CAP)
+        s.upload(sc)
+        ctxs.append(s)
+    rng = np.random.Generator(np.random.PCG64(4))
+    order = rng.integers(0, len(ctxs), loop_requests)
+    ctxs[0].step(p)
+    ctxs[0].synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for k in order:
+        t = time.perf_counter()
+        ctxs[k].step(p)
+        ctxs[k].synchronize()
+        lat.append(time.perf_counter() - t)
+    el = time.perf_counter() - t0
+    for s in ctxs:
+        s.close()
+    lat_ms = np.array(lat) * 1e3
+    out["loop_plain"] = {"requests": loop_requests, "env_steps_per_s": loop_requests / el,
+                         "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99))}
+    return out
+
+
 def copy_probe(mib: int):
     """A plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters give the
     factor between those counters and bytes moved."""
@@ -144,11 +202,20 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="only run the served workload (2048 sessions)")
+    ap.add_argument("--serve-sessions", type=int, default=2048)
+    ap.add_argument("--serve-duration", type=float, default=2.0)
+    ap.add_argument("--serve-shards", type=int, default=8)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.serve:
+        res = serve_bench(args.serve_sessions, (2000.0, 10000.0, 50000.0), args.serve_duration,
+                          args.serve_shards, 400)
+        print(json.dumps({"metric": "served env-steps/s and p50/p99 step latency", "serve": res}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

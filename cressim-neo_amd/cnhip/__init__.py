@@ -199,9 +199,23 @@ class Solver:
     def step(self, params: abi.StepParams):
         self._check(self.lib.cn_step(self._ctx, C.byref(params)), "cn_step")
 
+    def step_scenes(self, params: abi.StepParams, scenes):
+        """Step only the listed scenes of the pool (cn_step_scenes); the others keep their state."""
+        ids = np.ascontiguousarray(np.asarray(scenes, np.uint32))
+        self._check(self.lib.cn_step_scenes(self._ctx, C.byref(params), len(ids), ids.ctypes.data if len(ids) else None),
+                    "cn_step_scenes")
+
     def graphs(self, enable: bool):
         """Capture cn_step into a HIP graph and replay it (cn_graph_enable)."""
         self._check(self.lib.cn_graph_enable(self._ctx, 1 if enable else 0), "cn_graph_enable")
+
+    def done(self) -> bool:
+        """True when the context's enqueued work has finished (cn_query, non-blocking)."""
+        st = self.lib.cn_query(self._ctx)
+        if st == abi.CN_ERR_NOT_READY:
+            return False
+        self._check(st, "cn_query")
+        return True
 
     def synchronize(self):
         self._check(self.lib.cn_synchronize(self._ctx), "cn_synchronize")

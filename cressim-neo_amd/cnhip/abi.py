@@ -139,10 +139,12 @@ SIGNATURES = {
     "cn_batch_upload": (C.c_int, [_P, _U, C.POINTER(SceneDesc)]),
     "cn_batch_scene_range": (C.c_int, [_P, _U, C.POINTER(_U), C.POINTER(_U)]),
     "cn_graph_enable": (C.c_int, [_P, C.c_int]),
+    "cn_step_scenes": (C.c_int, [_P, C.POINTER(StepParams), _U, _P]),
     "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
     "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),
     "cn_validate_meta": (C.c_int, [_P, C.POINTER(Meta)]),
     "cn_synchronize": (C.c_int, [_P]),
+    "cn_query": (C.c_int, [_P]),
     "cn_pass_broadphase_keys": (C.c_int, [_P, _U, _P, C.c_float, _P, _P]),
     "cn_pass_sort_pairs": (C.c_int, [_P, _U, _P, _P]),
     "cn_pass_cell_ranges": (C.c_int, [_P, _U, _P, _U, _P]),

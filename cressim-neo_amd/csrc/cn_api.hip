@@ -49,6 +49,7 @@ struct cn_ctx {
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
+    std::vector<uint32_t> active_host;  // last per-scene active flags written to the device
     // profiling
     bool profiling = false;
     struct Ev {
@@ -104,7 +105,10 @@ void free_scene(SceneBuffers &s)
         dfree(s.kv[k]);
         dfree(s.dacc[k]);
     }
-    dfree(s.prev);
+    dfree(s.prev[0]);
+    dfree(s.prev[1]);
+    dfree(s.scene_active);
+    dfree(s.pid_scene);
     dfree(s.cellc);
     dfree(s.bnd);
     dfree(s.sort_scratch);
@@ -541,7 +545,8 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
         CN_HIP(ctx, dalloc(&s.kv[k], N));
         CN_HIP(ctx, dalloc(&s.dacc[k], N));
     }
-    CN_HIP(ctx, dalloc(&s.prev, N));
+    CN_HIP(ctx, dalloc(&s.prev[0], N));
+    CN_HIP(ctx, dalloc(&s.prev[1], N));
     CN_HIP(ctx, dalloc(&s.cellc, N));
     CN_HIP(ctx, dalloc(&s.bnd, N));
     CN_HIP(ctx, dalloc(&s.sort_scratch, sort_scratch_words((uint32_t)N)));
@@ -794,6 +799,10 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     dfree(s.colliders);
     dfree(s.scenes);
     dfree(s.fluid_t2);
+    dfree(s.scene_active);
+    dfree(s.pid_scene);
+    s.scene_active = nullptr;
+    s.pid_scene = nullptr;
     s.classes = nullptr;
     s.fluid_mats = nullptr;
     s.part_mats = nullptr;
@@ -815,6 +824,17 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     s.nscenes = count;
     CN_HIP(ctx, dalloc(&s.scenes, sinfo.size()));
     CN_HIP(ctx, hipMemcpy(s.scenes, sinfo.data(), sinfo.size() * sizeof(SceneInfo), hipMemcpyHostToDevice));
+    {
+        const std::vector<uint32_t> ones(count, 1u);
+        CN_HIP(ctx, dalloc(&s.scene_active, count));
+        CN_HIP(ctx, hipMemcpy(s.scene_active, ones.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice));
+        ctx->active_host.assign(count, 1u);
+        std::vector<uint32_t> ps(n);
+        for (uint32_t k = 0; k < count; ++k)
+            for (uint32_t i = ctx->scene_first[k]; i < ctx->scene_first[k + 1]; ++i) ps[i] = k;
+        CN_HIP(ctx, dalloc(&s.pid_scene, n));
+        CN_HIP(ctx, hipMemcpy(s.pid_scene, ps.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     ctx->max_h = 1.0e-4f;
     ctx->has_tension = false;
     std::vector<float> t2(fmh.size());
@@ -836,6 +856,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, hipMemcpy(s.out_pos, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.out_vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.out_prev, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    CN_HIP(ctx, hipMemcpy(s.prev[0], pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
 
     // soft constraints, their incidence lists and the particle -> slot map
     free_constraints(s);
@@ -969,6 +990,8 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         sa.inc_b_ref = s.inc + s.inc_off[3];
         sa.inc_t_start = s.inc + s.inc_off[4];
         sa.inc_t_ref = s.inc + s.inc_off[5];
+        sa.active = s.scene_active;
+        sa.pid_scene = s.pid_scene;
     }
     for (uint32_t sub = 0; sub < substeps; ++sub) {
         broadphase(ctx, c, key_bits);
@@ -1010,7 +1033,42 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
 
 extern "C" {
 
+static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p);
+
+// Writes the per-scene active flags to the device when they differ from the last ones written.
+static cn_status set_active(cn_ctx *ctx, const std::vector<uint32_t> &flags)
+{
+    if (flags == ctx->active_host) return CN_OK;
+    ctx->active_host = flags;
+    CN_HIP(ctx, hipMemcpyAsync(ctx->s.scene_active, ctx->active_host.data(), flags.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return CN_OK;
+}
+
 cn_status cn_step(cn_ctx *ctx, const cn_step_params *p)
+{
+    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step before cn_scene_upload");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    const cn_status st = set_active(ctx, std::vector<uint32_t>(ctx->nscenes, 1u));
+    return st != CN_OK ? st : step_impl(ctx, p);
+}
+
+cn_status cn_step_scenes(cn_ctx *ctx, const cn_step_params *p, uint32_t count, const uint32_t *scenes)
+{
+    if (!ctx || !p || (count && !scenes)) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step_scenes before an upload");
+    std::vector<uint32_t> flags(ctx->nscenes, 0u);
+    for (uint32_t k = 0; k < count; ++k) {
+        if (scenes[k] >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
+        flags[scenes[k]] = 1u;
+    }
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    const cn_status st = set_active(ctx, flags);
+    return st != CN_OK ? st : step_impl(ctx, p);
+}
+
+static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
 {
     if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
     if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_step before cn_scene_upload");
@@ -1093,6 +1151,15 @@ cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out)
     out->count = ctx->s.n;
     out->stream = ctx->stream;
     return CN_OK;
+}
+
+cn_status cn_query(cn_ctx *ctx)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    const hipError_t e = hipStreamQuery(ctx->stream);
+    if (e == hipSuccess) return CN_OK;
+    if (e == hipErrorNotReady) return CN_ERR_NOT_READY;
+    return fail(ctx, CN_ERR_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
 }
 
 cn_status cn_synchronize(cn_ctx *ctx)

@@ -62,6 +62,7 @@ struct FluidArgs {
     const float *t2;          // per fluid material: d2 < t2 <=> sqrt(d2) < h
     const Collider *col;
     const SceneInfo *scenes;
+    const uint32_t *active;   // per scene: stepped this cn_step
     uint32_t *meta;
     uint32_t it;
     uint32_t tile;            // halo particles this launch can stage in LDS
@@ -309,6 +310,7 @@ __device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, con
 template <bool UNIFORM>
 __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
 {
+    if (!a.active[a.classes[UNIFORM ? 0u : a.cls[i]].scene]) return;
     DensitySelf x;
     if (!density_begin(x, i, a.cur[i], UNIFORM ? 0u : a.cls[i], a)) return;
     DensitySum d;
@@ -517,6 +519,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
 template <bool UNIFORM>
 __device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, const SubstepConsts &c)
 {
+    if (!a.active[a.classes[UNIFORM ? 0u : a.cls[i]].scene]) return 0.0f;
     const int4 ci = a.cellc[i];
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return delta_particle<UNIFORM>(
@@ -605,6 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
             hd[kHdBase] = (uint32_t)bx;
             hd[kHdBase + 1] = (uint32_t)by;
             hd[kHdBase + 2] = (uint32_t)bz;
+            hd[kHdScene] = a.classes[a.cls[p0]].scene;
         }
     }
 }
@@ -638,6 +642,11 @@ __device__ __forceinline__ bool tile_begin(Next &&next, uint32_t *s_blk, uint32_
         const uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
         ts.p0 = hd[kHdP0];
         ts.p1 = hd[kHdP1];
+        if (!a.active[hd[kHdScene]]) {  // a scene this step leaves untouched
+            ts.skip = true;
+            __syncthreads();
+            return true;
+        }
         const uint32_t total = hd[kHdOff + kHalo];
         const bool pure = hd[kHdFlags] != 0u;
         ts.lds = pure && total <= a.tile;
@@ -746,6 +755,10 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     MetaAcc ma;
     TileState ts;
     while (tile_begin([&] { return bq.next(queue, nblk); }, &s_blk, hoff, hstart, ts, 0u, a, c)) {
+        if (ts.skip) {
+            __syncthreads();
+            continue;
+        }
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             for (uint32_t f = tid; f < total; f += kTileBlock) {
@@ -954,6 +967,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.t2 = s.fluid_t2;
     a.col = s.colliders;
     a.scenes = s.scenes;
+    a.active = s.scene_active;
     a.meta = s.meta;
     a.it = it;
     a.tile = 0;

@@ -29,6 +29,7 @@ constexpr uint32_t kHdFlags = 2 * kHalo + 1;
 constexpr uint32_t kHdP0 = kHdFlags + 1;
 constexpr uint32_t kHdP1 = kHdFlags + 2;
 constexpr uint32_t kHdBase = kHdFlags + 3;      // 3 words
+constexpr uint32_t kHdScene = kHdFlags + 6;
 constexpr uint32_t kHaloStride = 448;
 
 // ---- radix sort (cn_sort.hip) ----
@@ -45,7 +46,7 @@ struct SceneBuffers {
     uint32_t *cls[2] = {nullptr, nullptr};
     uint32_t *id[2] = {nullptr, nullptr};
     int cur = 0;
-    float4 *prev = nullptr;
+    float4 *prev[2] = {nullptr, nullptr};  // position before the last predict, in slot order
     int4 *cellc = nullptr;          // cell coordinates of each slot (w unused)
     uint32_t *bnd = nullptr;        // boundary-candidate bitmask over colliders in visit order
     // broadphase
@@ -91,6 +92,8 @@ struct SceneBuffers {
     uint32_t ncolliders = 0;
     SceneInfo *scenes = nullptr;    // one record per scene of the pool
     uint32_t nscenes = 0;
+    uint32_t *scene_active = nullptr;  // per scene: 1 = stepped by cn_step, 0 = state left untouched
+    uint32_t *pid_scene = nullptr;     // scene of each particle index
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
     // [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
     uint32_t *meta = nullptr;
@@ -145,6 +148,8 @@ struct SoftArgs {
     const uint32_t *inc_e_start, *inc_e_ref;  // incidence: ref = (constraint << 2) | slot
     const uint32_t *inc_b_start, *inc_b_ref;
     const uint32_t *inc_t_start, *inc_t_ref;
+    const uint32_t *active;    // per scene
+    const uint32_t *pid_scene; // particle index -> scene
 };
 void launch_soft_clear(const SoftArgs &a, hipStream_t st);
 void launch_soft_iteration(const SoftArgs &a, hipStream_t st);

@@ -56,14 +56,15 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float4 *__restrict__ pos,
                                                  const uint32_t *__restrict__ cls,
                                                  const ParticleClass *__restrict__ classes,
                                                  const cn_fluid_material *__restrict__ fm,
-                                                 const SceneInfo *__restrict__ scenes, SubstepConsts c,
+                                                 const SceneInfo *__restrict__ scenes,
+                                                 const uint32_t *__restrict__ active, SubstepConsts c,
                                                  uint2 *__restrict__ kv)
 {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= c.n) return;
     float4 p = pos[s], v = vel[s], pv;
     const ParticleClass pc = classes[c.uniform_class ? 0u : cls[s]];
-    predict(p, v, pv, gravity_scale(pc, fm), c);
+    if (active[pc.scene]) predict(p, v, pv, gravity_scale(pc, fm), c);
     const SceneInfo sc = scenes[pc.scene];
     const int3 g = cell_of(p.x, p.y, p.z, scene_inv_cell(sc, c));
     kv[s] = make_uint2(cell_key(g.x, g.y, g.z, sc.key_prefix, c.key_mode, c.morton_bits), s);
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float4 *__restrict__ pos,
 void launch_keys(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
     hipLaunchKernelGGL(k_keys, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.cls[s.cur],
-                       s.classes, s.fluid_mats, s.scenes, c, s.kv[0]);
+                       s.classes, s.fluid_mats, s.scenes, s.scene_active, c, s.kv[0]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -246,10 +247,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t *__restrict__ perm, const float4 *__restrict__ pos_o, const float4 *__restrict__ vel_o,
     const uint32_t *__restrict__ cls_o, const uint32_t *__restrict__ id_o, float4 *__restrict__ pos_n,
     float4 *__restrict__ vel_n, uint32_t *__restrict__ cls_n, uint32_t *__restrict__ id_n,
-    float4 *__restrict__ prev, float4 *__restrict__ cur, int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
+    const float4 *__restrict__ prev_o, float4 *__restrict__ prev, float4 *__restrict__ cur,
+    int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
     uint32_t *__restrict__ slot_of, const ParticleClass *__restrict__ classes,
     const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, const SceneInfo *__restrict__ scenes,
-    SubstepConsts c)
+    const uint32_t *__restrict__ active, SubstepConsts c)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= c.n) return;
@@ -257,7 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     float4 p = pos_o[o], v = vel_o[o], pv;
     const uint32_t k = cls_o[o];
     const ParticleClass pc = classes[c.uniform_class ? 0u : k];
-    predict(p, v, pv, gravity_scale(pc, fm), c);
+    // a scene this step leaves untouched keeps its state; its particles only change slot
+    if (active[pc.scene]) predict(p, v, pv, gravity_scale(pc, fm), c);
+    else pv = prev_o[o];
     pos_n[t] = p;
     cur[t] = p;  // solver position x* + delta with delta = 0 at the first iteration
     vel_n[t] = v;
@@ -298,9 +302,10 @@ void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
     const int o = s.cur, nw = 1 - s.cur;
     hipLaunchKernelGGL(k_reorder, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.perm, s.pos[o], s.vel[o], s.cls[o],
-                       s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev, s.curpos, s.cellc, s.bnd,
+                       s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev[o], s.prev[nw], s.curpos, s.cellc,
+                       s.bnd,
                        s.ne + s.nb + s.nd + s.nt ? s.slot_of : nullptr, s.classes, s.fluid_mats, s.colliders,
-                       s.scenes, c);
+                       s.scenes, s.scene_active, c);
     hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
                        s.run_info);
     s.cur = nw;
@@ -381,11 +386,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
                                                      const ParticleClass *__restrict__ classes,
                                                      const float4 *__restrict__ part_mats,
                                                      const Collider *__restrict__ col,
-                                                     const SceneInfo *__restrict__ scenes, SubstepConsts c)
+                                                     const SceneInfo *__restrict__ scenes,
+                                                     const uint32_t *__restrict__ active, SubstepConsts c)
 {
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= c.n) return;
     const ParticleClass pc = classes[c.uniform_class ? 0u : cls[i]];
+    if (!active[pc.scene]) return;
     float4 p = pos[i];
     float vx, vy, vz;
     const float dt = c.dt;
@@ -484,8 +491,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
 
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
-                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, s.scenes, c);
+    hipLaunchKernelGGL(k_finalize, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev[s.cur],
+                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, s.scenes, s.scene_active, c);
 }
 
 // slot order -> particle-index order
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ po
 
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev,
+    hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev[s.cur],
                        s.id[s.cur], s.n, state ? s.out_pos : nullptr, state ? s.out_vel : nullptr,
                        prev ? s.out_prev : nullptr);
 }

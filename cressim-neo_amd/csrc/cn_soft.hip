@@ -20,6 +20,9 @@ static inline uint32_t grid_n(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 __device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.pos[a.slot_of[particle]]; }
 
+// Constraints and particles of scenes that this step leaves untouched are skipped (records kept).
+__device__ __forceinline__ bool live(const SoftArgs &a, uint32_t particle) { return a.active[a.pid_scene[particle]] != 0u; }
+
 __device__ __forceinline__ void cross3d(float ax, float ay, float az, float bx, float by, float bz, float &ox, float &oy,
                                         float &oz)
 {
@@ -41,6 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= a.ne) return;
     const cn_soft_edge E = a.edges[e];
+    if (!live(a, E.a)) return;
     float4 *out = a.corr_e + 2 * (size_t)e;
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (E.enabled == 0u) {
@@ -86,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= a.nb) return;
     const cn_soft_bend B = a.bends[b];
+    if (!live(a, B.p0)) return;
     float4 *out = a.corr_b + 4 * (size_t)b;
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 p0 = ldp(a, B.p0), p1 = ldp(a, B.p1), p2 = ldp(a, B.p2);
@@ -143,6 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     if (d >= a.nd) return;
     const uint32_t ci = a.nb + d;
     const cn_cloth_dihedral D = a.dih[d];
+    if (!live(a, D.e0)) return;
     float4 *out = a.corr_b + 4 * (size_t)ci;
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 e0m = ldp(a, D.e0), e1m = ldp(a, D.e1), o0m = ldp(a, D.o0), o1m = ldp(a, D.o1);
@@ -202,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.nt) return;
     const cn_soft_tet T = a.tets[t];
+    if (!live(a, T.p[0])) return;
     const float4 p0 = ldp(a, T.p[0]), p1 = ldp(a, T.p[1]), p2 = ldp(a, T.p[2]), p3 = ldp(a, T.p[3]);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
     if ((w0 + w1 + w2 + w3) <= kEps) return;  // the record keeps its previous value
@@ -246,6 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     float4 p = a.pos[t];
     if (p.w <= kEps) return;
     const uint32_t pid = a.id[t];
+    if (!live(a, pid)) return;
     const uint32_t k0 = start[pid], k1 = start[pid + 1];
     if (k0 == k1) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;

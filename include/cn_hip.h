@@ -251,6 +251,10 @@ typedef struct cn_scene_desc {
  * cn_step then steps every scene with one launch per pass; cn_readback and cn_device_views_get
  * address the whole pool.  Requires CN_KEY_MORTON when scene_count > 1. */
 cn_status cn_batch_upload(cn_ctx *ctx, uint32_t scene_count, const cn_scene_desc *scenes);
+/* cn_step for the listed scenes of the pool only: every other scene's state is left bit for bit
+ * untouched (serving: one launch per pass for the scenes that have a pending step request).  No
+ * reference counterpart. */
+cn_status cn_step_scenes(cn_ctx *ctx, const cn_step_params *params, uint32_t count, const uint32_t *scenes);
 /* Pool index range of one uploaded scene. */
 cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *first, uint32_t *count);
 /* enable != 0: cn_step captures its kernel sequence into a HIP graph on the first call with given
@@ -265,6 +269,9 @@ cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
 /* validateGpuMetaBlocking analog; returns CN_ERR_CAPACITY if a capacity overflowed. */
 cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out);
 cn_status cn_synchronize(cn_ctx *ctx);
+/* Non-blocking: CN_OK when all work enqueued on the context stream has finished, CN_ERR_NOT_READY
+ * while some is still running (no reference counterpart; used by the serving loop). */
+cn_status cn_query(cn_ctx *ctx);
 
 /* -------- per-pass entries (host in/out; used by parity tests) -------- */
 /* buildParticleBroadPhaseEntries + buildParticleBroadPhaseKeys

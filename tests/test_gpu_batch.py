@@ -84,3 +84,30 @@ def test_graph_replays_equal_eager_steps(substeps):
                 np.testing.assert_array_equal(x, y, err_msg=f"step {step}")
             eager.meta()
             graphed.meta()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_step_scenes_leaves_others_untouched(solver, graphs):
+    """cn_step_scenes advances only the listed scenes; each scene ends where stepping it alone the
+    same number of times ends, bit for bit."""
+    batch = small_batch()
+    p = params()
+    ticks = [[0, 2, 4], [1], [0, 1, 2, 3, 4, 5, 6], [], [6, 3], [0]]
+    counts = np.zeros(len(batch), int)
+    with cnhip.Solver(0) as pool:
+        pool.reserve(sum(sc.n for sc in batch), 64)
+        pool.upload_batch(batch)
+        pool.graphs(graphs)
+        for t in ticks:
+            pool.step_scenes(p, t)
+            counts[t] += 1
+        got = [pool.readback_scene(k) for k in range(len(batch))]
+    for k, sc in enumerate(batch):
+        solver.reserve(sc.n, 64)
+        solver.upload(sc)
+        for _ in range(counts[k]):
+            solver.step(p)
+        pos, prev, vel = solver.readback()
+        np.testing.assert_array_equal(got[k][0], pos, err_msg=f"scene {k}")
+        np.testing.assert_array_equal(got[k][1], prev, err_msg=f"scene {k} prev")
+        np.testing.assert_array_equal(got[k][2], vel, err_msg=f"scene {k}")
