@@ -99,7 +99,7 @@ class StepParams(C.Structure):
 class Meta(C.Structure):
     _fields_ = [("max_fluid_neighbor_count", C.c_uint32),
                 ("fluid_neighbor_truncations", C.c_uint32),
-                ("max_cell_occupancy", C.c_uint32), ("reserved0", C.c_uint32),
+                ("max_cell_occupancy", C.c_uint32), ("contact_overflows", C.c_uint32),
                 ("fluid_neighbor_total", C.c_uint64), ("last_particle_count", C.c_uint32),
                 ("lds_tiles", C.c_uint32), ("run_table_tiles", C.c_uint32),
                 ("max_displacement2_bits", C.c_uint32), ("reserved", C.c_uint32 * 2)]

@@ -125,9 +125,14 @@ def compose(parts, fluid_mats, colliders, name, info=None) -> Scene:
         off += len(p["pos"])
     dts = {"edges": au.EDGE_DTYPE, "bends": au.BEND_DTYPE, "dihedrals": au.DIHEDRAL_DTYPE, "tets": au.TET_DTYPE}
     c = {k: (np.concatenate(v) if v else np.zeros(0, dts[k])) for k, v in cons.items()}
+    # one phase group per part (packParticlePhase, include/physics/particle_phase.h:16-27): soft
+    # bodies and cloths without self collision, fluids with the self-collide flag
+    # (physics_world.cpp:5083,5183,5381)
+    phase = np.concatenate([np.full(len(p["pos"]), (k + 1) | (0x80000000 if np.all(p["kinds"] == 1) else 0), np.uint32)
+                            for k, p in enumerate(parts)])
     return Scene(pos=cat["pos"], vel=np.zeros((n, 4), np.float32), radii=cat["radii"], kinds=cat["kinds"],
                  fluid_mat=cat["fluid_mat"], part_mat=cat["part_mat"], env=cat["env"],
-                 phase=np.zeros(n, np.uint32), layer=np.ones(n, np.uint32), mask=np.full(n, 0xFFFFFFFF, np.uint32),
+                 phase=phase, layer=np.ones(n, np.uint32), mask=np.full(n, 0xFFFFFFFF, np.uint32),
                  fluid_mats=fluid_mats, part_mats=np.array([SOFT_CONTACT], np.float32), colliders=colliders,
                  name=name, info=info or {}, edges=c["edges"], bends=c["bends"], dihedrals=c["dihedrals"],
                  tets=c["tets"])
@@ -180,6 +185,23 @@ def mixed(fluid_side=64, tet_res=32, cloth_n=256, seed=1) -> Scene:
     cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
     return compose([soft, cloth, fluid], fm, cols, f"mixed_{fluid_side}_{tet_res}_{cloth_n}",
                    dict(fluid=fluid_side ** 3, soft=tet_res ** 3, cloth=cloth_n ** 2))
+
+
+def contact_stack(block_res=6, cloth_n=24, fluid_side=6) -> Scene:
+    """Particle-particle contact workload: a tet soft block (spacing 0.05 m, r = 0.025 m) lying on a
+    cloth (spacing 0.03 m, r = 0.015 m) with a fluid block (spacing 0.05 m, r = 0.03 m) on top, every
+    layer overlapping the next by 0.01 m at the start, in a 4 x 2 x 2 m container."""
+    cloth = cloth_part(cloth_n, 0.03, 0.015, translation=(0.7, 0.30, 0.7))
+    half = 0.5 * (block_res - 1) * 0.05
+    c = 0.7 + 0.5 * (cloth_n - 1) * 0.03  # block centred over the cloth (the tet grid is centred)
+    soft = soft_block_part(block_res, 0.05, 0.025, translation=(c, 0.30 + 0.03 + half, c))
+    top = 0.30 + 0.03 + 2 * half
+    # fluid columns over the block's vertex columns (fluid_part's lattice starts half a spacing in)
+    fluid = fluid_part(fluid_side, 0.05, 0.03, origin=(c - half - 0.025, top + 0.045 - 0.025, c - half - 0.025))
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    return compose([soft, cloth, fluid], fluid_material(0.03), cols,
+                   f"contact_stack_{block_res}_{cloth_n}_{fluid_side}",
+                   dict(soft=block_res ** 3, cloth=cloth_n ** 2, fluid=fluid_side ** 3))
 
 
 def scaled_box_for(n: int, n_ref: int = 4_194_304, box_ref=(4.0, 2.0, 2.0)):

@@ -34,6 +34,7 @@ struct cn_ctx {
     float max_h = 0.0f;              // largest fluid smoothing radius of the scene
     bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
     bool has_tension = false;        // any fluid material with surface tension
+    bool has_solid = false;          // any solid particle: particle-particle contacts can exist
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
     // scene pool
     uint32_t nscenes = 0;
@@ -133,6 +134,11 @@ void free_scene(SceneBuffers &s)
     dfree(s.lcnt);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
+    dfree(s.ct_list);
+    dfree(s.ct_count);
+    dfree(s.ct_rec);
+    dfree(s.ct_bits);
+    dfree(s.ct_corr);
     dfree(s.out_pos);
     dfree(s.out_vel);
     dfree(s.out_prev);
@@ -629,6 +635,9 @@ cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
             return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid particle with an invalid fluid material index");
         if (pv->particle_material_indices[i] >= tb->particle_material_count)
             return fail(ctx, CN_ERR_INVALID_ARGUMENT, "invalid particle material index");
+        // self collision inside a phase group needs the group's constraint adjacency
+        if (kind == CN_PARTICLE_KIND_SOLID && pv->phases && (pv->phases[i] & 0x80000000u))
+            return fail(ctx, CN_ERR_UNSUPPORTED, "self-colliding solid phase group");
     }
     return CN_OK;
 }
@@ -682,6 +691,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->scene_first.assign(count + 1, 0);
     ctx->scene_cell.assign(count, 0.0f);
     ctx->has_fluid = false;
+    ctx->has_solid = false;
     uint32_t base = 0;
     double max_span = 1.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -709,6 +719,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
             pc.scene = k;
             ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
+            ctx->has_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             std::array<uint32_t, 9> key;
             std::memcpy(key.data(), &pc, sizeof(pc));
             if (last_id != 0xffffffffu && key == last) {
@@ -858,6 +869,26 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, hipMemcpy(s.out_prev, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.prev[0], pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
 
+    // particle-particle contact state: candidate lists and the last contacts, kContactPartners
+    // per particle (only a scene with a solid particle has contacts)
+    if (ctx->has_solid && n > 1 && s.ct_alloc < n) {
+        dfree(s.ct_list);
+        dfree(s.ct_count);
+        dfree(s.ct_rec);
+        dfree(s.ct_bits);
+        dfree(s.ct_corr);
+        s.ct_list = s.ct_count = s.ct_bits = nullptr;
+        s.ct_rec = s.ct_corr = nullptr;
+        s.ct_alloc = 0;
+        const size_t m = std::max<size_t>(n, ctx->capacity);
+        CN_HIP(ctx, dalloc(&s.ct_list, m * kContactPartners));
+        CN_HIP(ctx, dalloc(&s.ct_rec, m * kContactPartners));
+        CN_HIP(ctx, dalloc(&s.ct_count, m));
+        CN_HIP(ctx, dalloc(&s.ct_bits, m));
+        CN_HIP(ctx, dalloc(&s.ct_corr, m));
+        s.ct_alloc = m;
+    }
+
     // soft constraints, their incidence lists and the particle -> slot map
     free_constraints(s);
     if (!edges.empty() || !bends.empty() || !dih.empty() || !tets.empty()) {
@@ -964,6 +995,8 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
     const uint32_t contact_iters = resolve_iters(p->soft_contact_iterations, def);
     const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
     const bool has_soft = s.ne + s.nb + s.nd + s.nt > 0;
+    // particle-particle contacts need a solid particle (fluid pairs are never candidates)
+    const bool has_contacts = contact_iters > 0 && s.n > 1 && ctx->has_solid && s.ct_alloc >= s.n;
     SoftArgs sa{};
     if (has_soft) {
         sa.slot_of = s.slot_of;
@@ -1000,6 +1033,10 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
             sa.id = s.id[s.cur];
             launch_soft_clear(sa, ctx->stream);
         }
+        if (has_contacts) {
+            ProfScope ps(ctx, "contact_partners");
+            launch_contact_partners(s, c, ctx->stream);
+        }
         int din = 0;
         for (uint32_t it = 0; it < max_iters; ++it) {
             if (ctx->has_fluid && it < fluid_iters) {
@@ -1017,10 +1054,18 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
                 ProfScope ps(ctx, "soft");
                 launch_soft_iteration(sa, ctx->stream);
             }
+            if (has_contacts && it < contact_iters) {
+                ProfScope ps(ctx, "contacts");
+                launch_contact_iteration(s, c, din, ctx->stream);
+            }
         }
         {
             ProfScope ps(ctx, "finalize");
             launch_finalize(s, c, din, ctx->stream);
+        }
+        if (has_contacts) {
+            ProfScope ps(ctx, "contact_velocity");
+            for (uint32_t v = 0; v < contact_iters; ++v) launch_contact_velocity(s, c, ctx->stream);
         }
     }
     {
@@ -1185,13 +1230,15 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     std::memcpy(&tot, &m[2], sizeof(tot));
     r.fluid_neighbor_total = tot;
     r.last_particle_count = ctx->s.n;
-    r.reserved0 = 0;
+    r.contact_overflows = m[4];
     r.lds_tiles = m[5];
     r.run_table_tiles = m[6];
     r.max_displacement2_bits = m[7];
     if (out) *out = r;
     if (r.fluid_neighbor_truncations)
         return fail(ctx, CN_ERR_CAPACITY, "fluid neighbor list truncated at max_fluid_neighbors");
+    if (r.contact_overflows)
+        return fail(ctx, CN_ERR_CAPACITY, "particle with more contact candidates than its list holds");
     return CN_OK;
 }
 

@@ -1,0 +1,359 @@
+// cn_contact.hip -- particle-particle contacts on gfx950.
+//
+// Reference passes: buildParticleParticleCandidatePairs (physics_pass_dispatcher.cpp:1102-1197,
+// physics_particle_{count,emit}_particle_particle_candidates.cs.hlsl with
+// IsValidParticleParticleCandidate, physics_particle_broad_phase_query.hlsli:65-117) once per
+// substep from the predicted positions; per contact iteration
+// physics_particle_generate_explicit_contacts.cs.hlsl:80-189, the compaction of active contacts,
+// physics_particle_solve_explicit_contacts.cs.hlsl:25-246 and
+// physics_particle_apply_position_corrections.cs.hlsl:8-30; after the velocity update
+// physics_particle_solve_contact_velocities.cs.hlsl:26-170 and
+// physics_particle_apply_contact_velocities.cs.hlsl per velocity iteration.
+//
+// The reference emits each pair once (lower particle index first) and accumulates both sides'
+// corrections with float CAS atomics, in no fixed order.  Here every particle keeps its candidate
+// partners in its own 27-cell walk order, evaluates each pair's contact itself (the arithmetic of
+// the pair is the same from either side: lower index first) and sums its side's corrections in
+// that order, so results are deterministic and equal to the oracle's.  Contacts are Jacobi: a
+// solve pass writes every particle's summed correction, an apply pass adds them.  Only particle-
+// particle contacts of free particles exist here (no rigid proxies, no suturing).
+#include <hip/hip_runtime.h>
+
+#include "cn_internal.h"
+#include "cn_walk.h"
+
+namespace cn {
+
+constexpr uint32_t kPhaseGroupMask = 0x7fffffffu;
+constexpr int kCfgSoftContactRelax = 2;
+constexpr int kCfgSoftMaxCorrection = 3;
+constexpr int kCfgRestitutionVelocity = 4;
+constexpr int kCfgRestitutionPenetration = 5;
+
+struct ContactArgs {
+    float4 *pos;              // slot order
+    float4 *vel;
+    const float4 *prev;
+    float4 *cur;              // fluid solver positions x* + delta
+    const float4 *dacc;       // current fluid delta
+    const uint32_t *cls;
+    const uint32_t *id;
+    const ParticleClass *classes;
+    const float4 *part_mats;
+    const uint32_t *active;   // per scene
+    RunTable rt;
+    uint32_t *plist;          // partner slots, k-major: plist[k * n + t]
+    uint32_t *pcount;
+    float4 *crec;             // contact of each partner: (normal, penetration), k-major
+    uint32_t *cbits;          // bit k: partner k's contact is active
+    float4 *corr;             // summed correction of the pass
+    uint32_t *meta;
+    uint32_t n, pcap;
+};
+
+__device__ __forceinline__ const ParticleClass &pclass(const ContactArgs &a, uint32_t slot)
+{
+    return a.classes[a.cls[slot]];
+}
+
+// IsValidParticleParticleCandidate for the pair (lo, hi) = slots ordered by particle index.
+__device__ __forceinline__ bool pair_valid(const ContactArgs &a, uint32_t lo, uint32_t hi)
+{
+    const ParticleClass &A = pclass(a, lo), &B = pclass(a, hi);
+    if (A.env != B.env) return false;
+    if ((A.mask & B.layer) == 0u || (B.mask & A.layer) == 0u) return false;
+    if (A.kind == CN_PARTICLE_KIND_FLUID && B.kind == CN_PARTICLE_KIND_FLUID) return false;
+    // self collision of one phase group needs adjacency lists; such scenes are refused on upload
+    if (A.kind == CN_PARTICLE_KIND_SOLID && B.kind == CN_PARTICLE_KIND_SOLID &&
+        (A.phase & kPhaseGroupMask) == (B.phase & kPhaseGroupMask))
+        return false;
+    const float4 pa = a.pos[lo], pb = a.pos[hi];
+    const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
+    const float cr = A.radius + B.radius;
+    return dot3(dx, dy, dz, dx, dy, dz) <= cr * cr;
+}
+
+__device__ __forceinline__ void order_pair(const ContactArgs &a, uint32_t t, uint32_t q, uint32_t &lo, uint32_t &hi,
+                                           bool &t_is_hi)
+{
+    t_is_hi = a.id[q] < a.id[t];
+    lo = t_is_hi ? q : t;
+    hi = t_is_hi ? t : q;
+}
+
+// CombineContactMaterial, physics_rigid_solver_shared.hlsli:37-43: (friction, restitution, static)
+__device__ __forceinline__ float3 combine_material(const ContactArgs &a, uint32_t lo, uint32_t hi)
+{
+    const float4 ma = a.part_mats[pclass(a, lo).part_mat], mb = a.part_mats[pclass(a, hi).part_mat];
+    return make_float3(fsqrt(fmaxd(0.0f, ma.x) * fmaxd(0.0f, mb.x)), fmaxd(fmaxd(0.0f, ma.y), fmaxd(0.0f, mb.y)),
+                       fsqrt(fmaxd(0.0f, ma.w) * fmaxd(0.0f, mb.w)));
+}
+
+// SafeNormalize, physics_math.hlsli:6-12, with rsqrt as 1 / sqrt
+__device__ __forceinline__ float3 safe_normalize(float x, float y, float z)
+{
+    const float l2 = dot3(x, y, z, x, y, z);
+    if (l2 <= kEps) return make_float3(0.0f, 1.0f, 0.0f);
+    const float r = fdiv(1.0f, fsqrt(l2));
+    return make_float3(x * r, y * r, z * r);
+}
+
+// ComputePositionFrictionDelta, physics_rigid_solver_shared.hlsli:50-67
+__device__ __forceinline__ float3 friction_delta(const float *cfg, float ex, float ey, float ez, float pen, float kin,
+                                                 float stat)
+{
+    const float td = len3(ex, ey, ez);
+    if (td <= cfg[kCfgPosFrictionMin] || pen <= kEps) return make_float3(0.0f, 0.0f, 0.0f);
+    const float sl = fsat(stat) * pen;
+    if (td <= sl) return make_float3(ex, ey, ez);
+    const float ks = fmind(fdiv(fsat(kin) * pen, td), 1.0f);
+    return make_float3(ex * ks, ey * ks, ez * ks);
+}
+
+// ---------------------------------------------------------------------------------------------
+// candidates
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_contact_partners(ContactArgs a, SubstepConsts c)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    if (!a.active[pclass(a, t).scene]) return;
+    uint32_t cnt = 0;
+    walk_runs(a.rt.cellc[t], a.rt, c, [&](uint32_t q) {
+        if (q == t) return true;
+        uint32_t lo, hi;
+        bool t_hi;
+        order_pair(a, t, q, lo, hi, t_hi);
+        if (!pair_valid(a, lo, hi)) return true;
+        if (cnt < a.pcap) a.plist[(size_t)cnt * a.n + t] = q;
+        ++cnt;
+        return true;
+    });
+    if (cnt > a.pcap) {
+        atomicAdd(&a.meta[4], 1u);
+        cnt = a.pcap;
+    }
+    a.pcount[t] = cnt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// position iteration: generate + solve (sum in partner order), then apply
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_contact_solve(ContactArgs a, SubstepConsts c)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    if (!a.active[pclass(a, t).scene]) return;
+    const uint32_t cnt = a.pcount[t];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    uint32_t bits = 0;
+    const float relax = c.cfg[kCfgSoftContactRelax];
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t q = a.plist[(size_t)k * a.n + t];
+        uint32_t lo, hi;
+        bool t_hi;
+        order_pair(a, t, q, lo, hi, t_hi);
+        // generate_explicit_contacts :140-188
+        const float4 pa = a.pos[lo], pb = a.pos[hi];
+        const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
+        const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+        const float cr = pclass(a, lo).radius + pclass(a, hi).radius;
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f, pen = 0.0f;
+        bool on = false;
+        if (d2 <= kEps) {
+            on = true;
+            ny = 1.0f;
+            pen = cr;
+        } else {
+            const float dist = fsqrt(d2);
+            const float p0 = cr - dist;
+            if (p0 > 0.0f) {
+                on = true;
+                nx = fdiv(dx, dist);
+                ny = fdiv(dy, dist);
+                nz = fdiv(dz, dist);
+                pen = p0;
+            }
+        }
+        a.crec[(size_t)k * a.n + t] = make_float4(nx, ny, nz, pen);
+        if (!on) continue;
+        bits |= 1u << k;
+        // solve_explicit_contacts :103-246 for a pair of free particles
+        const float wa = pa.w, wb = pb.w;
+        if (wa <= kEps && wb <= kEps) continue;
+        const float pn = fmind(fmaxd(pen - c.cfg[kCfgContactSlop], 0.0f), c.cfg[kCfgSoftMaxCorrection]);
+        if (pn <= 0.0f) continue;
+        const float3 nr = safe_normalize(nx, ny, nz);
+        const float4 va = a.prev[lo], vb = a.prev[hi];
+        const float rx = (pb.x - vb.x) - (pa.x - va.x);
+        const float ry = (pb.y - vb.y) - (pa.y - va.y);
+        const float rz = (pb.z - vb.z) - (pa.z - va.z);
+        const float denom = wa + wb;
+        if (denom <= kEps) continue;
+        const float lambda = fdiv(pn, denom) * relax;
+        float cax = -nr.x * (wa * lambda), cay = -nr.y * (wa * lambda), caz = -nr.z * (wa * lambda);
+        float cbx = nr.x * (wb * lambda), cby = nr.y * (wb * lambda), cbz = nr.z * (wb * lambda);
+        const float rn = dot3(rx, ry, rz, nr.x, nr.y, nr.z);
+        const float3 mat = combine_material(a, lo, hi);
+        const float3 fd = friction_delta(c.cfg, rx - nr.x * rn, ry - nr.y * rn, rz - nr.z * rn, pn, mat.x, mat.z);
+        const float fdist = len3(fd.x, fd.y, fd.z);
+        if (fdist > 0.0f) {
+            const float tden = wa + wb;
+            if (tden > kEps) {
+                const float tl = fdiv(fdist, tden) * relax;
+                const float tx = fdiv(fd.x, fdist), ty = fdiv(fd.y, fdist), tz = fdiv(fd.z, fdist);
+                cax = cax + tx * (wa * tl);
+                cay = cay + ty * (wa * tl);
+                caz = caz + tz * (wa * tl);
+                cbx = cbx - tx * (wb * tl);
+                cby = cby - ty * (wb * tl);
+                cbz = cbz - tz * (wb * tl);
+            }
+        }
+        if (t_hi) {
+            if (wb > kEps) {
+                sx = sx + cbx;
+                sy = sy + cby;
+                sz = sz + cbz;
+            }
+        } else if (wa > kEps) {
+            sx = sx + cax;
+            sy = sy + cay;
+            sz = sz + caz;
+        }
+    }
+    a.cbits[t] = bits;
+    a.corr[t] = make_float4(sx, sy, sz, 0.0f);
+}
+
+// apply_position_corrections :8-30 (every free particle; those without contacts add zero); a fluid
+// particle's solver position follows its new x*
+__global__ __launch_bounds__(kBlock) void k_contact_apply(ContactArgs a)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    const ParticleClass &pc = pclass(a, t);
+    if (!a.active[pc.scene]) return;
+    float4 p = a.pos[t];
+    if (p.w <= kEps) return;
+    const float4 d = a.corr[t];
+    p.x = p.x + d.x;
+    p.y = p.y + d.y;
+    p.z = p.z + d.z;
+    a.pos[t] = p;
+    if (pc.kind == CN_PARTICLE_KIND_FLUID) {
+        const float4 dl = a.dacc[t];
+        a.cur[t] = make_float4(p.x + dl.x, p.y + dl.y, p.z + dl.z, p.w);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// velocity iteration over the last generated contacts
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_contact_velocity(ContactArgs a, SubstepConsts c)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    if (!a.active[pclass(a, t).scene]) return;
+    const uint32_t cnt = a.pcount[t], bits = a.cbits[t];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        if (!((bits >> k) & 1u)) continue;
+        const uint32_t q = a.plist[(size_t)k * a.n + t];
+        uint32_t lo, hi;
+        bool t_hi;
+        order_pair(a, t, q, lo, hi, t_hi);
+        const float wa = a.pos[lo].w, wb = a.pos[hi].w;
+        if (wa <= kEps && wb <= kEps) continue;
+        const float4 rec = a.crec[(size_t)k * a.n + t];
+        const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
+        const float4 va = a.vel[lo], vb = a.vel[hi];
+        const float nv = dot3(vb.x - va.x, vb.y - va.y, vb.z - va.z, nr.x, nr.y, nr.z);
+        if (nv >= 0.0f) continue;
+        const float denom = wa + wb;
+        if (denom <= kEps) continue;
+        const bool rest_on = (-nv > c.cfg[kCfgRestitutionVelocity]) &&
+                             (rec.w <= c.cfg[kCfgRestitutionPenetration] * c.cfg[kCfgContactSlop]);
+        const float restitution = rest_on ? fsat(combine_material(a, lo, hi).y) : 0.0f;
+        const float desired = rest_on ? (-restitution * nv) : 0.0f;
+        const float imp = fmaxd(0.0f, fdiv(desired - nv, denom));
+        const float tx = nr.x * imp, ty = nr.y * imp, tz = nr.z * imp;
+        if (t_hi) {
+            if (wb > kEps) {
+                sx = sx + tx * wb;
+                sy = sy + ty * wb;
+                sz = sz + tz * wb;
+            }
+        } else if (wa > kEps) {
+            sx = sx + -tx * wa;
+            sy = sy + -ty * wa;
+            sz = sz + -tz * wa;
+        }
+    }
+    a.corr[t] = make_float4(sx, sy, sz, 0.0f);
+}
+
+// apply_contact_velocities: v += correction for free particles
+__global__ __launch_bounds__(kBlock) void k_contact_velocity_apply(ContactArgs a)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    if (!a.active[pclass(a, t).scene]) return;
+    if (a.pos[t].w <= kEps) return;
+    float4 v = a.vel[t];
+    const float4 d = a.corr[t];
+    v.x = v.x + d.x;
+    v.y = v.y + d.y;
+    v.z = v.z + d.z;
+    a.vel[t] = v;
+}
+
+static ContactArgs contact_args(const SceneBuffers &s, const SubstepConsts &c, int din)
+{
+    ContactArgs a;
+    const int cur = s.cur;
+    a.pos = s.pos[cur];
+    a.vel = s.vel[cur];
+    a.prev = s.prev[cur];
+    a.cur = s.curpos;
+    a.dacc = s.dacc[din];
+    a.cls = s.cls[cur];
+    a.id = s.id[cur];
+    a.classes = s.classes;
+    a.part_mats = s.part_mats;
+    a.active = s.scene_active;
+    a.rt = RunTable{s.run_key, s.run_start, s.run_info, s.bucket, c.bucket_shift, s.cellc};
+    a.plist = s.ct_list;
+    a.pcount = s.ct_count;
+    a.crec = s.ct_rec;
+    a.cbits = s.ct_bits;
+    a.corr = s.ct_corr;
+    a.meta = s.meta;
+    a.n = c.n;
+    a.pcap = kContactPartners;
+    return a;
+}
+
+static uint32_t grid_c(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_contact_partners(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const ContactArgs a = contact_args(s, c, 0);
+    hipLaunchKernelGGL(k_contact_partners, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c);
+}
+
+void launch_contact_iteration(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st)
+{
+    const ContactArgs a = contact_args(s, c, din);
+    hipLaunchKernelGGL(k_contact_solve, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c);
+    hipLaunchKernelGGL(k_contact_apply, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a);
+}
+
+void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const ContactArgs a = contact_args(s, c, 0);
+    hipLaunchKernelGGL(k_contact_velocity, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c);
+    hipLaunchKernelGGL(k_contact_velocity_apply, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a);
+}
+
+}  // namespace cn

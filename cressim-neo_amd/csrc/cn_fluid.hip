@@ -26,6 +26,7 @@
 #include <algorithm>
 
 #include "cn_internal.h"
+#include "cn_walk.h"
 
 namespace cn {
 
@@ -68,13 +69,9 @@ struct FluidArgs {
     uint32_t tile;            // halo particles this launch can stage in LDS
 };
 
-__device__ __forceinline__ int find_run(uint32_t key, const FluidArgs &a)
+__device__ __forceinline__ RunTable run_table(const FluidArgs &a)
 {
-    const uint32_t b = key >> a.bucket_shift;
-    uint32_t r = a.bucket[b];
-    const uint32_t e = a.bucket[b + 1];
-    while (r < e && a.run_key[r] < key) ++r;
-    return (r < e && a.run_key[r] == key) ? (int)r : -1;
+    return RunTable{a.run_key, a.run_start, a.run_info, a.bucket, a.bucket_shift, a.cellc};
 }
 
 // ShouldProcessFluidNeighbor, physics_fluid_neighbor_build.hlsli:48-70 (the index test is the caller's)
@@ -101,34 +98,6 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 // ---------------------------------------------------------------------------------------------
 // candidate walks, in the reference's order: f(k) per candidate until f returns false
 // ---------------------------------------------------------------------------------------------
-
-// The 27 cells through the run table (hash keys, blocks without a staged halo).  A cell whose hash
-// equals the empty-table marker is never found (physics_fluid_build_neighbors.cs.hlsl:81-85 with
-// FindParticleCellRange); a run shared by several cells (hash collision, Morton wrap) is filtered
-// by cell coordinates.
-template <class F>
-__device__ __forceinline__ void walk_runs(int4 ci, const FluidArgs &a, const SubstepConsts &c, F &&f)
-{
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int tx = ci.x + dx, ty = ci.y + dy, tz = ci.z + dz;
-                if (cell_hash(tx, ty, tz) == kInvalid) continue;
-                const int r = find_run(cell_key(tx, ty, tz, (uint32_t)ci.w, c.key_mode, c.morton_bits), a);
-                if (r < 0) continue;
-                const int4 info = a.run_info[r];
-                const bool pure = info.w != 0;
-                if (pure && (info.x != tx || info.y != ty || info.z != tz)) continue;
-                const uint32_t qe = a.run_start[r + 1];
-                for (uint32_t q = a.run_start[r]; q < qe; ++q) {
-                    if (!pure) {
-                        const int4 cq = a.cellc[q];
-                        if (cq.x != tx || cq.y != ty || cq.z != tz) continue;
-                    }
-                    if (!f(q)) return;
-                }
-            }
-}
 
 // The same walk over a staged halo (hc = the particle's halo cell): nine LDS ranges.
 template <class F>
@@ -315,7 +284,7 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
     if (!density_begin(x, i, a.cur[i], UNIFORM ? 0u : a.cls[i], a)) return;
     DensitySum d;
     if (density_walks(x, UNIFORM, c))
-        walk_runs(a.cellc[i], a, c, [&](uint32_t q) {
+        walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
             if (q == i) return true;
             const float4 pj = a.cur[q];
             if (!density_test<UNIFORM>(x, pj, UNIFORM ? 0u : a.cls[q], a)) return true;
@@ -526,7 +495,7 @@ __device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, co
         i, i,
         [&](const DeltaSelf &x, DeltaSum &d) {
             uint32_t inr = 0;
-            walk_runs(ci, a, c, [&](uint32_t q) {
+            walk_runs(ci, run_table(a), c, [&](uint32_t q) {
                 if (q == i) return true;
                 const uint32_t qc = UNIFORM ? 0u : a.cls[q];
                 if (!UNIFORM && !neighbor_ok(x.pc, a.classes[qc])) return true;
@@ -578,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
         if (tid < kHalo) {
             const int tx = bx - 1 + (int)(tid % 6u), ty = by - 1 + (int)((tid / 6u) % 6u), tz = bz - 1 + (int)(tid / 36u);
             if (cell_hash(tx, ty, tz) != kInvalid) {
-                const int r = find_run(cell_key(tx, ty, tz, (uint32_t)c0.w, c.key_mode, c.morton_bits), a);
+                const int r = find_run(cell_key(tx, ty, tz, (uint32_t)c0.w, c.key_mode, c.morton_bits), run_table(a));
                 if (r >= 0) {
                     const int4 info = a.run_info[r];
                     if (info.w == 0) bad = true;
@@ -918,7 +887,7 @@ __global__ __launch_bounds__(kBlock) void k_fluid_lists(FluidArgs a, SubstepCons
     uint32_t cnt = 0;
     if (pc.kind == CN_PARTICLE_KIND_FLUID && s.w > kEps && c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))) {
         const float t2 = a.t2[pc.fluid_mat];
-        walk_runs(a.cellc[i], a, c, [&](uint32_t q) {
+        walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
             if (q == i) return true;
             if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
             const float4 pj = a.cur[q];

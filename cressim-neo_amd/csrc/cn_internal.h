@@ -31,6 +31,8 @@ constexpr uint32_t kHdP1 = kHdFlags + 2;
 constexpr uint32_t kHdBase = kHdFlags + 3;      // 3 words
 constexpr uint32_t kHdScene = kHdFlags + 6;
 constexpr uint32_t kHaloStride = 448;
+// candidate contact partners a particle keeps per substep; more are counted as overflows
+constexpr uint32_t kContactPartners = 32;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -79,6 +81,13 @@ struct SceneBuffers {
     uint32_t *nbr_cnt = nullptr;
     size_t nbr_alloc = 0;
     uint32_t cap = 0;
+    // particle-particle contacts (allocated when the scene has a solid particle)
+    uint32_t *ct_list = nullptr;    // candidate partner slots, k-major: ct_list[k * n + i]
+    uint32_t *ct_count = nullptr;
+    float4 *ct_rec = nullptr;       // (normal, penetration) of each candidate's last contact, k-major
+    uint32_t *ct_bits = nullptr;    // bit k: candidate k's last contact is active
+    float4 *ct_corr = nullptr;      // summed correction of the last contact pass
+    size_t ct_alloc = 0;
     // index-order exports
     float4 *out_pos = nullptr;
     float4 *out_vel = nullptr;
@@ -95,7 +104,7 @@ struct SceneBuffers {
     uint32_t *scene_active = nullptr;  // per scene: 1 = stepped by cn_step, 0 = state left untouched
     uint32_t *pid_scene = nullptr;     // scene of each particle index
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
-    // [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
+    // [4] contact partner overflows, [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
     uint32_t *meta = nullptr;
     // soft constraints
     uint32_t *slot_of = nullptr;
@@ -123,6 +132,11 @@ void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
 void launch_fill(uint32_t *p, uint32_t words, uint32_t value, hipStream_t st);
+
+// ---- particle-particle contacts (cn_contact.hip) ----
+void launch_contact_partners(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_contact_iteration(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
+void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 
 // ---- XPBD soft constraints (cn_soft.hip) ----
 struct SoftArgs {

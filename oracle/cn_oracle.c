@@ -1204,7 +1204,6 @@ static void csr_free(csr *c)
     free(c->ref);
 }
 
-static float pos_w(const cno_scene *s, uint32_t i) { return s->pos[4 * i + 3]; }
 
 /* physics_soft_solve_edge_constraints.cs.hlsl:10-74 */
 static void solve_edges(cno_scene *s, const float *cfg, float dt, float *lam)
@@ -1489,6 +1488,296 @@ static void apply_corrections(cno_scene *s, const csr *c, const float *rec, uint
  * ------------------------------------------------------------------------------------------ */
 static uint32_t resolve_iters(uint32_t v, uint32_t d) { return v > 0 ? v : d; }
 
+/* ---------------------------------------------------------------------------------------------
+ * Particle-particle contacts.
+ * Candidates once per substep from the predicted positions (buildParticleParticleCandidatePairs,
+ * physics_particle_{count,emit}_particle_particle_candidates.cs.hlsl with
+ * IsValidParticleParticleCandidate, physics_particle_broad_phase_query.hlsli:65-117); each contact
+ * iteration generates the contacts at the current positions
+ * (physics_particle_generate_explicit_contacts.cs.hlsl:80-189), solves them
+ * (physics_particle_solve_explicit_contacts.cs.hlsl:25-246) and folds the corrections into the
+ * positions (physics_particle_apply_position_corrections.cs.hlsl:8-30); after the velocity update
+ * the velocity iterations solve the last contacts
+ * (physics_particle_solve_contact_velocities.cs.hlsl:26-170,
+ * physics_particle_apply_contact_velocities.cs.hlsl).  The reference accumulates corrections with
+ * float CAS atomics in no fixed order; here each particle sums its contacts in its candidate walk
+ * order (27 cells dz, dy, dx; ascending particle index in a cell).  SafeNormalize's rsqrt is
+ * evaluated as 1 / sqrt.
+ * ------------------------------------------------------------------------------------------- */
+#define PHASE_GROUP_MASK 0x7fffffffu
+#define PHASE_SELF_COLLIDE 0x80000000u
+#define CFG_SOFT_CONTACT_RELAX 2
+#define CFG_SOFT_MAX_CORRECTION 3
+#define CFG_RESTITUTION_VELOCITY 4
+#define CFG_RESTITUTION_PENETRATION 5
+
+typedef struct contact_rec {
+    float n[3];
+    float pen;
+    float mat[3];   /* friction, restitution, static friction */
+    uint32_t active;
+} contact_rec;
+
+/* IsValidParticleParticleCandidate for the pair lo < hi (the test is symmetric in the pair) */
+static int pp_valid(const cno_scene *s, uint32_t lo, uint32_t hi)
+{
+    const uint32_t *ma = s->meta + 4 * (size_t)lo, *mb = s->meta + 4 * (size_t)hi;
+    if (ma[0] != mb[0])
+        return 0;
+    if ((ma[3] & mb[2]) == 0u || (mb[3] & ma[2]) == 0u)
+        return 0;
+    const uint32_t ka = s->kinds[lo], kb = s->kinds[hi];
+    if (ka == CN_PARTICLE_KIND_FLUID && kb == CN_PARTICLE_KIND_FLUID)
+        return 0;
+    if (ka == CN_PARTICLE_KIND_SOLID && kb == CN_PARTICLE_KIND_SOLID &&
+        (ma[1] & PHASE_GROUP_MASK) == (mb[1] & PHASE_GROUP_MASK))
+        return 0; /* self collision of a group: rejected up front (no adjacency lists) */
+    float d[3];
+    for (int a = 0; a < 3; ++a)
+        d[a] = s->pos[4 * (size_t)hi + a] - s->pos[4 * (size_t)lo + a];
+    const float cr = s->radii[lo] + s->radii[hi];
+    return dot3(d, d) <= cr * cr;
+}
+
+/* Candidate partners of every particle, in its 27-cell walk order; returns the overflow count. */
+uint32_t cno_contact_partners(const cno_scene *s, const int32_t *cells, const uint32_t *si,
+                              const uint32_t *tbl, uint32_t capacity, uint32_t pcap, uint32_t *pcount,
+                              uint32_t *plist)
+{
+    uint32_t overflow = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : overflow)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const int32_t *ci = cells + 3 * (size_t)i;
+        uint32_t cnt = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int32_t tx = ci[0] + dx, ty = ci[1] + dy, tz = ci[2] + dz;
+                    uint32_t start, end;
+                    if (!cno_find_cell_range(tbl, capacity, cell_hash(tx, ty, tz), &start, &end))
+                        continue;
+                    for (uint32_t q = start; q < end; ++q) {
+                        const uint32_t j = si[q];
+                        const int32_t *cj = cells + 3 * (size_t)j;
+                        if (j == i || cj[0] != tx || cj[1] != ty || cj[2] != tz)
+                            continue;
+                        if (!pp_valid(s, i < j ? i : j, i < j ? j : i))
+                            continue;
+                        if (cnt < pcap)
+                            plist[(size_t)i * pcap + cnt] = j;
+                        ++cnt;
+                    }
+                }
+        if (cnt > pcap) {
+            ++overflow;
+            cnt = pcap;
+        }
+        pcount[i] = cnt;
+    }
+    return overflow;
+}
+
+/* CombineContactMaterial, physics_rigid_solver_shared.hlsli:37-43 */
+static void combine_material(const cno_scene *s, uint32_t a, uint32_t b, float *m)
+{
+    const float *pa = s->part_mats + 4 * (size_t)s->part_mat[a], *pb = s->part_mats + 4 * (size_t)s->part_mat[b];
+    m[0] = sqrtf(maxf(0.0f, pa[0]) * maxf(0.0f, pb[0]));
+    m[1] = maxf(maxf(0.0f, pa[1]), maxf(0.0f, pb[1]));
+    m[2] = sqrtf(maxf(0.0f, pa[3]) * maxf(0.0f, pb[3]));
+}
+
+/* the contact of the pair lo < hi at the current positions (generate_explicit_contacts :140-188) */
+static void make_contact(const cno_scene *s, uint32_t lo, uint32_t hi, contact_rec *c)
+{
+    memset(c, 0, sizeof(*c));
+    combine_material(s, lo, hi, c->mat);
+    float d[3];
+    for (int a = 0; a < 3; ++a)
+        d[a] = s->pos[4 * (size_t)hi + a] - s->pos[4 * (size_t)lo + a];
+    const float d2 = dot3(d, d);
+    const float cr = s->radii[lo] + s->radii[hi];
+    if (d2 <= EPS) {
+        c->active = 1;
+        c->n[1] = 1.0f;
+        c->pen = cr;
+        return;
+    }
+    const float dist = sqrtf(d2);
+    const float pen = cr - dist;
+    if (pen > 0.0f) {
+        c->active = 1;
+        for (int a = 0; a < 3; ++a)
+            c->n[a] = d[a] / dist;
+        c->pen = pen;
+    }
+}
+
+/* SafeNormalize, physics_math.hlsli:6-12, with rsqrt as 1 / sqrt */
+static void safe_normalize(const float *v, float *o)
+{
+    const float l2 = dot3(v, v);
+    if (l2 <= EPS) {
+        o[0] = 0.0f;
+        o[1] = 1.0f;
+        o[2] = 0.0f;
+        return;
+    }
+    const float r = 1.0f / sqrtf(l2);
+    for (int a = 0; a < 3; ++a)
+        o[a] = v[a] * r;
+}
+
+/* solve_explicit_contacts :103-246 for a non-proxy pair; out = the correction of side hi (side_hi)
+ * or lo, zero when that side takes none */
+static void contact_position(const cno_scene *s, const float *cfg, uint32_t lo, uint32_t hi, const contact_rec *c,
+                             int side_hi, float *out)
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    const float *pa = s->pos + 4 * (size_t)lo, *pb = s->pos + 4 * (size_t)hi;
+    const float wa = pa[3], wb = pb[3];
+    if (wa <= EPS && wb <= EPS)
+        return;
+    const float pen = minf(maxf(c->pen - cfg[CFG_CONTACT_SLOP], 0.0f), cfg[CFG_SOFT_MAX_CORRECTION]);
+    if (pen <= 0.0f)
+        return;
+    float nrm[3], rel[3], tan[3], fd[3];
+    safe_normalize(c->n, nrm);
+    const float *va = s->prev + 4 * (size_t)lo, *vb = s->prev + 4 * (size_t)hi;
+    for (int a = 0; a < 3; ++a)
+        rel[a] = (pb[a] - vb[a]) - (pa[a] - va[a]);
+    const float denom = wa + wb;
+    if (denom <= EPS)
+        return;
+    const float relax = cfg[CFG_SOFT_CONTACT_RELAX];
+    const float lambda = (pen / denom) * relax;
+    float ca[3], cb[3];
+    for (int a = 0; a < 3; ++a) {
+        ca[a] = -nrm[a] * (wa * lambda);
+        cb[a] = nrm[a] * (wb * lambda);
+    }
+    const float rn = dot3(rel, nrm);
+    for (int a = 0; a < 3; ++a)
+        tan[a] = rel[a] - nrm[a] * rn;
+    friction_delta(cfg, tan, pen, c->mat[0], c->mat[2], fd);
+    const float fdist = len3(fd);
+    if (fdist > 0.0f) {
+        const float tden = wa + wb;
+        if (tden > EPS) {
+            const float tl = (fdist / tden) * relax;
+            for (int a = 0; a < 3; ++a) {
+                const float t = fd[a] / fdist;
+                ca[a] = ca[a] + t * (wa * tl);
+                cb[a] = cb[a] - t * (wb * tl);
+            }
+        }
+    }
+    if (side_hi) {
+        if (wb > EPS)
+            memcpy(out, cb, sizeof(cb));
+    } else if (wa > EPS) {
+        memcpy(out, ca, sizeof(ca));
+    }
+}
+
+/* solve_contact_velocities :100-170 for a non-proxy pair; out = the velocity correction of a side */
+static void contact_velocity(const cno_scene *s, const float *cfg, uint32_t lo, uint32_t hi, const contact_rec *c,
+                             int side_hi, float *out)
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    const float wa = s->pos[4 * (size_t)lo + 3], wb = s->pos[4 * (size_t)hi + 3];
+    if (wa <= EPS && wb <= EPS)
+        return;
+    float nrm[3], rv[3];
+    safe_normalize(c->n, nrm);
+    const float *va = s->vel + 4 * (size_t)lo, *vb = s->vel + 4 * (size_t)hi;
+    for (int a = 0; a < 3; ++a)
+        rv[a] = vb[a] - va[a];
+    const float nv = dot3(rv, nrm);
+    if (nv >= 0.0f)
+        return;
+    const float denom = wa + wb;
+    if (denom <= EPS)
+        return;
+    const int rest_on = (-nv > cfg[CFG_RESTITUTION_VELOCITY]) &&
+                        (c->pen <= cfg[CFG_RESTITUTION_PENETRATION] * cfg[CFG_CONTACT_SLOP]);
+    const float restitution = rest_on ? saturatef(c->mat[1]) : 0.0f;
+    const float desired = rest_on ? (-restitution * nv) : 0.0f;
+    const float imp = maxf(0.0f, (desired - nv) / denom);
+    for (int a = 0; a < 3; ++a) {
+        const float t = nrm[a] * imp;
+        if (side_hi) {
+            if (wb > EPS)
+                out[a] = t * wb;
+        } else if (wa > EPS) {
+            out[a] = -t * wa;
+        }
+    }
+}
+
+/* one contact iteration: generate every particle's contacts, sum its corrections in walk order,
+ * then fold them into the positions */
+static void contact_iteration(cno_scene *s, const float *cfg, uint32_t pcap, const uint32_t *pcount,
+                              const uint32_t *plist, contact_rec *recs, float *corr)
+{
+    const uint32_t n = s->n;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < pcount[i]; ++k) {
+            const uint32_t j = plist[(size_t)i * pcap + k];
+            const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+            contact_rec *c = recs + (size_t)i * pcap + k;
+            make_contact(s, lo, hi, c);
+            if (!c->active)
+                continue;
+            float d[3];
+            contact_position(s, cfg, lo, hi, c, i == hi, d);
+            for (int a = 0; a < 3; ++a)
+                acc[a] = acc[a] + d[a];
+        }
+        memcpy(corr + 4 * (size_t)i, acc, sizeof(acc));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        float *p = s->pos + 4 * (size_t)i;
+        if (p[3] <= EPS)
+            continue;
+        for (int a = 0; a < 3; ++a)
+            p[a] = p[a] + corr[4 * (size_t)i + a];
+    }
+}
+
+static void contact_velocity_iteration(cno_scene *s, const float *cfg, uint32_t pcap, const uint32_t *pcount,
+                                       const uint32_t *plist, const contact_rec *recs, float *corr)
+{
+    const uint32_t n = s->n;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < pcount[i]; ++k) {
+            const contact_rec *c = recs + (size_t)i * pcap + k;
+            if (!c->active)
+                continue;
+            const uint32_t j = plist[(size_t)i * pcap + k];
+            const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+            float d[3];
+            contact_velocity(s, cfg, lo, hi, c, i == hi, d);
+            for (int a = 0; a < 3; ++a)
+                acc[a] = acc[a] + d[a];
+        }
+        memcpy(corr + 4 * (size_t)i, acc, sizeof(acc));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        float *v = s->vel + 4 * (size_t)i;
+        if (s->pos[4 * (size_t)i + 3] <= EPS)
+            continue;
+        for (int a = 0; a < 3; ++a)
+            v[a] = v[a] + corr[4 * (size_t)i + a];
+    }
+}
+
 int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
 {
     const uint32_t n = s->n;
@@ -1538,6 +1827,28 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     cno_static_collider_order(s->n_colliders, s->colliders, order);
 
     const int has_soft = n > 0 && (s->n_edges || s->n_bends || s->n_dihedrals || s->n_tets);
+    /* particle-particle contacts: any pair needs a solid particle (fluid-fluid pairs are never
+     * candidates); self-collision of a phase group needs adjacency lists and is not supported */
+    uint32_t solid_count = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (s->kinds[i] == CN_PARTICLE_KIND_SOLID) {
+            ++solid_count;
+            if (s->meta[4 * (size_t)i + 1] & PHASE_SELF_COLLIDE)
+                return -2;
+        }
+    const int has_contacts = contact_iters > 0 && n > 1 && solid_count > 0;
+    const uint32_t pcap = CNO_CONTACT_PARTNERS;
+    uint32_t *pcount = NULL, *plist = NULL;
+    contact_rec *recs = NULL;
+    float *ccorr = NULL;
+    if (has_contacts) {
+        pcount = (uint32_t *)calloc(n, sizeof(uint32_t));
+        plist = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n * pcap);
+        recs = (contact_rec *)calloc((size_t)n * pcap, sizeof(contact_rec));
+        ccorr = (float *)calloc(4 * (size_t)n, sizeof(float));
+        if (!pcount || !plist || !recs || !ccorr)
+            return -1;
+    }
     csr inc_e = {0, 0}, inc_b = {0, 0}, inc_t = {0, 0};
     float *lam_e = NULL, *lam_b = NULL, *lam_t = NULL;
     if (has_soft) {
@@ -1574,6 +1885,8 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         }
         if (has_bnd)
             cno_fluid_boundary_candidates(s, order, br, bp, bcap);
+        if (has_contacts)
+            st.contact_overflows += cno_contact_partners(s, cells, idx, tbl, capacity, pcap, pcount, plist);
         if (has_soft) {
             /* clearSoftConstraintState, physics_clear_soft_constraint_state.cs.hlsl:14-54 */
             memset(lam_e, 0, sizeof(float) * (s->n_edges + 1));
@@ -1613,6 +1926,8 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
                 solve_tets(s, cfg, dt, lam_t);
                 apply_corrections(s, &inc_t, s->tet_corr, 16);
             }
+            if (has_contacts && it < contact_iters)
+                contact_iteration(s, cfg, pcap, pcount, plist, recs, ccorr);
         }
         cno_update_velocities(s, dt, dacc);
         if (has_fluid) {
@@ -1622,6 +1937,9 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
             cno_fluid_neighbors(s, cell, cells, keys, idx, tbl, capacity, dacc,
                                 max_iters > 0 ? max_iters - 1 : 0, cap, counts, list, NULL);
         }
+        if (has_contacts)
+            for (uint32_t v = 0; v < contact_iters; ++v)
+                contact_velocity_iteration(s, cfg, pcap, pcount, plist, recs, ccorr);
     }
     /* density-constraint residual of the last density pass of the step */
     if (has_fluid) {
@@ -1653,6 +1971,10 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     free(br);
     free(bp);
     free(order);
+    free(pcount);
+    free(plist);
+    free(recs);
+    free(ccorr);
     if (has_soft) {
         csr_free(&inc_e);
         csr_free(&inc_b);

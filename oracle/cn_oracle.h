@@ -58,7 +58,7 @@ typedef struct cno_stats {
     uint32_t truncations;
     uint64_t neighbor_total_first_iter;   /* sum of counts of the last substep's iteration 0 */
     uint32_t fluid_count;
-    uint32_t reserved;
+    uint32_t contact_overflows;           /* particles with more than CNO_CONTACT_PARTNERS candidates */
     double max_abs_constraint_last;       /* max |C| / scale-free residual of last density pass */
     double mean_constraint_last;
 } cno_stats;
@@ -115,6 +115,14 @@ void cno_predict(cno_scene *s, float dt, const float *gravity);
  * kernels evaluate the same sequence. */
 float cno_atan2f(float y, float x);
 float cno_acosf(float x);
+
+/* ---- particle-particle contacts ---- */
+#define CNO_CONTACT_PARTNERS 32u
+/* candidate partners of every particle in its 27-cell walk order (pcount[n], plist[n * pcap]);
+ * returns the number of particles whose candidates exceeded pcap */
+uint32_t cno_contact_partners(const cno_scene *s, const int32_t *cells, const uint32_t *sorted_idx,
+                              const uint32_t *ranges_tbl, uint32_t capacity, uint32_t pcap,
+                              uint32_t *pcount, uint32_t *plist);
 
 /* ---- full step: PhysicsSolver::step for the particle path ---- */
 int cno_step(cno_scene *s, const cn_step_params *params, cno_stats *stats);

@@ -32,7 +32,7 @@ class CnoScene(C.Structure):
 class CnoStats(C.Structure):
     _fields_ = [("max_required_neighbors", C.c_uint32), ("truncations", C.c_uint32),
                 ("neighbor_total_first_iter", C.c_uint64), ("fluid_count", C.c_uint32),
-                ("reserved", C.c_uint32), ("max_abs_constraint_last", C.c_double),
+                ("contact_overflows", C.c_uint32), ("max_abs_constraint_last", C.c_double),
                 ("mean_constraint_last", C.c_double)]
 
 
