@@ -209,6 +209,17 @@ class Solver:
         """Capture cn_step into a HIP graph and replay it (cn_graph_enable)."""
         self._check(self.lib.cn_graph_enable(self._ctx, 1 if enable else 0), "cn_graph_enable")
 
+    def autotune(self, params: abi.StepParams, steps: int = 2) -> dict:
+        """Pick the fluid passes' launch shape by timing cn_step from the current state
+        (cn_autotune); the state is left as it was."""
+        r = abi.TuneResult()
+        self._check(self.lib.cn_autotune(self._ctx, C.byref(params), steps, C.byref(r)), "cn_autotune")
+        return dict(tile_max=r.tile_max, grid_max=r.grid_max, candidates=r.candidates,
+                    ms_per_step=r.ms_per_step, default_ms_per_step=r.default_ms_per_step)
+
+    def tune_set(self, tile_max: int, grid_max: int):
+        self._check(self.lib.cn_tune_set(self._ctx, tile_max, grid_max), "cn_tune_set")
+
     def done(self) -> bool:
         """True when the context's enqueued work has finished (cn_query, non-blocking)."""
         st = self.lib.cn_query(self._ctx)

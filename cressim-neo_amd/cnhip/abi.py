@@ -120,6 +120,11 @@ class KernelStats(C.Structure):
                 ("total_ms", C.c_double * CN_KERNEL_SLOTS), ("used", C.c_uint32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_max", C.c_uint32), ("grid_max", C.c_uint32), ("candidates", C.c_uint32),
+                ("reserved", C.c_uint32), ("ms_per_step", C.c_float), ("default_ms_per_step", C.c_float)]
+
+
 # every symbol include/cn_hip.h declares, with its ctypes signature
 _P = C.c_void_p
 _U = C.c_uint32
@@ -139,6 +144,8 @@ SIGNATURES = {
     "cn_batch_upload": (C.c_int, [_P, _U, C.POINTER(SceneDesc)]),
     "cn_batch_scene_range": (C.c_int, [_P, _U, C.POINTER(_U), C.POINTER(_U)]),
     "cn_graph_enable": (C.c_int, [_P, C.c_int]),
+    "cn_autotune": (C.c_int, [_P, C.POINTER(StepParams), _U, C.POINTER(TuneResult)]),
+    "cn_tune_set": (C.c_int, [_P, _U, _U]),
     "cn_step_scenes": (C.c_int, [_P, C.POINTER(StepParams), _U, _P]),
     "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
     "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),

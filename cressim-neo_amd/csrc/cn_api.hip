@@ -36,6 +36,7 @@ struct cn_ctx {
     bool has_tension = false;        // any fluid material with surface tension
     bool has_solid = false;          // any solid particle: particle-particle contacts can exist
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
+    uint32_t grid_max = 1024;        // workgroups of a persistent fluid launch
     // scene pool
     uint32_t nscenes = 0;
     uint32_t scene_bits = 0;         // key bits above the Morton code holding the scene index
@@ -355,6 +356,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.tension = ctx->has_tension ? 1u : 0u;
     c.tile_max = ctx->tile_max;
     c.blk_cap = ctx->s.blk_cap;
+    c.grid_max = ctx->grid_max;
     fluid_tiles(c);
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
@@ -1171,6 +1173,143 @@ cn_status cn_graph_enable(cn_ctx *ctx, int enable)
 {
     if (!ctx) return CN_ERR_INVALID_ARGUMENT;
     ctx->graphs_enabled = enable != 0;
+    return CN_OK;
+}
+
+cn_status cn_tune_set(cn_ctx *ctx, uint32_t tile_max, uint32_t grid_max)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (tile_max < 256 || tile_max > 4096 || grid_max < kXcds || grid_max > 8192)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "tile_max outside [256, 4096] or grid_max outside [8, 8192]");
+    if (tile_max != ctx->tile_max || grid_max != ctx->grid_max) ctx->graph_valid = false;
+    ctx->tile_max = tile_max;
+    ctx->grid_max = grid_max;
+    return CN_OK;
+}
+
+// Times cn_step under each candidate launch shape from the same saved state, keeps the fastest
+// and puts the state back.  Results do not depend on the shape: a block whose halo exceeds the
+// tile is walked through the run table with the same arithmetic in the same order.
+cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_tune_result *out)
+{
+    if (!ctx || !p) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "cn_autotune before cn_scene_upload");
+    if (p->max_fluid_neighbors > ctx->cap_alloc)
+        return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
+    if (p->key_mode != CN_KEY_MORTON && p->key_mode != CN_KEY_HASH)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "unknown key_mode");
+    if (p->key_mode == CN_KEY_HASH && ctx->nscenes > 1)
+        return fail(ctx, CN_ERR_UNSUPPORTED, "hash keys hold one scene; a scene pool needs CN_KEY_MORTON");
+    steps = std::max<uint32_t>(steps, 1u);
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const cn_status st0 = set_active(ctx, std::vector<uint32_t>(ctx->nscenes, 1u));
+    if (st0 != CN_OK) return st0;
+    SceneBuffers &s = ctx->s;
+    const size_t n = s.n;
+    // saved state: slot-order particles, previous positions, exports and diagnostics
+    float4 *sp = nullptr, *sv = nullptr, *spr = nullptr, *so = nullptr;
+    uint32_t *sc = nullptr, *si = nullptr, *sm = nullptr;
+    auto release = [&]() {
+        dfree(sp);
+        dfree(sv);
+        dfree(spr);
+        dfree(so);
+        dfree(sc);
+        dfree(si);
+        dfree(sm);
+    };
+    const int cur0 = s.cur;
+    const bool stepped0 = ctx->stepped;
+    const uint32_t tile0 = ctx->tile_max, grid0 = ctx->grid_max;
+    if (dalloc(&sp, n) != hipSuccess || dalloc(&sv, n) != hipSuccess || dalloc(&spr, n) != hipSuccess ||
+        dalloc(&so, 3 * n) != hipSuccess || dalloc(&sc, n) != hipSuccess || dalloc(&si, n) != hipSuccess ||
+        dalloc(&sm, 8) != hipSuccess) {
+        release();
+        return fail(ctx, CN_ERR_OUT_OF_MEMORY, "cn_autotune: no room for the saved state");
+    }
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    hipError_t e = hipSuccess;
+    auto save = [&]() {
+        const size_t f4 = n * sizeof(float4), u4 = n * sizeof(uint32_t);
+        if (e == hipSuccess) e = copy(sp, s.pos[cur0], f4);
+        if (e == hipSuccess) e = copy(sv, s.vel[cur0], f4);
+        if (e == hipSuccess) e = copy(spr, s.prev[cur0], f4);
+        if (e == hipSuccess) e = copy(so, s.out_pos, f4);
+        if (e == hipSuccess) e = copy(so + n, s.out_vel, f4);
+        if (e == hipSuccess) e = copy(so + 2 * n, s.out_prev, f4);
+        if (e == hipSuccess) e = copy(sc, s.cls[cur0], u4);
+        if (e == hipSuccess) e = copy(si, s.id[cur0], u4);
+        if (e == hipSuccess) e = copy(sm, s.meta, 8 * sizeof(uint32_t));
+    };
+    auto restore = [&]() {
+        const size_t f4 = n * sizeof(float4), u4 = n * sizeof(uint32_t);
+        s.cur = cur0;
+        ctx->stepped = stepped0;
+        if (e == hipSuccess) e = copy(s.pos[cur0], sp, f4);
+        if (e == hipSuccess) e = copy(s.vel[cur0], sv, f4);
+        if (e == hipSuccess) e = copy(s.prev[cur0], spr, f4);
+        if (e == hipSuccess) e = copy(s.out_pos, so, f4);
+        if (e == hipSuccess) e = copy(s.out_vel, so + n, f4);
+        if (e == hipSuccess) e = copy(s.out_prev, so + 2 * n, f4);
+        if (e == hipSuccess) e = copy(s.cls[cur0], sc, u4);
+        if (e == hipSuccess) e = copy(s.id[cur0], si, u4);
+        if (e == hipSuccess) e = copy(s.meta, sm, 8 * sizeof(uint32_t));
+    };
+    save();
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ea);
+    if (e == hipSuccess) e = hipEventCreate(&eb);
+    const uint32_t tiles[] = {1024, 2048, 3072, 4096};
+    const uint32_t grids[] = {256, 512, 1024, 2048};
+    float best = 3.4e38f, dflt = 0.0f;
+    uint32_t best_tile = tile0, best_grid = grid0, tried = 0;
+    for (uint32_t tile : tiles)
+        for (uint32_t grid : grids) {
+            if (e != hipSuccess) break;
+            ctx->tile_max = tile;
+            ctx->grid_max = grid;
+            record_step(ctx, p);  // warm-up from the saved state
+            restore();
+            if (e == hipSuccess) e = hipEventRecord(ea, ctx->stream);
+            for (uint32_t k = 0; k < steps; ++k) record_step(ctx, p);
+            if (e == hipSuccess) e = hipEventRecord(eb, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(eb);
+            if (e == hipSuccess) e = hipGetLastError();
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+            restore();
+            ms /= (float)steps;
+            ++tried;
+            if (tile == 4096 && grid == 1024) dflt = ms;
+            if (ms < best) {
+                best = ms;
+                best_tile = tile;
+                best_grid = grid;
+            }
+        }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
+    release();
+    ctx->graph_valid = false;
+    if (e != hipSuccess) {
+        ctx->tile_max = tile0;
+        ctx->grid_max = grid0;
+        return fail(ctx, CN_ERR_DEVICE, std::string("cn_autotune: ") + hipGetErrorString(e));
+    }
+    ctx->tile_max = best_tile;
+    ctx->grid_max = best_grid;
+    if (out) {
+        out->tile_max = best_tile;
+        out->grid_max = best_grid;
+        out->candidates = tried;
+        out->reserved = 0;
+        out->ms_per_step = best;
+        out->default_ms_per_step = dflt;
+    }
     return CN_OK;
 }
 

@@ -83,6 +83,7 @@ struct SubstepConsts {
     uint32_t tile_delta;     // halo capacity of the delta pass from the second iteration on
     uint32_t tile_wide;      // halo capacity of the wide delta pass (previous delta read from HBM)
     uint32_t blk_cap;        // cell blocks with a stored halo descriptor
+    uint32_t grid_max;       // workgroups of a persistent fluid launch (upper bound)
     float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];
 };

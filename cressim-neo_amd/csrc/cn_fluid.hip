@@ -950,8 +950,11 @@ constexpr uint32_t kDynLdsBytes = 62u * 1024u;
 
 static bool tiled(const SubstepConsts &c, uint32_t it) { return c.key_mode == CN_KEY_MORTON && it < kIterStateWords; }
 
-// persistent grid: blocks in flight on all dies, each pulling cell blocks from the queues
-static uint32_t persistent_grid(uint32_t n) { return std::min<uint32_t>(1024u, std::max<uint32_t>(kXcds, (n + 127u) / 128u)); }
+// persistent grid: workgroups in flight on all dies, each pulling cell blocks from the queues
+static uint32_t persistent_grid(const SubstepConsts &c)
+{
+    return std::min<uint32_t>(std::max<uint32_t>(c.grid_max, kXcds), std::max<uint32_t>(kXcds, (c.n + 127u) / 128u));
+}
 
 // Halo capacities: the density pass stages positions (and classes), the delta pass records,
 // previous deltas, normals and classes, the wide delta pass the same without previous deltas but
@@ -980,9 +983,9 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
         a.tile = c.tile_dens;
         const size_t lds = (size_t)a.tile * per;
         if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c)), dim3(kTileBlock), lds, st, a, c);
         else
-            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c.n)), dim3(kTileBlock), lds, st, a, c);
+            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c)), dim3(kTileBlock), lds, st, a, c);
     } else {
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_density_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
@@ -999,7 +1002,7 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
         // first iteration: no previous delta, so the delta pass stages as much as the wide one
         a.tile = it > 0 ? c.tile_delta : c.tile_wide;
         size_t lds = (size_t)a.tile * (16u + (it > 0 ? 16u : 0u) + nc + cls);
-        const dim3 grid(persistent_grid(c.n)), block(kTileBlock);
+        const dim3 grid(persistent_grid(c)), block(kTileBlock);
         if (c.uniform_class)
             hipLaunchKernelGGL((k_fluid_delta_tiled<true, false>), grid, block, lds, st, a, c);
         else
@@ -1007,7 +1010,7 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
         if (it > 0 && c.tile_wide > c.tile_delta) {
             a.tile = c.tile_wide;
             lds = (size_t)a.tile * (20u + nc + cls);
-            const dim3 wgrid(std::min<uint32_t>(256u, persistent_grid(c.n)));
+            const dim3 wgrid(std::min<uint32_t>(256u, persistent_grid(c)));
             if (c.uniform_class)
                 hipLaunchKernelGGL((k_fluid_delta_tiled<true, true>), wgrid, block, lds, st, a, c);
             else

@@ -264,6 +264,24 @@ cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *firs
  * command lists per step).  Uploads and reserves drop the cached graphs.  Off by default. */
 cn_status cn_graph_enable(cn_ctx *ctx, int enable);
 
+/* Launch shape of the fluid passes: halo particles a workgroup stages in LDS and workgroups of a
+ * persistent launch.  Results do not depend on it. */
+typedef struct cn_tune_result {
+    uint32_t tile_max;            /* chosen halo capacity, particles (256..4096) */
+    uint32_t grid_max;            /* chosen workgroups of a persistent launch */
+    uint32_t candidates;          /* shapes timed */
+    uint32_t reserved;
+    float ms_per_step;            /* cn_step time under the chosen shape */
+    float default_ms_per_step;    /* under the default shape (4096, 1024) */
+} cn_tune_result;
+/* Times `steps` cn_step calls of the uploaded scene (or pool) under each candidate shape, starting
+ * every time from the current state, keeps the fastest and leaves the state as it found it (no
+ * reference counterpart: the reference's dispatch sizes are fixed in its shaders).  Allocates a
+ * copy of the particle state while it runs. */
+cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *params, uint32_t steps, cn_tune_result *out);
+/* Sets the launch shape directly (tile_max in [256, 4096], grid_max in [8, 8192]). */
+cn_status cn_tune_set(cn_ctx *ctx, uint32_t tile_max, uint32_t grid_max);
+
 /* Blocking readback (physics_scene_gpu_state.cpp:4082-4142). Any pointer may be NULL. */
 cn_status cn_readback(cn_ctx *ctx, float *positions_inv_mass, float *previous_positions,
                       float *velocities, uint32_t count);

@@ -70,3 +70,5 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.FluidMaterial) == 48
     assert C.sizeof(abi.StepParams) == 4 * (6 + 4 + 40 + 4)
     assert abi.Meta.fluid_neighbor_total.offset == 16
+    assert abi.Meta.contact_overflows.offset == 12
+    assert C.sizeof(abi.TuneResult) == 24
