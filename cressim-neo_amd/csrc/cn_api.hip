@@ -586,8 +586,8 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.out_pos, N));
     CN_HIP(ctx, dalloc(&s.out_vel, N));
     CN_HIP(ctx, dalloc(&s.out_prev, N));
-    CN_HIP(ctx, dalloc(&s.meta, 8));
-    CN_HIP(ctx, hipMemsetAsync(s.meta, 0, 8 * sizeof(uint32_t), ctx->stream));
+    CN_HIP(ctx, dalloc(&s.meta, kMetaWords));
+    CN_HIP(ctx, hipMemsetAsync(s.meta, 0, kMetaWords * sizeof(uint32_t), ctx->stream));
     CN_HIP(ctx, hipMemsetAsync(s.dacc[0], 0, N * sizeof(float4), ctx->stream));
     CN_HIP(ctx, hipMemsetAsync(s.dacc[1], 0, N * sizeof(float4), ctx->stream));
     ctx->capacity = n;
@@ -992,7 +992,7 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
     const float dt = p->dt / static_cast<float>(substeps);
     const SubstepConsts c = make_consts(ctx, p, dt);
     const uint32_t key_bits = key_bits_of(ctx, p);
-    launch_fill(s.meta, 8, 0u, ctx->stream);
+    launch_fill(s.meta, kMetaWords, 0u, ctx->stream);
     const uint32_t soft_iters = resolve_iters(p->soft_internal_iterations, def);
     const uint32_t contact_iters = resolve_iters(p->soft_contact_iterations, def);
     const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
@@ -1224,7 +1224,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
     const uint32_t tile0 = ctx->tile_max, grid0 = ctx->grid_max;
     if (dalloc(&sp, n) != hipSuccess || dalloc(&sv, n) != hipSuccess || dalloc(&spr, n) != hipSuccess ||
         dalloc(&so, 3 * n) != hipSuccess || dalloc(&sc, n) != hipSuccess || dalloc(&si, n) != hipSuccess ||
-        dalloc(&sm, 8) != hipSuccess) {
+        dalloc(&sm, kMetaWords) != hipSuccess) {
         release();
         return fail(ctx, CN_ERR_OUT_OF_MEMORY, "cn_autotune: no room for the saved state");
     }
@@ -1242,7 +1242,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
         if (e == hipSuccess) e = copy(so + 2 * n, s.out_prev, f4);
         if (e == hipSuccess) e = copy(sc, s.cls[cur0], u4);
         if (e == hipSuccess) e = copy(si, s.id[cur0], u4);
-        if (e == hipSuccess) e = copy(sm, s.meta, 8 * sizeof(uint32_t));
+        if (e == hipSuccess) e = copy(sm, s.meta, kMetaWords * sizeof(uint32_t));
     };
     auto restore = [&]() {
         const size_t f4 = n * sizeof(float4), u4 = n * sizeof(uint32_t);
@@ -1256,7 +1256,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
         if (e == hipSuccess) e = copy(s.out_prev, so + 2 * n, f4);
         if (e == hipSuccess) e = copy(s.cls[cur0], sc, u4);
         if (e == hipSuccess) e = copy(s.id[cur0], si, u4);
-        if (e == hipSuccess) e = copy(s.meta, sm, 8 * sizeof(uint32_t));
+        if (e == hipSuccess) e = copy(s.meta, sm, kMetaWords * sizeof(uint32_t));
     };
     save();
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -1359,7 +1359,7 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     if (!ctx) return CN_ERR_INVALID_ARGUMENT;
     if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
     CN_HIP(ctx, hipSetDevice(ctx->device));
-    uint32_t m[8];
+    uint32_t m[kMetaWords];
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CN_HIP(ctx, hipMemcpy(m, ctx->s.meta, sizeof(m), hipMemcpyDeviceToHost));
     cn_meta r{};
@@ -1370,6 +1370,7 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     r.fluid_neighbor_total = tot;
     r.last_particle_count = ctx->s.n;
     r.contact_overflows = m[4];
+    r.max_cell_occupancy = m[8];
     r.lds_tiles = m[5];
     r.run_table_tiles = m[6];
     r.max_displacement2_bits = m[7];

@@ -33,6 +33,7 @@ constexpr uint32_t kHdScene = kHdFlags + 6;
 constexpr uint32_t kHaloStride = 448;
 // candidate contact partners a particle keeps per substep; more are counted as overflows
 constexpr uint32_t kContactPartners = 32;
+constexpr uint32_t kMetaWords = 16;  // device diagnostics words
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -104,7 +105,8 @@ struct SceneBuffers {
     uint32_t *scene_active = nullptr;  // per scene: 1 = stepped by cn_step, 0 = state left untouched
     uint32_t *pid_scene = nullptr;     // scene of each particle index
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
-    // [4] contact partner overflows, [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits
+    // [4] contact partner overflows, [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits,
+    // [8] longest run of equal keys; kMetaWords words
     uint32_t *meta = nullptr;
     // soft constraints
     uint32_t *slot_of = nullptr;

@@ -281,21 +281,28 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // Morton coordinates) are filtered per particle like the reference (:94-98).
 __global__ __launch_bounds__(kBlock) void k_run_info(const uint32_t *__restrict__ run_start,
                                                      const uint32_t *__restrict__ run_count,
-                                                     const int4 *__restrict__ cellc, int4 *__restrict__ run_info)
+                                                     const int4 *__restrict__ cellc, int4 *__restrict__ run_info,
+                                                     uint32_t *__restrict__ meta)
 {
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= *run_count) return;
-    const uint32_t s = run_start[r], e = run_start[r + 1];
-    const int4 c0 = cellc[s];
-    int pure = 1;
-    for (uint32_t q = s + 1; q < e; ++q) {
-        const int4 cq = cellc[q];
-        if (cq.x != c0.x || cq.y != c0.y || cq.z != c0.z) {
-            pure = 0;
-            break;
+    uint32_t occ = 0;
+    if (r < *run_count) {
+        const uint32_t s = run_start[r], e = run_start[r + 1];
+        occ = e - s;
+        const int4 c0 = cellc[s];
+        int pure = 1;
+        for (uint32_t q = s + 1; q < e; ++q) {
+            const int4 cq = cellc[q];
+            if (cq.x != c0.x || cq.y != c0.y || cq.z != c0.z) {
+                pure = 0;
+                break;
+            }
         }
+        run_info[r] = make_int4(c0.x, c0.y, c0.z, pure);
     }
-    run_info[r] = make_int4(c0.x, c0.y, c0.z, pure);
+    // longest run: wave maximum, and an atomic only when it beats the value already stored
+    for (int o = 32; o > 0; o >>= 1) occ = max(occ, (uint32_t)__shfl_xor((int)occ, o));
+    if ((threadIdx.x & 63u) == 0 && occ > __builtin_nontemporal_load(meta + 8)) atomicMax(meta + 8, occ);
 }
 
 void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
@@ -307,7 +314,7 @@ void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
                        s.ne + s.nb + s.nd + s.nt ? s.slot_of : nullptr, s.classes, s.fluid_mats, s.colliders,
                        s.scenes, s.scene_active, c);
     hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
-                       s.run_info);
+                       s.run_info, s.meta);
     s.cur = nw;
 }
 

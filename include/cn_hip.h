@@ -182,7 +182,8 @@ typedef struct cn_step_params {
 typedef struct cn_meta {
     uint32_t max_fluid_neighbor_count;    /* largest required (untruncated) in-radius count */
     uint32_t fluid_neighbor_truncations;  /* particles whose list hit max_fluid_neighbors */
-    uint32_t max_cell_occupancy;          /* longest run of equal keys in the last sort */
+    uint32_t max_cell_occupancy;          /* longest run of equal keys over the substeps of the last
+                                             cn_step */
     uint32_t contact_overflows;           /* particles with more contact candidates than a list holds
                                              (32), summed over the substeps; the surplus ones
                                              are dropped */

@@ -24,6 +24,7 @@ def run_pair(sc, p, steps, solver, oracle):
         st = oracle.step(osc, p)
         m = solver.meta()
         assert m.contact_overflows == st.contact_overflows
+        assert m.max_cell_occupancy > 0
         pos, _, vel = solver.readback()
         assert np.isfinite(pos).all()
         out.append((float(np.abs(pos - osc.pos).max()), float(np.abs(vel - osc.vel).max()),
