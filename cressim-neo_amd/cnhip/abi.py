@@ -26,6 +26,7 @@ CN_PARTICLE_KIND_FLUID = 1
 CN_KEY_MORTON = 0
 CN_KEY_HASH = 1
 CN_SOLVER_CONFIG_FLOATS = 40
+CN_STEP_NO_CANDIDATE_REUSE = 1
 CN_KERNEL_SLOTS = 16
 
 
@@ -102,7 +103,8 @@ class Meta(C.Structure):
                 ("max_cell_occupancy", C.c_uint32), ("contact_overflows", C.c_uint32),
                 ("fluid_neighbor_total", C.c_uint64), ("last_particle_count", C.c_uint32),
                 ("lds_tiles", C.c_uint32), ("run_table_tiles", C.c_uint32),
-                ("max_displacement2_bits", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("max_displacement2_bits", C.c_uint32), ("candidate_reuses", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class DeviceViews(C.Structure):

@@ -133,6 +133,9 @@ void free_scene(SceneBuffers &s)
     dfree(s.fluid_t2);
     dfree(s.list);
     dfree(s.lcnt);
+    dfree(s.slist);
+    dfree(s.scnt);
+    dfree(s.fluid_sup);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
     dfree(s.ct_list);
@@ -357,6 +360,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.tile_max = ctx->tile_max;
     c.blk_cap = ctx->s.blk_cap;
     c.grid_max = ctx->grid_max;
+    c.superset = (p->flags & CN_STEP_NO_CANDIDATE_REUSE) ? 0u : 1u;
     fluid_tiles(c);
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
@@ -581,6 +585,8 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
     CN_HIP(ctx, dalloc(&s.list, N * (size_t)(cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
+    CN_HIP(ctx, dalloc(&s.slist, N * kSuperSlots));
+    CN_HIP(ctx, dalloc(&s.scnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
     CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, kStateWords * sizeof(uint32_t), ctx->stream));
     CN_HIP(ctx, dalloc(&s.out_pos, N));
@@ -708,6 +714,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         std::memcpy(vel.data() + 4ull * base, pv->velocities, 4ull * m * sizeof(float));
         std::array<uint32_t, 9> last{};
         uint32_t last_id = 0xffffffffu;
+        bool scene_solid = false;
         for (uint32_t i = 0; i < m; ++i) {
             ParticleClass pc{};
             pc.radius = pv->radii[i];
@@ -722,6 +729,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             pc.scene = k;
             ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
             ctx->has_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
+            scene_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             std::array<uint32_t, 9> key;
             std::memcpy(key.data(), &pc, sizeof(pc));
             if (last_id != 0xffffffffu && key == last) {
@@ -745,6 +753,10 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         const std::vector<Collider> sc = visit_ordered_colliders(tb->colliders, tb->collider_count);
         sinfo[k].col0 = (uint32_t)col.size();
         sinfo[k].ncol = (uint32_t)sc.size();
+        sinfo[k].fluid_only = (!scene_solid && (!cv || (cv->edge_count | cv->bend_count | cv->dihedral_count |
+                                                         cv->tet_count) == 0u))
+                                  ? 1u
+                                  : 0u;
         col.insert(col.end(), sc.begin(), sc.end());
         // constraints, shifted to pool indices
         if (cv) {
@@ -859,6 +871,20 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     }
     CN_HIP(ctx, dalloc(&s.fluid_t2, t2.size()));
     if (!t2.empty()) CN_HIP(ctx, hipMemcpy(s.fluid_t2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
+    // candidate lists: radius (1 + skin) h, rounded up; reusable while 2 max|delta| stays under
+    // 0.98 skin h (the margin covers the rounding of the squared distances)
+    std::vector<float2> sup(fmh.size());
+    for (size_t k = 0; k < fmh.size(); ++k) {
+        const double h = std::max(fmh[k].smoothing_radius, 1.0e-4f);
+        const double rs = (1.0 + kSuperSkin) * h, al = 0.49 * kSuperSkin * h;
+        sup[k].x = std::nextafter((float)(rs * rs), 3.4e38f);
+        sup[k].y = std::nextafter((float)(al * al), 0.0f);
+    }
+    dfree(s.fluid_sup);
+    s.fluid_sup = nullptr;
+    CN_HIP(ctx, dalloc(&s.fluid_sup, sup.size()));
+    if (!sup.empty())
+        CN_HIP(ctx, hipMemcpy(s.fluid_sup, sup.data(), sup.size() * sizeof(float2), hipMemcpyHostToDevice));
 
     std::vector<uint32_t> ids(n);
     for (uint32_t i = 0; i < n; ++i) ids[i] = i;
@@ -1374,6 +1400,7 @@ cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out)
     r.lds_tiles = m[5];
     r.run_table_tiles = m[6];
     r.max_displacement2_bits = m[7];
+    r.candidate_reuses = m[9];
     if (out) *out = r;
     if (r.fluid_neighbor_truncations)
         return fail(ctx, CN_ERR_CAPACITY, "fluid neighbor list truncated at max_fluid_neighbors");

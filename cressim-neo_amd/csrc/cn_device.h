@@ -52,6 +52,8 @@ struct SceneInfo {
     uint32_t key_prefix;  // scene index << (3 * morton_bits)
     uint32_t col0;        // first collider of the scene in the visit-ordered collider array
     uint32_t ncol;        // colliders of the scene (at most kMaxColliders)
+    uint32_t fluid_only;  // 1: no solid particle and no constraint, so fluid positions change
+                          // within a substep only by their deltas
 };
 
 // A static box collider in LBVH visit order, with its broad-phase AABB precomputed.
@@ -84,6 +86,7 @@ struct SubstepConsts {
     uint32_t tile_wide;      // halo capacity of the wide delta pass (previous delta read from HBM)
     uint32_t blk_cap;        // cell blocks with a stored halo descriptor
     uint32_t grid_max;       // workgroups of a persistent fluid launch (upper bound)
+    uint32_t superset;       // 1: LDS blocks of fluid-only scenes reuse iteration 0's candidate lists
     float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];
 };

@@ -48,6 +48,9 @@ struct FluidArgs {
     uint32_t *nbr_cnt;
     uint16_t *list;           // listed neighbors as halo offsets, k-major (list[k * n + i]), LDS blocks
     uint32_t *lcnt;           // listed neighbor count
+    uint16_t *slist;          // iteration-0 candidates as halo offsets, k-major, LDS blocks
+    uint16_t *scnt;           // candidate count, or kSuperNone
+    const float2 *sup;        // per fluid material: (candidate threshold, allowed max |delta|^2)
     const uint32_t *run_key;
     const uint32_t *run_start;
     const int4 *run_info;
@@ -145,7 +148,7 @@ struct BlockQueue {
 };
 
 struct MetaAcc {
-    uint32_t req = 0, trunc = 0, listed = 0, lds = 0, runs = 0;
+    uint32_t req = 0, trunc = 0, listed = 0, lds = 0, runs = 0, reuse = 0;
 };
 
 __device__ __forceinline__ void flush_meta(const MetaAcc &ma, const FluidArgs &a)
@@ -153,9 +156,11 @@ __device__ __forceinline__ void flush_meta(const MetaAcc &ma, const FluidArgs &a
     const uint32_t wmax = wave_max(ma.req);
     const uint32_t wtr = wave_sum(ma.trunc);
     const uint32_t wls = wave_sum(ma.listed);
+    const uint32_t wru = wave_sum(ma.reuse);
     if ((threadIdx.x & 63u) == 0) {
         if (wmax) atomicMax(&a.meta[0], wmax);
         if (wtr) atomicAdd(&a.meta[1], wtr);
+        if (wru) atomicAdd(&a.meta[9], wru);
         if (a.it == 0 && wls) atomicAdd((unsigned long long *)&a.meta[2], (unsigned long long)wls);
     }
     if (threadIdx.x == 0) {
@@ -179,14 +184,17 @@ struct DensitySelf {
     float4 s;
     float h, t2, kw;
     bool tension;
+    bool sup;         // candidate lists apply (fluid-only scene, tiled pass)
+    float ts2, allow; // candidate threshold; largest max |delta|^2 under which they stay complete
 };
 
 // Returns false (and writes the unreachable record) for particles that take no part in the solve.
 __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const float4 s, uint32_t ic,
-                                              const FluidArgs &a)
+                                              const FluidArgs &a, bool superset = false)
 {
     x.pc = a.classes[ic];
     x.s = s;
+    x.sup = false;
     if (x.pc.kind != CN_PARTICLE_KIND_FLUID || s.w <= kEps) {
         a.cs[i] = make_float4(kFarAway, kFarAway, kFarAway, 0.0f);
         return false;
@@ -196,6 +204,12 @@ __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const 
     x.t2 = a.t2[x.pc.fluid_mat];
     x.kw = spiky_coef(x.h);
     x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
+    if (superset && a.scenes[x.pc.scene].fluid_only) {
+        const float2 v = a.sup[x.pc.fluid_mat];
+        x.sup = true;
+        x.ts2 = v.x;
+        x.allow = v.y;
+    }
     return true;
 }
 
@@ -651,12 +665,16 @@ __device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
 // listed ones (their halo offsets) into the particle's k-major list, four candidates per step with
 // their LDS loads issued first.  Returns the untruncated neighbor count.  Staged particles with
 // zero inverse mass carry an unreachable position, so the distance test also rejects them.
-template <bool UNIFORM>
+//
+// SUP: also compacts the candidates within the wider radius sqrt(ts2) into ls (at most kSuperSlots,
+// scount = their untruncated number); later iterations filter those instead of walking.
+template <bool UNIFORM, bool SUP>
 __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
                                               const uint32_t *lcls, const DensitySelf &x, uint16_t *li,
-                                              const FluidArgs &a, const SubstepConsts &c)
+                                              const FluidArgs &a, const SubstepConsts &c, uint16_t *ls = nullptr,
+                                              uint32_t *scount = nullptr)
 {
-    uint32_t req = 0;
+    uint32_t req = 0, sreq = 0;
     const size_t n = c.n;
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
@@ -675,12 +693,54 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
             for (uint32_t u = 0; u < 4; ++u) {
                 const uint32_t f = k + u;
                 const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
-                bool ok = (f < f1) & (f != fs) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2);
-                if (!UNIFORM) ok = ok && neighbor_ok(x.pc, a.classes[qc[u]]);
-                if (ok) {
-                    if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
-                    ++req;
+                const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+                if (SUP) {
+                    bool cand = (f < f1) & (f != fs) & (d2 < x.ts2);
+                    if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
+                    if (cand) {
+                        if (sreq < kSuperSlots) ls[(size_t)sreq * n] = (uint16_t)f;
+                        ++sreq;
+                        if (d2 < x.t2) {
+                            if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
+                            ++req;
+                        }
+                    }
+                } else {
+                    bool ok = (f < f1) & (f != fs) & (d2 < x.t2);
+                    if (!UNIFORM) ok = ok && neighbor_ok(x.pc, a.classes[qc[u]]);
+                    if (ok) {
+                        if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
+                        ++req;
+                    }
                 }
+            }
+        }
+    }
+    if (SUP) *scount = sreq;
+    return req;
+}
+
+// A later iteration's list from the iteration-0 candidates: the same walk order, the same in-radius
+// test at the current positions, so the same list while every neighbor is still a candidate
+// (guaranteed while no fluid particle has moved more than half the skin since iteration 0).
+__device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scount, const float4 *lpos,
+                                               const DensitySelf &x, uint16_t *li, const SubstepConsts &c)
+{
+    uint32_t req = 0;
+    const size_t n = c.n;
+    for (uint32_t k = 0; k < scount; k += 4) {
+        uint32_t e[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) e[u] = ls[(size_t)min(k + u, scount - 1) * n];
+        float4 pj[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) pj[u] = lpos[e[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
+            if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
+                if (req < c.cap) li[(size_t)req * n] = (uint16_t)e[u];
+                ++req;
             }
         }
     }
@@ -717,6 +777,9 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     float4 *lpos = dyn;
     uint32_t *lcls = reinterpret_cast<uint32_t *>(dyn + a.tile);
     const uint32_t tid = threadIdx.x;
+    // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
+    // from its iteration-0 position
+    const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
     if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = *a.nblk;
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
@@ -744,11 +807,21 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
                 uint32_t cnt = 0;
-                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a)) {
+                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, c.superset != 0u)) {
                     DensitySum d;
                     uint16_t *li = a.list + i;
                     if (density_walks(x, UNIFORM, c)) {
-                        d.req = scan_rows<UNIFORM>(hoff, hc, fs, lpos, lcls, x, li, a, c);
+                        const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
+                        if (x.sup && a.it == 0) {
+                            uint32_t sreq;
+                            d.req = scan_rows<UNIFORM, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, a.slist + i, &sreq);
+                            a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
+                        } else if (sc != kSuperNone && md2 < x.allow) {
+                            d.req = scan_super(a.slist + i, sc, lpos, x, li, c);
+                            ++ma.reuse;
+                        } else {
+                            d.req = scan_rows<UNIFORM, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
+                        }
                         d.inr = min(d.req, c.cap);
                         density_list(li, d.inr, c.n, lpos, x, d);
                     }
@@ -921,6 +994,9 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.nbr_cnt = s.nbr_cnt;
     a.list = s.list;
     a.lcnt = s.lcnt;
+    a.slist = s.slist;
+    a.scnt = s.scnt;
+    a.sup = s.fluid_sup;
     a.run_key = s.run_key;
     a.run_start = s.run_start;
     a.run_info = s.run_info;

@@ -34,6 +34,11 @@ constexpr uint32_t kHaloStride = 448;
 // candidate contact partners a particle keeps per substep; more are counted as overflows
 constexpr uint32_t kContactPartners = 32;
 constexpr uint32_t kMetaWords = 16;  // device diagnostics words
+// candidate (superset) list slots of a fluid particle: neighbors within (1 + kSuperSkin) h at
+// iteration 0, reused while no fluid particle has moved more than kSuperSkin h / 2
+constexpr uint32_t kSuperSlots = 96;
+constexpr uint16_t kSuperNone = 0xffffu;  // count marker: the candidates did not fit, walk instead
+constexpr float kSuperSkin = 0.1f;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -77,6 +82,9 @@ struct SceneBuffers {
     uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
     float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
     uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, k-major: list[k * n + i]
+    uint16_t *slist = nullptr;      // candidate lists of iteration 0 as halo offsets, k-major
+    uint16_t *scnt = nullptr;       // candidate count per slot, or kSuperNone
+    float2 *fluid_sup = nullptr;    // per fluid material: (candidate threshold, allowed max |delta|^2)
     uint32_t *lcnt = nullptr;       // fluid neighbor count per slot
     uint32_t *nbr = nullptr;        // neighbor lists of the pass entry, k-major: nbr[k * n + i]
     uint32_t *nbr_cnt = nullptr;
@@ -106,7 +114,7 @@ struct SceneBuffers {
     uint32_t *pid_scene = nullptr;     // scene of each particle index
     // diagnostics (device): [0] max required neighbors, [1] truncations, [2..3] neighbor total,
     // [4] contact partner overflows, [5] LDS blocks, [6] run-table blocks, [7] max |delta|^2 bits,
-    // [8] longest run of equal keys; kMetaWords words
+    // [8] longest run of equal keys, [9] candidate-list reuses; kMetaWords words
     uint32_t *meta = nullptr;
     // soft constraints
     uint32_t *slot_of = nullptr;

@@ -174,8 +174,12 @@ typedef struct cn_step_params {
     float cell_size;                 /* particle grid cell size; 0 -> derived like the reference */
     uint32_t max_fluid_neighbors;    /* neighbor slots per fluid particle (reference: 128) */
     uint32_t key_mode;               /* CN_KEY_MORTON or CN_KEY_HASH */
-    uint32_t flags;                  /* reserved, 0 */
+    uint32_t flags;                  /* CN_STEP_* bits, 0 by default */
 } cn_step_params;
+
+/* flags: walk the 27 cells in every fluid iteration instead of filtering the iteration-0
+ * candidate lists (the results are the same either way; for measurement and tests). */
+#define CN_STEP_NO_CANDIDATE_REUSE 1u
 
 /* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
  * physics_solver.cpp:1403-1492), plus the fluid-neighbor truncation the reference lacks. */
@@ -194,7 +198,9 @@ typedef struct cn_meta {
     uint32_t lds_tiles;                   /* density-pass cell blocks whose halo was staged in LDS */
     uint32_t run_table_tiles;             /* density-pass cell blocks walked through the run table */
     uint32_t max_displacement2_bits;      /* float bits of the largest (|delta| + slack)^2 */
-    uint32_t reserved[2];
+    uint32_t candidate_reuses;            /* fluid particle iterations that filtered their iteration-0
+                                             candidate list instead of walking the 27 cells */
+    uint32_t reserved;
 } cn_meta;
 
 /* Raw device pointers of the resident scene (PhysicsGpuSceneView analog). */

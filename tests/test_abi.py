@@ -72,3 +72,4 @@ def test_struct_layouts_match_header():
     assert abi.Meta.fluid_neighbor_total.offset == 16
     assert abi.Meta.contact_overflows.offset == 12
     assert C.sizeof(abi.TuneResult) == 24
+    assert C.sizeof(abi.Meta) == 48 and abi.Meta.candidate_reuses.offset == 40

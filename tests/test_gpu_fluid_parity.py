@@ -184,3 +184,26 @@ def test_large_pbf_4m_properties(solver):
     assert (pos[:, 1:3] <= 2 - r + 1e-6).all()
     # the permutation is a bijection: every particle index exported exactly once
     assert np.abs(pos[:, 3] - sc.pos[:, 3]).max() == 0
+
+
+@pytest.mark.parametrize("case", ["dam_break_64k", "large_pbf_4m"])
+def test_candidate_reuse_equals_full_walk(solver, case):
+    """Later iterations filter the iteration-0 candidate lists (radius 1.1 h) while every fluid
+    particle has moved less than 0.049 h; the lists, and so the bits, equal the 27-cell walk's."""
+    sc = scenes.dam_break_64k() if case == "dam_break_64k" else scenes.large_pbf()
+    steps = 6 if case == "dam_break_64k" else 2
+    out, reuses = {}, {}
+    for flags in (0, cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE):
+        print(f"{case}: flags {flags}", flush=True)
+        p = params(cap=64, flags=flags)
+        solver.reserve(sc.n, 64)
+        solver.upload(sc)
+        reuses[flags] = 0
+        for _ in range(steps):
+            solver.step(p)
+            reuses[flags] += solver.meta().candidate_reuses
+        out[flags] = solver.readback()
+    print(reuses)
+    assert reuses[cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE] == 0 and reuses[0] > 0
+    for a, b in zip(out[0], out[cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE]):
+        np.testing.assert_array_equal(a, b)
