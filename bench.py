@@ -38,7 +38,7 @@ def algorithmic_bytes(kbar: float) -> dict:
             "fluid_delta": 112.0 + 4.0 * kbar}
 
 
-def cpu_baseline(n_sample: int, seed: int = 2):
+def cpu_baseline(n_sample: int, seed: int = 2, min_seconds: float = 10.0, max_steps: int = 12):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cn_oracle  # CPU oracle: the reported baseline only
     import cnhip
@@ -46,16 +46,20 @@ def cpu_baseline(n_sample: int, seed: int = 2):
     sc = scenes.large_pbf(n_sample, seed=seed, box=scenes.scaled_box_for(n_sample))
     p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
     osc = cn_oracle.OracleScene(sc)
-    t0 = time.perf_counter()
-    cn_oracle.step(osc, p)
-    el = time.perf_counter() - t0
-    return {"value": n_sample * SUBSTEPS / el, "unit": "particle-substeps/s", "cores": cn_oracle.threads(),
+    steps, el = 0, 0.0
+    while steps < max_steps and (steps == 0 or el < min_seconds):
+        t0 = time.perf_counter()
+        cn_oracle.step(osc, p)
+        el += time.perf_counter() - t0
+        steps += 1
+        print(f"bench: CPU oracle step {steps}, {el:.1f} s", file=sys.stderr, flush=True)
+    return {"value": n_sample * SUBSTEPS * steps / el, "unit": "particle-substeps/s", "cores": cn_oracle.threads(),
             "kind": "port",
-            "sample": f"1 step (4 substeps x 4 iterations) of a {n_sample}-particle random fill at the same "
-                      f"density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
+            "sample": f"{steps} steps (4 substeps x 4 iterations each) of a {n_sample}-particle random fill at the "
+                      f"same density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
 
 
-def batch_bench(count: int, steps: int, warmup: int, loop_steps: int):
+def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool = True):
     """The batched workload: `count` independent scenes (sizes log-uniform in [2k, 64k], 40 % PBF,
     30 % tet soft bodies, 30 % cloth, seed 3) stepped together as one pool with a graph-captured
     step, against looping the plain path (one context per scene, each stepped in turn)."""
@@ -71,6 +75,9 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int):
     pool = cnhip.Solver(0)
     pool.reserve(total, CAP)
     pool.upload_batch(batch)
+    tuned = pool.autotune(p, steps=1) if tune else None
+    if tuned:
+        print(f"bench: batch autotune {tuned}", file=sys.stderr, flush=True)
     pool.graphs(True)
     for _ in range(warmup):
         pool.step(p)
@@ -114,7 +121,8 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int):
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "particle_substeps_per_s": total * SUBSTEPS * steps / el,
             "loop_plain_env_steps_per_s": env_loop, "loop_plain_steps": loop_steps,
-            "speedup_vs_loop_plain": env / env_loop, "graph": True, "neighbor_truncations": trunc}
+            "speedup_vs_loop_plain": env / env_loop, "graph": True, "neighbor_truncations": trunc,
+            "autotune": tuned}
 
 
 def serve_bench(sessions: int, rates, duration: float, shards: int, loop_requests: int):
@@ -202,6 +210,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no cn_autotune)")
     ap.add_argument("--serve", action="store_true", help="only run the served workload (2048 sessions)")
     ap.add_argument("--serve-sessions", type=int, default=2048)
     ap.add_argument("--serve-duration", type=float, default=2.0)
@@ -236,6 +245,10 @@ def main():
     for _ in range(args.warmup):
         s.step(p)
     s.synchronize()
+    tuned = None
+    if not args.no_tune:  # launch shape of the fluid passes, timed from the warmed-up state
+        tuned = s.autotune(p, steps=2)
+        print(f"bench: autotune {tuned}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.step(p)
@@ -249,7 +262,7 @@ def main():
     maxd = struct.unpack("f", struct.pack("I", meta.max_displacement2_bits))[0] ** 0.5
     print(f"bench: {args.steps} steps in {el:.3f} s, {ms_per_step:.3f} ms/step, K = {kbar:.2f}, "
           f"LDS blocks {meta.lds_tiles}, run-table blocks {meta.run_table_tiles}, "
-          f"max displacement {maxd:.3e}", file=sys.stderr, flush=True)
+          f"candidate reuses {meta.candidate_reuses}, max displacement {maxd:.3e}", file=sys.stderr, flush=True)
 
     # per-kernel HIP-event timing on the context stream, in separate steps
     s.profile(True)
@@ -279,7 +292,7 @@ def main():
         "config": {"workload": "large_pbf_4194304_random_fill_4x2x2m", "particles": sc.n,
                    "substeps_per_step": SUBSTEPS, "iterations": ITERS, "neighbor_cap": CAP,
                    "key_mode": args.key_mode, "mean_neighbors": round(kbar, 3),
-                   "max_neighbors": meta.max_fluid_neighbor_count},
+                   "max_neighbors": meta.max_fluid_neighbor_count, "autotune": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_particle": bytes_pp[dom], "avg_launch_ms": avg[dom]},
@@ -289,7 +302,7 @@ def main():
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
     }
     if not args.no_batch:
-        out["batch"] = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps)
+        out["batch"] = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps, tune=not args.no_tune)
     if not args.no_cpu:
         print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

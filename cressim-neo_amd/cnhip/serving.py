@@ -23,14 +23,18 @@ import numpy as np
 from . import Scene, Solver
 
 # Device bytes per particle of a resident scene (state double-buffered in slot order, broadphase,
-# fluid records and lists at 64 neighbor slots, halo descriptors) and per soft constraint.
-BYTES_PER_PARTICLE = 640
+# fluid records, neighbor lists at 64 slots and candidate lists at 96, halo descriptors), per
+# particle of a scene with solid particles (contact candidates and records, 32 per particle) and
+# per soft constraint.
+BYTES_PER_PARTICLE = 840
+BYTES_PER_CONTACT_PARTICLE = 664
 BYTES_PER_CONSTRAINT = 80
 
 
 def estimate_bytes(sc: Scene) -> int:
     cons = sum(len(x) for x in (sc.edges, sc.bends, sc.dihedrals, sc.tets) if x is not None)
-    return int(sc.n * BYTES_PER_PARTICLE + cons * BYTES_PER_CONSTRAINT)
+    contact = BYTES_PER_CONTACT_PARTICLE if bool((np.asarray(sc.kinds) == 0).any()) else 0
+    return int(sc.n * (BYTES_PER_PARTICLE + contact) + cons * BYTES_PER_CONSTRAINT)
 
 
 def admit(scenes, budget_bytes: int):
