@@ -292,7 +292,8 @@ def main():
         "config": {"workload": "large_pbf_4194304_random_fill_4x2x2m", "particles": sc.n,
                    "substeps_per_step": SUBSTEPS, "iterations": ITERS, "neighbor_cap": CAP,
                    "key_mode": args.key_mode, "mean_neighbors": round(kbar, 3),
-                   "max_neighbors": meta.max_fluid_neighbor_count, "autotune": tuned},
+                   "max_neighbors": meta.max_fluid_neighbor_count},
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_particle": bytes_pp[dom], "avg_launch_ms": avg[dom]},
