@@ -10,7 +10,9 @@ pytestmark = pytest.mark.gpu
 
 
 def params(**kw):
-    return cnhip.default_params(dt=1 / 120, substeps=4, default_iterations=4, max_fluid_neighbors=64, **kw)
+    base = dict(dt=1 / 120, substeps=4, default_iterations=4, max_fluid_neighbors=64)
+    base.update(kw)
+    return cnhip.default_params(**base)
 
 
 def small_batch():
