@@ -135,7 +135,6 @@ void free_scene(SceneBuffers &s)
     dfree(s.lcnt);
     dfree(s.slist);
     dfree(s.scnt);
-    dfree(s.fluid_sup);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
     dfree(s.ct_list);
@@ -361,6 +360,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.blk_cap = ctx->s.blk_cap;
     c.grid_max = ctx->grid_max;
     c.superset = (p->flags & CN_STEP_NO_CANDIDATE_REUSE) ? 0u : 1u;
+    c.fluid_iters = p->fluid_iterations > 0 ? p->fluid_iterations : std::max<uint32_t>(p->default_iterations, 1u);
     fluid_tiles(c);
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
@@ -871,20 +871,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     }
     CN_HIP(ctx, dalloc(&s.fluid_t2, t2.size()));
     if (!t2.empty()) CN_HIP(ctx, hipMemcpy(s.fluid_t2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
-    // candidate lists: radius (1 + skin) h, rounded up; reusable while 2 max|delta| stays under
-    // 0.98 skin h (the margin covers the rounding of the squared distances)
-    std::vector<float2> sup(fmh.size());
-    for (size_t k = 0; k < fmh.size(); ++k) {
-        const double h = std::max(fmh[k].smoothing_radius, 1.0e-4f);
-        const double rs = (1.0 + kSuperSkin) * h, al = 0.49 * kSuperSkin * h;
-        sup[k].x = std::nextafter((float)(rs * rs), 3.4e38f);
-        sup[k].y = std::nextafter((float)(al * al), 0.0f);
-    }
-    dfree(s.fluid_sup);
-    s.fluid_sup = nullptr;
-    CN_HIP(ctx, dalloc(&s.fluid_sup, sup.size()));
-    if (!sup.empty())
-        CN_HIP(ctx, hipMemcpy(s.fluid_sup, sup.data(), sup.size() * sizeof(float2), hipMemcpyHostToDevice));
+
 
     std::vector<uint32_t> ids(n);
     for (uint32_t i = 0; i < n; ++i) ids[i] = i;

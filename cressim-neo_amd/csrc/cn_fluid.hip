@@ -50,7 +50,6 @@ struct FluidArgs {
     uint32_t *lcnt;           // listed neighbor count
     uint16_t *slist;          // iteration-0 candidates as halo offsets, k-major, LDS blocks
     uint16_t *scnt;           // candidate count, or kSuperNone
-    const float2 *sup;        // per fluid material: (candidate threshold, allowed max |delta|^2)
     const uint32_t *run_key;
     const uint32_t *run_start;
     const int4 *run_info;
@@ -189,8 +188,10 @@ struct DensitySelf {
 };
 
 // Returns false (and writes the unreachable record) for particles that take no part in the solve.
+// dprev >= 0: candidate lists apply to fluid-only scenes, with a skin from dprev, the previous
+// substep's largest displacement.
 __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const float4 s, uint32_t ic,
-                                              const FluidArgs &a, bool superset = false)
+                                              const FluidArgs &a, float dprev = -1.0f)
 {
     x.pc = a.classes[ic];
     x.s = s;
@@ -204,11 +205,15 @@ __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const 
     x.t2 = a.t2[x.pc.fluid_mat];
     x.kw = spiky_coef(x.h);
     x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
-    if (superset && a.scenes[x.pc.scene].fluid_only) {
-        const float2 v = a.sup[x.pc.fluid_mat];
+    if (dprev >= 0.0f && a.scenes[x.pc.scene].fluid_only) {
+        // a pair in radius at iteration k was within h + 2 max|delta_k| at iteration 0, so the
+        // candidates within h + skin hold every later list while 2 max|delta_k| < 0.98 skin (the
+        // margins cover the rounding of the squares)
+        const float skin = fmind(fmaxd(2.5f * dprev, 0.05f * x.h), 0.3f * x.h);
+        const float hs = x.h + skin, al = 0.49f * skin;
         x.sup = true;
-        x.ts2 = v.x;
-        x.allow = v.y;
+        x.ts2 = hs * hs * 1.000001f;
+        x.allow = al * al * 0.999999f;
     }
     return true;
 }
@@ -780,6 +785,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
     // from its iteration-0 position
     const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
+    const float dprev = c.superset ? fsqrt(__uint_as_float(a.iter_state[kStatePrevDisp])) : -1.0f;
     if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = *a.nblk;
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
@@ -807,7 +813,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
                 uint32_t cnt = 0;
-                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, c.superset != 0u)) {
+                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, dprev)) {
                     DensitySum d;
                     uint16_t *li = a.list + i;
                     if (density_walks(x, UNIFORM, c)) {
@@ -996,7 +1002,6 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.lcnt = s.lcnt;
     a.slist = s.slist;
     a.scnt = s.scnt;
-    a.sup = s.fluid_sup;
     a.run_key = s.run_key;
     a.run_start = s.run_start;
     a.run_info = s.run_info;

@@ -18,7 +18,9 @@ constexpr uint32_t kStateDelta = kIterStateWords * kXcds;
 constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds;
 constexpr uint32_t kStateWideCount = kStateWideQueue + kIterStateWords;
 constexpr uint32_t kStateMaxDisp = kStateWideCount + 1;
-constexpr uint32_t kStateWords = kStateMaxDisp + kIterStateWords;
+constexpr uint32_t kStatePrevDisp = kStateMaxDisp + kIterStateWords;  // the last iteration's word of the
+                                                                        // previous substep
+constexpr uint32_t kStateWords = kStatePrevDisp + 1;
 // per-block halo descriptor: start slot and exclusive particle offset of each of the 6x6x6 halo
 // cells (x fastest), the halo particle total, a purity flag, the block's particle range and the
 // cell coordinates of its low corner
@@ -34,11 +36,11 @@ constexpr uint32_t kHaloStride = 448;
 // candidate contact partners a particle keeps per substep; more are counted as overflows
 constexpr uint32_t kContactPartners = 32;
 constexpr uint32_t kMetaWords = 16;  // device diagnostics words
-// candidate (superset) list slots of a fluid particle: neighbors within (1 + kSuperSkin) h at
-// iteration 0, reused while no fluid particle has moved more than kSuperSkin h / 2
+// candidate list slots of a fluid particle: its neighbors within h + skin at iteration 0, reused
+// while no fluid particle has moved more than about skin / 2; the skin is 2.5 times the previous
+// substep's largest displacement, clamped to [0.05 h, 0.3 h]
 constexpr uint32_t kSuperSlots = 96;
 constexpr uint16_t kSuperNone = 0xffffu;  // count marker: the candidates did not fit, walk instead
-constexpr float kSuperSkin = 0.1f;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -84,7 +86,6 @@ struct SceneBuffers {
     uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, k-major: list[k * n + i]
     uint16_t *slist = nullptr;      // candidate lists of iteration 0 as halo offsets, k-major
     uint16_t *scnt = nullptr;       // candidate count per slot, or kSuperNone
-    float2 *fluid_sup = nullptr;    // per fluid material: (candidate threshold, allowed max |delta|^2)
     uint32_t *lcnt = nullptr;       // fluid neighbor count per slot
     uint32_t *nbr = nullptr;        // neighbor lists of the pass entry, k-major: nbr[k * n + i]
     uint32_t *nbr_cnt = nullptr;

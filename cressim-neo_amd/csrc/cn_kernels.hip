@@ -9,6 +9,8 @@
 // (physics_fluid_build_neighbors.cs.hlsl:67-141), so results do not depend on the slot order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cn_internal.h"
 
 namespace cn {
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ block_off,
                                                      uint32_t *__restrict__ blk_first_run,
                                                      uint32_t *__restrict__ nblk, uint32_t *__restrict__ work,
-                                                     uint32_t nwork)
+                                                     uint32_t nwork, uint32_t last_word, uint32_t prev_word)
 {
     __shared__ uint32_t sh[kBlock];
     const uint32_t R = *run_count;
@@ -370,8 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
             *nblk = idx;
         }
     }
-    if (blockIdx.x == 0)
+    if (blockIdx.x == 0) {
         for (uint32_t w = threadIdx.x; w < nwork; w += kBlock) work[w] = 0;
+        if (threadIdx.x == 0) work[prev_word] = work[last_word];
+    }
 }
 
 void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
@@ -379,8 +383,9 @@ void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
     const uint32_t nb = (c.n + kRunTile - 1) / kRunTile;
     hipLaunchKernelGGL(k_blk_count, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs);
     hipLaunchKernelGGL(k_scan_block_runs, dim3(1), dim3(1024), 0, st, s.block_runs, nb, s.nblk);
+    const uint32_t last = kStateMaxDisp + std::min<uint32_t>(std::max<uint32_t>(c.fluid_iters, 1u), kIterStateWords) - 1;
     hipLaunchKernelGGL(k_blk_emit, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs,
-                       s.blk_first_run, s.nblk, s.iter_state, kStateMaxDisp);
+                       s.blk_first_run, s.nblk, s.iter_state, kStateMaxDisp, last, kStatePrevDisp);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -188,10 +188,11 @@ def test_large_pbf_4m_properties(solver):
 
 @pytest.mark.parametrize("case", ["dam_break_64k", "large_pbf_4m"])
 def test_candidate_reuse_equals_full_walk(solver, case):
-    """Later iterations filter the iteration-0 candidate lists (radius 1.1 h) while every fluid
-    particle has moved less than 0.049 h; the lists, and so the bits, equal the 27-cell walk's."""
+    """Later iterations filter the iteration-0 candidate lists (radius h + skin, the skin 2.5 times
+    the previous substep's largest displacement within [0.05 h, 0.3 h]) while every fluid particle
+    has moved less than 0.49 skin; the lists, and so the bits, equal the 27-cell walk's."""
     sc = scenes.dam_break_64k() if case == "dam_break_64k" else scenes.large_pbf()
-    steps = 6 if case == "dam_break_64k" else 2
+    steps = 6 if case == "dam_break_64k" else 3
     out, reuses = {}, {}
     for flags in (0, cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE):
         print(f"{case}: flags {flags}", flush=True)
