@@ -211,6 +211,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no cn_autotune)")
+    ap.add_argument("--shape", default="", metavar="TILE,GRID",
+                    help="set the fluid launch shape (cn_tune_set) instead of autotuning it")
     ap.add_argument("--serve", action="store_true", help="only run the served workload (2048 sessions)")
     ap.add_argument("--serve-sessions", type=int, default=2048)
     ap.add_argument("--serve-duration", type=float, default=2.0)
@@ -246,7 +248,11 @@ def main():
         s.step(p)
     s.synchronize()
     tuned = None
-    if not args.no_tune:  # launch shape of the fluid passes, timed from the warmed-up state
+    if args.shape:
+        tile, grid = (int(v) for v in args.shape.split(","))
+        s.tune_set(tile, grid)
+        tuned = {"tile_max": tile, "grid_max": grid, "set": True}
+    elif not args.no_tune:  # launch shape of the fluid passes, timed from the warmed-up state
         tuned = s.autotune(p, steps=2)
         print(f"bench: autotune {tuned}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
