@@ -58,7 +58,9 @@ struct cn_ctx {
         int slot;
         hipEvent_t a, b;
     };
-    std::vector<Ev> events;
+    std::vector<Ev> events;           // recorded pairs since the last cn_profile_read
+    std::vector<hipEvent_t> ev_pool;  // timing events, created once and reused
+    size_t ev_used = 0;
     cn_kernel_stats stats{};
     cn_meta last_meta{};
 };
@@ -323,9 +325,23 @@ struct ProfScope {
             std::snprintf(st.name[slot], 32, "%s", name);
         }
         if (slot < 0) return;
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
+        a = take();
+        b = take();
+        if (!a || !b) {
+            a = nullptr;
+            return;
+        }
         (void)hipEventRecord(a, ctx->stream);
+    }
+    // Events without the system-scope fences: they only time work on the context's stream.
+    hipEvent_t take()
+    {
+        if (ctx->ev_used == ctx->ev_pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+            ctx->ev_pool.push_back(e);
+        }
+        return ctx->ev_pool[ctx->ev_used++];
     }
     ~ProfScope()
     {
@@ -428,10 +444,7 @@ void cn_ctx_destroy(cn_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &e : ctx->events) {
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-    }
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     clear_graphs(ctx);
     free_scene(ctx->s);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -583,9 +596,9 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
     CN_HIP(ctx, dalloc(&s.blk_halo, (size_t)s.blk_cap * kHaloStride));
     CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
-    CN_HIP(ctx, dalloc(&s.list, N * (size_t)(cap ? cap : 1)));
+    CN_HIP(ctx, dalloc(&s.list, list_slots(N) * (cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
-    CN_HIP(ctx, dalloc(&s.slist, N * kSuperSlots));
+    CN_HIP(ctx, dalloc(&s.slist, list_slots(N) * kSuperSlots));
     CN_HIP(ctx, dalloc(&s.scnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
     CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, kStateWords * sizeof(uint32_t), ctx->stream));
@@ -714,7 +727,8 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         std::memcpy(vel.data() + 4ull * base, pv->velocities, 4ull * m * sizeof(float));
         std::array<uint32_t, 9> last{};
         uint32_t last_id = 0xffffffffu;
-        bool scene_solid = false;
+        bool scene_solid = false, scene_fluid = false, two_groups = false;
+        uint32_t group0 = 0xffffffffu;
         for (uint32_t i = 0; i < m; ++i) {
             ParticleClass pc{};
             pc.radius = pv->radii[i];
@@ -730,6 +744,12 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
             ctx->has_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             scene_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
+            scene_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
+            if (pc.kind == CN_PARTICLE_KIND_SOLID) {
+                const uint32_t g = pc.phase & 0x7fffffffu;
+                if (group0 == 0xffffffffu) group0 = g;
+                two_groups |= g != group0;
+            }
             std::array<uint32_t, 9> key;
             std::memcpy(key.data(), &pc, sizeof(pc));
             if (last_id != 0xffffffffu && key == last) {
@@ -757,6 +777,8 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
                                                          cv->tet_count) == 0u))
                                   ? 1u
                                   : 0u;
+        sinfo[k].has_fluid = scene_fluid ? 1u : 0u;
+        sinfo[k].pairs = (scene_solid && scene_fluid) || two_groups ? 1u : 0u;
         col.insert(col.end(), sc.begin(), sc.end());
         // constraints, shifted to pool indices
         if (cv) {
@@ -1273,8 +1295,8 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
     };
     save();
     hipEvent_t ea = nullptr, eb = nullptr;
-    if (e == hipSuccess) e = hipEventCreate(&ea);
-    if (e == hipSuccess) e = hipEventCreate(&eb);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ea, hipEventDisableSystemFence);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&eb, hipEventDisableSystemFence);
     const uint32_t tiles[] = {1024, 2048, 3072, 4096};
     const uint32_t grids[] = {256, 512, 1024, 2048};
     float best = 3.4e38f, dflt = 0.0f;
@@ -1532,10 +1554,9 @@ cn_status cn_profile_read(cn_ctx *ctx, cn_kernel_stats *out)
             ctx->stats.total_ms[e.slot] += ms;
             ctx->stats.launches[e.slot] += 1;
         }
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
     }
     ctx->events.clear();
+    ctx->ev_used = 0;
     *out = ctx->stats;
     return CN_OK;
 }

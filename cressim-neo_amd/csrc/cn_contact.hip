@@ -41,6 +41,7 @@ struct ContactArgs {
     const ParticleClass *classes;
     const float4 *part_mats;
     const uint32_t *active;   // per scene
+    const SceneInfo *scenes;
     RunTable rt;
     uint32_t *plist;          // partner slots, k-major: plist[k * n + t]
     uint32_t *pcount;
@@ -117,9 +118,10 @@ __global__ __launch_bounds__(kBlock) void k_contact_partners(ContactArgs a, Subs
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n) return;
-    if (!a.active[pclass(a, t).scene]) return;
+    const uint32_t scene = pclass(a, t).scene;
+    if (!a.active[scene]) return;
     uint32_t cnt = 0;
-    walk_runs(a.rt.cellc[t], a.rt, c, [&](uint32_t q) {
+    if (a.scenes[scene].pairs) walk_runs(a.rt.cellc[t], a.rt, c, [&](uint32_t q) {
         if (q == t) return true;
         uint32_t lo, hi;
         bool t_hi;
@@ -322,6 +324,7 @@ static ContactArgs contact_args(const SceneBuffers &s, const SubstepConsts &c, i
     a.classes = s.classes;
     a.part_mats = s.part_mats;
     a.active = s.scene_active;
+    a.scenes = s.scenes;
     a.rt = RunTable{s.run_key, s.run_start, s.run_info, s.bucket, c.bucket_shift, s.cellc};
     a.plist = s.ct_list;
     a.pcount = s.ct_count;

@@ -54,6 +54,9 @@ struct SceneInfo {
     uint32_t ncol;        // colliders of the scene (at most kMaxColliders)
     uint32_t fluid_only;  // 1: no solid particle and no constraint, so fluid positions change
                           // within a substep only by their deltas
+    uint32_t has_fluid;   // 1: some fluid particle (the fluid passes have work in the scene)
+    uint32_t pairs;       // 1: a particle-particle contact pair can exist (a solid particle and a
+                          // fluid one, or solid particles of two phase groups)
 };
 
 // A static box collider in LBVH visit order, with its broad-phase AABB precomputed.

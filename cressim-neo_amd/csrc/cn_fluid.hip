@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "cn_internal.h"
 #include "cn_walk.h"
@@ -46,9 +47,9 @@ struct FluidArgs {
     float4 *nc;               // surface normal * tension, C
     uint32_t *nbr;
     uint32_t *nbr_cnt;
-    uint16_t *list;           // listed neighbors as halo offsets, k-major (list[k * n + i]), LDS blocks
+    uint16_t *list;           // listed neighbors as halo offsets in slot groups (list_group_base), LDS blocks
     uint32_t *lcnt;           // listed neighbor count
-    uint16_t *slist;          // iteration-0 candidates as halo offsets, k-major, LDS blocks
+    uint16_t *slist;          // iteration-0 candidates as halo offsets in slot groups, LDS blocks
     uint16_t *scnt;           // candidate count, or kSuperNone
     const uint32_t *run_key;
     const uint32_t *run_start;
@@ -551,6 +552,17 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
     const uint32_t tid = threadIdx.x;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t p0 = a.run_start[a.blk_first_run[b]], p1 = a.run_start[a.blk_first_run[b + 1]];
+        const uint32_t scene = a.classes[a.cls[p0]].scene;
+        uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
+        if (!a.scenes[scene].has_fluid) {  // a scene without fluid: no fluid pass visits the block
+            if (tid == 0) {
+                hd[kHdFlags] = kHdNoFluid;
+                hd[kHdP0] = p0;
+                hd[kHdP1] = p1;
+                hd[kHdScene] = scene;
+            }
+            continue;
+        }
         const int4 c0 = a.cellc[p0];
         const int bx = c0.x & ~3, by = c0.y & ~3, bz = c0.z & ~3;
         // particles of a wrapped Morton block that lie outside the aligned block, particles whose own
@@ -580,7 +592,6 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
         const int any_bad = __syncthreads_or(bad ? 1 : 0);
         uint32_t total;
         const uint32_t off = block_exclusive_scan(cnt, scan, total);
-        uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
         if (tid < kHalo) {
             hd[kHdStart + tid] = st;
             hd[kHdOff + tid] = off;
@@ -596,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
             hd[kHdBase] = (uint32_t)bx;
             hd[kHdBase + 1] = (uint32_t)by;
             hd[kHdBase + 2] = (uint32_t)bz;
-            hd[kHdScene] = a.classes[a.cls[p0]].scene;
+            hd[kHdScene] = scene;
         }
     }
 }
@@ -630,7 +641,7 @@ __device__ __forceinline__ bool tile_begin(Next &&next, uint32_t *s_blk, uint32_
         const uint32_t *hd = a.blk_halo + (size_t)b * kHaloStride;
         ts.p0 = hd[kHdP0];
         ts.p1 = hd[kHdP1];
-        if (!a.active[hd[kHdScene]]) {  // a scene this step leaves untouched
+        if (!a.active[hd[kHdScene]] || hd[kHdFlags] == kHdNoFluid) {  // untouched this step, or no fluid
             ts.skip = true;
             __syncthreads();
             return true;
@@ -652,6 +663,8 @@ __device__ __forceinline__ bool tile_begin(Next &&next, uint32_t *s_blk, uint32_
     } else {
         ts.p0 = a.run_start[a.blk_first_run[b]];
         ts.p1 = a.run_start[a.blk_first_run[b + 1]];
+        const uint32_t scene = a.classes[a.cls[ts.p0]].scene;
+        ts.skip = !a.active[scene] || !a.scenes[scene].has_fluid;
     }
     __syncthreads();
     return true;
@@ -667,7 +680,7 @@ __device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
 // ---------------------------------------------------------------------------------------------
 
 // The density walk over the staged halo: tests every candidate in walk order and compacts the
-// listed ones (their halo offsets) into the particle's k-major list, four candidates per step with
+// listed ones (their halo offsets) into the particle's list, four candidates per step with
 // their LDS loads issued first.  Returns the untruncated neighbor count.  Staged particles with
 // zero inverse mass carry an unreachable position, so the distance test also rejects them.
 //
@@ -680,7 +693,6 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
                                               uint32_t *scount = nullptr)
 {
     uint32_t req = 0, sreq = 0;
-    const size_t n = c.n;
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
         const int h0 = hc - 1 + 6 * (r % 3 - 1) + 36 * (r / 3 - 1);
@@ -703,10 +715,10 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
                     bool cand = (f < f1) & (f != fs) & (d2 < x.ts2);
                     if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
                     if (cand) {
-                        if (sreq < kSuperSlots) ls[(size_t)sreq * n] = (uint16_t)f;
+                        if (sreq < kSuperSlots) ls[sreq * kListGroup] = (uint16_t)f;
                         ++sreq;
                         if (d2 < x.t2) {
-                            if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
+                            if (req < c.cap) li[req * kListGroup] = (uint16_t)f;
                             ++req;
                         }
                     }
@@ -714,7 +726,7 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
                     bool ok = (f < f1) & (f != fs) & (d2 < x.t2);
                     if (!UNIFORM) ok = ok && neighbor_ok(x.pc, a.classes[qc[u]]);
                     if (ok) {
-                        if (req < c.cap) li[(size_t)req * n] = (uint16_t)f;
+                        if (req < c.cap) li[req * kListGroup] = (uint16_t)f;
                         ++req;
                     }
                 }
@@ -732,11 +744,10 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
                                                const DensitySelf &x, uint16_t *li, const SubstepConsts &c)
 {
     uint32_t req = 0;
-    const size_t n = c.n;
     for (uint32_t k = 0; k < scount; k += 4) {
         uint32_t e[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = ls[(size_t)min(k + u, scount - 1) * n];
+        for (uint32_t u = 0; u < 4; ++u) e[u] = ls[min(k + u, scount - 1) * kListGroup];
         float4 pj[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) pj[u] = lpos[e[u]];
@@ -744,7 +755,7 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
         for (uint32_t u = 0; u < 4; ++u) {
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
-                if (req < c.cap) li[(size_t)req * n] = (uint16_t)e[u];
+                if (req < c.cap) li[req * kListGroup] = (uint16_t)e[u];
                 ++req;
             }
         }
@@ -753,13 +764,13 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
 }
 
 // Density sums over a compacted list, four entries per step (loads first, sums in list order).
-__device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, size_t n, const float4 *lpos,
+__device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, const float4 *lpos,
                                              const DensitySelf &x, DensitySum &d)
 {
     for (uint32_t k = 0; k < cnt; k += 4) {
         uint32_t e[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = li[(size_t)min(k + u, cnt - 1) * n];
+        for (uint32_t u = 0; u < 4; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
         DensityTerm t[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) t[u] = density_term(x, lpos[e[u]]);
@@ -799,15 +810,39 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
         }
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
+            // a halo of one particle class (the common case in a pool of single-material scenes)
+            // takes the class filter out of the walk, as a pool of one class does
+            const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
+            bool mixed = false;
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 float4 v = a.cur[q];
                 if (!(v.w > kEps)) v.x = v.y = v.z = kFarAway;
                 lpos[f] = v;
-                if (!UNIFORM) lcls[f] = a.cls[q];
+                if (!UNIFORM) {
+                    const uint32_t qc = a.cls[q];
+                    lcls[f] = qc;
+                    mixed |= qc != ref;
+                }
             }
-            __syncthreads();
+            const bool bu = UNIFORM || !__syncthreads_or(mixed ? 1 : 0);
+            auto scan = [&](auto bu_tag, const DensitySelf &x, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
+                constexpr bool BU = decltype(bu_tag)::value;
+                const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
+                if (x.sup && a.it == 0) {
+                    uint32_t sreq;
+                    const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c,
+                                                             a.slist + list_group_base(i, kSuperSlots), &sreq);
+                    a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
+                    return req;
+                }
+                if (sc != kSuperNone && md2 < x.allow) {
+                    ++ma.reuse;
+                    return scan_super(a.slist + list_group_base(i, kSuperSlots), sc, lpos, x, li, c);
+                }
+                return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
+            };
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
@@ -815,21 +850,11 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                 uint32_t cnt = 0;
                 if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, dprev)) {
                     DensitySum d;
-                    uint16_t *li = a.list + i;
-                    if (density_walks(x, UNIFORM, c)) {
-                        const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
-                        if (x.sup && a.it == 0) {
-                            uint32_t sreq;
-                            d.req = scan_rows<UNIFORM, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, a.slist + i, &sreq);
-                            a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
-                        } else if (sc != kSuperNone && md2 < x.allow) {
-                            d.req = scan_super(a.slist + i, sc, lpos, x, li, c);
-                            ++ma.reuse;
-                        } else {
-                            d.req = scan_rows<UNIFORM, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
-                        }
+                    uint16_t *li = a.list + list_group_base(i, c.cap);
+                    if (density_walks(x, bu, c)) {
+                        d.req = bu ? scan(std::true_type{}, x, hc, fs, i, li) : scan(std::false_type{}, x, hc, fs, i, li);
                         d.inr = min(d.req, c.cap);
-                        density_list(li, d.inr, c.n, lpos, x, d);
+                        density_list(li, d.inr, lpos, x, d);
                     }
                     cnt = d.inr;
                     density_end(i, d, x, ma, a, c);
@@ -884,6 +909,8 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
         }
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
+            const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
+            bool mixed = false;
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
@@ -891,13 +918,18 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                 if (!WIDE && use_d) ldin[f] = a.din[q];
                 if (WIDE) lslot[f] = q;
                 if (tension) lnc[f] = a.nc[q];
-                if (!UNIFORM) lcls[f] = a.cls[q];
+                if (!UNIFORM) {
+                    const uint32_t qc = a.cls[q];
+                    lcls[f] = qc;
+                    mixed |= qc != ref;
+                }
             }
-            __syncthreads();
+            // one particle class in the halo: every listed neighbor has the particle's material
+            const bool bu = UNIFORM || !__syncthreads_or(mixed ? 1 : 0);
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
-                const uint16_t *li = a.list + i;
+                const uint16_t *li = a.list + list_group_base(i, c.cap);
                 const float d2 = delta_particle<UNIFORM>(
                     i, fs,
                     [&](const DeltaSelf &x, DeltaSum &d) {
@@ -907,7 +939,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                         for (uint32_t k = 0; k < cnt; k += 4) {
                             uint32_t e[4];
 #pragma unroll
-                            for (uint32_t u = 0; u < 4; ++u) e[u] = li[(size_t)min(k + u, cnt - 1) * c.n];
+                            for (uint32_t u = 0; u < 4; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
                             float4 dq[4];
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u)
@@ -915,8 +947,9 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                             DeltaTerm t[4];
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u)
-                                t[u] = delta_term<UNIFORM>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
-                                                           UNIFORM ? 0u : lcls[e[u]], a);
+                                t[u] = bu ? delta_term<true>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero, 0u, a)
+                                          : delta_term<false>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
+                                                              lcls[e[u]], a);
 #pragma unroll
                             for (uint32_t u = 0; u < 4; ++u)
                                 if (k + u < cnt) delta_add(d, t[u]);

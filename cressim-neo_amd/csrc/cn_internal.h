@@ -33,6 +33,7 @@ constexpr uint32_t kHdP1 = kHdFlags + 2;
 constexpr uint32_t kHdBase = kHdFlags + 3;      // 3 words
 constexpr uint32_t kHdScene = kHdFlags + 6;
 constexpr uint32_t kHaloStride = 448;
+constexpr uint32_t kHdNoFluid = 2u;  // kHdFlags value of a block in a scene without fluid particles
 // candidate contact partners a particle keeps per substep; more are counted as overflows
 constexpr uint32_t kContactPartners = 32;
 constexpr uint32_t kMetaWords = 16;  // device diagnostics words
@@ -41,6 +42,15 @@ constexpr uint32_t kMetaWords = 16;  // device diagnostics words
 // substep's largest displacement, clamped to [0.05 h, 0.3 h]
 constexpr uint32_t kSuperSlots = 96;
 constexpr uint16_t kSuperNone = 0xffffu;  // count marker: the candidates did not fit, walk instead
+// Per-slot lists are stored in groups of kListGroup consecutive slots: entry k of slot i sits at
+// (i / kListGroup) * kListGroup * rows + k * kListGroup + i % kListGroup, so one wave's row is one
+// 128-byte line and a group's rows are contiguous (few pages per wave).
+constexpr uint32_t kListGroup = 64;
+__host__ __device__ inline size_t list_group_base(uint32_t i, uint32_t rows)
+{
+    return (size_t)(i / kListGroup) * kListGroup * rows + (i % kListGroup);
+}
+__host__ __device__ inline size_t list_slots(size_t n) { return (n + kListGroup - 1) / kListGroup * kListGroup; }
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);
@@ -83,8 +93,8 @@ struct SceneBuffers {
     uint32_t *wide_blk = nullptr;   // blocks whose halo fits the wide delta tile only (blk_cap)
     uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
     float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
-    uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, k-major: list[k * n + i]
-    uint16_t *slist = nullptr;      // candidate lists of iteration 0 as halo offsets, k-major
+    uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, in slot groups (list_row)
+    uint16_t *slist = nullptr;      // candidate lists of iteration 0 as halo offsets, in slot groups
     uint16_t *scnt = nullptr;       // candidate count per slot, or kSuperNone
     uint32_t *lcnt = nullptr;       // fluid neighbor count per slot
     uint32_t *nbr = nullptr;        // neighbor lists of the pass entry, k-major: nbr[k * n + i]

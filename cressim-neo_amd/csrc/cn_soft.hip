@@ -243,19 +243,20 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
 }
 
-// Per slot: the particle's incident records in list order, summed, then added to the position.
+// Per particle, in particle-index order (the incidence lists and the constraint records of
+// neighboring particles are neighbors in memory): its incident records in list order, summed,
+// then added to its position in slot order.
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
                                                               const uint32_t *__restrict__ ref,
                                                               const float4 *__restrict__ rec, uint32_t per)
 {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= a.n) return;
+    const uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
+    if (pid >= a.n) return;
+    const uint32_t k0 = start[pid], k1 = start[pid + 1];
+    if (k0 == k1 || !live(a, pid)) return;
+    const uint32_t t = a.slot_of[pid];
     float4 p = a.pos[t];
     if (p.w <= kEps) return;
-    const uint32_t pid = a.id[t];
-    if (!live(a, pid)) return;
-    const uint32_t k0 = start[pid], k1 = start[pid + 1];
-    if (k0 == k1) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
     for (uint32_t k = k0; k < k1; ++k) {
         const uint32_t r = ref[k];
