@@ -826,7 +826,8 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                     mixed |= qc != ref;
                 }
             }
-            const bool bu = UNIFORM || !__syncthreads_or(mixed ? 1 : 0);
+            const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
+            const bool bu = UNIFORM || !any_mixed;
             auto scan = [&](auto bu_tag, const DensitySelf &x, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
                 constexpr bool BU = decltype(bu_tag)::value;
                 const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
@@ -925,7 +926,8 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                 }
             }
             // one particle class in the halo: every listed neighbor has the particle's material
-            const bool bu = UNIFORM || !__syncthreads_or(mixed ? 1 : 0);
+            const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
+            const bool bu = UNIFORM || !any_mixed;
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
