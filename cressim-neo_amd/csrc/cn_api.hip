@@ -39,7 +39,8 @@ struct cn_ctx {
     uint32_t grid_max = 1024;        // workgroups of a persistent fluid launch
     // scene pool
     uint32_t nscenes = 0;
-    uint32_t scene_bits = 0;         // key bits above the Morton code holding the scene index
+    uint32_t scene_bits = 0;         // key bits above the Morton code holding the scene rank
+    uint32_t fluid_scenes = 0;       // scenes with fluid particles (ranks [0, fluid_scenes))
     std::vector<uint32_t> scene_first;  // first particle of each scene, then the total
     std::vector<float> scene_cell;      // derived grid cell size of each scene
     bool graph_valid = false;         // the cached graphs refer to the current buffers
@@ -377,6 +378,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.grid_max = ctx->grid_max;
     c.superset = (p->flags & CN_STEP_NO_CANDIDATE_REUSE) ? 0u : 1u;
     c.fluid_iters = p->fluid_iterations > 0 ? p->fluid_iterations : std::max<uint32_t>(p->default_iterations, 1u);
+    c.fluid_scenes = ctx->fluid_scenes;
     fluid_tiles(c);
     c.cfl_guard = 1.001f;
     std::memcpy(c.cfg, p->solver_config, sizeof(c.cfg));
@@ -590,7 +592,7 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.curpos, N));
     CN_HIP(ctx, dalloc(&s.cs, N));
     CN_HIP(ctx, dalloc(&s.blk_first_run, N + 1));
-    CN_HIP(ctx, dalloc(&s.nblk, 1));
+    CN_HIP(ctx, dalloc(&s.nblk, 2));
     // halo descriptors for up to one cell block per 32 particles; sparser scenes walk the rest of
     // their blocks through the run table
     s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
@@ -837,7 +839,17 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     const uint32_t room = (32u - ctx->scene_bits) / 3u;
     ctx->morton_bits = std::min<uint32_t>(std::min<uint32_t>(10u, room),
                                           std::max<uint32_t>(1u, ceil_log2((uint64_t)std::ceil(max_span))));
-    for (uint32_t k = 0; k < count; ++k) sinfo[k].key_prefix = ctx->scene_bits ? k << (3u * ctx->morton_bits) : 0u;
+    // key prefixes: scenes with fluid first (in upload order), so the fluid passes' blocks are a
+    // prefix of the block list
+    ctx->fluid_scenes = 0;
+    for (uint32_t k = 0; k < count; ++k) ctx->fluid_scenes += sinfo[k].has_fluid;
+    {
+        uint32_t rf = 0, rs = ctx->fluid_scenes;
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t rank = sinfo[k].has_fluid ? rf++ : rs++;
+            sinfo[k].key_prefix = ctx->scene_bits ? rank << (3u * ctx->morton_bits) : 0u;
+        }
+    }
     ctx->nscenes = count;
 
     dfree(s.classes);

@@ -91,6 +91,7 @@ struct SubstepConsts {
     uint32_t grid_max;       // workgroups of a persistent fluid launch (upper bound)
     uint32_t superset;       // 1: LDS blocks of fluid-only scenes reuse iteration 0's candidate lists
     uint32_t fluid_iters;    // fluid iterations per substep
+    uint32_t fluid_scenes;   // scenes with fluid particles: they hold key prefixes [0, fluid_scenes)
     float cfl_guard;         // relative margin of the CFL-term skip test
     float cfg[CN_SOLVER_CONFIG_FLOATS];
 };

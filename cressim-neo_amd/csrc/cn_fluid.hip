@@ -548,7 +548,7 @@ __device__ __forceinline__ void flush_disp(float d2, const FluidArgs &a)
 __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts c)
 {
     __shared__ uint32_t scan[kBlock];
-    const uint32_t nb = min(*a.nblk, c.blk_cap);
+    const uint32_t nb = min(a.nblk[1], c.blk_cap);
     const uint32_t tid = threadIdx.x;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t p0 = a.run_start[a.blk_first_run[b]], p1 = a.run_start[a.blk_first_run[b + 1]];
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
     const float dprev = c.superset ? fsqrt(__uint_as_float(a.iter_state[kStatePrevDisp])) : -1.0f;
     if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
-    const uint32_t nblk = *a.nblk;
+    const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
     BlockQueue bq(blockIdx.x);
     MetaAcc ma;
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
     uint32_t *lcls = reinterpret_cast<uint32_t *>(lnc + (tension ? a.tile : 0u));
     uint32_t *lslot = lcls + (UNIFORM ? 0u : a.tile);
     const uint32_t tid = threadIdx.x;
-    const uint32_t nblk = *a.nblk;
+    const uint32_t nblk = a.nblk[1];
     uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds;
     uint32_t *wqueue = a.iter_state + kStateWideQueue + a.it;
     const uint32_t nwide = WIDE ? a.iter_state[kStateWideCount] : 0u;

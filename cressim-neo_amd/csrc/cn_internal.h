@@ -87,7 +87,7 @@ struct SceneBuffers {
     float4 *cs = nullptr;           // (x* + delta, density constraint): neighbor records of the delta pass
     float4 *nc = nullptr;           // surface normal * tension (only with surface tension)
     uint32_t *blk_first_run = nullptr;  // first run of each Morton 4x4x4 block, then the run count
-    uint32_t *nblk = nullptr;
+    uint32_t *nblk = nullptr;       // [0] Morton blocks, [1] blocks of scenes with fluid (a prefix)
     uint32_t *blk_halo = nullptr;   // kHaloStride words per block, for the first blk_cap blocks
     uint32_t blk_cap = 0;
     uint32_t *wide_blk = nullptr;   // blocks whose halo fits the wide delta tile only (blk_cap)

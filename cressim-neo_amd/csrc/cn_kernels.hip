@@ -330,9 +330,10 @@ __device__ __forceinline__ bool block_flag(const uint32_t *run_key, uint32_t r, 
 
 __global__ __launch_bounds__(kBlock) void k_blk_count(const uint32_t *__restrict__ run_key,
                                                       const uint32_t *__restrict__ run_count,
-                                                      uint32_t *__restrict__ block_cnt)
+                                                      uint32_t *__restrict__ block_cnt, uint32_t *__restrict__ nfluid)
 {
     __shared__ uint32_t sh[kBlock];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nfluid = 0u;
     const uint32_t R = *run_count;
     const uint32_t base = blockIdx.x * kRunTile + threadIdx.x * kRunItems;
     uint32_t cnt = 0;
@@ -348,7 +349,8 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ block_off,
                                                      uint32_t *__restrict__ blk_first_run,
                                                      uint32_t *__restrict__ nblk, uint32_t *__restrict__ work,
-                                                     uint32_t nwork, uint32_t last_word, uint32_t prev_word)
+                                                     uint32_t nwork, uint32_t last_word, uint32_t prev_word,
+                                                     uint32_t prefix_shift, uint32_t fluid_scenes)
 {
     __shared__ uint32_t sh[kBlock];
     const uint32_t R = *run_count;
@@ -366,7 +368,11 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
     for (int k = 0; k < kRunItems; ++k) {
         const uint32_t r = base + k;
         if (r >= R) break;
-        if (f[k]) blk_first_run[idx++] = r;
+        if (f[k]) {
+            // scenes with fluid hold the lowest key prefixes, so their blocks come first
+            if ((uint32_t)((uint64_t)run_key[r] >> prefix_shift) < fluid_scenes) atomicMax(nblk + 1, idx + 1);
+            blk_first_run[idx++] = r;
+        }
         if (r == R - 1) {
             blk_first_run[idx] = R;
             *nblk = idx;
@@ -381,11 +387,12 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
 void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 {
     const uint32_t nb = (c.n + kRunTile - 1) / kRunTile;
-    hipLaunchKernelGGL(k_blk_count, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs);
+    hipLaunchKernelGGL(k_blk_count, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs, s.nblk + 1);
     hipLaunchKernelGGL(k_scan_block_runs, dim3(1), dim3(1024), 0, st, s.block_runs, nb, s.nblk);
     const uint32_t last = kStateMaxDisp + std::min<uint32_t>(std::max<uint32_t>(c.fluid_iters, 1u), kIterStateWords) - 1;
     hipLaunchKernelGGL(k_blk_emit, dim3(nb), dim3(kBlock), 0, st, s.run_key, s.run_count, s.block_runs,
-                       s.blk_first_run, s.nblk, s.iter_state, kStateMaxDisp, last, kStatePrevDisp);
+                       s.blk_first_run, s.nblk, s.iter_state, kStateMaxDisp, last, kStatePrevDisp, 3u * c.morton_bits,
+                       c.fluid_scenes);
 }
 
 // ---------------------------------------------------------------------------------------------

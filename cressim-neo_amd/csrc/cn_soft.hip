@@ -258,12 +258,21 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     float4 p = a.pos[t];
     if (p.w <= kEps) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-    for (uint32_t k = k0; k < k1; ++k) {
-        const uint32_t r = ref[k];
-        const float4 c = rec[(size_t)(r >> 2) * per + (r & 3u)];
-        tx = tx + c.x;
-        ty = ty + c.y;
-        tz = tz + c.z;
+    // four records per step: the reference loads, then the record loads, then the sums in order
+    for (uint32_t k = k0; k < k1; k += 4) {
+        uint32_t r[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) r[u] = ref[min(k + u, k1 - 1)];
+        float4 cr[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) cr[u] = rec[(size_t)(r[u] >> 2) * per + (r[u] & 3u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (k + u < k1) {
+                tx = tx + cr[u].x;
+                ty = ty + cr[u].y;
+                tz = tz + cr[u].z;
+            }
     }
     p.x = p.x + tx;
     p.y = p.y + ty;
