@@ -34,7 +34,8 @@ struct cn_ctx {
     float max_h = 0.0f;              // largest fluid smoothing radius of the scene
     bool has_fluid = false;          // any fluid particle (hasFluidWork, physics_solver.cpp:465)
     bool has_tension = false;        // any fluid material with surface tension
-    bool has_solid = false;          // any solid particle: particle-particle contacts can exist
+    bool has_solid = false;          // any solid particle
+    bool any_pairs = false;          // some scene can hold a particle-particle contact pair
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
     uint32_t grid_max = 1024;        // workgroups of a persistent fluid launch
     // scene pool
@@ -715,6 +716,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->scene_cell.assign(count, 0.0f);
     ctx->has_fluid = false;
     ctx->has_solid = false;
+    ctx->any_pairs = false;
     uint32_t base = 0;
     double max_span = 1.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -781,6 +783,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
                                   : 0u;
         sinfo[k].has_fluid = scene_fluid ? 1u : 0u;
         sinfo[k].pairs = (scene_solid && scene_fluid) || two_groups ? 1u : 0u;
+        ctx->any_pairs |= sinfo[k].pairs != 0u;
         col.insert(col.end(), sc.begin(), sc.end());
         // constraints, shifted to pool indices
         if (cv) {
@@ -1045,7 +1048,9 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
     const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
     const bool has_soft = s.ne + s.nb + s.nd + s.nt > 0;
     // particle-particle contacts need a solid particle (fluid pairs are never candidates)
-    const bool has_contacts = contact_iters > 0 && s.n > 1 && ctx->has_solid && s.ct_alloc >= s.n;
+    // (a pool where no scene can hold a pair skips them: their zero corrections change nothing but
+    // the sign of zero coordinates)
+    const bool has_contacts = contact_iters > 0 && s.n > 1 && ctx->any_pairs && s.ct_alloc >= s.n;
     SoftArgs sa{};
     if (has_soft) {
         sa.slot_of = s.slot_of;

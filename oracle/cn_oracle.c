@@ -1829,14 +1829,25 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     const int has_soft = n > 0 && (s->n_edges || s->n_bends || s->n_dihedrals || s->n_tets);
     /* particle-particle contacts: any pair needs a solid particle (fluid-fluid pairs are never
      * candidates); self-collision of a phase group needs adjacency lists and is not supported */
-    uint32_t solid_count = 0;
+    /* a pair can exist only between a solid and a fluid particle or solids of two phase groups;
+     * without one the contact passes would only add zero corrections (turning -0.0 coordinates
+     * into +0.0), so they are skipped */
+    uint32_t solid_count = 0, fluid_count_c = 0, group0 = 0xffffffffu;
+    int two_groups = 0;
     for (uint32_t i = 0; i < n; ++i)
         if (s->kinds[i] == CN_PARTICLE_KIND_SOLID) {
             ++solid_count;
             if (s->meta[4 * (size_t)i + 1] & PHASE_SELF_COLLIDE)
                 return -2;
+            const uint32_t g = s->meta[4 * (size_t)i + 1] & PHASE_GROUP_MASK;
+            if (group0 == 0xffffffffu)
+                group0 = g;
+            two_groups |= g != group0;
+        } else {
+            ++fluid_count_c;
         }
-    const int has_contacts = contact_iters > 0 && n > 1 && solid_count > 0;
+    const int has_contacts =
+        contact_iters > 0 && n > 1 && solid_count > 0 && (fluid_count_c > 0 || two_groups);
     const uint32_t pcap = CNO_CONTACT_PARTNERS;
     uint32_t *pcount = NULL, *plist = NULL;
     contact_rec *recs = NULL;
