@@ -920,6 +920,9 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, hipMemcpy(s.out_vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.out_prev, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     CN_HIP(ctx, hipMemcpy(s.prev[0], pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    // a fresh scene starts with fresh iteration state (no displacement carried into the first
+    // substep's candidate lists)
+    CN_HIP(ctx, hipMemset(s.iter_state, 0, kStateWords * sizeof(uint32_t)));
 
     // particle-particle contact state: candidate lists and the last contacts, kContactPartners
     // per particle (only a scene with a solid particle has contacts)
