@@ -196,9 +196,12 @@ void free_constraints(SceneBuffers &s)
 
 // Particle -> incident constraints, ascending constraint order (physics_scene_gpu_state.cpp:3273-3360);
 // appends start[n + 1] then ref[] to out and returns the two offsets.
+// map (per slots per constraint) receives the CSR position of each constraint vertex; slots without
+// one point at the spare record past the last.
 template <class Fn>
 void build_incidence(uint32_t n, uint32_t groups, const uint32_t *counts, const uint32_t *arity, const uint32_t *offset,
-                     Fn parts, std::vector<uint32_t> &out, size_t &start_off, size_t &ref_off)
+                     Fn parts, std::vector<uint32_t> &out, size_t &start_off, size_t &ref_off,
+                     std::vector<uint32_t> &map, uint32_t per, size_t &refs)
 {
     std::vector<uint32_t> cnt(n + 1, 0);
     for (uint32_t g = 0; g < groups; ++g)
@@ -218,11 +221,18 @@ void build_incidence(uint32_t n, uint32_t groups, const uint32_t *counts, const 
     out.resize(out.size() + acc);
     std::vector<uint32_t> cur(n);
     for (uint32_t i = 0; i < n; ++i) cur[i] = out[start_off + i];
+    size_t ncons = 0;
+    for (uint32_t g = 0; g < groups; ++g) ncons += counts[g];
+    map.assign(ncons * per, acc);
+    refs = acc;
     for (uint32_t g = 0; g < groups; ++g)
         for (uint32_t c = 0; c < counts[g]; ++c)
             for (uint32_t k = 0; k < arity[g]; ++k) {
                 const uint32_t p = parts(g, c, k);
-                if (p < n) out[ref_off + cur[p]++] = ((c + offset[g]) << 2) | k;
+                if (p < n) {
+                    map[(size_t)(c + offset[g]) * per + k] = cur[p];
+                    out[ref_off + cur[p]++] = ((c + offset[g]) << 2) | k;
+                }
             }
 }
 
@@ -961,15 +971,13 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             CN_HIP(ctx, hipMemcpy(s.dihedrals, dih.data(), s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
         if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, tets.data(), s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
         const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
-        const size_t ncorr = 2ull * s.ne + 4ull * (s.nb + s.nd) + 4ull * s.nt;
         CN_HIP(ctx, dalloc(&s.lam, nlam));
-        CN_HIP(ctx, dalloc(&s.corr, ncorr));
-        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
-        std::vector<uint32_t> inc;
+        std::vector<uint32_t> inc, map_e, map_b, map_t;
+        size_t refs[3] = {0, 0, 0};
         {
             const uint32_t cnt[1] = {s.ne}, ar[1] = {2}, off[1] = {0};
             build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return k == 0 ? edges[c].a : edges[c].b; },
-                            inc, s.inc_off[0], s.inc_off[1]);
+                            inc, s.inc_off[0], s.inc_off[1], map_e, 2u, refs[0]);
         }
         {
             const uint32_t cnt[2] = {s.nb, s.nd}, ar[2] = {3, 4}, off[2] = {0, s.nb};
@@ -982,13 +990,26 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
                                 const cn_cloth_dihedral &D = dih[c];
                                 return k == 0 ? D.e0 : (k == 1 ? D.e1 : (k == 2 ? D.o0 : D.o1));
                             },
-                            inc, s.inc_off[2], s.inc_off[3]);
+                            inc, s.inc_off[2], s.inc_off[3], map_b, 4u, refs[1]);
         }
         {
             const uint32_t cnt[1] = {s.nt}, ar[1] = {4}, off[1] = {0};
             build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return tets[c].p[k]; }, inc,
-                            s.inc_off[4], s.inc_off[5]);
+                            s.inc_off[4], s.inc_off[5], map_t, 4u, refs[2]);
         }
+        // record maps after the incidence structures; records of each group in CSR order + a spare
+        s.inc_off[6] = inc.size();
+        inc.insert(inc.end(), map_e.begin(), map_e.end());
+        s.inc_off[7] = inc.size();
+        inc.insert(inc.end(), map_b.begin(), map_b.end());
+        s.inc_off[8] = inc.size();
+        inc.insert(inc.end(), map_t.begin(), map_t.end());
+        s.corr_off[0] = 0;
+        s.corr_off[1] = refs[0] + 1;
+        s.corr_off[2] = s.corr_off[1] + refs[1] + 1;
+        const size_t ncorr = s.corr_off[2] + refs[2] + 1;
+        CN_HIP(ctx, dalloc(&s.corr, ncorr));
+        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
         CN_HIP(ctx, dalloc(&s.inc, inc.size()));
         CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         dfree(s.slot_of);
@@ -1071,15 +1092,15 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         sa.lam_e = s.lam;
         sa.lam_b = s.lam + s.ne;
         sa.lam_t = s.lam + s.ne + s.nb + s.nd;
-        sa.corr_e = s.corr;
-        sa.corr_b = s.corr + 2ull * s.ne;
-        sa.corr_t = s.corr + 2ull * s.ne + 4ull * (s.nb + s.nd);
+        sa.corr_e = s.corr + s.corr_off[0];
+        sa.corr_b = s.corr + s.corr_off[1];
+        sa.corr_t = s.corr + s.corr_off[2];
+        sa.map_e = s.inc + s.inc_off[6];
+        sa.map_b = s.inc + s.inc_off[7];
+        sa.map_t = s.inc + s.inc_off[8];
         sa.inc_e_start = s.inc + s.inc_off[0];
-        sa.inc_e_ref = s.inc + s.inc_off[1];
         sa.inc_b_start = s.inc + s.inc_off[2];
-        sa.inc_b_ref = s.inc + s.inc_off[3];
         sa.inc_t_start = s.inc + s.inc_off[4];
-        sa.inc_t_ref = s.inc + s.inc_off[5];
         sa.active = s.scene_active;
         sa.pid_scene = s.pid_scene;
     }

@@ -135,9 +135,11 @@ struct SceneBuffers {
     cn_soft_tet *tets = nullptr;
     uint32_t ne = 0, nb = 0, nd = 0, nt = 0;
     float *lam = nullptr;           // edges | bends + dihedrals | tets
-    float4 *corr = nullptr;         // edges (2 each) | bends + dihedrals (4 each) | tets (4 each)
-    uint32_t *inc = nullptr;        // three CSR incidence structures back to back
-    size_t inc_off[6] = {0, 0, 0, 0, 0, 0};
+    float4 *corr = nullptr;         // correction records of edges | bends + dihedrals | tets, each group
+                                    // in its incidence (CSR) order, then one spare record
+    size_t corr_off[3] = {0, 0, 0};
+    uint32_t *inc = nullptr;        // three CSR incidence structures back to back, then the record maps
+    size_t inc_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // ---- broadphase + fluid launches (cn_kernels.hip) ----
@@ -169,20 +171,23 @@ struct SoftArgs {
     const cn_soft_edge *edges;
     uint32_t ne;
     float *lam_e;
-    float4 *corr_e;            // 2 per edge
+    float4 *corr_e;            // correction records in incidence order
+    const uint32_t *map_e;     // 2 per edge: the position of its vertex records in corr_e
     const cn_soft_bend *bends;
     uint32_t nb;
     const cn_cloth_dihedral *dih;
     uint32_t nd;
     float *lam_b;              // bends then dihedrals
-    float4 *corr_b;            // 4 per bend / dihedral
+    float4 *corr_b;
+    const uint32_t *map_b;     // 4 per bend / dihedral (a bend's fourth is the spare record)
     const cn_soft_tet *tets;
     uint32_t nt;
     float *lam_t;
-    float4 *corr_t;            // 4 per tet
-    const uint32_t *inc_e_start, *inc_e_ref;  // incidence: ref = (constraint << 2) | slot
-    const uint32_t *inc_b_start, *inc_b_ref;
-    const uint32_t *inc_t_start, *inc_t_ref;
+    float4 *corr_t;
+    const uint32_t *map_t;     // 4 per tet
+    const uint32_t *inc_e_start;  // incidence ranges: a particle's records are corr_x[start[p], start[p + 1])
+    const uint32_t *inc_b_start;
+    const uint32_t *inc_t_start;
     const uint32_t *active;    // per scene
     const uint32_t *pid_scene; // particle index -> scene
 };

@@ -18,6 +18,13 @@ namespace cn {
 
 static inline uint32_t grid_n(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
+// A constraint's correction records, written where its vertices' incidence lists read them.
+struct RecOut {
+    float4 *rec;
+    const uint32_t *map;
+    __device__ float4 &operator[](uint32_t k) const { return rec[map[k]]; }
+};
+
 __device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.pos[a.slot_of[particle]]; }
 
 // Constraints and particles of scenes that this step leaves untouched are skipped (records kept).
@@ -45,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     if (e >= a.ne) return;
     const cn_soft_edge E = a.edges[e];
     if (!live(a, E.a)) return;
-    float4 *out = a.corr_e + 2 * (size_t)e;
+    const RecOut out{a.corr_e, a.map_e + 2 * (size_t)e};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (E.enabled == 0u) {
         out[0] = z4;
@@ -91,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
     if (b >= a.nb) return;
     const cn_soft_bend B = a.bends[b];
     if (!live(a, B.p0)) return;
-    float4 *out = a.corr_b + 4 * (size_t)b;
+    const RecOut out{a.corr_b, a.map_b + 4 * (size_t)b};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 p0 = ldp(a, B.p0), p1 = ldp(a, B.p1), p2 = ldp(a, B.p2);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w;
@@ -149,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const uint32_t ci = a.nb + d;
     const cn_cloth_dihedral D = a.dih[d];
     if (!live(a, D.e0)) return;
-    float4 *out = a.corr_b + 4 * (size_t)ci;
+    const RecOut out{a.corr_b, a.map_b + 4 * (size_t)ci};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 e0m = ldp(a, D.e0), e1m = ldp(a, D.e1), o0m = ldp(a, D.o0), o1m = ldp(a, D.o1);
     // p0, p1 opposite vertices; p2, p3 edge vertices
@@ -236,19 +243,17 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     const float dl = fdiv(-(C + alpha * lambda), den);
     a.lam_t[t] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
-    float4 *out = a.corr_t + 4 * (size_t)t;
+    const RecOut out{a.corr_t, a.map_t + 4 * (size_t)t};
     out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
     out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
     out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
     out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
 }
 
-// Per particle, in particle-index order (the incidence lists and the constraint records of
-// neighboring particles are neighbors in memory): its incident records in list order, summed,
-// then added to its position in slot order.
+// Per particle, in particle-index order: its incident records, stored contiguously in incidence
+// order, summed in that order, then added to its position in slot order.
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
-                                                              const uint32_t *__restrict__ ref,
-                                                              const float4 *__restrict__ rec, uint32_t per)
+                                                              const float4 *__restrict__ rec)
 {
     const uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
     if (pid >= a.n) return;
@@ -258,14 +263,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     float4 p = a.pos[t];
     if (p.w <= kEps) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-    // four records per step: the reference loads, then the record loads, then the sums in order
     for (uint32_t k = k0; k < k1; k += 4) {
-        uint32_t r[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) r[u] = ref[min(k + u, k1 - 1)];
         float4 cr[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) cr[u] = rec[(size_t)(r[u] >> 2) * per + (r[u] & 3u)];
+        for (uint32_t u = 0; u < 4; ++u) cr[u] = rec[min(k + u, k1 - 1)];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u)
             if (k + u < k1) {
@@ -290,15 +291,15 @@ void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
 {
     // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
     if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start, a.inc_e_ref,
-                       (const float4 *)a.corr_e, 2u);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
+                       (const float4 *)a.corr_e);
     if (a.nb) hipLaunchKernelGGL(k_solve_bends, dim3(grid_n(a.nb)), dim3(kBlock), 0, st, a);
     if (a.nd) hipLaunchKernelGGL(k_solve_dihedrals, dim3(grid_n(a.nd)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start, a.inc_b_ref,
-                       (const float4 *)a.corr_b, 4u);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start,
+                       (const float4 *)a.corr_b);
     if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start, a.inc_t_ref,
-                       (const float4 *)a.corr_t, 4u);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
+                       (const float4 *)a.corr_t);
 }
 
 }  // namespace cn
