@@ -156,6 +156,8 @@ void free_scene(SceneBuffers &s)
     dfree(s.scenes);
     dfree(s.meta);
     dfree(s.slot_of);
+    dfree(s.posi);
+    dfree(s.soft_ids);
     dfree(s.edges);
     dfree(s.bends);
     dfree(s.dihedrals);
@@ -184,6 +186,11 @@ void free_constraints(SceneBuffers &s)
     dfree(s.lam);
     dfree(s.corr);
     dfree(s.inc);
+    dfree(s.posi);
+    dfree(s.soft_ids);
+    s.posi = nullptr;
+    s.soft_ids = nullptr;
+    s.nsoft = 0;
     s.edges = nullptr;
     s.bends = nullptr;
     s.dihedrals = nullptr;
@@ -1010,6 +1017,20 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         const size_t ncorr = s.corr_off[2] + refs[2] + 1;
         CN_HIP(ctx, dalloc(&s.corr, ncorr));
         CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
+        // constrained particles: any incidence in the three structures
+        {
+            std::vector<uint32_t> soft;
+            for (uint32_t i = 0; i < n; ++i) {
+                bool any = false;
+                for (int g = 0; g < 3; ++g) any |= inc[s.inc_off[2 * g] + i + 1] != inc[s.inc_off[2 * g] + i];
+                if (any) soft.push_back(i);
+            }
+            s.nsoft = (uint32_t)soft.size();
+            CN_HIP(ctx, dalloc(&s.soft_ids, soft.size()));
+            if (!soft.empty())
+                CN_HIP(ctx, hipMemcpy(s.soft_ids, soft.data(), soft.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            CN_HIP(ctx, dalloc(&s.posi, n));
+        }
         CN_HIP(ctx, dalloc(&s.inc, inc.size()));
         CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         dfree(s.slot_of);
@@ -1092,6 +1113,9 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         sa.lam_e = s.lam;
         sa.lam_b = s.lam + s.ne;
         sa.lam_t = s.lam + s.ne + s.nb + s.nd;
+        sa.posi = s.posi;
+        sa.soft_ids = s.soft_ids;
+        sa.nsoft = s.nsoft;
         sa.corr_e = s.corr + s.corr_off[0];
         sa.corr_b = s.corr + s.corr_off[1];
         sa.corr_t = s.corr + s.corr_off[2];

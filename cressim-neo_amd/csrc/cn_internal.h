@@ -129,6 +129,10 @@ struct SceneBuffers {
     uint32_t *meta = nullptr;
     // soft constraints
     uint32_t *slot_of = nullptr;
+    float4 *posi = nullptr;         // positions of the constrained particles in particle-index order
+                                    // during the soft phase of an iteration
+    uint32_t *soft_ids = nullptr;   // the constrained particles, ascending
+    uint32_t nsoft = 0;
     cn_soft_edge *edges = nullptr;
     cn_soft_bend *bends = nullptr;
     cn_cloth_dihedral *dihedrals = nullptr;
@@ -164,6 +168,9 @@ void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipS
 // ---- XPBD soft constraints (cn_soft.hip) ----
 struct SoftArgs {
     float4 *pos;               // slot order
+    float4 *posi;              // particle-index order (constrained particles) while the soft passes run
+    const uint32_t *soft_ids;  // constrained particles
+    uint32_t nsoft;
     const uint32_t *slot_of;   // particle index -> slot
     const uint32_t *id;        // slot -> particle index
     uint32_t n;

@@ -25,7 +25,7 @@ struct RecOut {
     __device__ float4 &operator[](uint32_t k) const { return rec[map[k]]; }
 };
 
-__device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.pos[a.slot_of[particle]]; }
+__device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.posi[particle]; }
 
 // Constraints and particles of scenes that this step leaves untouched are skipped (records kept).
 __device__ __forceinline__ bool live(const SoftArgs &a, uint32_t particle) { return a.active[a.pid_scene[particle]] != 0u; }
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
 }
 
 // Per particle, in particle-index order: its incident records, stored contiguously in incidence
-// order, summed in that order, then added to its position in slot order.
+// order, summed in that order, then added to its position.
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
                                                               const float4 *__restrict__ rec)
 {
@@ -259,8 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     if (pid >= a.n) return;
     const uint32_t k0 = start[pid], k1 = start[pid + 1];
     if (k0 == k1 || !live(a, pid)) return;
-    const uint32_t t = a.slot_of[pid];
-    float4 p = a.pos[t];
+    float4 p = a.posi[pid];
     if (p.w <= kEps) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
     for (uint32_t k = k0; k < k1; k += 4) {
@@ -278,7 +277,26 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     p.x = p.x + tx;
     p.y = p.y + ty;
     p.z = p.z + tz;
-    a.pos[t] = p;
+    a.posi[pid] = p;
+}
+
+// The soft phase works on the constrained particles in particle-index order, where the
+// constraints' vertices are neighbors in memory: gathered from slot order before it, put back after.
+__global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= a.nsoft) return;
+    const uint32_t pid = a.soft_ids[j];
+    a.posi[pid] = a.pos[a.slot_of[pid]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_soft_scatter(SoftArgs a)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= a.nsoft) return;
+    const uint32_t pid = a.soft_ids[j];
+    if (!live(a, pid)) return;
+    a.pos[a.slot_of[pid]] = a.posi[pid];
 }
 
 void launch_soft_clear(const SoftArgs &a, hipStream_t st)
@@ -290,6 +308,7 @@ void launch_soft_clear(const SoftArgs &a, hipStream_t st)
 void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
 {
     // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
+    if (a.nsoft) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
     if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
                        (const float4 *)a.corr_e);
@@ -300,6 +319,7 @@ void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
     if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
                        (const float4 *)a.corr_t);
+    if (a.nsoft) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
 }
 
 }  // namespace cn
