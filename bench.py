@@ -62,7 +62,8 @@ def cpu_baseline(n_sample: int, seed: int = 2, min_seconds: float = 10.0, max_st
 def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool = True):
     """The batched workload: `count` independent scenes (sizes log-uniform in [2k, 64k], 40 % PBF,
     30 % tet soft bodies, 30 % cloth, seed 3) stepped together as one pool with a graph-captured
-    step, against looping the plain path (one context per scene, each stepped in turn)."""
+    step, against looping the plain path (one context per scene, each stepped in turn; the contexts'
+    streams overlap on the device), and against the same loop waiting for each step."""
     import numpy as np
     import cnhip
     from cnhip import scenes
@@ -110,6 +111,13 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     for s in ctxs:
         s.synchronize()
     el_loop = time.perf_counter() - t0
+    # the same loop with each scene's step waited for before the next scene's (a blocking step call)
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for s in ctxs:
+            s.step(p)
+            s.synchronize()
+    el_sync = time.perf_counter() - t0
     for s in ctxs:
         s.close()
     lat_ms = np.array(lat) * 1e3
@@ -121,7 +129,9 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "particle_substeps_per_s": total * SUBSTEPS * steps / el,
             "loop_plain_env_steps_per_s": env_loop, "loop_plain_steps": loop_steps,
-            "speedup_vs_loop_plain": env / env_loop, "graph": True, "neighbor_truncations": trunc,
+            "speedup_vs_loop_plain": env / env_loop,
+            "loop_plain_sync_env_steps_per_s": count * loop_steps / el_sync,
+            "speedup_vs_loop_plain_sync": env / (count * loop_steps / el_sync), "graph": True, "neighbor_truncations": trunc,
             "autotune": tuned}
 
 

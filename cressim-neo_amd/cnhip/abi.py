@@ -26,7 +26,7 @@ CN_PARTICLE_KIND_FLUID = 1
 CN_KEY_MORTON = 0
 CN_KEY_HASH = 1
 CN_SOLVER_CONFIG_FLOATS = 40
-CN_STEP_NO_CANDIDATE_REUSE = 1
+CN_STEP_CANDIDATE_REUSE = 1
 CN_KERNEL_SLOTS = 16
 
 

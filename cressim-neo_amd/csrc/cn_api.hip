@@ -394,7 +394,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.tile_max = ctx->tile_max;
     c.blk_cap = ctx->s.blk_cap;
     c.grid_max = ctx->grid_max;
-    c.superset = (p->flags & CN_STEP_NO_CANDIDATE_REUSE) ? 0u : 1u;
+    c.superset = (p->flags & CN_STEP_CANDIDATE_REUSE) && ctx->s.slist ? 1u : 0u;
     c.fluid_iters = p->fluid_iterations > 0 ? p->fluid_iterations : std::max<uint32_t>(p->default_iterations, 1u);
     c.fluid_scenes = ctx->fluid_scenes;
     fluid_tiles(c);
@@ -618,8 +618,6 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
     CN_HIP(ctx, dalloc(&s.list, list_slots(N) * (cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
-    CN_HIP(ctx, dalloc(&s.slist, list_slots(N) * kSuperSlots));
-    CN_HIP(ctx, dalloc(&s.scnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
     CN_HIP(ctx, hipMemsetAsync(s.iter_state, 0, kStateWords * sizeof(uint32_t), ctx->stream));
     CN_HIP(ctx, dalloc(&s.out_pos, N));
@@ -1226,6 +1224,12 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
     if (p->max_fluid_neighbors > ctx->cap_alloc)
         return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
     CN_HIP(ctx, hipSetDevice(ctx->device));
+    if ((p->flags & CN_STEP_CANDIDATE_REUSE) && !ctx->s.slist) {  // candidate lists on first use
+        CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        CN_HIP(ctx, dalloc(&ctx->s.slist, list_slots(ctx->capacity) * kSuperSlots));
+        CN_HIP(ctx, dalloc(&ctx->s.scnt, std::max<uint32_t>(ctx->capacity, 1u)));
+        ctx->graph_valid = false;
+    }
     if (!ctx->graphs_enabled || ctx->profiling) {
         record_step(ctx, p);
         ctx->stepped = true;

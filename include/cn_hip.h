@@ -177,9 +177,11 @@ typedef struct cn_step_params {
     uint32_t flags;                  /* CN_STEP_* bits, 0 by default */
 } cn_step_params;
 
-/* flags: walk the 27 cells in every fluid iteration instead of filtering the iteration-0
- * candidate lists (the results are the same either way; for measurement and tests). */
-#define CN_STEP_NO_CANDIDATE_REUSE 1u
+/* flags: fluid iterations after the first filter iteration 0's candidate lists (neighbors within
+ * h + skin) instead of walking the 27 cells, while every fluid particle has moved less than half the
+ * skin (the results are the same either way; it pays when the walk dominates and displacements are
+ * small).  The lists take 194 bytes per particle, allocated on the first step that asks. */
+#define CN_STEP_CANDIDATE_REUSE 1u
 
 /* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
  * physics_solver.cpp:1403-1492), plus the fluid-neighbor truncation the reference lacks. */

@@ -194,7 +194,7 @@ def test_candidate_reuse_equals_full_walk(solver, case):
     sc = scenes.dam_break_64k() if case == "dam_break_64k" else scenes.large_pbf()
     steps = 6 if case == "dam_break_64k" else 3
     out, reuses = {}, {}
-    for flags in (0, cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE):
+    for flags in (cnhip.abi.CN_STEP_CANDIDATE_REUSE, 0):
         print(f"{case}: flags {flags}", flush=True)
         p = params(cap=64, flags=flags)
         solver.reserve(sc.n, 64)
@@ -205,6 +205,6 @@ def test_candidate_reuse_equals_full_walk(solver, case):
             reuses[flags] += solver.meta().candidate_reuses
         out[flags] = solver.readback()
     print(reuses)
-    assert reuses[cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE] == 0 and reuses[0] > 0
-    for a, b in zip(out[0], out[cnhip.abi.CN_STEP_NO_CANDIDATE_REUSE]):
+    assert reuses[0] == 0 and reuses[cnhip.abi.CN_STEP_CANDIDATE_REUSE] > 0
+    for a, b in zip(out[0], out[cnhip.abi.CN_STEP_CANDIDATE_REUSE]):
         np.testing.assert_array_equal(a, b)
