@@ -59,11 +59,12 @@ def cpu_baseline(n_sample: int, seed: int = 2, min_seconds: float = 10.0, max_st
                       f"same density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
 
 
-def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool = True):
+def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool = True, shape=None):
     """The batched workload: `count` independent scenes (sizes log-uniform in [2k, 64k], 40 % PBF,
     30 % tet soft bodies, 30 % cloth, seed 3) stepped together as one pool with a graph-captured
     step, against looping the plain path (one context per scene, each stepped in turn; the contexts'
-    streams overlap on the device), and against the same loop waiting for each step."""
+    streams overlap on the device), and against the same loop waiting for each step.  loop_steps 0
+    times the pool alone."""
     import numpy as np
     import cnhip
     from cnhip import scenes
@@ -76,9 +77,13 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     pool = cnhip.Solver(0)
     pool.reserve(total, CAP)
     pool.upload_batch(batch)
-    tuned = pool.autotune(p, steps=1) if tune else None
+    if shape:
+        pool.tune_set(*shape)
+        tuned = {"tile_max": shape[0], "grid_max": shape[1], "set": True}
+    else:
+        tuned = pool.autotune(p, steps=1) if tune else None
     if tuned:
-        print(f"bench: batch autotune {tuned}", file=sys.stderr, flush=True)
+        print(f"bench: batch launch shape {tuned}", file=sys.stderr, flush=True)
     pool.graphs(True)
     for _ in range(warmup):
         pool.step(p)
@@ -93,7 +98,17 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     el = time.perf_counter() - t0
     trunc = pool.meta().fluid_neighbor_truncations
     pool.close()
-    print(f"bench: batched {steps} steps in {el:.3f} s; looping the plain path next", file=sys.stderr, flush=True)
+    print(f"bench: batched {steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    lat_ms = np.array(lat) * 1e3
+    env = count * steps / el
+    out = {"workload": f"{count} scenes, log-uniform 2k-64k particles, 40% PBF / 30% tet / 30% cloth, seed 3",
+           "scenes": count, "particles": total, "steps": steps, "warmup": warmup,
+           "env_steps_per_s": env, "ms_per_step": el * 1e3 / steps,
+           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+           "particle_substeps_per_s": total * SUBSTEPS * steps / el, "graph": True,
+           "neighbor_truncations": trunc, "autotune": tuned}
+    if loop_steps <= 0:
+        return out
     ctxs = []
     for sc in batch:
         s = cnhip.Solver(0)
@@ -120,19 +135,11 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     el_sync = time.perf_counter() - t0
     for s in ctxs:
         s.close()
-    lat_ms = np.array(lat) * 1e3
-    env = count * steps / el
     env_loop = count * loop_steps / el_loop
-    return {"workload": f"{count} scenes, log-uniform 2k-64k particles, 40% PBF / 30% tet / 30% cloth, seed 3",
-            "scenes": count, "particles": total, "steps": steps, "warmup": warmup,
-            "env_steps_per_s": env, "ms_per_step": el * 1e3 / steps,
-            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-            "particle_substeps_per_s": total * SUBSTEPS * steps / el,
-            "loop_plain_env_steps_per_s": env_loop, "loop_plain_steps": loop_steps,
-            "speedup_vs_loop_plain": env / env_loop,
-            "loop_plain_sync_env_steps_per_s": count * loop_steps / el_sync,
-            "speedup_vs_loop_plain_sync": env / (count * loop_steps / el_sync), "graph": True, "neighbor_truncations": trunc,
-            "autotune": tuned}
+    env_sync = count * loop_steps / el_sync
+    out.update(loop_plain_env_steps_per_s=env_loop, loop_plain_steps=loop_steps, speedup_vs_loop_plain=env / env_loop,
+               loop_plain_sync_env_steps_per_s=env_sync, speedup_vs_loop_plain_sync=env / env_sync)
+    return out
 
 
 def serve_bench(sessions: int, rates, duration: float, shards: int, loop_requests: int):
@@ -219,6 +226,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched workload (with --loop-steps 0: the pool alone, for profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no cn_autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,GRID",
                     help="set the fluid launch shape (cn_tune_set) instead of autotuning it")
@@ -236,6 +245,11 @@ def main():
         res = serve_bench(args.serve_sessions, (2000.0, 10000.0, 50000.0), args.serve_duration,
                           args.serve_shards, 400)
         print(json.dumps({"metric": "served env-steps/s and p50/p99 step latency", "serve": res}), flush=True)
+        return
+    if args.batch_only:
+        shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
+        res = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps, tune=not args.no_tune, shape=shape)
+        print(json.dumps({"metric": "batched env-steps/s and p50/p99 step latency", "batch": res}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
