@@ -783,7 +783,9 @@ __device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, c
 // ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
-template <bool UNIFORM>
+// SUP: the candidate lists are in use (a separate instantiation, so the default pass carries none of
+// their registers)
+template <bool UNIFORM, bool SUP>
 __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
     // from its iteration-0 position
     const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
-    const float dprev = c.superset ? fsqrt(__uint_as_float(a.iter_state[kStatePrevDisp])) : -1.0f;
+    const float dprev = SUP ? fsqrt(__uint_as_float(a.iter_state[kStatePrevDisp])) : -1.0f;
     if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
@@ -830,6 +832,7 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
             const bool bu = UNIFORM || !any_mixed;
             auto scan = [&](auto bu_tag, const DensitySelf &x, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
                 constexpr bool BU = decltype(bu_tag)::value;
+                if (!SUP) return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
                 const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
                 if (x.sup && a.it == 0) {
                     uint32_t sreq;
@@ -1098,10 +1101,18 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
         const uint32_t per = 16u + (c.uniform_class ? 0u : 4u);
         a.tile = c.tile_dens;
         const size_t lds = (size_t)a.tile * per;
-        if (c.uniform_class)
-            hipLaunchKernelGGL(k_fluid_density_tiled<true>, dim3(persistent_grid(c)), dim3(kTileBlock), lds, st, a, c);
-        else
-            hipLaunchKernelGGL(k_fluid_density_tiled<false>, dim3(persistent_grid(c)), dim3(kTileBlock), lds, st, a, c);
+        const dim3 grid(persistent_grid(c)), block(kTileBlock);
+        if (c.superset) {
+            if (c.uniform_class)
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, true>), grid, block, lds, st, a, c);
+            else
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, true>), grid, block, lds, st, a, c);
+        } else {
+            if (c.uniform_class)
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, false>), grid, block, lds, st, a, c);
+            else
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, false>), grid, block, lds, st, a, c);
+        }
     } else {
         if (c.uniform_class)
             hipLaunchKernelGGL(k_fluid_density_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
