@@ -686,18 +686,54 @@ __device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
 //
 // SUP: also compacts the candidates within the wider radius sqrt(ts2) into ls (at most kSuperSlots,
 // scount = their untruncated number); later iterations filter those instead of walking.
+//
+// Cells that no candidate inside can reach are left out of the walk (the order of the others is
+// unchanged): a candidate's position at the substep start lies in its cell and has moved by at most
+// the largest displacement since, so when the particle's distance to the cell's box exceeds the
+// search radius plus that displacement (in cell units, with slack for the rounding of the cell
+// assignment), none of the cell's particles can be within the radius.
+struct RowCull {
+    float ux, uy, uz;  // the particle's current position in cell units
+    int cx, cy, cz;    // its cell
+    float lim2;        // squared limit in cell units; +inf walks every cell
+};
+
+__device__ __forceinline__ RowCull row_cull(const float4 s, const int4 ci, float inv, float radius, float dmax)
+{
+    RowCull rc;
+    rc.ux = s.x * inv;
+    rc.uy = s.y * inv;
+    rc.uz = s.z * inv;
+    rc.cx = ci.x;
+    rc.cy = ci.y;
+    rc.cz = ci.z;
+    const float slack = 1.0e-4f + 1.0e-6f * fmaxd(fmaxd(fabsf(rc.ux), fabsf(rc.uy)), fabsf(rc.uz));
+    const float lim = (radius + dmax) * inv * 1.00001f + slack;
+    rc.lim2 = dmax < 3.0e38f && lim < 1.0e18f ? lim * lim : __builtin_huge_valf();
+    return rc;
+}
+
+__device__ __forceinline__ float gap_below(float u, int c) { return fmaxd(u - (float)c, 0.0f); }
+__device__ __forceinline__ float gap_above(float u, int c) { return fmaxd((float)(c + 1) - u, 0.0f); }
+
 template <bool UNIFORM, bool SUP>
 __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
                                               const uint32_t *lcls, const DensitySelf &x, uint16_t *li,
-                                              const FluidArgs &a, const SubstepConsts &c, uint16_t *ls = nullptr,
-                                              uint32_t *scount = nullptr)
+                                              const FluidArgs &a, const SubstepConsts &c, const RowCull &rc,
+                                              uint16_t *ls = nullptr, uint32_t *scount = nullptr)
 {
     uint32_t req = 0, sreq = 0;
+    const float gxl = gap_below(rc.ux, rc.cx), gxr = gap_above(rc.ux, rc.cx);
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
-        const int h0 = hc - 1 + 6 * (r % 3 - 1) + 36 * (r / 3 - 1);
-        const uint32_t f1 = hoff[h0 + 3];
-        for (uint32_t k = hoff[h0]; k < f1; k += 4) {
+        const int ry = r % 3 - 1, rz = r / 3 - 1;
+        const float gy = ry < 0 ? gap_below(rc.uy, rc.cy) : (ry > 0 ? gap_above(rc.uy, rc.cy) : 0.0f);
+        const float gz = rz < 0 ? gap_below(rc.uz, rc.cz) : (rz > 0 ? gap_above(rc.uz, rc.cz) : 0.0f);
+        const float perp = gy * gy + gz * gz;
+        if (!(perp < rc.lim2)) continue;
+        const int h0 = hc - 1 + 6 * ry + 36 * rz;
+        const uint32_t f1 = hoff[h0 + (gxr * gxr + perp < rc.lim2 ? 3 : 2)];
+        for (uint32_t k = hoff[h0 + (gxl * gxl + perp < rc.lim2 ? 0 : 1)]; k < f1; k += 4) {
             float4 pj[4];
             uint32_t qc[4];
 #pragma unroll
@@ -830,13 +866,19 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
             }
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
-            auto scan = [&](auto bu_tag, const DensitySelf &x, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
+            auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
                 constexpr bool BU = decltype(bu_tag)::value;
-                if (!SUP) return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
+                const float inv = c.inv_cell > 0.0f ? c.inv_cell : a.scenes[x.pc.scene].inv_cell;
+                // displacement since the cells were assigned: none at iteration 0; later the largest
+                // |delta| of the previous iteration, where only the fluid passes move fluid particles
+                const float dmax = a.it == 0 ? 0.0f : (a.scenes[x.pc.scene].fluid_only ? fsqrt(md2) : 3.4e38f);
+                const RowCull rc = row_cull(x.s, ci, inv, x.h, dmax);
+                if (!SUP) return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
                 const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
                 if (x.sup && a.it == 0) {
                     uint32_t sreq;
-                    const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c,
+                    const RowCull rs = row_cull(x.s, ci, inv, fsqrt(x.ts2), dmax);
+                    const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, rs,
                                                              a.slist + list_group_base(i, kSuperSlots), &sreq);
                     a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
                     return req;
@@ -845,10 +887,11 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                     ++ma.reuse;
                     return scan_super(a.slist + list_group_base(i, kSuperSlots), sc, lpos, x, li, c);
                 }
-                return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c);
+                return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
             };
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
-                const int hc = halo_cell(ts, a.cellc[i]);
+                const int4 ci = a.cellc[i];
+                const int hc = halo_cell(ts, ci);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
                 uint32_t cnt = 0;
@@ -856,7 +899,8 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
                     DensitySum d;
                     uint16_t *li = a.list + list_group_base(i, c.cap);
                     if (density_walks(x, bu, c)) {
-                        d.req = bu ? scan(std::true_type{}, x, hc, fs, i, li) : scan(std::false_type{}, x, hc, fs, i, li);
+                        d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, i, li)
+                                   : scan(std::false_type{}, x, ci, hc, fs, i, li);
                         d.inr = min(d.req, c.cap);
                         density_list(li, d.inr, lpos, x, d);
                     }
