@@ -157,7 +157,6 @@ void free_scene(SceneBuffers &s)
     dfree(s.meta);
     dfree(s.slot_of);
     dfree(s.posi);
-    dfree(s.posi2);
     dfree(s.soft_ids);
     dfree(s.edges);
     dfree(s.bends);
@@ -188,10 +187,8 @@ void free_constraints(SceneBuffers &s)
     dfree(s.corr);
     dfree(s.inc);
     dfree(s.posi);
-    dfree(s.posi2);
     dfree(s.soft_ids);
     s.posi = nullptr;
-    s.posi2 = nullptr;
     s.soft_ids = nullptr;
     s.nsoft = 0;
     s.edges = nullptr;
@@ -978,7 +975,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         if (s.nd)
             CN_HIP(ctx, hipMemcpy(s.dihedrals, dih.data(), s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
         if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, tets.data(), s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
-        const size_t nlam = 2ull * s.ne + 2ull * (s.nb + s.nd) + s.nt;
+        const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
         CN_HIP(ctx, dalloc(&s.lam, nlam));
         std::vector<uint32_t> inc, map_e, map_b, map_t;
         size_t refs[3] = {0, 0, 0};
@@ -1005,13 +1002,17 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             build_incidence(n, 1, cnt, ar, off, [&](uint32_t, uint32_t c, uint32_t k) { return tets[c].p[k]; }, inc,
                             s.inc_off[4], s.inc_off[5], map_t, 4u, refs[2]);
         }
-        // the tet record map after the incidence structures (tet records in CSR order + a spare);
-        // edges, bends and dihedrals are solved per vertex and keep no records
-        s.inc_off[6] = s.inc_off[7] = s.inc_off[8] = inc.size();
+        // record maps after the incidence structures; records of each group in CSR order + a spare
+        s.inc_off[6] = inc.size();
+        inc.insert(inc.end(), map_e.begin(), map_e.end());
+        s.inc_off[7] = inc.size();
+        inc.insert(inc.end(), map_b.begin(), map_b.end());
+        s.inc_off[8] = inc.size();
         inc.insert(inc.end(), map_t.begin(), map_t.end());
-        (void)refs[0];
-        (void)refs[1];
-        const size_t ncorr = refs[2] + 1;
+        s.corr_off[0] = 0;
+        s.corr_off[1] = refs[0] + 1;
+        s.corr_off[2] = s.corr_off[1] + refs[1] + 1;
+        const size_t ncorr = s.corr_off[2] + refs[2] + 1;
         CN_HIP(ctx, dalloc(&s.corr, ncorr));
         CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
         // constrained particles: any incidence in the three structures
@@ -1027,7 +1028,6 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             if (!soft.empty())
                 CN_HIP(ctx, hipMemcpy(s.soft_ids, soft.data(), soft.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             CN_HIP(ctx, dalloc(&s.posi, n));
-            CN_HIP(ctx, dalloc(&s.posi2, n));
         }
         CN_HIP(ctx, dalloc(&s.inc, inc.size()));
         CN_HIP(ctx, hipMemcpy(s.inc, inc.data(), inc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1108,21 +1108,20 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         sa.nd = s.nd;
         sa.tets = s.tets;
         sa.nt = s.nt;
-        sa.lam_e2[0] = s.lam;
-        sa.lam_e2[1] = s.lam + s.ne;
-        sa.lam_b2[0] = s.lam + 2ull * s.ne;
-        sa.lam_b2[1] = sa.lam_b2[0] + (s.nb + s.nd);
-        sa.lam_t = sa.lam_b2[1] + (s.nb + s.nd);
+        sa.lam_e = s.lam;
+        sa.lam_b = s.lam + s.ne;
+        sa.lam_t = s.lam + s.ne + s.nb + s.nd;
         sa.posi = s.posi;
-        sa.posi2 = s.posi2;
         sa.soft_ids = s.soft_ids;
         sa.nsoft = s.nsoft;
-        sa.corr_t = s.corr;
+        sa.corr_e = s.corr + s.corr_off[0];
+        sa.corr_b = s.corr + s.corr_off[1];
+        sa.corr_t = s.corr + s.corr_off[2];
+        sa.map_e = s.inc + s.inc_off[6];
+        sa.map_b = s.inc + s.inc_off[7];
         sa.map_t = s.inc + s.inc_off[8];
         sa.inc_e_start = s.inc + s.inc_off[0];
-        sa.inc_e_ref = s.inc + s.inc_off[1];
         sa.inc_b_start = s.inc + s.inc_off[2];
-        sa.inc_b_ref = s.inc + s.inc_off[3];
         sa.inc_t_start = s.inc + s.inc_off[4];
         sa.active = s.scene_active;
         sa.pid_scene = s.pid_scene;
@@ -1153,7 +1152,7 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
             }
             if (has_soft && it < soft_iters) {
                 ProfScope ps(ctx, "soft");
-                launch_soft_iteration(sa, it, ctx->stream);
+                launch_soft_iteration(sa, ctx->stream);
             }
             if (has_contacts && it < contact_iters) {
                 ProfScope ps(ctx, "contacts");

@@ -130,8 +130,7 @@ struct SceneBuffers {
     // soft constraints
     uint32_t *slot_of = nullptr;
     float4 *posi = nullptr;         // positions of the constrained particles in particle-index order
-                                    // during the soft phase of an iteration (two buffers)
-    float4 *posi2 = nullptr;
+                                    // during the soft phase of an iteration
     uint32_t *soft_ids = nullptr;   // the constrained particles, ascending
     uint32_t nsoft = 0;
     cn_soft_edge *edges = nullptr;
@@ -139,8 +138,10 @@ struct SceneBuffers {
     cn_cloth_dihedral *dihedrals = nullptr;
     cn_soft_tet *tets = nullptr;
     uint32_t ne = 0, nb = 0, nd = 0, nt = 0;
-    float *lam = nullptr;           // edges x2 | bends + dihedrals x2 | tets
-    float4 *corr = nullptr;         // tet correction records in incidence (CSR) order
+    float *lam = nullptr;           // edges | bends + dihedrals | tets
+    float4 *corr = nullptr;         // correction records of edges | bends + dihedrals | tets, each group
+                                    // in its incidence (CSR) order, then one spare record
+    size_t corr_off[3] = {0, 0, 0};
     uint32_t *inc = nullptr;        // three CSR incidence structures back to back, then the record maps
     size_t inc_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -168,7 +169,6 @@ void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipS
 struct SoftArgs {
     float4 *pos;               // slot order
     float4 *posi;              // particle-index order (constrained particles) while the soft passes run
-    float4 *posi2;             // its second buffer (the per-vertex solves read one and write the other)
     const uint32_t *soft_ids;  // constrained particles
     uint32_t nsoft;
     const uint32_t *slot_of;   // particle index -> slot
@@ -177,25 +177,29 @@ struct SoftArgs {
     float dt, relax;
     const cn_soft_edge *edges;
     uint32_t ne;
-    float *lam_e2[2];          // lambdas, double-buffered across iterations
+    float *lam_e;
+    float4 *corr_e;            // correction records in incidence order
+    const uint32_t *map_e;     // 2 per edge: the position of its vertex records in corr_e
     const cn_soft_bend *bends;
     uint32_t nb;
     const cn_cloth_dihedral *dih;
     uint32_t nd;
-    float *lam_b2[2];          // bends then dihedrals, double-buffered
+    float *lam_b;              // bends then dihedrals
+    float4 *corr_b;
+    const uint32_t *map_b;     // 4 per bend / dihedral (a bend's fourth is the spare record)
     const cn_soft_tet *tets;
     uint32_t nt;
     float *lam_t;
-    float4 *corr_t;            // tet correction records in incidence order
-    const uint32_t *map_t;     // 4 per tet: the position of its vertex records in corr_t
-    const uint32_t *inc_e_start, *inc_e_ref;  // incidence: ref = (constraint << 2) | vertex slot
-    const uint32_t *inc_b_start, *inc_b_ref;
-    const uint32_t *inc_t_start;  // a particle's tet records are corr_t[start[p], start[p + 1])
+    float4 *corr_t;
+    const uint32_t *map_t;     // 4 per tet
+    const uint32_t *inc_e_start;  // incidence ranges: a particle's records are corr_x[start[p], start[p + 1])
+    const uint32_t *inc_b_start;
+    const uint32_t *inc_t_start;
     const uint32_t *active;    // per scene
     const uint32_t *pid_scene; // particle index -> scene
 };
 void launch_soft_clear(const SoftArgs &a, hipStream_t st);
-void launch_soft_iteration(const SoftArgs &a, uint32_t it, hipStream_t st);
+void launch_soft_iteration(const SoftArgs &a, hipStream_t st);
 
 // per-pass entries in the reference's data order (cn_passes.hip)
 void launch_ref_keys(const float4 *pos, uint32_t n, float inv_cell, uint32_t *keys, int32_t *cells,

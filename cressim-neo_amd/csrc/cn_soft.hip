@@ -5,22 +5,12 @@
 // physics_cloth_solve_dihedral_constraints.cs.hlsl:20-95, physics_soft_solve_tet_constraints.cs.hlsl
 // :15-87, physics_soft_apply_{edge,bend,tet}_corrections.cs.hlsl and
 // physics_clear_soft_constraint_state.cs.hlsl:14-54.  Each solve is Jacobi over its constraint type:
-// every constraint reads the positions before the type's corrections, updates its lambda and
-// produces one correction per vertex; each particle then adds the corrections of its incident
-// constraints, summed in ascending constraint order (the incidence lists of
-// physics_scene_gpu_state.cpp:3273-3360).
-//
-// Edges, bends and dihedrals are solved per vertex: a particle's thread evaluates each incident
-// constraint itself (the arithmetic of a constraint is the same from every vertex), takes its own
-// vertex's correction, and the constraint's first vertex writes the new lambda.  Positions and
-// lambdas are double-buffered so every evaluation reads the values before the type's update.  No
-// correction records are written or read.  Tets keep records (written at the vertex's incidence
-// slot, gathered per particle): when a tet's solve bails out the reference leaves its record as
-// it was and applies it again, which needs the record.  Constraints refer to particle indices; the
-// soft phase works on the constrained particles gathered into particle-index order.
+// one thread per constraint reads the current positions, updates its lambda and writes one
+// correction record per vertex; the apply pass then sums, per particle, the records of its incident
+// constraints in ascending constraint order (the incidence lists of
+// physics_scene_gpu_state.cpp:3273-3360) and adds the sum to the position.  Constraints refer to
+// particle indices; positions live in slot order, reached through slot_of[particle index].
 #include <hip/hip_runtime.h>
-
-#include <utility>
 
 #include "cn_internal.h"
 
@@ -51,58 +41,66 @@ __device__ __forceinline__ void cross3d(float ax, float ay, float az, float bx, 
 __global__ __launch_bounds__(kBlock) void k_soft_clear(SoftArgs a)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < a.ne) a.lam_e2[0][i] = 0.0f;
-    if (i < a.nb + a.nd) a.lam_b2[0][i] = 0.0f;
+    if (i < a.ne) a.lam_e[i] = 0.0f;
+    if (i < a.nb + a.nd) a.lam_b[i] = 0.0f;
     if (i < a.nt) a.lam_t[i] = 0.0f;
 }
 
-// One solve of a constraint: the lambda after it (unchanged when the solve does not apply) and the
-// vertex corrections (zero then).
-struct Solved {
-    float lam;
-    float4 c[4];
-};
-
-__device__ __forceinline__ Solved edge_solve(const SoftArgs &a, const float4 *P, uint32_t e, float lam)
+__global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
     const cn_soft_edge E = a.edges[e];
-    Solved res;
+    if (!live(a, E.a)) return;
+    const RecOut out{a.corr_e, a.map_e + 2 * (size_t)e};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    res.lam = lam;
-    res.c[0] = res.c[1] = res.c[2] = res.c[3] = z4;
-    float4 *out = res.c;
-    if (E.enabled == 0u) return res;
-    const float4 pa = P[E.a], pb = P[E.b];
+    if (E.enabled == 0u) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
+    const float4 pa = ldp(a, E.a), pb = ldp(a, E.b);
     const float wa = pa.w, wb = pb.w, wsum = wa + wb;
-    if (wsum <= kEps) return res;
+    if (wsum <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
     const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
     const float l2 = dot3(dx, dy, dz, dx, dy, dz);
-    if (l2 <= kEps) return res;
+    if (l2 <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
     const float len = fsqrt(l2);
     const float gax = fdiv(-dx, len), gay = fdiv(-dy, len), gaz = fdiv(-dz, len);
     const float gbx = -gax, gby = -gay, gbz = -gaz;
     const float alpha = fdiv(fmaxd(E.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
     const float C = len - E.rest_length;
-    const float lambda = lam;
+    const float lambda = a.lam_e[e];
     const float den = wsum + alpha;
-    if (den <= kEps) return res;
+    if (den <= kEps) {
+        out[0] = z4;
+        out[1] = z4;
+        return;
+    }
     const float dl = fdiv(-(C + alpha * lambda), den);
-    res.lam = lambda + dl;
+    a.lam_e[e] = lambda + dl;
     const float fa = wa * dl, fb = wb * dl;
     out[0] = make_float4(fa * gax * a.relax, fa * gay * a.relax, fa * gaz * a.relax, 0.0f);
     out[1] = make_float4(fb * gbx * a.relax, fb * gby * a.relax, fb * gbz * a.relax, 0.0f);
-    return res;
 }
 
-__device__ __forceinline__ Solved bend_solve(const SoftArgs &a, const float4 *P, uint32_t b, float lam)
+__global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= a.nb) return;
     const cn_soft_bend B = a.bends[b];
-    Solved res;
+    if (!live(a, B.p0)) return;
+    const RecOut out{a.corr_b, a.map_b + 4 * (size_t)b};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    res.lam = lam;
-    res.c[0] = res.c[1] = res.c[2] = res.c[3] = z4;
-    float4 *out = res.c;
-    const float4 p0 = P[B.p0], p1 = P[B.p1], p2 = P[B.p2];
+    const float4 p0 = ldp(a, B.p0), p1 = ldp(a, B.p1), p2 = ldp(a, B.p2);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w;
     bool zero = w0 + w1 + w2 <= kEps;
     float g0x = 0, g0y = 0, g0z = 0, g1x = 0, g1y = 0, g1z = 0, g2x = 0, g2y = 0, g2z = 0, dl = 0;
@@ -134,29 +132,33 @@ __device__ __forceinline__ Solved bend_solve(const SoftArgs &a, const float4 *P,
             zero = den + alpha <= kEps;
             if (!zero) {
                 const float C = theta - B.rest_angle;
-                const float lambda = lam;
+                const float lambda = a.lam_b[b];
                 dl = fdiv(-(C + alpha * lambda), den + alpha);
-                res.lam = lambda + dl;
+                a.lam_b[b] = lambda + dl;
             }
         }
     }
-    if (zero) return res;
+    if (zero) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, r = a.relax;
     out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
     out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
     out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
-    return res;
+    out[3] = z4;
 }
 
-__device__ __forceinline__ Solved dihedral_solve(const SoftArgs &a, const float4 *P, uint32_t d, float lam)
+__global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
 {
+    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    if (d >= a.nd) return;
+    const uint32_t ci = a.nb + d;
     const cn_cloth_dihedral D = a.dih[d];
-    Solved res;
+    if (!live(a, D.e0)) return;
+    const RecOut out{a.corr_b, a.map_b + 4 * (size_t)ci};
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    res.lam = lam;
-    res.c[0] = res.c[1] = res.c[2] = res.c[3] = z4;
-    float4 *out = res.c;
-    const float4 e0m = P[D.e0], e1m = P[D.e1], o0m = P[D.o0], o1m = P[D.o1];
+    const float4 e0m = ldp(a, D.e0), e1m = ldp(a, D.e1), o0m = ldp(a, D.o0), o1m = ldp(a, D.o1);
     // p0, p1 opposite vertices; p2, p3 edge vertices
     const float ex = e1m.x - e0m.x, ey = e1m.y - e0m.y, ez = e1m.z - e0m.z;
     const float el2 = dot3(ex, ey, ez, ex, ey, ez);
@@ -164,7 +166,10 @@ __device__ __forceinline__ Solved dihedral_solve(const SoftArgs &a, const float4
     cross3d(e0m.x - o0m.x, e0m.y - o0m.y, e0m.z - o0m.z, e1m.x - o0m.x, e1m.y - o0m.y, e1m.z - o0m.z, n0x, n0y, n0z);
     cross3d(e1m.x - o1m.x, e1m.y - o1m.y, e1m.z - o1m.z, e0m.x - o1m.x, e0m.y - o1m.y, e0m.z - o1m.z, n1x, n1y, n1z);
     const float n0s = dot3(n0x, n0y, n0z, n0x, n0y, n0z), n1s = dot3(n1x, n1y, n1z, n1x, n1y, n1z);
-    if (el2 <= kEps || n0s <= kEps || n1s <= kEps) return res;
+    if (el2 <= kEps || n0s <= kEps || n1s <= kEps) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
     const float el = fsqrt(el2);
     const float iel = fdiv(1.0f, el);
     const float s0x = fdiv(n0x, n0s), s0y = fdiv(n0y, n0s), s0z = fdiv(n0z, n0s);
@@ -191,16 +196,18 @@ __device__ __forceinline__ Solved dihedral_solve(const SoftArgs &a, const float4
     const float den = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
                       w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
     const float alpha = fdiv(fmaxd(D.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
-    if (den + alpha <= kEps) return res;
-    const float lambda = lam;
+    if (den + alpha <= kEps) {
+        out[0] = out[1] = out[2] = out[3] = z4;
+        return;
+    }
+    const float lambda = a.lam_b[ci];
     const float dl = fdiv(-(C + alpha * lambda), den + alpha);
-    res.lam = lambda + dl;
+    a.lam_b[ci] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
     out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
     out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
     out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
     out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
-    return res;
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
@@ -273,55 +280,6 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     a.posi[pid] = p;
 }
 
-// Per constrained particle: every incident constraint of the type solved from the input positions
-// and lambdas, its own vertex's corrections summed in incidence order and added to its position in
-// the output buffer (a copy for particles of scenes left untouched, pinned ones and those without
-// constraints of the type); the constraint's first vertex writes its new lambda.
-template <class Solve>
-__device__ __forceinline__ void solve_gather(const SoftArgs &a, const uint32_t *start, const uint32_t *ref,
-                                             const float4 *pin, float4 *pout, const float *lin, float *lout,
-                                             Solve &&solve)
-{
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= a.nsoft) return;
-    const uint32_t pid = a.soft_ids[j];
-    float4 p = pin[pid];
-    const uint32_t k0 = start[pid], k1 = start[pid + 1];
-    if (k0 != k1 && live(a, pid)) {
-        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t r = ref[k], c = r >> 2, slot = r & 3u;
-            const Solved sv = solve(c, lin[c]);
-            if (slot == 0u) lout[c] = sv.lam;
-            const float4 d = sv.c[slot];
-            tx = tx + d.x;
-            ty = ty + d.y;
-            tz = tz + d.z;
-        }
-        if (p.w > kEps) {
-            p.x = p.x + tx;
-            p.y = p.y + ty;
-            p.z = p.z + tz;
-        }
-    }
-    pout[pid] = p;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gather_edges(SoftArgs a, const float4 *pin, float4 *pout, const float *lin,
-                                                         float *lout)
-{
-    solve_gather(a, a.inc_e_start, a.inc_e_ref, pin, pout, lin, lout,
-                 [&](uint32_t e, float lam) { return edge_solve(a, pin, e, lam); });
-}
-
-__global__ __launch_bounds__(kBlock) void k_gather_bends(SoftArgs a, const float4 *pin, float4 *pout, const float *lin,
-                                                         float *lout)
-{
-    solve_gather(a, a.inc_b_start, a.inc_b_ref, pin, pout, lin, lout, [&](uint32_t c, float lam) {
-        return c < a.nb ? bend_solve(a, pin, c, lam) : dihedral_solve(a, pin, c - a.nb, lam);
-    });
-}
-
 // The soft phase works on the constrained particles in particle-index order, where the
 // constraints' vertices are neighbors in memory: gathered from slot order before it, put back after.
 __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
@@ -347,30 +305,21 @@ void launch_soft_clear(const SoftArgs &a, hipStream_t st)
     if (m) hipLaunchKernelGGL(k_soft_clear, dim3(grid_n(m)), dim3(kBlock), 0, st, a);
 }
 
-void launch_soft_iteration(const SoftArgs &a0, uint32_t it, hipStream_t st)
+void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
 {
-    // physics_solver.cpp:915-1019: edges, then bends + dihedrals, then tets solve/apply
-    if (!a0.nsoft) return;
-    const dim3 grid(grid_n(a0.nsoft)), block(kBlock);
-    SoftArgs a = a0;
-    hipLaunchKernelGGL(k_soft_gather, grid, block, 0, st, a);
-    float4 *cur = a.posi, *other = a.posi2;
-    const int in = (int)(it & 1u), out = 1 - in;
-    if (a.ne) {
-        hipLaunchKernelGGL(k_gather_edges, grid, block, 0, st, a, cur, other, a.lam_e2[in], a.lam_e2[out]);
-        std::swap(cur, other);
-    }
-    if (a.nb + a.nd) {
-        hipLaunchKernelGGL(k_gather_bends, grid, block, 0, st, a, cur, other, a.lam_b2[in], a.lam_b2[out]);
-        std::swap(cur, other);
-    }
-    a.posi = cur;
-    if (a.nt) {
-        hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), block, 0, st, a);
-        hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), block, 0, st, a, a.inc_t_start,
-                           (const float4 *)a.corr_t);
-    }
-    hipLaunchKernelGGL(k_soft_scatter, grid, block, 0, st, a);
+    // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
+    if (a.nsoft) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
+    if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
+                       (const float4 *)a.corr_e);
+    if (a.nb) hipLaunchKernelGGL(k_solve_bends, dim3(grid_n(a.nb)), dim3(kBlock), 0, st, a);
+    if (a.nd) hipLaunchKernelGGL(k_solve_dihedrals, dim3(grid_n(a.nd)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start,
+                       (const float4 *)a.corr_b);
+    if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
+                       (const float4 *)a.corr_t);
+    if (a.nsoft) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
 }
 
 }  // namespace cn
