@@ -98,6 +98,35 @@ struct SubstepConsts {
 
 __device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
 __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
+
+// a / b for b > 0 from y = 1 / b, the correctly rounded reciprocal computed once per divisor:
+// q0 = a * y lies within one ulp of a / b, the remainder a - b * q0 is exact in an fma, and
+// q0 + remainder * y rounds to the correctly rounded quotient (Markstein's theorem).  Exact while
+// neither the remainder nor the quotient underflows: a == 0, or |a| >= 2^-96 with
+// 2^-100 <= |a / b| < 2^100.  The last step is taken negated so that a zero dividend gives the
+// signed zero a / b gives.  Three VALU operations against the ten of the scaled IEEE sequence.
+__device__ __forceinline__ float fdiv_rcp(float a, float b, float y)
+{
+    const float q0 = __fmul_rn(a, y);
+    const float r = __builtin_fmaf(-q0, b, a);
+    return -__builtin_fmaf(-r, y, -q0);
+}
+
+// Square root by the hardware estimate (within one ulp) and the rounding test on its two
+// neighbours: s - ulp is the root when x - (s - ulp) * s <= 0, s + ulp when x - (s + ulp) * s > 0.
+// Equal to fsqrt for x == 0 and for 2^-96 <= x < inf, where fsqrt applies no scaling; for
+// 0 < x < 2^-96 the result is within a few ulps of the root and below 2^-47, which the fluid
+// terms consume only through dist <= eps tests and through d / h < 2^-25 (1 - d / h rounds to 1).
+__device__ __forceinline__ float fsqrt_nearzero_free(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x);
+    const float rp = __builtin_fmaf(-sp, s, x);
+    const float t = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : t;
+}
 __device__ __forceinline__ float fmaxd(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fmind(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float fsat(float a) { return fmind(fmaxd(a, 0.0f), 1.0f); }
@@ -167,6 +196,22 @@ __device__ __forceinline__ float cohesion_kernel(float d, float h, float c1, flo
 {
     if (d <= kEps || d >= h) return 0.0f;
     const float q = fdiv(d, h);
+    const float a = __fmul_rn(__fmul_rn(__fmul_rn(c1, q), q), q);
+    const float b = __fmul_rn(__fmul_rn(c2, q), q);
+    return __fsub_rn(__fadd_rn(a, b), 1.0f);
+}
+
+// The same kernels from q = d / h supplied by the caller (the per-neighbor fluid terms share
+// one quotient between the kernels of a pair).
+__device__ __forceinline__ float spiky_w_q(float d, float h, float q, float k)
+{
+    if (d >= h) return 0.0f;
+    const float omq = __fsub_rn(1.0f, q);
+    return __fmul_rn(__fmul_rn(k, omq), omq);
+}
+__device__ __forceinline__ float cohesion_q(float d, float h, float q, float c1, float c2)
+{
+    if (d <= kEps || d >= h) return 0.0f;
     const float a = __fmul_rn(__fmul_rn(__fmul_rn(c1, q), q), q);
     const float b = __fmul_rn(__fmul_rn(c2, q), q);
     return __fsub_rn(__fadd_rn(a, b), 1.0f);

@@ -178,12 +178,54 @@ struct DensitySum {
     uint32_t inr = 0, req = 0;
 };
 
+// Quotients of one neighbor pair: n = (dx, dy, dz) / dist, q = dist / h and the spiky
+// derivative der = (-2 k (1 - q)) / h (FluidSpikyKernelDerivative, physics_fluid_common.hlsli:31-42;
+// 0 unless eps < dist < h).  FAST divides through reciprocals (fdiv_rcp: 1 / dist once per pair,
+// 1 / h once per particle); quot_fast_ok says when that is exact, and the other particles take
+// the IEEE divisions.
+struct PairQuot {
+    float nx, ny, nz, q, der;
+};
+
+// fdiv_rcp is exact for every quotient of pair_quot when h <= 1024 and each coordinate of the
+// particle has magnitude >= 2^-62: a nonzero coordinate difference is then >= 2^-86 (both values
+// are multiples of 2^-86, or the difference exceeds half the larger one), the live distances lie
+// in (eps, h), and with h in [1e-4, 1024] the spiky numerator 2 k (1 - q) lies in
+// [2^-51, 2^43] (k = 15 / (pi h^3), 1 - q >= 2^-24 when nonzero), so every dividend is 0 or
+// >= 2^-96 and every quotient lies in [2^-100, 2^100).
+__device__ __forceinline__ bool quot_fast_ok(float h, float sx, float sy, float sz)
+{
+    return h <= 1024.0f && fminf(fminf(fabsf(sx), fabsf(sy)), fabsf(sz)) >= 0x1p-62f;
+}
+
+template <bool FAST>
+__device__ __forceinline__ void pair_quot(PairQuot &p, float dx, float dy, float dz, float dist, float h, float ih,
+                                          float k)
+{
+    if (FAST) {
+        const float y = fdiv(1.0f, dist);
+        p.nx = fdiv_rcp(dx, dist, y);
+        p.ny = fdiv_rcp(dy, dist, y);
+        p.nz = fdiv_rcp(dz, dist, y);
+        p.q = fdiv_rcp(dist, h, ih);
+    } else {
+        p.nx = fdiv(dx, dist);
+        p.ny = fdiv(dy, dist);
+        p.nz = fdiv(dz, dist);
+        p.q = fdiv(dist, h);
+    }
+    const float num = __fmul_rn(__fmul_rn(-2.0f, k), __fsub_rn(1.0f, p.q));
+    const float der = FAST ? fdiv_rcp(num, h, ih) : fdiv(num, h);
+    p.der = (dist <= kEps || dist >= h) ? 0.0f : der;
+}
+
 struct DensitySelf {
     ParticleClass pc;
     cn_fluid_material m;
     float4 s;
-    float h, t2, kw;
+    float h, t2, kw, ih;  // ih = 1 / h
     bool tension;
+    bool fd;          // the neighbor quotients may use fdiv_rcp (pair_quot)
     bool sup;         // candidate lists apply (fluid-only scene, tiled pass)
     float ts2, allow; // candidate threshold; largest max |delta|^2 under which they stay complete
 };
@@ -205,6 +247,8 @@ __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const 
     x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
     x.t2 = a.t2[x.pc.fluid_mat];
     x.kw = spiky_coef(x.h);
+    x.ih = fdiv(1.0f, x.h);
+    x.fd = quot_fast_ok(x.h, s.x, s.y, s.z);
     x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
     if (dprev >= 0.0f && a.scenes[x.pc.scene].fluid_only) {
         // a pair in radius at iteration k was within h + 2 max|delta_k| at iteration 0, so the
@@ -243,15 +287,19 @@ __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const 
 {
     DensityTerm t;
     const float dx = x.s.x - pj.x, dy = x.s.y - pj.y, dz = x.s.z - pj.z;
-    const float dist = fsqrt(dot3(dx, dy, dz, dx, dy, dz));
-    t.w = spiky_w_k(dist, x.h, x.kw);
+    const float dist = fsqrt_nearzero_free(dot3(dx, dy, dz, dx, dy, dz));
+    float q = fdiv_rcp(dist, x.h, x.ih);
+    if (!x.fd) q = fdiv(dist, x.h);
+    t.w = spiky_w_q(dist, x.h, q, x.kw);
     t.gx = t.gy = t.gz = 0.0f;
     if (x.tension && !(dist <= kEps || dist >= x.h)) {
         // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53
-        const float dw = spiky_dw_k(dist, x.h, x.kw);
-        t.gx = dw * fdiv(dx, dist);
-        t.gy = dw * fdiv(dy, dist);
-        t.gz = dw * fdiv(dz, dist);
+        PairQuot p;
+        pair_quot<true>(p, dx, dy, dz, dist, x.h, x.ih, x.kw);
+        if (!x.fd) pair_quot<false>(p, dx, dy, dz, dist, x.h, x.ih, x.kw);
+        t.gx = p.der * p.nx;
+        t.gy = p.der * p.ny;
+        t.gz = p.der * p.nz;
     }
     return t;
 }
@@ -327,8 +375,8 @@ struct DeltaSelf {
     ParticleClass pc;
     cn_fluid_material m;
     float4 di, nci;
-    float sx, sy, sz, Ci, h, t2, kw, cohesion, viscosity, cfl, dt;
-    bool tension, visc_on, cfl_on, need_rel;
+    float sx, sy, sz, Ci, h, t2, kw, ih, cohesion, viscosity, cfl, dt;
+    bool tension, visc_on, cfl_on, need_rel, fd;
 };
 
 struct DeltaTerm {
@@ -343,10 +391,12 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
 {
     DeltaTerm t;
     const float ddx = x.sx - cq.x, ddy = x.sy - cq.y, ddz = x.sz - cq.z;
-    const float dist = fsqrt(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
+    const float dist = fsqrt_nearzero_free(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
     t.live = !(dist <= kEps);
-    const float nx = fdiv(ddx, dist), ny = fdiv(ddy, dist), nz = fdiv(ddz, dist);
-    const float der = spiky_dw_k(dist, x.h, x.kw);
+    PairQuot p;
+    pair_quot<true>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
+    if (!x.fd) pair_quot<false>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
+    const float nx = p.nx, ny = p.ny, nz = p.nz, der = p.der;
     const float sc = -0.5f * (x.Ci + cq.w) * der;
     float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
     const bool same = UNIFORM || x.pc.fluid_mat == a.classes[qcls].fluid_mat;
@@ -358,13 +408,13 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
             rz = x.di.z - dq.z;
         }
         if (x.cohesion > kEps) {
-            const float ck = x.cohesion * cohesion_kernel(dist, x.h, x.m.cohesion1, x.m.cohesion2);
+            const float ck = x.cohesion * cohesion_q(dist, x.h, p.q, x.m.cohesion1, x.m.cohesion2);
             pdx = pdx - nx * ck;
             pdy = pdy - ny * ck;
             pdz = pdz - nz * ck;
         }
         if (x.visc_on) {
-            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_k(dist, x.h, x.kw));
+            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_q(dist, x.h, p.q, x.kw));
             pdx = pdx - vs * rx;
             pdy = pdy - vs * ry;
             pdz = pdz - vs * rz;
@@ -431,6 +481,8 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
         x.t2 = a.t2[pc.fluid_mat];
         x.kw = spiky_coef(x.h);
+        x.ih = fdiv(1.0f, x.h);
+        x.fd = quot_fast_ok(x.h, x.sx, x.sy, x.sz);
         x.dt = c.dt;
         x.cohesion = c.dt * fmaxd(x.m.cohesion, 0.0f);
         x.viscosity = c.dt * fmaxd(x.m.viscosity, 0.0f);

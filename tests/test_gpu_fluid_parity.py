@@ -152,6 +152,45 @@ def test_step_parity_hash_equals_morton(solver, oracle):
         np.testing.assert_array_equal(a, b)
 
 
+def quotient_edge_scene(n=1500, seed=21):
+    """A cloud around the origin: coordinates exactly 0, below 2^-62 in magnitude, and pairs a
+    hair apart, with a second material that has surface tension and cohesion.  Those particles
+    take the IEEE divisions of the neighbor terms; the rest take the reciprocal form."""
+    rng = np.random.default_rng(seed)
+    r = 0.02
+    pts = (rng.random((n, 3)).astype(np.float32) - F(0.5)) * F(0.5)
+    pts[0:60, 0] = 0.0
+    pts[60:90, 1] = F(1e-25)
+    pts[90:110, 2] = F(-3e-22)
+    pts[110:120, 0] = F(2.0 ** -70)
+    pts[200] = pts[201] + np.array([1e-30, 0, 0], np.float32)
+    pts[202] = pts[203]
+    sc = scenes._fluid_scene(pts, r, scenes.container((-0.3, -0.3, -0.3), (0.3, 0.3, 0.3)), "qedge", {})
+    sc.fluid_mats = np.concatenate([cnhip.fluid_material(r), cnhip.fluid_material(r, cohesion=0.5,
+                                                                                  surface_tension=0.2)])
+    sc.fluid_mat = (rng.random(n) < 0.4).astype(np.uint32)
+    return sc
+
+
+def huge_radius_scene(n=40, seed=22):
+    """Smoothing radius above 1024: the neighbor terms take the IEEE divisions."""
+    rng = np.random.default_rng(seed)
+    r = 600.0
+    sc = scenes._fluid_scene(rng.random((n, 3)).astype(np.float32) * F(3000.0), r,
+                             scenes.container((-100, -100, -100), (3100, 3100, 3100), thickness=50.0),
+                             "huge", {})
+    assert float(sc.fluid_mats["smoothing_radius"][0]) > 1024.0
+    return sc
+
+
+@pytest.mark.parametrize("builder", ["edges", "huge_h"])
+def test_step_parity_quotient_edge_cases(solver, oracle, builder):
+    sc = {"edges": quotient_edge_scene, "huge_h": huge_radius_scene}[builder]()
+    res = run_pair(sc, params(cap=64), 4, solver, oracle, resync=False)
+    print(builder, res)
+    assert all(b for _, _, b in res), res
+
+
 def test_step_parity_dam_break_64k(solver, oracle):
     sc = scenes.dam_break_64k()
     res = run_pair(sc, params(cap=64), 3, solver, oracle)
