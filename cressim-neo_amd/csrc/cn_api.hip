@@ -37,7 +37,7 @@ struct cn_ctx {
     bool has_solid = false;          // any solid particle
     bool any_pairs = false;          // some scene can hold a particle-particle contact pair
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
-    uint32_t grid_max = 1024;        // workgroups of a persistent fluid launch
+    uint32_t grid_max = 768;         // workgroups of a persistent fluid launch (three per CU)
     // scene pool
     uint32_t nscenes = 0;
     uint32_t scene_bits = 0;         // key bits above the Morton code holding the scene rank
@@ -1367,7 +1367,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ea, hipEventDisableSystemFence);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&eb, hipEventDisableSystemFence);
     const uint32_t tiles[] = {1024, 2048, 3072, 4096};
-    const uint32_t grids[] = {256, 512, 1024, 2048};
+    const uint32_t grids[] = {256, 512, 768, 1024};
     float best = 3.4e38f, dflt = 0.0f;
     uint32_t best_tile = tile0, best_grid = grid0, tried = 0;
     for (uint32_t tile : tiles)

@@ -169,6 +169,39 @@ __device__ __forceinline__ void flush_meta(const MetaAcc &ma, const FluidArgs &a
     }
 }
 
+// The persistent tiled passes gather one block's counters at a time into workgroup totals in LDS
+// (so no per-lane accumulator lives across blocks) and flush those once at the end.
+constexpr uint32_t kWgMetaWords = 6;  // largest request, truncations, listed, reuses, LDS blocks, run blocks
+
+__device__ __forceinline__ void meta_to_wg(const MetaAcc &ma, uint32_t *s_meta)
+{
+    const uint32_t wmax = wave_max(ma.req);
+    const uint32_t wtr = wave_sum(ma.trunc);
+    const uint32_t wls = wave_sum(ma.listed);
+    const uint32_t wru = wave_sum(ma.reuse);
+    if ((threadIdx.x & 63u) == 0) {
+        if (wmax) atomicMax(&s_meta[0], wmax);
+        if (wtr) atomicAdd(&s_meta[1], wtr);
+        if (wls) atomicAdd(&s_meta[2], wls);
+        if (wru) atomicAdd(&s_meta[3], wru);
+    }
+    if (threadIdx.x == 0) {
+        s_meta[4] += ma.lds;
+        s_meta[5] += ma.runs;
+    }
+}
+
+__device__ __forceinline__ void flush_wg_meta(const uint32_t *s_meta, const FluidArgs &a)
+{
+    if (threadIdx.x != 0) return;
+    if (s_meta[0]) atomicMax(&a.meta[0], s_meta[0]);
+    if (s_meta[1]) atomicAdd(&a.meta[1], s_meta[1]);
+    if (s_meta[3]) atomicAdd(&a.meta[9], s_meta[3]);
+    if (a.it == 0 && s_meta[2]) atomicAdd((unsigned long long *)&a.meta[2], (unsigned long long)s_meta[2]);
+    if (s_meta[4]) atomicAdd(&a.meta[5], s_meta[4]);
+    if (s_meta[5]) atomicAdd(&a.meta[6], s_meta[5]);
+}
+
 // ---------------------------------------------------------------------------------------------
 // density pass: physics_fluid_build_neighbors.cs.hlsl:88-131 (the list filter) with
 // physics_fluid_compute_density_lambda.cs.hlsl:26-115 (density, surface normal, constraint)
@@ -665,6 +698,11 @@ __global__ __launch_bounds__(kBlock) void k_blk_halo(FluidArgs a, SubstepConsts 
 }
 
 constexpr int kTileBlock = 512;  // threads of a tiled fluid workgroup: one pass over a block's particles
+// waves per SIMD the tiled passes are compiled for (at most 80 VGPRs): three 8-wave workgroups per
+// CU, which their LDS budget (kDynLdsBytes) also allows
+constexpr int kTileWaves = 6;
+constexpr uint32_t kTermStep = 4;  // listed neighbors whose loads are issued together in the term loops
+constexpr uint32_t kTilePad = 4;   // spare entries after each staged density array (walk loads past a range)
 
 // Common head of the tiled passes: claim a block (next() runs on thread 0), decide LDS or run
 // table, load the halo table.  skip_to is a halo size above the LDS capacity up to which a block is
@@ -768,14 +806,65 @@ __device__ __forceinline__ RowCull row_cull(const float4 s, const int4 ci, float
 __device__ __forceinline__ float gap_below(float u, int c) { return fmaxd(u - (float)c, 0.0f); }
 __device__ __forceinline__ float gap_above(float u, int c) { return fmaxd((float)(c + 1) - u, 0.0f); }
 
+// Appends listed halo offsets to a particle's list: p is the next entry (lists are grouped, one
+// entry every kListGroup halfwords), n the untruncated count.
+struct ListOut {
+    uint16_t *p;
+    uint32_t n;
+    __device__ __forceinline__ void push(uint32_t f, uint32_t cap)
+    {
+        if (n < cap) *p = (uint16_t)f;
+        p += kListGroup;
+        ++n;
+    }
+};
+
+// One contiguous range [k, f1) of the walk, four candidates per step: the loads read through a
+// pointer with immediate offsets and may run up to three entries past f1 (the staged arrays carry
+// kTilePad spare entries), which the k + u < f1 test then drops.
+template <bool UNIFORM, bool SUP>
+__device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4 *lpos, const uint32_t *lcls,
+                                           const DensitySelf &x, const FluidArgs &a, uint32_t cap, ListOut &lo,
+                                           ListOut &so)
+{
+    const float4 *pp = lpos + k;
+    const uint32_t *pc = lcls + k;
+    for (; k < f1; k += 4, pp += 4, pc += 4) {
+        float4 pj[4];
+        uint32_t qc[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            pj[u] = pp[u];
+            qc[u] = UNIFORM ? 0u : pc[u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
+            const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+            bool cand = (SUP ? d2 < x.ts2 : d2 < x.t2) & (u == 0 || k + u < f1);
+            if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
+            if (cand) {
+                if (SUP) {
+                    so.push(k + u, kSuperSlots);
+                    if (d2 < x.t2) lo.push(k + u, cap);
+                } else {
+                    lo.push(k + u, cap);
+                }
+            }
+        }
+    }
+}
+
 template <bool UNIFORM, bool SUP>
 __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
                                               const uint32_t *lcls, const DensitySelf &x, uint16_t *li,
                                               const FluidArgs &a, const SubstepConsts &c, const RowCull &rc,
                                               uint16_t *ls = nullptr, uint32_t *scount = nullptr)
 {
-    uint32_t req = 0, sreq = 0;
+    ListOut lo{li, 0u}, so{ls, 0u};
     const float gxl = gap_below(rc.ux, rc.cx), gxr = gap_above(rc.ux, rc.cx);
+    // rows of three cells along x, z outer, y inner; the centre row (r = 4) is walked as the
+    // ranges before and after the particle itself
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
         const int ry = r % 3 - 1, rz = r / 3 - 1;
@@ -785,44 +874,16 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
         if (!(perp < rc.lim2)) continue;
         const int h0 = hc - 1 + 6 * ry + 36 * rz;
         const uint32_t f1 = hoff[h0 + (gxr * gxr + perp < rc.lim2 ? 3 : 2)];
-        for (uint32_t k = hoff[h0 + (gxl * gxl + perp < rc.lim2 ? 0 : 1)]; k < f1; k += 4) {
-            float4 pj[4];
-            uint32_t qc[4];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t f = min(k + u, f1 - 1);
-                pj[u] = lpos[f];
-                qc[u] = UNIFORM ? 0u : lcls[f];
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t f = k + u;
-                const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
-                const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-                if (SUP) {
-                    bool cand = (f < f1) & (f != fs) & (d2 < x.ts2);
-                    if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
-                    if (cand) {
-                        if (sreq < kSuperSlots) ls[sreq * kListGroup] = (uint16_t)f;
-                        ++sreq;
-                        if (d2 < x.t2) {
-                            if (req < c.cap) li[req * kListGroup] = (uint16_t)f;
-                            ++req;
-                        }
-                    }
-                } else {
-                    bool ok = (f < f1) & (f != fs) & (d2 < x.t2);
-                    if (!UNIFORM) ok = ok && neighbor_ok(x.pc, a.classes[qc[u]]);
-                    if (ok) {
-                        if (req < c.cap) li[req * kListGroup] = (uint16_t)f;
-                        ++req;
-                    }
-                }
-            }
+        const uint32_t k0 = hoff[h0 + (gxl * gxl + perp < rc.lim2 ? 0 : 1)];
+        if (r == 4) {
+            scan_range<UNIFORM, SUP>(k0, fs, lpos, lcls, x, a, c.cap, lo, so);
+            scan_range<UNIFORM, SUP>(fs + 1, f1, lpos, lcls, x, a, c.cap, lo, so);
+        } else {
+            scan_range<UNIFORM, SUP>(k0, f1, lpos, lcls, x, a, c.cap, lo, so);
         }
     }
-    if (SUP) *scount = sreq;
-    return req;
+    if (SUP) *scount = so.n;
+    return lo.n;
 }
 
 // A later iteration's list from the iteration-0 candidates: the same walk order, the same in-radius
@@ -851,19 +912,30 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
     return req;
 }
 
-// Density sums over a compacted list, four entries per step (loads first, sums in list order).
+// Density sums over a compacted list, kTermStep entries per step (loads first, sums in list order).
+// List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past the end).
+__device__ __forceinline__ void list_group(const uint16_t *li, uint32_t k, uint32_t cnt, uint32_t (&e)[kTermStep])
+{
+#pragma unroll
+    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
+}
+
 __device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, const float4 *lpos,
                                              const DensitySelf &x, DensitySum &d)
 {
-    for (uint32_t k = 0; k < cnt; k += 4) {
-        uint32_t e[4];
+    // the next group's entries are loaded while the current group's terms are evaluated
+    uint32_t en[kTermStep];
+    if (cnt > 0) list_group(li, 0, cnt, en);
+    for (uint32_t k = 0; k < cnt; k += kTermStep) {
+        uint32_t e[kTermStep];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
-        DensityTerm t[4];
+        for (uint32_t u = 0; u < kTermStep; ++u) e[u] = en[u];
+        list_group(li, k + kTermStep, cnt, en);
+        DensityTerm t[kTermStep];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) t[u] = density_term(x, lpos[e[u]]);
+        for (uint32_t u = 0; u < kTermStep; ++u) t[u] = density_term(x, lpos[e[u]]);
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
+        for (uint32_t u = 0; u < kTermStep; ++u)
             if (k + u < cnt) density_add(d, x, t[u]);
     }
 }
@@ -874,14 +946,16 @@ __device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, c
 // SUP: the candidate lists are in use (a separate instantiation, so the default pass carries none of
 // their registers)
 template <bool UNIFORM, bool SUP>
-__global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_density_tiled(
+    FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
     __shared__ uint32_t hstart[kHalo];
     __shared__ uint32_t s_blk;
+    __shared__ uint32_t s_meta[kWgMetaWords];
     float4 *lpos = dyn;
-    uint32_t *lcls = reinterpret_cast<uint32_t *>(dyn + a.tile);
+    uint32_t *lcls = reinterpret_cast<uint32_t *>(dyn + a.tile + kTilePad);
     const uint32_t tid = threadIdx.x;
     // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
     // from its iteration-0 position
@@ -890,14 +964,15 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
     if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
+    if (tid < kWgMetaWords) s_meta[tid] = 0u;  // ordered before any use by tile_begin's barrier
     BlockQueue bq(blockIdx.x);
-    MetaAcc ma;
     TileState ts;
     while (tile_begin([&] { return bq.next(queue, nblk); }, &s_blk, hoff, hstart, ts, 0u, a, c)) {
         if (ts.skip) {
             __syncthreads();
             continue;
         }
+        MetaAcc ma;
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             // a halo of one particle class (the common case in a pool of single-material scenes)
@@ -966,16 +1041,18 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_density_tiled(FluidArgs a,
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) density_global<UNIFORM>(i, ma, a, c);
             if (tid == 0) ++ma.runs;
         }
+        meta_to_wg(ma, s_meta);
         __syncthreads();
     }
-    flush_meta(ma, a);
+    flush_wg_meta(s_meta, a);
 }
 
 // WIDE = false: every block, the previous delta staged in LDS beside the records; blocks whose halo
 // only fits without it are skipped and left to the WIDE = true launch, which takes the listed wide
 // blocks and reads the previous delta from HBM through a staged slot map.
 template <bool UNIFORM, bool WIDE>
-__global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, SubstepConsts c)
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_delta_tiled(
+    FluidArgs a, SubstepConsts c)
 {
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
@@ -1034,25 +1111,29 @@ __global__ __launch_bounds__(kTileBlock) void k_fluid_delta_tiled(FluidArgs a, S
                 const float d2 = delta_particle<UNIFORM>(
                     i, fs,
                     [&](const DeltaSelf &x, DeltaSum &d) {
-                        // the listed neighbors, four per step: loads first, terms independent,
-                        // sums in list order
+                        // the listed neighbors, kTermStep per step: loads first (the next group's
+                        // entries while this group's terms are evaluated), terms independent, sums
+                        // in list order
                         const uint32_t cnt = a.lcnt[i];
-                        for (uint32_t k = 0; k < cnt; k += 4) {
-                            uint32_t e[4];
+                        uint32_t en[kTermStep];
+                        if (cnt > 0) list_group(li, 0, cnt, en);
+                        for (uint32_t k = 0; k < cnt; k += kTermStep) {
+                            uint32_t e[kTermStep];
 #pragma unroll
-                            for (uint32_t u = 0; u < 4; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
-                            float4 dq[4];
+                            for (uint32_t u = 0; u < kTermStep; ++u) e[u] = en[u];
+                            list_group(li, k + kTermStep, cnt, en);
+                            float4 dq[kTermStep];
 #pragma unroll
-                            for (uint32_t u = 0; u < 4; ++u)
+                            for (uint32_t u = 0; u < kTermStep; ++u)
                                 dq[u] = !x.need_rel ? zero : (WIDE ? a.din[lslot[e[u]]] : ldin[e[u]]);
-                            DeltaTerm t[4];
+                            DeltaTerm t[kTermStep];
 #pragma unroll
-                            for (uint32_t u = 0; u < 4; ++u)
+                            for (uint32_t u = 0; u < kTermStep; ++u)
                                 t[u] = bu ? delta_term<true>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero, 0u, a)
                                           : delta_term<false>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
                                                               lcls[e[u]], a);
 #pragma unroll
-                            for (uint32_t u = 0; u < 4; ++u)
+                            for (uint32_t u = 0; u < kTermStep; ++u)
                                 if (k + u < cnt) delta_add(d, t[u]);
                         }
                     },
@@ -1160,8 +1241,9 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
 
 static uint32_t grid_of(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
-// LDS budget of one tiled workgroup: the dynamic tile plus the static halo table stays under 64 KiB
-constexpr uint32_t kDynLdsBytes = 62u * 1024u;
+// LDS budget of one tiled workgroup: the dynamic tile plus the static halo table stays under a third
+// of the CU's 160 KiB, so three workgroups share a CU
+constexpr uint32_t kDynLdsBytes = 50u * 1024u;
 
 static bool tiled(const SubstepConsts &c, uint32_t it) { return c.key_mode == CN_KEY_MORTON && it < kIterStateWords; }
 
@@ -1171,14 +1253,14 @@ static uint32_t persistent_grid(const SubstepConsts &c)
     return std::min<uint32_t>(std::max<uint32_t>(c.grid_max, kXcds), std::max<uint32_t>(kXcds, (c.n + 127u) / 128u));
 }
 
-// Halo capacities: the density pass stages positions (and classes), the delta pass records,
-// previous deltas, normals and classes, the wide delta pass the same without previous deltas but
-// with a slot map.  density >= wide >= delta, so every block the delta passes stage was staged (and
+// Halo capacities: the density pass stages positions (and classes, kTilePad spare entries each),
+// the delta pass records, previous deltas, normals and classes, the wide delta pass the same
+// without previous deltas but with a slot map.  density >= wide >= delta, so every block the delta passes stage was staged (and
 // listed in halo offsets) by the density pass.
 void fluid_tiles(SubstepConsts &c)
 {
     const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
-    c.tile_dens = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (16u + cls));
+    c.tile_dens = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (16u + cls) - kTilePad);
     c.tile_delta = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (32u + nc + cls));
     c.tile_wide = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (20u + nc + cls));
 }
@@ -1196,7 +1278,7 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
     if (tiled(c, it)) {
         const uint32_t per = 16u + (c.uniform_class ? 0u : 4u);
         a.tile = c.tile_dens;
-        const size_t lds = (size_t)a.tile * per;
+        const size_t lds = (size_t)(a.tile + kTilePad) * per;
         const dim3 grid(persistent_grid(c)), block(kTileBlock);
         if (c.superset) {
             if (c.uniform_class)
