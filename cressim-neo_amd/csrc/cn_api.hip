@@ -1014,7 +1014,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         s.corr_off[2] = s.corr_off[1] + refs[1] + 1;
         const size_t ncorr = s.corr_off[2] + refs[2] + 1;
         CN_HIP(ctx, dalloc(&s.corr, ncorr));
-        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(float4)));
+        CN_HIP(ctx, hipMemset(s.corr, 0, ncorr * sizeof(CorrRec)));
         // constrained particles: any incidence in the three structures
         {
             std::vector<uint32_t> soft;

@@ -58,6 +58,12 @@ uint2 *radix_sort_pairs(uint2 *buf_a, uint2 *buf_b, uint32_t n, uint32_t key_bit
                         hipStream_t stream);
 
 // Device pointers of one resident scene (single scene or one slot pool).
+// One constraint's position correction for one of its vertices (12 bytes: the records are written
+// by the constraint solves and summed per vertex, so their size is most of the soft phase's traffic).
+struct CorrRec {
+    float x, y, z;
+};
+
 struct SceneBuffers {
     uint32_t n = 0;
     // particle state in slot order, double-buffered across the per-substep reorder
@@ -139,7 +145,7 @@ struct SceneBuffers {
     cn_soft_tet *tets = nullptr;
     uint32_t ne = 0, nb = 0, nd = 0, nt = 0;
     float *lam = nullptr;           // edges | bends + dihedrals | tets
-    float4 *corr = nullptr;         // correction records of edges | bends + dihedrals | tets, each group
+    CorrRec *corr = nullptr;        // correction records of edges | bends + dihedrals | tets, each group
                                     // in its incidence (CSR) order, then one spare record
     size_t corr_off[3] = {0, 0, 0};
     uint32_t *inc = nullptr;        // three CSR incidence structures back to back, then the record maps
@@ -178,19 +184,19 @@ struct SoftArgs {
     const cn_soft_edge *edges;
     uint32_t ne;
     float *lam_e;
-    float4 *corr_e;            // correction records in incidence order
+    CorrRec *corr_e;           // correction records in incidence order
     const uint32_t *map_e;     // 2 per edge: the position of its vertex records in corr_e
     const cn_soft_bend *bends;
     uint32_t nb;
     const cn_cloth_dihedral *dih;
     uint32_t nd;
     float *lam_b;              // bends then dihedrals
-    float4 *corr_b;
+    CorrRec *corr_b;
     const uint32_t *map_b;     // 4 per bend / dihedral (a bend's fourth is the spare record)
     const cn_soft_tet *tets;
     uint32_t nt;
     float *lam_t;
-    float4 *corr_t;
+    CorrRec *corr_t;
     const uint32_t *map_t;     // 4 per tet
     const uint32_t *inc_e_start;  // incidence ranges: a particle's records are corr_x[start[p], start[p + 1])
     const uint32_t *inc_b_start;

@@ -20,10 +20,12 @@ static inline uint32_t grid_n(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 // A constraint's correction records, written where its vertices' incidence lists read them.
 struct RecOut {
-    float4 *rec;
+    CorrRec *rec;
     const uint32_t *map;
-    __device__ float4 &operator[](uint32_t k) const { return rec[map[k]]; }
+    __device__ CorrRec &operator[](uint32_t k) const { return rec[map[k]]; }
 };
+
+__device__ __forceinline__ CorrRec rec3(float x, float y, float z) { return CorrRec{x, y, z}; }
 
 __device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.posi[particle]; }
 
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     const cn_soft_edge E = a.edges[e];
     if (!live(a, E.a)) return;
     const RecOut out{a.corr_e, a.map_e + 2 * (size_t)e};
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
     if (E.enabled == 0u) {
         out[0] = z4;
         out[1] = z4;
@@ -88,8 +90,8 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     const float dl = fdiv(-(C + alpha * lambda), den);
     a.lam_e[e] = lambda + dl;
     const float fa = wa * dl, fb = wb * dl;
-    out[0] = make_float4(fa * gax * a.relax, fa * gay * a.relax, fa * gaz * a.relax, 0.0f);
-    out[1] = make_float4(fb * gbx * a.relax, fb * gby * a.relax, fb * gbz * a.relax, 0.0f);
+    out[0] = rec3(fa * gax * a.relax, fa * gay * a.relax, fa * gaz * a.relax);
+    out[1] = rec3(fb * gbx * a.relax, fb * gby * a.relax, fb * gbz * a.relax);
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
     const cn_soft_bend B = a.bends[b];
     if (!live(a, B.p0)) return;
     const RecOut out{a.corr_b, a.map_b + 4 * (size_t)b};
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
     const float4 p0 = ldp(a, B.p0), p1 = ldp(a, B.p1), p2 = ldp(a, B.p2);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w;
     bool zero = w0 + w1 + w2 <= kEps;
@@ -143,9 +145,9 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
         return;
     }
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, r = a.relax;
-    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
-    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
-    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
+    out[0] = rec3(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r);
+    out[1] = rec3(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r);
+    out[2] = rec3(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r);
     out[3] = z4;
 }
 
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const cn_cloth_dihedral D = a.dih[d];
     if (!live(a, D.e0)) return;
     const RecOut out{a.corr_b, a.map_b + 4 * (size_t)ci};
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
     const float4 e0m = ldp(a, D.e0), e1m = ldp(a, D.e1), o0m = ldp(a, D.o0), o1m = ldp(a, D.o1);
     // p0, p1 opposite vertices; p2, p3 edge vertices
     const float ex = e1m.x - e0m.x, ey = e1m.y - e0m.y, ez = e1m.z - e0m.z;
@@ -204,10 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const float dl = fdiv(-(C + alpha * lambda), den + alpha);
     a.lam_b[ci] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
-    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
-    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
-    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
-    out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
+    out[0] = rec3(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r);
+    out[1] = rec3(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r);
+    out[2] = rec3(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r);
+    out[3] = rec3(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r);
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
@@ -244,16 +246,16 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     a.lam_t[t] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
     const RecOut out{a.corr_t, a.map_t + 4 * (size_t)t};
-    out[0] = make_float4(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r, 0.0f);
-    out[1] = make_float4(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r, 0.0f);
-    out[2] = make_float4(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r, 0.0f);
-    out[3] = make_float4(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r, 0.0f);
+    out[0] = rec3(f0 * g0x * r, f0 * g0y * r, f0 * g0z * r);
+    out[1] = rec3(f1 * g1x * r, f1 * g1y * r, f1 * g1z * r);
+    out[2] = rec3(f2 * g2x * r, f2 * g2y * r, f2 * g2z * r);
+    out[3] = rec3(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r);
 }
 
 // Per particle, in particle-index order: its incident records, stored contiguously in incidence
 // order, summed in that order, then added to its position.
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
-                                                              const float4 *__restrict__ rec)
+                                                              const CorrRec *__restrict__ rec)
 {
     const uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
     if (pid >= a.n) return;
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
     if (p.w <= kEps) return;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
     for (uint32_t k = k0; k < k1; k += 4) {
-        float4 cr[4];
+        CorrRec cr[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) cr[u] = rec[min(k + u, k1 - 1)];
 #pragma unroll
@@ -311,14 +313,14 @@ void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
     if (a.nsoft) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
     if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
-                       (const float4 *)a.corr_e);
+                       a.corr_e);
     if (a.nb) hipLaunchKernelGGL(k_solve_bends, dim3(grid_n(a.nb)), dim3(kBlock), 0, st, a);
     if (a.nd) hipLaunchKernelGGL(k_solve_dihedrals, dim3(grid_n(a.nd)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start,
-                       (const float4 *)a.corr_b);
+                       a.corr_b);
     if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
-                       (const float4 *)a.corr_t);
+                       a.corr_t);
     if (a.nsoft) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
 }
 
