@@ -970,11 +970,29 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         CN_HIP(ctx, dalloc(&s.bends, s.nb));
         CN_HIP(ctx, dalloc(&s.dihedrals, s.nd));
         CN_HIP(ctx, dalloc(&s.tets, s.nt));
-        if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, edges.data(), s.ne * sizeof(cn_soft_edge), hipMemcpyHostToDevice));
-        if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, bends.data(), s.nb * sizeof(cn_soft_bend), hipMemcpyHostToDevice));
-        if (s.nd)
-            CN_HIP(ctx, hipMemcpy(s.dihedrals, dih.data(), s.nd * sizeof(cn_cloth_dihedral), hipMemcpyHostToDevice));
-        if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, tets.data(), s.nt * sizeof(cn_soft_tet), hipMemcpyHostToDevice));
+        {
+            // device forms: padding dropped, max(compliance, 0) and max(rest volume, 0) taken here,
+            // a disabled edge carried as a NaN compliance
+            auto clamp0 = [](float v) { return v > 0.0f ? v : 0.0f; };
+            std::vector<EdgeDev> de(s.ne);
+            for (uint32_t e = 0; e < s.ne; ++e)
+                de[e] = {edges[e].a, edges[e].b, edges[e].rest_length,
+                         edges[e].enabled ? clamp0(edges[e].compliance) : std::nanf("")};
+            std::vector<BendDev> db(s.nb);
+            for (uint32_t q = 0; q < s.nb; ++q)
+                db[q] = {bends[q].p0, bends[q].p1, bends[q].p2, bends[q].rest_angle, clamp0(bends[q].compliance)};
+            std::vector<DihedralDev> dd(s.nd);
+            for (uint32_t q = 0; q < s.nd; ++q)
+                dd[q] = {dih[q].e0, dih[q].e1, dih[q].o0, dih[q].o1, dih[q].rest_angle, clamp0(dih[q].compliance)};
+            std::vector<TetDev> dt(s.nt);
+            for (uint32_t t = 0; t < s.nt; ++t)
+                dt[t] = {{tets[t].p[0], tets[t].p[1], tets[t].p[2], tets[t].p[3]}, clamp0(tets[t].rest_volume),
+                         clamp0(tets[t].compliance)};
+            if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, de.data(), s.ne * sizeof(EdgeDev), hipMemcpyHostToDevice));
+            if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, db.data(), s.nb * sizeof(BendDev), hipMemcpyHostToDevice));
+            if (s.nd) CN_HIP(ctx, hipMemcpy(s.dihedrals, dd.data(), s.nd * sizeof(DihedralDev), hipMemcpyHostToDevice));
+            if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, dt.data(), s.nt * sizeof(TetDev), hipMemcpyHostToDevice));
+        }
         const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
         CN_HIP(ctx, dalloc(&s.lam, nlam));
         std::vector<uint32_t> inc, map_e, map_b, map_t;

@@ -58,6 +58,27 @@ uint2 *radix_sort_pairs(uint2 *buf_a, uint2 *buf_b, uint32_t n, uint32_t key_bit
                         hipStream_t stream);
 
 // Device pointers of one resident scene (single scene or one slot pool).
+// Device forms of the soft constraints: the caller's records (cn_soft_edge, cn_soft_bend,
+// cn_cloth_dihedral, cn_soft_tet) without their padding, with the compliance and the tet rest volume
+// clamped at zero on upload (the solves use only max(value, 0)) and a disabled edge marked by a NaN
+// compliance, so a solve reads 16-24 bytes per constraint instead of 32.
+struct EdgeDev {
+    uint32_t a, b;
+    float rest_length, compliance;
+};
+struct BendDev {
+    uint32_t p0, p1, p2;
+    float rest_angle, compliance;
+};
+struct DihedralDev {
+    uint32_t e0, e1, o0, o1;
+    float rest_angle, compliance;
+};
+struct TetDev {
+    uint32_t p[4];
+    float rest_volume, compliance;
+};
+
 // One constraint's position correction for one of its vertices (12 bytes: the records are written
 // by the constraint solves and summed per vertex, so their size is most of the soft phase's traffic).
 struct CorrRec {
@@ -139,10 +160,10 @@ struct SceneBuffers {
                                     // during the soft phase of an iteration
     uint32_t *soft_ids = nullptr;   // the constrained particles, ascending
     uint32_t nsoft = 0;
-    cn_soft_edge *edges = nullptr;
-    cn_soft_bend *bends = nullptr;
-    cn_cloth_dihedral *dihedrals = nullptr;
-    cn_soft_tet *tets = nullptr;
+    EdgeDev *edges = nullptr;
+    BendDev *bends = nullptr;
+    DihedralDev *dihedrals = nullptr;
+    TetDev *tets = nullptr;
     uint32_t ne = 0, nb = 0, nd = 0, nt = 0;
     float *lam = nullptr;           // edges | bends + dihedrals | tets
     CorrRec *corr = nullptr;        // correction records of edges | bends + dihedrals | tets, each group
@@ -181,19 +202,19 @@ struct SoftArgs {
     const uint32_t *id;        // slot -> particle index
     uint32_t n;
     float dt, relax;
-    const cn_soft_edge *edges;
+    const EdgeDev *edges;
     uint32_t ne;
     float *lam_e;
     CorrRec *corr_e;           // correction records in incidence order
     const uint32_t *map_e;     // 2 per edge: the position of its vertex records in corr_e
-    const cn_soft_bend *bends;
+    const BendDev *bends;
     uint32_t nb;
-    const cn_cloth_dihedral *dih;
+    const DihedralDev *dih;
     uint32_t nd;
     float *lam_b;              // bends then dihedrals
     CorrRec *corr_b;
     const uint32_t *map_b;     // 4 per bend / dihedral (a bend's fourth is the spare record)
-    const cn_soft_tet *tets;
+    const TetDev *tets;
     uint32_t nt;
     float *lam_t;
     CorrRec *corr_t;

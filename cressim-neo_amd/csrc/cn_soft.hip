@@ -52,11 +52,11 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 {
     const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= a.ne) return;
-    const cn_soft_edge E = a.edges[e];
+    const EdgeDev E = a.edges[e];
     if (!live(a, E.a)) return;
     const RecOut out{a.corr_e, a.map_e + 2 * (size_t)e};
     const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
-    if (E.enabled == 0u) {
+    if (E.compliance != E.compliance) {  // disabled
         out[0] = z4;
         out[1] = z4;
         return;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     const float len = fsqrt(l2);
     const float gax = fdiv(-dx, len), gay = fdiv(-dy, len), gaz = fdiv(-dz, len);
     const float gbx = -gax, gby = -gay, gbz = -gaz;
-    const float alpha = fdiv(fmaxd(E.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    const float alpha = fdiv(E.compliance, fmaxd(a.dt * a.dt, kEps));
     const float C = len - E.rest_length;
     const float lambda = a.lam_e[e];
     const float den = wsum + alpha;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= a.nb) return;
-    const cn_soft_bend B = a.bends[b];
+    const BendDev B = a.bends[b];
     if (!live(a, B.p0)) return;
     const RecOut out{a.corr_b, a.map_b + 4 * (size_t)b};
     const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
             g1z = -(g0z + g2z);
             const float den = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
                               w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z);
-            const float alpha = fdiv(fmaxd(B.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+            const float alpha = fdiv(B.compliance, fmaxd(a.dt * a.dt, kEps));
             zero = den + alpha <= kEps;
             if (!zero) {
                 const float C = theta - B.rest_angle;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
     if (d >= a.nd) return;
     const uint32_t ci = a.nb + d;
-    const cn_cloth_dihedral D = a.dih[d];
+    const DihedralDev D = a.dih[d];
     if (!live(a, D.e0)) return;
     const RecOut out{a.corr_b, a.map_b + 4 * (size_t)ci};
     const CorrRec z4 = rec3(0.0f, 0.0f, 0.0f);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const float w0 = e0m.w, w1 = e1m.w, w2 = o0m.w, w3 = o1m.w;
     const float den = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
                       w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
-    const float alpha = fdiv(fmaxd(D.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    const float alpha = fdiv(D.compliance, fmaxd(a.dt * a.dt, kEps));
     if (den + alpha <= kEps) {
         out[0] = out[1] = out[2] = out[3] = z4;
         return;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.nt) return;
-    const cn_soft_tet T = a.tets[t];
+    const TetDev T = a.tets[t];
     if (!live(a, T.p[0])) return;
     const float4 p0 = ldp(a, T.p[0]), p1 = ldp(a, T.p[1]), p2 = ldp(a, T.p[2]), p3 = ldp(a, T.p[3]);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     cross3d(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, cx, cy, cz);
     const float sv = fdiv(dot3(cx, cy, cz, p3.x - p0.x, p3.y - p0.y, p3.z - p0.z), 6.0f);
     const float sgn = sv >= 0.0f ? 1.0f : -1.0f;
-    const float C = fabsf(sv) - fmaxd(T.rest_volume, 0.0f);
+    const float C = fabsf(sv) - T.rest_volume;
     const float f = fdiv(sgn, 6.0f);
     float g0x, g0y, g0z, g1x, g1y, g1z, g2x, g2y, g2z, g3x, g3y, g3z;
     cross3d(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z, p2.x - p1.x, p2.y - p1.y, p2.z - p1.z, g0x, g0y, g0z);
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     g3x = g3x * f; g3y = g3y * f; g3z = g3z * f;
     const float sg = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
                      w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
-    const float alpha = fdiv(fmaxd(T.compliance, 0.0f), fmaxd(a.dt * a.dt, kEps));
+    const float alpha = fdiv(T.compliance, fmaxd(a.dt * a.dt, kEps));
     const float lambda = a.lam_t[t];
     const float den = sg + alpha;
     if (den <= kEps) return;
