@@ -105,11 +105,28 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
 // neither the remainder nor the quotient underflows: a == 0, or |a| >= 2^-96 with
 // 2^-100 <= |a / b| < 2^100.  The last step is taken negated so that a zero dividend gives the
 // signed zero a / b gives.  Three VALU operations against the ten of the scaled IEEE sequence.
-__device__ __forceinline__ float fdiv_rcp(float a, float b, float y)
+__device__ __forceinline__ float fdiv_rcp_neg(float a, float b, float y)
 {
     const float q0 = __fmul_rn(a, y);
     const float r = __builtin_fmaf(-q0, b, a);
-    return -__builtin_fmaf(-r, y, -q0);
+    return __builtin_fmaf(-r, y, -q0);  // -(a / b)
+}
+__device__ __forceinline__ float fdiv_rcp(float a, float b, float y) { return -fdiv_rcp_neg(a, b, y); }
+
+// 1 / b, correctly rounded, for a normal b > 0: the hardware estimate (within one ulp) refined by
+// two Newton steps in fma, which reach the correctly rounded reciprocal from any estimate within
+// two ulps except for the significand 1.11...1 (checked exhaustively over significands); that one
+// takes the IEEE division.
+__device__ __forceinline__ float frcp_exact(float b)
+{
+    float y = __builtin_amdgcn_rcpf(b);
+    y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    if ((__float_as_uint(b) & 0x7fffffu) == 0x7fffffu) {
+        __asm__ volatile("");  // a side effect keeps this rare path a branch rather than a select
+        y = fdiv(1.0f, b);
+    }
+    return y;
 }
 
 // Square root by the hardware estimate (within one ulp) and the rounding test on its two
@@ -203,10 +220,9 @@ __device__ __forceinline__ float cohesion_kernel(float d, float h, float c1, flo
 
 // The same kernels from q = d / h supplied by the caller (the per-neighbor fluid terms share
 // one quotient between the kernels of a pair).
-__device__ __forceinline__ float spiky_w_q(float d, float h, float q, float k)
+__device__ __forceinline__ float spiky_w_om(float d, float h, float omq, float k)  // omq = 1 - d / h
 {
     if (d >= h) return 0.0f;
-    const float omq = __fsub_rn(1.0f, q);
     return __fmul_rn(__fmul_rn(k, omq), omq);
 }
 __device__ __forceinline__ float cohesion_q(float d, float h, float q, float c1, float c2)

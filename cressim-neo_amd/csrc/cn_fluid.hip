@@ -211,13 +211,15 @@ struct DensitySum {
     uint32_t inr = 0, req = 0;
 };
 
-// Quotients of one neighbor pair: n = (dx, dy, dz) / dist, q = dist / h and the spiky
+// Quotients of one neighbor pair, held negated where that saves the final negation of
+// fdiv_rcp: nn = -(dx, dy, dz) / dist, nq = -(dist / h), omq = 1 - dist / h (as 1 + nq) and the spiky
 // derivative der = (-2 k (1 - q)) / h (FluidSpikyKernelDerivative, physics_fluid_common.hlsli:31-42;
-// 0 unless eps < dist < h).  FAST divides through reciprocals (fdiv_rcp: 1 / dist once per pair,
-// 1 / h once per particle); quot_fast_ok says when that is exact, and the other particles take
-// the IEEE divisions.
+// 0 unless eps < dist < h), computed as -((2 k omq) / h).  The consumers take -(a b) = (-a) b and
+// x - (-y) = x + y, which are exact, so every result has the bits of the plain forms.  FAST divides
+// through reciprocals (fdiv_rcp: 1 / dist once per pair, 1 / h once per particle); quot_fast_ok
+// says when that is exact, and the other particles take the IEEE divisions.
 struct PairQuot {
-    float nx, ny, nz, q, der;
+    float nnx, nny, nnz, nq, omq, der;  // nq = -(dist / h)
 };
 
 // fdiv_rcp is exact for every quotient of pair_quot when h <= 1024 and each coordinate of the
@@ -235,20 +237,22 @@ template <bool FAST>
 __device__ __forceinline__ void pair_quot(PairQuot &p, float dx, float dy, float dz, float dist, float h, float ih,
                                           float k)
 {
+    float &nq = p.nq;
     if (FAST) {
-        const float y = fdiv(1.0f, dist);
-        p.nx = fdiv_rcp(dx, dist, y);
-        p.ny = fdiv_rcp(dy, dist, y);
-        p.nz = fdiv_rcp(dz, dist, y);
-        p.q = fdiv_rcp(dist, h, ih);
+        const float y = frcp_exact(dist);
+        p.nnx = fdiv_rcp_neg(dx, dist, y);
+        p.nny = fdiv_rcp_neg(dy, dist, y);
+        p.nnz = fdiv_rcp_neg(dz, dist, y);
+        nq = fdiv_rcp_neg(dist, h, ih);
     } else {
-        p.nx = fdiv(dx, dist);
-        p.ny = fdiv(dy, dist);
-        p.nz = fdiv(dz, dist);
-        p.q = fdiv(dist, h);
+        p.nnx = fdiv(-dx, dist);
+        p.nny = fdiv(-dy, dist);
+        p.nnz = fdiv(-dz, dist);
+        nq = -fdiv(dist, h);
     }
-    const float num = __fmul_rn(__fmul_rn(-2.0f, k), __fsub_rn(1.0f, p.q));
-    const float der = FAST ? fdiv_rcp(num, h, ih) : fdiv(num, h);
+    p.omq = __fadd_rn(1.0f, nq);
+    const float num = __fmul_rn(__fmul_rn(2.0f, k), p.omq);
+    const float der = FAST ? fdiv_rcp_neg(num, h, ih) : -fdiv(num, h);
     p.der = (dist <= kEps || dist >= h) ? 0.0f : der;
 }
 
@@ -311,9 +315,9 @@ __device__ __forceinline__ bool density_test(const DensitySelf &x, const float4 
     return dot3(dx, dy, dz, dx, dy, dz) < x.t2;
 }
 
-// Density and surface-normal terms of one listed neighbor.
+// Density and (negated) surface-normal terms of one listed neighbor.
 struct DensityTerm {
-    float w, gx, gy, gz;
+    float w, ngx, ngy, ngz;
 };
 
 __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const float4 pj)
@@ -321,18 +325,18 @@ __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const 
     DensityTerm t;
     const float dx = x.s.x - pj.x, dy = x.s.y - pj.y, dz = x.s.z - pj.z;
     const float dist = fsqrt_nearzero_free(dot3(dx, dy, dz, dx, dy, dz));
-    float q = fdiv_rcp(dist, x.h, x.ih);
-    if (!x.fd) q = fdiv(dist, x.h);
-    t.w = spiky_w_q(dist, x.h, q, x.kw);
-    t.gx = t.gy = t.gz = 0.0f;
+    float omq = __fadd_rn(1.0f, fdiv_rcp_neg(dist, x.h, x.ih));  // 1 - dist / h
+    if (!x.fd) omq = __fsub_rn(1.0f, fdiv(dist, x.h));
+    t.w = spiky_w_om(dist, x.h, omq, x.kw);
+    t.ngx = t.ngy = t.ngz = 0.0f;
     if (x.tension && !(dist <= kEps || dist >= x.h)) {
-        // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53
+        // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53, negated: der * -(d / dist)
         PairQuot p;
         pair_quot<true>(p, dx, dy, dz, dist, x.h, x.ih, x.kw);
         if (!x.fd) pair_quot<false>(p, dx, dy, dz, dist, x.h, x.ih, x.kw);
-        t.gx = p.der * p.nx;
-        t.gy = p.der * p.ny;
-        t.gz = p.der * p.nz;
+        t.ngx = p.der * p.nnx;
+        t.ngy = p.der * p.nny;
+        t.ngz = p.der * p.nnz;
     }
     return t;
 }
@@ -341,9 +345,9 @@ __device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x,
 {
     d.density = d.density + t.w;
     if (x.tension) {
-        d.snx = d.snx + t.gx;
-        d.sny = d.sny + t.gy;
-        d.snz = d.snz + t.gz;
+        d.snx = d.snx - t.ngx;
+        d.sny = d.sny - t.ngy;
+        d.snz = d.snz - t.ngz;
     }
 }
 
@@ -429,9 +433,9 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
     PairQuot p;
     pair_quot<true>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
     if (!x.fd) pair_quot<false>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
-    const float nx = p.nx, ny = p.ny, nz = p.nz, der = p.der;
-    const float sc = -0.5f * (x.Ci + cq.w) * der;
-    float pdx = sc * nx, pdy = sc * ny, pdz = sc * nz;
+    // sc * n with sc = -0.5 (Ci + Cj) der, as (0.5 (Ci + Cj) der) * -n
+    const float msc = 0.5f * (x.Ci + cq.w) * p.der;
+    float pdx = msc * p.nnx, pdy = msc * p.nny, pdz = msc * p.nnz;
     const bool same = UNIFORM || x.pc.fluid_mat == a.classes[qcls].fluid_mat;
     if (same) {
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
@@ -441,24 +445,25 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
             rz = x.di.z - dq.z;
         }
         if (x.cohesion > kEps) {
-            const float ck = x.cohesion * cohesion_q(dist, x.h, p.q, x.m.cohesion1, x.m.cohesion2);
-            pdx = pdx - nx * ck;
-            pdy = pdy - ny * ck;
-            pdz = pdz - nz * ck;
+            const float ck = x.cohesion * cohesion_q(dist, x.h, -p.nq, x.m.cohesion1, x.m.cohesion2);
+            pdx = pdx + p.nnx * ck;
+            pdy = pdy + p.nny * ck;
+            pdz = pdz + p.nnz * ck;
         }
         if (x.visc_on) {
-            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_q(dist, x.h, p.q, x.kw));
+            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_om(dist, x.h, p.omq, x.kw));
             pdx = pdx - vs * rx;
             pdy = pdy - vs * ry;
             pdz = pdz - vs * rz;
         }
         if (x.cfl_on) {
-            const float proj = dot3(rx, ry, rz, nx, ny, nz);
-            if (proj < -x.cfl) {
-                const float pcf = proj + x.cfl;
-                pdx = pdx - nx * pcf * 0.5f;
-                pdy = pdy - ny * pcf * 0.5f;
-                pdz = pdz - nz * pcf * 0.5f;
+            // proj = r . n = -(r . nn); proj < -cfl, pcf = proj + cfl = cfl - (r . nn)
+            const float nproj = dot3(rx, ry, rz, p.nnx, p.nny, p.nnz);
+            if (nproj > x.cfl) {
+                const float pcf = x.cfl - nproj;
+                pdx = pdx + p.nnx * pcf * 0.5f;
+                pdy = pdy + p.nny * pcf * 0.5f;
+                pdz = pdz + p.nnz * pcf * 0.5f;
             }
         }
         if (x.tension) {

@@ -13,6 +13,10 @@
  *    Checked for every estimate in {RN - ulp, RN, RN + ulp} of RN(sqrt(x)) over every
  *    significand of x at exponents 2^-96 .. 2^100, and at x == 0.
  *
+ * 3. Reciprocal refinement: for a normal b whose significand is not all ones and any estimate y0
+ *    within two ulps of RN(1/b), two Newton steps in fma, y <- fma(fma(-b, y, 1), y, y), give
+ *    RN(1/b).  Checked for every significand of b at exponents -20, 0 and 20 and every such y0.
+ *
  * Prints "ok <checks>" and exits 0, or prints the first mismatches and exits 1. */
 #include <math.h>
 #include <stdint.h>
@@ -36,6 +40,12 @@ static float sqrt_sel(float x, float s)
     const float rm = fmaf(-sm, s, x), rp = fmaf(-sp, s, x);
     const float t = rm <= 0.0f ? sm : s;
     return rp > 0.0f ? sp : t;
+}
+
+static float rcp_refine(float b, float y)
+{
+    y = fmaf(fmaf(-b, y, 1.0f), y, y);
+    return fmaf(fmaf(-b, y, 1.0f), y, y);
 }
 
 static long check_divisor(float b)
@@ -93,6 +103,21 @@ int main(int argc, char **argv)
     if (fb(sqrt_sel(0.0f, 0.0f)) != fb(0.0f) || fb(sqrt_sel(0.0f, bf(1u))) != fb(0.0f)) ++sbad;
     if (sbad) printf("sqrt selection: %ld mismatches\n", sbad);
     bad += sbad;
+    long rbad = 0;
+    static const int re[3] = {-20, 0, 20};
+#pragma omp parallel for reduction(+ : rbad)
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        if (m == 0x7fffffu) continue;
+        for (int t = 0; t < 3; ++t) {
+            const float b = bf(((uint32_t)(re[t] + 127) << 23) | m);
+            const float rn = 1.0f / b;
+            for (int d = -2; d <= 2; ++d)
+                if (fb(rcp_refine(b, bf(fb(rn) + (uint32_t)d))) != fb(rn)) ++rbad;
+        }
+    }
+    checks += 15l * ((1l << 23) - 1);
+    if (rbad) printf("reciprocal refinement: %ld mismatches\n", rbad);
+    bad += rbad;
     if (bad) return 1;
     printf("ok %ld\n", checks);
     return 0;

@@ -1,8 +1,10 @@
-"""The rounding identities behind the fluid neighbor terms' reciprocal divisions and square-root
-selection (cn_device.h fdiv_rcp / fsqrt_nearzero_free), checked on the CPU with the same binary32
-operations: exhaustive over dividend significands for the smoothing radii the tests and the bench
-use plus pseudo-random divisors, and over every significand of the square-root argument in
-[2^-96, 2^101).  The GPU parity tests then compare whole substeps bit for bit with the oracle."""
+"""The rounding identities behind the fluid neighbor terms' reciprocal divisions, reciprocal
+refinement and square-root selection (cn_device.h fdiv_rcp / frcp_exact / fsqrt_nearzero_free),
+checked on the CPU with the same binary32 operations: exhaustive over dividend significands for the
+smoothing radii the tests and the bench use plus pseudo-random divisors, over every divisor
+significand for the refined reciprocal from any estimate within two ulps, and over every
+significand of the square-root argument in [2^-96, 2^101).  The GPU parity tests then compare
+whole substeps bit for bit with the oracle."""
 import os
 import shutil
 import subprocess
