@@ -1320,7 +1320,7 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
         if (it > 0 && c.tile_wide > c.tile_delta) {
             a.tile = c.tile_wide;
             lds = (size_t)a.tile * (20u + nc + cls);
-            const dim3 wgrid(std::min<uint32_t>(256u, persistent_grid(c)));
+            const dim3 wgrid(persistent_grid(c));
             if (c.uniform_class)
                 hipLaunchKernelGGL((k_fluid_delta_tiled<true, true>), wgrid, block, lds, st, a, c);
             else
