@@ -258,7 +258,6 @@ __device__ __forceinline__ void pair_quot(PairQuot &p, float dx, float dy, float
 
 struct DensitySelf {
     ParticleClass pc;
-    cn_fluid_material m;
     float4 s;
     float h, t2, kw, ih;  // ih = 1 / h
     bool tension;
@@ -280,13 +279,13 @@ __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const 
         a.cs[i] = make_float4(kFarAway, kFarAway, kFarAway, 0.0f);
         return false;
     }
-    x.m = a.fm[x.pc.fluid_mat];
-    x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
+    const cn_fluid_material &m = a.fm[x.pc.fluid_mat];
+    x.h = fmaxd(m.smoothing_radius, 1.0e-4f);
     x.t2 = a.t2[x.pc.fluid_mat];
     x.kw = spiky_coef(x.h);
     x.ih = fdiv(1.0f, x.h);
     x.fd = quot_fast_ok(x.h, s.x, s.y, s.z);
-    x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
+    x.tension = fmaxd(m.surface_tension, 0.0f) > kEps;
     if (dprev >= 0.0f && a.scenes[x.pc.scene].fluid_only) {
         // a pair in radius at iteration k was within h + 2 max|delta_k| at iteration 0, so the
         // candidates within h + skin hold every later list while 2 max|delta_k| < 0.98 skin (the
@@ -367,12 +366,14 @@ __device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, con
         if (gd >= x.h) continue;
         density = density + spiky_w_k(gd, x.h, x.kw) * bscale;
     }
-    const float rho0 = fmaxd(x.m.rest_density, 1.0f);
+    // the material is read again here rather than held in registers across the walk
+    const cn_fluid_material &m = a.fm[x.pc.fluid_mat];
+    const float rho0 = fmaxd(m.rest_density, 1.0f);
     const float C =
-        fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) * x.m.density_constraint_scale;
+        fmaxd(density - rho0, -rho0 * fmaxd(c.cfg[kCfgFluidMaxUnder], 0.0f)) * m.density_constraint_scale;
     a.cs[i] = make_float4(x.s.x, x.s.y, x.s.z, C);
     if (x.tension) {
-        const float st = fmaxd(x.m.surface_tension, 0.0f);
+        const float st = fmaxd(m.surface_tension, 0.0f);
         a.nc[i] = make_float4(d.snx * st, d.sny * st, d.snz * st, C);
     }
     ma.req = max(ma.req, d.req);
@@ -410,7 +411,6 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
 // ---------------------------------------------------------------------------------------------
 struct DeltaSelf {
     ParticleClass pc;
-    cn_fluid_material m;
     float4 di, nci;
     float sx, sy, sz, Ci, h, t2, kw, ih, cohesion, viscosity, cfl, dt;
     bool tension, visc_on, cfl_on, need_rel, fd;
@@ -445,7 +445,9 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
             rz = x.di.z - dq.z;
         }
         if (x.cohesion > kEps) {
-            const float ck = x.cohesion * cohesion_q(dist, x.h, -p.nq, x.m.cohesion1, x.m.cohesion2);
+            // the cohesion coefficients are read here (a rare path) rather than held across the loop
+            const cn_fluid_material &m = a.fm[x.pc.fluid_mat];
+            const float ck = x.cohesion * cohesion_q(dist, x.h, -p.nq, m.cohesion1, m.cohesion2);
             pdx = pdx + p.nnx * ck;
             pdy = pdy + p.nny * ck;
             pdz = pdz + p.nnz * ck;
@@ -515,17 +517,17 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         x.sy = csi.y;
         x.sz = csi.z;
         x.Ci = csi.w;
-        x.m = a.fm[pc.fluid_mat];
-        x.h = fmaxd(x.m.smoothing_radius, 1.0e-4f);
+        const cn_fluid_material &m = a.fm[pc.fluid_mat];
+        x.h = fmaxd(m.smoothing_radius, 1.0e-4f);
         x.t2 = a.t2[pc.fluid_mat];
         x.kw = spiky_coef(x.h);
         x.ih = fdiv(1.0f, x.h);
         x.fd = quot_fast_ok(x.h, x.sx, x.sy, x.sz);
         x.dt = c.dt;
-        x.cohesion = c.dt * fmaxd(x.m.cohesion, 0.0f);
-        x.viscosity = c.dt * fmaxd(x.m.viscosity, 0.0f);
-        x.cfl = fmaxd(x.m.cfl_radius, 0.0f);
-        x.tension = fmaxd(x.m.surface_tension, 0.0f) > kEps;
+        x.cohesion = c.dt * fmaxd(m.cohesion, 0.0f);
+        x.viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
+        x.cfl = fmaxd(m.cfl_radius, 0.0f);
+        x.tension = fmaxd(m.surface_tension, 0.0f) > kEps;
         // relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can only
         // act when |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl
         x.cfl_on = false;
