@@ -1169,8 +1169,11 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
                 din = 1 - din;
             }
             if (has_soft && it < soft_iters) {
+                // Without contact passes nothing but the soft solves moves a constrained particle
+                // within the substep (the fluid passes list fluid particles only), so the positions
+                // stay in particle-index order from the first soft iteration to the last.
                 ProfScope ps(ctx, "soft");
-                launch_soft_iteration(sa, ctx->stream);
+                launch_soft_iteration(sa, it == 0 || has_contacts, it + 1 == soft_iters || has_contacts, ctx->stream);
             }
             if (has_contacts && it < contact_iters) {
                 ProfScope ps(ctx, "contacts");

@@ -226,7 +226,9 @@ struct SoftArgs {
     const uint32_t *pid_scene; // particle index -> scene
 };
 void launch_soft_clear(const SoftArgs &a, hipStream_t st);
-void launch_soft_iteration(const SoftArgs &a, hipStream_t st);
+// gather / scatter: copy the constrained particles' positions from slot order before the solves and
+// back after them (needed only around iterations where another pass moves those particles)
+void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, hipStream_t st);
 
 // per-pass entries in the reference's data order (cn_passes.hip)
 void launch_ref_keys(const float4 *pos, uint32_t n, float inv_cell, uint32_t *keys, int32_t *cells,

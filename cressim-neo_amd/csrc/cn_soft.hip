@@ -307,10 +307,10 @@ void launch_soft_clear(const SoftArgs &a, hipStream_t st)
     if (m) hipLaunchKernelGGL(k_soft_clear, dim3(grid_n(m)), dim3(kBlock), 0, st, a);
 }
 
-void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
+void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, hipStream_t st)
 {
     // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
-    if (a.nsoft) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
+    if (a.nsoft && gather) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
     if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
                        a.corr_e);
@@ -321,7 +321,7 @@ void launch_soft_iteration(const SoftArgs &a, hipStream_t st)
     if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
                        a.corr_t);
-    if (a.nsoft) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
+    if (a.nsoft && scatter) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
 }
 
 }  // namespace cn
