@@ -575,6 +575,8 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
 {
     if (!ctx) return CN_ERR_INVALID_ARGUMENT;
     if (n <= ctx->capacity && cap <= ctx->cap_alloc && ctx->s.pos[0]) return CN_OK;
+    if ((uint64_t)list_slots(n ? n : 1) * (cap ? cap : 1) * sizeof(uint16_t) > kListBytesMax)
+        return fail(ctx, CN_ERR_CAPACITY, "neighbor lists of n x max_fluid_neighbors entries exceed 4 GiB");
     CN_HIP(ctx, hipSetDevice(ctx->device));
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_scene(ctx->s);
@@ -1246,6 +1248,8 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
         return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
     CN_HIP(ctx, hipSetDevice(ctx->device));
     if ((p->flags & CN_STEP_CANDIDATE_REUSE) && !ctx->s.slist) {  // candidate lists on first use
+        if ((uint64_t)list_slots(ctx->capacity) * kSuperSlots * sizeof(uint16_t) > kListBytesMax)
+            return fail(ctx, CN_ERR_CAPACITY, "candidate lists of this many particles exceed 4 GiB");
         CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
         CN_HIP(ctx, dalloc(&ctx->s.slist, list_slots(ctx->capacity) * kSuperSlots));
         CN_HIP(ctx, dalloc(&ctx->s.scnt, std::max<uint32_t>(ctx->capacity, 1u)));

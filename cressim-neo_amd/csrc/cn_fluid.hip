@@ -813,15 +813,31 @@ __device__ __forceinline__ RowCull row_cull(const float4 s, const int4 ci, float
 __device__ __forceinline__ float gap_below(float u, int c) { return fmaxd(u - (float)c, 0.0f); }
 __device__ __forceinline__ float gap_above(float u, int c) { return fmaxd((float)(c + 1) - u, 0.0f); }
 
-// Appends listed halo offsets to a particle's list: p is the next entry (lists are grouped, one
-// entry every kListGroup halfwords), n the untruncated count.
+// A particle's list in a grouped list buffer: the buffer (a kernel argument, so a scalar base) and
+// the byte offset of entry 0; entry k sits 2 * kListGroup * k bytes further.  The offsets are
+// 32-bit (list buffers stay below 4 GiB), so an entry's address is one 32-bit add from the base.
+struct ListRef {
+    char *base;
+    uint32_t off;
+    __device__ __forceinline__ uint16_t &at(uint32_t k) const
+    {
+        return *reinterpret_cast<uint16_t *>(base + (off + k * (2u * kListGroup)));
+    }
+};
+
+__device__ __forceinline__ ListRef list_ref(uint16_t *buf, uint32_t i, uint32_t rows)
+{
+    return ListRef{reinterpret_cast<char *>(buf), (uint32_t)(list_group_base(i, rows) * sizeof(uint16_t))};
+}
+
+// Appends listed halo offsets to a particle's list; n is the untruncated count.
 struct ListOut {
-    uint16_t *p;
+    ListRef r;
     uint32_t n;
     __device__ __forceinline__ void push(uint32_t f, uint32_t cap)
     {
-        if (n < cap) *p = (uint16_t)f;
-        p += kListGroup;
+        if (n < cap) *reinterpret_cast<uint16_t *>(r.base + r.off) = (uint16_t)f;
+        r.off += 2u * kListGroup;
         ++n;
     }
 };
@@ -864,9 +880,9 @@ __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4
 
 template <bool UNIFORM, bool SUP>
 __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
-                                              const uint32_t *lcls, const DensitySelf &x, uint16_t *li,
+                                              const uint32_t *lcls, const DensitySelf &x, const ListRef &li,
                                               const FluidArgs &a, const SubstepConsts &c, const RowCull &rc,
-                                              uint16_t *ls = nullptr, uint32_t *scount = nullptr)
+                                              const ListRef &ls = ListRef{nullptr, 0u}, uint32_t *scount = nullptr)
 {
     ListOut lo{li, 0u}, so{ls, 0u};
     const float gxl = gap_below(rc.ux, rc.cx), gxr = gap_above(rc.ux, rc.cx);
@@ -896,14 +912,14 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
 // A later iteration's list from the iteration-0 candidates: the same walk order, the same in-radius
 // test at the current positions, so the same list while every neighbor is still a candidate
 // (guaranteed while no fluid particle has moved more than half the skin since iteration 0).
-__device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scount, const float4 *lpos,
-                                               const DensitySelf &x, uint16_t *li, const SubstepConsts &c)
+__device__ __forceinline__ uint32_t scan_super(const ListRef &ls, uint32_t scount, const float4 *lpos,
+                                               const DensitySelf &x, const ListRef &li, const SubstepConsts &c)
 {
     uint32_t req = 0;
     for (uint32_t k = 0; k < scount; k += 4) {
         uint32_t e[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = ls[min(k + u, scount - 1) * kListGroup];
+        for (uint32_t u = 0; u < 4; ++u) e[u] = ls.at(min(k + u, scount - 1));
         float4 pj[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) pj[u] = lpos[e[u]];
@@ -911,7 +927,7 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
         for (uint32_t u = 0; u < 4; ++u) {
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
-                if (req < c.cap) li[req * kListGroup] = (uint16_t)e[u];
+                if (req < c.cap) li.at(req) = (uint16_t)e[u];
                 ++req;
             }
         }
@@ -921,13 +937,13 @@ __device__ __forceinline__ uint32_t scan_super(const uint16_t *ls, uint32_t scou
 
 // Density sums over a compacted list, kTermStep entries per step (loads first, sums in list order).
 // List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past the end).
-__device__ __forceinline__ void list_group(const uint16_t *li, uint32_t k, uint32_t cnt, uint32_t (&e)[kTermStep])
+__device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32_t cnt, uint32_t (&e)[kTermStep])
 {
 #pragma unroll
-    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li[min(k + u, cnt - 1) * kListGroup];
+    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li.at(min(k + u, cnt - 1));
 }
 
-__device__ __forceinline__ void density_list(const uint16_t *li, uint32_t cnt, const float4 *lpos,
+__device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, const float4 *lpos,
                                              const DensitySelf &x, DensitySum &d)
 {
     // the next group's entries are loaded while the current group's terms are evaluated
@@ -1000,7 +1016,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             }
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
-            auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, uint32_t i, uint16_t *li) {
+            auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, uint32_t i, const ListRef &li) {
                 constexpr bool BU = decltype(bu_tag)::value;
                 const float inv = c.inv_cell > 0.0f ? c.inv_cell : a.scenes[x.pc.scene].inv_cell;
                 // displacement since the cells were assigned: none at iteration 0; later the largest
@@ -1013,13 +1029,13 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                     uint32_t sreq;
                     const RowCull rs = row_cull(x.s, ci, inv, fsqrt(x.ts2), dmax);
                     const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, rs,
-                                                             a.slist + list_group_base(i, kSuperSlots), &sreq);
+                                                             list_ref(a.slist, i, kSuperSlots), &sreq);
                     a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
                     return req;
                 }
                 if (sc != kSuperNone && md2 < x.allow) {
                     ++ma.reuse;
-                    return scan_super(a.slist + list_group_base(i, kSuperSlots), sc, lpos, x, li, c);
+                    return scan_super(list_ref(a.slist, i, kSuperSlots), sc, lpos, x, li, c);
                 }
                 return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
             };
@@ -1031,7 +1047,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                 uint32_t cnt = 0;
                 if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, dprev)) {
                     DensitySum d;
-                    uint16_t *li = a.list + list_group_base(i, c.cap);
+                    const ListRef li = list_ref(a.list, i, c.cap);
                     if (density_walks(x, bu, c)) {
                         d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, i, li)
                                    : scan(std::false_type{}, x, ci, hc, fs, i, li);
@@ -1114,7 +1130,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int hc = halo_cell(ts, a.cellc[i]);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
-                const uint16_t *li = a.list + list_group_base(i, c.cap);
+                const ListRef li = list_ref(a.list, i, c.cap);
                 const float d2 = delta_particle<UNIFORM>(
                     i, fs,
                     [&](const DeltaSelf &x, DeltaSum &d) {

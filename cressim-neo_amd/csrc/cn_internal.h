@@ -51,6 +51,9 @@ __host__ __device__ inline size_t list_group_base(uint32_t i, uint32_t rows)
     return (size_t)(i / kListGroup) * kListGroup * rows + (i % kListGroup);
 }
 __host__ __device__ inline size_t list_slots(size_t n) { return (n + kListGroup - 1) / kListGroup * kListGroup; }
+// A list buffer is addressed with 32-bit byte offsets from its base (one 32-bit add per entry on the
+// device), so it stays within 4 GiB: n x max_fluid_neighbors < 2^31 entries.
+constexpr uint64_t kListBytesMax = 0xffffffffull;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);

@@ -68,3 +68,15 @@ def test_tune_set_rejects_bad_shapes(solver):
         solver.tune_set(100, 1024)
     with pytest.raises(cnhip.CnError):
         solver.tune_set(2048, 4)
+
+
+def test_reserve_rejects_lists_above_4gib():
+    # list entries are addressed with 32-bit byte offsets: 2^26 particles x 64 slots x 2 bytes is
+    # 8 GiB, refused before anything is allocated
+    s = cnhip.Solver(0)
+    try:
+        with pytest.raises(cnhip.CnError) as e:
+            s.reserve(1 << 26, 64)
+        assert e.value.status == cnhip.abi.CN_ERR_CAPACITY
+    finally:
+        s.close()
