@@ -1278,8 +1278,8 @@ static uint32_t persistent_grid(const SubstepConsts &c)
 
 // Halo capacities: the density pass stages positions (and classes, kTilePad spare entries each),
 // the delta pass records, previous deltas, normals and classes, the wide delta pass the same
-// without previous deltas but with a slot map.  density >= wide >= delta, so every block the delta passes stage was staged (and
-// listed in halo offsets) by the density pass.
+// without previous deltas but with a slot map.  density >= wide >= delta, so every block the delta
+// passes stage was staged (and listed in halo offsets) by the density pass.
 void fluid_tiles(SubstepConsts &c)
 {
     const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
