@@ -409,6 +409,20 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
 // physics_fluid_apply_delta_positions.cs.hlsl:11-31 and physics_fluid_clamp_boundary.cs.hlsl:23-80
 // on the same particle.
 // ---------------------------------------------------------------------------------------------
+// relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can only act when
+// |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl (max|delta| of the previous iteration).
+__device__ __forceinline__ bool cfl_term_on(const cn_fluid_material &m, const FluidArgs &a, const SubstepConsts &c)
+{
+    const float cfl = fmaxd(m.cfl_radius, 0.0f);
+    if (!(a.it > 0 && cfl > kEps)) return false;
+    const float md2 = a.it - 1 < kIterStateWords ? __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]) : 3.4e38f;
+    return !(4.0f * md2 * c.cfl_guard < cfl * cfl);
+}
+__device__ __forceinline__ bool visc_term_on(const cn_fluid_material &m, const FluidArgs &a, const SubstepConsts &c)
+{
+    return a.it > 0 && c.dt * fmaxd(m.viscosity, 0.0f) > kEps;
+}
+
 struct DeltaSelf {
     ParticleClass pc;
     float4 di, nci;
@@ -528,15 +542,8 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         x.viscosity = c.dt * fmaxd(m.viscosity, 0.0f);
         x.cfl = fmaxd(m.cfl_radius, 0.0f);
         x.tension = fmaxd(m.surface_tension, 0.0f) > kEps;
-        // relDelta = delta_i - delta_j is zero on the first iteration, and the CFL term can only
-        // act when |relDelta| >= cfl, i.e. never while 2 max|delta| < cfl
-        x.cfl_on = false;
-        if (use_d && x.cfl > kEps) {
-            const float md2 = a.it - 1 < kIterStateWords ? __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1])
-                                                         : 3.4e38f;
-            x.cfl_on = !(4.0f * md2 * c.cfl_guard < x.cfl * x.cfl);
-        }
-        x.visc_on = use_d && x.viscosity > kEps;
+        x.cfl_on = cfl_term_on(m, a, c);
+        x.visc_on = visc_term_on(m, a, c);
         x.need_rel = x.cfl_on || x.visc_on;
         x.nci = x.tension ? rnc[own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         DeltaSum d;
@@ -1110,12 +1117,30 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
+            // the neighbors' previous deltas are read only by the viscosity and CFL terms: staged
+            // only when one of the block's fluid particles has one of them on
+            bool rel = false;
+            if (!WIDE && use_d) {
+                if (UNIFORM) {
+                    const ParticleClass pc0 = a.classes[0];
+                    rel = pc0.kind == CN_PARTICLE_KIND_FLUID &&
+                          (cfl_term_on(a.fm[pc0.fluid_mat], a, c) || visc_term_on(a.fm[pc0.fluid_mat], a, c));
+                } else {
+                    bool mine = false;
+                    for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
+                        const ParticleClass pc = a.classes[a.cls[i]];
+                        mine |= pc.kind == CN_PARTICLE_KIND_FLUID &&
+                                (cfl_term_on(a.fm[pc.fluid_mat], a, c) || visc_term_on(a.fm[pc.fluid_mat], a, c));
+                    }
+                    rel = __syncthreads_or(mine ? 1 : 0) != 0;
+                }
+            }
             bool mixed = false;
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 lcs[f] = a.cs[q];
-                if (!WIDE && use_d) ldin[f] = a.din[q];
+                if (rel) ldin[f] = a.din[q];
                 if (WIDE) lslot[f] = q;
                 if (tension) lnc[f] = a.nc[q];
                 if (!UNIFORM) {
