@@ -40,6 +40,27 @@ def test_soft_and_cloth_bitexact(solver, oracle, iters):
     assert all(b for _, _, b in res), res
 
 
+def test_constraint_fields_edge_cases(solver, oracle):
+    """Disabled edges, negative and NaN compliances and a negative tet rest volume: the device
+    uploads compact constraint forms (compliance and rest volume clamped at zero, a disabled edge
+    as a NaN compliance), which must step as the C-ABI records do in the oracle."""
+    sc = soft_and_cloth()
+    rng = np.random.default_rng(12)
+    e = sc.edges
+    e["enabled"][rng.random(len(e)) < 0.2] = 0
+    e["compliance"][rng.random(len(e)) < 0.1] = -1e-3
+    e["compliance"][3] = np.nan
+    e["enabled"][3] = 1
+    t = sc.tets
+    t["compliance"][rng.random(len(t)) < 0.1] = -5e-4
+    t["rest_volume"][rng.random(len(t)) < 0.05] *= -1.0
+    d = sc.dihedrals
+    d["compliance"][rng.random(len(d)) < 0.1] = -2e-3
+    res = run_pair(sc, params(), 4, solver, oracle)
+    print(res)
+    assert all(b for _, _, b in res), res
+
+
 def test_soft_iterations_differ_from_fluid_iterations(solver, oracle):
     sc = scenes.mixed(12, 6, 16)
     p = params(iters=3, fluid_iterations=2, soft_internal_iterations=5)
