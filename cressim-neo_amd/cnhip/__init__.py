@@ -113,6 +113,9 @@ class Solver:
             raise CnError(st, f"cn_ctx_create failed on device {device}")
         self.scene: Scene | None = None
         self._keep = []
+        self.device = device
+        self._nviews = 0            # live DLPack exports of the resident buffers
+        self._close_pending = False
 
     # -- helpers
     def _check(self, st: int, what: str):
@@ -121,6 +124,11 @@ class Solver:
             raise CnError(st, f"{what}: {msg.decode() if msg else ''}")
 
     def close(self):
+        """Destroys the context; while exported tensor views are alive the buffers they alias are
+        kept and the context goes when the last view is released."""
+        if self._nviews > 0:
+            self._close_pending = True
+            return
         if self._ctx:
             self.lib.cn_ctx_destroy(self._ctx)
             self._ctx = C.c_void_p()
@@ -139,6 +147,8 @@ class Solver:
 
     # -- scene
     def reserve(self, n: int, max_fluid_neighbors: int = 128):
+        if self._nviews > 0:
+            raise RuntimeError("release the exported tensor views before reserve() reallocates their buffers")
         self._check(self.lib.cn_scene_reserve(self._ctx, n, max_fluid_neighbors), "cn_scene_reserve")
 
     @staticmethod
@@ -253,6 +263,42 @@ class Solver:
         v = abi.DeviceViews()
         self._check(self.lib.cn_device_views_get(self._ctx, C.byref(v)), "cn_device_views_get")
         return v
+
+    # -- zero-copy tensor views (the reference's shared_buffer_to_dlpack, python/bindings.cpp:282-340)
+    def _acquire_view(self):
+        self._nviews += 1
+
+    def _release_view(self):
+        self._nviews -= 1
+        if self._nviews == 0 and self._close_pending:
+            self._close_pending = False
+            self.close()
+
+    def dlpack_views(self) -> dict:
+        """DLPack capsules of the resident particle state after the last step, particle-index order:
+        ``positions_inv_mass`` and ``velocities``, float32 (count, 4).  The buffers are written on
+        the solver's stream (``torch_stream``); they stay valid until ``reserve`` reallocates them
+        (refused while views are alive) and keep the context alive past ``close``."""
+        from . import dlpack
+        v = self.device_views()
+        if v.count == 0:
+            raise RuntimeError("no resident scene")
+        out = {}
+        for name in ("positions_inv_mass", "velocities"):
+            out[name] = dlpack.to_dlpack(getattr(v, name), (v.count, 4), "float32", dlpack.KDL_ROCM, self.device,
+                                         owner=self, size_bytes=16 * v.count)
+        return out
+
+    def torch_views(self) -> dict:
+        """``dlpack_views`` as torch tensors on the solver's device, without a copy."""
+        import torch
+        return {k: torch.from_dlpack(c) for k, c in self.dlpack_views().items()}
+
+    def torch_stream(self):
+        """The solver's HIP stream as a torch stream: order torch work after ``step`` with
+        ``torch.cuda.current_stream().wait_stream(solver.torch_stream())``."""
+        import torch
+        return torch.cuda.ExternalStream(self.device_views().stream, device=torch.device("cuda", self.device))
 
     # -- per-pass entries
     def pass_keys(self, pos: np.ndarray, cell: float):

@@ -103,6 +103,8 @@ class Server:
 
     def close(self):
         for s in self.shards:
+            if hasattr(s, "_tensor_views"):
+                del s._tensor_views
             s.close()
         self.shards = []
 
@@ -111,6 +113,19 @@ class Server:
         j, local = self.where[k]
         self.shards[j].synchronize()
         return self.shards[j].readback_scene(local)
+
+    def session_tensors(self, k: int):
+        """Zero-copy torch views (positions_inv_mass, velocities; float32 (count, 4), particle-index
+        order) of session k's resident state, and the shard's torch stream that writes them: read
+        them after ``torch.cuda.current_stream().wait_stream(stream)`` once the tick that served the
+        session is done.  The views alias the shard's buffers, so they follow later ticks."""
+        j, local = self.where[k]
+        s = self.shards[j]
+        if not hasattr(s, "_tensor_views"):
+            s._tensor_views = s.torch_views()
+        first, count = s.ranges[local]
+        v = s._tensor_views
+        return (v["positions_inv_mass"][first:first + count], v["velocities"][first:first + count]), s.torch_stream()
 
     def serve(self, arrivals, warm: bool = True):
         """Serve (time, session) requests in real time; returns the completed Requests (deferred
