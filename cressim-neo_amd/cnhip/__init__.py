@@ -294,6 +294,31 @@ class Solver:
         import torch
         return {k: torch.from_dlpack(c) for k, c in self.dlpack_views().items()}
 
+    def set_scene_state(self, k: int, pos, vel):
+        """Overwrite scene k's positions (xyz + inverse mass) and velocities, float32 (count, 4) in the
+        scene's particle order: numpy arrays, or torch tensors on the solver's device (read in
+        place after the current torch stream's work)."""
+        first, count = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.cn_batch_scene_range(self._ctx, k, C.byref(first), C.byref(count)), "cn_batch_scene_range")
+        if hasattr(pos, "data_ptr") and getattr(pos, "is_cuda", False):
+            import torch
+            for t in (pos, vel):
+                if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != (count.value, 4):
+                    raise ValueError(f"state tensors must be contiguous float32 ({count.value}, 4)")
+            lib_stream = self.torch_stream()
+            lib_stream.wait_stream(torch.cuda.current_stream())
+            self._check(self.lib.cn_scene_set_state(self._ctx, k, pos.data_ptr(), vel.data_ptr(),
+                                                    abi.CN_STATE_DEVICE_POINTERS), "cn_scene_set_state")
+            # later torch work (which may reuse the tensors' memory once they are freed) waits for
+            # the library's read; no record_stream, which would outlive the context's stream
+            torch.cuda.current_stream().wait_stream(lib_stream)
+            return
+        p = np.ascontiguousarray(pos, np.float32)
+        v = np.ascontiguousarray(vel, np.float32)
+        if p.shape != (count.value, 4) or v.shape != p.shape:
+            raise ValueError(f"state arrays must be float32 ({count.value}, 4)")
+        self._check(self.lib.cn_scene_set_state(self._ctx, k, p.ctypes.data, v.ctypes.data, 0), "cn_scene_set_state")
+
     def torch_stream(self):
         """The solver's HIP stream as a torch stream: order torch work after ``step`` with
         ``torch.cuda.current_stream().wait_stream(solver.torch_stream())``."""

@@ -27,6 +27,7 @@ CN_KEY_MORTON = 0
 CN_KEY_HASH = 1
 CN_SOLVER_CONFIG_FLOATS = 40
 CN_STEP_CANDIDATE_REUSE = 1
+CN_STATE_DEVICE_POINTERS = 1
 CN_KERNEL_SLOTS = 16
 
 
@@ -151,6 +152,7 @@ SIGNATURES = {
     "cn_step_scenes": (C.c_int, [_P, C.POINTER(StepParams), _U, _P]),
     "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
     "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),
+    "cn_scene_set_state": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]),
     "cn_validate_meta": (C.c_int, [_P, C.POINTER(Meta)]),
     "cn_synchronize": (C.c_int, [_P]),
     "cn_query": (C.c_int, [_P]),

@@ -127,6 +127,12 @@ class Server:
         v = s._tensor_views
         return (v["positions_inv_mass"][first:first + count], v["velocities"][first:first + count]), s.torch_stream()
 
+    def reset_session(self, k: int, pos, vel):
+        """Overwrite session k's particle state (an environment reset) without re-uploading its
+        shard: ordered after the shard's in-flight tick on the shard's stream."""
+        j, local = self.where[k]
+        self.shards[j].set_scene_state(local, pos, vel)
+
     def serve(self, arrivals, warm: bool = True):
         """Serve (time, session) requests in real time; returns the completed Requests (deferred
         sessions' requests are refused and not returned)."""

@@ -54,6 +54,8 @@ struct cn_ctx {
     };
     std::vector<Graph> graphs;
     std::vector<uint32_t> active_host;  // last per-scene active flags written to the device
+    float4 *stage = nullptr;            // host-provided particle state on its way to the device
+    size_t stage_cap = 0;
     // profiling
     bool profiling = false;
     struct Ev {
@@ -467,6 +469,7 @@ void cn_ctx_destroy(cn_ctx *ctx)
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     clear_graphs(ctx);
     free_scene(ctx->s);
+    dfree(ctx->stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1453,6 +1456,39 @@ cn_status cn_readback(cn_ctx *ctx, float *pos, float *prev, float *vel, uint32_t
     if (pos) CN_HIP(ctx, hipMemcpy(pos, ctx->s.out_pos, count * sizeof(float4), hipMemcpyDeviceToHost));
     if (prev) CN_HIP(ctx, hipMemcpy(prev, ctx->s.out_prev, count * sizeof(float4), hipMemcpyDeviceToHost));
     if (vel) CN_HIP(ctx, hipMemcpy(vel, ctx->s.out_vel, count * sizeof(float4), hipMemcpyDeviceToHost));
+    return CN_OK;
+}
+
+cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions_inv_mass, const float *velocities,
+                             uint32_t flags)
+{
+    if (!ctx || !positions_inv_mass || !velocities) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (scene >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const uint32_t first = ctx->scene_first[scene], count = ctx->scene_first[scene + 1] - first;
+    if (count == 0) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    const float4 *P = reinterpret_cast<const float4 *>(positions_inv_mass);
+    const float4 *V = reinterpret_cast<const float4 *>(velocities);
+    const bool host = !(flags & CN_STATE_DEVICE_POINTERS);
+    if (host) {
+        if (ctx->stage_cap < 2 * (size_t)count) {
+            CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            dfree(ctx->stage);
+            ctx->stage = nullptr;
+            ctx->stage_cap = 0;
+            CN_HIP(ctx, dalloc(&ctx->stage, 2 * (size_t)count));
+            ctx->stage_cap = 2 * (size_t)count;
+        }
+        CN_HIP(ctx, hipMemcpyAsync(ctx->stage, P, count * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        CN_HIP(ctx, hipMemcpyAsync(ctx->stage + count, V, count * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        P = ctx->stage;
+        V = ctx->stage + count;
+    }
+    launch_set_state(ctx->s, first, count, P, V, ctx->stream);
+    CN_HIP(ctx, hipGetLastError());
+    // host arrays (and the stage) may be reused as soon as the call returns
+    if (host) CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CN_OK;
 }
 

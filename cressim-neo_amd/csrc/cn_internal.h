@@ -188,6 +188,8 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
 void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
+void launch_set_state(SceneBuffers &s, uint32_t first, uint32_t count, const float4 *P, const float4 *V,
+                      hipStream_t st);
 void launch_fill(uint32_t *p, uint32_t words, uint32_t value, hipStream_t st);
 
 // ---- particle-particle contacts (cn_contact.hip) ----

@@ -535,6 +535,34 @@ void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st)
                        prev ? s.out_prev : nullptr);
 }
 
+// overwrite the state of the particles [first, first + count) (particle-index order) from P, V
+__global__ __launch_bounds__(kBlock) void k_set_state(const uint32_t *__restrict__ id, uint32_t n, uint32_t first,
+                                                      uint32_t count, const float4 *__restrict__ P,
+                                                      const float4 *__restrict__ V, float4 *__restrict__ pos,
+                                                      float4 *__restrict__ vel, float4 *__restrict__ prev,
+                                                      float4 *__restrict__ opos, float4 *__restrict__ ovel,
+                                                      float4 *__restrict__ oprev)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t d = id[t];
+    if (d < first || d - first >= count) return;
+    const float4 p = P[d - first], v = V[d - first];
+    pos[t] = p;
+    vel[t] = v;
+    prev[t] = p;
+    opos[d] = p;
+    ovel[d] = v;
+    oprev[d] = p;
+}
+
+void launch_set_state(SceneBuffers &s, uint32_t first, uint32_t count, const float4 *P, const float4 *V,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_state, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.id[s.cur], s.n, first, count, P, V,
+                       s.pos[s.cur], s.vel[s.cur], s.prev[s.cur], s.out_pos, s.out_vel, s.out_prev);
+}
+
 // word fill as a kernel (stays a kernel node when the step is captured into a graph)
 __global__ __launch_bounds__(kBlock) void k_fill(uint32_t *__restrict__ p, uint32_t words, uint32_t value)
 {

@@ -296,6 +296,17 @@ cn_status cn_tune_set(cn_ctx *ctx, uint32_t tile_max, uint32_t grid_max);
 cn_status cn_readback(cn_ctx *ctx, float *positions_inv_mass, float *previous_positions,
                       float *velocities, uint32_t count);
 cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
+/* Overwrites the particle state of scene `scene` (0 for a single uploaded scene): positions with
+   inverse masses and velocities, float4 per particle in the scene's particle order (cn_readback's
+   order within cn_batch_scene_range); the previous positions become the new positions.  The other
+   scenes, the topology, the materials and the colliders stay, so an environment reset in a pool
+   needs no re-upload (the reference re-uploads its whole particle set when the world's particle
+   revision changes, physics_scene_gpu_state.cpp:2393-2486).  flags: CN_STATE_DEVICE_POINTERS when
+   both arrays are device memory, read on the context's stream (e.g. torch tensors ordered before
+   it); otherwise host memory, copied before the call returns. */
+#define CN_STATE_DEVICE_POINTERS 1u
+cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions_inv_mass,
+                             const float *velocities, uint32_t flags);
 /* validateGpuMetaBlocking analog; returns CN_ERR_CAPACITY if a capacity overflowed. */
 cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out);
 cn_status cn_synchronize(cn_ctx *ctx);
