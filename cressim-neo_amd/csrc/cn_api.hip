@@ -36,6 +36,7 @@ struct cn_ctx {
     bool has_tension = false;        // any fluid material with surface tension
     bool has_solid = false;          // any solid particle
     bool any_pairs = false;          // some scene can hold a particle-particle contact pair
+    bool has_vorticity = false;      // a fluid particle's material confines vorticity
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
     uint32_t grid_max = 768;         // workgroups of a persistent fluid launch (three per CU)
     // scene pool
@@ -737,6 +738,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->has_fluid = false;
     ctx->has_solid = false;
     ctx->any_pairs = false;
+    ctx->has_vorticity = false;
     uint32_t base = 0;
     double max_span = 1.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -766,6 +768,8 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
             pc.scene = k;
             ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
+            if (pc.kind == CN_PARTICLE_KIND_FLUID && fmi < tb->fluid_material_count)
+                ctx->has_vorticity |= std::max(tb->fluid_materials[fmi].vorticity_confinement, 0.0f) > kEps;
             ctx->has_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             scene_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             scene_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
@@ -1188,6 +1192,12 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         {
             ProfScope ps(ctx, "finalize");
             launch_finalize(s, c, din, ctx->stream);
+        }
+        // physics_solver.cpp:1261-1283; with dt <= epsilon every velocity is zero after the update,
+        // so the curl and the confinement vanish
+        if (ctx->has_fluid && ctx->has_vorticity && c.dt > kEps) {
+            ProfScope ps(ctx, "fluid_vorticity");
+            launch_fluid_vorticity(s, c, ctx->stream);
         }
         if (has_contacts) {
             ProfScope ps(ctx, "contact_velocity");

@@ -1243,6 +1243,83 @@ __global__ __launch_bounds__(kBlock) void k_fluid_lists(FluidArgs a, SubstepCons
     a.nbr_cnt[i] = cnt;
 }
 
+// Vorticity confinement after the velocity update: physics_fluid_compute_vorticity.cs.hlsl:16-81
+// (APPLY = false: vort[i] = (curl, |curl|), zero for particles that do not confine) and
+// physics_fluid_apply_vorticity_confinement.cs.hlsl:16-94 (APPLY = true).  Both walk the
+// post-update neighbor list (physics_solver.cpp:1262) instead of storing it: the first cap
+// in-radius neighbors in walk order on the final positions, of which the same-material ones
+// contribute.  Inactive scenes are skipped whole (their neighbors are their own).
+template <bool UNIFORM, bool APPLY>
+__global__ __launch_bounds__(kBlock) void k_fluid_vorticity(FluidArgs a, SubstepConsts c, float4 *__restrict__ vel,
+                                                            float4 *__restrict__ vort)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.n) return;
+    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    if (!a.active[pc.scene]) return;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (pc.kind != CN_PARTICLE_KIND_FLUID) {
+        if (!APPLY) vort[i] = zero;
+        return;
+    }
+    const cn_fluid_material &m = a.fm[pc.fluid_mat];
+    const float conf = fmaxd(m.vorticity_confinement, 0.0f);
+    const float4 s = a.pos[i];
+    if (conf <= kEps || s.w <= kEps) {
+        if (!APPLY) vort[i] = zero;
+        return;
+    }
+    float4 self = APPLY ? vort[i] : vel[i];
+    if (APPLY && self.w <= kEps) return;
+    const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
+    const float t2 = a.t2[pc.fluid_mat];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    uint32_t cnt = 0;
+    if (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc)))
+        walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
+            if (q == i) return true;
+            const uint32_t qmat = UNIFORM ? pc.fluid_mat : a.classes[a.cls[q]].fluid_mat;
+            if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
+            const float4 pj = a.pos[q];
+            if (pj.w <= kEps) return true;
+            const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
+            if (!(dot3(dx, dy, dz, dx, dy, dz) < t2)) return true;
+            if (qmat == pc.fluid_mat) {
+                const float dist = len3(dx, dy, dz);
+                if (!(dist <= kEps || dist >= h)) {
+                    // FluidSpikyKernelGradient, physics_fluid_common.hlsli:45-54
+                    const float dw = spiky_dw(dist, h);
+                    const float gx = dw * fdiv(dx, dist), gy = dw * fdiv(dy, dist), gz = dw * fdiv(dz, dist);
+                    if (APPLY) {
+                        const float mj = vort[q].w;
+                        ax = ax + mj * gx;
+                        ay = ay + mj * gy;
+                        az = az + mj * gz;
+                    } else {
+                        const float4 vj = vel[q];
+                        const float wx = vj.x - self.x, wy = vj.y - self.y, wz = vj.z - self.z;
+                        ax = ax + (wy * gz - wz * gy);
+                        ay = ay + (wz * gx - wx * gz);
+                        az = az + (wx * gy - wy * gx);
+                    }
+                }
+            }
+            return ++cnt < c.cap;
+        });
+    if (!APPLY) {
+        vort[i] = make_float4(ax, ay, az, len3(ax, ay, az));
+        return;
+    }
+    const float gl = len3(ax, ay, az);
+    if (gl <= kEps) return;
+    const float nx = fdiv(ax, gl), ny = fdiv(ay, gl), nz = fdiv(az, gl);
+    const float fx = conf * (ny * self.z - nz * self.y);
+    const float fy = conf * (nz * self.x - nx * self.z);
+    const float fz = conf * (nx * self.y - ny * self.x);
+    const float4 v = vel[i];
+    vel[i] = make_float4(v.x + c.dt * fx, v.y + c.dt * fy, v.z + c.dt * fz, 0.0f);
+}
+
 // ---------------------------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------------------------
@@ -1374,6 +1451,20 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
             hipLaunchKernelGGL(k_fluid_delta_global<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
         else
             hipLaunchKernelGGL(k_fluid_delta_global<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c);
+    }
+}
+
+void launch_fluid_vorticity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, 0, 0);
+    float4 *vel = s.vel[s.cur];
+    float4 *vort = s.cs;  // free between the last delta pass of a substep and the next density pass
+    if (c.uniform_class) {
+        hipLaunchKernelGGL((k_fluid_vorticity<true, false>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
+        hipLaunchKernelGGL((k_fluid_vorticity<true, true>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
+    } else {
+        hipLaunchKernelGGL((k_fluid_vorticity<false, false>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
+        hipLaunchKernelGGL((k_fluid_vorticity<false, true>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
     }
 }
 

@@ -186,6 +186,8 @@ void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t 
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st);
 void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+// vorticity confinement on the finalized positions and velocities of the substep
+void launch_fluid_vorticity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
 void launch_set_state(SceneBuffers &s, uint32_t first, uint32_t count, const float4 *P, const float4 *V,

@@ -1140,6 +1140,113 @@ static void sub3(const float *a, const float *b, float *o)
     o[2] = a[2] - b[2];
 }
 
+/* FluidSpikyKernelGradient, physics_fluid_common.hlsli:45-54 */
+static void spiky_grad(const float *d, float dist, float h, float *g)
+{
+    g[0] = g[1] = g[2] = 0.0f;
+    if (dist <= EPS || dist >= h)
+        return;
+    const float dw = spiky_dw(dist, h);
+    for (int a = 0; a < 3; ++a)
+        g[a] = dw * (d[a] / dist);
+}
+
+/* physics_fluid_compute_vorticity.cs.hlsl:16-81 over the post-update neighbor lists: per fluid
+ * particle of a confining material, curl = sum over same-material neighbors of
+ * (v_j - v_i) x gradW(x_i - x_j), stored as (curl, |curl|); zero for every other particle. */
+void cno_fluid_vorticity(const cno_scene *s, uint32_t cap, const uint32_t *counts, const uint32_t *list,
+                         float *vort)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float *o = vort + 4 * (size_t)i;
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const cn_fluid_material *m = &s->fluid_mats[s->fluid_mat[i]];
+        if (maxf(m->vorticity_confinement, 0.0f) <= EPS)
+            continue;
+        const float *pi = s->pos + 4 * (size_t)i;
+        if (pi[3] <= EPS)
+            continue;
+        const float *vi = s->vel + 4 * (size_t)i;
+        const float h = maxf(m->smoothing_radius, 1.0e-4f);
+        float curl[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < counts[i]; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            if (s->fluid_mat[j] != s->fluid_mat[i])
+                continue;
+            const float *pj = s->pos + 4 * (size_t)j;
+            float d[3], g[3], w[3], c[3];
+            sub3(pi, pj, d);
+            const float dist = len3(d);
+            if (dist <= EPS || dist >= h)
+                continue;
+            sub3(s->vel + 4 * (size_t)j, vi, w);
+            spiky_grad(d, dist, h, g);
+            cross3(w, g, c);
+            for (int a = 0; a < 3; ++a)
+                curl[a] = curl[a] + c[a];
+        }
+        o[0] = curl[0];
+        o[1] = curl[1];
+        o[2] = curl[2];
+        o[3] = len3(curl);
+    }
+}
+
+/* physics_fluid_apply_vorticity_confinement.cs.hlsl:16-94: eta = sum over same-material
+ * neighbors of |curl_j| gradW(x_i - x_j); v_i += dt * confinement * ((eta / |eta|) x curl_i),
+ * with the velocity's w written as zero. */
+void cno_fluid_vorticity_apply(cno_scene *s, float dt, uint32_t cap, const uint32_t *counts,
+                               const uint32_t *list, const float *vort)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID)
+            continue;
+        const cn_fluid_material *m = &s->fluid_mats[s->fluid_mat[i]];
+        const float conf = maxf(m->vorticity_confinement, 0.0f);
+        if (conf <= EPS)
+            continue;
+        const float *pi = s->pos + 4 * (size_t)i;
+        if (pi[3] <= EPS)
+            continue;
+        const float *ci = vort + 4 * (size_t)i;
+        if (ci[3] <= EPS)
+            continue;
+        const float h = maxf(m->smoothing_radius, 1.0e-4f);
+        float eta[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < counts[i]; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            if (s->fluid_mat[j] != s->fluid_mat[i])
+                continue;
+            float d[3], g[3];
+            sub3(pi, s->pos + 4 * (size_t)j, d);
+            const float dist = len3(d);
+            if (dist <= EPS || dist >= h)
+                continue;
+            spiky_grad(d, dist, h, g);
+            const float mj = vort[4 * (size_t)j + 3];
+            for (int a = 0; a < 3; ++a)
+                eta[a] = eta[a] + mj * g[a];
+        }
+        const float gl = len3(eta);
+        if (gl <= EPS)
+            continue;
+        float nrm[3], f[3];
+        for (int a = 0; a < 3; ++a)
+            nrm[a] = eta[a] / gl;
+        cross3(nrm, ci, f);
+        float *v = s->vel + 4 * (size_t)i;
+        for (int a = 0; a < 3; ++a)
+            v[a] = v[a] + dt * (conf * f[a]);
+        v[3] = 0.0f;
+    }
+}
+
 /* Particle -> incident constraint lists (physics_scene_gpu_state.cpp:3273-3360): per particle the
  * references in ascending constraint index; ref = (constraint << 2) | slot.  Bend corrections are
  * indexed bends first, then dihedrals at softBendCount + dihedralIndex. */
@@ -1943,10 +2050,13 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         cno_update_velocities(s, dt, dacc);
         if (has_fluid) {
             cno_project_boundary_velocities(s, cfg, dt, br, bp);
-            /* post-update neighbor rebuild (physics_solver.cpp:1262); it feeds only vorticity
-             * and render anisotropy, so it changes no particle state */
+            /* post-update neighbor rebuild (physics_solver.cpp:1262); it feeds vorticity
+             * confinement and render anisotropy */
             cno_fluid_neighbors(s, cell, cells, keys, idx, tbl, capacity, dacc,
                                 max_iters > 0 ? max_iters - 1 : 0, cap, counts, list, NULL);
+            /* physics_solver.cpp:1269-1283 */
+            cno_fluid_vorticity(s, cap, counts, list, dtmp);
+            cno_fluid_vorticity_apply(s, dt, cap, counts, list, dtmp);
         }
         if (has_contacts)
             for (uint32_t v = 0; v < contact_iters; ++v)

@@ -109,6 +109,11 @@ void cno_update_velocities(cno_scene *s, float dt, float *delta_acc);
 void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt,
                                      const uint32_t *bnd_ranges, const uint32_t *bnd_pairs);
 void cno_predict(cno_scene *s, float dt, const float *gravity);
+/* vorticity confinement over the post-update neighbor lists; vort float4[n] = (curl, |curl|) */
+void cno_fluid_vorticity(const cno_scene *s, uint32_t cap, const uint32_t *counts, const uint32_t *list,
+                         float *vort);
+void cno_fluid_vorticity_apply(cno_scene *s, float dt, uint32_t cap, const uint32_t *counts,
+                               const uint32_t *list, const float *vort);
 
 /* ---- XPBD soft constraints ---- */
 /* Cephes single-precision atan2f / acosf (S. L. Moshier), evaluated without contraction; the GPU

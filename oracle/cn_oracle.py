@@ -64,6 +64,8 @@ def load():
             "cno_fluid_neighbors": (None, [_P, C.c_float, _P, _P, _P, _P, _U, _P, _U, _U, _P, _P, _P]),
             "cno_predict": (None, [_P, C.c_float, _P]),
             "cno_fluid_density": (None, [_P, _P, _P, _U, _U, _P, _P, _P, _P, _P]),
+            "cno_fluid_vorticity": (None, [_P, _U, _P, _P, _P]),
+            "cno_fluid_vorticity_apply": (None, [_P, C.c_float, _U, _P, _P, _P]),
             "cno_step": (C.c_int, [_P, _P, _P]),
             "cno_threads": (C.c_int, []),
             "cno_set_threads": (None, [C.c_int]),
@@ -224,6 +226,27 @@ def fluid_density(osc: OracleScene, params, counts, lists, dacc=None, iteration=
     lib.cno_fluid_density(C.byref(osc.c), cfg.ctypes.data, dacc.ctypes.data, iteration, cap,
                           counts.ctypes.data, lists.ctypes.data, br.ctypes.data, bp.ctypes.data, nc.ctypes.data)
     return nc
+
+
+def fluid_vorticity(osc: OracleScene, counts, lists) -> np.ndarray:
+    """physics_fluid_compute_vorticity restated: (curl, |curl|) per particle."""
+    lib = load()
+    lists = np.ascontiguousarray(lists, np.uint32)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    vort = np.zeros((osc.n, 4), np.float32)
+    lib.cno_fluid_vorticity(C.byref(osc.c), lists.shape[1], counts.ctypes.data, lists.ctypes.data,
+                            vort.ctypes.data)
+    return vort
+
+
+def fluid_vorticity_apply(osc: OracleScene, dt: float, counts, lists, vort):
+    """physics_fluid_apply_vorticity_confinement restated: updates osc.vel in place."""
+    lib = load()
+    lists = np.ascontiguousarray(lists, np.uint32)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    vort = np.ascontiguousarray(vort, np.float32)
+    lib.cno_fluid_vorticity_apply(C.byref(osc.c), C.c_float(dt), lists.shape[1], counts.ctypes.data,
+                                  lists.ctypes.data, vort.ctypes.data)
 
 
 def threads() -> int:
