@@ -206,6 +206,17 @@ class Solver:
         f, c = self.ranges[k]
         return pos[f:f + c], prev[f:f + c], vel[f:f + c]
 
+    def render_anisotropy_enable(self, enable: bool = True):
+        """Make every step end with the render anisotropy (cn_render_anisotropy_enable)."""
+        self._check(self.lib.cn_render_anisotropy_enable(self._ctx, 1 if enable else 0), "cn_render_anisotropy_enable")
+
+    def render_anisotropy(self) -> np.ndarray:
+        """(3, n, 4) float32: axis1, axis2, axis3 of every particle in particle order, w = scale."""
+        out = np.zeros((3, self.n, 4), np.float32)
+        self._check(self.lib.cn_render_anisotropy_read(self._ctx, out[0].ctypes.data, out[1].ctypes.data,
+                                                       out[2].ctypes.data, self.n), "cn_render_anisotropy_read")
+        return out
+
     def step(self, params: abi.StepParams):
         self._check(self.lib.cn_step(self._ctx, C.byref(params)), "cn_step")
 

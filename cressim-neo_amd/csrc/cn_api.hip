@@ -37,6 +37,8 @@ struct cn_ctx {
     bool has_solid = false;          // any solid particle
     bool any_pairs = false;          // some scene can hold a particle-particle contact pair
     bool has_vorticity = false;      // a fluid particle's material confines vorticity
+    bool aniso_enabled = false;      // cn_step ends with the render anisotropy
+    bool aniso_ready = false;        // a step has written it since it was enabled or uploaded
     uint32_t tile_max = 4096;        // halo particles a fluid-pass workgroup may stage in LDS
     uint32_t grid_max = 768;         // workgroups of a persistent fluid launch (three per CU)
     // scene pool
@@ -152,6 +154,7 @@ void free_scene(SceneBuffers &s)
     dfree(s.out_pos);
     dfree(s.out_vel);
     dfree(s.out_prev);
+    dfree(s.aniso);
     dfree(s.classes);
     dfree(s.fluid_mats);
     dfree(s.part_mats);
@@ -739,6 +742,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->has_solid = false;
     ctx->any_pairs = false;
     ctx->has_vorticity = false;
+    ctx->aniso_ready = false;
     uint32_t base = 0;
     double max_span = 1.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -1199,6 +1203,11 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
             ProfScope ps(ctx, "fluid_vorticity");
             launch_fluid_vorticity(s, c, ctx->stream);
         }
+        // physics_solver.cpp:1285-1290; each substep overwrites the previous one's output
+        if (ctx->has_fluid && ctx->aniso_enabled && sub + 1 == substeps) {
+            ProfScope ps(ctx, "fluid_render_anisotropy");
+            launch_fluid_render_aniso(s, c, s.aniso, ctx->stream);
+        }
         if (has_contacts) {
             ProfScope ps(ctx, "contact_velocity");
             for (uint32_t v = 0; v < contact_iters; ++v) launch_contact_velocity(s, c, ctx->stream);
@@ -1268,6 +1277,16 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
         CN_HIP(ctx, dalloc(&ctx->s.scnt, std::max<uint32_t>(ctx->capacity, 1u)));
         ctx->graph_valid = false;
     }
+    if (ctx->aniso_enabled && ctx->s.aniso_alloc < ctx->s.n) {  // output on first use
+        CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        dfree(ctx->s.aniso);
+        ctx->s.aniso = nullptr;
+        ctx->s.aniso_alloc = 0;
+        CN_HIP(ctx, dalloc(&ctx->s.aniso, 3ull * ctx->s.n));
+        ctx->s.aniso_alloc = ctx->s.n;
+        ctx->graph_valid = false;
+    }
+    if (ctx->aniso_enabled && ctx->has_fluid) ctx->aniso_ready = true;
     if (!ctx->graphs_enabled || ctx->profiling) {
         record_step(ctx, p);
         ctx->stepped = true;
@@ -1466,6 +1485,32 @@ cn_status cn_readback(cn_ctx *ctx, float *pos, float *prev, float *vel, uint32_t
     if (pos) CN_HIP(ctx, hipMemcpy(pos, ctx->s.out_pos, count * sizeof(float4), hipMemcpyDeviceToHost));
     if (prev) CN_HIP(ctx, hipMemcpy(prev, ctx->s.out_prev, count * sizeof(float4), hipMemcpyDeviceToHost));
     if (vel) CN_HIP(ctx, hipMemcpy(vel, ctx->s.out_vel, count * sizeof(float4), hipMemcpyDeviceToHost));
+    return CN_OK;
+}
+
+cn_status cn_render_anisotropy_enable(cn_ctx *ctx, int enable)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (ctx->aniso_enabled != (enable != 0)) ctx->graph_valid = false;
+    ctx->aniso_enabled = enable != 0;
+    ctx->aniso_ready = false;
+    return CN_OK;
+}
+
+cn_status cn_render_anisotropy_read(cn_ctx *ctx, float *axis1, float *axis2, float *axis3, uint32_t count)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (count > ctx->s.n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "count exceeds particle count");
+    if (!ctx->aniso_ready)
+        return fail(ctx, CN_ERR_NOT_READY, "no fluid step has written the render anisotropy since it was enabled");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float *dst[3] = {axis1, axis2, axis3};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k] && count)
+            CN_HIP(ctx, hipMemcpy(dst[k], ctx->s.aniso + (size_t)k * ctx->s.n, count * sizeof(float4),
+                                  hipMemcpyDeviceToHost));
     return CN_OK;
 }
 

@@ -1320,6 +1320,179 @@ __global__ __launch_bounds__(kBlock) void k_fluid_vorticity(FluidArgs a, Substep
     vel[i] = make_float4(v.x + c.dt * fx, v.y + c.dt * fy, v.z + c.dt * fz, 0.0f);
 }
 
+// FluidAnisotropyJacobiRotate, physics_fluid_anisotropy_common.hlsli:28-74 (rsqrt read as
+// 1/sqrt): the rotation in the (P, Q) plane that zeroes A[P][Q], accumulated into R.
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rotate(float (&A)[3][3], float (&R)[3][3])
+{
+    constexpr int K = 3 - P - Q;
+    const float apq = A[P][Q];
+    if (fabsf(apq) <= 1.0e-20f) return;
+    const float d = fdiv(A[P][P] - A[Q][Q], 2.0f * apq);
+    float t = fdiv(1.0f, fabsf(d) + fsqrt(d * d + 1.0f));
+    if (d < 0.0f) t = -t;
+    const float cs = fdiv(1.0f, fsqrt(t * t + 1.0f));
+    const float sn = t * cs;
+    A[P][P] = A[P][P] + t * apq;
+    A[Q][Q] = A[Q][Q] - t * apq;
+    A[P][Q] = 0.0f;
+    A[Q][P] = 0.0f;
+    const float akp = cs * A[K][P] + sn * A[K][Q];
+    const float akq = -sn * A[K][P] + cs * A[K][Q];
+    A[K][P] = akp;
+    A[P][K] = akp;
+    A[K][Q] = akq;
+    A[Q][K] = akq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float rkp = cs * R[k][P] + sn * R[k][Q];
+        const float rkq = -sn * R[k][P] + cs * R[k][Q];
+        R[k][P] = rkp;
+        R[k][Q] = rkq;
+    }
+}
+
+template <int A_, int B_>
+__device__ __forceinline__ void sort_pair(float (&ev)[3], float (&ax)[3][3])
+{
+    if (ev[A_] < ev[B_]) {
+        const float t = ev[A_];
+        ev[A_] = ev[B_];
+        ev[B_] = t;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float u = ax[A_][c];
+            ax[A_][c] = ax[B_][c];
+            ax[B_][c] = u;
+        }
+    }
+}
+
+// Render anisotropy: physics_fluid_build_render_anisotropy.cs.hlsl:34-171 with
+// physics_fluid_anisotropy_common.hlsli:9-156 (rcp and rsqrt read as correctly rounded 1/x and
+// 1/sqrt), over the post-update list walked again, twice: the weighted mean of the listed
+// neighbors, then their weighted covariance; four largest-pivot Jacobi rotations, axes sorted by
+// decreasing eigenvalue, scales 1.1 sqrt(eigenvalue) clamped to [r, 2r].  A pinned fluid particle
+// has no list (the reference reads a count its build never wrote there) and gets the isotropic
+// frame.  Written in particle-index order: out[k * n + particle] for axis k.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void k_fluid_render_aniso(FluidArgs a, SubstepConsts c,
+                                                               const uint32_t *__restrict__ id,
+                                                               float4 *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.n) return;
+    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    if (!a.active[pc.scene]) return;
+    const size_t pid = id[i];
+    const size_t n = c.n;
+    const float radius = fmaxd(pc.radius, 1.0e-4f);
+    float ax[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    float sc[3] = {radius, radius, radius};
+    if (pc.kind == CN_PARTICLE_KIND_FLUID) {
+        const cn_fluid_material &m = a.fm[pc.fluid_mat];
+        const float support = fmaxd(m.smoothing_radius, radius);
+        const float inv = fdiv(1.0f, support);
+        const float iso = fmaxd(1.0f * radius, radius);
+        const float4 s = a.pos[i];
+        const float t2 = a.t2[pc.fluid_mat];
+        const bool walk = s.w > kEps && c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc));
+        // f(neighbor position, weight) per contributing entry of the list
+        auto each = [&](auto &&f) {
+            uint32_t cnt = 0;
+            walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
+                if (q == i) return true;
+                if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
+                const float4 pj = a.pos[q];
+                if (pj.w <= kEps) return true;
+                const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
+                const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+                if (!(d2 < t2)) return true;
+                if (d2 > 0.0f) {
+                    const float dist = fsqrt(d2);
+                    if (dist < support) {
+                        const float x = dist * inv;
+                        f(pj, 1.0f - x * x * x);
+                    }
+                }
+                return ++cnt < c.cap;
+            });
+        };
+        float mx = 0.0f, my = 0.0f, mz = 0.0f, wsum = 0.0f;
+        if (walk)
+            each([&](const float4 &pj, float w) {
+                wsum = wsum + w;
+                mx = mx + pj.x * w;
+                my = my + pj.y * w;
+                mz = mz + pj.z * w;
+            });
+        if (!(wsum <= 0.0f)) {
+            mx = fdiv(mx, wsum);
+            my = fdiv(my, wsum);
+            mz = fdiv(mz, wsum);
+            float C[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+            each([&](const float4 &pj, float w) {
+                const float cc[3] = {pj.x - mx, pj.y - my, pj.z - mz};
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) C[r][q] = C[r][q] + w * cc[r] * cc[q];
+            });
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) C[r][q] = fdiv(C[r][q], wsum);
+            C[0][1] = C[1][0] = 0.5f * (C[0][1] + C[1][0]);
+            C[0][2] = C[2][0] = 0.5f * (C[0][2] + C[2][0]);
+            C[1][2] = C[2][1] = 0.5f * (C[1][2] + C[2][1]);
+            // FluidAnisotropyEigenDecomposition, physics_fluid_anisotropy_common.hlsli:76-122
+            float R[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+            for (int it = 0; it < 4; ++it) {
+                int pair = 0;
+                float mxv = fabsf(C[0][1]);
+                const float a02 = fabsf(C[0][2]);
+                if (a02 > mxv) {
+                    pair = 1;
+                    mxv = a02;
+                }
+                const float a12 = fabsf(C[1][2]);
+                if (a12 > mxv) {
+                    pair = 2;
+                    mxv = a12;
+                }
+                if (mxv < 1.0e-15f) break;
+                if (pair == 0)
+                    jacobi_rotate<0, 1>(C, R);
+                else if (pair == 1)
+                    jacobi_rotate<0, 2>(C, R);
+                else
+                    jacobi_rotate<1, 2>(C, R);
+            }
+            float ev[3] = {fmaxd(C[0][0], 0.0f), fmaxd(C[1][1], 0.0f), fmaxd(C[2][2], 0.0f)};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float vx = R[0][k], vy = R[1][k], vz = R[2][k];
+                const float l2 = dot3(vx, vy, vz, vx, vy, vz);
+                if (l2 > 1.0e-8f) {
+                    const float rl = fdiv(1.0f, fsqrt(l2));
+                    ax[k][0] = vx * rl;
+                    ax[k][1] = vy * rl;
+                    ax[k][2] = vz * rl;
+                }
+            }
+            sort_pair<0, 1>(ev, ax);
+            sort_pair<0, 2>(ev, ax);
+            sort_pair<1, 2>(ev, ax);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sc[k] = fmind(fmaxd(fsqrt(ev[k]) * 1.1f, 1.0f * radius), 2.0f * radius);
+        } else {
+            sc[0] = sc[1] = sc[2] = iso;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k * n + pid] = make_float4(ax[k][0], ax[k][1], ax[k][2], sc[k]);
+}
+
 // ---------------------------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------------------------
@@ -1466,6 +1639,15 @@ void launch_fluid_vorticity(const SceneBuffers &s, const SubstepConsts &c, hipSt
         hipLaunchKernelGGL((k_fluid_vorticity<false, false>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
         hipLaunchKernelGGL((k_fluid_vorticity<false, true>), dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, vel, vort);
     }
+}
+
+void launch_fluid_render_aniso(const SceneBuffers &s, const SubstepConsts &c, float4 *out, hipStream_t st)
+{
+    const FluidArgs a = fluid_args(s, c, 0, 0);
+    if (c.uniform_class)
+        hipLaunchKernelGGL(k_fluid_render_aniso<true>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, s.id[s.cur], out);
+    else
+        hipLaunchKernelGGL(k_fluid_render_aniso<false>, dim3(grid_of(c.n)), dim3(kBlock), 0, st, a, c, s.id[s.cur], out);
 }
 
 void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)

@@ -142,6 +142,8 @@ struct SceneBuffers {
     float4 *out_pos = nullptr;
     float4 *out_vel = nullptr;
     float4 *out_prev = nullptr;
+    float4 *aniso = nullptr;        // render anisotropy: axis1 | axis2 | axis3, n each, particle order
+    size_t aniso_alloc = 0;
     // tables
     ParticleClass *classes = nullptr;
     uint32_t nclasses = 0;
@@ -188,6 +190,8 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
 void launch_fluid_lists(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 // vorticity confinement on the finalized positions and velocities of the substep
 void launch_fluid_vorticity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+// render anisotropy of the finalized positions into out (3 n float4, particle-index order)
+void launch_fluid_render_aniso(const SceneBuffers &s, const SubstepConsts &c, float4 *out, hipStream_t st);
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st);
 void launch_set_state(SceneBuffers &s, uint32_t first, uint32_t count, const float4 *P, const float4 *V,

@@ -307,6 +307,18 @@ cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
 #define CN_STATE_DEVICE_POINTERS 1u
 cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions_inv_mass,
                              const float *velocities, uint32_t flags);
+/* Render anisotropy (buildFluidRenderAnisotropy, physics_pass_dispatcher.cpp:3098-3138 with
+   physics_fluid_build_render_anisotropy.cs.hlsl:34-171, the renderer's fluidAnisotropy1..3
+   buffers).  When enabled, every cn_step / cn_step_scenes with fluid ends by writing, per
+   particle of the stepped scenes, three float4 (axis, scale): the principal axes of its
+   neighbors' weighted covariance on the final positions, by decreasing spread, with scales
+   1.1 sqrt(eigenvalue) clamped to [r, 2r]; other particles get the identity frame with scale r.
+   Off by default (it changes no particle state).  cn_render_anisotropy_read copies axis k of the
+   first `count` particles in particle order to axisk (each may be NULL); CN_ERR_NOT_READY until
+   a step with fluid has written them since they were enabled or the scene was uploaded. */
+cn_status cn_render_anisotropy_enable(cn_ctx *ctx, int enable);
+cn_status cn_render_anisotropy_read(cn_ctx *ctx, float *axis1, float *axis2, float *axis3,
+                                    uint32_t count);
 /* validateGpuMetaBlocking analog; returns CN_ERR_CAPACITY if a capacity overflowed. */
 cn_status cn_validate_meta(cn_ctx *ctx, cn_meta *out);
 cn_status cn_synchronize(cn_ctx *ctx);

@@ -1247,6 +1247,209 @@ void cno_fluid_vorticity_apply(cno_scene *s, float dt, uint32_t cap, const uint3
     }
 }
 
+/* FluidAnisotropyJacobiRotate, physics_fluid_anisotropy_common.hlsli:28-74 (rsqrt read as
+ * 1/sqrt) */
+static void jacobi_rotate(float A[3][3], float R[3][3], int p, int q)
+{
+    if (fabsf(A[p][q]) <= 1.0e-20f)
+        return;
+    const float d = (A[p][p] - A[q][q]) / (2.0f * A[p][q]);
+    float t = 1.0f / (fabsf(d) + sqrtf(d * d + 1.0f));
+    if (d < 0.0f)
+        t = -t;
+    const float c = 1.0f / sqrtf(t * t + 1.0f);
+    const float sn = t * c;
+    A[p][p] = A[p][p] + t * A[p][q];
+    A[q][q] = A[q][q] - t * A[p][q];
+    A[p][q] = 0.0f;
+    A[q][p] = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        if (k == p || k == q)
+            continue;
+        const float akp = c * A[k][p] + sn * A[k][q];
+        const float akq = -sn * A[k][p] + c * A[k][q];
+        A[k][p] = akp;
+        A[p][k] = akp;
+        A[k][q] = akq;
+        A[q][k] = akq;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float rkp = c * R[k][p] + sn * R[k][q];
+        const float rkq = -sn * R[k][p] + c * R[k][q];
+        R[k][p] = rkp;
+        R[k][q] = rkq;
+    }
+}
+
+/* FluidAnisotropyEigenDecomposition, physics_fluid_anisotropy_common.hlsli:76-122: four
+ * largest-pivot Jacobi rotations */
+static void jacobi_eigen(float A[3][3], float R[3][3], float ev[3])
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            R[r][c] = r == c ? 1.0f : 0.0f;
+    for (int it = 0; it < 4; ++it) {
+        int pair = 0;
+        float mx = fabsf(A[0][1]);
+        const float a02 = fabsf(A[0][2]);
+        if (a02 > mx) {
+            pair = 1;
+            mx = a02;
+        }
+        const float a12 = fabsf(A[1][2]);
+        if (a12 > mx) {
+            pair = 2;
+            mx = a12;
+        }
+        if (mx < 1.0e-15f)
+            break;
+        if (pair == 0)
+            jacobi_rotate(A, R, 0, 1);
+        else if (pair == 1)
+            jacobi_rotate(A, R, 0, 2);
+        else
+            jacobi_rotate(A, R, 1, 2);
+    }
+    for (int k = 0; k < 3; ++k)
+        ev[k] = maxf(A[k][k], 0.0f);
+}
+
+/* SafeNormalize of physics_fluid_build_render_anisotropy.cs.hlsl:21-25 (rsqrt read as 1/sqrt) */
+static void aniso_normalize(const float *v, int axis, float *o)
+{
+    const float l2 = dot3(v, v);
+    if (l2 > 1.0e-8f) {
+        const float r = 1.0f / sqrtf(l2);
+        for (int a = 0; a < 3; ++a)
+            o[a] = v[a] * r;
+    } else {
+        for (int a = 0; a < 3; ++a)
+            o[a] = a == axis ? 1.0f : 0.0f;
+    }
+}
+
+static void write_aniso(float *out, size_t n, size_t i, const float ax[3][3], const float *scale)
+{
+    for (int k = 0; k < 3; ++k) {
+        float *o = out + 4 * (k * n + i);
+        o[0] = ax[k][0];
+        o[1] = ax[k][1];
+        o[2] = ax[k][2];
+        o[3] = scale[k];
+    }
+}
+
+/* physics_fluid_build_render_anisotropy.cs.hlsl:34-171: per fluid particle, the weighted
+ * covariance of its listed fluid neighbors (weight 1 - (r / support)^3), its eigenvectors by
+ * four Jacobi rotations, sorted by decreasing eigenvalue, scales 1.1 sqrt(eigenvalue) clamped
+ * to [radius, 2 radius]; other particles and particles with no weight get the isotropic frame.
+ * A pinned fluid particle (inverse mass <= epsilon) has no post-update list: the reference reads
+ * a count its neighbor build never wrote there; here its list is empty. */
+void cno_fluid_render_anisotropy(const cno_scene *s, uint32_t cap, const uint32_t *counts,
+                                 const uint32_t *list, float *out)
+{
+    const size_t n = s->n;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const float radius = maxf(s->radii[i], 1.0e-4f);
+        float iso_ax[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+        if (s->kinds[i] != CN_PARTICLE_KIND_FLUID) {
+            const float sc[3] = {radius, radius, radius};
+            write_aniso(out, n, i, iso_ax, sc);
+            continue;
+        }
+        const cn_fluid_material *m = &s->fluid_mats[s->fluid_mat[i]];
+        const float support = maxf(m->smoothing_radius, radius);
+        const float inv = 1.0f / support;
+        const float iso = maxf(1.0f * radius, radius);
+        const float *pi = s->pos + 4 * (size_t)i;
+        const uint32_t cnt = (pi[3] <= EPS || cap == 0) ? 0u : counts[i];
+        float mean[3] = {0.0f, 0.0f, 0.0f}, wsum = 0.0f;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            if (s->kinds[j] != CN_PARTICLE_KIND_FLUID)
+                continue;
+            const float *pj = s->pos + 4 * (size_t)j;
+            float d[3];
+            sub3(pi, pj, d);
+            const float d2 = dot3(d, d);
+            if (d2 <= 0.0f)
+                continue;
+            const float dist = sqrtf(d2);
+            if (dist >= support)
+                continue;
+            const float x = dist * inv;
+            const float w = 1.0f - x * x * x;
+            wsum = wsum + w;
+            for (int a = 0; a < 3; ++a)
+                mean[a] = mean[a] + pj[a] * w;
+        }
+        if (wsum <= 0.0f) {
+            const float sc[3] = {iso, iso, iso};
+            write_aniso(out, n, i, iso_ax, sc);
+            continue;
+        }
+        for (int a = 0; a < 3; ++a)
+            mean[a] = mean[a] / wsum;
+        float C[3][3] = {{0.0f}};
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t j = list[(uint64_t)i * cap + k];
+            if (s->kinds[j] != CN_PARTICLE_KIND_FLUID)
+                continue;
+            const float *pj = s->pos + 4 * (size_t)j;
+            float d[3];
+            sub3(pi, pj, d);
+            const float d2 = dot3(d, d);
+            if (d2 <= 0.0f)
+                continue;
+            const float dist = sqrtf(d2);
+            if (dist >= support)
+                continue;
+            const float x = dist * inv;
+            const float w = 1.0f - x * x * x;
+            float cc[3];
+            sub3(pj, mean, cc);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    C[r][c] = C[r][c] + w * cc[r] * cc[c];
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                C[r][c] = C[r][c] / wsum;
+        C[0][1] = C[1][0] = 0.5f * (C[0][1] + C[1][0]);
+        C[0][2] = C[2][0] = 0.5f * (C[0][2] + C[2][0]);
+        C[1][2] = C[2][1] = 0.5f * (C[1][2] + C[2][1]);
+        float R[3][3], ev[3], ax[3][3];
+        jacobi_eigen(C, R, ev);
+        for (int k = 0; k < 3; ++k) {
+            const float col[3] = {R[0][k], R[1][k], R[2][k]};
+            aniso_normalize(col, k, ax[k]);
+        }
+        /* FluidAnisotropySortDescending, physics_fluid_anisotropy_common.hlsli:138-156 */
+        static const int order[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int o = 0; o < 3; ++o) {
+            const int a = order[o][0], b = order[o][1];
+            if (ev[a] < ev[b]) {
+                const float t = ev[a];
+                ev[a] = ev[b];
+                ev[b] = t;
+                for (int c = 0; c < 3; ++c) {
+                    const float u = ax[a][c];
+                    ax[a][c] = ax[b][c];
+                    ax[b][c] = u;
+                }
+            }
+        }
+        float sc[3];
+        for (int k = 0; k < 3; ++k) {
+            const float l = sqrtf(ev[k]) * 1.1f;
+            sc[k] = minf(maxf(l, 1.0f * radius), 2.0f * radius);
+        }
+        write_aniso(out, n, i, ax, sc);
+    }
+}
+
 /* Particle -> incident constraint lists (physics_scene_gpu_state.cpp:3273-3360): per particle the
  * references in ascending constraint index; ref = (constraint << 2) | slot.  Bend corrections are
  * indexed bends first, then dihedrals at softBendCount + dihedralIndex. */
@@ -2057,6 +2260,9 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
             /* physics_solver.cpp:1269-1283 */
             cno_fluid_vorticity(s, cap, counts, list, dtmp);
             cno_fluid_vorticity_apply(s, dt, cap, counts, list, dtmp);
+            /* physics_solver.cpp:1285-1290; each substep overwrites the last one's output */
+            if (s->render_aniso && sub + 1 == substeps)
+                cno_fluid_render_anisotropy(s, cap, counts, list, s->render_aniso);
         }
         if (has_contacts)
             for (uint32_t v = 0; v < contact_iters; ++v)

@@ -51,6 +51,9 @@ typedef struct cno_scene {
     float *edge_corr;           /* 8 floats per edge (two float4) */
     float *bend_corr;           /* 16 floats per bend + dihedral (four float4) */
     float *tet_corr;            /* 16 floats per tet */
+    /* optional output: float4[3n], the render anisotropy of the last substep (axis1 of every
+     * particle, then axis2, then axis3; w = scale), or NULL */
+    float *render_aniso;
 } cno_scene;
 
 typedef struct cno_stats {
@@ -109,6 +112,9 @@ void cno_update_velocities(cno_scene *s, float dt, float *delta_acc);
 void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt,
                                      const uint32_t *bnd_ranges, const uint32_t *bnd_pairs);
 void cno_predict(cno_scene *s, float dt, const float *gravity);
+/* render anisotropy over the post-update neighbor lists; out float4[3n] */
+void cno_fluid_render_anisotropy(const cno_scene *s, uint32_t cap, const uint32_t *counts,
+                                 const uint32_t *list, float *out);
 /* vorticity confinement over the post-update neighbor lists; vort float4[n] = (curl, |curl|) */
 void cno_fluid_vorticity(const cno_scene *s, uint32_t cap, const uint32_t *counts, const uint32_t *list,
                          float *vort);

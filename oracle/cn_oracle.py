@@ -26,7 +26,8 @@ class CnoScene(C.Structure):
                 ("n_bends", C.c_uint32), ("bends", C.c_void_p),
                 ("n_dihedrals", C.c_uint32), ("dihedrals", C.c_void_p),
                 ("n_tets", C.c_uint32), ("tets", C.c_void_p),
-                ("edge_corr", C.c_void_p), ("bend_corr", C.c_void_p), ("tet_corr", C.c_void_p)]
+                ("edge_corr", C.c_void_p), ("bend_corr", C.c_void_p), ("tet_corr", C.c_void_p),
+                ("render_aniso", C.c_void_p)]
 
 
 class CnoStats(C.Structure):
@@ -66,6 +67,7 @@ def load():
             "cno_fluid_density": (None, [_P, _P, _P, _U, _U, _P, _P, _P, _P, _P]),
             "cno_fluid_vorticity": (None, [_P, _U, _P, _P, _P]),
             "cno_fluid_vorticity_apply": (None, [_P, C.c_float, _U, _P, _P, _P]),
+            "cno_fluid_render_anisotropy": (None, [_P, _U, _P, _P, _P]),
             "cno_step": (C.c_int, [_P, _P, _P]),
             "cno_threads": (C.c_int, []),
             "cno_set_threads": (None, [C.c_int]),
@@ -114,6 +116,13 @@ class OracleScene:
     @property
     def n(self):
         return self.pos.shape[0]
+
+    def enable_render_anisotropy(self) -> np.ndarray:
+        """Makes every step write the last substep's render anisotropy into the returned
+        (3, n, 4) array: axis1, axis2, axis3 of every particle, w = scale."""
+        self.render_aniso = np.zeros((3, self.n, 4), np.float32)
+        self.c.render_aniso = self.render_aniso.ctypes.data
+        return self.render_aniso
 
 
 def step(osc: OracleScene, params) -> CnoStats:
@@ -247,6 +256,17 @@ def fluid_vorticity_apply(osc: OracleScene, dt: float, counts, lists, vort):
     vort = np.ascontiguousarray(vort, np.float32)
     lib.cno_fluid_vorticity_apply(C.byref(osc.c), C.c_float(dt), lists.shape[1], counts.ctypes.data,
                                   lists.ctypes.data, vort.ctypes.data)
+
+
+def fluid_render_anisotropy(osc: OracleScene, counts, lists) -> np.ndarray:
+    """physics_fluid_build_render_anisotropy restated: (3, n, 4) = axis1..3 with their scales."""
+    lib = load()
+    lists = np.ascontiguousarray(lists, np.uint32)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros((3, osc.n, 4), np.float32)
+    lib.cno_fluid_render_anisotropy(C.byref(osc.c), lists.shape[1], counts.ctypes.data, lists.ctypes.data,
+                                    out.ctypes.data)
+    return out
 
 
 def threads() -> int:

@@ -67,3 +67,35 @@ def test_vorticity_truncated_lists_bitexact(solver, oracle):
     res, _ = step_pair(sc, params(cap=9, substeps=2, iters=2), 4, solver, oracle)
     print(res)
     assert all(res), res
+
+
+@pytest.mark.parametrize("builder", ["dam", "mixed"])
+def test_render_anisotropy_bitexact(solver, oracle, builder):
+    """physics_fluid_build_render_anisotropy.cs.hlsl after the step: frames and scales bit for bit,
+    the identity frame for solid and pinned particles; unreadable before a step writes them."""
+    if builder == "dam":
+        sc = swirl(scenes.dam_break_64k(side=12))
+    else:
+        sc = swirl(mixed_scene())
+        sc.fluid_mats = np.concatenate([cnhip.fluid_material(0.02, vorticity_confinement=0.8),
+                                        cnhip.fluid_material(0.02, viscosity=50.0, cohesion=0.5)])
+    p = params(cap=64, substeps=2, iters=3)
+    osc = oracle.OracleScene(sc)
+    want = osc.enable_render_anisotropy()
+    solver.reserve(sc.n, 64)
+    solver.upload(sc)
+    solver.render_anisotropy_enable(True)
+    with pytest.raises(cnhip.CnError):
+        solver.render_anisotropy()
+    for _ in range(3):
+        solver.step(p)
+        oracle.step(osc, p)
+        got = solver.render_anisotropy()
+        assert np.array_equal(got, want), np.abs(got - want).max()
+    fixed = (sc.kinds != cnhip.CN_PARTICLE_KIND_FLUID) | (sc.pos[:, 3] <= 1e-6)
+    if fixed.any():
+        assert np.array_equal(got[:, fixed, :3], np.broadcast_to(np.eye(3, dtype=np.float32)[:, None, :],
+                                                                (3, int(fixed.sum()), 3)))
+    pos, _, vel = solver.readback()
+    assert np.array_equal(pos, osc.pos) and np.array_equal(vel, osc.vel)
+    solver.render_anisotropy_enable(False)

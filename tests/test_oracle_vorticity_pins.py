@@ -141,3 +141,85 @@ def test_step_applies_confinement_only_when_enabled(oracle):
     assert np.array_equal(on.prev, off.prev)
     assert not np.array_equal(on.vel, off.vel)
     assert np.all(on.vel[:, 3] == 0.0)
+
+
+# ---- render anisotropy (physics_fluid_build_render_anisotropy.cs.hlsl:34-171) ----
+
+def _aniso_scene(pts, radius, h, kinds=None):
+    sc = scenes._fluid_scene(pts, radius, np.zeros(0, cnhip.COLLIDER_DTYPE), "aniso", {})
+    sc.fluid_mats["smoothing_radius"] = h
+    if kinds is not None:
+        sc.kinds = kinds.astype(np.uint32)
+    return sc
+
+
+def test_anisotropy_isotropic_cases(oracle):
+    # a lone fluid particle, a solid particle and a pinned fluid particle: identity frame, scale r
+    pts = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [0.01, 0.0, 0.0], [9.0, 9.0, 9.0]], np.float32)
+    kinds = np.array([1, 1, 0, 1])
+    sc = _aniso_scene(pts, 0.004, 0.05, kinds)
+    sc.pos[3, 3] = 0.0
+    osc = oracle.OracleScene(sc)
+    counts, lists = _lists(pts, 0.05, osc.pos[:, 3])
+    counts[3] = 7  # a pinned particle's stale count is never read
+    out = oracle.fluid_render_anisotropy(osc, counts, lists)
+    for i in (1, 2, 3):
+        np.testing.assert_array_equal(out[:, i, :3], np.eye(3, dtype=np.float32))
+        assert np.all(out[:, i, 3] == F(0.004))
+
+
+def test_anisotropy_of_a_line_and_a_plane(oracle):
+    r, h, s = 0.01, 0.1, 0.012
+    u = np.array([1.0, 2.0, 2.0]) / 3.0
+    t = np.array([-4, -3, -2, -1, 1, 2, 3, 4], np.float64) * s
+    line = np.concatenate([[[0.5, 0.5, 0.5]], 0.5 + t[:, None] * u[None, :]]).astype(np.float32)
+    g = np.arange(-3, 4) * s
+    gx, gy = np.meshgrid(g, g)
+    plane = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], 1)
+    plane = plane[np.argsort(np.abs(plane).sum(1), kind="stable")] + np.array([2.0, 2.0, 2.0])
+    pts = np.concatenate([line, plane]).astype(np.float32)
+    sc = _aniso_scene(pts, r, h)
+    osc = oracle.OracleScene(sc)
+    counts, lists = _lists(pts, h)
+    out = oracle.fluid_render_anisotropy(osc, counts, lists)
+    # the line: one axis along u with 1.1 sqrt(weighted variance); the other scales clamp to r
+    p = osc.pos[:9, :3].astype(np.float64)
+    d = np.linalg.norm(p[1:] - p[0], axis=1)
+    w = 1.0 - (d / h) ** 3
+    var = (w * d ** 2).sum() / w.sum()
+    assert abs(abs(out[0, 0, :3].astype(np.float64) @ u) - 1.0) < 1e-5
+    assert out[0, 0, 3] == pytest.approx(min(max(1.1 * np.sqrt(var), r), 2 * r), rel=1e-5)
+    assert out[1, 0, 3] == F(r) and out[2, 0, 3] == F(r)
+    # the plane z = 2 (centre is its first point): the last axis is the normal, scales descend
+    c = 9
+    assert abs(abs(out[2, c, 2]) - 1.0) < 1e-6
+    assert out[0, c, 3] >= out[1, c, 3] >= out[2, c, 3] == F(r)
+    q = osc.pos[9:, :3].astype(np.float64)
+    dd = np.linalg.norm(q - q[0], axis=1)
+    m = (dd > 0) & (dd < h)
+    ww = 1.0 - (dd[m] / h) ** 3
+    mean = (ww[:, None] * q[m]).sum(0) / ww.sum()
+    cov = ((ww[:, None, None] * (q[m] - mean)[:, :, None] * (q[m] - mean)[:, None, :]).sum(0)) / ww.sum()
+    lam = np.sort(np.linalg.eigvalsh(cov))[::-1]
+    want = np.clip(1.1 * np.sqrt(np.maximum(lam, 0)), r, 2 * r)
+    assert out[:, c, 3] == pytest.approx(want, rel=1e-4)
+    # every frame is orthonormal
+    for i in (0, c):
+        ax = out[:, i, :3].astype(np.float64)
+        np.testing.assert_allclose(ax @ ax.T, np.eye(3), atol=1e-5)
+
+
+def test_step_writes_the_anisotropy_of_the_final_positions(oracle):
+    pts = _lattice(side=6, s=0.02)
+    sc = _scene(pts, 0.05, 0.0)
+    sc.radii[:] = F(0.02)  # scales clamp to [0.02, 0.04]: the lattice's spread lands inside
+    p = cnhip.default_params(dt=1 / 120, substeps=2, default_iterations=2, max_fluid_neighbors=96)
+    osc = oracle.OracleScene(sc)
+    out = osc.enable_render_anisotropy()
+    oracle.step(osc, p)
+    counts, lists = _lists(osc.pos[:, :3], 0.05)
+    again = oracle.fluid_render_anisotropy(osc, counts, lists)
+    # the lists here are in index order, the step's in walk order: the scales agree to rounding
+    # (axes of near-equal eigenvalues are not determined by the data, so only the scales compare)
+    np.testing.assert_allclose(out[:, :, 3], again[:, :, 3], rtol=1e-4)
+    assert np.any(out[:, :, 3] != out[0, 0, 3])
