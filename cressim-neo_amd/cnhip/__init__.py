@@ -206,6 +206,12 @@ class Solver:
         f, c = self.ranges[k]
         return pos[f:f + c], prev[f:f + c], vel[f:f + c]
 
+    def set_scene_colliders(self, k: int, colliders: np.ndarray):
+        """Replace scene k's static box colliders (COLLIDER_DTYPE, the count of its upload)."""
+        c = np.ascontiguousarray(colliders, COLLIDER_DTYPE)
+        self._check(self.lib.cn_scene_set_colliders(self._ctx, k, c.ctypes.data if len(c) else None, len(c)),
+                    "cn_scene_set_colliders")
+
     def render_anisotropy_enable(self, enable: bool = True):
         """Make every step end with the render anisotropy (cn_render_anisotropy_enable)."""
         self._check(self.lib.cn_render_anisotropy_enable(self._ctx, 1 if enable else 0), "cn_render_anisotropy_enable")

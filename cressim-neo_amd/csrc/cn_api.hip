@@ -47,6 +47,8 @@ struct cn_ctx {
     uint32_t fluid_scenes = 0;       // scenes with fluid particles (ranks [0, fluid_scenes))
     std::vector<uint32_t> scene_first;  // first particle of each scene, then the total
     std::vector<float> scene_cell;      // derived grid cell size of each scene
+    std::vector<uint32_t> scene_col_cap;  // collider slots of each scene: its uploaded collider count
+    std::vector<SceneInfo> scene_info;    // host copy of the device scene records
     bool graph_valid = false;         // the cached graphs refer to the current buffers
     bool graphs_enabled = false;
     struct Graph {
@@ -738,6 +740,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     std::vector<uint32_t> cls(n);
     ctx->scene_first.assign(count + 1, 0);
     ctx->scene_cell.assign(count, 0.0f);
+    ctx->scene_col_cap.assign(count, 0u);
     ctx->has_fluid = false;
     ctx->has_solid = false;
     ctx->any_pairs = false;
@@ -813,6 +816,9 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         sinfo[k].pairs = (scene_solid && scene_fluid) || two_groups ? 1u : 0u;
         ctx->any_pairs |= sinfo[k].pairs != 0u;
         col.insert(col.end(), sc.begin(), sc.end());
+        // slots for the disabled colliders too, so cn_scene_set_colliders can enable them in place
+        col.resize(col.size() + (tb->collider_count - sc.size()));
+        ctx->scene_col_cap[k] = tb->collider_count;
         // constraints, shifted to pool indices
         if (cv) {
             for (uint32_t e = 0; e < cv->edge_count; ++e) {
@@ -914,6 +920,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     s.nscenes = count;
     CN_HIP(ctx, dalloc(&s.scenes, sinfo.size()));
     CN_HIP(ctx, hipMemcpy(s.scenes, sinfo.data(), sinfo.size() * sizeof(SceneInfo), hipMemcpyHostToDevice));
+    ctx->scene_info = sinfo;
     {
         const std::vector<uint32_t> ones(count, 1u);
         CN_HIP(ctx, dalloc(&s.scene_active, count));
@@ -1544,6 +1551,29 @@ cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions
     CN_HIP(ctx, hipGetLastError());
     // host arrays (and the stage) may be reused as soon as the call returns
     if (host) CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CN_OK;
+}
+
+cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_collider *colliders, uint32_t count)
+{
+    if (!ctx || (count && !colliders)) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (scene >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const uint32_t cap = ctx->scene_col_cap[scene];
+    if (count != cap)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "collider count differs from the scene's upload; re-upload to add or remove colliders");
+    if (count == 0) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<Collider> sc = visit_ordered_colliders(colliders, count);
+    const uint32_t ncol = (uint32_t)sc.size();
+    sc.resize(cap);
+    SceneInfo &si = ctx->scene_info[scene];
+    si.ncol = ncol;
+    CN_HIP(ctx, hipMemcpyAsync(ctx->s.colliders + si.col0, sc.data(), cap * sizeof(Collider), hipMemcpyHostToDevice,
+                               ctx->stream));
+    CN_HIP(ctx, hipMemcpyAsync(ctx->s.scenes + scene, &si, sizeof(SceneInfo), hipMemcpyHostToDevice, ctx->stream));
+    // the host sources may go as soon as the call returns
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CN_OK;
 }
 

@@ -307,6 +307,15 @@ cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
 #define CN_STATE_DEVICE_POINTERS 1u
 cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions_inv_mass,
                              const float *velocities, uint32_t flags);
+/* Replaces the static box colliders of scene `scene` (0 for a single uploaded scene) with
+   `colliders`: the same count as the scene's upload, each of which may be moved, resized,
+   re-materialled, enabled or disabled; the visit order is rebuilt, and the next cn_step uses them.
+   Particles, topology, materials and the other scenes stay resident (the reference's
+   uploadWorldState, physics_scene_gpu_state.cpp:2348-2392, takes the world's collider set
+   whenever it uploads).  CN_ERR_INVALID_ARGUMENT if the count differs: adding or removing
+   colliders needs a re-upload. */
+cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_collider *colliders,
+                                 uint32_t count);
 /* Render anisotropy (buildFluidRenderAnisotropy, physics_pass_dispatcher.cpp:3098-3138 with
    physics_fluid_build_render_anisotropy.cs.hlsl:34-171, the renderer's fluidAnisotropy1..3
    buffers).  When enabled, every cn_step / cn_step_scenes with fluid ends by writing, per

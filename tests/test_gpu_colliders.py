@@ -1,0 +1,72 @@
+"""Collider edits without a re-upload (cn_scene_set_colliders): after moving, resizing and
+disabling static boxes, the steps match the oracle stepping the edited scene bit for bit, in a
+single scene and in one scene of a pool (the others untouched); a changed count is refused."""
+import numpy as np
+import pytest
+
+import cnhip
+from cnhip import scenes
+
+from test_gpu_fluid_parity import params, small_cloud
+
+pytestmark = pytest.mark.gpu
+
+
+def edited(cols):
+    new = scenes.container((0.0, 0.0, 0.0), (0.6, 0.42, 0.6))   # a lower lid
+    assert len(new) == len(cols)
+    new["center"][0] += np.float32(0.03)                         # the x-low wall moved inward
+    new["enabled"][5] = 0                                        # the z-high wall removed
+    return new
+
+
+def test_set_colliders_single_scene_bitexact(solver, oracle):
+    sc = small_cloud(1500, seed=7)
+    p = params(cap=64, substeps=2, iters=3)
+    osc = oracle.OracleScene(sc)
+    keep = sc.copy()
+    keep.colliders = sc.colliders.copy()                         # OracleScene aliases the array
+    old = oracle.OracleScene(keep)                               # keeps the uploaded colliders
+    solver.reserve(sc.n, 64)
+    solver.upload(sc)
+    for _ in range(2):
+        solver.step(p)
+        oracle.step(osc, p)
+        oracle.step(old, p)
+    new = edited(sc.colliders)
+    solver.set_scene_colliders(0, new)
+    osc.colliders[:] = new
+    for s in range(4):
+        solver.step(p)
+        oracle.step(osc, p)
+        pos, _, vel = solver.readback()
+        assert np.array_equal(pos, osc.pos) and np.array_equal(vel, osc.vel), s
+        oracle.step(old, p)
+    assert not np.array_equal(old.pos, osc.pos)                 # the edit took effect
+    with pytest.raises(cnhip.CnError):
+        solver.set_scene_colliders(0, new[:-1])
+
+
+def test_set_colliders_in_pool_bitexact(oracle):
+    ses = [small_cloud(900, seed=11), small_cloud(700, seed=12)]
+    p = params(cap=64, substeps=2, iters=2)
+    oscs = [oracle.OracleScene(sc) for sc in ses]
+    pool = cnhip.Solver(0)
+    try:
+        pool.reserve(sum(sc.n for sc in ses), 64)
+        pool.upload_batch(ses)
+        pool.step(p)
+        for o in oscs:
+            oracle.step(o, p)
+        new = edited(ses[1].colliders)
+        pool.set_scene_colliders(1, new)
+        oscs[1].colliders[:] = new
+        for _ in range(3):
+            pool.step(p)
+            for o in oscs:
+                oracle.step(o, p)
+        for k, o in enumerate(oscs):
+            pos, _, vel = pool.readback_scene(k)
+            assert np.array_equal(pos, o.pos) and np.array_equal(vel, o.vel), k
+    finally:
+        pool.close()
