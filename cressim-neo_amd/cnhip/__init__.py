@@ -212,6 +212,16 @@ class Solver:
         self._check(self.lib.cn_scene_set_colliders(self._ctx, k, c.ctypes.data if len(c) else None, len(c)),
                     "cn_scene_set_colliders")
 
+    def set_scene_materials(self, k: int, fluid_mats=None, particle_mats=None):
+        """Replace scene k's fluid materials (FLUID_MATERIAL_DTYPE) and/or particle materials
+        (float32 (m, 4)), each with the count of its upload; None leaves a table as it is."""
+        f = None if fluid_mats is None else np.ascontiguousarray(fluid_mats, FLUID_MATERIAL_DTYPE)
+        pm = None if particle_mats is None else np.ascontiguousarray(particle_mats, np.float32).reshape(-1, 4)
+        self._check(self.lib.cn_scene_set_materials(
+            self._ctx, k, f.ctypes.data if f is not None and len(f) else None, 0 if f is None else len(f),
+            pm.ctypes.data if pm is not None and len(pm) else None, 0 if pm is None else len(pm)),
+            "cn_scene_set_materials")
+
     def render_anisotropy_enable(self, enable: bool = True):
         """Make every step end with the render anisotropy (cn_render_anisotropy_enable)."""
         self._check(self.lib.cn_render_anisotropy_enable(self._ctx, 1 if enable else 0), "cn_render_anisotropy_enable")

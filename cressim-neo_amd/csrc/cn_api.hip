@@ -49,6 +49,9 @@ struct cn_ctx {
     std::vector<float> scene_cell;      // derived grid cell size of each scene
     std::vector<uint32_t> scene_col_cap;  // collider slots of each scene: its uploaded collider count
     std::vector<SceneInfo> scene_info;    // host copy of the device scene records
+    std::vector<uint32_t> scene_fm;       // per scene: first fluid material, count (pairs)
+    std::vector<uint32_t> scene_pm;       // per scene: first particle material, count (pairs)
+    std::vector<cn_fluid_material> fm_host;  // host copy of the pool's fluid material table
     bool graph_valid = false;         // the cached graphs refer to the current buffers
     bool graphs_enabled = false;
     struct Graph {
@@ -741,6 +744,8 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->scene_first.assign(count + 1, 0);
     ctx->scene_cell.assign(count, 0.0f);
     ctx->scene_col_cap.assign(count, 0u);
+    ctx->scene_fm.assign(2 * (size_t)count, 0u);
+    ctx->scene_pm.assign(2 * (size_t)count, 0u);
     ctx->has_fluid = false;
     ctx->has_solid = false;
     ctx->any_pairs = false;
@@ -754,6 +759,10 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         const cn_constraint_view *cv = descs[k].constraints;
         const uint32_t m = pv->count;
         const uint32_t fm0 = (uint32_t)fmh.size(), pm0 = (uint32_t)(pmh.size() / 4);
+        ctx->scene_fm[2 * k] = fm0;
+        ctx->scene_fm[2 * k + 1] = tb->fluid_material_count;
+        ctx->scene_pm[2 * k] = pm0;
+        ctx->scene_pm[2 * k + 1] = tb->particle_material_count;
         fmh.insert(fmh.end(), tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
         pmh.insert(pmh.end(), tb->particle_materials, tb->particle_materials + 4 * tb->particle_material_count);
         std::memcpy(pos.data() + 4ull * base, pv->positions_inv_mass, 4ull * m * sizeof(float));
@@ -921,6 +930,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, dalloc(&s.scenes, sinfo.size()));
     CN_HIP(ctx, hipMemcpy(s.scenes, sinfo.data(), sinfo.size() * sizeof(SceneInfo), hipMemcpyHostToDevice));
     ctx->scene_info = sinfo;
+    ctx->fm_host = fmh;
     {
         const std::vector<uint32_t> ones(count, 1u);
         CN_HIP(ctx, dalloc(&s.scene_active, count));
@@ -1573,6 +1583,47 @@ cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_colli
                                ctx->stream));
     CN_HIP(ctx, hipMemcpyAsync(ctx->s.scenes + scene, &si, sizeof(SceneInfo), hipMemcpyHostToDevice, ctx->stream));
     // the host sources may go as soon as the call returns
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CN_OK;
+}
+
+cn_status cn_scene_set_materials(cn_ctx *ctx, uint32_t scene, const cn_fluid_material *fluid_materials,
+                                 uint32_t fluid_count, const float *particle_materials, uint32_t particle_count)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (scene >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const uint32_t f0 = ctx->scene_fm[2 * scene], fn = ctx->scene_fm[2 * scene + 1];
+    const uint32_t p0 = ctx->scene_pm[2 * scene], pn = ctx->scene_pm[2 * scene + 1];
+    if (fluid_materials && fluid_count != fn)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid material count differs from the scene's upload");
+    if (particle_materials && particle_count != pn)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "particle material count differs from the scene's upload");
+    if (fluid_materials)
+        for (uint32_t k = 0; k < fn; ++k)
+            if (std::memcmp(&fluid_materials[k].smoothing_radius, &ctx->fm_host[f0 + k].smoothing_radius,
+                            sizeof(float)) != 0)
+                return fail(ctx, CN_ERR_UNSUPPORTED, "a new smoothing radius changes the scene's grid; re-upload");
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    if (fluid_materials && fn) {
+        std::copy(fluid_materials, fluid_materials + fn, ctx->fm_host.begin() + f0);
+        CN_HIP(ctx, hipMemcpyAsync(ctx->s.fluid_mats + f0, fluid_materials, fn * sizeof(cn_fluid_material),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (particle_materials && pn)
+        CN_HIP(ctx, hipMemcpyAsync(ctx->s.part_mats + p0, particle_materials, pn * sizeof(float4),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    // the flags that pick kernels follow the whole table (the passes still test each particle's
+    // own material), and a captured graph holds the old choice
+    bool tension = false, vort = false;
+    for (const cn_fluid_material &m : ctx->fm_host) {
+        tension |= (m.surface_tension > 0.0f ? m.surface_tension : 0.0f) > kEps;
+        vort |= std::max(m.vorticity_confinement, 0.0f) > kEps;
+    }
+    if (tension != ctx->has_tension || vort != ctx->has_vorticity) ctx->graph_valid = false;
+    ctx->has_tension = tension;
+    ctx->has_vorticity = vort;
+    // the host tables may go as soon as the call returns
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CN_OK;
 }

@@ -316,6 +316,15 @@ cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions
    colliders needs a re-upload. */
 cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_collider *colliders,
                                  uint32_t count);
+/* Replaces the material tables of scene `scene` (0 for a single uploaded scene): its fluid
+   materials (cn_fluid_material_from_desc records) and particle materials (float4 each), each with
+   the count of the scene's upload; a NULL table is left as it is.  The next cn_step uses them
+   (the reference's uploadWorldState, physics_scene_gpu_state.cpp:2348-2392, takes the world's
+   material tables whenever it uploads).  CN_ERR_UNSUPPORTED if a fluid material's smoothing
+   radius changes: that changes the scene's grid cell size, which needs a re-upload. */
+cn_status cn_scene_set_materials(cn_ctx *ctx, uint32_t scene, const cn_fluid_material *fluid_materials,
+                                 uint32_t fluid_count, const float *particle_materials,
+                                 uint32_t particle_count);
 /* Render anisotropy (buildFluidRenderAnisotropy, physics_pass_dispatcher.cpp:3098-3138 with
    physics_fluid_build_render_anisotropy.cs.hlsl:34-171, the renderer's fluidAnisotropy1..3
    buffers).  When enabled, every cn_step / cn_step_scenes with fluid ends by writing, per

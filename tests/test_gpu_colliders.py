@@ -70,3 +70,38 @@ def test_set_colliders_in_pool_bitexact(oracle):
             assert np.array_equal(pos, o.pos) and np.array_equal(vel, o.vel), k
     finally:
         pool.close()
+
+
+def test_set_materials_bitexact(solver, oracle):
+    """cn_scene_set_materials: new viscosity, cohesion, surface tension (switching the tension
+    passes on), vorticity and particle damping take effect on the next step, bit for bit with the
+    oracle stepping the edited tables; a new smoothing radius is refused."""
+    from test_gpu_fluid_parity import mixed_scene
+    sc = mixed_scene(2500, seed=31)
+    sc.fluid_mats = sc.fluid_mats.copy()
+    sc.part_mats = sc.part_mats.copy()
+    p = params(cap=64, substeps=2, iters=3)
+    osc = oracle.OracleScene(sc)
+    solver.reserve(sc.n, 64)
+    solver.upload(sc)
+    for _ in range(2):
+        solver.step(p)
+        oracle.step(osc, p)
+    r = 0.02
+    fm = np.concatenate([cnhip.fluid_material(r, viscosity=5.0, vorticity_confinement=0.7),
+                         cnhip.fluid_material(r, viscosity=20.0, cohesion=0.9, surface_tension=0.4)])
+    pm = np.array([[0.1, 0.0, 0.3, -1.0]], np.float32)
+    solver.set_scene_materials(0, fm, pm)
+    osc.fluid_mats[:] = fm
+    osc.part_mats[:] = pm
+    for s in range(3):
+        solver.step(p)
+        oracle.step(osc, p)
+        pos, _, vel = solver.readback()
+        assert np.array_equal(pos, osc.pos) and np.array_equal(vel, osc.vel), s
+    bad = fm.copy()
+    bad["smoothing_radius"][0] *= np.float32(1.5)
+    with pytest.raises(cnhip.CnError):
+        solver.set_scene_materials(0, bad)
+    with pytest.raises(cnhip.CnError):
+        solver.set_scene_materials(0, fm[:1])
