@@ -1,6 +1,8 @@
-"""Collider edits without a re-upload (cn_scene_set_colliders): after moving, resizing and
-disabling static boxes, the steps match the oracle stepping the edited scene bit for bit, in a
-single scene and in one scene of a pool (the others untouched); a changed count is refused."""
+"""Scene edits without a re-upload: colliders (cn_scene_set_colliders: moved, resized, disabled
+boxes), material tables (cn_scene_set_materials) and constraint parameters
+(cn_scene_set_constraints).  After each edit the steps match the oracle stepping the edited scene
+bit for bit, in a single scene and in scenes of a pool; changed counts, a new smoothing radius and
+constraints on other particles are refused."""
 import numpy as np
 import pytest
 
@@ -105,3 +107,59 @@ def test_set_materials_bitexact(solver, oracle):
         solver.set_scene_materials(0, bad)
     with pytest.raises(cnhip.CnError):
         solver.set_scene_materials(0, fm[:1])
+
+
+def test_set_constraints_bitexact(oracle):
+    """cn_scene_set_constraints: new compliances, rest values and edge enable flags of a tet block
+    and a cloth in a pool take effect on the next step, bit for bit with the oracle stepping the
+    edited constraints; other particles or other counts are refused."""
+    ses = [scenes.batch_scene(1500, 1, seed=41), scenes.batch_scene(1200, 2, seed=42)]
+    p = params(cap=64, substeps=2, iters=3)
+    oscs = [oracle.OracleScene(sc) for sc in ses]
+    pool = cnhip.Solver(0)
+    try:
+        pool.reserve(sum(sc.n for sc in ses), 64)
+        pool.upload_batch(ses)
+        for _ in range(2):
+            pool.step(p)
+            for o in oscs:
+                oracle.step(o, p)
+        eds = [sc.copy() for sc in ses]
+        for ed in eds:
+            if ed.edges is not None and len(ed.edges):
+                ed.edges["rest_length"] *= np.float32(1.05)
+                ed.edges["compliance"] = np.float32(2e-5)
+                ed.edges["enabled"][::7] = 0
+            if ed.tets is not None and len(ed.tets):
+                ed.tets["rest_volume"] *= np.float32(0.9)
+                ed.tets["compliance"] = np.float32(1e-5)
+            if ed.bends is not None and len(ed.bends):
+                ed.bends["compliance"] = np.float32(3e-3)
+            if ed.dihedrals is not None and len(ed.dihedrals):
+                ed.dihedrals["rest_angle"] += np.float32(0.1)
+                ed.dihedrals["compliance"] = np.float32(1e-3)
+        for k, (ed, o) in enumerate(zip(eds, oscs)):
+            pool.set_scene_constraints(k, ed)
+            e, b, d, t = ed.constraint_arrays()
+            o.edges[:] = e
+            o.bends[:] = b
+            o.dihedrals[:] = d
+            o.tets[:] = t
+        for s in range(3):
+            pool.step(p)
+            for o in oscs:
+                oracle.step(o, p)
+            for k, o in enumerate(oscs):
+                pos, _, vel = pool.readback_scene(k)
+                assert np.array_equal(pos, o.pos) and np.array_equal(vel, o.vel), (s, k)
+        bad = eds[0].copy()
+        if bad.edges is not None and len(bad.edges):
+            bad.edges["b"][0] = bad.edges["a"][0]
+        else:
+            bad.tets["p"][0, 0] = bad.tets["p"][0, 1]
+        with pytest.raises(cnhip.CnError):
+            pool.set_scene_constraints(0, bad)
+        with pytest.raises(cnhip.CnError):
+            pool.set_scene_constraints(0, eds[1])
+    finally:
+        pool.close()
