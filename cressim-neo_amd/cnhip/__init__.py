@@ -222,6 +222,15 @@ class Solver:
             pm.ctypes.data if pm is not None and len(pm) else None, 0 if pm is None else len(pm)),
             "cn_scene_set_materials")
 
+    def set_scene_constraints(self, k: int, sc: Scene):
+        """Replace scene k's constraint parameters with those of sc's constraint arrays (same
+        counts, order and particles as the upload)."""
+        e, b, d, t = sc.constraint_arrays()
+        p = abi.ptr
+        cons = abi.ConstraintView(len(e), p(e) if len(e) else None, len(b), p(b) if len(b) else None,
+                                  len(d), p(d) if len(d) else None, len(t), p(t) if len(t) else None)
+        self._check(self.lib.cn_scene_set_constraints(self._ctx, k, C.byref(cons)), "cn_scene_set_constraints")
+
     def render_anisotropy_enable(self, enable: bool = True):
         """Make every step end with the render anisotropy (cn_render_anisotropy_enable)."""
         self._check(self.lib.cn_render_anisotropy_enable(self._ctx, 1 if enable else 0), "cn_render_anisotropy_enable")

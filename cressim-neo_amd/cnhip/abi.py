@@ -155,6 +155,7 @@ SIGNATURES = {
     "cn_scene_set_state": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]),
     "cn_scene_set_colliders": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32]),
     "cn_scene_set_materials": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "cn_scene_set_constraints": (C.c_int, [_P, C.c_uint32, C.c_void_p]),
     "cn_render_anisotropy_enable": (C.c_int, [_P, C.c_int]),
     "cn_render_anisotropy_read": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "cn_validate_meta": (C.c_int, [_P, C.POINTER(Meta)]),

@@ -52,6 +52,11 @@ struct cn_ctx {
     std::vector<uint32_t> scene_fm;       // per scene: first fluid material, count (pairs)
     std::vector<uint32_t> scene_pm;       // per scene: first particle material, count (pairs)
     std::vector<cn_fluid_material> fm_host;  // host copy of the pool's fluid material table
+    std::vector<uint32_t> scene_con;      // per scene: first edge, bend, dihedral, tet (4 words)
+    std::vector<EdgeDev> edges_host;      // host copies of the device constraint records
+    std::vector<BendDev> bends_host;
+    std::vector<DihedralDev> dih_host;
+    std::vector<TetDev> tets_host;
     bool graph_valid = false;         // the cached graphs refer to the current buffers
     bool graphs_enabled = false;
     struct Graph {
@@ -252,6 +257,26 @@ void build_incidence(uint32_t n, uint32_t groups, const uint32_t *counts, const 
                     out[ref_off + cur[p]++] = ((c + offset[g]) << 2) | k;
                 }
             }
+}
+
+// device forms of the constraint records, particle indices shifted by base: padding dropped,
+// max(compliance, 0) and max(rest volume, 0) taken here, a disabled edge carried as a NaN compliance
+float clamp0(float v) { return v > 0.0f ? v : 0.0f; }
+EdgeDev edge_dev(const cn_soft_edge &E, uint32_t base)
+{
+    return {E.a + base, E.b + base, E.rest_length, E.enabled ? clamp0(E.compliance) : std::nanf("")};
+}
+BendDev bend_dev(const cn_soft_bend &B, uint32_t base)
+{
+    return {B.p0 + base, B.p1 + base, B.p2 + base, B.rest_angle, clamp0(B.compliance)};
+}
+DihedralDev dihedral_dev(const cn_cloth_dihedral &D, uint32_t base)
+{
+    return {D.e0 + base, D.e1 + base, D.o0 + base, D.o1 + base, D.rest_angle, clamp0(D.compliance)};
+}
+TetDev tet_dev(const cn_soft_tet &T, uint32_t base)
+{
+    return {{T.p[0] + base, T.p[1] + base, T.p[2] + base, T.p[3] + base}, clamp0(T.rest_volume), clamp0(T.compliance)};
 }
 
 // host restatement of the static-collider visit order: enabled colliders, Morton codes of their
@@ -745,6 +770,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->scene_cell.assign(count, 0.0f);
     ctx->scene_col_cap.assign(count, 0u);
     ctx->scene_fm.assign(2 * (size_t)count, 0u);
+    ctx->scene_con.assign(4 * (size_t)count, 0u);
     ctx->scene_pm.assign(2 * (size_t)count, 0u);
     ctx->has_fluid = false;
     ctx->has_solid = false;
@@ -829,6 +855,10 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         col.resize(col.size() + (tb->collider_count - sc.size()));
         ctx->scene_col_cap[k] = tb->collider_count;
         // constraints, shifted to pool indices
+        ctx->scene_con[4 * k] = (uint32_t)edges.size();
+        ctx->scene_con[4 * k + 1] = (uint32_t)bends.size();
+        ctx->scene_con[4 * k + 2] = (uint32_t)dih.size();
+        ctx->scene_con[4 * k + 3] = (uint32_t)tets.size();
         if (cv) {
             for (uint32_t e = 0; e < cv->edge_count; ++e) {
                 cn_soft_edge E = cv->edges[e];
@@ -1003,25 +1033,22 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         {
             // device forms: padding dropped, max(compliance, 0) and max(rest volume, 0) taken here,
             // a disabled edge carried as a NaN compliance
-            auto clamp0 = [](float v) { return v > 0.0f ? v : 0.0f; };
             std::vector<EdgeDev> de(s.ne);
-            for (uint32_t e = 0; e < s.ne; ++e)
-                de[e] = {edges[e].a, edges[e].b, edges[e].rest_length,
-                         edges[e].enabled ? clamp0(edges[e].compliance) : std::nanf("")};
+            for (uint32_t e = 0; e < s.ne; ++e) de[e] = edge_dev(edges[e], 0);
             std::vector<BendDev> db(s.nb);
-            for (uint32_t q = 0; q < s.nb; ++q)
-                db[q] = {bends[q].p0, bends[q].p1, bends[q].p2, bends[q].rest_angle, clamp0(bends[q].compliance)};
+            for (uint32_t q = 0; q < s.nb; ++q) db[q] = bend_dev(bends[q], 0);
             std::vector<DihedralDev> dd(s.nd);
-            for (uint32_t q = 0; q < s.nd; ++q)
-                dd[q] = {dih[q].e0, dih[q].e1, dih[q].o0, dih[q].o1, dih[q].rest_angle, clamp0(dih[q].compliance)};
+            for (uint32_t q = 0; q < s.nd; ++q) dd[q] = dihedral_dev(dih[q], 0);
             std::vector<TetDev> dt(s.nt);
-            for (uint32_t t = 0; t < s.nt; ++t)
-                dt[t] = {{tets[t].p[0], tets[t].p[1], tets[t].p[2], tets[t].p[3]}, clamp0(tets[t].rest_volume),
-                         clamp0(tets[t].compliance)};
+            for (uint32_t t = 0; t < s.nt; ++t) dt[t] = tet_dev(tets[t], 0);
             if (s.ne) CN_HIP(ctx, hipMemcpy(s.edges, de.data(), s.ne * sizeof(EdgeDev), hipMemcpyHostToDevice));
             if (s.nb) CN_HIP(ctx, hipMemcpy(s.bends, db.data(), s.nb * sizeof(BendDev), hipMemcpyHostToDevice));
             if (s.nd) CN_HIP(ctx, hipMemcpy(s.dihedrals, dd.data(), s.nd * sizeof(DihedralDev), hipMemcpyHostToDevice));
             if (s.nt) CN_HIP(ctx, hipMemcpy(s.tets, dt.data(), s.nt * sizeof(TetDev), hipMemcpyHostToDevice));
+            ctx->edges_host = std::move(de);
+            ctx->bends_host = std::move(db);
+            ctx->dih_host = std::move(dd);
+            ctx->tets_host = std::move(dt);
         }
         const size_t nlam = (size_t)s.ne + s.nb + s.nd + s.nt;
         CN_HIP(ctx, dalloc(&s.lam, nlam));
@@ -1626,6 +1653,67 @@ cn_status cn_scene_set_materials(cn_ctx *ctx, uint32_t scene, const cn_fluid_mat
     // the host tables may go as soon as the call returns
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CN_OK;
+}
+
+extern "C++" {
+// Converts and writes records [first, first + count) of one constraint type (host copy and device).
+template <class Dev, class Src, class Conv>
+static cn_status set_records(cn_ctx *ctx, std::vector<Dev> &host, Dev *dev, uint32_t first, uint32_t count,
+                             const Src *src, uint32_t base, Conv conv)
+{
+    if (!count) return CN_OK;
+    for (uint32_t q = 0; q < count; ++q) host[first + q] = conv(src[q], base);
+    CN_HIP(ctx, hipMemcpyAsync(dev + first, host.data() + first, count * sizeof(Dev), hipMemcpyHostToDevice,
+                               ctx->stream));
+    return CN_OK;
+}
+}  // extern "C++"
+
+cn_status cn_scene_set_constraints(cn_ctx *ctx, uint32_t scene, const cn_constraint_view *cv)
+{
+    if (!ctx || !cv) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if (scene >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const uint32_t *c0 = &ctx->scene_con[4 * scene];
+    const bool last = scene + 1 == ctx->nscenes;
+    const uint32_t *c1 = last ? nullptr : &ctx->scene_con[4 * (scene + 1)];
+    const SceneBuffers &s = ctx->s;
+    const uint32_t ne = (last ? s.ne : c1[0]) - c0[0], nb = (last ? s.nb : c1[1]) - c0[1];
+    const uint32_t nd = (last ? s.nd : c1[2]) - c0[2], nt = (last ? s.nt : c1[3]) - c0[3];
+    if (cv->edge_count != ne || cv->bend_count != nb || cv->dihedral_count != nd || cv->tet_count != nt)
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "constraint counts differ from the scene's upload");
+    if ((ne && !cv->edges) || (nb && !cv->bends) || (nd && !cv->dihedrals) || (nt && !cv->tets))
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null constraint array");
+    // every record keeps its particles (the incidence lists and record maps built at upload stay
+    // valid); all are checked before any is written
+    const uint32_t base = ctx->scene_first[scene];
+    for (uint32_t q = 0; q < ne; ++q) {
+        const EdgeDev a = edge_dev(cv->edges[q], base), &b = ctx->edges_host[c0[0] + q];
+        if (a.a != b.a || a.b != b.b) return fail(ctx, CN_ERR_UNSUPPORTED, "an edge changed its particles; re-upload");
+    }
+    for (uint32_t q = 0; q < nb; ++q) {
+        const BendDev a = bend_dev(cv->bends[q], base), &b = ctx->bends_host[c0[1] + q];
+        if (a.p0 != b.p0 || a.p1 != b.p1 || a.p2 != b.p2)
+            return fail(ctx, CN_ERR_UNSUPPORTED, "a bend changed its particles; re-upload");
+    }
+    for (uint32_t q = 0; q < nd; ++q) {
+        const DihedralDev a = dihedral_dev(cv->dihedrals[q], base), &b = ctx->dih_host[c0[2] + q];
+        if (a.e0 != b.e0 || a.e1 != b.e1 || a.o0 != b.o0 || a.o1 != b.o1)
+            return fail(ctx, CN_ERR_UNSUPPORTED, "a dihedral changed its particles; re-upload");
+    }
+    for (uint32_t q = 0; q < nt; ++q) {
+        const TetDev a = tet_dev(cv->tets[q], base), &b = ctx->tets_host[c0[3] + q];
+        if (std::memcmp(a.p, b.p, sizeof(a.p)) != 0)
+            return fail(ctx, CN_ERR_UNSUPPORTED, "a tet changed its particles; re-upload");
+    }
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    cn_status st = set_records(ctx, ctx->edges_host, s.edges, c0[0], ne, cv->edges, base, edge_dev);
+    if (st == CN_OK) st = set_records(ctx, ctx->bends_host, s.bends, c0[1], nb, cv->bends, base, bend_dev);
+    if (st == CN_OK) st = set_records(ctx, ctx->dih_host, s.dihedrals, c0[2], nd, cv->dihedrals, base, dihedral_dev);
+    if (st == CN_OK) st = set_records(ctx, ctx->tets_host, s.tets, c0[3], nt, cv->tets, base, tet_dev);
+    // the host sources may go as soon as the call returns
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
 }
 
 cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out)

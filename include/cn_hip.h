@@ -325,6 +325,14 @@ cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_colli
 cn_status cn_scene_set_materials(cn_ctx *ctx, uint32_t scene, const cn_fluid_material *fluid_materials,
                                  uint32_t fluid_count, const float *particle_materials,
                                  uint32_t particle_count);
+/* Replaces the parameters of scene `scene`'s constraints (rest lengths, angles and volumes,
+   compliances, edge enable flags): `cv` holds the scene's constraints in the counts and order of
+   its upload, each on the same particles (the scene's own indices).  The next cn_step uses them;
+   the incidence lists built at upload stay.  CN_ERR_INVALID_ARGUMENT if a count differs,
+   CN_ERR_UNSUPPORTED if a constraint names other particles (a topology change needs a re-upload;
+   the reference re-uploads its soft topology when its revision changes,
+   physics_scene_gpu_state.cpp:3255). */
+cn_status cn_scene_set_constraints(cn_ctx *ctx, uint32_t scene, const cn_constraint_view *cv);
 /* Render anisotropy (buildFluidRenderAnisotropy, physics_pass_dispatcher.cpp:3098-3138 with
    physics_fluid_build_render_anisotropy.cs.hlsl:34-171, the renderer's fluidAnisotropy1..3
    buffers).  When enabled, every cn_step / cn_step_scenes with fluid ends by writing, per
