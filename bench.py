@@ -231,6 +231,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no cn_autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,GRID",
                     help="set the fluid launch shape (cn_tune_set) instead of autotuning it")
+    ap.add_argument("--vorticity", type=float, default=0.0, metavar="C",
+                    help="vorticity confinement of the 4M workload's fluid material (off in the headline)")
+    ap.add_argument("--render-anisotropy", action="store_true",
+                    help="end every 4M step with the render anisotropy (off in the headline)")
     ap.add_argument("--serve", action="store_true", help="only run the served workload (2048 sessions)")
     ap.add_argument("--serve-sessions", type=int, default=2048)
     ap.add_argument("--serve-duration", type=float, default=2.0)
@@ -260,12 +264,16 @@ def main():
     from cnhip import scenes
 
     sc = scenes.large_pbf(args.n)
+    if args.vorticity > 0.0:
+        sc.fluid_mats = cnhip.fluid_material(float(sc.radii[0]), vorticity_confinement=args.vorticity)
     km = cnhip.CN_KEY_MORTON if args.key_mode == "morton" else cnhip.CN_KEY_HASH
     p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP,
                              key_mode=km)
     s = cnhip.Solver(0)
     s.reserve(sc.n, CAP)
     s.upload(sc)
+    if args.render_anisotropy:
+        s.render_anisotropy_enable(True)
     print(f"bench: {sc.n} particles uploaded; warmup {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.step(p)
@@ -326,6 +334,7 @@ def main():
         "config": {"workload": "large_pbf_4194304_random_fill_4x2x2m", "particles": sc.n,
                    "substeps_per_step": SUBSTEPS, "iterations": ITERS, "neighbor_cap": CAP,
                    "key_mode": args.key_mode, "mean_neighbors": round(kbar, 3),
+                   "vorticity_confinement": args.vorticity, "render_anisotropy": args.render_anisotropy,
                    "max_neighbors": meta.max_fluid_neighbor_count},
         "autotune": tuned,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
