@@ -65,6 +65,11 @@ def load():
             "cno_fluid_neighbors": (None, [_P, C.c_float, _P, _P, _P, _P, _U, _P, _U, _U, _P, _P, _P]),
             "cno_predict": (None, [_P, C.c_float, _P]),
             "cno_fluid_density": (None, [_P, _P, _P, _U, _U, _P, _P, _P, _P, _P]),
+            "cno_fluid_delta": (None, [_P, _P, C.c_float, _P, _U, _U, _P, _P, _P, _P, _P, _P]),
+            "cno_fluid_apply": (None, [_P, _P, _U, _P, _P]),
+            "cno_fluid_clamp": (None, [_P, _P, _P, _P, _P]),
+            "cno_update_velocities": (None, [_P, C.c_float, _P]),
+            "cno_project_boundary_velocities": (None, [_P, _P, C.c_float, _P, _P]),
             "cno_fluid_vorticity": (None, [_P, _U, _P, _P, _P]),
             "cno_fluid_vorticity_apply": (None, [_P, C.c_float, _U, _P, _P, _P]),
             "cno_fluid_render_anisotropy": (None, [_P, _U, _P, _P, _P]),
@@ -235,6 +240,71 @@ def fluid_density(osc: OracleScene, params, counts, lists, dacc=None, iteration=
     lib.cno_fluid_density(C.byref(osc.c), cfg.ctypes.data, dacc.ctypes.data, iteration, cap,
                           counts.ctypes.data, lists.ctypes.data, br.ctypes.data, bp.ctypes.data, nc.ctypes.data)
     return nc
+
+
+def _boundary(n, bnd_ranges, bnd_pairs):
+    br = np.zeros((n, 2), np.uint32) if bnd_ranges is None else np.ascontiguousarray(bnd_ranges, np.uint32)
+    bp = np.zeros(1, np.uint32) if bnd_pairs is None or len(bnd_pairs) == 0 else \
+        np.ascontiguousarray(bnd_pairs, np.uint32)
+    return br, bp
+
+
+def fluid_delta(osc: OracleScene, cfg, dt: float, counts, lists, nc, dacc=None, iteration=0,
+                bnd_ranges=None, bnd_pairs=None) -> np.ndarray:
+    """physics_fluid_compute_delta_positions restated: (delta, interaction weight) per particle.
+    nc holds each particle's (surface normal, C) from the density pass; bnd_ranges (n, 2) the
+    (offset, count) of each particle's static colliders in bnd_pairs."""
+    lib = load()
+    n = osc.n
+    cfg = np.ascontiguousarray(cfg, np.float32)
+    dacc = np.zeros((n, 4), np.float32) if dacc is None else np.ascontiguousarray(dacc, np.float32)
+    br, bp = _boundary(n, bnd_ranges, bnd_pairs)
+    lists = np.ascontiguousarray(lists, np.uint32)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    nc = np.ascontiguousarray(nc, np.float32)
+    out = np.zeros((n, 4), np.float32)
+    lib.cno_fluid_delta(C.byref(osc.c), cfg.ctypes.data, C.c_float(dt), dacc.ctypes.data, iteration, lists.shape[1],
+                        counts.ctypes.data, lists.ctypes.data, br.ctypes.data, bp.ctypes.data, nc.ctypes.data,
+                        out.ctypes.data)
+    return out
+
+
+def fluid_apply(osc: OracleScene, cfg, iteration: int, din, dacc) -> np.ndarray:
+    """physics_fluid_apply_delta_positions restated: the accumulated delta after one iteration."""
+    lib = load()
+    cfg = np.ascontiguousarray(cfg, np.float32)
+    din = np.ascontiguousarray(din, np.float32)
+    acc = np.ascontiguousarray(dacc, np.float32).copy()
+    lib.cno_fluid_apply(C.byref(osc.c), cfg.ctypes.data, iteration, din.ctypes.data, acc.ctypes.data)
+    return acc
+
+
+def fluid_clamp(osc: OracleScene, dacc, bnd_ranges, bnd_pairs) -> np.ndarray:
+    """physics_fluid_clamp_boundary restated: the accumulated delta pushed out of the colliders."""
+    lib = load()
+    din = np.ascontiguousarray(dacc, np.float32)
+    br, bp = _boundary(osc.n, bnd_ranges, bnd_pairs)
+    out = np.zeros_like(din)
+    lib.cno_fluid_clamp(C.byref(osc.c), din.ctypes.data, br.ctypes.data, bp.ctypes.data, out.ctypes.data)
+    return out
+
+
+def update_velocities(osc: OracleScene, dt: float, dacc) -> np.ndarray:
+    """physics_particle_update_velocities restated, in place on osc (pos, vel); returns the
+    cleared delta buffer."""
+    lib = load()
+    acc = np.ascontiguousarray(dacc, np.float32).copy()
+    lib.cno_update_velocities(C.byref(osc.c), C.c_float(dt), acc.ctypes.data)
+    return acc
+
+
+def project_boundary_velocities(osc: OracleScene, cfg, dt: float, bnd_ranges, bnd_pairs):
+    """physics_fluid_project_boundary_velocities restated, in place on osc.vel."""
+    lib = load()
+    cfg = np.ascontiguousarray(cfg, np.float32)
+    br, bp = _boundary(osc.n, bnd_ranges, bnd_pairs)
+    lib.cno_project_boundary_velocities(C.byref(osc.c), cfg.ctypes.data, C.c_float(dt), br.ctypes.data,
+                                        bp.ctypes.data)
 
 
 def fluid_vorticity(osc: OracleScene, counts, lists) -> np.ndarray:

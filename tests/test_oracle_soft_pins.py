@@ -172,3 +172,111 @@ def test_soft_block_holds_volume_under_gravity(oracle):
     # free fall of the whole block: centre of mass follows g t^2 / 2 within float noise
     t = 20 / 120
     assert pts[:, 1].mean() == pytest.approx(float(sc.pos[:, 1].mean()) - 0.5 * 9.81 * t * t, abs=2e-3)
+
+
+# ---- XPBD magnitudes of bends and dihedrals --------------------------------------------------------
+# One constraint alone, unequal inverse masses, compliance > 0, 2 substeps x 3 iterations: a float64
+# XPBD run with the finite-difference gradient of the angle -- delta lambda = -(C + alpha lambda) /
+# (sum w |grad C|^2 + alpha), x_k += relaxation w_k delta lambda grad_k C, lambda zeroed each substep
+# (physics_soft_solve_bend_constraints.cs.hlsl:20-86, physics_cloth_solve_dihedral_constraints.cs.hlsl
+# :20-95, physics_clear_soft_constraint_state.cs.hlsl:14-54) -- pins the gradient scale, alpha, the
+# lambda accumulation and reset, the inverse-mass weights and the relaxation factor.
+
+K_SOFT_RELAX = 4 * 6 + 0      # kSoftInternalRelaxation: g_SoftSolverConfig0.x (physics_solver_config.hlsli)
+
+
+def _bend_theta(p):
+    u, v = p[0] - p[1], p[2] - p[1]
+    return math.acos(float(np.dot(u, v) / np.linalg.norm(u) / np.linalg.norm(v)))
+
+
+def _dihedral_theta(p, e0, e1, o0, o1):
+    a, b, c, d = p[o0], p[o1], p[e0], p[e1]       # p0 = opposite0, p1 = opposite1, p2 = edge0, p3 = edge1
+    edge = d - c
+    n0 = np.cross(c - a, d - a)
+    n1 = np.cross(d - b, c - b)
+    n0 /= np.linalg.norm(n0)
+    n1 /= np.linalg.norm(n1)
+    return math.atan2(float(np.dot(edge / np.linalg.norm(edge), np.cross(n0, n1))),
+                      min(max(float(np.dot(n0, n1)), -1.0), 1.0))
+
+
+def _fd_grad(f, p, eps=1e-7):
+    g = np.zeros_like(p)
+    for k in range(p.shape[0]):
+        for a in range(3):
+            d = np.zeros_like(p)
+            d[k, a] = eps
+            g[k, a] = (f(p + d) - f(p - d)) / (2 * eps)
+    return g
+
+
+def _xpbd_expected(p0, w, theta, rest, compliance, dt_frame, substeps, iters, relax, wrap=False):
+    x = p0.astype(np.float64).copy()
+    v = np.zeros_like(x)
+    dt = dt_frame / substeps
+    alpha = compliance / max(dt * dt, 1e-6)
+    for _ in range(substeps):
+        prev = x.copy()
+        x = x + v * dt * (w[:, None] > 1e-6)
+        lam = 0.0
+        for _ in range(iters):
+            g = _fd_grad(theta, x)
+            c = theta(x) - rest
+            if wrap:
+                c = c - 2 * math.pi * math.floor((c + math.pi) / (2 * math.pi))
+            dl = -(c + alpha * lam) / (float((w[:, None] * g * g).sum()) + alpha)
+            x = x + relax * w[:, None] * dl * g
+            lam += dl
+        v = (x - prev) / dt
+    return x
+
+
+def _soft_params(substeps=2, iters=3, relax=0.35):
+    p = _params(iters=iters, substeps=substeps)
+    p.solver_config[K_SOFT_RELAX] = relax
+    return p
+
+
+def test_bend_xpbd_magnitude(oracle):
+    pts = np.array([[-1.0, 0.35, 0.1], [0.0, 0.0, 0.0], [0.9, 0.4, -0.2]], np.float32)
+    w = np.array([1.0, 0.5, 2.0])
+    b = np.zeros(1, au.BEND_DTYPE)
+    b["p0"], b["p1"], b["p2"], b["rest_angle"], b["compliance"] = 0, 1, 2, 2.9, 2e-5
+    sc = _solid_scene(pts, 1.0, bends=b)
+    sc.pos[:, 3] = w
+    osc = oracle.OracleScene(sc)
+    p = _soft_params()
+    oracle.step(osc, p)
+    want = _xpbd_expected(pts, w, _bend_theta, 2.9, 2e-5, p.dt, 2, 3, 0.35)
+    moved = want - pts
+    assert np.abs(moved).max() > 1e-2
+    np.testing.assert_allclose(osc.pos[:, :3], want, rtol=0, atol=2e-6 * np.abs(moved).max() + 1e-6)
+
+
+@pytest.mark.parametrize("rest, lift", [(0.3, 0.45), (-3.0, -0.9)])
+def test_dihedral_xpbd_magnitude(oracle, rest, lift):
+    """A folded hinge with unequal masses; the second case's angle minus its rest angle lies
+    beyond pi, so the constraint wraps by 2 pi."""
+    quad = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1], [1, 0, 1]], np.float32)
+    _, d = au.cloth_constraints(quad, np.array([[0, 1, 3], [0, 3, 2]], np.uint32))
+    d["rest_angle"], d["compliance"] = rest, 5e-5
+    fold = quad.copy()
+    fold[2, 1] = lift
+    fold[1, 1] = -0.1
+    w = np.array([1.0, 0.25, 2.0, 0.7])
+    sc = _solid_scene(fold, 1.0, dihedrals=d)
+    sc.pos[:, 3] = w
+    osc = oracle.OracleScene(sc)
+    e0, e1, o0, o1 = (int(d[k][0]) for k in ("e0", "e1", "o0", "o1"))
+
+    def theta(p):
+        return _dihedral_theta(p, e0, e1, o0, o1)
+    c0 = theta(fold.astype(np.float64)) - rest
+    assert (abs(c0) > math.pi) == (rest < -2)
+    p = _soft_params()
+    oracle.step(osc, p)
+    want = _xpbd_expected(fold, w, theta, rest, 5e-5, p.dt, 2, 3, 0.35, wrap=True)
+    moved = want - fold
+    assert np.abs(moved).max() > 1e-2
+    np.testing.assert_allclose(osc.pos[:, :3], want, rtol=0, atol=2e-6 * np.abs(moved).max() + 1e-6)
