@@ -147,8 +147,7 @@ class Solver:
 
     # -- scene
     def reserve(self, n: int, max_fluid_neighbors: int = 128):
-        if self._nviews > 0:
-            raise RuntimeError("release the exported tensor views before reserve() reallocates their buffers")
+        self._refuse_while_viewed("reserve()")
         self._check(self.lib.cn_scene_reserve(self._ctx, n, max_fluid_neighbors), "cn_scene_reserve")
 
     @staticmethod
@@ -175,7 +174,14 @@ class Solver:
                                   len(d), p(d) if len(d) else None, len(t), p(t) if len(t) else None)
         return view, tables, cons, (arrs, e, b, d, t)
 
+    def _refuse_while_viewed(self, what: str):
+        # an upload may reallocate the buffers exported tensors alias (a device use-after-free),
+        # or lay another scene over them
+        if self._nviews > 0:
+            raise RuntimeError(f"release the exported tensor views before {what} replaces their buffers")
+
     def upload(self, sc: Scene):
+        self._refuse_while_viewed("upload()")
         self.scene = sc
         self.scenes = [sc]
         self.n = sc.n
@@ -185,6 +191,7 @@ class Solver:
 
     def upload_batch(self, scenes):
         """Upload independent scenes into one pool (cn_batch_upload); step() then steps them all."""
+        self._refuse_while_viewed("upload_batch()")
         self.scene = None
         self.scenes = list(scenes)
         views = [self._views(sc) for sc in self.scenes]
@@ -328,6 +335,7 @@ class Solver:
     def torch_views(self) -> dict:
         """``dlpack_views`` as torch tensors on the solver's device, without a copy."""
         import torch
+        abi.require_single_hip_runtime("torch_views")
         return {k: torch.from_dlpack(c) for k, c in self.dlpack_views().items()}
 
     def set_scene_state(self, k: int, pos, vel):
@@ -338,6 +346,7 @@ class Solver:
         self._check(self.lib.cn_batch_scene_range(self._ctx, k, C.byref(first), C.byref(count)), "cn_batch_scene_range")
         if hasattr(pos, "data_ptr") and getattr(pos, "is_cuda", False):
             import torch
+            abi.require_single_hip_runtime("set_scene_state")
             for t in (pos, vel):
                 if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != (count.value, 4):
                     raise ValueError(f"state tensors must be contiguous float32 ({count.value}, 4)")
@@ -359,6 +368,7 @@ class Solver:
         """The solver's HIP stream as a torch stream: order torch work after ``step`` with
         ``torch.cuda.current_stream().wait_stream(solver.torch_stream())``."""
         import torch
+        abi.require_single_hip_runtime("torch_stream")
         return torch.cuda.ExternalStream(self.device_views().stream, device=torch.device("cuda", self.device))
 
     # -- per-pass entries

@@ -172,14 +172,66 @@ SIGNATURES = {
 _lib = None
 
 
+def torch_hip_runtime() -> str | None:
+    """Path of the HIP runtime that torch bundles (``torch/lib/libamdhip64.so``), found without
+    importing torch; None when torch is not installed or bundles none."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.origin:
+        return None
+    p = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    return p if os.path.exists(p) else None
+
+
+def hip_runtimes_mapped() -> list:
+    """The distinct libamdhip64 files mapped into this process (from /proc/self/maps)."""
+    seen = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 6 and os.path.basename(parts[5]).startswith("libamdhip64"):
+                    seen.add(os.path.realpath(parts[5]))
+    except OSError:  # pragma: no cover - not Linux
+        return []
+    return sorted(seen)
+
+
+def require_single_hip_runtime(what: str):
+    """Raises CnLibraryError when two HIP runtimes are mapped: device pointers and streams of one
+    are not valid in the other, so handing the library's buffers to torch would be undefined."""
+    rts = hip_runtimes_mapped()
+    if len(rts) > 1:
+        raise CnLibraryError(
+            f"{what}: two HIP runtimes are mapped ({', '.join(rts)}); load libcn_hip.so through "
+            "cnhip.load_library() (which maps torch's runtime first) or import torch before loading it")
+
+
 def load_library(path: str | None = None) -> C.CDLL:
-    """Load libcn_hip.so and bind every C-ABI symbol; raises CnLibraryError if absent."""
+    """Load libcn_hip.so and bind every C-ABI symbol; raises CnLibraryError if absent.
+
+    torch's ROCm build links the unversioned ``libamdhip64.so`` from its own ``lib`` directory,
+    while libcn_hip.so needs the SONAME ``libamdhip64.so.7``.  Loaded first, the library would map
+    /opt/rocm's runtime, and a later ``import torch`` would map torch's copy as a second, separate
+    HIP runtime (its NEEDED name does not match the loaded SONAME).  So when torch is installed its
+    runtime is mapped first: its SONAME is ``libamdhip64.so.7``, the library binds to it, and a
+    later ``import torch`` finds the same file already loaded.  One runtime serves both, whatever
+    the import order."""
     global _lib
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise CnLibraryError(f"{p} not found: build the HIP library first (__graft_entry__.build())")
+    trt = torch_hip_runtime()
+    if trt is not None:
+        try:
+            C.CDLL(trt, mode=C.RTLD_GLOBAL)
+        except OSError as e:  # pragma: no cover - a broken torch install
+            raise CnLibraryError(f"cannot load torch's HIP runtime {trt}: {e}") from e
     try:
         lib = C.CDLL(p)
     except OSError as e:  # pragma: no cover - depends on the ROCm runtime being present

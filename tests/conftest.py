@@ -23,20 +23,6 @@ def gpu_available() -> bool:
         return False
 
 
-def pytest_collection_finish(session):
-    # torch bundles its own HIP and HSA runtimes beside the library's (/opt/rocm): when both drive
-    # the GPU in one process, torch's device init runs first (torch has reported no HIP devices
-    # when it first initialised after the library's context and streams were live)
-    if not any(item.get_closest_marker("gpu") for item in session.items):
-        return
-    try:
-        import torch
-    except ImportError:
-        return
-    if torch.cuda.device_count() > 0:
-        torch.cuda.init()
-
-
 @pytest.fixture(scope="session")
 def oracle():
     import cn_oracle
