@@ -52,6 +52,7 @@ struct cn_ctx {
     std::vector<uint32_t> scene_fm;       // per scene: first fluid material, count (pairs)
     std::vector<uint32_t> scene_pm;       // per scene: first particle material, count (pairs)
     std::vector<cn_fluid_material> fm_host;  // host copy of the pool's fluid material table
+    std::vector<uint8_t> fm_fluid_used;      // per fluid material: referenced by a fluid particle
     std::vector<uint32_t> scene_con;      // per scene: first edge, bend, dihedral, tet (4 words)
     std::vector<EdgeDev> edges_host;      // host copies of the device constraint records
     std::vector<BendDev> bends_host;
@@ -777,6 +778,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->any_pairs = false;
     ctx->has_vorticity = false;
     ctx->aniso_ready = false;
+    ctx->fm_fluid_used.clear();
     uint32_t base = 0;
     double max_span = 1.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -790,6 +792,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         ctx->scene_pm[2 * k] = pm0;
         ctx->scene_pm[2 * k + 1] = tb->particle_material_count;
         fmh.insert(fmh.end(), tb->fluid_materials, tb->fluid_materials + tb->fluid_material_count);
+        ctx->fm_fluid_used.resize(fmh.size(), 0u);
         pmh.insert(pmh.end(), tb->particle_materials, tb->particle_materials + 4 * tb->particle_material_count);
         std::memcpy(pos.data() + 4ull * base, pv->positions_inv_mass, 4ull * m * sizeof(float));
         std::memcpy(vel.data() + 4ull * base, pv->velocities, 4ull * m * sizeof(float));
@@ -810,8 +813,10 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             pc.mask = pv->collision_masks ? pv->collision_masks[i] : 0xffffffffu;
             pc.scene = k;
             ctx->has_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
-            if (pc.kind == CN_PARTICLE_KIND_FLUID && fmi < tb->fluid_material_count)
+            if (pc.kind == CN_PARTICLE_KIND_FLUID && fmi < tb->fluid_material_count) {
                 ctx->has_vorticity |= std::max(tb->fluid_materials[fmi].vorticity_confinement, 0.0f) > kEps;
+                ctx->fm_fluid_used[fm0 + fmi] = 1u;
+            }
             ctx->has_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             scene_solid |= pc.kind == CN_PARTICLE_KIND_SOLID;
             scene_fluid |= pc.kind == CN_PARTICLE_KIND_FLUID;
@@ -1640,12 +1645,14 @@ cn_status cn_scene_set_materials(cn_ctx *ctx, uint32_t scene, const cn_fluid_mat
     if (particle_materials && pn)
         CN_HIP(ctx, hipMemcpyAsync(ctx->s.part_mats + p0, particle_materials, pn * sizeof(float4),
                                    hipMemcpyHostToDevice, ctx->stream));
-    // the flags that pick kernels follow the whole table (the passes still test each particle's
-    // own material), and a captured graph holds the old choice
+    // the flags that pick kernels are recomputed as the upload computes them (tension over the
+    // whole table, vorticity over the materials a fluid particle references; the passes still
+    // test each particle's own material), and a captured graph holds the old choice
     bool tension = false, vort = false;
-    for (const cn_fluid_material &m : ctx->fm_host) {
+    for (size_t k = 0; k < ctx->fm_host.size(); ++k) {
+        const cn_fluid_material &m = ctx->fm_host[k];
         tension |= (m.surface_tension > 0.0f ? m.surface_tension : 0.0f) > kEps;
-        vort |= std::max(m.vorticity_confinement, 0.0f) > kEps;
+        if (ctx->fm_fluid_used[k]) vort |= std::max(m.vorticity_confinement, 0.0f) > kEps;
     }
     if (tension != ctx->has_tension || vort != ctx->has_vorticity) ctx->graph_valid = false;
     ctx->has_tension = tension;

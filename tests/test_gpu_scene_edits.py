@@ -163,3 +163,43 @@ def test_set_constraints_bitexact(oracle):
             pool.set_scene_constraints(0, eds[1])
     finally:
         pool.close()
+
+
+def vorticity_launches(solver, p) -> int:
+    """Vorticity-pass launches of one profiled step (the context's statistics accumulate)."""
+    before = solver.profile_read().get("fluid_vorticity", (0, 0.0))[0]
+    solver.profile(True)
+    solver.step(p)
+    after = solver.profile_read().get("fluid_vorticity", (0, 0.0))[0]
+    solver.profile(False)
+    return after - before
+
+
+def test_set_materials_unused_vorticity_runs_no_vorticity_pass(solver, oracle):
+    """A table edit that gives vorticity confinement only to a material no fluid particle
+    references leaves the vorticity passes off (as an upload of that table would) and the steps
+    equal the oracle's."""
+    sc = small_cloud(1200, seed=17)
+    r = float(sc.radii[0])
+    sc.fluid_mats = np.concatenate([sc.fluid_mats, cnhip.fluid_material(r)])   # material 1: unused
+    p = params(cap=64, substeps=2, iters=2)
+    osc = oracle.OracleScene(sc)
+    solver.reserve(sc.n, 64)
+    solver.upload(sc)
+    solver.step(p)
+    oracle.step(osc, p)
+    fm = sc.fluid_mats.copy()
+    fm[1] = cnhip.fluid_material(r, vorticity_confinement=0.8)[0]
+    solver.set_scene_materials(0, fm)
+    osc.fluid_mats[:] = fm
+    assert vorticity_launches(solver, p) == 0
+    oracle.step(osc, p)
+    pos, _, vel = solver.readback()
+    assert np.array_equal(pos, osc.pos) and np.array_equal(vel, osc.vel)
+    fm[0] = cnhip.fluid_material(r, vorticity_confinement=0.8)[0]   # now a used material confines
+    solver.set_scene_materials(0, fm)
+    osc.fluid_mats[:] = fm
+    assert vorticity_launches(solver, p) > 0
+    oracle.step(osc, p)
+    pos, _, vel = solver.readback()
+    assert np.array_equal(pos, osc.pos) and np.array_equal(vel, osc.vel)
