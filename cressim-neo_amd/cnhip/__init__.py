@@ -387,6 +387,12 @@ class Solver:
         self._check(self.lib.cn_pass_sort_pairs(self._ctx, len(k), k.ctypes.data, v.ctypes.data), "sort")
         return k, v
 
+    def pass_sort_device_noalloc(self, keys_ptr: int, idx_ptr: int, count: int):
+        """cn_pass_sort_pairs_noalloc: sorts count (key, index) pairs held in two device arrays in
+        place, on the solver's stream, with no device allocation (the library's fixed scratch, at
+        most CN_NOALLOC_MAX_PAIRS pairs).  Asynchronous: synchronize() before reading the result."""
+        self._check(self.lib.cn_pass_sort_pairs_noalloc(self._ctx, count, keys_ptr, idx_ptr), "sort (no allocation)")
+
     def pass_cell_ranges(self, sorted_keys: np.ndarray, capacity: int):
         k = np.ascontiguousarray(sorted_keys, np.uint32)
         out = np.zeros((capacity, 4), np.uint32)

@@ -27,6 +27,7 @@ CN_KEY_MORTON = 0
 CN_KEY_HASH = 1
 CN_SOLVER_CONFIG_FLOATS = 40
 CN_STEP_CANDIDATE_REUSE = 1
+CN_NOALLOC_MAX_PAIRS = 1 << 20
 CN_STATE_DEVICE_POINTERS = 1
 CN_KERNEL_SLOTS = 16
 
@@ -163,6 +164,7 @@ SIGNATURES = {
     "cn_query": (C.c_int, [_P]),
     "cn_pass_broadphase_keys": (C.c_int, [_P, _U, _P, C.c_float, _P, _P]),
     "cn_pass_sort_pairs": (C.c_int, [_P, _U, _P, _P]),
+    "cn_pass_sort_pairs_noalloc": (C.c_int, [_P, _U, C.c_void_p, C.c_void_p]),
     "cn_pass_cell_ranges": (C.c_int, [_P, _U, _P, _U, _P]),
     "cn_pass_substep_broadphase": (C.c_int, [_P, C.POINTER(StepParams), _P, _P, _P, _P]),
     "cn_profile_enable": (C.c_int, [_P, C.c_int]),

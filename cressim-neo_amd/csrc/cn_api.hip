@@ -12,12 +12,51 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "cn_internal.h"
 
 using namespace cn;
+
+// Device scratch of cn_pass_sort_pairs_noalloc: two pair buffers of CN_NOALLOC_MAX_PAIRS entries and
+// the radix sort's per-tile digit counts for that many pairs (256 digits x 256 tiles + 256 totals).
+// Every GPU the library runs on holds its own copy from module load until the process ends.  In
+// host code `scratch` is only a stand-in for the symbol: its device address on the current GPU
+// comes from hipGetSymbolAddress (noalloc_scratch below), and only that address reaches a kernel.
+constexpr size_t kNoAllocPairBytes = (size_t)CN_NOALLOC_MAX_PAIRS * sizeof(uint2);
+constexpr size_t kNoAllocCountWords = 256u * (CN_NOALLOC_MAX_PAIRS / 4096u) + 256u;
+constexpr size_t kNoAllocScratchBytes = 2 * kNoAllocPairBytes + kNoAllocCountWords * sizeof(uint32_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU (null: not resolved yet)
+
+// Resolves the scratch array's address on the current device once per device (called with that
+// device current); false when the runtime cannot give it.
+bool resolve_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return false;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return true;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return false;
+    g_scratch[device] = p;
+    return true;
+}
+
+void *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
+
+}  // namespace
 
 struct cn_ctx {
     int device = 0;
@@ -489,6 +528,11 @@ cn_status cn_ctx_create(const cn_device_desc *desc, cn_ctx **out)
         return CN_ERR_DEVICE;
     }
     if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return CN_ERR_DEVICE;
+    }
+    if (!resolve_noalloc_scratch(ctx->device)) {  // the no-allocation entry points' scratch on this GPU
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return CN_ERR_DEVICE;
     }
@@ -1825,6 +1869,28 @@ cn_status cn_pass_sort_pairs(cn_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *
     dfree(a);
     dfree(b);
     dfree(scr);
+    return CN_OK;
+}
+
+cn_status cn_pass_sort_pairs_noalloc(cn_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *indices)
+{
+    if (!ctx || (n && (!keys || !indices))) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > CN_NOALLOC_MAX_PAIRS)
+        return fail(ctx, CN_ERR_CAPACITY,
+                    "cn_pass_sort_pairs_noalloc: " + std::to_string(n) + " pairs exceed its scratch of " +
+                        std::to_string(CN_NOALLOC_MAX_PAIRS) + " (use cn_pass_sort_pairs)");
+    if (n <= 1) return CN_OK;
+    CN_HIP(ctx, hipSetDevice(ctx->device));
+    char *base = static_cast<char *>(noalloc_scratch(ctx->device));
+    if (!base) return fail(ctx, CN_ERR_DEVICE, "cn_pass_sort_pairs_noalloc: scratch address not resolved");
+    uint2 *a = reinterpret_cast<uint2 *>(base);
+    uint2 *b = reinterpret_cast<uint2 *>(base + kNoAllocPairBytes);
+    uint32_t *counts = reinterpret_cast<uint32_t *>(base + 2 * kNoAllocPairBytes);
+    // the sort writes every count and total it reads, so nothing left by an earlier call is read
+    launch_pack_pairs(keys, indices, n, a, ctx->stream);
+    const uint2 *res = radix_sort_pairs(a, b, n, 32u, counts, ctx->stream);
+    launch_unpack_pairs(res, n, keys, indices, ctx->stream);
+    CN_HIP(ctx, hipGetLastError());
     return CN_OK;
 }
 

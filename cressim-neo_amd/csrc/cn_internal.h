@@ -59,6 +59,8 @@ constexpr uint64_t kListBytesMax = 0xffffffffull;
 size_t sort_scratch_words(uint32_t n);
 uint2 *radix_sort_pairs(uint2 *buf_a, uint2 *buf_b, uint32_t n, uint32_t key_bits, uint32_t *scratch,
                         hipStream_t stream);
+void launch_pack_pairs(const uint32_t *keys, const uint32_t *vals, uint32_t n, uint2 *out, hipStream_t stream);
+void launch_unpack_pairs(const uint2 *in, uint32_t n, uint32_t *keys, uint32_t *vals, hipStream_t stream);
 
 // Device pointers of one resident scene (single scene or one slot pool).
 // Device forms of the soft constraints: the caller's records (cn_soft_edge, cn_soft_bend,

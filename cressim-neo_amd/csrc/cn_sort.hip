@@ -143,6 +143,42 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint2 *__r
     }
 }
 
+// (key, value) pairs from two separate arrays, and back: the layout of the pair entry points that
+// take the caller's device arrays.
+__global__ __launch_bounds__(kSortThreads) void k_pack_pairs(const uint32_t *__restrict__ keys,
+                                                             const uint32_t *__restrict__ vals, uint32_t n,
+                                                             uint2 *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kSortThreads + threadIdx.x;
+    if (i < n) out[i] = make_uint2(keys[i], vals[i]);
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_unpack_pairs(const uint2 *__restrict__ in, uint32_t n,
+                                                               uint32_t *__restrict__ keys,
+                                                               uint32_t *__restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * kSortThreads + threadIdx.x;
+    if (i < n) {
+        const uint2 v = in[i];
+        keys[i] = v.x;
+        vals[i] = v.y;
+    }
+}
+
+void launch_pack_pairs(const uint32_t *keys, const uint32_t *vals, uint32_t n, uint2 *out, hipStream_t stream)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_pack_pairs, dim3((n + kSortThreads - 1) / kSortThreads), dim3(kSortThreads), 0, stream,
+                       keys, vals, n, out);
+}
+
+void launch_unpack_pairs(const uint2 *in, uint32_t n, uint32_t *keys, uint32_t *vals, hipStream_t stream)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_unpack_pairs, dim3((n + kSortThreads - 1) / kSortThreads), dim3(kSortThreads), 0, stream,
+                       in, n, keys, vals);
+}
+
 // Sorts n pairs by the low key_bits of .x, stably.  buf_a holds the input; the result ends in
 // the returned buffer (buf_a or buf_b).  scratch needs sort_scratch_words(n) uint32 words.
 size_t sort_scratch_words(uint32_t n)

@@ -360,6 +360,17 @@ cn_status cn_pass_broadphase_keys(cn_ctx *ctx, uint32_t count, const float *posi
 /* sortParticleBroadPhase (physics_pass_dispatcher.cpp:3963-4063): stable ascending sort of
  * (key, index) pairs, in place. */
 cn_status cn_pass_sort_pairs(cn_ctx *ctx, uint32_t count, uint32_t *keys, uint32_t *indices);
+/* No reference counterpart: the sort of cn_pass_sort_pairs for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * keys and indices are DEVICE arrays of count entries on the context's GPU, sorted in place
+ * (the same stable ascending order as cn_pass_sort_pairs).  The call is enqueued on the context
+ * stream and returns without waiting (cn_synchronize waits); it calls no hipMalloc/hipFree.  Its
+ * working memory is a fixed array in the library's device code, one per GPU
+ * (2 * 8 * CN_NOALLOC_MAX_PAIRS bytes plus the sort's digit counts), shared by every call on that
+ * GPU: calls that overlap on one GPU (two contexts' streams at once) race on it.  count above
+ * CN_NOALLOC_MAX_PAIRS returns CN_ERR_CAPACITY. */
+#define CN_NOALLOC_MAX_PAIRS (1u << 20)
+cn_status cn_pass_sort_pairs_noalloc(cn_ctx *ctx, uint32_t count, uint32_t *keys, uint32_t *indices);
 /* clear/mark/build ParticleCellRanges (physics_particle_*_cell_range*.cs.hlsl); ranges holds
  * capacity uint4 records {key, start, end, 0}, unused {0xffffffff, 0, 0, 0}. */
 cn_status cn_pass_cell_ranges(cn_ctx *ctx, uint32_t count, const uint32_t *sorted_keys,
