@@ -16,7 +16,7 @@ from . import abi
 from .abi import (CN_KEY_HASH, CN_KEY_MORTON, CN_PARTICLE_KIND_FLUID, CN_PARTICLE_KIND_SOLID,
                   CnError, CnLibraryError, load_library)
 
-__all__ = ["Scene", "Solver", "default_params", "fluid_material", "CnError", "CnLibraryError",
+__all__ = ["Scene", "Solver", "default_params", "fluid_material", "shape_collider", "CnError", "CnLibraryError",
            "CN_KEY_HASH", "CN_KEY_MORTON", "CN_PARTICLE_KIND_FLUID", "CN_PARTICLE_KIND_SOLID",
            "load_library"]
 
@@ -26,6 +26,14 @@ COLLIDER_DTYPE = np.dtype([("center", np.float32, 3), ("half_extents", np.float3
                            ("layer", np.uint32), ("mask", np.uint32), ("enabled", np.uint32),
                            ("reserved", np.uint32, 2)])
 assert COLLIDER_DTYPE.itemsize == C.sizeof(abi.BoxCollider) == 64
+# cn_collider: a static collider of any shape (sphere, oriented box, capsule) on a static body
+SHAPE_COLLIDER_DTYPE = np.dtype([("shape", np.uint32), ("environment", np.uint32), ("layer", np.uint32),
+                                 ("mask", np.uint32), ("body_position", np.float32, 4),
+                                 ("body_orientation", np.float32, 4), ("body_scale", np.float32, 4),
+                                 ("local_position", np.float32, 4), ("local_orientation", np.float32, 4),
+                                 ("shape_params", np.float32, 4), ("material", np.float32, 4),
+                                 ("enabled", np.uint32), ("reserved", np.uint32, 3)])
+assert SHAPE_COLLIDER_DTYPE.itemsize == C.sizeof(abi.Collider) == 144
 assert FLUID_MATERIAL_DTYPE.itemsize == C.sizeof(abi.FluidMaterial) == 48
 
 
@@ -51,6 +59,11 @@ class Scene:
     bends: np.ndarray | None = None      # authoring.BEND_DTYPE
     dihedrals: np.ndarray | None = None  # authoring.DIHEDRAL_DTYPE
     tets: np.ndarray | None = None       # authoring.TET_DTYPE
+    # constraint adjacency (offsets (n,), counts (n,), indices) uint32, or None: the particles each
+    # particle shares a constraint with, which a self-colliding phase group does not collide with
+    adjacency: tuple | None = None
+    # static colliders of any shape (SHAPE_COLLIDER_DTYPE), after the boxes of `colliders`
+    shape_colliders: np.ndarray | None = None
 
     def constraint_arrays(self):
         from . import authoring as au
@@ -99,6 +112,26 @@ def fluid_material(radius: float, viscosity=0.01, cohesion=0.0, surface_tension=
     arr = np.zeros(1, FLUID_MATERIAL_DTYPE)
     C.memmove(arr.ctypes.data, C.byref(out), C.sizeof(out))
     return arr
+
+
+def shape_collider(shape: int, position, orientation=(0.0, 0.0, 0.0, 1.0), params=(0.1, 0.1, 0.1),
+                   scale=(1.0, 1.0, 1.0), local_position=(0.0, 0.0, 0.0), local_orientation=(0.0, 0.0, 0.0, 1.0),
+                   material=(0.0, 0.0, 0.0, -1.0), environment=0, layer=1, mask=0xFFFFFFFF, enabled=1) -> np.ndarray:
+    """One static collider record (SHAPE_COLLIDER_DTYPE): a static body at `position` with
+    `orientation` (x, y, z, w) and `scale`, carrying a sphere (params = radius,), an oriented box
+    (half extents) or a capsule (radius, half height along local y) at its local pose."""
+    out = np.zeros(1, SHAPE_COLLIDER_DTYPE)
+    out["shape"] = shape
+    out["body_position"][0, :3] = position
+    out["body_orientation"][0] = orientation
+    out["body_scale"][0, :3] = scale
+    out["local_position"][0, :3] = local_position
+    out["local_orientation"][0] = local_orientation
+    prm = list(params) + [0.0] * (4 - len(params))
+    out["shape_params"][0] = prm
+    out["material"][0] = material
+    out["environment"], out["layer"], out["mask"], out["enabled"] = environment, layer, mask, enabled
+    return out
 
 
 class Solver:
@@ -167,8 +200,19 @@ class Solver:
         view = abi.ParticleSoaView(sc.n, p(arrs["pos"]), p(arrs["vel"]), p(arrs["radii"]), p(arrs["kinds"]),
                                    p(arrs["fm"]), p(arrs["pm"]), p(arrs["env"]), p(arrs["ph"]),
                                    p(arrs["ly"]), p(arrs["mk"]))
+        if sc.adjacency is not None:
+            ao, ac, ai = (np.ascontiguousarray(a, np.uint32) for a in sc.adjacency)
+            arrs.update(ao=ao, ac=ac, ai=ai)
+            view.adjacency_offsets, view.adjacency_counts = p(ao), p(ac)
+            view.adjacency_index_count = len(ai)
+            view.adjacency_indices = p(ai) if len(ai) else None
         tables = abi.SceneTables(len(arrs["fmats"]), p(arrs["fmats"]), len(arrs["pmats"]), p(arrs["pmats"]),
                                  len(arrs["cols"]), p(arrs["cols"]) if len(arrs["cols"]) else None)
+        if sc.shape_colliders is not None and len(sc.shape_colliders):
+            arrs["shapes"] = np.ascontiguousarray(sc.shape_colliders)
+            assert arrs["shapes"].dtype == SHAPE_COLLIDER_DTYPE
+            tables.shape_collider_count = len(arrs["shapes"])
+            tables.shape_colliders = p(arrs["shapes"])
         e, b, d, t = sc.constraint_arrays()
         cons = abi.ConstraintView(len(e), p(e) if len(e) else None, len(b), p(b) if len(b) else None,
                                   len(d), p(d) if len(d) else None, len(t), p(t) if len(t) else None)
@@ -363,6 +407,17 @@ class Solver:
         if p.shape != (count.value, 4) or v.shape != p.shape:
             raise ValueError(f"state arrays must be float32 ({count.value}, 4)")
         self._check(self.lib.cn_scene_set_state(self._ctx, k, p.ctypes.data, v.ctypes.data, 0), "cn_scene_set_state")
+
+    def update_particles(self, first: int, pos, vel):
+        """Dirty-run upload (cn_particles_update_range): overwrite the state of the uploaded
+        particles [first, first + len(pos)) (cn_readback's order), leaving the rest resident."""
+        p = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        v = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+        if v.shape != p.shape:
+            raise ValueError("positions and velocities must have the same shape")
+        self._check(self.lib.cn_particles_update_range(self._ctx, first, len(p), p.ctypes.data if len(p) else None,
+                                                       v.ctypes.data if len(v) else None, 0),
+                    "cn_particles_update_range")
 
     def torch_stream(self):
         """The solver's HIP stream as a torch stream: order torch work after ``step`` with

@@ -28,6 +28,7 @@ CN_KEY_HASH = 1
 CN_SOLVER_CONFIG_FLOATS = 40
 CN_STEP_CANDIDATE_REUSE = 1
 CN_NOALLOC_MAX_PAIRS = 1 << 20
+CN_SHAPE_SPHERE, CN_SHAPE_BOX, CN_SHAPE_CAPSULE = 0, 1, 2
 CN_STATE_DEVICE_POINTERS = 1
 CN_KERNEL_SLOTS = 16
 
@@ -69,13 +70,24 @@ class ParticleSoaView(C.Structure):
                 ("fluid_material_indices", C.c_void_p),
                 ("particle_material_indices", C.c_void_p),
                 ("environment_indices", C.c_void_p), ("phases", C.c_void_p),
-                ("collision_layers", C.c_void_p), ("collision_masks", C.c_void_p)]
+                ("collision_layers", C.c_void_p), ("collision_masks", C.c_void_p),
+                ("adjacency_offsets", C.c_void_p), ("adjacency_counts", C.c_void_p),
+                ("adjacency_index_count", C.c_uint32), ("adjacency_indices", C.c_void_p)]
 
 
 class SceneTables(C.Structure):
     _fields_ = [("fluid_material_count", C.c_uint32), ("fluid_materials", C.c_void_p),
                 ("particle_material_count", C.c_uint32), ("particle_materials", C.c_void_p),
-                ("collider_count", C.c_uint32), ("colliders", C.c_void_p)]
+                ("collider_count", C.c_uint32), ("colliders", C.c_void_p),
+                ("shape_collider_count", C.c_uint32), ("shape_colliders", C.c_void_p)]
+
+
+class Collider(C.Structure):
+    _fields_ = [("shape", C.c_uint32), ("environment", C.c_uint32), ("layer", C.c_uint32), ("mask", C.c_uint32),
+                ("body_position", C.c_float * 4), ("body_orientation", C.c_float * 4),
+                ("body_scale", C.c_float * 4), ("local_position", C.c_float * 4),
+                ("local_orientation", C.c_float * 4), ("shape_params", C.c_float * 4),
+                ("material", C.c_float * 4), ("enabled", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class ConstraintView(C.Structure):
@@ -154,6 +166,7 @@ SIGNATURES = {
     "cn_readback": (C.c_int, [_P, _P, _P, _P, _U]),
     "cn_device_views_get": (C.c_int, [_P, C.POINTER(DeviceViews)]),
     "cn_scene_set_state": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "cn_particles_update_range": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]),
     "cn_scene_set_colliders": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32]),
     "cn_scene_set_materials": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "cn_scene_set_constraints": (C.c_int, [_P, C.c_uint32, C.c_void_p]),

@@ -153,3 +153,32 @@ def cloth_constraints(pos, tris, offset=0, structural_compliance=0.0, bend_compl
         dih["rest_angle"] = [_signed_dihedral(p[a], p[b], p[c], p[e])
                              for a, b, c, e in zip(elo[two], ehi[two], oc, od)]
     return edges, dih
+
+
+def adjacency(n: int, pairs) -> tuple:
+    """Constraint adjacency as the reference builds it: every pair in both directions, each
+    particle's list sorted and deduplicated (soft_body_authoring.cpp:392-408 for tet edges,
+    cloth_authoring.cpp:139-188 for structural edges and dihedral-hinge opposites), laid out as
+    the pool's offsets / counts / indices (physics_world.cpp:5774-5787).  pairs: (m, 2)."""
+    pr = np.asarray(pairs, np.int64).reshape(-1, 2)
+    a = np.concatenate([pr[:, 0], pr[:, 1]])
+    b = np.concatenate([pr[:, 1], pr[:, 0]])
+    key = np.unique(a * np.int64(max(n, 1)) + b)
+    a, b = key // max(n, 1), key % max(n, 1)
+    counts = np.bincount(a, minlength=n).astype(np.uint32)
+    offsets = np.zeros(n, np.uint32)
+    if n > 1:
+        offsets[1:] = np.cumsum(counts[:-1], dtype=np.uint64).astype(np.uint32)
+    return offsets, counts, b.astype(np.uint32)
+
+
+def tet_adjacency_pairs(edges) -> np.ndarray:
+    """The pairs of a tet soft body's adjacency: its tet edges."""
+    return np.stack([edges["a"], edges["b"]], 1).astype(np.int64)
+
+
+def cloth_adjacency_pairs(edges, dihedrals) -> np.ndarray:
+    """The pairs of a cloth's adjacency: structural edges and the two opposite vertices of every
+    bending hinge."""
+    return np.concatenate([np.stack([edges["a"], edges["b"]], 1), np.stack([dihedrals["o0"], dihedrals["o1"]], 1)]
+                          ).astype(np.int64)

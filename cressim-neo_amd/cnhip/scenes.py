@@ -113,8 +113,11 @@ def compose(parts, fluid_mats, colliders, name, info=None) -> Scene:
     n = sum(len(p["pos"]) for p in parts)
     cat = {k: np.concatenate([p[k] for p in parts]) for k in ("pos", "radii", "kinds", "fluid_mat", "part_mat", "env")}
     cons = {"edges": [], "bends": [], "dihedrals": [], "tets": []}
+    adj = []
     off = 0
     for p in parts:
+        if p.get("adj_pairs") is not None and len(p["adj_pairs"]):
+            adj.append(np.asarray(p["adj_pairs"], np.int64) + off)
         for k in cons:
             if k in p and p[k] is not None and len(p[k]):
                 a = p[k].copy()
@@ -126,16 +129,17 @@ def compose(parts, fluid_mats, colliders, name, info=None) -> Scene:
     dts = {"edges": au.EDGE_DTYPE, "bends": au.BEND_DTYPE, "dihedrals": au.DIHEDRAL_DTYPE, "tets": au.TET_DTYPE}
     c = {k: (np.concatenate(v) if v else np.zeros(0, dts[k])) for k, v in cons.items()}
     # one phase group per part (packParticlePhase, include/physics/particle_phase.h:16-27): soft
-    # bodies and cloths without self collision, fluids with the self-collide flag
-    # (physics_world.cpp:5083,5183,5381)
-    phase = np.concatenate([np.full(len(p["pos"]), (k + 1) | (0x80000000 if np.all(p["kinds"] == 1) else 0), np.uint32)
+    # bodies and cloths without self collision unless the part asks for it (self_collide), fluids
+    # with the self-collide flag (physics_world.cpp:5083,5183,5381)
+    phase = np.concatenate([np.full(len(p["pos"]), (k + 1) | (0x80000000 if (np.all(p["kinds"] == 1) or
+                                                                            p.get("self_collide")) else 0), np.uint32)
                             for k, p in enumerate(parts)])
     return Scene(pos=cat["pos"], vel=np.zeros((n, 4), np.float32), radii=cat["radii"], kinds=cat["kinds"],
                  fluid_mat=cat["fluid_mat"], part_mat=cat["part_mat"], env=cat["env"],
                  phase=phase, layer=np.ones(n, np.uint32), mask=np.full(n, 0xFFFFFFFF, np.uint32),
                  fluid_mats=fluid_mats, part_mats=np.array([SOFT_CONTACT], np.float32), colliders=colliders,
                  name=name, info=info or {}, edges=c["edges"], bends=c["bends"], dihedrals=c["dihedrals"],
-                 tets=c["tets"])
+                 tets=c["tets"], adjacency=au.adjacency(n, np.concatenate(adj)) if adj else None)
 
 
 def soft_block_part(res=8, spacing=0.05, radius=None, translation=(1.0, 0.6, 1.0), mass=0.1, compliance=1e-6):
@@ -144,7 +148,7 @@ def soft_block_part(res=8, spacing=0.05, radius=None, translation=(1.0, 0.6, 1.0
     pos, tets = au.tet_grid((size, size, size), resolution=(res, res, res), translation=translation)
     edges, tc = au.tet_constraints(pos, tets, 0, compliance, compliance)
     part = _solid_part(pos, radius if radius is not None else 0.5 * spacing, 1.0 / mass)
-    part.update(edges=edges, tets=tc)
+    part.update(edges=edges, tets=tc, adj_pairs=au.tet_adjacency_pairs(edges))
     return part
 
 
@@ -153,7 +157,7 @@ def cloth_part(n=32, spacing=0.02, radius=None, translation=(0.5, 1.5, 0.5), mas
     pos, tris = au.cloth_grid(n, n, spacing, translation)
     edges, dih = au.cloth_constraints(pos, tris, 0, compliance, compliance)
     part = _solid_part(pos, radius if radius is not None else 0.5 * spacing, 1.0 / mass)
-    part.update(edges=edges, dihedrals=dih)
+    part.update(edges=edges, dihedrals=dih, adj_pairs=au.cloth_adjacency_pairs(edges, dih))
     return part
 
 
@@ -204,6 +208,49 @@ def contact_stack(block_res=6, cloth_n=24, fluid_side=6) -> Scene:
                    dict(soft=block_res ** 3, cloth=cloth_n ** 2, fluid=fluid_side ** 3))
 
 
+def obstacles(material=(0.0, 0.0, 0.0, -1.0)) -> np.ndarray:
+    """Three static shape colliders on the floor of the 4 x 2 x 2 m container: a sphere (r 0.12 m),
+    a capsule lying along z (r 0.06 m, half height 0.2 m) and a box turned 30 degrees about y."""
+    from . import shape_collider
+    s = np.float32(np.sin(np.pi / 4)), np.float32(np.sin(np.pi / 12))
+    c = np.float32(np.cos(np.pi / 4)), np.float32(np.cos(np.pi / 12))
+    return np.concatenate([
+        shape_collider(0, (0.25, 0.15, 0.25), params=(0.12,), material=material),
+        shape_collider(2, (0.5, 0.1, 0.35), orientation=(s[0], 0.0, 0.0, c[0]), params=(0.06, 0.2),
+                       material=material),
+        shape_collider(1, (0.3, 0.08, 0.5), orientation=(0.0, s[1], 0.0, c[1]), params=(0.1, 0.05, 0.1),
+                       material=material),
+    ])
+
+
+def shapes_scene(side: int = 10, seed: int = 0) -> Scene:
+    """A jittered fluid block (spacing 0.05 m, r = 0.03 m) released above the obstacles() in the
+    4 x 2 x 2 m container."""
+    part = fluid_part(side, 0.05, 0.03, origin=(0.05, 0.35, 0.05), seed=seed, jitter=0.005)
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    sc = compose([part], fluid_material(0.03), cols, f"shapes_{side ** 3}", dict(side=side))
+    sc.shape_colliders = obstacles()
+    return sc
+
+
+def folded_cloth(n: int = 16, spacing: float = 0.03, gap: float = 0.75, self_collide: bool = True) -> Scene:
+    """A self-colliding cloth folded onto itself: an n x n grid (spacing 0.03 m, r = spacing / 2)
+    authored flat, then the half beyond the middle column mirrored over it, gap x 2r above, so
+    every upper-layer particle overlaps the one below it.  Across the crease the mirrored columns
+    are hinge opposites, so constraint adjacency keeps those pairs apart; the others collide."""
+    cloth = cloth_part(n, spacing, 0.5 * spacing, translation=(1.0, 0.5, 1.0))
+    cloth["self_collide"] = self_collide
+    p = cloth["pos"]
+    mid = np.float32(1.0 + (n // 2) * spacing - 0.5 * (n - 1) * spacing)  # x of the middle column
+    xs = np.unique(p[:, 0])
+    mid = xs[n // 2] if len(xs) == n else mid
+    over = p[:, 0] > mid
+    p[over, 0] = np.float32(2.0) * mid - p[over, 0]
+    p[over, 1] = p[over, 1] + np.float32(gap * spacing)
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    return compose([cloth], fluid_material(0.03), cols, f"folded_cloth_{n}", dict(n=n, self_collide=self_collide))
+
+
 def scaled_box_for(n: int, n_ref: int = 4_194_304, box_ref=(4.0, 2.0, 2.0)):
     """Box with the same aspect and number density as box_ref holds for n_ref particles."""
     s = (n / n_ref) ** (1.0 / 3.0)
@@ -222,7 +269,7 @@ def batch_sizes(count: int = 512, seed: int = 3, lo: int = 2000, hi: int = 64000
 def batch_scene(n: int, kind: int, seed: int = 0) -> Scene:
     """One scene of the batched workload with about n particles in its own 4 x 2 x 2 m container:
     a jittered PBF block (spacing 0.05 m, r = 0.03 m, the dam-break geometry), a tet soft block
-    (spacing 0.04 m) or a cloth grid spanning 1.9 m, compliance 1e-6."""
+    (spacing 0.04 m) 0.05 m above the floor, or a cloth grid spanning 1.9 m, compliance 1e-6."""
     cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
     fm = fluid_material(0.03)
     if kind == 0:
@@ -230,7 +277,8 @@ def batch_scene(n: int, kind: int, seed: int = 0) -> Scene:
         part = fluid_part(side, 0.05, 0.03, origin=(0.05, 0.02, 0.05), seed=seed, jitter=0.005)
     elif kind == 1:
         res = max(2, int(round(n ** (1.0 / 3.0))))
-        part = soft_block_part(res, 0.04, 0.02, translation=(1.0, 0.2, 0.2))
+        half = 0.5 * (res - 1) * 0.04  # the tet grid is centred on its translation
+        part = soft_block_part(res, 0.04, 0.02, translation=(2.0, half + 0.05, 1.0))
     else:
         m = max(2, int(round(n ** 0.5)))
         sp = 1.9 / (m - 1)
@@ -238,7 +286,17 @@ def batch_scene(n: int, kind: int, seed: int = 0) -> Scene:
     return compose([part], fm, cols, f"batch_{('pbf', 'tet', 'cloth')[kind]}_{n}", dict(n=int(n), kind=int(kind)))
 
 
-def batched(count: int = 512, seed: int = 3, lo: int = 2000, hi: int = 64000):
-    """The batched workload: count independent scenes sized and typed by batch_sizes()."""
+def _batch_scene_args(args):
+    return batch_scene(*args)
+
+
+def batched(count: int = 512, seed: int = 3, lo: int = 2000, hi: int = 64000, workers: int = 1):
+    """The batched workload: count independent scenes sized and typed by batch_sizes().  workers > 1
+    builds them in that many fresh (spawned) processes; the scenes are the same."""
     n, kind = batch_sizes(count, seed, lo, hi)
-    return [batch_scene(int(n[k]), int(kind[k]), seed=k) for k in range(count)]
+    args = [(int(n[k]), int(kind[k]), k) for k in range(count)]
+    if workers <= 1:
+        return [batch_scene(*a) for a in args]
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_batch_scene_args, args, chunksize=max(1, count // (4 * workers)))

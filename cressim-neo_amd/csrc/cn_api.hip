@@ -75,6 +75,7 @@ struct cn_ctx {
     bool has_tension = false;        // any fluid material with surface tension
     bool has_solid = false;          // any solid particle
     bool any_pairs = false;          // some scene can hold a particle-particle contact pair
+    bool any_rigid = false;          // some scene can hold a particle-rigid contact
     bool has_vorticity = false;      // a fluid particle's material confines vorticity
     bool aniso_enabled = false;      // cn_step ends with the render anisotropy
     bool aniso_ready = false;        // a step has written it since it was enabled or uploaded
@@ -86,7 +87,8 @@ struct cn_ctx {
     uint32_t fluid_scenes = 0;       // scenes with fluid particles (ranks [0, fluid_scenes))
     std::vector<uint32_t> scene_first;  // first particle of each scene, then the total
     std::vector<float> scene_cell;      // derived grid cell size of each scene
-    std::vector<uint32_t> scene_col_cap;  // collider slots of each scene: its uploaded collider count
+    std::vector<uint32_t> scene_col_cap;  // box colliders of each scene: its uploaded box count
+    std::vector<std::vector<cn_collider>> scene_shapes;  // each scene's shape colliders, as uploaded
     std::vector<SceneInfo> scene_info;    // host copy of the device scene records
     std::vector<uint32_t> scene_fm;       // per scene: first fluid material, count (pairs)
     std::vector<uint32_t> scene_pm;       // per scene: first particle material, count (pairs)
@@ -188,12 +190,17 @@ void free_scene(SceneBuffers &s)
     dfree(s.nblk);
     dfree(s.blk_halo);
     dfree(s.wide_blk);
+    dfree(s.fb_blk);
     dfree(s.iter_state);
     dfree(s.fluid_t2);
     dfree(s.list);
     dfree(s.lcnt);
     dfree(s.slist);
     dfree(s.scnt);
+    dfree(s.disp);
+    dfree(s.adj_off);
+    dfree(s.adj_cnt);
+    dfree(s.adj_idx);
     dfree(s.nbr);
     dfree(s.nbr_cnt);
     dfree(s.ct_list);
@@ -201,6 +208,7 @@ void free_scene(SceneBuffers &s)
     dfree(s.ct_rec);
     dfree(s.ct_bits);
     dfree(s.ct_corr);
+    dfree(s.ct_gen);
     dfree(s.out_pos);
     dfree(s.out_vel);
     dfree(s.out_prev);
@@ -331,63 +339,171 @@ uint32_t expand10(uint32_t v)
     return v;
 }
 
-std::vector<Collider> visit_ordered_colliders(const cn_box_collider *c, uint32_t nc)
+// Static colliders resolved to their world pose (physics_math.hlsli with rsqrt taken as 1 / sqrt,
+// sums left to right): ComputeFluidBoundaryContactInfo's pose (body pose composed with the
+// unscaled local position, physics_fluid_boundary_common.hlsli:21-33) and the broad-phase AABB of
+// physics_rigid_update_world_aabbs.cs.hlsl:51-91 (scaled local position, ComputeBodyAabb of
+// physics_shape_common.hlsli:49-75, the margin added after).
+struct Quat {
+    float x, y, z, w;
+};
+Quat quat_normalize(const float *q)
 {
-    struct E {
-        uint32_t idx;
-        float mn[3], mx[3];
-        uint32_t code;
-    };
-    std::vector<E> es;
-    for (uint32_t i = 0; i < nc; ++i) {
-        if (!c[i].enabled) continue;
-        E e;
-        e.idx = i;
-        for (int a = 0; a < 3; ++a) {
-            e.mn[a] = (c[i].center[a] - c[i].half_extents[a]) - kBroadPhaseMargin;
-            e.mx[a] = (c[i].center[a] + c[i].half_extents[a]) + kBroadPhaseMargin;
-        }
-        es.push_back(e);
+    const float l = ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3];
+    if (l <= kEps) return {0.0f, 0.0f, 0.0f, 1.0f};
+    const float r = 1.0f / std::sqrt(l);
+    return {q[0] * r, q[1] * r, q[2] * r, q[3] * r};
+}
+Quat quat_mul(const Quat &a, const Quat &b)
+{
+    return {((a.w * b.x + a.x * b.w) + a.y * b.z) - a.z * b.y, ((a.w * b.y - a.x * b.z) + a.y * b.w) + a.z * b.x,
+            ((a.w * b.z + a.x * b.y) - a.y * b.x) + a.z * b.w, ((a.w * b.w - a.x * b.x) - a.y * b.y) - a.z * b.z};
+}
+void quat_rotate_h(const Quat &q, const float *v, float *o)
+{
+    const float t[3] = {2.0f * (q.y * v[2] - q.z * v[1]), 2.0f * (q.z * v[0] - q.x * v[2]),
+                        2.0f * (q.x * v[1] - q.y * v[0])};
+    o[0] = (v[0] + q.w * t[0]) + (q.y * t[2] - q.z * t[1]);
+    o[1] = (v[1] + q.w * t[1]) + (q.z * t[0] - q.x * t[2]);
+    o[2] = (v[2] + q.w * t[2]) + (q.x * t[1] - q.y * t[0]);
+}
+
+Collider box_collider(const cn_box_collider &b, uint32_t index)
+{
+    Collider k{};
+    for (int a = 0; a < 3; ++a) {
+        k.c[a] = b.center[a];
+        k.he[a] = b.half_extents[a];
+        k.aabb_min[a] = (b.center[a] - b.half_extents[a]) - kBroadPhaseMargin;
+        k.aabb_max[a] = (b.center[a] + b.half_extents[a]) + kBroadPhaseMargin;
     }
+    for (int a = 0; a < 4; ++a) k.material[a] = b.material[a];
+    k.env = b.environment;
+    k.layer = b.layer;
+    k.mask = b.mask;
+    k.enabled = b.enabled;
+    k.index = index;
+    k.shape = CN_SHAPE_BOX;
+    k.rotated = 0u;
+    k.q[3] = 1.0f;
+    return k;
+}
+
+Collider shape_collider(const cn_collider &g, uint32_t index)
+{
+    Collider k{};
+    const Quat bq = quat_normalize(g.body_orientation), lq = quat_normalize(g.local_orientation);
+    const Quat q = [&] {
+        const Quat m = quat_mul(bq, lq);
+        const float v[4] = {m.x, m.y, m.z, m.w};
+        return quat_normalize(v);
+    }();
+    const float sx = std::fabs(g.body_scale[0]), sy = std::fabs(g.body_scale[1]), sz = std::fabs(g.body_scale[2]);
+    const float *prm = g.shape_params;
+    float rot[3];
+    quat_rotate_h(bq, g.local_position, rot);
+    for (int a = 0; a < 3; ++a) k.c[a] = g.body_position[a] + rot[a];
+    k.q[0] = q.x;
+    k.q[1] = q.y;
+    k.q[2] = q.z;
+    k.q[3] = q.w;
+    k.rotated = (q.x == 0.0f && q.y == 0.0f && q.z == 0.0f && q.w == 1.0f) ? 0u : 1u;
+    k.shape = g.shape;
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    float axis[3];
+    quat_rotate_h(q, up, axis);
+    float r = 0.0f, hh = 0.0f;
+    if (g.shape == CN_SHAPE_SPHERE) {
+        r = prm[0] * std::max(sx, std::max(sy, sz));  // SphereRadius
+        k.he[0] = r;
+    } else if (g.shape == CN_SHAPE_BOX) {
+        k.he[0] = prm[0] * sx;  // BoxHalfExtents
+        k.he[1] = prm[1] * sy;
+        k.he[2] = prm[2] * sz;
+    } else {  // CapsuleSegment
+        r = prm[0] * std::max(sx, sz);
+        hh = prm[1] * sy;
+        k.he[0] = r;
+        for (int a = 0; a < 3; ++a) {
+            k.sa[a] = k.c[a] - axis[a] * hh;
+            k.sb[a] = k.c[a] + axis[a] * hh;
+        }
+    }
+    // the broad-phase AABB from the scaled local position
+    const float lps[3] = {g.local_position[0] * g.body_scale[0], g.local_position[1] * g.body_scale[1],
+                          g.local_position[2] * g.body_scale[2]};
+    quat_rotate_h(bq, lps, rot);
+    float ca[3], ext[3];
+    for (int a = 0; a < 3; ++a) ca[a] = g.body_position[a] + rot[a];
+    if (g.shape == CN_SHAPE_SPHERE) {
+        ext[0] = ext[1] = ext[2] = r;
+    } else if (g.shape == CN_SHAPE_BOX) {
+        const float ex[3] = {1.0f, 0.0f, 0.0f}, ey[3] = {0.0f, 1.0f, 0.0f}, ez[3] = {0.0f, 0.0f, 1.0f};
+        float ax[3], ay[3], az[3];
+        quat_rotate_h(q, ex, ax);
+        quat_rotate_h(q, ey, ay);
+        quat_rotate_h(q, ez, az);
+        for (int a = 0; a < 3; ++a)
+            ext[a] = (std::fabs(ax[a]) * k.he[0] + std::fabs(ay[a]) * k.he[1]) + std::fabs(az[a]) * k.he[2];
+    } else {
+        float A[3], B[3];
+        for (int a = 0; a < 3; ++a) {
+            A[a] = ca[a] - axis[a] * hh;
+            B[a] = ca[a] + axis[a] * hh;
+        }
+        const float d[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        const float len = std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+        for (int a = 0; a < 3; ++a) ext[a] = std::fabs(axis[a]) * (0.5f * len) + r;
+    }
+    for (int a = 0; a < 3; ++a) {
+        k.aabb_min[a] = (ca[a] - ext[a]) - kBroadPhaseMargin;
+        k.aabb_max[a] = (ca[a] + ext[a]) + kBroadPhaseMargin;
+    }
+    for (int a = 0; a < 4; ++a) k.material[a] = g.material[a];
+    k.env = g.environment;
+    k.layer = g.layer;
+    k.mask = g.mask;
+    k.enabled = g.enabled;
+    k.index = index;
+    return k;
+}
+
+// Host restatement of the static-collider visit order: the enabled colliders (boxes, then shape
+// colliders), Morton codes of their broad-phase AABB centres in the static extent
+// (physics_rigid_morton_codes.cs.hlsl:8-50), stable order, walked right child first
+// (physics_particle_emit_fluid_boundary_candidates.cs.hlsl:158-225).
+std::vector<Collider> visit_ordered_colliders(const cn_box_collider *boxes, uint32_t nbox, const cn_collider *shapes,
+                                              uint32_t nshape)
+{
+    std::vector<Collider> all;
+    for (uint32_t i = 0; i < nbox; ++i)
+        if (boxes[i].enabled) all.push_back(box_collider(boxes[i], i));
+    for (uint32_t i = 0; i < nshape; ++i)
+        if (shapes[i].enabled) all.push_back(shape_collider(shapes[i], nbox + i));
     float gmn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
     float gmx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
-    for (const E &e : es)
+    for (const Collider &e : all)
         for (int a = 0; a < 3; ++a) {
-            gmn[a] = std::min(gmn[a], e.mn[a]);
-            gmx[a] = std::max(gmx[a], e.mx[a]);
+            gmn[a] = std::min(gmn[a], e.aabb_min[a]);
+            gmx[a] = std::max(gmx[a], e.aabb_max[a]);
         }
-    for (E &e : es) {
+    std::vector<std::pair<uint32_t, uint32_t>> code(all.size());  // (Morton code, position)
+    for (size_t k = 0; k < all.size(); ++k) {
         uint32_t q[3];
         for (int a = 0; a < 3; ++a) {
             const float size = std::max(gmx[a] - gmn[a], 1.0e-5f);
-            const float center = 0.5f * (e.mn[a] + e.mx[a]);
+            const float center = 0.5f * (all[k].aabb_min[a] + all[k].aabb_max[a]);
             float m = (center - gmn[a]) / size;
             m = std::min(std::max(m, 0.0f), 1.0f);
-            float s = m * 1024.0f;
-            s = std::min(std::max(s, 0.0f), 1023.0f);
-            q[a] = static_cast<uint32_t>(s);
+            float sc = m * 1024.0f;
+            sc = std::min(std::max(sc, 0.0f), 1023.0f);
+            q[a] = static_cast<uint32_t>(sc);
         }
-        e.code = expand10(q[0]) * 4u + expand10(q[1]) * 2u + expand10(q[2]);
+        code[k] = {expand10(q[0]) * 4u + expand10(q[1]) * 2u + expand10(q[2]), (uint32_t)k};
     }
-    std::stable_sort(es.begin(), es.end(), [](const E &x, const E &y) { return x.code < y.code; });
+    std::stable_sort(code.begin(), code.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
     std::vector<Collider> out;
-    for (auto it = es.rbegin(); it != es.rend(); ++it) {
-        const cn_box_collider &b = c[it->idx];
-        Collider k{};
-        for (int a = 0; a < 3; ++a) {
-            k.c[a] = b.center[a];
-            k.he[a] = b.half_extents[a];
-            k.aabb_min[a] = it->mn[a];
-            k.aabb_max[a] = it->mx[a];
-        }
-        for (int a = 0; a < 4; ++a) k.material[a] = b.material[a];
-        k.env = b.environment;
-        k.layer = b.layer;
-        k.mask = b.mask;
-        k.enabled = b.enabled;
-        k.index = it->idx;
-        out.push_back(k);
-    }
+    for (auto it = code.rbegin(); it != code.rend(); ++it) out.push_back(all[it->second]);
     return out;
 }
 
@@ -573,6 +689,7 @@ void cn_default_step_params(cn_step_params *p)
     std::memcpy(p->solver_config, cfg, sizeof(cfg));
     p->max_fluid_neighbors = 128;  // kDefaultFluidMaxNeighborhood, src/physics/rigid_body_common.h:304
     p->key_mode = CN_KEY_MORTON;
+    p->flags = CN_STEP_CANDIDATE_REUSE;  // same results, fewer neighbor walks
 }
 
 // toFluidSolverMaterial and its helpers, src/physics/physics_world.cpp:99-263
@@ -700,6 +817,7 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
     CN_HIP(ctx, dalloc(&s.blk_halo, (size_t)s.blk_cap * kHaloStride));
     CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
+    CN_HIP(ctx, dalloc(&s.fb_blk, s.blk_cap));
     CN_HIP(ctx, dalloc(&s.list, list_slots(N) * (cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
@@ -750,7 +868,12 @@ cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
             for (int k = 0; k < 4; ++k)
                 if (cv->tets[t].p[k] >= n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "tet particle index out of range");
     }
-    if (tb->collider_count > kMaxColliders) return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+    if ((uint64_t)tb->collider_count + tb->shape_collider_count > kMaxColliders)
+        return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+    if (tb->shape_collider_count && !tb->shape_colliders) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null shape colliders");
+    for (uint32_t k = 0; k < tb->shape_collider_count; ++k)
+        if (tb->shape_colliders[k].shape > CN_SHAPE_CAPSULE)
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "unknown collider shape");
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t kind = pv->kinds[i];
         const uint32_t fm = pv->fluid_material_indices ? pv->fluid_material_indices[i] : 0xffffffffu;
@@ -758,9 +881,16 @@ cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
             return fail(ctx, CN_ERR_INVALID_ARGUMENT, "fluid particle with an invalid fluid material index");
         if (pv->particle_material_indices[i] >= tb->particle_material_count)
             return fail(ctx, CN_ERR_INVALID_ARGUMENT, "invalid particle material index");
-        // self collision inside a phase group needs the group's constraint adjacency
-        if (kind == CN_PARTICLE_KIND_SOLID && pv->phases && (pv->phases[i] & 0x80000000u))
-            return fail(ctx, CN_ERR_UNSUPPORTED, "self-colliding solid phase group");
+    }
+    if (pv->adjacency_offsets) {  // constraint adjacency: every list inside the index array, in range
+        if (!pv->adjacency_counts || (pv->adjacency_index_count && !pv->adjacency_indices))
+            return fail(ctx, CN_ERR_INVALID_ARGUMENT, "adjacency offsets without counts or indices");
+        for (uint32_t i = 0; i < n; ++i)
+            if ((uint64_t)pv->adjacency_offsets[i] + pv->adjacency_counts[i] > pv->adjacency_index_count)
+                return fail(ctx, CN_ERR_INVALID_ARGUMENT, "adjacency list past the index array");
+        for (uint32_t k = 0; k < pv->adjacency_index_count; ++k)
+            if (pv->adjacency_indices[k] >= n)
+                return fail(ctx, CN_ERR_INVALID_ARGUMENT, "adjacency particle index out of range");
     }
     return CN_OK;
 }
@@ -811,15 +941,25 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     std::map<std::array<uint32_t, 9>, uint32_t> cmap;
     std::vector<ParticleClass> classes;
     std::vector<uint32_t> cls(n);
+    // constraint adjacency of the pool (pool particle indices), built only when a scene has one
+    bool any_adj = false;
+    for (uint32_t k = 0; k < count; ++k) any_adj |= descs[k].particles->adjacency_offsets != nullptr;
+    std::vector<uint32_t> adj_off, adj_cnt, adj_idx;
+    if (any_adj) {
+        adj_off.assign(n, 0u);
+        adj_cnt.assign(n, 0u);
+    }
     ctx->scene_first.assign(count + 1, 0);
     ctx->scene_cell.assign(count, 0.0f);
     ctx->scene_col_cap.assign(count, 0u);
+    ctx->scene_shapes.assign(count, {});
     ctx->scene_fm.assign(2 * (size_t)count, 0u);
     ctx->scene_con.assign(4 * (size_t)count, 0u);
     ctx->scene_pm.assign(2 * (size_t)count, 0u);
     ctx->has_fluid = false;
     ctx->has_solid = false;
     ctx->any_pairs = false;
+    ctx->any_rigid = false;
     ctx->has_vorticity = false;
     ctx->aniso_ready = false;
     ctx->fm_fluid_used.clear();
@@ -842,8 +982,15 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         std::memcpy(vel.data() + 4ull * base, pv->velocities, 4ull * m * sizeof(float));
         std::array<uint32_t, 9> last{};
         uint32_t last_id = 0xffffffffu;
-        bool scene_solid = false, scene_fluid = false, two_groups = false;
+        bool scene_solid = false, scene_fluid = false, two_groups = false, self_group = false;
         uint32_t group0 = 0xffffffffu;
+        if (pv->adjacency_offsets)
+            for (uint32_t i = 0; i < m; ++i) {
+                adj_off[base + i] = (uint32_t)adj_idx.size();
+                adj_cnt[base + i] = pv->adjacency_counts[i];
+                for (uint32_t q = 0; q < pv->adjacency_counts[i]; ++q)
+                    adj_idx.push_back(base + pv->adjacency_indices[pv->adjacency_offsets[i] + q]);
+            }
         for (uint32_t i = 0; i < m; ++i) {
             ParticleClass pc{};
             pc.radius = pv->radii[i];
@@ -868,6 +1015,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
                 const uint32_t g = pc.phase & 0x7fffffffu;
                 if (group0 == 0xffffffffu) group0 = g;
                 two_groups |= g != group0;
+                self_group |= (pc.phase & 0x80000000u) != 0u;
             }
             std::array<uint32_t, 9> key;
             std::memcpy(key.data(), &pc, sizeof(pc));
@@ -889,7 +1037,9 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             last_id = id;
         }
         // static colliders in the scene's own LBVH visit order
-        const std::vector<Collider> sc = visit_ordered_colliders(tb->colliders, tb->collider_count);
+        const std::vector<Collider> sc =
+            visit_ordered_colliders(tb->colliders, tb->collider_count, tb->shape_colliders, tb->shape_collider_count);
+        ctx->scene_shapes[k].assign(tb->shape_colliders, tb->shape_colliders + tb->shape_collider_count);
         sinfo[k].col0 = (uint32_t)col.size();
         sinfo[k].ncol = (uint32_t)sc.size();
         sinfo[k].fluid_only = (!scene_solid && (!cv || (cv->edge_count | cv->bend_count | cv->dihedral_count |
@@ -897,11 +1047,15 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
                                   ? 1u
                                   : 0u;
         sinfo[k].has_fluid = scene_fluid ? 1u : 0u;
-        sinfo[k].pairs = (scene_solid && scene_fluid) || two_groups ? 1u : 0u;
+        sinfo[k].pairs = (scene_solid && scene_fluid) || two_groups || self_group ? 1u : 0u;
         ctx->any_pairs |= sinfo[k].pairs != 0u;
+        // hasSoftRigidContactWork, physics_solver.cpp:486-488, for a scene whose solid particles
+        // can meet a static collider (disabled colliders count: an edit may enable them)
+        sinfo[k].rigid = scene_solid && tb->collider_count + tb->shape_collider_count > 0 ? 1u : 0u;
+        ctx->any_rigid |= sinfo[k].rigid != 0u;
         col.insert(col.end(), sc.begin(), sc.end());
         // slots for the disabled colliders too, so cn_scene_set_colliders can enable them in place
-        col.resize(col.size() + (tb->collider_count - sc.size()));
+        col.resize(col.size() + (tb->collider_count + tb->shape_collider_count - sc.size()));
         ctx->scene_col_cap[k] = tb->collider_count;
         // constraints, shifted to pool indices
         ctx->scene_con[4 * k] = (uint32_t)edges.size();
@@ -993,6 +1147,19 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     s.colliders = nullptr;
     s.scenes = nullptr;
     s.fluid_t2 = nullptr;
+    dfree(s.adj_off);
+    dfree(s.adj_cnt);
+    dfree(s.adj_idx);
+    s.adj_off = s.adj_cnt = s.adj_idx = nullptr;
+    if (any_adj) {
+        CN_HIP(ctx, dalloc(&s.adj_off, n));
+        CN_HIP(ctx, dalloc(&s.adj_cnt, n));
+        CN_HIP(ctx, dalloc(&s.adj_idx, adj_idx.size()));
+        CN_HIP(ctx, hipMemcpy(s.adj_off, adj_off.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        CN_HIP(ctx, hipMemcpy(s.adj_cnt, adj_cnt.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!adj_idx.empty())
+            CN_HIP(ctx, hipMemcpy(s.adj_idx, adj_idx.data(), adj_idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     s.nclasses = (uint32_t)classes.size();
     CN_HIP(ctx, dalloc(&s.classes, classes.size()));
     CN_HIP(ctx, hipMemcpy(s.classes, classes.data(), classes.size() * sizeof(ParticleClass), hipMemcpyHostToDevice));
@@ -1048,16 +1215,18 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     // substep's candidate lists)
     CN_HIP(ctx, hipMemset(s.iter_state, 0, kStateWords * sizeof(uint32_t)));
 
-    // particle-particle contact state: candidate lists and the last contacts, kContactPartners
-    // per particle (only a scene with a solid particle has contacts)
-    if (ctx->has_solid && n > 1 && s.ct_alloc < n) {
+    // contact state: particle-particle candidate lists and the last contacts, kContactPartners
+    // per particle, and the particle-rigid generation positions (only a scene with a solid
+    // particle has contacts)
+    if (ctx->has_solid && s.ct_alloc < n) {
         dfree(s.ct_list);
         dfree(s.ct_count);
         dfree(s.ct_rec);
         dfree(s.ct_bits);
         dfree(s.ct_corr);
+        dfree(s.ct_gen);
         s.ct_list = s.ct_count = s.ct_bits = nullptr;
-        s.ct_rec = s.ct_corr = nullptr;
+        s.ct_rec = s.ct_corr = s.ct_gen = nullptr;
         s.ct_alloc = 0;
         const size_t m = std::max<size_t>(n, ctx->capacity);
         CN_HIP(ctx, dalloc(&s.ct_list, m * kContactPartners));
@@ -1065,6 +1234,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         CN_HIP(ctx, dalloc(&s.ct_count, m));
         CN_HIP(ctx, dalloc(&s.ct_bits, m));
         CN_HIP(ctx, dalloc(&s.ct_corr, m));
+        CN_HIP(ctx, dalloc(&s.ct_gen, m));
         s.ct_alloc = m;
     }
 
@@ -1214,10 +1384,11 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
     const uint32_t contact_iters = resolve_iters(p->soft_contact_iterations, def);
     const uint32_t max_iters = std::max(std::max(fluid_iters, soft_iters), std::max(contact_iters, def));
     const bool has_soft = s.ne + s.nb + s.nd + s.nt > 0;
-    // particle-particle contacts need a solid particle (fluid pairs are never candidates)
-    // (a pool where no scene can hold a pair skips them: their zero corrections change nothing but
-    // the sign of zero coordinates)
-    const bool has_contacts = contact_iters > 0 && s.n > 1 && ctx->any_pairs && s.ct_alloc >= s.n;
+    // contacts need a solid particle (fluid pairs are never candidates, and static bodies take
+    // solid particles only); a pool where no scene can hold a contact skips them, and so does each
+    // such scene of a pool: their zero corrections change nothing but the sign of zero coordinates
+    const bool has_contacts =
+        contact_iters > 0 && s.ct_alloc >= s.n && ((s.n > 1 && ctx->any_pairs) || ctx->any_rigid);
     SoftArgs sa{};
     if (has_soft) {
         sa.slot_of = s.slot_of;
@@ -1301,9 +1472,14 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
             ProfScope ps(ctx, "fluid_render_anisotropy");
             launch_fluid_render_aniso(s, c, s.aniso, ctx->stream);
         }
-        if (has_contacts) {
+        if (has_contacts && ctx->any_pairs) {
             ProfScope ps(ctx, "contact_velocity");
             for (uint32_t v = 0; v < contact_iters; ++v) launch_contact_velocity(s, c, ctx->stream);
+        }
+        // physics_solver.cpp:1301-1309, after the particle-particle velocity iterations
+        if (has_contacts && ctx->any_rigid) {
+            ProfScope ps(ctx, "rigid_velocity");
+            launch_rigid_velocity(s, c, contact_iters, ctx->stream);
         }
     }
     {
@@ -1362,12 +1538,13 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
     if (p->max_fluid_neighbors > ctx->cap_alloc)
         return fail(ctx, CN_ERR_CAPACITY, "max_fluid_neighbors exceeds the neighbor slots of cn_scene_reserve");
     CN_HIP(ctx, hipSetDevice(ctx->device));
-    if ((p->flags & CN_STEP_CANDIDATE_REUSE) && !ctx->s.slist) {  // candidate lists on first use
-        if ((uint64_t)list_slots(ctx->capacity) * kSuperSlots * sizeof(uint16_t) > kListBytesMax)
-            return fail(ctx, CN_ERR_CAPACITY, "candidate lists of this many particles exceed 4 GiB");
+    // candidate lists on first use, for scenes with fluid, when they fit the 32-bit list addressing
+    if ((p->flags & CN_STEP_CANDIDATE_REUSE) && !ctx->s.slist && ctx->has_fluid &&
+        (uint64_t)list_slots(ctx->capacity) * kSuperSlots * sizeof(uint16_t) <= kListBytesMax) {
         CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
         CN_HIP(ctx, dalloc(&ctx->s.slist, list_slots(ctx->capacity) * kSuperSlots));
         CN_HIP(ctx, dalloc(&ctx->s.scnt, std::max<uint32_t>(ctx->capacity, 1u)));
+        CN_HIP(ctx, dalloc(&ctx->s.disp, std::max<uint32_t>(ctx->capacity, 1u)));
         ctx->graph_valid = false;
     }
     if (ctx->aniso_enabled && ctx->s.aniso_alloc < ctx->s.n) {  // output on first use
@@ -1614,6 +1791,15 @@ cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions
     if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
     if (scene >= ctx->nscenes) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "scene index out of range");
     const uint32_t first = ctx->scene_first[scene], count = ctx->scene_first[scene + 1] - first;
+    return cn_particles_update_range(ctx, first, count, positions_inv_mass, velocities, flags);
+}
+
+cn_status cn_particles_update_range(cn_ctx *ctx, uint32_t first, uint32_t count, const float *positions_inv_mass,
+                                    const float *velocities, uint32_t flags)
+{
+    if (!ctx || (count && (!positions_inv_mass || !velocities))) return CN_ERR_INVALID_ARGUMENT;
+    if (!ctx->uploaded) return fail(ctx, CN_ERR_NOT_READY, "no scene");
+    if ((uint64_t)first + count > ctx->s.n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "particle run out of range");
     if (count == 0) return CN_OK;
     CN_HIP(ctx, hipSetDevice(ctx->device));
     const float4 *P = reinterpret_cast<const float4 *>(positions_inv_mass);
@@ -1650,12 +1836,14 @@ cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_colli
         return fail(ctx, CN_ERR_INVALID_ARGUMENT, "collider count differs from the scene's upload; re-upload to add or remove colliders");
     if (count == 0) return CN_OK;
     CN_HIP(ctx, hipSetDevice(ctx->device));
-    std::vector<Collider> sc = visit_ordered_colliders(colliders, count);
+    const std::vector<cn_collider> &shapes = ctx->scene_shapes[scene];
+    std::vector<Collider> sc = visit_ordered_colliders(colliders, count, shapes.data(), (uint32_t)shapes.size());
     const uint32_t ncol = (uint32_t)sc.size();
-    sc.resize(cap);
+    const size_t slots = cap + shapes.size();  // the scene's collider slots: boxes and shape colliders
+    sc.resize(slots);
     SceneInfo &si = ctx->scene_info[scene];
     si.ncol = ncol;
-    CN_HIP(ctx, hipMemcpyAsync(ctx->s.colliders + si.col0, sc.data(), cap * sizeof(Collider), hipMemcpyHostToDevice,
+    CN_HIP(ctx, hipMemcpyAsync(ctx->s.colliders + si.col0, sc.data(), slots * sizeof(Collider), hipMemcpyHostToDevice,
                                ctx->stream));
     CN_HIP(ctx, hipMemcpyAsync(ctx->s.scenes + scene, &si, sizeof(SceneInfo), hipMemcpyHostToDevice, ctx->stream));
     // the host sources may go as soon as the call returns

@@ -15,8 +15,21 @@
 // partners in its own 27-cell walk order, evaluates each pair's contact itself (the arithmetic of
 // the pair is the same from either side: lower index first) and sums its side's corrections in
 // that order, so results are deterministic and equal to the oracle's.  Contacts are Jacobi: a
-// solve pass writes every particle's summed correction, an apply pass adds them.  Only particle-
-// particle contacts of free particles exist here (no rigid proxies, no suturing).
+// solve pass writes every particle's summed correction, an apply pass adds them.  Particle-
+// particle contacts are those of free particles (no rigid proxies, no suturing).  Solids of one
+// phase group pair up only when the group collides with itself, minus constraint-adjacent pairs.
+//
+// Particle-rigid contacts: solid particles against the scene's static colliders, every collider a
+// static body of its own (buildParticleRigidCandidatePairs with
+// physics_particle_count_particle_rigid_candidates.cs.hlsl:112-151; per contact iteration
+// physics_particle_rigid_generate_contacts.cs.hlsl:30-190 and
+// physics_particle_rigid_solve_contacts.cs.hlsl:26-140 into the same correction, after the
+// particle-particle terms; after the particle-particle velocity iterations
+// physics_particle_rigid_solve_contact_velocities.cs.hlsl:27-130 with its apply per iteration).
+// The candidates are the particle's collider bitmask of the reorder pass, in LBVH visit order.  A
+// static body has zero inverse mass and inertia and does not move: its effective mass is zero and
+// its contact point's displacement and velocity are exactly zero.  The velocity pass regenerates
+// each contact from the position of the last contact iteration, which is kept per particle.
 #include <hip/hip_runtime.h>
 
 #include "cn_internal.h"
@@ -25,6 +38,7 @@
 namespace cn {
 
 constexpr uint32_t kPhaseGroupMask = 0x7fffffffu;
+constexpr uint32_t kPhaseSelfCollide = 0x80000000u;
 constexpr int kCfgSoftContactRelax = 2;
 constexpr int kCfgSoftMaxCorrection = 3;
 constexpr int kCfgRestitutionVelocity = 4;
@@ -49,6 +63,12 @@ struct ContactArgs {
     uint32_t *cbits;          // bit k: partner k's contact is active
     float4 *corr;             // summed correction of the pass
     uint32_t *meta;
+    const uint32_t *adj_off;  // constraint adjacency by particle index (null: none)
+    const uint32_t *adj_cnt;
+    const uint32_t *adj_idx;
+    const Collider *col;      // static colliders in visit order (SceneInfo.col0 per scene)
+    const uint32_t *bnd;      // per slot: candidate collider bitmask (a solid's rigid candidates)
+    float4 *rgen;             // per slot: position of the last contact generation
     uint32_t n, pcap;
 };
 
@@ -57,17 +77,31 @@ __device__ __forceinline__ const ParticleClass &pclass(const ContactArgs &a, uin
     return a.classes[a.cls[slot]];
 }
 
-// IsValidParticleParticleCandidate for the pair (lo, hi) = slots ordered by particle index.
+// IsAdjacentParticle, physics_particle_broad_phase_query.hlsli:50-62: particle index `other` among
+// the adjacency list of particle index `self`.
+__device__ __forceinline__ bool adjacent(const ContactArgs &a, uint32_t self, uint32_t other)
+{
+    if (!a.adj_off) return false;
+    const uint32_t off = a.adj_off[self], cnt = a.adj_cnt[self];
+    for (uint32_t k = 0; k < cnt; ++k)
+        if (a.adj_idx[off + k] == other) return true;
+    return false;
+}
+
+// IsValidParticleParticleCandidate for the pair (lo, hi) = slots ordered by particle index (lo is
+// the reference's particleIndex, the lower one).
 __device__ __forceinline__ bool pair_valid(const ContactArgs &a, uint32_t lo, uint32_t hi)
 {
     const ParticleClass &A = pclass(a, lo), &B = pclass(a, hi);
     if (A.env != B.env) return false;
     if ((A.mask & B.layer) == 0u || (B.mask & A.layer) == 0u) return false;
     if (A.kind == CN_PARTICLE_KIND_FLUID && B.kind == CN_PARTICLE_KIND_FLUID) return false;
-    // self collision of one phase group needs adjacency lists; such scenes are refused on upload
+    // one phase group: only when both have self collision on and they share no constraint
     if (A.kind == CN_PARTICLE_KIND_SOLID && B.kind == CN_PARTICLE_KIND_SOLID &&
-        (A.phase & kPhaseGroupMask) == (B.phase & kPhaseGroupMask))
-        return false;
+        (A.phase & kPhaseGroupMask) == (B.phase & kPhaseGroupMask)) {
+        if (!(A.phase & kPhaseSelfCollide) || !(B.phase & kPhaseSelfCollide) || adjacent(a, a.id[lo], a.id[hi]))
+            return false;
+    }
     const float4 pa = a.pos[lo], pb = a.pos[hi];
     const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
     const float cr = A.radius + B.radius;
@@ -111,6 +145,63 @@ __device__ __forceinline__ float3 friction_delta(const float *cfg, float ex, flo
     return make_float3(ex * ks, ey * ks, ez * ks);
 }
 
+// physics_particle_rigid_generate_contacts.cs.hlsl:82-176 for one static collider: the position
+// in the collider frame, the shape's signed distance and local normal, penetration = radius -
+// distance.  Returns true with (world normal, penetration) when the penetration is positive; the
+// world normal is SafeNormalize of the rotated local normal.
+__device__ __forceinline__ bool rigid_contact(const Collider &c, float px, float py, float pz, float radius,
+                                              float4 &rec)
+{
+    if (!c.enabled) return false;
+    float ux, uy, uz;
+    quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], __fsub_rn(px, c.c[0]), __fsub_rn(py, c.c[1]),
+                __fsub_rn(pz, c.c[2]), ux, uy, uz);  // QuaternionInverseRotate
+    float nx = 0.0f, ny = 1.0f, nz = 0.0f, sd = 0.0f;
+    if (c.shape == CN_SHAPE_SPHERE) {
+        const float len = len3(ux, uy, uz);
+        if (len > kEps) {
+            nx = fdiv(ux, len); ny = fdiv(uy, len); nz = fdiv(uz, len);
+        }
+        sd = __fsub_rn(len, c.he[0]);
+    } else if (c.shape == CN_SHAPE_BOX) {
+        box_local_contact(c.he, ux, uy, uz, sd, nx, ny, nz);
+    } else {
+        // capsule: the closest point of the world segment; the world normal (fallback +x) is taken
+        // into the collider frame
+        const float gx = __fsub_rn(c.sb[0], c.sa[0]), gy = __fsub_rn(c.sb[1], c.sa[1]), gz = __fsub_rn(c.sb[2], c.sa[2]);
+        const float lsq = dot3(gx, gy, gz, gx, gy, gz);
+        float t = 0.0f;
+        if (lsq > kEps) {
+            const float wx = __fsub_rn(px, c.sa[0]), wy = __fsub_rn(py, c.sa[1]), wz = __fsub_rn(pz, c.sa[2]);
+            t = fsat(fdiv(dot3(wx, wy, wz, gx, gy, gz), lsq));
+        }
+        const float ex = __fsub_rn(px, __fadd_rn(c.sa[0], __fmul_rn(t, gx)));
+        const float ey = __fsub_rn(py, __fadd_rn(c.sa[1], __fmul_rn(t, gy)));
+        const float ez = __fsub_rn(pz, __fadd_rn(c.sa[2], __fmul_rn(t, gz)));
+        const float len = len3(ex, ey, ez);
+        float wx = 1.0f, wy = 0.0f, wz = 0.0f;
+        if (len > kEps) {
+            wx = fdiv(ex, len); wy = fdiv(ey, len); wz = fdiv(ez, len);
+        }
+        sd = __fsub_rn(len, c.he[0]);
+        quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], wx, wy, wz, nx, ny, nz);
+    }
+    const float pen = __fsub_rn(radius, sd);
+    if (!(pen > 0.0f)) return false;
+    float wx, wy, wz;
+    quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], nx, ny, nz, wx, wy, wz);
+    const float3 n = safe_normalize(wx, wy, wz);
+    rec = make_float4(n.x, n.y, n.z, pen);
+    return true;
+}
+
+// CombineContactMaterial of a particle material and a collider's
+__device__ __forceinline__ float3 rigid_material(const float4 pm, const float *cm)
+{
+    return make_float3(fsqrt(fmaxd(0.0f, pm.x) * fmaxd(0.0f, cm[0])), fmaxd(fmaxd(0.0f, pm.y), fmaxd(0.0f, cm[1])),
+                       fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, cm[3])));
+}
+
 // ---------------------------------------------------------------------------------------------
 // candidates
 // ---------------------------------------------------------------------------------------------
@@ -145,7 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_contact_solve(ContactArgs a, Substep
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n) return;
-    if (!a.active[pclass(a, t).scene]) return;
+    const ParticleClass &pc = pclass(a, t);
+    if (!a.active[pc.scene]) return;
+    const SceneInfo sc = a.scenes[pc.scene];
+    if (!(sc.pairs | sc.rigid)) return;
     const uint32_t cnt = a.pcount[t];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     uint32_t bits = 0;
@@ -225,6 +319,48 @@ __global__ __launch_bounds__(kBlock) void k_contact_solve(ContactArgs a, Substep
         }
     }
     a.cbits[t] = bits;
+    // particle-rigid contacts of a solid particle (rigid solve_contacts :45-140 with a static body)
+    const uint32_t rm = pc.kind == CN_PARTICLE_KIND_SOLID ? a.bnd[t] : 0u;
+    if (rm) {
+        const float4 p = a.pos[t];
+        a.rgen[t] = p;
+        const float4 pv = a.prev[t];
+        const float4 pm = a.part_mats[pc.part_mat];
+        const float w = p.w;
+        for (uint32_t m = rm; m; m &= m - 1u) {
+            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+            float4 rec;
+            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) continue;
+            if (w <= kEps) continue;
+            const float pn = fmind(fmaxd(rec.w - c.cfg[kCfgContactSlop], 0.0f), c.cfg[kCfgSoftMaxCorrection]);
+            if (pn <= 0.0f) continue;
+            const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
+            const float3 mat = rigid_material(pm, cc.material);
+            const float denom = w + 0.0f;
+            if (denom <= kEps) continue;
+            const float lambda = fdiv(pn, denom) * relax;
+            float cx = nr.x * (w * lambda), cy = nr.y * (w * lambda), cz = nr.z * (w * lambda);
+            // relative displacement = (contact point - its previous position) - (x - x_prev), the
+            // first term +0 for a static body
+            const float rx = 0.0f - (p.x - pv.x), ry = 0.0f - (p.y - pv.y), rz = 0.0f - (p.z - pv.z);
+            const float rn = dot3(rx, ry, rz, nr.x, nr.y, nr.z);
+            const float3 fd = friction_delta(c.cfg, rx - nr.x * rn, ry - nr.y * rn, rz - nr.z * rn, pn, mat.x, mat.z);
+            const float fdist = len3(fd.x, fd.y, fd.z);
+            if (fdist > 0.0f) {
+                const float tden = w + 0.0f;
+                if (tden > kEps) {
+                    const float tl = fdiv(fdist, tden);
+                    const float k = (tl * w) * relax;
+                    cx = cx + fdiv(fd.x, fdist) * k;
+                    cy = cy + fdiv(fd.y, fdist) * k;
+                    cz = cz + fdiv(fd.z, fdist) * k;
+                }
+            }
+            sx = sx + cx;
+            sy = sy + cy;
+            sz = sz + cz;
+        }
+    }
     a.corr[t] = make_float4(sx, sy, sz, 0.0f);
 }
 
@@ -236,6 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_contact_apply(ContactArgs a)
     if (t >= a.n) return;
     const ParticleClass &pc = pclass(a, t);
     if (!a.active[pc.scene]) return;
+    const SceneInfo &sc = a.scenes[pc.scene];
+    if (!(sc.pairs | sc.rigid)) return;
     float4 p = a.pos[t];
     if (p.w <= kEps) return;
     const float4 d = a.corr[t];
@@ -256,7 +394,8 @@ __global__ __launch_bounds__(kBlock) void k_contact_velocity(ContactArgs a, Subs
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n) return;
-    if (!a.active[pclass(a, t).scene]) return;
+    const uint32_t scene = pclass(a, t).scene;
+    if (!a.active[scene] || !a.scenes[scene].pairs) return;
     const uint32_t cnt = a.pcount[t], bits = a.cbits[t];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -300,13 +439,60 @@ __global__ __launch_bounds__(kBlock) void k_contact_velocity_apply(ContactArgs a
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n) return;
-    if (!a.active[pclass(a, t).scene]) return;
+    const uint32_t scene = pclass(a, t).scene;
+    if (!a.active[scene] || !a.scenes[scene].pairs) return;
     if (a.pos[t].w <= kEps) return;
     float4 v = a.vel[t];
     const float4 d = a.corr[t];
     v.x = v.x + d.x;
     v.y = v.y + d.y;
     v.z = v.z + d.z;
+    a.vel[t] = v;
+}
+
+// The particle-rigid velocity iterations (rigid solve_contact_velocities :44-128 with a static
+// body, each followed by apply_contact_velocities): a particle's correction reads only its own
+// velocity and the static colliders, so all iterations of a particle run in one thread, each
+// adding its summed correction (zero without contacts) to a free particle's velocity.
+__global__ __launch_bounds__(kBlock) void k_rigid_velocity(ContactArgs a, SubstepConsts c, uint32_t iters)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n) return;
+    const ParticleClass &pc = pclass(a, t);
+    if (!a.active[pc.scene]) return;
+    const SceneInfo sc = a.scenes[pc.scene];
+    if (!sc.rigid) return;
+    const float w = a.pos[t].w;
+    if (w <= kEps) return;
+    const uint32_t rm = pc.kind == CN_PARTICLE_KIND_SOLID ? a.bnd[t] : 0u;
+    const float4 g = rm ? a.rgen[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 pm = a.part_mats[pc.part_mat];
+    float4 v = a.vel[t];
+    for (uint32_t it = 0; it < iters; ++it) {
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (uint32_t m = rm; m; m &= m - 1u) {
+            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+            float4 rec;
+            if (!rigid_contact(cc, g.x, g.y, g.z, pc.radius, rec)) continue;
+            const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
+            // the contact point's velocity is +0, so the relative velocity is v - 0 = v
+            const float nv = dot3(v.x, v.y, v.z, nr.x, nr.y, nr.z);
+            if (nv >= 0.0f) continue;
+            const float denom = w + 0.0f;
+            if (denom <= kEps) continue;
+            const bool rest_on = (-nv > c.cfg[kCfgRestitutionVelocity]) &&
+                                 (rec.w <= c.cfg[kCfgRestitutionPenetration] * c.cfg[kCfgContactSlop]);
+            const float restitution = rest_on ? fsat(rigid_material(pm, cc.material).y) : 0.0f;
+            const float desired = rest_on ? (-restitution * nv) : 0.0f;
+            const float imp = fmaxd(0.0f, fdiv(desired - nv, denom));
+            sx = sx + (nr.x * imp) * w;
+            sy = sy + (nr.y * imp) * w;
+            sz = sz + (nr.z * imp) * w;
+        }
+        v.x = v.x + sx;
+        v.y = v.y + sy;
+        v.z = v.z + sz;
+    }
     a.vel[t] = v;
 }
 
@@ -332,6 +518,12 @@ static ContactArgs contact_args(const SceneBuffers &s, const SubstepConsts &c, i
     a.cbits = s.ct_bits;
     a.corr = s.ct_corr;
     a.meta = s.meta;
+    a.adj_off = s.adj_off;
+    a.adj_cnt = s.adj_cnt;
+    a.adj_idx = s.adj_idx;
+    a.col = s.colliders;
+    a.bnd = s.bnd;
+    a.rgen = s.ct_gen;
     a.n = c.n;
     a.pcap = kContactPartners;
     return a;
@@ -357,6 +549,12 @@ void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipS
     const ContactArgs a = contact_args(s, c, 0);
     hipLaunchKernelGGL(k_contact_velocity, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c);
     hipLaunchKernelGGL(k_contact_velocity_apply, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a);
+}
+
+void launch_rigid_velocity(const SceneBuffers &s, const SubstepConsts &c, uint32_t iters, hipStream_t st)
+{
+    const ContactArgs a = contact_args(s, c, 0);
+    hipLaunchKernelGGL(k_rigid_velocity, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c, iters);
 }
 
 }  // namespace cn

@@ -56,19 +56,28 @@ struct SceneInfo {
                           // within a substep only by their deltas
     uint32_t has_fluid;   // 1: some fluid particle (the fluid passes have work in the scene)
     uint32_t pairs;       // 1: a particle-particle contact pair can exist (a solid particle and a
-                          // fluid one, or solid particles of two phase groups)
+                          // fluid one, solid particles of two phase groups, or a self-colliding
+                          // phase group)
+    uint32_t rigid;       // 1: particle-rigid contacts can exist (a solid particle and a static
+                          // collider slot)
 };
 
-// A static box collider in LBVH visit order, with its broad-phase AABB precomputed.
+// A static collider in LBVH visit order, its world pose resolved on the host once (the body pose
+// composed with the collider's local pose, quaternions normalized) and its broad-phase AABB
+// precomputed.
 struct Collider {
-    float c[3];
-    float he[3];
+    float c[3];          // world position of the collider
+    float he[3];         // box: half extents; sphere and capsule: radius in he[0]
     float aabb_min[3];
     float aabb_max[3];
     float material[4];
     uint32_t env, layer, mask, enabled;
-    uint32_t index;  // collider index in the caller's table
-    uint32_t pad[3];
+    uint32_t index;      // collider index in the caller's tables (boxes, then shape colliders)
+    uint32_t shape;      // CN_SHAPE_*
+    uint32_t rotated;    // 0: identity orientation (the box path rotates nothing)
+    uint32_t pad;
+    float q[4];          // world orientation (x, y, z, w), normalized
+    float sa[4], sb[4];  // capsule segment ends (xyz)
 };
 
 // Solver-wide scalars of one substep.
@@ -233,32 +242,85 @@ __device__ __forceinline__ float cohesion_q(float d, float h, float q, float c1,
     return __fsub_rn(__fadd_rn(a, b), 1.0f);
 }
 
-// ComputeFluidBoundaryContactInfo for an identity-oriented box,
-// physics_fluid_boundary_common.hlsli:10-66 with ClosestFaceNormalLocal
-// (physics_shape_common.hlsli:100-108).
-__device__ __forceinline__ bool box_contact(const Collider &c, float px, float py, float pz,
-                                            float &sd, float &nx, float &ny, float &nz)
+// QuaternionRotate (physics_math.hlsli:33-37): v + w t + q x t with t = 2 (q x v), left to right.
+__device__ __forceinline__ void quat_rotate(float qx, float qy, float qz, float qw, float vx, float vy, float vz,
+                                            float &ox, float &oy, float &oz)
 {
-    nx = 0.0f; ny = 1.0f; nz = 0.0f; sd = 0.0f;
-    if (!c.enabled) return false;
-    const float lx = __fsub_rn(px, c.c[0]), ly = __fsub_rn(py, c.c[1]), lz = __fsub_rn(pz, c.c[2]);
-    const float dx = __fsub_rn(lx, fclamp(lx, -c.he[0], c.he[0]));
-    const float dy = __fsub_rn(ly, fclamp(ly, -c.he[1], c.he[1]));
-    const float dz = __fsub_rn(lz, fclamp(lz, -c.he[2], c.he[2]));
+    const float tx = 2.0f * __fsub_rn(__fmul_rn(qy, vz), __fmul_rn(qz, vy));
+    const float ty = 2.0f * __fsub_rn(__fmul_rn(qz, vx), __fmul_rn(qx, vz));
+    const float tz = 2.0f * __fsub_rn(__fmul_rn(qx, vy), __fmul_rn(qy, vx));
+    ox = __fadd_rn(__fadd_rn(vx, __fmul_rn(qw, tx)), __fsub_rn(__fmul_rn(qy, tz), __fmul_rn(qz, ty)));
+    oy = __fadd_rn(__fadd_rn(vy, __fmul_rn(qw, ty)), __fsub_rn(__fmul_rn(qz, tx), __fmul_rn(qx, tz)));
+    oz = __fadd_rn(__fadd_rn(vz, __fmul_rn(qw, tz)), __fsub_rn(__fmul_rn(qx, ty), __fmul_rn(qy, tx)));
+}
+
+// The box branch of ComputeFluidBoundaryContactInfo in the box's frame (local position l):
+// closest point, outside distance and normal, or the inside depth and ClosestFaceNormalLocal
+// (physics_shape_common.hlsli:100-108).
+__device__ __forceinline__ void box_local_contact(const float *he, float lx, float ly, float lz, float &sd,
+                                                  float &nx, float &ny, float &nz)
+{
+    const float dx = __fsub_rn(lx, fclamp(lx, -he[0], he[0]));
+    const float dy = __fsub_rn(ly, fclamp(ly, -he[1], he[1]));
+    const float dz = __fsub_rn(lz, fclamp(lz, -he[2], he[2]));
     const float dl = len3(dx, dy, dz);
     if (dl > kEps) {
         nx = fdiv(dx, dl); ny = fdiv(dy, dl); nz = fdiv(dz, dl);
         sd = dl;
     } else {
-        const float ex = __fsub_rn(c.he[0], fabsf(lx));
-        const float ey = __fsub_rn(c.he[1], fabsf(ly));
-        const float ez = __fsub_rn(c.he[2], fabsf(lz));
+        const float ex = __fsub_rn(he[0], fabsf(lx));
+        const float ey = __fsub_rn(he[1], fabsf(ly));
+        const float ez = __fsub_rn(he[2], fabsf(lz));
         nx = 0.0f; ny = 0.0f; nz = 0.0f;
         if (ex <= ey && ex <= ez) nx = lx >= 0.0f ? 1.0f : -1.0f;
         else if (ey <= ez) ny = ly >= 0.0f ? 1.0f : -1.0f;
         else nz = lz >= 0.0f ? 1.0f : -1.0f;
         sd = -fmind(ex, fmind(ey, ez));
     }
+}
+
+// ComputeFluidBoundaryContactInfo, physics_fluid_boundary_common.hlsli:10-84: signed distance and
+// outward normal of point p against a sphere, a box or a capsule.  A box with identity orientation
+// takes its frame without a rotation (the quaternion rotation by the identity changes at most the
+// sign of a zero).
+__device__ __forceinline__ bool collider_contact(const Collider &c, float px, float py, float pz, float &sd,
+                                                 float &nx, float &ny, float &nz)
+{
+    nx = 0.0f; ny = 1.0f; nz = 0.0f; sd = 0.0f;
+    if (!c.enabled) return false;
+    if (c.shape == CN_SHAPE_BOX) {
+        const float lx = __fsub_rn(px, c.c[0]), ly = __fsub_rn(py, c.c[1]), lz = __fsub_rn(pz, c.c[2]);
+        if (!c.rotated) {
+            box_local_contact(c.he, lx, ly, lz, sd, nx, ny, nz);
+            return true;
+        }
+        float ux, uy, uz, mx, my, mz;
+        quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], lx, ly, lz, ux, uy, uz);  // QuaternionInverseRotate
+        box_local_contact(c.he, ux, uy, uz, sd, mx, my, mz);
+        quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], mx, my, mz, nx, ny, nz);
+        return true;
+    }
+    float dx, dy, dz;
+    if (c.shape == CN_SHAPE_SPHERE) {
+        dx = __fsub_rn(px, c.c[0]); dy = __fsub_rn(py, c.c[1]); dz = __fsub_rn(pz, c.c[2]);
+    } else {  // capsule: closest point of the segment, t = saturate(dot(p - a, b - a) / |b - a|^2)
+        const float gx = __fsub_rn(c.sb[0], c.sa[0]), gy = __fsub_rn(c.sb[1], c.sa[1]), gz = __fsub_rn(c.sb[2], c.sa[2]);
+        const float lsq = dot3(gx, gy, gz, gx, gy, gz);
+        float t = 0.0f;
+        if (lsq > kEps) {
+            const float wx = __fsub_rn(px, c.sa[0]), wy = __fsub_rn(py, c.sa[1]), wz = __fsub_rn(pz, c.sa[2]);
+            t = fsat(fdiv(dot3(wx, wy, wz, gx, gy, gz), lsq));
+        }
+        // lerp(a, b, t) = a + t (b - a)
+        dx = __fsub_rn(px, __fadd_rn(c.sa[0], __fmul_rn(t, gx)));
+        dy = __fsub_rn(py, __fadd_rn(c.sa[1], __fmul_rn(t, gy)));
+        dz = __fsub_rn(pz, __fadd_rn(c.sa[2], __fmul_rn(t, gz)));
+    }
+    const float dist = len3(dx, dy, dz);
+    if (dist > kEps) {
+        nx = fdiv(dx, dist); ny = fdiv(dy, dist); nz = fdiv(dz, dist);
+    }
+    sd = __fsub_rn(dist, c.he[0]);
     return true;
 }
 
@@ -268,7 +330,7 @@ __device__ __forceinline__ bool ghost_delta(const Collider &c, float px, float p
 {
     float sd, nx, ny, nz;
     gx = gy = gz = 0.0f;
-    if (!box_contact(c, px, py, pz, sd, nx, ny, nz)) return false;
+    if (!collider_contact(c, px, py, pz, sd, nx, ny, nz)) return false;
     if (sd < 0.0f || sd >= h) return false;
     const float t = __fmul_rn(2.0f, sd);
     gx = __fmul_rn(nx, t); gy = __fmul_rn(ny, t); gz = __fmul_rn(nz, t);

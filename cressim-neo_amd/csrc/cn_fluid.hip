@@ -47,10 +47,11 @@ struct FluidArgs {
     float4 *nc;               // surface normal * tension, C
     uint32_t *nbr;
     uint32_t *nbr_cnt;
-    uint16_t *list;           // listed neighbors as halo offsets in slot groups (list_group_base), LDS blocks
+    uint16_t *list;           // listed neighbors as halo offsets in slot groups (list_entry_base)
     uint32_t *lcnt;           // listed neighbor count
     uint16_t *slist;          // iteration-0 candidates as halo offsets in slot groups, LDS blocks
     uint16_t *scnt;           // candidate count, or kSuperNone
+    float *disp;              // per slot: bound on the displacement since iteration 0 (null: not kept)
     const uint32_t *run_key;
     const uint32_t *run_start;
     const int4 *run_info;
@@ -60,6 +61,7 @@ struct FluidArgs {
     const uint32_t *nblk;
     uint32_t *blk_halo;
     uint32_t *wide_blk;
+    uint32_t *fb_blk;         // blocks the candidate-list pass leaves to the walking pass
     uint32_t *iter_state;
     const ParticleClass *classes;
     const cn_fluid_material *fm;
@@ -263,14 +265,14 @@ struct DensitySelf {
     bool tension;
     bool fd;          // the neighbor quotients may use fdiv_rcp (pair_quot)
     bool sup;         // candidate lists apply (fluid-only scene, tiled pass)
-    float ts2, allow; // candidate threshold; largest max |delta|^2 under which they stay complete
+    float ts2, allow; // candidate threshold; bound on D_i + D_j under which they stay complete
 };
 
 // Returns false (and writes the unreachable record) for particles that take no part in the solve.
-// dprev >= 0: candidate lists apply to fluid-only scenes, with a skin from dprev, the previous
-// substep's largest displacement.
+// sup_on: candidate lists apply to fluid-only scenes (only the fluid passes move their particles
+// within a substep, and they record every displacement).
 __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const float4 s, uint32_t ic,
-                                              const FluidArgs &a, float dprev = -1.0f)
+                                              const FluidArgs &a, bool sup_on = false)
 {
     x.pc = a.classes[ic];
     x.s = s;
@@ -286,15 +288,17 @@ __device__ __forceinline__ bool density_begin(DensitySelf &x, uint32_t i, const 
     x.ih = fdiv(1.0f, x.h);
     x.fd = quot_fast_ok(x.h, s.x, s.y, s.z);
     x.tension = fmaxd(m.surface_tension, 0.0f) > kEps;
-    if (dprev >= 0.0f && a.scenes[x.pc.scene].fluid_only) {
-        // a pair in radius at iteration k was within h + 2 max|delta_k| at iteration 0, so the
-        // candidates within h + skin hold every later list while 2 max|delta_k| < 0.98 skin (the
-        // margins cover the rounding of the squares)
-        const float skin = fmind(fmaxd(2.5f * dprev, 0.05f * x.h), 0.3f * x.h);
-        const float hs = x.h + skin, al = 0.49f * skin;
+    if (sup_on && a.scenes[x.pc.scene].fluid_only) {
+        // a pair j within h of particle i at iteration k was within h + D_i + D_j at iteration 0
+        // (D: displacement since iteration 0, x* to x* + delta), so the candidates within
+        // h + skin hold every later list while D_i + D_j < skin; D_j is bounded by the largest D in
+        // the block's halo, which holds all of i's 27 cells.  The margins cover the rounding of the
+        // squared distances (a few ulps) and of the recorded displacements.
+        const float skin = kSkinFrac * x.h;
+        const float hs = x.h + skin;
         x.sup = true;
-        x.ts2 = hs * hs * 1.000001f;
-        x.allow = al * al * 0.999999f;
+        x.ts2 = hs * hs * 1.00001f;
+        x.allow = 0.98f * skin;
     }
     return true;
 }
@@ -516,6 +520,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
     const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
     if (pc.kind != CN_PARTICLE_KIND_FLUID) {
         a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (a.disp) a.disp[i] = 0.0f;
         return 0.0f;
     }
     const float4 p = a.pos[i];
@@ -583,7 +588,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         for (uint32_t k = 0; k < scb.ncol; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float sd, nx, ny, nz;
-            if (!box_contact(a.col[scb.col0 + k], cx, cy, cz, sd, nx, ny, nz)) continue;
+            if (!collider_contact(a.col[scb.col0 + k], cx, cy, cz, sd, nx, ny, nz)) continue;
             if (sd >= r) continue;
             const float push = r - sd;
             cx = cx + nx * push;
@@ -600,6 +605,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
     // |delta| plus a few ulps of the position: bounds the next iteration's relDelta for the CFL skip
     const float slack = fmaxd(fmaxd(fabsf(nx), fabsf(ny)), fabsf(nz)) * 4.76837158e-7f + 1.0e-30f;
     const float dl = len3(ex, ey, ez) + slack;
+    if (a.disp) a.disp[i] = dl;
     return dl * dl;
 }
 
@@ -820,31 +826,31 @@ __device__ __forceinline__ RowCull row_cull(const float4 s, const int4 ci, float
 __device__ __forceinline__ float gap_below(float u, int c) { return fmaxd(u - (float)c, 0.0f); }
 __device__ __forceinline__ float gap_above(float u, int c) { return fmaxd((float)(c + 1) - u, 0.0f); }
 
-// A particle's list in a grouped list buffer: the buffer (a kernel argument, so a scalar base) and
-// the byte offset of entry 0; entry k sits 2 * kListGroup * k bytes further.  The offsets are
-// 32-bit (list buffers stay below 4 GiB), so an entry's address is one 32-bit add from the base.
+// A particle's list in a grouped list buffer (list_entry_base): the buffer (a kernel argument, so a
+// scalar base) and the byte offset of entry 0; entry k sits 2 * kListGroup * k bytes further.  The
+// offsets are 32-bit (list buffers stay below 4 GiB), so an address is one 32-bit add from the base.
+constexpr uint32_t kEntryStride = 2u * kListGroup;
 struct ListRef {
     char *base;
     uint32_t off;
-    __device__ __forceinline__ uint16_t &at(uint32_t k) const
+    __device__ __forceinline__ uint16_t &entry(uint32_t k) const
     {
-        return *reinterpret_cast<uint16_t *>(base + (off + k * (2u * kListGroup)));
+        return *reinterpret_cast<uint16_t *>(base + (off + k * kEntryStride));
     }
 };
 
 __device__ __forceinline__ ListRef list_ref(uint16_t *buf, uint32_t i, uint32_t rows)
 {
-    return ListRef{reinterpret_cast<char *>(buf), (uint32_t)(list_group_base(i, rows) * sizeof(uint16_t))};
+    return ListRef{reinterpret_cast<char *>(buf), (uint32_t)list_entry_base(i, rows)};
 }
 
-// Appends listed halo offsets to a particle's list; n is the untruncated count.
+// Appends listed halo offsets to a particle's list, storing the first cap; n is the untruncated count.
 struct ListOut {
     ListRef r;
     uint32_t n;
     __device__ __forceinline__ void push(uint32_t f, uint32_t cap)
     {
-        if (n < cap) *reinterpret_cast<uint16_t *>(r.base + r.off) = (uint16_t)f;
-        r.off += 2u * kListGroup;
+        if (n < cap) r.entry(n) = (uint16_t)f;
         ++n;
     }
 };
@@ -916,40 +922,51 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
     return lo.n;
 }
 
-// A later iteration's list from the iteration-0 candidates: the same walk order, the same in-radius
-// test at the current positions, so the same list while every neighbor is still a candidate
-// (guaranteed while no fluid particle has moved more than half the skin since iteration 0).
-__device__ __forceinline__ uint32_t scan_super(const ListRef &ls, uint32_t scount, const float4 *lpos,
-                                               const DensitySelf &x, const ListRef &li, const SubstepConsts &c)
-{
-    uint32_t req = 0;
-    for (uint32_t k = 0; k < scount; k += 4) {
-        uint32_t e[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = ls.at(min(k + u, scount - 1));
-        float4 pj[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) pj[u] = lpos[e[u]];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
-            const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
-            if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
-                if (req < c.cap) li.at(req) = (uint16_t)e[u];
-                ++req;
-            }
-        }
-    }
-    return req;
-}
-
-// Density sums over a compacted list, kTermStep entries per step (loads first, sums in list order).
-// List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past the end).
+// List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past
+// the end, so the LDS reads they drive stay in range).
 __device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32_t cnt, uint32_t (&e)[kTermStep])
 {
 #pragma unroll
-    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li.at(min(k + u, cnt - 1));
+    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li.entry(min(k + u, cnt - 1));
 }
 
+// A later iteration from the iteration-0 candidates (complete while the displacements stay under
+// the skin, checked by the caller): the in-radius test at the current positions, the listed
+// neighbors written to the list in candidate order (= walk order), and the density terms of the
+// first cap summed in the same pass.  A step's four candidates are loaded first; each term is then
+// evaluated before the test decides whether it is added, so every lane runs the same instructions.
+// Returns the untruncated neighbor count.
+__device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t scount, const float4 *lpos,
+                                                  const DensitySelf &x, const ListRef &li, uint32_t cap,
+                                                  DensitySum &d)
+{
+    ListOut lo{li, 0u};
+    uint32_t en[kTermStep];
+    list_group(ls, 0, scount, en);
+    for (uint32_t k = 0; k < scount; k += kTermStep) {
+        float4 pj[kTermStep];
+        uint32_t e[kTermStep];
+#pragma unroll
+        for (uint32_t u = 0; u < kTermStep; ++u) {
+            e[u] = en[u];
+            pj[u] = lpos[e[u]];
+        }
+        // the next step's entries while this step's terms are evaluated
+        list_group(ls, k + kTermStep, scount, en);
+#pragma unroll
+        for (uint32_t u = 0; u < kTermStep; ++u) {
+            const DensityTerm t = density_term(x, pj[u]);
+            const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
+            if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
+                if (lo.n < cap) density_add(d, x, t);
+                lo.push(e[u], cap);
+            }
+        }
+    }
+    return lo.n;
+}
+
+// Density sums over a compacted list, kTermStep entries per step (loads first, sums in list order).
 __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, const float4 *lpos,
                                              const DensitySelf &x, DensitySum &d)
 {
@@ -973,9 +990,13 @@ __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, co
 // ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
-// SUP: the candidate lists are in use (a separate instantiation, so the default pass carries none of
-// their registers)
-template <bool UNIFORM, bool SUP>
+// MODE: kDensPlain walks every iteration; with candidate lists, kDensBuild (a substep's first
+// iteration) walks the wider radius and writes them, kDensReuse (a later iteration) filters them in
+// every block whose particles' displacements allow it and lists the other blocks, and
+// kDensFallback walks the listed blocks.  Separate instantiations, so each carries only its own
+// registers (the filter pass has no walk).
+constexpr int kDensPlain = 0, kDensBuild = 1, kDensReuse = 2, kDensFallback = 3;
+template <bool UNIFORM, int MODE>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_density_tiled(
     FluidArgs a, SubstepConsts c)
 {
@@ -984,20 +1005,32 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     __shared__ uint32_t hstart[kHalo];
     __shared__ uint32_t s_blk;
     __shared__ uint32_t s_meta[kWgMetaWords];
+    __shared__ uint32_t s_hmax;  // largest displacement bound in the staged halo (float bits)
+    constexpr bool SUP = MODE == kDensBuild || MODE == kDensReuse;  // candidate lists apply
+    constexpr bool REUSE = MODE == kDensReuse;
     float4 *lpos = dyn;
     uint32_t *lcls = reinterpret_cast<uint32_t *>(dyn + a.tile + kTilePad);
     const uint32_t tid = threadIdx.x;
     // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
     // from its iteration-0 position
     const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
-    const float dprev = SUP ? fsqrt(__uint_as_float(a.iter_state[kStatePrevDisp])) : -1.0f;
-    if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
+    // the iteration's first density launch resets the displacement word its delta pass raises
+    if (MODE != kDensFallback && blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
+    uint32_t *fb_count = a.iter_state + kStateFbCount + a.it;
+    uint32_t *fb_queue = a.iter_state + kStateFbQueue + a.it;
+    const uint32_t nfb = MODE == kDensFallback ? *fb_count : 0u;
     if (tid < kWgMetaWords) s_meta[tid] = 0u;  // ordered before any use by tile_begin's barrier
+    if (tid == 0) s_hmax = 0u;
     BlockQueue bq(blockIdx.x);
     TileState ts;
-    while (tile_begin([&] { return bq.next(queue, nblk); }, &s_blk, hoff, hstart, ts, 0u, a, c)) {
+    auto next = [&]() -> uint32_t {
+        if (MODE != kDensFallback) return bq.next(queue, nblk);
+        const uint32_t t = atomicAdd(fb_queue, 1u);
+        return t < nfb ? a.fb_blk[t] : kInvalid;
+    };
+    while (tile_begin(next, &s_blk, hoff, hstart, ts, 0u, a, c)) {
         if (ts.skip) {
             __syncthreads();
             continue;
@@ -1009,20 +1042,49 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             // takes the class filter out of the walk, as a pool of one class does
             const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
             bool mixed = false;
+            uint32_t dmb = 0;  // displacement bounds are non-negative floats: their bits order them
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 float4 v = a.cur[q];
                 if (!(v.w > kEps)) v.x = v.y = v.z = kFarAway;
                 lpos[f] = v;
+                if (REUSE) dmb = max(dmb, __float_as_uint(a.disp[q]));
                 if (!UNIFORM) {
                     const uint32_t qc = a.cls[q];
                     lcls[f] = qc;
                     mixed |= qc != ref;
                 }
             }
+            if (REUSE) {
+                dmb = wave_max(dmb);
+                if ((tid & 63u) == 0u) atomicMax(&s_hmax, dmb);
+            }
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
+            if (REUSE) {
+                // The candidates of iteration 0 (within h + skin of x*) hold particle i's list while
+                // D_i + D_j < skin for every j in its 27 cells (D: displacement since iteration 0);
+                // D_j is bounded by the halo's largest.  A block where some particle fails, or has
+                // no complete candidate list, goes to the walking launch whole.
+                const float hmax = __uint_as_float(s_hmax);
+                bool bad = false;
+                for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
+                    const int hc = halo_cell(ts, a.cellc[i]);
+                    const uint32_t fs = hoff[hc] + (i - hstart[hc]);
+                    const ParticleClass pc = a.classes[UNIFORM ? 0u : lcls[fs]];
+                    if (pc.kind != CN_PARTICLE_KIND_FLUID || !(lpos[fs].w > kEps) || c.cap == 0u) continue;
+                    const float h = fmaxd(a.fm[pc.fluid_mat].smoothing_radius, 1.0e-4f);
+                    bad |= !a.scenes[pc.scene].fluid_only || a.scnt[i] == kSuperNone ||
+                           !(a.disp[i] + hmax < 0.98f * (kSkinFrac * h));
+                }
+                if (__syncthreads_or(bad ? 1 : 0)) {
+                    if (tid == 0) a.fb_blk[atomicAdd(fb_count, 1u)] = ts.blk;
+                    __syncthreads();
+                    if (tid == 0) s_hmax = 0u;
+                    continue;
+                }
+            }
             auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, uint32_t i, const ListRef &li) {
                 constexpr bool BU = decltype(bu_tag)::value;
                 const float inv = c.inv_cell > 0.0f ? c.inv_cell : a.scenes[x.pc.scene].inv_cell;
@@ -1030,19 +1092,13 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                 // |delta| of the previous iteration, where only the fluid passes move fluid particles
                 const float dmax = a.it == 0 ? 0.0f : (a.scenes[x.pc.scene].fluid_only ? fsqrt(md2) : 3.4e38f);
                 const RowCull rc = row_cull(x.s, ci, inv, x.h, dmax);
-                if (!SUP) return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
-                const uint32_t sc = x.sup && a.it > 0 ? a.scnt[i] : kSuperNone;
-                if (x.sup && a.it == 0) {
+                if (MODE == kDensBuild && x.sup) {
                     uint32_t sreq;
                     const RowCull rs = row_cull(x.s, ci, inv, fsqrt(x.ts2), dmax);
                     const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, rs,
                                                              list_ref(a.slist, i, kSuperSlots), &sreq);
                     a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
                     return req;
-                }
-                if (sc != kSuperNone && md2 < x.allow) {
-                    ++ma.reuse;
-                    return scan_super(list_ref(a.slist, i, kSuperSlots), sc, lpos, x, li, c);
                 }
                 return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
             };
@@ -1052,14 +1108,19 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
                 uint32_t cnt = 0;
-                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, dprev)) {
+                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, SUP)) {
                     DensitySum d;
                     const ListRef li = list_ref(a.list, i, c.cap);
                     if (density_walks(x, bu, c)) {
-                        d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, i, li)
-                                   : scan(std::false_type{}, x, ci, hc, fs, i, li);
+                        if (REUSE) {
+                            ++ma.reuse;
+                            d.req = density_super(list_ref(a.slist, i, kSuperSlots), a.scnt[i], lpos, x, li, c.cap, d);
+                        } else {
+                            d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, i, li)
+                                       : scan(std::false_type{}, x, ci, hc, fs, i, li);
+                        }
                         d.inr = min(d.req, c.cap);
-                        density_list(li, d.inr, lpos, x, d);
+                        if (!REUSE) density_list(li, d.inr, lpos, x, d);
                     }
                     cnt = d.inr;
                     density_end(i, d, x, ma, a, c);
@@ -1073,6 +1134,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
         }
         meta_to_wg(ma, s_meta);
         __syncthreads();
+        if (tid == 0) s_hmax = 0u;  // read again only after the next block's staging barrier
     }
     flush_wg_meta(s_meta, a);
 }
@@ -1515,6 +1577,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.lcnt = s.lcnt;
     a.slist = s.slist;
     a.scnt = s.scnt;
+    a.disp = c.superset ? s.disp : nullptr;
     a.run_key = s.run_key;
     a.run_start = s.run_start;
     a.run_info = s.run_info;
@@ -1524,6 +1587,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.nblk = s.nblk;
     a.blk_halo = s.blk_halo;
     a.wide_blk = s.wide_blk;
+    a.fb_blk = s.fb_blk;
     a.iter_state = s.iter_state;
     a.classes = s.classes;
     a.fm = s.fluid_mats;
@@ -1578,16 +1642,26 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
         a.tile = c.tile_dens;
         const size_t lds = (size_t)(a.tile + kTilePad) * per;
         const dim3 grid(persistent_grid(c)), block(kTileBlock);
-        if (c.superset) {
+        if (c.superset && it == 0) {
             if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, true>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensBuild>), grid, block, lds, st, a, c);
             else
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, true>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensBuild>), grid, block, lds, st, a, c);
+        } else if (c.superset) {
+            // the candidate-list pass, then the walk over the blocks it could not take
+            const dim3 fgrid(std::min<uint32_t>(persistent_grid(c), 256u));
+            if (c.uniform_class) {
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensReuse>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensFallback>), fgrid, block, lds, st, a, c);
+            } else {
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensReuse>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensFallback>), fgrid, block, lds, st, a, c);
+            }
         } else {
             if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, false>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensPlain>), grid, block, lds, st, a, c);
             else
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, false>), grid, block, lds, st, a, c);
+                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensPlain>), grid, block, lds, st, a, c);
         }
     } else {
         if (c.uniform_class)

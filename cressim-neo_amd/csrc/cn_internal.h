@@ -17,7 +17,10 @@ constexpr uint32_t kStateDensity = 0;
 constexpr uint32_t kStateDelta = kIterStateWords * kXcds;
 constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds;
 constexpr uint32_t kStateWideCount = kStateWideQueue + kIterStateWords;
-constexpr uint32_t kStateMaxDisp = kStateWideCount + 1;
+// per iteration: blocks the candidate-list pass handed to the walking pass, and their queue
+constexpr uint32_t kStateFbQueue = kStateWideCount + 1;
+constexpr uint32_t kStateFbCount = kStateFbQueue + kIterStateWords;
+constexpr uint32_t kStateMaxDisp = kStateFbCount + kIterStateWords;
 constexpr uint32_t kStatePrevDisp = kStateMaxDisp + kIterStateWords;  // the last iteration's word of the
                                                                         // previous substep
 constexpr uint32_t kStateWords = kStatePrevDisp + 1;
@@ -37,18 +40,20 @@ constexpr uint32_t kHdNoFluid = 2u;  // kHdFlags value of a block in a scene wit
 // candidate contact partners a particle keeps per substep; more are counted as overflows
 constexpr uint32_t kContactPartners = 32;
 constexpr uint32_t kMetaWords = 16;  // device diagnostics words
-// candidate list slots of a fluid particle: its neighbors within h + skin at iteration 0, reused
-// while no fluid particle has moved more than about skin / 2; the skin is 2.5 times the previous
-// substep's largest displacement, clamped to [0.05 h, 0.3 h]
+// candidate list slots of a fluid particle: its neighbors within h + skin at iteration 0
+// (skin = kSkinFrac h), reused by a later iteration of the substep while its own displacement since
+// iteration 0 plus the largest one in its block's halo stays below the skin
 constexpr uint32_t kSuperSlots = 96;
+constexpr float kSkinFrac = 0.1f;
 constexpr uint16_t kSuperNone = 0xffffu;  // count marker: the candidates did not fit, walk instead
-// Per-slot lists are stored in groups of kListGroup consecutive slots: entry k of slot i sits at
-// (i / kListGroup) * kListGroup * rows + k * kListGroup + i % kListGroup, so one wave's row is one
-// 128-byte line and a group's rows are contiguous (few pages per wave).
+// Per-slot lists of 16-bit entries are stored in groups of kListGroup consecutive slots: entry k
+// of slot i sits at (i / kListGroup) * kListGroup * rows + k * kListGroup + i % kListGroup, so one
+// wave's row is one 128-byte line and a group's rows are contiguous (few pages per wave).
 constexpr uint32_t kListGroup = 64;
-__host__ __device__ inline size_t list_group_base(uint32_t i, uint32_t rows)
+// byte offset of entry 0 of slot i
+__host__ __device__ inline size_t list_entry_base(uint32_t i, uint32_t rows)
 {
-    return (size_t)(i / kListGroup) * kListGroup * rows + (i % kListGroup);
+    return ((size_t)(i / kListGroup) * kListGroup * rows + (i % kListGroup)) * sizeof(uint16_t);
 }
 __host__ __device__ inline size_t list_slots(size_t n) { return (n + kListGroup - 1) / kListGroup * kListGroup; }
 // A list buffer is addressed with 32-bit byte offsets from its base (one 32-bit add per entry on the
@@ -123,11 +128,18 @@ struct SceneBuffers {
     uint32_t *blk_halo = nullptr;   // kHaloStride words per block, for the first blk_cap blocks
     uint32_t blk_cap = 0;
     uint32_t *wide_blk = nullptr;   // blocks whose halo fits the wide delta tile only (blk_cap)
+    uint32_t *fb_blk = nullptr;     // blocks whose candidate lists may be incomplete (blk_cap)
     uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
     float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
     uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, in slot groups (list_row)
     uint16_t *slist = nullptr;      // candidate lists of iteration 0 as halo offsets, in slot groups
     uint16_t *scnt = nullptr;       // candidate count per slot, or kSuperNone
+    float *disp = nullptr;          // per slot: bound on |x* + delta - x*| after the last delta pass
+    // constraint adjacency in particle-index order (pool indices): particle i's adjacent
+    // particles are adj_idx[adj_off[i] .. adj_off[i] + adj_cnt[i]); null when no scene has any
+    uint32_t *adj_off = nullptr;
+    uint32_t *adj_cnt = nullptr;
+    uint32_t *adj_idx = nullptr;
     uint32_t *lcnt = nullptr;       // fluid neighbor count per slot
     uint32_t *nbr = nullptr;        // neighbor lists of the pass entry, k-major: nbr[k * n + i]
     uint32_t *nbr_cnt = nullptr;
@@ -139,6 +151,7 @@ struct SceneBuffers {
     float4 *ct_rec = nullptr;       // (normal, penetration) of each candidate's last contact, k-major
     uint32_t *ct_bits = nullptr;    // bit k: candidate k's last contact is active
     float4 *ct_corr = nullptr;      // summed correction of the last contact pass
+    float4 *ct_gen = nullptr;       // position of each solid's last particle-rigid contact generation
     size_t ct_alloc = 0;
     // index-order exports
     float4 *out_pos = nullptr;
@@ -204,6 +217,7 @@ void launch_fill(uint32_t *p, uint32_t words, uint32_t value, hipStream_t st);
 void launch_contact_partners(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_contact_iteration(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_rigid_velocity(const SceneBuffers &s, const SubstepConsts &c, uint32_t iters, hipStream_t st);
 
 // ---- XPBD soft constraints (cn_soft.hip) ----
 struct SoftArgs {

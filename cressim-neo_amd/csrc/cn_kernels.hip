@@ -220,18 +220,28 @@ void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, u
 // reorder: gather state into the new slot order, predict, cell coordinates, boundary candidates
 // ---------------------------------------------------------------------------------------------
 
-// Candidate colliders of a fluid particle (physics_particle_count_fluid_boundary_candidates.cs.hlsl
-// :20-28 query radius, :43-85 filters), as a bitmask over the colliders in LBVH visit order.
+// Candidate colliders of a particle as a bitmask over the scene's colliders in LBVH visit order:
+// a fluid particle's fluid-boundary candidates (physics_particle_count_fluid_boundary_candidates
+// .cs.hlsl:20-28 query radius, :43-85 filters), a solid particle's particle-rigid candidates
+// (physics_particle_count_particle_rigid_candidates.cs.hlsl:112-151: the box of its radius; a
+// static body takes solid particles only; every collider is a body of its own).
 __device__ __forceinline__ uint32_t boundary_mask(const ParticleClass &pc, const cn_fluid_material *fm,
                                                   const Collider *__restrict__ col, uint32_t ncol, float x,
                                                   float y, float z)
 {
-    if (pc.kind != CN_PARTICLE_KIND_FLUID || ncol == 0) return 0u;
-    const float r = fmaxd(pc.radius, 0.0f);
-    const cn_fluid_material m = fm[pc.fluid_mat];
-    const float support = fmaxd(m.smoothing_radius, r);
-    const float reuse = fmaxd(fmaxd(m.cfl_radius, r), support);
-    const float qr = support + reuse;
+    if (ncol == 0) return 0u;
+    float qr;
+    if (pc.kind == CN_PARTICLE_KIND_FLUID) {
+        const float r = fmaxd(pc.radius, 0.0f);
+        const cn_fluid_material m = fm[pc.fluid_mat];
+        const float support = fmaxd(m.smoothing_radius, r);
+        const float reuse = fmaxd(fmaxd(m.cfl_radius, r), support);
+        qr = support + reuse;
+    } else if (pc.kind == CN_PARTICLE_KIND_SOLID) {
+        qr = pc.radius;
+    } else {
+        return 0u;
+    }
     const float x0 = x - qr, y0 = y - qr, z0 = z - qr;
     const float x1 = x + qr, y1 = y + qr, z1 = z + qr;
     uint32_t mask = 0;
@@ -446,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
         for (uint32_t k = 0; k < sc.ncol; ++k) {
             if (!((bm >> k) & 1u)) continue;
             float sd, nx, ny, nz;
-            if (!box_contact(scol[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
+            if (!collider_contact(scol[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
             const float cdist = r + 2.0f * slop;
             if (sd > cdist) continue;
             if (sd < best) {

@@ -91,6 +91,31 @@ typedef struct cn_box_collider {
     uint32_t reserved[2];
 } cn_box_collider;
 
+/* A static collider of any shape: the reference's static rigid body (position, orientation,
+ * scale) carrying one collider (shape, local position and orientation, shape parameters), as
+ * ComputeFluidBoundaryContactInfo reads them (physics_fluid_boundary_common.hlsli:10-84 with
+ * physics_shape_common.hlsli and physics_math.hlsli).  shape_params: sphere (radius), box (half
+ * extents), capsule (radius, half height along the local y axis); quaternions are (x, y, z, w) and
+ * are normalized like the reference's.  material as for cn_box_collider. */
+#define CN_SHAPE_SPHERE 0u
+#define CN_SHAPE_BOX 1u
+#define CN_SHAPE_CAPSULE 2u
+typedef struct cn_collider {
+    uint32_t shape;
+    uint32_t environment;
+    uint32_t layer;
+    uint32_t mask;
+    float body_position[4];     /* xyz */
+    float body_orientation[4];
+    float body_scale[4];        /* xyz */
+    float local_position[4];    /* xyz */
+    float local_orientation[4];
+    float shape_params[4];
+    float material[4];
+    uint32_t enabled;
+    uint32_t reserved[3];
+} cn_collider;
+
 /* Host view of the unified particle pool (ParticleSoAHost, include/physics/physics_types.h:858). */
 typedef struct cn_particle_soa_view {
     uint32_t count;
@@ -104,6 +129,15 @@ typedef struct cn_particle_soa_view {
     const uint32_t *phases;
     const uint32_t *collision_layers;
     const uint32_t *collision_masks;
+    /* Constraint adjacency (ParticleSoA adjacencyOffsets / adjacencyCounts / adjacencyIndices,
+     * physics_world.cpp:5774-5787): particle i's adjacent particles are
+     * adjacency_indices[adjacency_offsets[i] .. adjacency_offsets[i] + adjacency_counts[i]).
+     * IsAdjacentParticle (physics_particle_broad_phase_query.hlsli:50-62) keeps them out of the
+     * contact pairs of a self-colliding phase group.  NULL offsets: no particle has any. */
+    const uint32_t *adjacency_offsets;
+    const uint32_t *adjacency_counts;
+    uint32_t adjacency_index_count;
+    const uint32_t *adjacency_indices;
 } cn_particle_soa_view;
 
 /* Scene tables uploaded beside the particles. */
@@ -114,6 +148,10 @@ typedef struct cn_scene_tables {
     const float *particle_materials;    /* float4 (friction, restitution, damping, staticFriction) */
     uint32_t collider_count;
     const cn_box_collider *colliders;
+    /* static colliders of any shape (spheres, oriented boxes, capsules), after the boxes: collider
+     * k >= collider_count is shape_colliders[k - collider_count]; at most 32 colliders in all */
+    uint32_t shape_collider_count;
+    const cn_collider *shape_colliders;
 } cn_scene_tables;
 
 /* Soft/cloth constraints (physics_particle_types.hlsli:147-205 layouts). */
@@ -174,13 +212,16 @@ typedef struct cn_step_params {
     float cell_size;                 /* particle grid cell size; 0 -> derived like the reference */
     uint32_t max_fluid_neighbors;    /* neighbor slots per fluid particle (reference: 128) */
     uint32_t key_mode;               /* CN_KEY_MORTON or CN_KEY_HASH */
-    uint32_t flags;                  /* CN_STEP_* bits, 0 by default */
+    uint32_t flags;                  /* CN_STEP_* bits (cn_default_step_params: CN_STEP_CANDIDATE_REUSE) */
 } cn_step_params;
 
-/* flags: fluid iterations after the first filter iteration 0's candidate lists (neighbors within
- * h + skin) instead of walking the 27 cells, while every fluid particle has moved less than half the
- * skin (the results are the same either way; it pays when the walk dominates and displacements are
- * small).  The lists take 194 bytes per particle, allocated on the first step that asks. */
+/* flags: in fluid-only scenes the fluid iterations after a substep's first filter the candidate
+ * lists built by its first iteration (neighbors within 1.1 h of the predicted positions) instead of
+ * walking the 27 cells again, for every particle whose displacement since the first iteration plus
+ * the largest one in its cell block's halo stays below 0.1 h; the others walk.  The lists are the
+ * same either way (same neighbors, same order), so the results are too.  The candidate lists take
+ * 198 bytes per particle, allocated on the first step that asks (left off when they would exceed
+ * 4 GiB).  On in cn_default_step_params; clear the bit to walk every iteration. */
 #define CN_STEP_CANDIDATE_REUSE 1u
 
 /* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
@@ -307,6 +348,14 @@ cn_status cn_device_views_get(cn_ctx *ctx, cn_device_views *out);
 #define CN_STATE_DEVICE_POINTERS 1u
 cn_status cn_scene_set_state(cn_ctx *ctx, uint32_t scene, const float *positions_inv_mass,
                              const float *velocities, uint32_t flags);
+/* Dirty-run upload of particle state (uploadContiguousRuns, physics_scene_gpu_state.cpp:255-284):
+   overwrites positions with inverse masses and velocities of the uploaded particles
+   [first, first + count) in cn_readback's particle order (a pool's particle index space), leaving
+   every other particle, the topology, materials and colliders resident; the previous positions of
+   the run become its new positions.  A caller that tracks dirty particle indices sends one call per
+   contiguous run.  flags as for cn_scene_set_state. */
+cn_status cn_particles_update_range(cn_ctx *ctx, uint32_t first, uint32_t count, const float *positions_inv_mass,
+                                    const float *velocities, uint32_t flags);
 /* Replaces the static box colliders of scene `scene` (0 for a single uploaded scene) with
    `colliders`: the same count as the scene's upload, each of which may be moved, resized,
    re-materialled, enabled or disabled; the visit order is rebuilt, and the next cn_step uses them.

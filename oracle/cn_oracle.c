@@ -389,15 +389,146 @@ static uint32_t morton3d(float x, float y, float z)
     return expand_bits((uint32_t)x) * 4u + expand_bits((uint32_t)y) * 2u + expand_bits((uint32_t)z);
 }
 
-/* World AABB of a static identity-oriented box collider with the broad-phase margin:
- * physics_rigid_update_world_aabbs.cs.hlsl:51-91 with ComputeBodyAabb
- * (physics_shape_common.hlsli:49-75). */
-static void collider_aabb(const cn_box_collider *c, float *mn, float *mx)
+/* A static collider resolved to its world pose: ComputeFluidBoundaryContactInfo's pose (the body
+ * pose composed with the unscaled local position, physics_fluid_boundary_common.hlsli:21-33) and
+ * the broad-phase AABB of physics_rigid_update_world_aabbs.cs.hlsl:51-91 (scaled local position,
+ * ComputeBodyAabb of physics_shape_common.hlsli:49-75, the margin added after).  Quaternion helpers
+ * follow physics_math.hlsli with rsqrt taken as 1 / sqrt, sums left to right. */
+typedef struct ocol {
+    uint32_t shape, rotated, enabled, env, layer, mask;
+    float c[3], he[3], q[4], sa[3], sb[3], mn[3], mx[3], material[4];
+} ocol;
+
+static void quat_normalize(const float *q, float *o)
 {
-    for (int a = 0; a < 3; ++a) {
-        mn[a] = (c->center[a] - c->half_extents[a]) - BROADPHASE_MARGIN;
-        mx[a] = (c->center[a] + c->half_extents[a]) + BROADPHASE_MARGIN;
+    const float l = ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3];
+    if (l <= EPS) {
+        o[0] = o[1] = o[2] = 0.0f;
+        o[3] = 1.0f;
+        return;
     }
+    const float r = 1.0f / sqrtf(l);
+    for (int a = 0; a < 4; ++a)
+        o[a] = q[a] * r;
+}
+static void quat_mul(const float *a, const float *b, float *o)
+{
+    o[0] = ((a[3] * b[0] + a[0] * b[3]) + a[1] * b[2]) - a[2] * b[1];
+    o[1] = ((a[3] * b[1] - a[0] * b[2]) + a[1] * b[3]) + a[2] * b[0];
+    o[2] = ((a[3] * b[2] + a[0] * b[1]) - a[1] * b[0]) + a[2] * b[3];
+    o[3] = ((a[3] * b[3] - a[0] * b[0]) - a[1] * b[1]) - a[2] * b[2];
+}
+/* QuaternionRotate (physics_math.hlsli:33-37) */
+static void quat_rotate(const float *q, const float *v, float *o)
+{
+    const float t[3] = {2.0f * (q[1] * v[2] - q[2] * v[1]), 2.0f * (q[2] * v[0] - q[0] * v[2]),
+                        2.0f * (q[0] * v[1] - q[1] * v[0])};
+    o[0] = (v[0] + q[3] * t[0]) + (q[1] * t[2] - q[2] * t[1]);
+    o[1] = (v[1] + q[3] * t[1]) + (q[2] * t[0] - q[0] * t[2]);
+    o[2] = (v[2] + q[3] * t[2]) + (q[0] * t[1] - q[1] * t[0]);
+}
+
+static void ocol_box(const cn_box_collider *b, ocol *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->shape = CN_SHAPE_BOX;
+    o->q[3] = 1.0f;
+    for (int a = 0; a < 3; ++a) {
+        o->c[a] = b->center[a];
+        o->he[a] = b->half_extents[a];
+        o->mn[a] = (b->center[a] - b->half_extents[a]) - BROADPHASE_MARGIN;
+        o->mx[a] = (b->center[a] + b->half_extents[a]) + BROADPHASE_MARGIN;
+    }
+    memcpy(o->material, b->material, sizeof(o->material));
+    o->enabled = b->enabled;
+    o->env = b->environment;
+    o->layer = b->layer;
+    o->mask = b->mask;
+}
+
+static void ocol_shape(const cn_collider *g, ocol *o)
+{
+    memset(o, 0, sizeof(*o));
+    float bq[4], lq[4], m[4], rot[3];
+    quat_normalize(g->body_orientation, bq);
+    quat_normalize(g->local_orientation, lq);
+    quat_mul(bq, lq, m);
+    quat_normalize(m, o->q);
+    const float *q = o->q;
+    o->rotated = !(q[0] == 0.0f && q[1] == 0.0f && q[2] == 0.0f && q[3] == 1.0f);
+    o->shape = g->shape;
+    const float sx = fabsf(g->body_scale[0]), sy = fabsf(g->body_scale[1]), sz = fabsf(g->body_scale[2]);
+    const float *prm = g->shape_params;
+    quat_rotate(bq, g->local_position, rot);
+    for (int a = 0; a < 3; ++a)
+        o->c[a] = g->body_position[a] + rot[a];
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    float axis[3];
+    quat_rotate(q, up, axis);
+    float r = 0.0f, hh = 0.0f;
+    if (g->shape == CN_SHAPE_SPHERE) { /* SphereRadius */
+        r = prm[0] * maxf(sx, maxf(sy, sz));
+        o->he[0] = r;
+    } else if (g->shape == CN_SHAPE_BOX) { /* BoxHalfExtents */
+        o->he[0] = prm[0] * sx;
+        o->he[1] = prm[1] * sy;
+        o->he[2] = prm[2] * sz;
+    } else { /* CapsuleSegment */
+        r = prm[0] * maxf(sx, sz);
+        hh = prm[1] * sy;
+        o->he[0] = r;
+        for (int a = 0; a < 3; ++a) {
+            o->sa[a] = o->c[a] - axis[a] * hh;
+            o->sb[a] = o->c[a] + axis[a] * hh;
+        }
+    }
+    const float lps[3] = {g->local_position[0] * g->body_scale[0], g->local_position[1] * g->body_scale[1],
+                          g->local_position[2] * g->body_scale[2]};
+    quat_rotate(bq, lps, rot);
+    float ca[3], ext[3];
+    for (int a = 0; a < 3; ++a)
+        ca[a] = g->body_position[a] + rot[a];
+    if (g->shape == CN_SHAPE_SPHERE) {
+        ext[0] = ext[1] = ext[2] = r;
+    } else if (g->shape == CN_SHAPE_BOX) {
+        const float ex[3] = {1.0f, 0.0f, 0.0f}, ey[3] = {0.0f, 1.0f, 0.0f}, ez[3] = {0.0f, 0.0f, 1.0f};
+        float ax[3], ay[3], az[3];
+        quat_rotate(q, ex, ax);
+        quat_rotate(q, ey, ay);
+        quat_rotate(q, ez, az);
+        for (int a = 0; a < 3; ++a)
+            ext[a] = (fabsf(ax[a]) * o->he[0] + fabsf(ay[a]) * o->he[1]) + fabsf(az[a]) * o->he[2];
+    } else {
+        float A[3], B[3];
+        for (int a = 0; a < 3; ++a) {
+            A[a] = ca[a] - axis[a] * hh;
+            B[a] = ca[a] + axis[a] * hh;
+        }
+        const float d[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        const float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+        for (int a = 0; a < 3; ++a)
+            ext[a] = fabsf(axis[a]) * (0.5f * len) + r;
+    }
+    for (int a = 0; a < 3; ++a) {
+        o->mn[a] = (ca[a] - ext[a]) - BROADPHASE_MARGIN;
+        o->mx[a] = (ca[a] + ext[a]) + BROADPHASE_MARGIN;
+    }
+    memcpy(o->material, g->material, sizeof(o->material));
+    o->enabled = g->enabled;
+    o->env = g->environment;
+    o->layer = g->layer;
+    o->mask = g->mask;
+}
+
+static uint32_t n_cols(const cno_scene *s) { return s->n_colliders + s->n_shape_colliders; }
+
+/* collider k of the scene: the boxes, then the shape colliders */
+static void ocol_get(const cno_scene *s, uint32_t k, ocol *o)
+{
+    if (k < s->n_colliders)
+        ocol_box(&s->colliders[k], o);
+    else
+        ocol_shape(&s->shape_colliders[k - s->n_colliders], o);
 }
 
 /* Visit order of the static LBVH traversal used by the fluid-boundary candidate passes
@@ -406,7 +537,7 @@ static void collider_aabb(const cn_box_collider *c, float *mn, float *mx)
  * colliders (physics_pass_dispatcher.cpp:4163-4256, physics_rigid_morton_codes.cs.hlsl,
  * physics_rigid_bvh_hierarchy.cs.hlsl), whose in-order leaf sequence is the stable Morton order;
  * a right-first depth-first walk therefore meets the leaves in reverse stable Morton order. */
-void cno_static_collider_order(uint32_t nc, const cn_box_collider *c, uint32_t *order)
+static void order_ocols(uint32_t nc, const ocol *c, uint32_t *order)
 {
     uint32_t ne = 0;
     uint32_t *elem = (uint32_t *)malloc(sizeof(uint32_t) * (nc ? nc : 1));
@@ -415,22 +546,18 @@ void cno_static_collider_order(uint32_t nc, const cn_box_collider *c, uint32_t *
             elem[ne++] = i;
     float gmin[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
     float gmax[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
-    for (uint32_t e = 0; e < ne; ++e) {
-        float mn[3], mx[3];
-        collider_aabb(&c[elem[e]], mn, mx);
+    for (uint32_t e = 0; e < ne; ++e)
         for (int a = 0; a < 3; ++a) {
-            gmin[a] = minf(gmin[a], mn[a]);
-            gmax[a] = maxf(gmax[a], mx[a]);
+            gmin[a] = minf(gmin[a], c[elem[e]].mn[a]);
+            gmax[a] = maxf(gmax[a], c[elem[e]].mx[a]);
         }
-    }
     uint32_t *codes = (uint32_t *)malloc(sizeof(uint32_t) * (ne ? ne : 1));
     uint32_t *eidx = (uint32_t *)malloc(sizeof(uint32_t) * (ne ? ne : 1));
     for (uint32_t e = 0; e < ne; ++e) {
-        float mn[3], mx[3], m[3];
-        collider_aabb(&c[elem[e]], mn, mx);
+        float m[3];
         for (int a = 0; a < 3; ++a) {
             const float size = maxf(gmax[a] - gmin[a], 1.0e-5f);
-            const float center = 0.5f * (mn[a] + mx[a]);
+            const float center = 0.5f * (c[elem[e]].mn[a] + c[elem[e]].mx[a]);
             m[a] = saturatef((center - gmin[a]) / size);
         }
         codes[e] = morton3d(m[0], m[1], m[2]);
@@ -446,21 +573,33 @@ void cno_static_collider_order(uint32_t nc, const cn_box_collider *c, uint32_t *
     free(elem);
 }
 
-/* ComputeFluidBoundaryContactInfo for a box with identity orientation,
- * assets/shaders/include/physics/fluid/physics_fluid_boundary_common.hlsli:10-66 and
- * ClosestFaceNormalLocal (physics_shape_common.hlsli:100-108). */
-static int box_contact(const cn_box_collider *c, const float *p, float *sd, float *nrm)
+void cno_static_collider_order(uint32_t nc, const cn_box_collider *c, uint32_t *order)
 {
-    nrm[0] = 0.0f;
-    nrm[1] = 1.0f;
-    nrm[2] = 0.0f;
-    *sd = 0.0f;
-    if (!c->enabled)
-        return 0;
-    float local[3], d[3];
+    ocol *oc = (ocol *)malloc(sizeof(ocol) * (nc ? nc : 1));
+    for (uint32_t i = 0; i < nc; ++i)
+        ocol_box(&c[i], &oc[i]);
+    order_ocols(nc, oc, order);
+    free(oc);
+}
+
+/* the visit order of every collider of the scene (boxes and shape colliders) */
+static void scene_collider_order(const cno_scene *s, uint32_t *order)
+{
+    const uint32_t nc = n_cols(s);
+    ocol *oc = (ocol *)malloc(sizeof(ocol) * (nc ? nc : 1));
+    for (uint32_t i = 0; i < nc; ++i)
+        ocol_get(s, i, &oc[i]);
+    order_ocols(nc, oc, order);
+    free(oc);
+}
+
+/* The box branch of ComputeFluidBoundaryContactInfo in the box frame (local position l) with
+ * ClosestFaceNormalLocal (physics_shape_common.hlsli:100-108). */
+static void box_local_contact(const float *he, const float *local, float *sd, float *nrm)
+{
+    float d[3];
     for (int a = 0; a < 3; ++a) {
-        local[a] = p[a] - c->center[a];
-        const float cl = clampf(local[a], -c->half_extents[a], c->half_extents[a]);
+        const float cl = clampf(local[a], -he[a], he[a]);
         d[a] = local[a] - cl;
     }
     const float dl = len3(d);
@@ -472,7 +611,7 @@ static int box_contact(const cn_box_collider *c, const float *p, float *sd, floa
     } else {
         float dist[3];
         for (int a = 0; a < 3; ++a)
-            dist[a] = c->half_extents[a] - fabsf(local[a]);
+            dist[a] = he[a] - fabsf(local[a]);
         nrm[0] = nrm[1] = nrm[2] = 0.0f;
         if (dist[0] <= dist[1] && dist[0] <= dist[2])
             nrm[0] = local[0] >= 0.0f ? 1.0f : -1.0f;
@@ -482,7 +621,76 @@ static int box_contact(const cn_box_collider *c, const float *p, float *sd, floa
             nrm[2] = local[2] >= 0.0f ? 1.0f : -1.0f;
         *sd = -minf(dist[0], minf(dist[1], dist[2]));
     }
+}
+
+/* ComputeFluidBoundaryContactInfo, physics_fluid_boundary_common.hlsli:10-84, for a sphere, a box
+ * or a capsule; an identity-oriented box takes its frame without a rotation (the rotation by the
+ * identity quaternion changes at most the sign of a zero). */
+static int collider_contact(const ocol *c, const float *p, float *sd, float *nrm)
+{
+    nrm[0] = 0.0f;
+    nrm[1] = 1.0f;
+    nrm[2] = 0.0f;
+    *sd = 0.0f;
+    if (!c->enabled)
+        return 0;
+    if (c->shape == CN_SHAPE_BOX) {
+        float local[3];
+        for (int a = 0; a < 3; ++a)
+            local[a] = p[a] - c->c[a];
+        if (!c->rotated) {
+            box_local_contact(c->he, local, sd, nrm);
+            return 1;
+        }
+        const float qc[4] = {-c->q[0], -c->q[1], -c->q[2], c->q[3]};
+        float u[3], m[3];
+        quat_rotate(qc, local, u); /* QuaternionInverseRotate */
+        box_local_contact(c->he, u, sd, m);
+        quat_rotate(c->q, m, nrm);
+        return 1;
+    }
+    float d[3];
+    if (c->shape == CN_SHAPE_SPHERE) {
+        for (int a = 0; a < 3; ++a)
+            d[a] = p[a] - c->c[a];
+    } else {
+        const float g[3] = {c->sb[0] - c->sa[0], c->sb[1] - c->sa[1], c->sb[2] - c->sa[2]};
+        const float lsq = dot3(g, g);
+        float t = 0.0f;
+        if (lsq > EPS) {
+            const float w[3] = {p[0] - c->sa[0], p[1] - c->sa[1], p[2] - c->sa[2]};
+            t = saturatef(dot3(w, g) / lsq);
+        }
+        for (int a = 0; a < 3; ++a)
+            d[a] = p[a] - (c->sa[a] + t * g[a]); /* lerp(a, b, t) = a + t (b - a) */
+    }
+    const float dist = len3(d);
+    if (dist > EPS) {
+        nrm[0] = d[0] / dist;
+        nrm[1] = d[1] / dist;
+        nrm[2] = d[2] / dist;
+    }
+    *sd = dist - c->he[0];
     return 1;
+}
+
+int cno_collider_contact(const cn_collider *g, const float *p, float *sd, float *nrm, float *aabb)
+{
+    ocol o;
+    ocol_shape(g, &o);
+    if (aabb)
+        for (int a = 0; a < 3; ++a) {
+            aabb[a] = o.mn[a];
+            aabb[3 + a] = o.mx[a];
+        }
+    return collider_contact(&o, p, sd, nrm);
+}
+
+static int collider_contact_k(const cno_scene *s, uint32_t k, const float *p, float *sd, float *nrm)
+{
+    ocol o;
+    ocol_get(s, k, &o);
+    return collider_contact(&o, p, sd, nrm);
 }
 
 /* Count + scan + emit of buildFluidBoundaryCandidatePairs (physics_pass_dispatcher.cpp:1305-1407,
@@ -497,7 +705,11 @@ uint32_t cno_fluid_boundary_candidates(const cno_scene *s, const uint32_t *order
     uint32_t *counts = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
     uint32_t *offs = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
     uint32_t nvis = 0;
-    while (nvis < s->n_colliders && order[nvis] != INVALID)
+    const uint32_t ncol = n_cols(s);
+    ocol *oc = (ocol *)malloc(sizeof(ocol) * (ncol ? ncol : 1));
+    for (uint32_t k = 0; k < ncol; ++k)
+        ocol_get(s, k, &oc[k]);
+    while (nvis < ncol && order[nvis] != INVALID)
         ++nvis;
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1)
@@ -528,15 +740,12 @@ uint32_t cno_fluid_boundary_candidates(const cno_scene *s, const uint32_t *order
             }
             uint32_t cnt = 0, w = pass ? offs[i] : 0;
             for (uint32_t k = 0; k < nvis; ++k) {
-                const cn_box_collider *c = &s->colliders[order[k]];
-                float mn[3], mx[3];
-                collider_aabb(c, mn, mx);
-                const int overlap = qmin[0] <= mx[0] && qmax[0] >= mn[0] && qmin[1] <= mx[1] &&
-                                    qmax[1] >= mn[1] && qmin[2] <= mx[2] && qmax[2] >= mn[2];
+                const ocol *c = &oc[order[k]];
+                const int overlap = qmin[0] <= c->mx[0] && qmax[0] >= c->mn[0] && qmin[1] <= c->mx[1] &&
+                                    qmax[1] >= c->mn[1] && qmin[2] <= c->mx[2] && qmax[2] >= c->mn[2];
                 if (!overlap)
                     continue;
-                if (!(c->enabled && c->environment == md[0] && (md[3] & c->layer) != 0u &&
-                      (c->mask & md[2]) != 0u))
+                if (!(c->enabled && c->env == md[0] && (md[3] & c->layer) != 0u && (c->mask & md[2]) != 0u))
                     continue;
                 if (pass == 1) {
                     if (w >= capacity)
@@ -556,6 +765,7 @@ uint32_t cno_fluid_boundary_candidates(const cno_scene *s, const uint32_t *order
     const uint32_t required = n ? offs[n - 1] + counts[n - 1] : 0;
     free(counts);
     free(offs);
+    free(oc);
     return required;
 }
 
@@ -675,7 +885,7 @@ static int ghost_delta(const cno_scene *s, uint32_t collider, const float *p, fl
 {
     float sd, n[3];
     g[0] = g[1] = g[2] = 0.0f;
-    if (!box_contact(&s->colliders[collider], p, &sd, n))
+    if (!collider_contact_k(s, collider, p, &sd, n))
         return 0;
     if (sd < 0.0f || sd >= h)
         return 0;
@@ -895,7 +1105,7 @@ void cno_fluid_clamp(const cno_scene *s, const float *din, const uint32_t *br, c
             int changed = 0;
             for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
                 float sd, n[3];
-                if (!box_contact(&s->colliders[bp[br[2 * i] + k]], cur, &sd, n))
+                if (!collider_contact_k(s, bp[br[2 * i] + k], cur, &sd, n))
                     continue;
                 if (sd >= r)
                     continue;
@@ -985,7 +1195,9 @@ void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt, c
         for (uint32_t k = 0; k < br[2 * i + 1]; ++k) {
             const uint32_t ci = bp[br[2 * i] + k];
             float sd, n[3];
-            if (!box_contact(&s->colliders[ci], p, &sd, n))
+            ocol co;
+            ocol_get(s, ci, &co);
+            if (!collider_contact(&co, p, &sd, n))
                 continue;
             const float cd = r + 2.0f * slop;
             if (sd > cd)
@@ -995,7 +1207,7 @@ void cno_project_boundary_velocities(cno_scene *s, const float *cfg, float dt, c
                 bn[0] = n[0];
                 bn[1] = n[1];
                 bn[2] = n[2];
-                const float *cm = s->colliders[ci].material;
+                const float *cm = co.material;
                 bm[0] = sqrtf(maxf(0.0f, pm[0]) * maxf(0.0f, cm[0]));
                 bm[1] = maxf(maxf(0.0f, pm[1]), maxf(0.0f, cm[1]));
                 bm[2] = sqrtf(maxf(0.0f, pm[3]) * maxf(0.0f, cm[3]));
@@ -1828,7 +2040,22 @@ typedef struct contact_rec {
     uint32_t active;
 } contact_rec;
 
-/* IsValidParticleParticleCandidate for the pair lo < hi (the test is symmetric in the pair) */
+/* IsAdjacentParticle, physics_particle_broad_phase_query.hlsli:50-62: candidate among the
+ * particle's adjacency list, scanned in order */
+static int is_adjacent(const cno_scene *s, uint32_t particle, uint32_t candidate)
+{
+    if (!s->adj_offsets)
+        return 0;
+    const uint32_t off = s->adj_offsets[particle], cnt = s->adj_counts[particle];
+    for (uint32_t k = 0; k < cnt; ++k)
+        if (s->adj_indices[off + k] == candidate)
+            return 1;
+    return 0;
+}
+
+/* IsValidParticleParticleCandidate (physics_particle_broad_phase_query.hlsli:65-117) for the pair
+ * lo < hi, lo being the reference's particleIndex; the test is symmetric in the pair because the
+ * adjacency lists are */
 static int pp_valid(const cno_scene *s, uint32_t lo, uint32_t hi)
 {
     const uint32_t *ma = s->meta + 4 * (size_t)lo, *mb = s->meta + 4 * (size_t)hi;
@@ -1840,8 +2067,11 @@ static int pp_valid(const cno_scene *s, uint32_t lo, uint32_t hi)
     if (ka == CN_PARTICLE_KIND_FLUID && kb == CN_PARTICLE_KIND_FLUID)
         return 0;
     if (ka == CN_PARTICLE_KIND_SOLID && kb == CN_PARTICLE_KIND_SOLID &&
-        (ma[1] & PHASE_GROUP_MASK) == (mb[1] & PHASE_GROUP_MASK))
-        return 0; /* self collision of a group: rejected up front (no adjacency lists) */
+        (ma[1] & PHASE_GROUP_MASK) == (mb[1] & PHASE_GROUP_MASK)) {
+        /* one group: only when both have self collision on and they share no constraint */
+        if (!(ma[1] & PHASE_SELF_COLLIDE) || !(mb[1] & PHASE_SELF_COLLIDE) || is_adjacent(s, lo, hi))
+            return 0;
+    }
     float d[3];
     for (int a = 0; a < 3; ++a)
         d[a] = s->pos[4 * (size_t)hi + a] - s->pos[4 * (size_t)lo + a];
@@ -2025,17 +2255,272 @@ static void contact_velocity(const cno_scene *s, const float *cfg, uint32_t lo, 
     }
 }
 
-/* one contact iteration: generate every particle's contacts, sum its corrections in walk order,
- * then fold them into the positions */
-static void contact_iteration(cno_scene *s, const float *cfg, uint32_t pcap, const uint32_t *pcount,
-                              const uint32_t *plist, contact_rec *recs, float *corr)
+/* ---------------------------------------------------------------------------------------------
+ * Particle-rigid contacts of solid particles against the static colliders, every collider being a
+ * static body of its own.  Candidates once per substep from the predicted positions
+ * (buildParticleRigidCandidatePairs, physics_particle_count_particle_rigid_candidates.cs.hlsl
+ * :112-151: the particle's radius box against the static LBVH, solid particles only for a static
+ * body, the environment / layer / mask filters), in LBVH visit order.  Each contact iteration
+ * generates every candidate's contact at the current position
+ * (physics_particle_rigid_generate_contacts.cs.hlsl:30-190) and solves it
+ * (physics_particle_rigid_solve_contacts.cs.hlsl:26-140) into the correction the
+ * particle-particle contacts of the iteration sum into, after them.  After the particle-particle
+ * velocity iterations, the rigid velocity iterations solve the contacts of the last contact
+ * iteration (physics_particle_rigid_solve_contact_velocities.cs.hlsl:27-130), each followed by the
+ * velocity apply.  A static body has zero inverse mass and inertia and does not move, so its
+ * effective mass (ComputeContactEffectiveMass) is zero and its contact point's displacement and
+ * velocity are exactly zero.
+ * ------------------------------------------------------------------------------------------- */
+
+/* candidate colliders of particle i as a bitmask over the LBVH visit positions */
+static uint32_t rigid_mask(const cno_scene *s, const ocol *oc, const uint32_t *order, uint32_t nvis, uint32_t i)
+{
+    if (s->kinds[i] != CN_PARTICLE_KIND_SOLID)
+        return 0u;
+    const float *p = s->pos + 4 * (size_t)i;
+    const uint32_t *md = s->meta + 4 * (size_t)i;
+    const float r = s->radii[i];
+    float qmin[3], qmax[3];
+    for (int a = 0; a < 3; ++a) {
+        qmin[a] = p[a] - r;
+        qmax[a] = p[a] + r;
+    }
+    uint32_t mask = 0u;
+    for (uint32_t k = 0; k < nvis; ++k) {
+        const ocol *c = &oc[order[k]];
+        const int overlap = qmin[0] <= c->mx[0] && qmax[0] >= c->mn[0] && qmin[1] <= c->mx[1] &&
+                            qmax[1] >= c->mn[1] && qmin[2] <= c->mx[2] && qmax[2] >= c->mn[2];
+        if (overlap && c->enabled && c->env == md[0] && (md[3] & c->layer) != 0u && (c->mask & md[2]) != 0u)
+            mask |= 1u << k;
+    }
+    return mask;
+}
+
+/* generate_contacts :82-176 for one collider: the particle position p in the collider frame, the
+ * shape's signed distance and local normal, penetration = radius - distance; returns 1 with the
+ * world normal (SafeNormalize of the rotated local normal) when the penetration is positive */
+static int rigid_contact(const ocol *c, const float *p, float radius, float *nrm, float *pen)
+{
+    if (!c->enabled)
+        return 0;
+    const float qc[4] = {-c->q[0], -c->q[1], -c->q[2], c->q[3]};
+    float d[3], u[3], nl[3] = {0.0f, 1.0f, 0.0f};
+    float sd = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        d[a] = p[a] - c->c[a];
+    quat_rotate(qc, d, u); /* QuaternionInverseRotate */
+    if (c->shape == CN_SHAPE_SPHERE) {
+        const float len = sqrtf(dot3(u, u));
+        if (len > EPS)
+            for (int a = 0; a < 3; ++a)
+                nl[a] = u[a] / len;
+        sd = len - c->he[0];
+    } else if (c->shape == CN_SHAPE_BOX) {
+        float e[3];
+        for (int a = 0; a < 3; ++a)
+            e[a] = u[a] - clampf(u[a], -c->he[a], c->he[a]);
+        const float el = sqrtf(dot3(e, e));
+        if (el > EPS) {
+            for (int a = 0; a < 3; ++a)
+                nl[a] = e[a] / el;
+            sd = el;
+        } else {
+            /* ClosestFaceNormalLocal, physics_shape_common.hlsli:100-108 */
+            const float dx = c->he[0] - fabsf(u[0]), dy = c->he[1] - fabsf(u[1]), dz = c->he[2] - fabsf(u[2]);
+            nl[1] = 0.0f;
+            if (dx <= dy && dx <= dz)
+                nl[0] = u[0] >= 0.0f ? 1.0f : -1.0f;
+            else if (dy <= dz)
+                nl[1] = u[1] >= 0.0f ? 1.0f : -1.0f;
+            else
+                nl[2] = u[2] >= 0.0f ? 1.0f : -1.0f;
+            sd = -minf(dx, minf(dy, dz));
+        }
+    } else {
+        /* capsule: the closest point of the world segment, the world normal (fallback +x) taken
+         * into the collider frame */
+        const float g[3] = {c->sb[0] - c->sa[0], c->sb[1] - c->sa[1], c->sb[2] - c->sa[2]};
+        const float lsq = dot3(g, g);
+        float t = 0.0f;
+        if (lsq > EPS) {
+            const float w[3] = {p[0] - c->sa[0], p[1] - c->sa[1], p[2] - c->sa[2]};
+            t = saturatef(dot3(w, g) / lsq);
+        }
+        float e[3], wn[3] = {1.0f, 0.0f, 0.0f};
+        for (int a = 0; a < 3; ++a)
+            e[a] = p[a] - (c->sa[a] + t * g[a]);
+        const float len = sqrtf(dot3(e, e));
+        if (len > EPS)
+            for (int a = 0; a < 3; ++a)
+                wn[a] = e[a] / len;
+        sd = len - c->he[0];
+        quat_rotate(qc, wn, nl);
+    }
+    *pen = radius - sd;
+    if (!(*pen > 0.0f))
+        return 0;
+    float wn[3];
+    quat_rotate(c->q, nl, wn);
+    safe_normalize(wn, nrm);
+    return 1;
+}
+
+int cno_rigid_contact(const cn_collider *g, const float *p, float radius, float *nrm, float *pen)
+{
+    ocol o;
+    ocol_shape(g, &o);
+    *pen = 0.0f;
+    return rigid_contact(&o, p, radius, nrm, pen);
+}
+
+/* CombineContactMaterial of particle i's material and the collider's */
+static void rigid_material(const cno_scene *s, uint32_t i, const float *cm, float *m)
+{
+    const float *pa = s->part_mats + 4 * (size_t)s->part_mat[i];
+    m[0] = sqrtf(maxf(0.0f, pa[0]) * maxf(0.0f, cm[0]));
+    m[1] = maxf(maxf(0.0f, pa[1]), maxf(0.0f, cm[1]));
+    m[2] = sqrtf(maxf(0.0f, pa[3]) * maxf(0.0f, cm[3]));
+}
+
+/* solve_contacts :45-140 against a static body: particle i's position correction */
+static void rigid_position(const cno_scene *s, const float *cfg, uint32_t i, const ocol *c, const float *n,
+                           float pen0, float *out)
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    const float *p = s->pos + 4 * (size_t)i, *pv = s->prev + 4 * (size_t)i;
+    const float w = p[3];
+    if (w <= EPS)
+        return;
+    const float pen = minf(maxf(pen0 - cfg[CFG_CONTACT_SLOP], 0.0f), cfg[CFG_SOFT_MAX_CORRECTION]);
+    if (pen <= 0.0f)
+        return;
+    float nrm[3], mat[3], rel[3], tan[3], fd[3], corr[3];
+    safe_normalize(n, nrm);
+    rigid_material(s, i, c->material, mat);
+    const float denom = w + 0.0f;
+    if (denom <= EPS)
+        return;
+    const float relax = cfg[CFG_SOFT_CONTACT_RELAX];
+    const float lambda = (pen / denom) * relax;
+    for (int a = 0; a < 3; ++a) {
+        corr[a] = nrm[a] * (w * lambda);
+        rel[a] = 0.0f - (p[a] - pv[a]);
+    }
+    const float rn = dot3(rel, nrm);
+    for (int a = 0; a < 3; ++a)
+        tan[a] = rel[a] - nrm[a] * rn;
+    friction_delta(cfg, tan, pen, mat[0], mat[2], fd);
+    const float fdist = len3(fd);
+    if (fdist > 0.0f) {
+        const float tden = w + 0.0f;
+        if (tden > EPS) {
+            const float tl = fdist / tden;
+            for (int a = 0; a < 3; ++a)
+                corr[a] = corr[a] + (fd[a] / fdist) * ((tl * w) * relax);
+        }
+    }
+    memcpy(out, corr, sizeof(corr));
+}
+
+/* solve_contact_velocities :44-128 against a static body: particle i's velocity correction */
+static void rigid_velocity(const cno_scene *s, const float *cfg, uint32_t i, const ocol *c, const float *n,
+                           float pen0, float *out)
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    const float w = s->pos[4 * (size_t)i + 3];
+    if (w <= EPS)
+        return;
+    float nrm[3], rv[3], mat[3];
+    safe_normalize(n, nrm);
+    const float *v = s->vel + 4 * (size_t)i;
+    for (int a = 0; a < 3; ++a)
+        rv[a] = v[a] - 0.0f;
+    const float nv = dot3(rv, nrm);
+    if (nv >= 0.0f)
+        return;
+    const float denom = w + 0.0f;
+    if (denom <= EPS)
+        return;
+    rigid_material(s, i, c->material, mat);
+    const int rest_on = (-nv > cfg[CFG_RESTITUTION_VELOCITY]) &&
+                        (pen0 <= cfg[CFG_RESTITUTION_PENETRATION] * cfg[CFG_CONTACT_SLOP]);
+    const float restitution = rest_on ? saturatef(mat[1]) : 0.0f;
+    const float desired = rest_on ? (-restitution * nv) : 0.0f;
+    const float imp = maxf(0.0f, (desired - nv) / denom);
+    for (int a = 0; a < 3; ++a)
+        out[a] = (nrm[a] * imp) * w;
+}
+
+/* the rigid contacts' corrections of particle i at the current position, added to acc after its
+ * particle-particle ones; gen keeps the position they were generated at */
+static void rigid_iteration_particle(const cno_scene *s, const float *cfg, const ocol *oc, const uint32_t *order,
+                                     uint32_t i, uint32_t mask, float *gen, float *acc)
+{
+    const float *p = s->pos + 4 * (size_t)i;
+    memcpy(gen + 4 * (size_t)i, p, 4 * sizeof(float));
+    for (uint32_t k = 0; k < 32u; ++k) {
+        if (!((mask >> k) & 1u))
+            continue;
+        const ocol *c = &oc[order[k]];
+        float nrm[3], pen, d[3];
+        if (!rigid_contact(c, p, s->radii[i], nrm, &pen))
+            continue;
+        rigid_position(s, cfg, i, c, nrm, pen, d);
+        for (int a = 0; a < 3; ++a)
+            acc[a] = acc[a] + d[a];
+    }
+}
+
+/* one rigid velocity iteration over the contacts generated at gen, then the apply */
+static void rigid_velocity_iteration(cno_scene *s, const float *cfg, const ocol *oc, const uint32_t *order,
+                                     const uint32_t *rmask, const float *gen, float *corr)
 {
     const uint32_t n = s->n;
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
         float acc[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t k = 0; k < pcount[i]; ++k) {
+        for (uint32_t k = 0; k < 32u; ++k) {
+            if (!((rmask[i] >> k) & 1u))
+                continue;
+            const ocol *c = &oc[order[k]];
+            float nrm[3], pen, d[3];
+            if (!rigid_contact(c, gen + 4 * (size_t)i, s->radii[i], nrm, &pen))
+                continue;
+            rigid_velocity(s, cfg, i, c, nrm, pen, d);
+            for (int a = 0; a < 3; ++a)
+                acc[a] = acc[a] + d[a];
+        }
+        memcpy(corr + 4 * (size_t)i, acc, sizeof(acc));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        float *v = s->vel + 4 * (size_t)i;
+        if (s->pos[4 * (size_t)i + 3] <= EPS)
+            continue;
+        for (int a = 0; a < 3; ++a)
+            v[a] = v[a] + corr[4 * (size_t)i + a];
+    }
+}
+
+typedef struct rigid_state {
+    const ocol *oc;        /* the scene's colliders (index order) */
+    const uint32_t *order; /* LBVH visit order */
+    uint32_t *mask;        /* per particle: candidate visit positions */
+    float *gen;            /* per particle: position of the last generation (float4) */
+} rigid_state;
+
+/* one contact iteration: generate every particle's contacts, sum its corrections (the
+ * particle-particle ones in walk order when pcount is given, then the rigid ones in visit order
+ * when rs is), then fold them into the positions */
+static void contact_iteration(cno_scene *s, const float *cfg, uint32_t pcap, const uint32_t *pcount,
+                              const uint32_t *plist, contact_rec *recs, const rigid_state *rs, float *corr)
+{
+    const uint32_t n = s->n;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; pcount && k < pcount[i]; ++k) {
             const uint32_t j = plist[(size_t)i * pcap + k];
             const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
             contact_rec *c = recs + (size_t)i * pcap + k;
@@ -2047,6 +2532,8 @@ static void contact_iteration(cno_scene *s, const float *cfg, uint32_t pcap, con
             for (int a = 0; a < 3; ++a)
                 acc[a] = acc[a] + d[a];
         }
+        if (rs && rs->mask[i])
+            rigid_iteration_particle(s, cfg, rs->oc, rs->order, i, rs->mask[i], rs->gen, acc);
         memcpy(corr + 4 * (size_t)i, acc, sizeof(acc));
     }
     for (uint32_t i = 0; i < n; ++i) {
@@ -2114,7 +2601,7 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     for (uint32_t i = 0; i < n; ++i)
         fluid_count += s->kinds[i] == CN_PARTICLE_KIND_FLUID;
     const int has_fluid = fluid_count > 0 && n > 0;
-    const int has_bnd = has_fluid && s->n_colliders > 0;
+    const int has_bnd = has_fluid && n_cols(s) > 0;
 
     const uint32_t capacity = cno_next_pow2(n * 2u > 1u ? n * 2u : 1u);
     uint32_t *keys = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
@@ -2130,25 +2617,23 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     uint32_t *br = (uint32_t *)calloc(2 * (size_t)(n ? n : 1), sizeof(uint32_t));
     const uint32_t bcap = 32u * n;  /* fluidBoundaryCandidatePairCapacity, physics_scene_gpu_state.cpp:728-729 */
     uint32_t *bp = (uint32_t *)calloc(bcap ? bcap : 1, sizeof(uint32_t));
-    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_colliders ? s->n_colliders : 1));
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (n_cols(s) ? n_cols(s) : 1));
     if (!keys || !idx || !cells || !tbl || !counts || !req || !list || !dacc || !dtmp || !nc ||
         !br || !bp || !order)
         return -1;
-    cno_static_collider_order(s->n_colliders, s->colliders, order);
+    scene_collider_order(s, order);
 
     const int has_soft = n > 0 && (s->n_edges || s->n_bends || s->n_dihedrals || s->n_tets);
-    /* particle-particle contacts: any pair needs a solid particle (fluid-fluid pairs are never
-     * candidates); self-collision of a phase group needs adjacency lists and is not supported */
-    /* a pair can exist only between a solid and a fluid particle or solids of two phase groups;
-     * without one the contact passes would only add zero corrections (turning -0.0 coordinates
-     * into +0.0), so they are skipped */
+    /* particle-particle contacts: a pair can exist only between a solid and a fluid particle,
+     * solids of two phase groups, or solids of a self-colliding group; without one the contact
+     * passes would only add zero corrections (turning -0.0 coordinates into +0.0), so they are
+     * skipped */
     uint32_t solid_count = 0, fluid_count_c = 0, group0 = 0xffffffffu;
-    int two_groups = 0;
+    int two_groups = 0, self_group = 0;
     for (uint32_t i = 0; i < n; ++i)
         if (s->kinds[i] == CN_PARTICLE_KIND_SOLID) {
             ++solid_count;
-            if (s->meta[4 * (size_t)i + 1] & PHASE_SELF_COLLIDE)
-                return -2;
+            self_group |= (s->meta[4 * (size_t)i + 1] & PHASE_SELF_COLLIDE) != 0u;
             const uint32_t g = s->meta[4 * (size_t)i + 1] & PHASE_GROUP_MASK;
             if (group0 == 0xffffffffu)
                 group0 = g;
@@ -2157,7 +2642,11 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
             ++fluid_count_c;
         }
     const int has_contacts =
-        contact_iters > 0 && n > 1 && solid_count > 0 && (fluid_count_c > 0 || two_groups);
+        contact_iters > 0 && n > 1 && solid_count > 0 && (fluid_count_c > 0 || two_groups || self_group);
+    /* particle-rigid contacts: a solid particle and a static collider (hasSoftRigidContactWork,
+     * physics_solver.cpp:486-488; without a solid particle no candidate exists, and the passes
+     * would again only add zero corrections) */
+    const int has_rigid = contact_iters > 0 && solid_count > 0 && n_cols(s) > 0;
     const uint32_t pcap = CNO_CONTACT_PARTNERS;
     uint32_t *pcount = NULL, *plist = NULL;
     contact_rec *recs = NULL;
@@ -2166,8 +2655,28 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         pcount = (uint32_t *)calloc(n, sizeof(uint32_t));
         plist = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n * pcap);
         recs = (contact_rec *)calloc((size_t)n * pcap, sizeof(contact_rec));
+        if (!pcount || !plist || !recs)
+            return -1;
+    }
+    rigid_state rs = {NULL, order, NULL, NULL};
+    ocol *roc = NULL;
+    uint32_t rnvis = 0;
+    if (has_rigid) {
+        const uint32_t nc = n_cols(s);
+        roc = (ocol *)malloc(sizeof(ocol) * nc);
+        rs.mask = (uint32_t *)calloc(n, sizeof(uint32_t));
+        rs.gen = (float *)calloc(4 * (size_t)n, sizeof(float));
+        if (!roc || !rs.mask || !rs.gen)
+            return -1;
+        for (uint32_t k = 0; k < nc; ++k)
+            ocol_get(s, k, &roc[k]);
+        rs.oc = roc;
+        while (rnvis < nc && order[rnvis] != INVALID)
+            ++rnvis;
+    }
+    if (has_contacts || has_rigid) {
         ccorr = (float *)calloc(4 * (size_t)n, sizeof(float));
-        if (!pcount || !plist || !recs || !ccorr)
+        if (!ccorr)
             return -1;
     }
     csr inc_e = {0, 0}, inc_b = {0, 0}, inc_t = {0, 0};
@@ -2193,7 +2702,7 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     cno_stats st;
     memset(&st, 0, sizeof(st));
     st.fluid_count = fluid_count;
-    const int has_bp_work = (contact_iters > 0 && n > 1) || has_fluid || s->n_colliders > 0;
+    const int has_bp_work = (contact_iters > 0 && n > 1) || has_fluid || n_cols(s) > 0;
 
     for (uint32_t sub = 0; sub < substeps; ++sub) {
         cno_predict(s, dt, prm->gravity);
@@ -2208,6 +2717,9 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
             cno_fluid_boundary_candidates(s, order, br, bp, bcap);
         if (has_contacts)
             st.contact_overflows += cno_contact_partners(s, cells, idx, tbl, capacity, pcap, pcount, plist);
+        if (has_rigid)
+            for (uint32_t i = 0; i < n; ++i)
+                rs.mask[i] = rigid_mask(s, roc, order, rnvis, i);
         if (has_soft) {
             /* clearSoftConstraintState, physics_clear_soft_constraint_state.cs.hlsl:14-54 */
             memset(lam_e, 0, sizeof(float) * (s->n_edges + 1));
@@ -2247,8 +2759,9 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
                 solve_tets(s, cfg, dt, lam_t);
                 apply_corrections(s, &inc_t, s->tet_corr, 16);
             }
-            if (has_contacts && it < contact_iters)
-                contact_iteration(s, cfg, pcap, pcount, plist, recs, ccorr);
+            if ((has_contacts || has_rigid) && it < contact_iters)
+                contact_iteration(s, cfg, pcap, has_contacts ? pcount : NULL, plist, recs, has_rigid ? &rs : NULL,
+                                  ccorr);
         }
         cno_update_velocities(s, dt, dacc);
         if (has_fluid) {
@@ -2267,6 +2780,10 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         if (has_contacts)
             for (uint32_t v = 0; v < contact_iters; ++v)
                 contact_velocity_iteration(s, cfg, pcap, pcount, plist, recs, ccorr);
+        /* physics_solver.cpp:1301-1309, after the particle-particle velocity iterations */
+        if (has_rigid)
+            for (uint32_t v = 0; v < contact_iters; ++v)
+                rigid_velocity_iteration(s, cfg, roc, order, rs.mask, rs.gen, ccorr);
     }
     /* density-constraint residual of the last density pass of the step */
     if (has_fluid) {
@@ -2302,6 +2819,9 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     free(plist);
     free(recs);
     free(ccorr);
+    free(roc);
+    free(rs.mask);
+    free(rs.gen);
     if (has_soft) {
         csr_free(&inc_e);
         csr_free(&inc_b);

@@ -54,6 +54,16 @@ typedef struct cno_scene {
     /* optional output: float4[3n], the render anisotropy of the last substep (axis1 of every
      * particle, then axis2, then axis3; w = scale), or NULL */
     float *render_aniso;
+    /* constraint adjacency of the particles (ParticleSoA adjacencyOffsets/Counts/Indices,
+     * physics_world.cpp:5774-5787), read by IsAdjacentParticle for self-colliding phase groups;
+     * NULL offsets: no particle has any */
+    const uint32_t *adj_offsets;
+    const uint32_t *adj_counts;
+    const uint32_t *adj_indices;
+    /* static colliders of any shape after the boxes: collider k >= n_colliders is
+     * shape_colliders[k - n_colliders] */
+    uint32_t n_shape_colliders;
+    const cn_collider *shape_colliders;
 } cno_scene;
 
 typedef struct cno_stats {
@@ -86,6 +96,13 @@ int cno_find_cell_range(const uint32_t *ranges, uint32_t capacity, uint32_t key,
 /* Order in which the static LBVH visits the colliders (reverse Morton order). */
 void cno_static_collider_order(uint32_t n_colliders, const cn_box_collider *colliders,
                                uint32_t *order);
+/* ComputeFluidBoundaryContactInfo of point p against one resolved static collider: returns 0 when
+ * it is disabled; sd = signed distance, nrm = normal, aabb (may be NULL) = broad-phase min, max. */
+int cno_collider_contact(const cn_collider *collider, const float *p, float *sd, float *nrm, float *aabb);
+/* The particle-rigid contact of a solid particle (position p, radius) against one resolved static
+ * collider (physics_particle_rigid_generate_contacts.cs.hlsl:82-176): returns 1 with the world
+ * normal and the penetration when the penetration is positive. */
+int cno_rigid_contact(const cn_collider *collider, const float *p, float radius, float *nrm, float *pen);
 /* counts[n], offsets[n] (exclusive), pairs[capacity] = collider indices; ranges uint2[n] */
 uint32_t cno_fluid_boundary_candidates(const cno_scene *s, const uint32_t *order, uint32_t *ranges,
                                        uint32_t *pairs, uint32_t capacity);

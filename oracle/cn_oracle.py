@@ -27,7 +27,9 @@ class CnoScene(C.Structure):
                 ("n_dihedrals", C.c_uint32), ("dihedrals", C.c_void_p),
                 ("n_tets", C.c_uint32), ("tets", C.c_void_p),
                 ("edge_corr", C.c_void_p), ("bend_corr", C.c_void_p), ("tet_corr", C.c_void_p),
-                ("render_aniso", C.c_void_p)]
+                ("render_aniso", C.c_void_p),
+                ("adj_offsets", C.c_void_p), ("adj_counts", C.c_void_p), ("adj_indices", C.c_void_p),
+                ("n_shape_colliders", C.c_uint32), ("shape_colliders", C.c_void_p)]
 
 
 class CnoStats(C.Structure):
@@ -117,6 +119,17 @@ class OracleScene:
                           len(self.edges), p(self.edges), len(self.bends), p(self.bends),
                           len(self.dihedrals), p(self.dihedrals), len(self.tets), p(self.tets),
                           self.edge_corr.ctypes.data, self.bend_corr.ctypes.data, self.tet_corr.ctypes.data)
+        shapes = getattr(sc, "shape_colliders", None)
+        if shapes is not None and len(shapes):
+            self.shape_colliders = np.ascontiguousarray(shapes)
+            self.c.n_shape_colliders = len(self.shape_colliders)
+            self.c.shape_colliders = self.shape_colliders.ctypes.data
+        adj = getattr(sc, "adjacency", None)
+        if adj is not None:
+            self.adj = tuple(np.ascontiguousarray(a, np.uint32) for a in adj)
+            self.c.adj_offsets = self.adj[0].ctypes.data
+            self.c.adj_counts = self.adj[1].ctypes.data
+            self.c.adj_indices = self.adj[2].ctypes.data if len(self.adj[2]) else None
 
     @property
     def n(self):
@@ -185,6 +198,32 @@ def find_cell_range(ranges: np.ndarray, key: int):
     e = C.c_uint32()
     ok = lib.cno_find_cell_range(r.ctypes.data, r.shape[0], key, C.byref(s), C.byref(e))
     return (int(s.value), int(e.value)) if ok else None
+
+
+def collider_contact(collider: np.ndarray, p):
+    """(hit, signed distance, normal, aabb (6,)) of point p against one shape collider record."""
+    lib = load()
+    c = np.ascontiguousarray(collider).reshape(1)
+    pt = np.ascontiguousarray(p, np.float32).reshape(3)
+    sd = C.c_float()
+    nrm = np.zeros(3, np.float32)
+    aabb = np.zeros(6, np.float32)
+    hit = lib.cno_collider_contact(C.c_void_p(c.ctypes.data), C.c_void_p(pt.ctypes.data), C.byref(sd),
+                                   C.c_void_p(nrm.ctypes.data), C.c_void_p(aabb.ctypes.data))
+    return bool(hit), float(sd.value), nrm, aabb
+
+
+def rigid_contact(collider: np.ndarray, p, radius: float):
+    """(hit, normal, penetration) of a solid particle at p with the given radius against one shape
+    collider record (a static body of its own)."""
+    lib = load()
+    c = np.ascontiguousarray(collider).reshape(1)
+    pt = np.ascontiguousarray(p, np.float32).reshape(3)
+    pen = C.c_float()
+    nrm = np.zeros(3, np.float32)
+    hit = lib.cno_rigid_contact(C.c_void_p(c.ctypes.data), C.c_void_p(pt.ctypes.data), C.c_float(radius),
+                                C.c_void_p(nrm.ctypes.data), C.byref(pen))
+    return bool(hit), nrm, float(pen.value)
 
 
 def static_collider_order(colliders: np.ndarray) -> np.ndarray:

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# The CPU oracle runs OpenMP loops: under pytest-xdist every worker would start one thread per
+# core and spin at each barrier, so workers share the cores instead (set before the oracle loads).
+_workers = int(os.environ.get("PYTEST_XDIST_WORKER_COUNT", "1") or 1)
+if _workers > 1:
+    os.environ.setdefault("OMP_NUM_THREADS", str(max(1, (os.cpu_count() or 1) // _workers)))
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for sub in ("cressim-neo_amd", "oracle"):
     p = os.path.join(ROOT, sub)

@@ -1,5 +1,6 @@
 """GPU parity of particle-particle contacts (candidate lists, Jacobi position solve, velocity solve)
-against the CPU oracle, alone and inside a scene pool, plus the capacity and refusal paths."""
+against the CPU oracle, alone and inside a scene pool, self collision of a folded cloth, plus the
+capacity path."""
 import numpy as np
 import pytest
 
@@ -70,13 +71,37 @@ def test_contact_overflow_is_reported(solver, oracle):
     assert e.value.status == cnhip.abi.CN_ERR_CAPACITY
 
 
-def test_self_colliding_solid_group_is_refused(solver):
-    sc = scenes.contact_stack(4, 12, 3)
-    sc.phase[sc.kinds == 0] |= np.uint32(0x80000000)
-    solver.reserve(sc.n, 64)
-    with pytest.raises(cnhip.CnError) as e:
-        solver.upload(sc)
-    assert e.value.status == cnhip.abi.CN_ERR_UNSUPPORTED
+@pytest.mark.parametrize("iters,substeps", [(1, 1), (4, 4)])
+def test_self_colliding_cloth_fold_bitexact(solver, oracle, iters, substeps):
+    """A cloth folded onto itself with self collision: the layers push apart where they are not
+    constraint-adjacent (IsAdjacentParticle keeps the hinge opposites across the crease out), bit
+    for bit against the oracle; without the flag the layers pass through each other."""
+    print(f"folded_cloth(16) iters={iters} substeps={substeps}", flush=True)
+    p = params(iters=iters, substeps=substeps)
+    res = run_pair(scenes.folded_cloth(16), p, 4, solver, oracle)
+    print(res)
+    assert all(b for _, _, b in res), res
+    collided = solver.readback()[0]
+    res = run_pair(scenes.folded_cloth(16, self_collide=False), p, 4, solver, oracle)
+    assert all(b for _, _, b in res), res
+    assert not np.array_equal(collided, solver.readback()[0])
+
+
+def test_self_collision_in_a_pool_equals_alone(solver, oracle):
+    batch = [scenes.folded_cloth(12), scenes.contact_stack(4, 12, 3), scenes.folded_cloth(20)]
+    p = params(iters=3, substeps=2)
+    with cnhip.Solver(0) as pool:
+        pool.reserve(sum(sc.n for sc in batch), 64)
+        pool.upload_batch(batch)
+        for _ in range(2):
+            pool.step(p)
+        got = [pool.readback_scene(k) for k in range(len(batch))]
+    for k, sc in enumerate(batch):
+        osc = oracle.OracleScene(sc)
+        for _ in range(2):
+            oracle.step(osc, p)
+        np.testing.assert_array_equal(got[k][0], osc.pos, err_msg=f"scene {k}")
+        np.testing.assert_array_equal(got[k][2], osc.vel, err_msg=f"scene {k}")
 
 
 def test_contact_scenes_in_a_pool_equal_alone(solver, oracle):

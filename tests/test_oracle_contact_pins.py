@@ -1,6 +1,7 @@
 """Pins of the oracle's particle-particle contacts (CPU only): closed forms of one Jacobi contact
-solve between two free particles, the candidate filters of IsValidParticleParticleCandidate, the
-velocity solve and the partner-list capacity."""
+solve between two free particles, the candidate filters of IsValidParticleParticleCandidate
+(self collision inside a phase group and its constraint-adjacency filter included), the velocity
+solve and the partner-list capacity."""
 import numpy as np
 import pytest
 
@@ -151,11 +152,63 @@ def test_partner_overflow_is_counted(oracle):
     assert st.contact_overflows == 3
 
 
-def test_self_colliding_solid_group_is_refused(oracle):
-    sc = _pair((0, 0, 0), (0.015, 0, 0))
-    sc.phase[:] |= np.uint32(0x80000000)
-    with pytest.raises(RuntimeError):
-        _run(oracle, sc)
+SELF = np.uint32(0x80000000)
+
+
+def _one_group(flags=(True, True), adjacent=False):
+    """Two overlapping solid particles of one phase group (radii 0.01 + 0.01 at distance 0.015),
+    each with its self-collide flag as given, constraint-adjacent or not."""
+    sc = _pair((0, 0, 0), (0.015, 0, 0), same_part=True)
+    assert sc.phase[0] == sc.phase[1]
+    for k in range(2):
+        if flags[k]:
+            sc.phase[k] |= SELF
+    if adjacent:
+        from cnhip import authoring as au
+        sc.adjacency = au.adjacency(2, [(0, 1)])
+    else:
+        sc.adjacency = (np.zeros(2, np.uint32), np.zeros(2, np.uint32), np.zeros(0, np.uint32))
+    return sc
+
+
+def test_self_collision_inside_a_group(oracle):
+    """IsValidParticleParticleCandidate's group branch: one group collides with itself only when
+    both particles have the self-collide flag and they are not constraint-adjacent; then the pair
+    separates exactly as a pair of two groups does."""
+    lam = (0.005 - SLOP) / 2 * RELAX
+    osc, _ = _run(oracle, _one_group((True, True)))
+    assert osc.pos[0, 0] == pytest.approx(-lam, abs=2e-8)
+    assert osc.pos[1, 0] == pytest.approx(0.015 + lam, abs=2e-8)
+    two, _ = _run(oracle, _pair((0, 0, 0), (0.015, 0, 0)))
+    np.testing.assert_array_equal(osc.pos, two.pos)
+    for flags in ((True, False), (False, True), (False, False)):
+        osc, _ = _run(oracle, _one_group(flags))
+        assert osc.pos[0, 0] == 0 and osc.pos[1, 0] == F(0.015), flags
+    osc, _ = _run(oracle, _one_group((True, True), adjacent=True))
+    assert osc.pos[0, 0] == 0 and osc.pos[1, 0] == F(0.015)
+
+
+def test_adjacency_is_the_reference_construction():
+    """Tet soft bodies: the tet edges; cloths: structural edges plus the opposite vertices of
+    every bending hinge; both directions, each list sorted and unique
+    (soft_body_authoring.cpp:392-408, cloth_authoring.cpp:139-188)."""
+    from cnhip import authoring as au
+    quad = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1], [1, 0, 1]], np.float32)
+    e, d = au.cloth_constraints(quad, np.array([[0, 1, 3], [0, 3, 2]], np.uint32))
+    off, cnt, idx = au.adjacency(4, au.cloth_adjacency_pairs(e, d))
+    lists = [idx[off[i]:off[i] + cnt[i]].tolist() for i in range(4)]
+    # five structural edges (0-1, 0-2, 0-3, 1-3, 2-3) and the hinge 0-3 whose opposites are 1 and 2
+    assert lists == [[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]]
+    pos, tets = au.tet_grid((1.0, 1.0, 1.0), resolution=(2, 2, 2))
+    edges, _ = au.tet_constraints(pos, tets)
+    off, cnt, idx = au.adjacency(len(pos), au.tet_adjacency_pairs(edges))
+    want = [set() for _ in range(len(pos))]
+    for t in tets:
+        for a in range(4):
+            for b in range(4):
+                if a != b:
+                    want[int(t[a])].add(int(t[b]))
+    assert [idx[off[i]:off[i] + cnt[i]].tolist() for i in range(len(pos))] == [sorted(w) for w in want]
 
 
 def test_contact_stack_has_contacts(oracle):
