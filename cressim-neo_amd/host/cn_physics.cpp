@@ -245,7 +245,64 @@ bool PhysicsSolver::upload(PhysicsWorld &w)
         return fail("reserve failed");
     if (cn_scene_upload(ctx_, &v, &t, &c) != CN_OK) return fail("upload failed");
     uploadedRevision_ = w.revision();
+    uploadedColliders_ = w.colliderRevision();
+    uploadedMaterials_ = w.materialRevision();
+    uploadedConstraints_ = w.constraintRevision();
     uploadedCount_ = n;
+    w.clearDirtyParticles();
+    lastUpload_ = 1;
+    return true;
+}
+
+// The categories whose revisions moved, and the dirty particles as contiguous runs
+// (uploadContiguousRuns, physics_scene_gpu_state.cpp:255-284).
+bool PhysicsSolver::uploadChanges(PhysicsWorld &w)
+{
+    lastUpload_ = 0;
+    lastRuns_ = 0;
+    if (w.colliderRevision() != uploadedColliders_) {
+        if (cn_scene_set_colliders(ctx_, 0, w.staticBoxes.data(), static_cast<std::uint32_t>(w.staticBoxes.size())) !=
+            CN_OK)
+            return fail("collider update failed");
+        uploadedColliders_ = w.colliderRevision();
+        lastUpload_ = 2;
+    }
+    if (w.materialRevision() != uploadedMaterials_) {
+        if (cn_scene_set_materials(ctx_, 0, w.fluidMaterials.data(), static_cast<std::uint32_t>(w.fluidMaterials.size()),
+                                   &w.particleContactMaterials[0].x,
+                                   static_cast<std::uint32_t>(w.particleContactMaterials.size())) != CN_OK)
+            return fail("material update failed");
+        uploadedMaterials_ = w.materialRevision();
+        lastUpload_ = 2;
+    }
+    if (w.constraintRevision() != uploadedConstraints_) {
+        cn_constraint_view c{};
+        c.edge_count = static_cast<std::uint32_t>(w.softEdges.size());
+        c.edges = w.softEdges.data();
+        c.tet_count = static_cast<std::uint32_t>(w.softTets.size());
+        c.tets = w.softTets.data();
+        if (cn_scene_set_constraints(ctx_, 0, &c) != CN_OK) return fail("constraint update failed");
+        uploadedConstraints_ = w.constraintRevision();
+        lastUpload_ = 2;
+    }
+    std::vector<std::uint32_t> idx = w.dirtyParticles();
+    if (!idx.empty()) {
+        std::sort(idx.begin(), idx.end());
+        idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+        if (idx.back() >= uploadedCount_) return fail("dirty particle index out of range");
+        std::size_t r0 = 0;
+        for (std::size_t k = 1; k <= idx.size(); ++k) {
+            if (k < idx.size() && idx[k] == idx[k - 1] + 1u) continue;
+            const std::uint32_t first = idx[r0], count = static_cast<std::uint32_t>(k - r0);
+            if (cn_particles_update_range(ctx_, first, count, &w.positionsInvMass[first].x, &w.velocities[first].x, 0u) !=
+                CN_OK)
+                return fail("particle update failed");
+            ++lastRuns_;
+            r0 = k;
+        }
+        w.clearDirtyParticles();
+        lastUpload_ = 2;
+    }
     return true;
 }
 
@@ -253,7 +310,11 @@ bool PhysicsSolver::step(const FrameContext &frame, PhysicsWorld &world)
 {
     if (!ctx_) return fail("step before initialize");
     if (world.particleCount() == 0) return true;  // nothing on the particle path
-    if (world.revision() != uploadedRevision_ && !upload(world)) return false;
+    if (world.revision() != uploadedRevision_) {
+        if (!upload(world)) return false;
+    } else if (!uploadChanges(world)) {
+        return false;
+    }
     cn_step_params p;
     cn_default_step_params(&p);
     p.dt = frame.deltaSeconds;

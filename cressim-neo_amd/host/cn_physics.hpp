@@ -97,15 +97,39 @@ public:
     void addStaticBox(const StaticBoxDesc &desc);
 
     std::uint32_t particleCount() const { return static_cast<std::uint32_t>(positionsInvMass.size()); }
+    // Layout revision: particle count, per-particle attributes, topology.  A change re-uploads the
+    // world on the next step.
     std::uint64_t revision() const { return revision_; }
     // Call after editing the arrays directly (the next step uploads the world again).
     void markDirty() { ++revision_; }
+
+    // Narrower edits, uploaded without a re-upload on the next step (the reference's per-category
+    // revisions and dirty-index runs, physics_scene_gpu_state.cpp:255-284, 2348-2420):
+    // positionsInvMass / velocities of these particles (sent as contiguous runs)
+    void markParticleDirty(std::uint32_t index) { dirtyParticles_.push_back(index); }
+    void markParticleRangeDirty(std::uint32_t first, std::uint32_t count)
+    {
+        for (std::uint32_t i = 0; i < count; ++i) dirtyParticles_.push_back(first + i);
+    }
+    // staticBoxes moved, resized, re-materialled, enabled or disabled (same count)
+    void markCollidersDirty() { ++colliderRevision_; }
+    // fluidMaterials (same smoothing radii) or particleContactMaterials (same counts)
+    void markMaterialsDirty() { ++materialRevision_; }
+    // rest lengths, rest volumes, compliances, enable flags of softEdges / softTets (same topology)
+    void markConstraintsDirty() { ++constraintRevision_; }
+    std::uint64_t colliderRevision() const { return colliderRevision_; }
+    std::uint64_t materialRevision() const { return materialRevision_; }
+    std::uint64_t constraintRevision() const { return constraintRevision_; }
+    const std::vector<std::uint32_t> &dirtyParticles() const { return dirtyParticles_; }
+    void clearDirtyParticles() { dirtyParticles_.clear(); }
 
 private:
     std::uint32_t appendParticles(std::size_t count, std::uint32_t kind, float invMass, float radius,
                                   std::uint32_t fluidMaterial, std::uint32_t env, std::uint32_t phase);
     std::uint32_t nextGroup_ = 1;
     std::uint64_t revision_ = 0;
+    std::uint64_t colliderRevision_ = 0, materialRevision_ = 0, constraintRevision_ = 0;
+    std::vector<std::uint32_t> dirtyParticles_;
 };
 
 class PhysicsSolver {
@@ -119,10 +143,14 @@ public:
     void shutdown();
     PhysicsSolverDesc &desc() { return desc_; }
 
-    // Uploads the world when it changed since the last step, runs one frame (desc().substeps
-    // substeps) and reads the particle state back into the world (blocking, like the reference's
+    // Uploads the world when its layout changed since the last step, otherwise only what its
+    // narrower revisions and dirty particles name; runs one frame (desc().substeps substeps) and
+    // reads the particle state back into the world (blocking, like the reference's
     // readbackParticles after a step).
     bool step(const FrameContext &frame, PhysicsWorld &world);
+    // what the last step sent: 0 nothing, 1 a full upload, 2 partial updates
+    int lastUploadKind() const { return lastUpload_; }
+    std::uint32_t lastUploadRuns() const { return lastRuns_; }
     // false when an overflow counter of the last step is set (fluid-neighbor truncation, contact
     // candidates beyond a particle's list); the counters stay in meta()
     bool validateGpuMetaBlocking();
@@ -133,7 +161,11 @@ public:
 private:
     bool fail(const std::string &what);
     bool upload(PhysicsWorld &world);
+    bool uploadChanges(PhysicsWorld &world);
     int device_;
+    std::uint64_t uploadedColliders_ = 0, uploadedMaterials_ = 0, uploadedConstraints_ = 0;
+    int lastUpload_ = 0;
+    std::uint32_t lastRuns_ = 0;
     cn_ctx *ctx_ = nullptr;
     PhysicsSolverDesc desc_;
     std::uint64_t uploadedRevision_ = ~0ull;

@@ -30,7 +30,7 @@ def test_world_soa_layout():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["physics_fluid_compute_smoke", "physics_solver_broad_phase_overflow",
-                                  "physics_soft_gpu_scene_upload"])
+                                  "physics_soft_gpu_scene_upload", "physics_world_revision_upload"])
 def test_cpp_gpu(name):
     print(f"running {name}", flush=True)
     assert "checks passed" in run(name)

@@ -74,3 +74,38 @@ def test_reset_only_touches_its_scene_and_takes_tensors():
             pool.set_scene_state(1, ses[0].pos, ses[0].vel)  # wrong particle count
     finally:
         pool.close()
+
+
+def test_dirty_runs_equal_whole_scene_writes():
+    """cn_particles_update_range: runs of edited particles (one inside a scene, one across a scene
+    boundary) give the bits of writing the edited scenes whole, and leave the rest alone."""
+    ses = [scenes.batch_scene(n, k, seed=20 + i) for i, (n, k) in enumerate([(1000, 0), (800, 1), (600, 0)])]
+    runs = [(37, 5), (ses[0].n - 3, 9)]  # the second run starts in scene 0 and ends in scene 1
+    pools = [cnhip.Solver(0), cnhip.Solver(0)]
+    try:
+        for pool in pools:
+            pool.reserve(sum(sc.n for sc in ses), 64)
+            pool.upload_batch(ses)
+            for _ in range(2):
+                pool.step(params())
+        pos, _, vel = pools[0].readback()
+        for first, count in runs:
+            pos[first:first + count, 1] += np.float32(0.004)
+            vel[first:first + count, 0] = np.float32(0.25)
+            pools[0].update_particles(first, pos[first:first + count], vel[first:first + count])
+        for k in (0, 1):  # the scenes the runs touch, written whole
+            first, count = pools[1].ranges[k]
+            pools[1].set_scene_state(k, pos[first:first + count], vel[first:first + count])
+        a0 = pools[0].readback()
+        np.testing.assert_array_equal(a0[0], pos)
+        np.testing.assert_array_equal(a0[2], vel)
+        for pool in pools:
+            pool.step(params())
+        a, b = pools[0].readback(), pools[1].readback()
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+        with pytest.raises(cnhip.CnError):
+            pools[0].update_particles(sum(sc.n for sc in ses) - 2, pos[:3], vel[:3])  # past the end
+    finally:
+        for pool in pools:
+            pool.close()
