@@ -22,6 +22,7 @@ HBM_PEAK_GBS = 8000.0
 SUBSTEPS = 4
 ITERS = 4
 CAP = 64
+BUILD_WORKERS = 8  # processes that build the batched scenes on the host (the CPU share of one GPU)
 
 
 def algorithmic_bytes(kbar: float) -> dict:
@@ -69,7 +70,7 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     import cnhip
     from cnhip import scenes
     t0 = time.perf_counter()
-    batch = scenes.batched(count)
+    batch = scenes.batched(count, workers=BUILD_WORKERS)
     gen_s = time.perf_counter() - t0
     total = sum(sc.n for sc in batch)
     p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
@@ -115,30 +116,42 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
         s.reserve(sc.n, CAP)
         s.upload(sc)
         ctxs.append(s)
+
+    def loop(wait: bool):
+        t0 = time.perf_counter()
+        for _ in range(loop_steps):
+            for s in ctxs:
+                s.step(p)
+                if wait:
+                    s.synchronize()
+        for s in ctxs:
+            s.synchronize()
+        return count * loop_steps / (time.perf_counter() - t0)
+
     for s in ctxs:
         s.step(p)
     for s in ctxs:
         s.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(loop_steps):
-        for s in ctxs:
-            s.step(p)
+    # each scene's step issued in turn, the contexts' streams overlapping on the device
+    env_loop = loop(False)
+    # the same loop with each scene's step waited for before the next scene's (a blocking step call)
+    env_sync = loop(True)
+    # the overlapping loop with every context's step graph-captured (captured by the first step)
+    for s in ctxs:
+        s.graphs(True)
+        s.step(p)
     for s in ctxs:
         s.synchronize()
-    el_loop = time.perf_counter() - t0
-    # the same loop with each scene's step waited for before the next scene's (a blocking step call)
-    t0 = time.perf_counter()
-    for _ in range(loop_steps):
-        for s in ctxs:
-            s.step(p)
-            s.synchronize()
-    el_sync = time.perf_counter() - t0
+    env_graph = loop(False)
     for s in ctxs:
         s.close()
-    env_loop = count * loop_steps / el_loop
-    env_sync = count * loop_steps / el_sync
-    out.update(loop_plain_env_steps_per_s=env_loop, loop_plain_steps=loop_steps, speedup_vs_loop_plain=env / env_loop,
-               loop_plain_sync_env_steps_per_s=env_sync, speedup_vs_loop_plain_sync=env / env_sync)
+    best = max(env_loop, env_graph)
+    print(f"bench: plain loops {env_loop:.0f} (overlapping) / {env_graph:.0f} (graphs) / {env_sync:.0f} (waiting) "
+          f"env-steps/s", file=sys.stderr, flush=True)
+    out.update(loop_plain_steps=loop_steps, loop_plain_env_steps_per_s=env_loop,
+               loop_plain_graph_env_steps_per_s=env_graph, loop_plain_sync_env_steps_per_s=env_sync,
+               speedup_vs_loop_plain=env / best, speedup_vs_loop_plain_overlap=env / env_loop,
+               speedup_vs_loop_plain_graph=env / env_graph, speedup_vs_loop_plain_sync=env / env_sync)
     return out
 
 
@@ -151,7 +164,7 @@ def serve_bench(sessions: int, rates, duration: float, shards: int, loop_request
     import cnhip
     from cnhip import scenes, serving
     t0 = time.perf_counter()
-    ses = scenes.batched(sessions)
+    ses = scenes.batched(sessions, workers=BUILD_WORKERS)
     total = sum(sc.n for sc in ses)
     p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
     print(f"bench: {sessions} sessions, {total} particles (built in {time.perf_counter() - t0:.1f} s)",
@@ -223,8 +236,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=512)
-    ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--loop-steps", type=int, default=10)
+    ap.add_argument("--late-steps", type=int, default=20,
+                    help="steps timed in the later window (steps 26-45 of the 4M workload); 0 skips it")
+    ap.add_argument("--no-serve", action="store_true", help="leave the served leg out of the default line")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered req/s of the default line's served leg")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched workload (with --loop-steps 0: the pool alone, for profiling)")
@@ -237,7 +254,7 @@ def main():
                     help="end every 4M step with the render anisotropy (off in the headline)")
     ap.add_argument("--serve", action="store_true", help="only run the served workload (2048 sessions)")
     ap.add_argument("--serve-sessions", type=int, default=2048)
-    ap.add_argument("--serve-duration", type=float, default=2.0)
+    ap.add_argument("--serve-duration", type=float, default=1.5)
     ap.add_argument("--serve-shards", type=int, default=8)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
@@ -301,6 +318,33 @@ def main():
           f"LDS blocks {meta.lds_tiles}, run-table blocks {meta.run_table_tiles}, "
           f"candidate reuses {meta.candidate_reuses}, max displacement {maxd:.3e}", file=sys.stderr, flush=True)
 
+    # a later window of the same run: the fluid compresses under gravity, so the mean neighbor count
+    # and the cost per step grow; each step is waited for and its diagnostics read (truncations
+    # summed over the window)
+    late = None
+    if args.late_steps > 0:
+        done_steps = args.warmup + args.steps
+        first = max(done_steps + 1, 26)
+        for _ in range(first - 1 - done_steps):
+            s.step(p)
+        s.synchronize()
+        el_late, ks, kmax, trunc = 0.0, [], 0, 0
+        for _ in range(args.late_steps):
+            t = time.perf_counter()
+            s.step(p)
+            s.synchronize()
+            el_late += time.perf_counter() - t
+            m = s.meta()
+            ks.append(m.fluid_neighbor_total / (sc.n * SUBSTEPS))
+            kmax = max(kmax, m.max_fluid_neighbor_count)
+            trunc += m.fluid_neighbor_truncations
+        late = {"steps": f"{first}-{first + args.late_steps - 1}", "ms_per_step": el_late * 1e3 / args.late_steps,
+                "particle_substeps_per_s": sc.n * SUBSTEPS * args.late_steps / el_late,
+                "mean_neighbors_first": round(ks[0], 3), "mean_neighbors_last": round(ks[-1], 3),
+                "max_neighbors": kmax, "neighbor_truncations": trunc,
+                "timing": "each step synchronized and its diagnostics read"}
+        print(f"bench: later window {late}", file=sys.stderr, flush=True)
+
     # per-kernel HIP-event timing on the context stream over the same steps: the scene is uploaded
     # again and the warmup and timed steps are replayed (the solver is deterministic, so they visit
     # the same states) with an event pair around every pass
@@ -344,9 +388,13 @@ def main():
                              "frac": sub_ach / HBM_PEAK_GBS, "bytes_per_particle_substep": bytes_pp["substep"],
                              "ms_per_substep": sub_ms},
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "window": f"steps {args.warmup + 1}-{args.warmup + args.steps} from the seeded state",
+        "later_window": late,
     }
     if not args.no_batch:
         out["batch"] = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps, tune=not args.no_tune)
+    if not args.no_serve:
+        out["serve"] = serve_bench(args.serve_sessions, (args.serve_rate,), args.serve_duration, args.serve_shards, 200)
     if not args.no_cpu:
         print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

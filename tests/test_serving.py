@@ -59,3 +59,64 @@ def test_served_sessions_equal_scenes_alone(solver):
         pos, _, vel = solver.readback()
         np.testing.assert_array_equal(states[k][0], pos, err_msg=f"session {k} ({served[k]} steps)")
         np.testing.assert_array_equal(states[k][2], vel, err_msg=f"session {k}")
+
+
+def test_admission_queue_and_close_without_gpu():
+    """The admission bookkeeping of a server: greedy admission in session order, the deferred
+    queue, closes of deferred sessions (CPU only: no shard is built)."""
+    ses = small_sessions()
+    b = [serving.estimate_bytes(sc) for sc in ses]
+    adm, dfr, used = serving.admit(ses, sum(b[:6]))
+    assert adm == [0, 1, 2, 3, 4, 5] and dfr == [6, 7] and used == sum(b[:6])
+
+
+@pytest.mark.gpu
+def test_deferred_sessions_admitted_as_memory_frees(solver):
+    """Sessions beyond the budget wait; closing every session of one shard drops that shard,
+    returns its bytes and admits the waiting sessions into a new shard (the other shards stay as
+    they are); the waiting sessions' requests are then served, and every open session ends where
+    stepping it alone as often as it was served ends."""
+    ses = small_sessions()
+    p = cnhip.default_params(dt=1 / 120, substeps=4, default_iterations=4, max_fluid_neighbors=64)
+    b = [serving.estimate_bytes(sc) for sc in ses]
+    srv = serving.Server(ses, p, budget_bytes=sum(b[:6]), shards=3, graphs=True)
+    try:
+        assert list(srv.deferred) == [6, 7] and srv.admitted == [0, 1, 2, 3, 4, 5]
+        first = srv.shards[0]
+        gone = list(first.sessions)
+        others = [id(sh.solver) for sh in srv.shards[1:]]
+        arr = serving.poisson_arrivals(600.0, 0.15, range(len(ses)), seed=9)
+        done = srv.serve(arr, events=[(0.05, lambda s: [s.close_session(k) for k in gone])])
+        assert srv.shards[0] is None and 6 in srv.where
+        assert [id(sh.solver) for sh in srv.shards[1:3]] == others  # untouched
+        assert srv.bytes_used <= srv.budget
+        served = np.bincount([r.session for r in done], minlength=len(ses))
+        assert served[6] > 0
+        for r in done:
+            assert r.t_done >= r.t_arrival
+        open_ = [k for k in range(len(ses)) if k in srv.where]
+        states = {k: srv.session_state(k) for k in open_}
+        # compact: close one of two sessions in a shard, rebuild it from the survivor's state
+        two = next(sh for sh in srv.shards if sh is not None and len(sh.open) >= 2)
+        k_close, k_keep = sorted(two.open)[:2]
+        srv.close_session(k_close)
+        assert srv.compact(min_closed=0.3) >= 1
+        more = serving.poisson_arrivals(300.0, 0.05, [k_keep], seed=11)
+        done2 = srv.serve(more)
+        # every new request, plus those a session admitted by the compaction kept waiting
+        assert sum(r.session == k_keep for r in done2) == len(more)
+        for r in done2:
+            assert r.t_done >= r.t_arrival
+            served[r.session] += 1
+        open_ = [k for k in range(len(ses)) if k in srv.where]
+        states = {k: srv.session_state(k) for k in open_}
+    finally:
+        srv.close()
+    for k in open_:
+        solver.reserve(ses[k].n, 64)
+        solver.upload(ses[k])
+        for _ in range(served[k]):
+            solver.step(p)
+        pos, _, vel = solver.readback()
+        np.testing.assert_array_equal(states[k][0], pos, err_msg=f"session {k} ({served[k]} steps)")
+        np.testing.assert_array_equal(states[k][2], vel, err_msg=f"session {k}")
