@@ -7,7 +7,10 @@ cn_step (one frame = 4 substeps); the metric counts particle-substeps.
 
 Prints ONE JSON line with value (particle-substeps/s), roofline (dominant kernel, algorithmic
 bytes over its HIP-event-timed average launch), substep_roofline (whole substep against the HBM
-peak) and cpu_baseline (the CPU oracle on a 262,144-particle slice of the same workload).
+peak), later_window (steps 26-45 of the same run), configs (the 64k dam break and the mixed scene,
+GPU and CPU oracle), batch (512 scenes as one pool against looping the plain path), serve (2,048
+sessions under Poisson requests) and cpu_baseline (the CPU oracle on a 262,144-particle slice of
+the 4M workload).
 """
 import argparse
 import json
@@ -58,6 +61,53 @@ def cpu_baseline(n_sample: int, seed: int = 2, min_seconds: float = 10.0, max_st
             "kind": "port",
             "sample": f"{steps} steps (4 substeps x 4 iterations each) of a {n_sample}-particle random fill at the "
                       f"same density in a proportionally scaled box, CPU oracle with OpenMP, {el:.1f} s"}
+
+
+def config_legs(cpu: bool, min_cpu_seconds: float = 6.0):
+    """The two other single-scene workloads: the 64k dam break (40^3 lattice, r 0.03 m) and the mixed
+    scene (64^3 fluid block, 32^3 tet block, 256 x 256 cloth, particle-particle and particle-rigid
+    contacts), each stepped on the GPU (graph-captured, from its seeded state) and, unless cpu is
+    false, by the CPU oracle on the same scene (a bounded number of steps)."""
+    import cnhip
+    from cnhip import scenes
+    p = cnhip.default_params(dt=1 / 120, substeps=SUBSTEPS, default_iterations=ITERS, max_fluid_neighbors=CAP)
+    out = {}
+    for name, build, steps in (("dam_break_64k", scenes.dam_break_64k, 40), ("mixed_full", scenes.mixed, 20)):
+        sc = build()
+        s = cnhip.Solver(0)
+        s.reserve(sc.n, CAP)
+        s.upload(sc)
+        s.graphs(True)
+        for _ in range(3):
+            s.step(p)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step(p)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        m = s.meta()
+        s.close()
+        leg = {"particles": sc.n, "steps": f"4-{3 + steps}", "gpu_ms_per_step": el * 1e3 / steps,
+               "gpu_particle_substeps_per_s": sc.n * SUBSTEPS * steps / el,
+               "max_neighbors": m.max_fluid_neighbor_count, "neighbor_truncations": m.fluid_neighbor_truncations}
+        if cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import cn_oracle  # CPU oracle: the reported baseline only
+            osc = cn_oracle.OracleScene(sc)
+            n_cpu, el_cpu = 0, 0.0
+            while n_cpu < 10 and (n_cpu == 0 or el_cpu < min_cpu_seconds):
+                t = time.perf_counter()
+                cn_oracle.step(osc, p)
+                el_cpu += time.perf_counter() - t
+                n_cpu += 1
+            leg["cpu_baseline"] = {"value": sc.n * SUBSTEPS * n_cpu / el_cpu, "unit": "particle-substeps/s",
+                                   "cores": cn_oracle.threads(), "kind": "port",
+                                   "sample": f"steps 1-{n_cpu} of the same scene, CPU oracle with OpenMP, "
+                                             f"{el_cpu:.1f} s"}
+        print(f"bench: {name} {leg}", file=sys.stderr, flush=True)
+        out[name] = leg
+    return out
 
 
 def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool = True, shape=None):
@@ -243,6 +293,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="leave the served leg out of the default line")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered req/s of the default line's served leg")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="leave out the 64k dam-break and mixed-scene legs")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched workload (with --loop-steps 0: the pool alone, for profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no cn_autotune)")
@@ -391,6 +442,8 @@ def main():
         "window": f"steps {args.warmup + 1}-{args.warmup + args.steps} from the seeded state",
         "later_window": late,
     }
+    if not args.no_configs:
+        out["configs"] = config_legs(cpu=not args.no_cpu)
     if not args.no_batch:
         out["batch"] = batch_bench(args.batch_scenes, args.batch_steps, 2, args.loop_steps, tune=not args.no_tune)
     if not args.no_serve:
