@@ -302,6 +302,15 @@ class Solver:
         self._check(self.lib.cn_step_scenes(self._ctx, C.byref(params), len(ids), ids.ctypes.data if len(ids) else None),
                     "cn_step_scenes")
 
+    def tick_policy(self, compact_below: float):
+        """cn_tick_policy: a cn_step_scenes whose scenes hold at most this share of the pool's
+        particles steps their slots alone (0: always the whole pool)."""
+        self._check(self.lib.cn_tick_policy(self._ctx, float(compact_below)), "cn_tick_policy")
+
+    def concurrent_phases(self, enable: bool):
+        """cn_concurrent_phases: the soft phase on a second stream beside the fluid passes."""
+        self._check(self.lib.cn_concurrent_phases(self._ctx, 1 if enable else 0), "cn_concurrent_phases")
+
     def graphs(self, enable: bool):
         """Capture cn_step into a HIP graph and replay it (cn_graph_enable)."""
         self._check(self.lib.cn_graph_enable(self._ctx, 1 if enable else 0), "cn_graph_enable")

@@ -160,6 +160,8 @@ SIGNATURES = {
     "cn_batch_upload": (C.c_int, [_P, _U, C.POINTER(SceneDesc)]),
     "cn_batch_scene_range": (C.c_int, [_P, _U, C.POINTER(_U), C.POINTER(_U)]),
     "cn_graph_enable": (C.c_int, [_P, C.c_int]),
+    "cn_tick_policy": (C.c_int, [_P, C.c_float]),
+    "cn_concurrent_phases": (C.c_int, [_P, C.c_int]),
     "cn_autotune": (C.c_int, [_P, C.POINTER(StepParams), _U, C.POINTER(TuneResult)]),
     "cn_tune_set": (C.c_int, [_P, _U, _U]),
     "cn_step_scenes": (C.c_int, [_P, C.POINTER(StepParams), _U, _P]),

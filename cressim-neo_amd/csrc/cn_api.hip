@@ -61,6 +61,12 @@ void *noalloc_scratch(int device)
 struct cn_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the soft phase's stream while it runs beside the fluid passes (created on first use, default
+    // priority like the context stream), and the events that fork it from and join it to `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool split_phases = true;        // run the soft and fluid phases of a substep concurrently
+    bool side_failed = false;        // an event call of the split phases failed in the last step
     std::string err;
     SceneBuffers s;
     uint32_t capacity = 0;       // particle capacity of the allocation
@@ -75,7 +81,8 @@ struct cn_ctx {
     bool has_tension = false;        // any fluid material with surface tension
     bool has_solid = false;          // any solid particle
     bool any_pairs = false;          // some scene can hold a particle-particle contact pair
-    bool any_rigid = false;          // some scene can hold a particle-rigid contact
+    bool any_rigid = false;          // some scene solves particle-rigid contacts in slot order
+    bool any_rigid_pid = false;      // some scene solves them in particle-index order (soft phase)
     bool has_vorticity = false;      // a fluid particle's material confines vorticity
     bool aniso_enabled = false;      // cn_step ends with the render anisotropy
     bool aniso_ready = false;        // a step has written it since it was enabled or uploaded
@@ -109,6 +116,16 @@ struct cn_ctx {
     };
     std::vector<Graph> graphs;
     std::vector<uint32_t> active_host;  // last per-scene active flags written to the device
+    // compact ticks (cn_step_scenes over a small part of a pool)
+    std::vector<uint32_t> slot_base;    // per scene: first slot of its segment in the pool's slot order
+    bool slots_by_prefix = false;       // the slot order holds every scene as one segment (a step ran)
+    float compact_below = 0.5f;         // compact a tick whose scenes hold at most this share of the pool
+    SceneBuffers tick;                  // the compact tick's per-slot arrays (the rest are the pool's)
+    uint32_t tick_cap = 0;              // slots they hold
+    uint32_t tick_list_cap = 0;         // neighbor slots per particle of tick.list
+    uint4 *tick_seg = nullptr;          // per listed scene: pool slot start, compact start, count
+    uint32_t tick_seg_cap = 0;
+    std::vector<uint4> tick_seg_host;   // its host copy (pageable: staged when the copy is enqueued)
     float4 *stage = nullptr;            // host-provided particle state on its way to the device
     size_t stage_cap = 0;
     // profiling
@@ -567,7 +584,7 @@ uint32_t key_bits_of(const cn_ctx *ctx, const cn_step_params *p)
     return p->key_mode == CN_KEY_HASH ? 32u : 3u * ctx->morton_bits + ctx->scene_bits;
 }
 
-SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
+SubstepConsts make_consts(const cn_ctx *ctx, const SceneBuffers &s, const cn_step_params *p, float dt)
 {
     SubstepConsts c{};
     c.dt = dt;
@@ -576,17 +593,18 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
     c.gravity[0] = p->gravity[0];
     c.gravity[1] = p->gravity[1];
     c.gravity[2] = p->gravity[2];
-    c.n = ctx->s.n;
+    c.n = s.n;
+    c.npid = s.npid;
     c.cap = p->max_fluid_neighbors;
     c.key_mode = p->key_mode;
     c.morton_bits = ctx->morton_bits;
-    c.bucket_shift = key_bits_of(ctx, p) - ctx->s.bucket_bits;
-    c.uniform_class = ctx->s.nclasses == 1 ? 1u : 0u;
+    c.bucket_shift = key_bits_of(ctx, p) - s.bucket_bits;
+    c.uniform_class = s.nclasses == 1 ? 1u : 0u;
     c.tension = ctx->has_tension ? 1u : 0u;
     c.tile_max = ctx->tile_max;
-    c.blk_cap = ctx->s.blk_cap;
+    c.blk_cap = s.blk_cap;
     c.grid_max = ctx->grid_max;
-    c.superset = (p->flags & CN_STEP_CANDIDATE_REUSE) && ctx->s.slist ? 1u : 0u;
+    c.superset = (p->flags & CN_STEP_CANDIDATE_REUSE) && s.slist ? 1u : 0u;
     c.fluid_iters = p->fluid_iterations > 0 ? p->fluid_iterations : std::max<uint32_t>(p->default_iterations, 1u);
     c.fluid_scenes = ctx->fluid_scenes;
     fluid_tiles(c);
@@ -598,9 +616,8 @@ SubstepConsts make_consts(const cn_ctx *ctx, const cn_step_params *p, float dt)
 
 
 // predict + keys + sort + runs + reorder for one substep; returns the sorted (key, old slot) pairs
-uint2 *broadphase(cn_ctx *ctx, const SubstepConsts &c, uint32_t key_bits)
+uint2 *broadphase(cn_ctx *ctx, SceneBuffers &s, const SubstepConsts &c, uint32_t key_bits)
 {
-    SceneBuffers &s = ctx->s;
     {
         ProfScope ps(ctx, "keys");
         launch_keys(s, c, ctx->stream);
@@ -661,9 +678,15 @@ void cn_ctx_destroy(cn_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     clear_graphs(ctx);
     free_scene(ctx->s);
+    free_scene(ctx->tick);
+    dfree(ctx->tick_seg);
     dfree(ctx->stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -689,7 +712,7 @@ void cn_default_step_params(cn_step_params *p)
     std::memcpy(p->solver_config, cfg, sizeof(cfg));
     p->max_fluid_neighbors = 128;  // kDefaultFluidMaxNeighborhood, src/physics/rigid_body_common.h:304
     p->key_mode = CN_KEY_MORTON;
-    p->flags = CN_STEP_CANDIDATE_REUSE;  // same results, fewer neighbor walks
+    p->flags = 0u;  // candidate reuse measured no faster (see CN_STEP_CANDIDATE_REUSE)
 }
 
 // toFluidSolverMaterial and its helpers, src/physics/physics_world.cpp:99-263
@@ -779,6 +802,8 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     CN_HIP(ctx, hipSetDevice(ctx->device));
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_scene(ctx->s);
+    free_scene(ctx->tick);
+    ctx->tick_cap = 0;
     ctx->uploaded = false;
     ctx->graph_valid = false;
     SceneBuffers &s = ctx->s;
@@ -926,7 +951,9 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, hipSetDevice(ctx->device));
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s.n = n;
+    s.npid = n;
     s.cur = 0;
+    ctx->slots_by_prefix = false;
 
     // per-scene tables, concatenated
     std::vector<float> pos(4ull * n), vel(4ull * n);
@@ -960,6 +987,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     ctx->has_solid = false;
     ctx->any_pairs = false;
     ctx->any_rigid = false;
+    ctx->any_rigid_pid = false;
     ctx->has_vorticity = false;
     ctx->aniso_ready = false;
     ctx->fm_fluid_used.clear();
@@ -1052,7 +1080,26 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         // hasSoftRigidContactWork, physics_solver.cpp:486-488, for a scene whose solid particles
         // can meet a static collider (disabled colliders count: an edit may enable them)
         sinfo[k].rigid = scene_solid && tb->collider_count + tb->shape_collider_count > 0 ? 1u : 0u;
+        // a scene without particle-particle pairs whose every particle is constrained solves its
+        // particle-rigid contacts in particle-index order, in the soft phase
+        if (sinfo[k].rigid && !sinfo[k].pairs && cv) {
+            std::vector<uint8_t> held(m, 0u);
+            for (uint32_t e = 0; e < cv->edge_count; ++e) held[cv->edges[e].a] = held[cv->edges[e].b] = 1u;
+            for (uint32_t b = 0; b < cv->bend_count; ++b)
+                held[cv->bends[b].p0] = held[cv->bends[b].p1] = held[cv->bends[b].p2] = 1u;
+            for (uint32_t d = 0; d < cv->dihedral_count; ++d) {
+                const cn_cloth_dihedral &D = cv->dihedrals[d];
+                held[D.e0] = held[D.e1] = held[D.o0] = held[D.o1] = 1u;
+            }
+            for (uint32_t t = 0; t < cv->tet_count; ++t)
+                for (int q = 0; q < 4; ++q) held[cv->tets[t].p[q]] = 1u;
+            if (std::all_of(held.begin(), held.end(), [](uint8_t v) { return v != 0u; })) {
+                sinfo[k].rigid = 0u;
+                sinfo[k].rigid_pid = 1u;
+            }
+        }
         ctx->any_rigid |= sinfo[k].rigid != 0u;
+        ctx->any_rigid_pid |= sinfo[k].rigid_pid != 0u;
         col.insert(col.end(), sc.begin(), sc.end());
         // slots for the disabled colliders too, so cn_scene_set_colliders can enable them in place
         col.resize(col.size() + (tb->collider_count + tb->shape_collider_count - sc.size()));
@@ -1124,9 +1171,19 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     for (uint32_t k = 0; k < count; ++k) ctx->fluid_scenes += sinfo[k].has_fluid;
     {
         uint32_t rf = 0, rs = ctx->fluid_scenes;
+        std::vector<uint32_t> by_rank(count);
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t rank = sinfo[k].has_fluid ? rf++ : rs++;
             sinfo[k].key_prefix = ctx->scene_bits ? rank << (3u * ctx->morton_bits) : 0u;
+            by_rank[rank] = k;
+        }
+        // once sorted, the slots hold the scenes in rank order, each as one segment
+        ctx->slot_base.assign(count, 0u);
+        uint32_t first = 0;
+        for (uint32_t r = 0; r < count; ++r) {
+            const uint32_t k = by_rank[r];
+            ctx->slot_base[k] = first;
+            first += ctx->scene_first[k + 1] - ctx->scene_first[k];
         }
     }
     ctx->nscenes = count;
@@ -1369,15 +1426,15 @@ cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *firs
 
 namespace {
 
-// Enqueues one cn_step (every substep) on the context stream; advances the slot-order parity.
-void record_step(cn_ctx *ctx, const cn_step_params *p)
+// Enqueues one cn_step (every substep) on the context stream over the buffer set s (the pool's,
+// or a compact tick's view of the listed scenes); advances s's slot-order parity.
+void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
 {
-    SceneBuffers &s = ctx->s;
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
     const uint32_t def = std::max<uint32_t>(p->default_iterations, 1u);
     const uint32_t fluid_iters = resolve_iters(p->fluid_iterations, def);
     const float dt = p->dt / static_cast<float>(substeps);
-    const SubstepConsts c = make_consts(ctx, p, dt);
+    const SubstepConsts c = make_consts(ctx, s, p, dt);
     const uint32_t key_bits = key_bits_of(ctx, p);
     launch_fill(s.meta, kMetaWords, 0u, ctx->stream);
     const uint32_t soft_iters = resolve_iters(p->soft_internal_iterations, def);
@@ -1389,10 +1446,22 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
     // such scene of a pool: their zero corrections change nothing but the sign of zero coordinates
     const bool has_contacts =
         contact_iters > 0 && s.ct_alloc >= s.n && ((s.n > 1 && ctx->any_pairs) || ctx->any_rigid);
+    // particle-rigid contacts of the scenes that solve them in particle-index order, inside the soft
+    // phase; the constrained positions then stay in particle-index order from the substep's first
+    // soft pass to its last one, unless a slot-order contact pass moves particles in between
+    const bool rigid_pid = contact_iters > 0 && has_soft && ctx->any_rigid_pid && s.ct_alloc >= s.n;
+    const uint32_t pid_iters = std::max(soft_iters, rigid_pid ? contact_iters : 0u);
+    // The soft phase (with the particle-rigid contacts solved inside it) moves constrained particles
+    // only and the fluid passes fluid particles only; without a slot-order contact pass between
+    // them they are two independent chains within a substep, so the soft chain runs on a second
+    // stream beside the fluid passes (a bandwidth-bound chain next to a VALU-bound one) and joins
+    // the context stream before the velocity update.
+    const bool split = has_soft && ctx->has_fluid && !has_contacts && ctx->side && !ctx->profiling;
+    hipStream_t soft_st = split ? ctx->side : ctx->stream;
     SoftArgs sa{};
     if (has_soft) {
         sa.slot_of = s.slot_of;
-        sa.n = s.n;
+        sa.n = s.npid;
         sa.dt = dt;
         sa.relax = p->solver_config[kCfgSoftRelax];
         sa.edges = s.edges;
@@ -1422,11 +1491,14 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         sa.pid_scene = s.pid_scene;
     }
     for (uint32_t sub = 0; sub < substeps; ++sub) {
-        broadphase(ctx, c, key_bits);
+        broadphase(ctx, s, c, key_bits);
+        if (split && (hipEventRecord(ctx->fork_ev, ctx->stream) != hipSuccess ||
+                      hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0) != hipSuccess))
+            ctx->side_failed = true;
         if (has_soft) {
             sa.pos = s.pos[s.cur];
             sa.id = s.id[s.cur];
-            launch_soft_clear(sa, ctx->stream);
+            launch_soft_clear(sa, soft_st);
         }
         if (has_contacts) {
             ProfScope ps(ctx, "contact_partners");
@@ -1445,18 +1517,28 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
                 }
                 din = 1 - din;
             }
-            if (has_soft && it < soft_iters) {
-                // Without contact passes nothing but the soft solves moves a constrained particle
-                // within the substep (the fluid passes list fluid particles only), so the positions
-                // stay in particle-index order from the first soft iteration to the last.
+            if (has_soft && it < pid_iters) {
+                // Without slot-order contact passes nothing but the soft phase moves a constrained
+                // particle within the substep (the fluid passes list fluid particles only), so the
+                // positions stay in particle-index order from the first soft iteration to the last.
                 ProfScope ps(ctx, "soft");
-                launch_soft_iteration(sa, it == 0 || has_contacts, it + 1 == soft_iters || has_contacts, ctx->stream);
+                const bool gather = it == 0 || has_contacts, scatter = it + 1 == pid_iters || has_contacts;
+                if (rigid_pid && it < contact_iters) {
+                    launch_soft_iteration(sa, gather, false, it < soft_iters, soft_st);
+                    launch_rigid_pid(s, c, soft_st);
+                    if (scatter) launch_soft_iteration(sa, false, true, false, soft_st);
+                } else {
+                    launch_soft_iteration(sa, gather, scatter, it < soft_iters, soft_st);
+                }
             }
             if (has_contacts && it < contact_iters) {
                 ProfScope ps(ctx, "contacts");
                 launch_contact_iteration(s, c, din, ctx->stream);
             }
         }
+        if (split && (hipEventRecord(ctx->join_ev, ctx->side) != hipSuccess ||
+                      hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0) != hipSuccess))
+            ctx->side_failed = true;
         {
             ProfScope ps(ctx, "finalize");
             launch_finalize(s, c, din, ctx->stream);
@@ -1477,7 +1559,7 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
             for (uint32_t v = 0; v < contact_iters; ++v) launch_contact_velocity(s, c, ctx->stream);
         }
         // physics_solver.cpp:1301-1309, after the particle-particle velocity iterations
-        if (has_contacts && ctx->any_rigid) {
+        if ((has_contacts && ctx->any_rigid) || rigid_pid) {
             ProfScope ps(ctx, "rigid_velocity");
             launch_rigid_velocity(s, c, contact_iters, ctx->stream);
         }
@@ -1486,6 +1568,186 @@ void record_step(cn_ctx *ctx, const cn_step_params *p)
         ProfScope ps(ctx, "export");
         launch_export(s, true, false, ctx->stream);
     }
+}
+
+// The soft phase's stream and the fork / join events, on first use.
+cn_status ensure_side(cn_ctx *ctx)
+{
+    if (!ctx->split_phases || ctx->side) return CN_OK;
+    CN_HIP(ctx, hipStreamCreate(&ctx->side));
+    CN_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    CN_HIP(ctx, hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    ctx->graph_valid = false;
+    return CN_OK;
+}
+
+// The per-slot arrays of a compact tick for m slots (grown to at least twice the last size, and
+// to a sixteenth of the pool, so a serving loop settles on one allocation): the arrays the step
+// reads and writes in slot order.  The tables, constraints and their state, the particle-index
+// arrays, the diagnostics and the iteration state are the pool's.
+cn_status ensure_tick(cn_ctx *ctx, uint32_t m)
+{
+    const SceneBuffers &s = ctx->s;
+    SceneBuffers &t = ctx->tick;
+    const bool want_ct = s.ct_alloc > 0, want_sl = s.slist != nullptr;
+    if (t.pos[0] && m <= ctx->tick_cap && ctx->tick_list_cap == ctx->cap_alloc && (t.ct_alloc > 0) == want_ct &&
+        (t.slist != nullptr) == want_sl)
+        return CN_OK;
+    CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_scene(t);
+    ctx->tick_cap = 0;
+    const size_t N = std::min<size_t>(std::max<size_t>({(size_t)m, 2 * (size_t)ctx->tick_cap, s.n / 16, 1}), s.n);
+    for (int k = 0; k < 2; ++k) {
+        CN_HIP(ctx, dalloc(&t.pos[k], N));
+        CN_HIP(ctx, dalloc(&t.vel[k], N));
+        CN_HIP(ctx, dalloc(&t.cls[k], N));
+        CN_HIP(ctx, dalloc(&t.id[k], N));
+        CN_HIP(ctx, dalloc(&t.kv[k], N));
+        CN_HIP(ctx, dalloc(&t.dacc[k], N));
+        CN_HIP(ctx, dalloc(&t.prev[k], N));
+        CN_HIP(ctx, hipMemsetAsync(t.dacc[k], 0, N * sizeof(float4), ctx->stream));
+    }
+    CN_HIP(ctx, dalloc(&t.cellc, N));
+    CN_HIP(ctx, dalloc(&t.bnd, N));
+    CN_HIP(ctx, dalloc(&t.sort_scratch, sort_scratch_words((uint32_t)N)));
+    CN_HIP(ctx, dalloc(&t.elem_run, N));
+    CN_HIP(ctx, dalloc(&t.run_key, N));
+    CN_HIP(ctx, dalloc(&t.run_start, N + 1));
+    CN_HIP(ctx, dalloc(&t.run_info, N));
+    t.bucket_bits = std::min<uint32_t>(std::max<uint32_t>(4u, ceil_log2(N) > 1 ? ceil_log2(N) - 1 : 1), 30u);
+    CN_HIP(ctx, dalloc(&t.bucket, (size_t(1) << t.bucket_bits) + 1));
+    CN_HIP(ctx, dalloc(&t.block_runs, (N + 1023) / 1024 + 1));
+    CN_HIP(ctx, dalloc(&t.run_count, 1));
+    CN_HIP(ctx, dalloc(&t.perm, N));
+    CN_HIP(ctx, dalloc(&t.nc, N));
+    CN_HIP(ctx, dalloc(&t.curpos, N));
+    CN_HIP(ctx, dalloc(&t.cs, N));
+    CN_HIP(ctx, dalloc(&t.blk_first_run, N + 1));
+    CN_HIP(ctx, dalloc(&t.nblk, 2));
+    t.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
+    CN_HIP(ctx, dalloc(&t.blk_halo, (size_t)t.blk_cap * kHaloStride));
+    CN_HIP(ctx, dalloc(&t.wide_blk, t.blk_cap));
+    CN_HIP(ctx, dalloc(&t.fb_blk, t.blk_cap));
+    CN_HIP(ctx, dalloc(&t.list, list_slots(N) * std::max<uint32_t>(ctx->cap_alloc, 1u)));
+    CN_HIP(ctx, dalloc(&t.lcnt, N));
+    if (want_sl) {
+        CN_HIP(ctx, dalloc(&t.slist, list_slots(N) * kSuperSlots));
+        CN_HIP(ctx, dalloc(&t.scnt, N));
+        CN_HIP(ctx, dalloc(&t.disp, N));
+    }
+    if (want_ct) {
+        CN_HIP(ctx, dalloc(&t.ct_list, N * kContactPartners));
+        CN_HIP(ctx, dalloc(&t.ct_rec, N * kContactPartners));
+        CN_HIP(ctx, dalloc(&t.ct_count, N));
+        CN_HIP(ctx, dalloc(&t.ct_bits, N));
+        CN_HIP(ctx, dalloc(&t.ct_corr, N));
+        CN_HIP(ctx, dalloc(&t.ct_gen, N));
+        t.ct_alloc = N;
+    }
+    ctx->tick_cap = (uint32_t)N;
+    ctx->tick_list_cap = ctx->cap_alloc;
+    return CN_OK;
+}
+
+// The buffer set of a compact tick of m slots: the pool's, with every per-slot array replaced by
+// the tick's.
+SceneBuffers tick_view(const cn_ctx *ctx, uint32_t m, const cn_step_params *p)
+{
+    SceneBuffers v = ctx->s;
+    const SceneBuffers &t = ctx->tick;
+    v.n = m;
+    for (int k = 0; k < 2; ++k) {
+        v.pos[k] = t.pos[k];
+        v.vel[k] = t.vel[k];
+        v.cls[k] = t.cls[k];
+        v.id[k] = t.id[k];
+        v.kv[k] = t.kv[k];
+        v.dacc[k] = t.dacc[k];
+        v.prev[k] = t.prev[k];
+    }
+    v.cur = 0;
+    v.cellc = t.cellc;
+    v.bnd = t.bnd;
+    v.sort_scratch = t.sort_scratch;
+    v.elem_run = t.elem_run;
+    v.run_key = t.run_key;
+    v.run_start = t.run_start;
+    v.run_info = t.run_info;
+    v.bucket = t.bucket;
+    const uint32_t bb = std::max<uint32_t>(4u, ceil_log2(m) > 1 ? ceil_log2(m) - 1 : 1);
+    v.bucket_bits = std::min<uint32_t>(std::min<uint32_t>(bb, key_bits_of(ctx, p)), t.bucket_bits);
+    v.block_runs = t.block_runs;
+    v.run_count = t.run_count;
+    v.perm = t.perm;
+    v.nc = t.nc;
+    v.curpos = t.curpos;
+    v.cs = t.cs;
+    v.blk_first_run = t.blk_first_run;
+    v.nblk = t.nblk;
+    v.blk_halo = t.blk_halo;
+    v.blk_cap = t.blk_cap;
+    v.wide_blk = t.wide_blk;
+    v.fb_blk = t.fb_blk;
+    v.list = t.list;
+    v.lcnt = t.lcnt;
+    v.slist = t.slist;
+    v.scnt = t.scnt;
+    v.disp = t.disp;
+    v.nbr = nullptr;
+    v.nbr_cnt = nullptr;
+    v.nbr_alloc = 0;
+    v.ct_list = t.ct_list;
+    v.ct_count = t.ct_count;
+    v.ct_rec = t.ct_rec;
+    v.ct_bits = t.ct_bits;
+    v.ct_corr = t.ct_corr;
+    v.ct_gen = t.ct_gen;
+    v.ct_alloc = t.ct_alloc;
+    return v;
+}
+
+// One step of the listed scenes (flags) on a compact slot range: their segments of the pool's slot
+// order gathered, stepped as a pool of those scenes alone, and put back; m = their particles.  The
+// scenes' results are those of the full-pool step (a pool steps each scene as it steps alone).
+cn_status compact_step(cn_ctx *ctx, const cn_step_params *p, const std::vector<uint32_t> &flags, uint32_t m)
+{
+    std::vector<uint32_t> act;
+    for (uint32_t k = 0; k < ctx->nscenes; ++k)
+        if (flags[k]) act.push_back(k);
+    std::sort(act.begin(), act.end(), [&](uint32_t a, uint32_t b) { return ctx->slot_base[a] < ctx->slot_base[b]; });
+    std::vector<uint4> &seg = ctx->tick_seg_host;
+    seg.resize(act.size());
+    uint32_t at = 0;
+    for (size_t q = 0; q < act.size(); ++q) {
+        const uint32_t k = act[q], cnt = ctx->scene_first[k + 1] - ctx->scene_first[k];
+        if ((uint64_t)ctx->slot_base[k] + cnt > ctx->s.n) return fail(ctx, CN_ERR_DEVICE, "scene segment out of range");
+        seg[q] = make_uint4(ctx->slot_base[k], at, cnt, 0u);
+        at += cnt;
+    }
+    if (at != m || m == 0) return fail(ctx, CN_ERR_DEVICE, "compact tick: segment sizes disagree");
+    const cn_status st = ensure_tick(ctx, m);
+    if (st != CN_OK) return st;
+    if (ctx->tick_seg_cap < seg.size()) {
+        CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        dfree(ctx->tick_seg);
+        ctx->tick_seg = nullptr;
+        ctx->tick_seg_cap = 0;
+        CN_HIP(ctx, dalloc(&ctx->tick_seg, std::max<size_t>(seg.size(), ctx->nscenes)));
+        ctx->tick_seg_cap = std::max<uint32_t>((uint32_t)seg.size(), ctx->nscenes);
+    }
+    CN_HIP(ctx, hipMemcpyAsync(ctx->tick_seg, seg.data(), seg.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                               ctx->stream));
+    SceneBuffers v = tick_view(ctx, m, p);
+    launch_tick_gather(ctx->s, v, ctx->tick_seg, (uint32_t)seg.size(), ctx->stream);
+    record_step(ctx, v, p);
+    launch_tick_scatter(ctx->s, v, ctx->tick_seg, (uint32_t)seg.size(), ctx->stream);
+    ctx->stepped = true;
+    if (ctx->side_failed) {
+        ctx->side_failed = false;
+        return fail(ctx, CN_ERR_DEVICE, "event record / wait between the context's streams failed");
+    }
+    CN_HIP(ctx, hipGetLastError());
+    return CN_OK;
 }
 
 }  // namespace
@@ -1557,9 +1819,26 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
         ctx->graph_valid = false;
     }
     if (ctx->aniso_enabled && ctx->has_fluid) ctx->aniso_ready = true;
+    {
+        const cn_status st = ensure_side(ctx);
+        if (st != CN_OK) return st;
+    }
+    // a step of a small part of a pool runs on the listed scenes' slots only
+    if (ctx->compact_below > 0.0f && ctx->nscenes > 1 && ctx->slots_by_prefix && p->key_mode == CN_KEY_MORTON) {
+        uint64_t m = 0;
+        for (uint32_t k = 0; k < ctx->nscenes; ++k)
+            if (ctx->active_host[k]) m += ctx->scene_first[k + 1] - ctx->scene_first[k];
+        if (m > 0 && (double)m <= (double)ctx->compact_below * ctx->s.n)
+            return compact_step(ctx, p, ctx->active_host, (uint32_t)m);
+    }
     if (!ctx->graphs_enabled || ctx->profiling) {
-        record_step(ctx, p);
+        record_step(ctx, ctx->s, p);
         ctx->stepped = true;
+        ctx->slots_by_prefix = true;
+        if (ctx->side_failed) {
+            ctx->side_failed = false;
+            return fail(ctx, CN_ERR_DEVICE, "event record / wait between the context's streams failed");
+        }
         CN_HIP(ctx, hipGetLastError());
         return CN_OK;
     }
@@ -1574,15 +1853,18 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
             CN_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
             ctx->s.cur = g.cur1;
             ctx->stepped = true;
+            ctx->slots_by_prefix = true;
             return CN_OK;
         }
     cn_ctx::Graph g{};
     g.params = *p;
     g.cur0 = cur0;
     CN_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    record_step(ctx, p);
+    record_step(ctx, ctx->s, p);
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &g.graph);
-    if (ce != hipSuccess) {
+    if (ce != hipSuccess || ctx->side_failed) {
+        ctx->side_failed = false;
+        if (ce == hipSuccess) (void)hipGraphDestroy(g.graph);
         ctx->s.cur = cur0;
         return fail(ctx, CN_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
     }
@@ -1597,6 +1879,33 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
     ctx->graphs.push_back(g);
     CN_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
     ctx->stepped = true;
+    ctx->slots_by_prefix = true;
+    return CN_OK;
+}
+
+cn_status cn_tick_policy(cn_ctx *ctx, float compact_below)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if (!(compact_below >= 0.0f && compact_below <= 1.0f))
+        return fail(ctx, CN_ERR_INVALID_ARGUMENT, "compact_below outside [0, 1]");
+    ctx->compact_below = compact_below;
+    return CN_OK;
+}
+
+cn_status cn_concurrent_phases(cn_ctx *ctx, int enable)
+{
+    if (!ctx) return CN_ERR_INVALID_ARGUMENT;
+    if ((enable != 0) != ctx->split_phases) ctx->graph_valid = false;
+    ctx->split_phases = enable != 0;
+    if (!ctx->split_phases && ctx->side) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipEventDestroy(ctx->fork_ev);
+        (void)hipEventDestroy(ctx->join_ev);
+        (void)hipStreamDestroy(ctx->side);
+        ctx->side = nullptr;
+        ctx->fork_ev = ctx->join_ev = nullptr;
+    }
     return CN_OK;
 }
 
@@ -1636,6 +1945,8 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const cn_status st0 = set_active(ctx, std::vector<uint32_t>(ctx->nscenes, 1u));
     if (st0 != CN_OK) return st0;
+    const cn_status st1 = ensure_side(ctx);
+    if (st1 != CN_OK) return st1;
     SceneBuffers &s = ctx->s;
     const size_t n = s.n;
     // saved state: slot-order particles, previous positions, exports and diagnostics
@@ -1651,7 +1962,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
         dfree(sm);
     };
     const int cur0 = s.cur;
-    const bool stepped0 = ctx->stepped;
+    const bool stepped0 = ctx->stepped, prefix0 = ctx->slots_by_prefix;
     const uint32_t tile0 = ctx->tile_max, grid0 = ctx->grid_max;
     if (dalloc(&sp, n) != hipSuccess || dalloc(&sv, n) != hipSuccess || dalloc(&spr, n) != hipSuccess ||
         dalloc(&so, 3 * n) != hipSuccess || dalloc(&sc, n) != hipSuccess || dalloc(&si, n) != hipSuccess ||
@@ -1679,6 +1990,7 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
         const size_t f4 = n * sizeof(float4), u4 = n * sizeof(uint32_t);
         s.cur = cur0;
         ctx->stepped = stepped0;
+        ctx->slots_by_prefix = prefix0;
         if (e == hipSuccess) e = copy(s.pos[cur0], sp, f4);
         if (e == hipSuccess) e = copy(s.vel[cur0], sv, f4);
         if (e == hipSuccess) e = copy(s.prev[cur0], spr, f4);
@@ -1702,10 +2014,10 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
             if (e != hipSuccess) break;
             ctx->tile_max = tile;
             ctx->grid_max = grid;
-            record_step(ctx, p);  // warm-up from the saved state
+            record_step(ctx, ctx->s, p);  // warm-up from the saved state
             restore();
             if (e == hipSuccess) e = hipEventRecord(ea, ctx->stream);
-            for (uint32_t k = 0; k < steps; ++k) record_step(ctx, p);
+            for (uint32_t k = 0; k < steps; ++k) record_step(ctx, ctx->s, p);
             if (e == hipSuccess) e = hipEventRecord(eb, ctx->stream);
             if (e == hipSuccess) e = hipEventSynchronize(eb);
             if (e == hipSuccess) e = hipGetLastError();
@@ -2111,7 +2423,7 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
     SceneBuffers &s = ctx->s;
     const uint32_t n = s.n;
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
-    const SubstepConsts c = make_consts(ctx, p, p->dt / static_cast<float>(substeps));
+    const SubstepConsts c = make_consts(ctx, s, p, p->dt / static_cast<float>(substeps));
     const size_t slots = (size_t)n * std::max<uint32_t>(c.cap, 1u);
     if (s.nbr_alloc < slots) {
         dfree(s.nbr);
@@ -2124,7 +2436,8 @@ cn_status cn_pass_substep_broadphase(cn_ctx *ctx, const cn_step_params *p, uint3
         s.nbr_alloc = slots;
     }
     const int old_cur = s.cur;
-    uint2 *sorted = broadphase(ctx, c, key_bits_of(ctx, p));
+    uint2 *sorted = broadphase(ctx, s, c, key_bits_of(ctx, p));
+    ctx->slots_by_prefix = true;
     launch_fluid_lists(s, c, ctx->stream);
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<uint2> kv(n);

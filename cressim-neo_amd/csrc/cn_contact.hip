@@ -202,6 +202,45 @@ __device__ __forceinline__ float3 rigid_material(const float4 pm, const float *c
                        fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, cm[3])));
 }
 
+// rigid solve_contacts :45-140 against a static body: the position correction of a particle at p
+// (inverse mass p.w; pv its position at the start of the substep) with contact rec = (normal,
+// penetration) against collider cc; false when the contact adds nothing.  The displacement of a
+// static body's contact point is +0, its effective mass 0.
+__device__ __forceinline__ bool rigid_position(const float4 p, const float4 pv, const float4 rec, const float4 pm,
+                                               const Collider &cc, const SubstepConsts &c, float3 &d)
+{
+    const float w = p.w;
+    if (w <= kEps) return false;
+    const float pn = fmind(fmaxd(rec.w - c.cfg[kCfgContactSlop], 0.0f), c.cfg[kCfgSoftMaxCorrection]);
+    if (pn <= 0.0f) return false;
+    const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
+    const float3 mat = rigid_material(pm, cc.material);
+    const float denom = w + 0.0f;
+    if (denom <= kEps) return false;
+    const float relax = c.cfg[kCfgSoftContactRelax];
+    const float lambda = fdiv(pn, denom) * relax;
+    float cx = nr.x * (w * lambda), cy = nr.y * (w * lambda), cz = nr.z * (w * lambda);
+    // relative displacement = (contact point - its previous position) - (x - x_prev), the first
+    // term +0 for a static body
+    const float rx = 0.0f - (p.x - pv.x), ry = 0.0f - (p.y - pv.y), rz = 0.0f - (p.z - pv.z);
+    const float rn = dot3(rx, ry, rz, nr.x, nr.y, nr.z);
+    const float3 fd = friction_delta(c.cfg, rx - nr.x * rn, ry - nr.y * rn, rz - nr.z * rn, pn, mat.x, mat.z);
+    const float fdist = len3(fd.x, fd.y, fd.z);
+    if (fdist > 0.0f) {
+        const float tden = w + 0.0f;
+        if (tden > kEps) {
+            const float tl = fdiv(fdist, tden);
+            const float k = (tl * w) * relax;
+            cx = cx + fdiv(fd.x, fdist) * k;
+            cy = cy + fdiv(fd.y, fdist) * k;
+            cz = cz + fdiv(fd.z, fdist) * k;
+        }
+    }
+    d = make_float3(cx, cy, cz);
+    return true;
+}
+
+
 // ---------------------------------------------------------------------------------------------
 // candidates
 // ---------------------------------------------------------------------------------------------
@@ -326,39 +365,15 @@ __global__ __launch_bounds__(kBlock) void k_contact_solve(ContactArgs a, Substep
         a.rgen[t] = p;
         const float4 pv = a.prev[t];
         const float4 pm = a.part_mats[pc.part_mat];
-        const float w = p.w;
         for (uint32_t m = rm; m; m &= m - 1u) {
             const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
             float4 rec;
+            float3 d;
             if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) continue;
-            if (w <= kEps) continue;
-            const float pn = fmind(fmaxd(rec.w - c.cfg[kCfgContactSlop], 0.0f), c.cfg[kCfgSoftMaxCorrection]);
-            if (pn <= 0.0f) continue;
-            const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
-            const float3 mat = rigid_material(pm, cc.material);
-            const float denom = w + 0.0f;
-            if (denom <= kEps) continue;
-            const float lambda = fdiv(pn, denom) * relax;
-            float cx = nr.x * (w * lambda), cy = nr.y * (w * lambda), cz = nr.z * (w * lambda);
-            // relative displacement = (contact point - its previous position) - (x - x_prev), the
-            // first term +0 for a static body
-            const float rx = 0.0f - (p.x - pv.x), ry = 0.0f - (p.y - pv.y), rz = 0.0f - (p.z - pv.z);
-            const float rn = dot3(rx, ry, rz, nr.x, nr.y, nr.z);
-            const float3 fd = friction_delta(c.cfg, rx - nr.x * rn, ry - nr.y * rn, rz - nr.z * rn, pn, mat.x, mat.z);
-            const float fdist = len3(fd.x, fd.y, fd.z);
-            if (fdist > 0.0f) {
-                const float tden = w + 0.0f;
-                if (tden > kEps) {
-                    const float tl = fdiv(fdist, tden);
-                    const float k = (tl * w) * relax;
-                    cx = cx + fdiv(fd.x, fdist) * k;
-                    cy = cy + fdiv(fd.y, fdist) * k;
-                    cz = cz + fdiv(fd.z, fdist) * k;
-                }
-            }
-            sx = sx + cx;
-            sy = sy + cy;
-            sz = sz + cz;
+            if (!rigid_position(p, pv, rec, pm, cc, c, d)) continue;
+            sx = sx + d.x;
+            sy = sy + d.y;
+            sz = sz + d.z;
         }
     }
     a.corr[t] = make_float4(sx, sy, sz, 0.0f);
@@ -461,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_rigid_velocity(ContactArgs a, Subste
     const ParticleClass &pc = pclass(a, t);
     if (!a.active[pc.scene]) return;
     const SceneInfo sc = a.scenes[pc.scene];
-    if (!sc.rigid) return;
+    if (!(sc.rigid | sc.rigid_pid)) return;
     const float w = a.pos[t].w;
     if (w <= kEps) return;
     const uint32_t rm = pc.kind == CN_PARTICLE_KIND_SOLID ? a.bnd[t] : 0u;
@@ -494,6 +509,66 @@ __global__ __launch_bounds__(kBlock) void k_rigid_velocity(ContactArgs a, Subste
         v.z = v.z + sz;
     }
     a.vel[t] = v;
+}
+
+// The particle-rigid contacts of a scene whose every particle is constrained and which holds no
+// particle-particle pair, in particle-index order on the soft phase's positions right after its
+// last apply of the iteration (the reference's contact pass of the iteration, for such a scene):
+// each candidate's contact at the current position, the static-body solve, their sum (zero without
+// contacts) added to a free particle.  The scene then needs no slot-order contact pass, and its
+// constrained positions stay in particle-index order across the iteration.
+struct RigidPidArgs {
+    float4 *posi;
+    const uint32_t *soft_ids;
+    uint32_t nsoft;
+    const uint32_t *slot_of;
+    const uint32_t *pid_scene;
+    const uint32_t *active;
+    const SceneInfo *scenes;
+    const Collider *col;
+    const uint32_t *bnd;      // slot order
+    const float4 *prev;       // slot order
+    const uint32_t *cls;      // slot order
+    const ParticleClass *classes;
+    const float4 *part_mats;
+    float4 *rgen;             // slot order
+    uint32_t npid;
+};
+
+__global__ __launch_bounds__(kBlock) void k_rigid_pid(RigidPidArgs a, SubstepConsts c)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= a.nsoft) return;
+    const uint32_t pid = a.soft_ids[j];
+    const uint32_t scene = a.pid_scene[pid];
+    if (!a.active[scene]) return;
+    const SceneInfo &sc = a.scenes[scene];
+    if (!sc.rigid_pid) return;
+    float4 p = a.posi[pid];
+    if (p.w <= kEps) return;
+    const uint32_t t = a.slot_of[min(pid, a.npid - 1u)];
+    const ParticleClass &pc = a.classes[c.uniform_class ? 0u : a.cls[t]];
+    const uint32_t rm = pc.kind == CN_PARTICLE_KIND_SOLID ? a.bnd[t] : 0u;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    if (rm) {
+        a.rgen[t] = p;
+        const float4 pv = a.prev[t];
+        const float4 pm = a.part_mats[pc.part_mat];
+        for (uint32_t m = rm; m; m &= m - 1u) {
+            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+            float4 rec;
+            float3 d;
+            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) continue;
+            if (!rigid_position(p, pv, rec, pm, cc, c, d)) continue;
+            sx = sx + d.x;
+            sy = sy + d.y;
+            sz = sz + d.z;
+        }
+    }
+    p.x = p.x + sx;
+    p.y = p.y + sy;
+    p.z = p.z + sz;
+    a.posi[pid] = p;
 }
 
 static ContactArgs contact_args(const SceneBuffers &s, const SubstepConsts &c, int din)
@@ -549,6 +624,28 @@ void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipS
     const ContactArgs a = contact_args(s, c, 0);
     hipLaunchKernelGGL(k_contact_velocity, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a, c);
     hipLaunchKernelGGL(k_contact_velocity_apply, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a);
+}
+
+void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+{
+    if (!s.nsoft) return;
+    RigidPidArgs a;
+    a.posi = s.posi;
+    a.soft_ids = s.soft_ids;
+    a.nsoft = s.nsoft;
+    a.slot_of = s.slot_of;
+    a.pid_scene = s.pid_scene;
+    a.active = s.scene_active;
+    a.scenes = s.scenes;
+    a.col = s.colliders;
+    a.bnd = s.bnd;
+    a.prev = s.prev[s.cur];
+    a.cls = s.cls[s.cur];
+    a.classes = s.classes;
+    a.part_mats = s.part_mats;
+    a.rgen = s.ct_gen;
+    a.npid = s.npid;
+    hipLaunchKernelGGL(k_rigid_pid, dim3(grid_c(s.nsoft)), dim3(kBlock), 0, st, a, c);
 }
 
 void launch_rigid_velocity(const SceneBuffers &s, const SubstepConsts &c, uint32_t iters, hipStream_t st)

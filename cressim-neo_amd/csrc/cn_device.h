@@ -59,7 +59,9 @@ struct SceneInfo {
                           // fluid one, solid particles of two phase groups, or a self-colliding
                           // phase group)
     uint32_t rigid;       // 1: particle-rigid contacts can exist (a solid particle and a static
-                          // collider slot)
+                          // collider slot), solved in the slot-order contact pass
+    uint32_t rigid_pid;   // 1: the same, for a scene without particle-particle pairs whose every
+                          // particle is constrained: solved in particle-index order in the soft phase
 };
 
 // A static collider in LBVH visit order, its world pose resolved on the host once (the body pose
@@ -85,7 +87,8 @@ struct SubstepConsts {
     float dt;
     float inv_cell;       // > 0: 1 / max(cell, 1e-4) for every scene; 0: each scene's own
     float gravity[3];
-    uint32_t n;
+    uint32_t n;           // slots stepped (the pool, or the listed scenes of a compact tick)
+    uint32_t npid;        // particles of the pool: the bound of particle-index arrays
     uint32_t cap;         // neighbor slots per particle
     uint32_t key_mode;
     uint32_t morton_bits; // bits per axis of the Morton key

@@ -935,7 +935,8 @@ __device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32
 // neighbors written to the list in candidate order (= walk order), and the density terms of the
 // first cap summed in the same pass.  A step's four candidates are loaded first; each term is then
 // evaluated before the test decides whether it is added, so every lane runs the same instructions.
-// Returns the untruncated neighbor count.
+// (Filtering first and summing over the written list afterwards measured slower.)  Returns the
+// untruncated neighbor count.
 __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t scount, const float4 *lpos,
                                                   const DensitySelf &x, const ListRef &li, uint32_t cap,
                                                   DensitySum &d)
@@ -1447,7 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_fluid_render_aniso(FluidArgs a, Subs
     const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
     if (!a.active[pc.scene]) return;
     const size_t pid = id[i];
-    const size_t n = c.n;
+    const size_t n = c.npid;
     const float radius = fmaxd(pc.radius, 1.0e-4f);
     float ax[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
     float sc[3] = {radius, radius, radius};

@@ -96,7 +96,8 @@ struct CorrRec {
 };
 
 struct SceneBuffers {
-    uint32_t n = 0;
+    uint32_t n = 0;                 // slots of the per-slot arrays in use
+    uint32_t npid = 0;              // particles of the pool (particle-index arrays hold npid entries)
     // particle state in slot order, double-buffered across the per-substep reorder
     float4 *pos[2] = {nullptr, nullptr};
     float4 *vel[2] = {nullptr, nullptr};
@@ -218,6 +219,12 @@ void launch_contact_partners(const SceneBuffers &s, const SubstepConsts &c, hipS
 void launch_contact_iteration(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_rigid_velocity(const SceneBuffers &s, const SubstepConsts &c, uint32_t iters, hipStream_t st);
+// the particle-rigid contacts of scenes solved in particle-index order, on the soft phase's positions
+void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+void launch_tick_gather(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
+                        hipStream_t st);
+void launch_tick_scatter(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
+                         hipStream_t st);
 
 // ---- XPBD soft constraints (cn_soft.hip) ----
 struct SoftArgs {
@@ -255,7 +262,8 @@ struct SoftArgs {
 void launch_soft_clear(const SoftArgs &a, hipStream_t st);
 // gather / scatter: copy the constrained particles' positions from slot order before the solves and
 // back after them (needed only around iterations where another pass moves those particles)
-void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, hipStream_t st);
+// solve = false: only the gather and / or the scatter
+void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, bool solve, hipStream_t st);
 
 // per-pass entries in the reference's data order (cn_passes.hip)
 void launch_ref_keys(const float4 *pos, uint32_t n, float inv_cell, uint32_t *keys, int32_t *cells,

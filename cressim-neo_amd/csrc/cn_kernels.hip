@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     cls_n[t] = k;
     const uint32_t pid = id_o[o];
     id_n[t] = pid;
-    if (slot_of) slot_of[min(pid, c.n - 1u)] = t;
+    if (slot_of) slot_of[min(pid, c.npid - 1u)] = t;
     const SceneInfo sc = scenes[pc.scene];
     const int3 g = cell_of(p.x, p.y, p.z, scene_inv_cell(sc, c));
     // w: the scene's key prefix, from which neighbor-cell keys are formed
@@ -527,12 +527,12 @@ void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hip
 // slot order -> particle-index order
 __global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                                                    const float4 *__restrict__ prev, const uint32_t *__restrict__ id,
-                                                   uint32_t n, float4 *__restrict__ opos, float4 *__restrict__ ovel,
-                                                   float4 *__restrict__ oprev)
+                                                   uint32_t n, uint32_t npid, float4 *__restrict__ opos,
+                                                   float4 *__restrict__ ovel, float4 *__restrict__ oprev)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    const uint32_t d = min(id[t], n - 1u);
+    const uint32_t d = min(id[t], npid - 1u);
     if (opos) opos[d] = pos[t];
     if (ovel) ovel[d] = vel[t];
     if (oprev) oprev[d] = prev[t];
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4 *__restrict__ po
 void launch_export(const SceneBuffers &s, bool state, bool prev, hipStream_t st)
 {
     hipLaunchKernelGGL(k_export, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev[s.cur],
-                       s.id[s.cur], s.n, state ? s.out_pos : nullptr, state ? s.out_vel : nullptr,
+                       s.id[s.cur], s.n, s.npid, state ? s.out_pos : nullptr, state ? s.out_vel : nullptr,
                        prev ? s.out_prev : nullptr);
 }
 
@@ -571,6 +571,81 @@ void launch_set_state(SceneBuffers &s, uint32_t first, uint32_t count, const flo
 {
     hipLaunchKernelGGL(k_set_state, dim3(grid1(s.n)), dim3(kBlock), 0, st, s.id[s.cur], s.n, first, count, P, V,
                        s.pos[s.cur], s.vel[s.cur], s.prev[s.cur], s.out_pos, s.out_vel, s.out_prev);
+}
+
+// ---------------------------------------------------------------------------------------------
+// compact ticks: the slots of the listed scenes of a pool, gathered into a compact slot range and
+// put back after the step.  A pool's slot order holds each scene as one segment (the scene's key
+// prefix orders the sort), so a listed scene is the segment seg.x .. seg.x + seg.z of the pool and
+// seg.y .. seg.y + seg.z of the compact range; segments are in slot order.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t tick_pool_slot(const uint4 *__restrict__ seg, uint32_t nseg, uint32_t j)
+{
+    uint32_t lo = 0, hi = nseg;  // the last segment whose compact start is <= j
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg[mid].y <= j) lo = mid;
+        else hi = mid;
+    }
+    return seg[lo].x + (j - seg[lo].y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tick_gather(const uint4 *__restrict__ seg, uint32_t nseg, uint32_t m,
+                                                        const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                        const float4 *__restrict__ prev, const uint32_t *__restrict__ cls,
+                                                        const uint32_t *__restrict__ id, float4 *__restrict__ tpos,
+                                                        float4 *__restrict__ tvel, float4 *__restrict__ tprev,
+                                                        uint32_t *__restrict__ tcls, uint32_t *__restrict__ tid)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t t = tick_pool_slot(seg, nseg, j);
+    tpos[j] = pos[t];
+    tvel[j] = vel[t];
+    tprev[j] = prev[t];
+    tcls[j] = cls[t];
+    tid[j] = id[t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_tick_scatter(const uint4 *__restrict__ seg, uint32_t nseg, uint32_t m,
+                                                         uint32_t npid, const float4 *__restrict__ tpos,
+                                                         const float4 *__restrict__ tvel,
+                                                         const float4 *__restrict__ tprev,
+                                                         const uint32_t *__restrict__ tcls,
+                                                         const uint32_t *__restrict__ tid, float4 *__restrict__ pos,
+                                                         float4 *__restrict__ vel, float4 *__restrict__ prev,
+                                                         uint32_t *__restrict__ cls, uint32_t *__restrict__ id,
+                                                         uint32_t *__restrict__ slot_of)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t t = tick_pool_slot(seg, nseg, j);
+    const uint32_t pid = tid[j];
+    pos[t] = tpos[j];
+    vel[t] = tvel[j];
+    prev[t] = tprev[j];
+    cls[t] = tcls[j];
+    id[t] = pid;
+    if (slot_of) slot_of[min(pid, npid - 1u)] = t;
+}
+
+void launch_tick_gather(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
+                        hipStream_t st)
+{
+    const int o = pool.cur, w = tick.cur;
+    hipLaunchKernelGGL(k_tick_gather, dim3(grid1(tick.n)), dim3(kBlock), 0, st, seg, nseg, tick.n, pool.pos[o],
+                       pool.vel[o], pool.prev[o], pool.cls[o], pool.id[o], tick.pos[w], tick.vel[w], tick.prev[w],
+                       tick.cls[w], tick.id[w]);
+}
+
+void launch_tick_scatter(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
+                         hipStream_t st)
+{
+    const int o = tick.cur, w = pool.cur;
+    hipLaunchKernelGGL(k_tick_scatter, dim3(grid1(tick.n)), dim3(kBlock), 0, st, seg, nseg, tick.n, pool.npid,
+                       tick.pos[o], tick.vel[o], tick.prev[o], tick.cls[o], tick.id[o], pool.pos[w], pool.vel[w],
+                       pool.prev[w], pool.cls[w], pool.id[w],
+                       pool.ne + pool.nb + pool.nd + pool.nt ? pool.slot_of : nullptr);
 }
 
 // word fill as a kernel (stays a kernel node when the step is captured into a graph)

@@ -289,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
+    if (!live(a, pid)) return;  // a scene left out of the step may not be in the slots stepped
     a.posi[pid] = a.pos[a.slot_of[pid]];
 }
 
@@ -307,10 +308,14 @@ void launch_soft_clear(const SoftArgs &a, hipStream_t st)
     if (m) hipLaunchKernelGGL(k_soft_clear, dim3(grid_n(m)), dim3(kBlock), 0, st, a);
 }
 
-void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, hipStream_t st)
+void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, bool solve, hipStream_t st)
 {
     // physics_solver.cpp:915-1019: edges solve/apply, bends + dihedrals solve/apply, tets solve/apply
     if (a.nsoft && gather) hipLaunchKernelGGL(k_soft_gather, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
+    if (!solve) {
+        if (a.nsoft && scatter) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
+        return;
+    }
     if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
                        a.corr_e);

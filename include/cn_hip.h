@@ -212,7 +212,7 @@ typedef struct cn_step_params {
     float cell_size;                 /* particle grid cell size; 0 -> derived like the reference */
     uint32_t max_fluid_neighbors;    /* neighbor slots per fluid particle (reference: 128) */
     uint32_t key_mode;               /* CN_KEY_MORTON or CN_KEY_HASH */
-    uint32_t flags;                  /* CN_STEP_* bits (cn_default_step_params: CN_STEP_CANDIDATE_REUSE) */
+    uint32_t flags;                  /* CN_STEP_* bits (cn_default_step_params: none) */
 } cn_step_params;
 
 /* flags: in fluid-only scenes the fluid iterations after a substep's first filter the candidate
@@ -221,7 +221,8 @@ typedef struct cn_step_params {
  * the largest one in its cell block's halo stays below 0.1 h; the others walk.  The lists are the
  * same either way (same neighbors, same order), so the results are too.  The candidate lists take
  * 198 bytes per particle, allocated on the first step that asks (left off when they would exceed
- * 4 GiB).  On in cn_default_step_params; clear the bit to walk every iteration. */
+ * 4 GiB).  Off in cn_default_step_params: it measured no faster on a 4M-particle fluid and slower
+ * on pools of small scenes, where many blocks fail the displacement check and are walked anyway. */
 #define CN_STEP_CANDIDATE_REUSE 1u
 
 /* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
@@ -308,6 +309,18 @@ cn_status cn_batch_upload(cn_ctx *ctx, uint32_t scene_count, const cn_scene_desc
  * untouched (serving: one launch per pass for the scenes that have a pending step request).  No
  * reference counterpart. */
 cn_status cn_step_scenes(cn_ctx *ctx, const cn_step_params *params, uint32_t count, const uint32_t *scenes);
+/* No reference counterpart.  When the scenes listed to cn_step_scenes hold at most compact_below
+ * of the pool's particles (default 0.5; 0 never), the step runs on their slots alone: their
+ * segments of the pool's slot order are gathered into a compact range, stepped as a pool of those
+ * scenes, and put back, so its cost follows the listed particles instead of the pool's.  Results
+ * are the same either way.  Such a step is not captured into a graph. */
+cn_status cn_tick_policy(cn_ctx *ctx, float compact_below);
+/* No reference counterpart.  enable != 0 (the default): in a substep with no particle-particle
+ * contact pass between them, the soft-constraint phase (and the particle-rigid contacts solved in
+ * it) runs on a second HIP stream of the context beside the fluid passes, joined before the
+ * velocity update; they move disjoint particles, so the results are the same either way.
+ * enable == 0 runs every pass on the context stream. */
+cn_status cn_concurrent_phases(cn_ctx *ctx, int enable);
 /* Pool index range of one uploaded scene. */
 cn_status cn_batch_scene_range(const cn_ctx *ctx, uint32_t scene, uint32_t *first, uint32_t *count);
 /* enable != 0: cn_step captures its kernel sequence into a HIP graph on the first call with given
