@@ -233,6 +233,42 @@ def shapes_scene(side: int = 10, seed: int = 0) -> Scene:
     return sc
 
 
+def pegboard_colliders(nx: int = 10, nz: int = 6, y: float = 0.25, spacing: float = 0.12,
+                       origin=(0.2, 0.2)) -> np.ndarray:
+    """A grid of nx x nz static pegs at height y, cycling sphere (r 0.03 m), capsule lying along x
+    (r 0.02 m, half height 0.03 m) and box turned about y (half extents 0.03 x 0.02 x 0.025 m)."""
+    from . import shape_collider
+    out = []
+    for i in range(nx):
+        for j in range(nz):
+            c = (origin[0] + i * spacing, y, origin[1] + j * spacing)
+            kind = (i + j) % 3
+            if kind == 0:
+                out.append(shape_collider(0, c, params=(0.03,)))
+            elif kind == 1:
+                h = np.float32(np.sqrt(0.5))
+                out.append(shape_collider(2, c, orientation=(0.0, 0.0, h, h), params=(0.02, 0.03)))
+            else:
+                a = 0.3 + 0.1 * i
+                out.append(shape_collider(1, c, orientation=(0.0, np.sin(a / 2), 0.0, np.cos(a / 2)),
+                                          params=(0.03, 0.02, 0.025)))
+    return np.concatenate(out)
+
+
+def pegboard(fluid_side: int = 10, cloth_n: int = 12, nx: int = 10, nz: int = 6) -> Scene:
+    """A fluid block and a cloth dropped onto a board of nx x nz pegs (more than 32 static colliders
+    with the container's six walls) in the 4 x 2 x 2 m container."""
+    spacing = 0.12
+    span = (nx - 1) * spacing
+    fluid = fluid_part(fluid_side, 0.04, 0.02, origin=(0.2, 0.4, 0.2), seed=3, jitter=0.004)
+    cloth = cloth_part(cloth_n, 0.04, 0.02, translation=(0.2 + 0.5 * span, 0.9, 0.2 + 0.3))
+    cols = container((0.0, 0.0, 0.0), (4.0, 2.0, 2.0))
+    sc = compose([cloth, fluid], fluid_material(0.02), cols, f"pegboard_{nx}x{nz}",
+                 dict(fluid=fluid_side ** 3, cloth=cloth_n ** 2, pegs=nx * nz))
+    sc.shape_colliders = pegboard_colliders(nx, nz, spacing=spacing)
+    return sc
+
+
 def folded_cloth(n: int = 16, spacing: float = 0.03, gap: float = 0.75, self_collide: bool = True) -> Scene:
     """A self-colliding cloth folded onto itself: an n x n grid (spacing 0.03 m, r = spacing / 2)
     authored flat, then the half beyond the middle column mirrored over it, gap x 2r above, so

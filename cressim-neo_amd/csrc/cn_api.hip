@@ -191,6 +191,8 @@ void free_scene(SceneBuffers &s)
     dfree(s.pid_scene);
     dfree(s.cellc);
     dfree(s.bnd);
+    dfree(s.bndl);
+    dfree(s.col_groups);
     dfree(s.sort_scratch);
     dfree(s.elem_run);
     dfree(s.run_key);
@@ -864,6 +866,34 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
 
 namespace {
 
+// The candidate groups of a scene of more than kMaskColliders colliders: kColGroup consecutive
+// colliders of its visit order (the enabled ones, sc[0 .. ncol)) and their AABB union; `slots`
+// groups are written (those past the colliders empty), so edits that enable colliders fit.
+std::vector<ColGroup> collider_groups(const std::vector<Collider> &sc, uint32_t ncol, uint32_t slots)
+{
+    std::vector<ColGroup> g(slots);
+    for (uint32_t k = 0; k < slots; ++k) {
+        ColGroup &G = g[k];
+        G.first = k * kColGroup;
+        G.count = G.first < ncol ? std::min<uint32_t>(kColGroup, ncol - G.first) : 0u;
+        for (int a = 0; a < 3; ++a) {
+            G.mn[a] = 3.402823466e+38f;
+            G.mx[a] = -3.402823466e+38f;
+        }
+        for (uint32_t q = G.first; q < G.first + G.count; ++q)
+            for (int a = 0; a < 3; ++a) {
+                G.mn[a] = std::min(G.mn[a], sc[q].aabb_min[a]);
+                G.mx[a] = std::max(G.mx[a], sc[q].aabb_max[a]);
+            }
+    }
+    return g;
+}
+
+uint32_t group_slots(uint32_t collider_slots)
+{
+    return collider_slots > kMaskColliders ? (collider_slots + kColGroup - 1) / kColGroup : 0u;
+}
+
 cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
 {
     const cn_particle_soa_view *pv = d.particles;
@@ -894,7 +924,7 @@ cn_status check_scene(cn_ctx *ctx, const cn_scene_desc &d)
                 if (cv->tets[t].p[k] >= n) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "tet particle index out of range");
     }
     if ((uint64_t)tb->collider_count + tb->shape_collider_count > kMaxColliders)
-        return fail(ctx, CN_ERR_UNSUPPORTED, "more than 32 static colliders");
+        return fail(ctx, CN_ERR_UNSUPPORTED, "more than 65535 static colliders in one scene");
     if (tb->shape_collider_count && !tb->shape_colliders) return fail(ctx, CN_ERR_INVALID_ARGUMENT, "null shape colliders");
     for (uint32_t k = 0; k < tb->shape_collider_count; ++k)
         if (tb->shape_colliders[k].shape > CN_SHAPE_CAPSULE)
@@ -960,6 +990,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     std::vector<cn_fluid_material> fmh;
     std::vector<float> pmh;
     std::vector<Collider> col;
+    std::vector<ColGroup> groups;  // candidate groups of the scenes of more than kMaskColliders colliders
     std::vector<SceneInfo> sinfo(count);
     std::vector<cn_soft_edge> edges;
     std::vector<cn_soft_bend> bends;
@@ -1100,6 +1131,12 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
         }
         ctx->any_rigid |= sinfo[k].rigid != 0u;
         ctx->any_rigid_pid |= sinfo[k].rigid_pid != 0u;
+        {
+            const uint32_t slots = tb->collider_count + tb->shape_collider_count;
+            const std::vector<ColGroup> g = collider_groups(sc, (uint32_t)sc.size(), group_slots(slots));
+            sinfo[k].grp0 = (uint32_t)groups.size();
+            groups.insert(groups.end(), g.begin(), g.end());
+        }
         col.insert(col.end(), sc.begin(), sc.end());
         // slots for the disabled colliders too, so cn_scene_set_colliders can enable them in place
         col.resize(col.size() + (tb->collider_count + tb->shape_collider_count - sc.size()));
@@ -1229,6 +1266,16 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
     CN_HIP(ctx, dalloc(&s.colliders, col.size()));
     if (!col.empty())
         CN_HIP(ctx, hipMemcpy(s.colliders, col.data(), col.size() * sizeof(Collider), hipMemcpyHostToDevice));
+    // scenes of more than kMaskColliders colliders: their groups, and a candidate list per slot
+    dfree(s.col_groups);
+    dfree(s.bndl);
+    s.col_groups = nullptr;
+    s.bndl = nullptr;
+    if (!groups.empty()) {
+        CN_HIP(ctx, dalloc(&s.col_groups, groups.size()));
+        CN_HIP(ctx, hipMemcpy(s.col_groups, groups.data(), groups.size() * sizeof(ColGroup), hipMemcpyHostToDevice));
+        CN_HIP(ctx, dalloc(&s.bndl, (size_t)std::max<uint32_t>(ctx->capacity, n) * kBndListCap));
+    }
     s.nscenes = count;
     CN_HIP(ctx, dalloc(&s.scenes, sinfo.size()));
     CN_HIP(ctx, hipMemcpy(s.scenes, sinfo.data(), sinfo.size() * sizeof(SceneInfo), hipMemcpyHostToDevice));
@@ -1589,9 +1636,9 @@ cn_status ensure_tick(cn_ctx *ctx, uint32_t m)
 {
     const SceneBuffers &s = ctx->s;
     SceneBuffers &t = ctx->tick;
-    const bool want_ct = s.ct_alloc > 0, want_sl = s.slist != nullptr;
+    const bool want_ct = s.ct_alloc > 0, want_sl = s.slist != nullptr, want_bl = s.bndl != nullptr;
     if (t.pos[0] && m <= ctx->tick_cap && ctx->tick_list_cap == ctx->cap_alloc && (t.ct_alloc > 0) == want_ct &&
-        (t.slist != nullptr) == want_sl)
+        (t.slist != nullptr) == want_sl && (t.bndl != nullptr) == want_bl)
         return CN_OK;
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_scene(t);
@@ -1609,6 +1656,7 @@ cn_status ensure_tick(cn_ctx *ctx, uint32_t m)
     }
     CN_HIP(ctx, dalloc(&t.cellc, N));
     CN_HIP(ctx, dalloc(&t.bnd, N));
+    if (want_bl) CN_HIP(ctx, dalloc(&t.bndl, N * kBndListCap));
     CN_HIP(ctx, dalloc(&t.sort_scratch, sort_scratch_words((uint32_t)N)));
     CN_HIP(ctx, dalloc(&t.elem_run, N));
     CN_HIP(ctx, dalloc(&t.run_key, N));
@@ -1668,6 +1716,7 @@ SceneBuffers tick_view(const cn_ctx *ctx, uint32_t m, const cn_step_params *p)
     v.cur = 0;
     v.cellc = t.cellc;
     v.bnd = t.bnd;
+    v.bndl = t.bndl;
     v.sort_scratch = t.sort_scratch;
     v.elem_run = t.elem_run;
     v.run_key = t.run_key;
@@ -2157,6 +2206,10 @@ cn_status cn_scene_set_colliders(cn_ctx *ctx, uint32_t scene, const cn_box_colli
     si.ncol = ncol;
     CN_HIP(ctx, hipMemcpyAsync(ctx->s.colliders + si.col0, sc.data(), slots * sizeof(Collider), hipMemcpyHostToDevice,
                                ctx->stream));
+    const std::vector<ColGroup> g = collider_groups(sc, ncol, group_slots((uint32_t)slots));
+    if (!g.empty())
+        CN_HIP(ctx, hipMemcpyAsync(ctx->s.col_groups + si.grp0, g.data(), g.size() * sizeof(ColGroup),
+                                   hipMemcpyHostToDevice, ctx->stream));
     CN_HIP(ctx, hipMemcpyAsync(ctx->s.scenes + scene, &si, sizeof(SceneInfo), hipMemcpyHostToDevice, ctx->stream));
     // the host sources may go as soon as the call returns
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));

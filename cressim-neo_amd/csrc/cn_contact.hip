@@ -67,7 +67,8 @@ struct ContactArgs {
     const uint32_t *adj_cnt;
     const uint32_t *adj_idx;
     const Collider *col;      // static colliders in visit order (SceneInfo.col0 per scene)
-    const uint32_t *bnd;      // per slot: candidate collider bitmask (a solid's rigid candidates)
+    const uint32_t *bnd;      // per slot: candidate colliders (a solid's rigid candidates)
+    const uint16_t *bndl;     // their lists (scenes of many colliders)
     float4 *rgen;             // per slot: position of the last contact generation
     uint32_t n, pcap;
 };
@@ -365,16 +366,16 @@ __global__ __launch_bounds__(kBlock) void k_contact_solve(ContactArgs a, Substep
         a.rgen[t] = p;
         const float4 pv = a.prev[t];
         const float4 pm = a.part_mats[pc.part_mat];
-        for (uint32_t m = rm; m; m &= m - 1u) {
-            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+        each_candidate(rm, a.bndl, t, [&](uint32_t k) {
+            const Collider &cc = a.col[sc.col0 + k];
             float4 rec;
             float3 d;
-            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) continue;
-            if (!rigid_position(p, pv, rec, pm, cc, c, d)) continue;
+            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) return;
+            if (!rigid_position(p, pv, rec, pm, cc, c, d)) return;
             sx = sx + d.x;
             sy = sy + d.y;
             sz = sz + d.z;
-        }
+        });
     }
     a.corr[t] = make_float4(sx, sy, sz, 0.0f);
 }
@@ -485,16 +486,16 @@ __global__ __launch_bounds__(kBlock) void k_rigid_velocity(ContactArgs a, Subste
     float4 v = a.vel[t];
     for (uint32_t it = 0; it < iters; ++it) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        for (uint32_t m = rm; m; m &= m - 1u) {
-            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+        each_candidate(rm, a.bndl, t, [&](uint32_t k) {
+            const Collider &cc = a.col[sc.col0 + k];
             float4 rec;
-            if (!rigid_contact(cc, g.x, g.y, g.z, pc.radius, rec)) continue;
+            if (!rigid_contact(cc, g.x, g.y, g.z, pc.radius, rec)) return;
             const float3 nr = safe_normalize(rec.x, rec.y, rec.z);
             // the contact point's velocity is +0, so the relative velocity is v - 0 = v
             const float nv = dot3(v.x, v.y, v.z, nr.x, nr.y, nr.z);
-            if (nv >= 0.0f) continue;
+            if (nv >= 0.0f) return;
             const float denom = w + 0.0f;
-            if (denom <= kEps) continue;
+            if (denom <= kEps) return;
             const bool rest_on = (-nv > c.cfg[kCfgRestitutionVelocity]) &&
                                  (rec.w <= c.cfg[kCfgRestitutionPenetration] * c.cfg[kCfgContactSlop]);
             const float restitution = rest_on ? fsat(rigid_material(pm, cc.material).y) : 0.0f;
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_rigid_velocity(ContactArgs a, Subste
             sx = sx + (nr.x * imp) * w;
             sy = sy + (nr.y * imp) * w;
             sz = sz + (nr.z * imp) * w;
-        }
+        });
         v.x = v.x + sx;
         v.y = v.y + sy;
         v.z = v.z + sz;
@@ -527,6 +528,7 @@ struct RigidPidArgs {
     const SceneInfo *scenes;
     const Collider *col;
     const uint32_t *bnd;      // slot order
+    const uint16_t *bndl;
     const float4 *prev;       // slot order
     const uint32_t *cls;      // slot order
     const ParticleClass *classes;
@@ -554,16 +556,16 @@ __global__ __launch_bounds__(kBlock) void k_rigid_pid(RigidPidArgs a, SubstepCon
         a.rgen[t] = p;
         const float4 pv = a.prev[t];
         const float4 pm = a.part_mats[pc.part_mat];
-        for (uint32_t m = rm; m; m &= m - 1u) {
-            const Collider &cc = a.col[sc.col0 + __builtin_ctz(m)];
+        each_candidate(rm, a.bndl, t, [&](uint32_t k) {
+            const Collider &cc = a.col[sc.col0 + k];
             float4 rec;
             float3 d;
-            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) continue;
-            if (!rigid_position(p, pv, rec, pm, cc, c, d)) continue;
+            if (!rigid_contact(cc, p.x, p.y, p.z, pc.radius, rec)) return;
+            if (!rigid_position(p, pv, rec, pm, cc, c, d)) return;
             sx = sx + d.x;
             sy = sy + d.y;
             sz = sz + d.z;
-        }
+        });
     }
     p.x = p.x + sx;
     p.y = p.y + sy;
@@ -598,6 +600,7 @@ static ContactArgs contact_args(const SceneBuffers &s, const SubstepConsts &c, i
     a.adj_idx = s.adj_idx;
     a.col = s.colliders;
     a.bnd = s.bnd;
+    a.bndl = s.bndl;
     a.rgen = s.ct_gen;
     a.n = c.n;
     a.pcap = kContactPartners;
@@ -639,6 +642,7 @@ void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t
     a.scenes = s.scenes;
     a.col = s.colliders;
     a.bnd = s.bnd;
+    a.bndl = s.bndl;
     a.prev = s.prev[s.cur];
     a.cls = s.cls[s.cur];
     a.classes = s.classes;

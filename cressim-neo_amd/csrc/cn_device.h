@@ -18,7 +18,11 @@ constexpr float kEps = 1.0e-6f;              // kEpsilon, physics_base.hlsli:7
 constexpr uint32_t kInvalid = 0xffffffffu;   // kInvalidIndex, physics_base.hlsli:6
 constexpr float kKernelPi = 3.14159265359f;  // physics_fluid_common.hlsli:26
 constexpr float kBroadPhaseMargin = 0.05f;   // physics_shape_common.hlsli:10
-constexpr uint32_t kMaxColliders = 32;       // colliders a particle's candidate bitmask can name
+constexpr uint32_t kMaskColliders = 32;      // a scene of at most this many colliders: candidate bitmasks
+constexpr uint32_t kMaxColliders = 65535;    // colliders of one scene (16-bit candidate entries)
+constexpr uint32_t kBndList = 0x80000000u;   // candidate word: the low bits count list entries
+constexpr uint32_t kBndListCap = 32;         // candidate colliders a particle holds in a list
+constexpr uint32_t kColGroup = 32;           // colliders per group of the two-level candidate search
 constexpr int kBlock = 256;                  // threads per workgroup for per-particle kernels
 
 // GpuPhysicsSolverConfig float indices (physics_solver_config.hlsli:19-51)
@@ -51,7 +55,8 @@ struct SceneInfo {
     float inv_cell;       // 1 / max(cell size, 1e-4) of the scene
     uint32_t key_prefix;  // scene index << (3 * morton_bits)
     uint32_t col0;        // first collider of the scene in the visit-ordered collider array
-    uint32_t ncol;        // colliders of the scene (at most kMaxColliders)
+    uint32_t ncol;        // enabled colliders of the scene (at most kMaxColliders)
+    uint32_t grp0;        // first collider group of the scene (scenes of more than kMaskColliders)
     uint32_t fluid_only;  // 1: no solid particle and no constraint, so fluid positions change
                           // within a substep only by their deltas
     uint32_t has_fluid;   // 1: some fluid particle (the fluid passes have work in the scene)
@@ -63,6 +68,28 @@ struct SceneInfo {
     uint32_t rigid_pid;   // 1: the same, for a scene without particle-particle pairs whose every
                           // particle is constrained: solved in particle-index order in the soft phase
 };
+
+// kColGroup consecutive colliders of a scene's visit order and the union of their broad-phase
+// AABBs: a scene of more than kMaskColliders colliders tests a group before its members.
+struct ColGroup {
+    float mn[3], mx[3];
+    uint32_t first, count;  // colliders first .. first + count - 1 of the scene's visit order
+};
+
+// Calls f(k) for every candidate collider of a slot, k its index in the scene's visit order, in
+// that order: `word` is a bitmask over the scene's colliders, or kBndList | n with the candidates in
+// the slot's kBndListCap-entry list.
+template <class F>
+__device__ __forceinline__ void each_candidate(uint32_t word, const uint16_t *__restrict__ list, uint32_t slot, F &&f)
+{
+    if (!(word & kBndList)) {
+        for (uint32_t m = word; m; m &= m - 1u) f((uint32_t)__builtin_ctz(m));
+    } else {
+        const uint32_t n = min(word & 0xffu, kBndListCap);
+        const uint16_t *l = list + (size_t)slot * kBndListCap;
+        for (uint32_t q = 0; q < n; ++q) f((uint32_t)l[q]);
+    }
+}
 
 // A static collider in LBVH visit order, its world pose resolved on the host once (the body pose
 // composed with the collider's local pose, quaternions normalized) and its broad-phase AABB

@@ -40,6 +40,7 @@ struct FluidArgs {
     const uint32_t *cls;
     const int4 *cellc;
     const uint32_t *bnd;
+    const uint16_t *bndl;
     const float4 *din;        // accumulated delta of the previous iteration
     float4 *dout;
     float4 *cur;              // (x* + delta, inverse mass) at the start of the iteration
@@ -359,17 +360,15 @@ __device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, con
                                             const FluidArgs &a, const SubstepConsts &c)
 {
     float density = d.density;
-    const uint32_t bm = a.bnd[i];
     const float bscale = c.cfg[kCfgFluidBndDensity];
-    const SceneInfo sc = a.scenes[x.pc.scene];
-    for (uint32_t k = 0; k < sc.ncol; ++k) {
-        if (!((bm >> k) & 1u)) continue;
+    const uint32_t col0 = a.scenes[x.pc.scene].col0;
+    each_candidate(a.bnd[i], a.bndl, i, [&](uint32_t k) {
         float gx, gy, gz;
-        if (!ghost_delta(a.col[sc.col0 + k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) continue;
+        if (!ghost_delta(a.col[col0 + k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) return;
         const float gd = len3(gx, gy, gz);
-        if (gd >= x.h) continue;
+        if (gd >= x.h) return;
         density = density + spiky_w_k(gd, x.h, x.kw) * bscale;
-    }
+    });
     // the material is read again here rather than held in registers across the walk
     const cn_fluid_material &m = a.fm[x.pc.fluid_mat];
     const float rho0 = fmaxd(m.rest_density, 1.0f);
@@ -556,19 +555,17 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         // ComputeFluidBoundaryDeltaFromCandidates, physics_fluid_boundary_common.hlsli:140-167
         float bx = 0.0f, by = 0.0f, bz = 0.0f;
         const float scn = fmaxd(x.Ci, 0.0f);
-        const uint32_t bm = a.bnd[i];
-        const SceneInfo sc = a.scenes[pc.scene];
-        for (uint32_t k = 0; k < sc.ncol; ++k) {
-            if (!((bm >> k) & 1u)) continue;
+        const uint32_t col0 = a.scenes[pc.scene].col0;
+        each_candidate(a.bnd[i], a.bndl, i, [&](uint32_t k) {
             float gx, gy, gz;
-            if (!ghost_delta(a.col[sc.col0 + k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) continue;
+            if (!ghost_delta(a.col[col0 + k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) return;
             const float gd = len3(gx, gy, gz);
-            if (gd <= kEps || gd >= x.h) continue;
+            if (gd <= kEps || gd >= x.h) return;
             const float f = -scn * spiky_dw_k(gd, x.h, x.kw);
             bx = bx + f * fdiv(gx, gd);
             by = by + f * fdiv(gy, gd);
             bz = bz + f * fdiv(gz, gd);
-        }
+        });
         const float solve = c.cfg[kCfgFluidSolve], bdel = c.cfg[kCfgFluidBndDelta];
         ox = d.x * solve + bx * bdel;
         oy = d.y * solve + by * bdel;
@@ -581,21 +578,20 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
     // clamp
     const float r = fmaxd(pc.radius, 0.0f);
     float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
-    const uint32_t bm = a.bnd[i];
-    const SceneInfo scb = a.scenes[pc.scene];
+    const uint32_t bw = a.bnd[i];
+    const uint32_t bcol0 = a.scenes[pc.scene].col0;
     for (int sweep = 0; sweep < 2; ++sweep) {
         bool changed = false;
-        for (uint32_t k = 0; k < scb.ncol; ++k) {
-            if (!((bm >> k) & 1u)) continue;
+        each_candidate(bw, a.bndl, i, [&](uint32_t k) {
             float sd, nx, ny, nz;
-            if (!collider_contact(a.col[scb.col0 + k], cx, cy, cz, sd, nx, ny, nz)) continue;
-            if (sd >= r) continue;
+            if (!collider_contact(a.col[bcol0 + k], cx, cy, cz, sd, nx, ny, nz)) return;
+            if (sd >= r) return;
             const float push = r - sd;
             cx = cx + nx * push;
             cy = cy + ny * push;
             cz = cz + nz * push;
             changed = true;
-        }
+        });
         if (!changed) break;
     }
     const float ex = cx - p.x, ey = cy - p.y, ez = cz - p.z;
@@ -1567,6 +1563,7 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.cls = s.cls[cur];
     a.cellc = s.cellc;
     a.bnd = s.bnd;
+    a.bndl = s.bndl;
     a.din = s.dacc[din];
     a.dout = s.dacc[1 - din];
     a.cur = s.curpos;

@@ -106,7 +106,10 @@ struct SceneBuffers {
     int cur = 0;
     float4 *prev[2] = {nullptr, nullptr};  // position before the last predict, in slot order
     int4 *cellc = nullptr;          // cell coordinates of each slot (w unused)
-    uint32_t *bnd = nullptr;        // boundary-candidate bitmask over colliders in visit order
+    uint32_t *bnd = nullptr;        // candidate colliders: bitmask over the scene's visit order, or a
+                                    // list count (kBndList) with the entries in bndl
+    uint16_t *bndl = nullptr;       // kBndListCap candidate colliders per slot (scenes of more than
+                                    // kMaskColliders colliders; null when the pool has none)
     // broadphase
     uint2 *kv[2] = {nullptr, nullptr};
     uint32_t *sort_scratch = nullptr;
@@ -167,6 +170,7 @@ struct SceneBuffers {
     float4 *part_mats = nullptr;
     Collider *colliders = nullptr;
     uint32_t ncolliders = 0;
+    ColGroup *col_groups = nullptr;  // per scene of more than kMaskColliders colliders, from grp0
     SceneInfo *scenes = nullptr;    // one record per scene of the pool
     uint32_t nscenes = 0;
     uint32_t *scene_active = nullptr;  // per scene: 1 = stepped by cn_step, 0 = state left untouched

@@ -225,34 +225,60 @@ void launch_runs(SceneBuffers &s, const SubstepConsts &c, const uint2 *sorted, u
 // .cs.hlsl:20-28 query radius, :43-85 filters), a solid particle's particle-rigid candidates
 // (physics_particle_count_particle_rigid_candidates.cs.hlsl:112-151: the box of its radius; a
 // static body takes solid particles only; every collider is a body of its own).
-__device__ __forceinline__ uint32_t boundary_mask(const ParticleClass &pc, const cn_fluid_material *fm,
-                                                  const Collider *__restrict__ col, uint32_t ncol, float x,
-                                                  float y, float z)
+// The half width of a particle's candidate query box: 0 when it takes no candidates.
+__device__ __forceinline__ float candidate_query(const ParticleClass &pc, const cn_fluid_material *fm)
 {
-    if (ncol == 0) return 0u;
-    float qr;
     if (pc.kind == CN_PARTICLE_KIND_FLUID) {
         const float r = fmaxd(pc.radius, 0.0f);
         const cn_fluid_material m = fm[pc.fluid_mat];
         const float support = fmaxd(m.smoothing_radius, r);
         const float reuse = fmaxd(fmaxd(m.cfl_radius, r), support);
-        qr = support + reuse;
-    } else if (pc.kind == CN_PARTICLE_KIND_SOLID) {
-        qr = pc.radius;
-    } else {
-        return 0u;
+        return support + reuse;
     }
+    return pc.kind == CN_PARTICLE_KIND_SOLID ? pc.radius : 0.0f;
+}
+
+__device__ __forceinline__ bool candidate_ok(const Collider &cc, const ParticleClass &pc, float x0, float y0, float z0,
+                                             float x1, float y1, float z1)
+{
+    const bool ov = x0 <= cc.aabb_max[0] && x1 >= cc.aabb_min[0] && y0 <= cc.aabb_max[1] && y1 >= cc.aabb_min[1] &&
+                    z0 <= cc.aabb_max[2] && z1 >= cc.aabb_min[2];
+    return ov && cc.enabled && cc.env == pc.env && (pc.mask & cc.layer) != 0u && (cc.mask & pc.layer) != 0u;
+}
+
+// A particle's candidate colliders in the scene's LBVH visit order: a bitmask over the scene's
+// colliders for a scene of at most kMaskColliders, else the count (with kBndList) of the entries
+// written to the slot's list, the colliders found group by group; more than kBndListCap are
+// counted in meta[4] and the first kBndListCap kept.
+__device__ __forceinline__ uint32_t boundary_candidates(const ParticleClass &pc, const cn_fluid_material *fm,
+                                                        const Collider *__restrict__ col, uint32_t ncol,
+                                                        const ColGroup *__restrict__ grp, uint16_t *__restrict__ list,
+                                                        uint32_t *__restrict__ meta, float x, float y, float z)
+{
+    if (ncol == 0 || (pc.kind != CN_PARTICLE_KIND_FLUID && pc.kind != CN_PARTICLE_KIND_SOLID)) return 0u;
+    const float qr = candidate_query(pc, fm);
     const float x0 = x - qr, y0 = y - qr, z0 = z - qr;
     const float x1 = x + qr, y1 = y + qr, z1 = z + qr;
-    uint32_t mask = 0;
-    for (uint32_t k = 0; k < ncol; ++k) {
-        const Collider &cc = col[k];
-        const bool ov = x0 <= cc.aabb_max[0] && x1 >= cc.aabb_min[0] && y0 <= cc.aabb_max[1] &&
-                        y1 >= cc.aabb_min[1] && z0 <= cc.aabb_max[2] && z1 >= cc.aabb_min[2];
-        if (ov && cc.enabled && cc.env == pc.env && (pc.mask & cc.layer) != 0u && (cc.mask & pc.layer) != 0u)
-            mask |= 1u << k;
+    if (ncol <= kMaskColliders) {
+        uint32_t mask = 0;
+        for (uint32_t k = 0; k < ncol; ++k)
+            if (candidate_ok(col[k], pc, x0, y0, z0, x1, y1, z1)) mask |= 1u << k;
+        return mask;
     }
-    return mask;
+    uint32_t n = 0;
+    const uint32_t ngrp = (ncol + kColGroup - 1) / kColGroup;
+    for (uint32_t gi = 0; gi < ngrp; ++gi) {
+        const ColGroup g = grp[gi];
+        if (!(x0 <= g.mx[0] && x1 >= g.mn[0] && y0 <= g.mx[1] && y1 >= g.mn[1] && z0 <= g.mx[2] && z1 >= g.mn[2]))
+            continue;
+        for (uint32_t k = g.first; k < g.first + g.count; ++k) {
+            if (!candidate_ok(col[k], pc, x0, y0, z0, x1, y1, z1)) continue;
+            if (n < kBndListCap) list[n] = (uint16_t)k;
+            ++n;
+        }
+    }
+    if (n > kBndListCap) atomicAdd(meta + 4, 1u);
+    return n ? kBndList | min(n, kBndListCap) : 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_reorder(
@@ -260,10 +286,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t *__restrict__ cls_o, const uint32_t *__restrict__ id_o, float4 *__restrict__ pos_n,
     float4 *__restrict__ vel_n, uint32_t *__restrict__ cls_n, uint32_t *__restrict__ id_n,
     const float4 *__restrict__ prev_o, float4 *__restrict__ prev, float4 *__restrict__ cur,
-    int4 *__restrict__ cellc, uint32_t *__restrict__ bnd,
+    int4 *__restrict__ cellc, uint32_t *__restrict__ bnd, uint16_t *__restrict__ bndl,
     uint32_t *__restrict__ slot_of, const ParticleClass *__restrict__ classes,
-    const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, const SceneInfo *__restrict__ scenes,
-    const uint32_t *__restrict__ active, SubstepConsts c)
+    const cn_fluid_material *__restrict__ fm, const Collider *__restrict__ col, const ColGroup *__restrict__ grp,
+    const SceneInfo *__restrict__ scenes, const uint32_t *__restrict__ active, uint32_t *__restrict__ meta,
+    SubstepConsts c)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= c.n) return;
@@ -286,7 +313,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const int3 g = cell_of(p.x, p.y, p.z, scene_inv_cell(sc, c));
     // w: the scene's key prefix, from which neighbor-cell keys are formed
     cellc[t] = make_int4(g.x, g.y, g.z, (int)sc.key_prefix);
-    bnd[t] = boundary_mask(pc, fm, col + sc.col0, sc.ncol, p.x, p.y, p.z);
+    // a scene with more than kMaskColliders colliders has groups, and the pool a list per slot
+    bnd[t] = boundary_candidates(pc, fm, col + sc.col0, bndl ? sc.ncol : min(sc.ncol, kMaskColliders),
+                                 grp + sc.grp0, bndl + (size_t)t * kBndListCap, meta, p.x, p.y, p.z);
 }
 
 // (cx, cy, cz, pure): pure runs hold a single cell; mixed runs (hash collisions or wrapped
@@ -322,9 +351,9 @@ void launch_reorder(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
     const int o = s.cur, nw = 1 - s.cur;
     hipLaunchKernelGGL(k_reorder, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.perm, s.pos[o], s.vel[o], s.cls[o],
                        s.id[o], s.pos[nw], s.vel[nw], s.cls[nw], s.id[nw], s.prev[o], s.prev[nw], s.curpos, s.cellc,
-                       s.bnd,
+                       s.bnd, s.bndl,
                        s.ne + s.nb + s.nd + s.nt ? s.slot_of : nullptr, s.classes, s.fluid_mats, s.colliders,
-                       s.scenes, s.scene_active, c);
+                       s.col_groups, s.scenes, s.scene_active, s.meta, c);
     hipLaunchKernelGGL(k_run_info, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.run_start, s.run_count, s.cellc,
                        s.run_info, s.meta);
     s.cur = nw;
@@ -412,6 +441,7 @@ void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
 __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                      const float4 *__restrict__ prev, const uint32_t *__restrict__ cls,
                                                      const float4 *__restrict__ dacc, const uint32_t *__restrict__ bnd,
+                                                     const uint16_t *__restrict__ bndl,
                                                      const ParticleClass *__restrict__ classes,
                                                      const float4 *__restrict__ part_mats,
                                                      const Collider *__restrict__ col,
@@ -451,14 +481,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
         const float damping = fmaxd(pm.z, 0.0f);
         float best = 3.402823466e+38f, bnx = 0.0f, bny = 1.0f, bnz = 0.0f, bmf = 0.0f, bms = 0.0f;
         bool found = false;
-        const uint32_t bm = bnd[i];
         const Collider *scol = col + sc.col0;
-        for (uint32_t k = 0; k < sc.ncol; ++k) {
-            if (!((bm >> k) & 1u)) continue;
+        each_candidate(bnd[i], bndl, i, [&](uint32_t k) {
             float sd, nx, ny, nz;
-            if (!collider_contact(scol[k], p.x, p.y, p.z, sd, nx, ny, nz)) continue;
+            if (!collider_contact(scol[k], p.x, p.y, p.z, sd, nx, ny, nz)) return;
             const float cdist = r + 2.0f * slop;
-            if (sd > cdist) continue;
+            if (sd > cdist) return;
             if (sd < best) {
                 best = sd;
                 bnx = nx;
@@ -468,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
                 bms = fsqrt(fmaxd(0.0f, pm.w) * fmaxd(0.0f, scol[k].material[3]));
                 found = true;
             }
-        }
+        });
         if (found) {
             const float pen = fmaxd(r - best, 0.0f);
             const float cdist = fmaxd(r + 2.0f * slop, 1.0e-5f);
@@ -521,7 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(float4 *__restrict__ pos, f
 void launch_finalize(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finalize, dim3(grid1(c.n)), dim3(kBlock), 0, st, s.pos[s.cur], s.vel[s.cur], s.prev[s.cur],
-                       s.cls[s.cur], s.dacc[din], s.bnd, s.classes, s.part_mats, s.colliders, s.scenes, s.scene_active, c);
+                       s.cls[s.cur], s.dacc[din], s.bnd, s.bndl, s.classes, s.part_mats, s.colliders, s.scenes,
+                       s.scene_active, c);
 }
 
 // slot order -> particle-index order

@@ -149,7 +149,9 @@ typedef struct cn_scene_tables {
     uint32_t collider_count;
     const cn_box_collider *colliders;
     /* static colliders of any shape (spheres, oriented boxes, capsules), after the boxes: collider
-     * k >= collider_count is shape_colliders[k - collider_count]; at most 32 colliders in all */
+     * k >= collider_count is shape_colliders[k - collider_count]; at most 65535 in all.  A particle
+     * holds at most 32 candidate colliders at a time (more: cn_validate_meta reports
+     * contact_overflows and returns CN_ERR_CAPACITY). */
     uint32_t shape_collider_count;
     const cn_collider *shape_colliders;
 } cn_scene_tables;
@@ -233,6 +235,8 @@ typedef struct cn_meta {
     uint32_t max_cell_occupancy;          /* longest run of equal keys over the substeps of the last
                                              cn_step */
     uint32_t contact_overflows;           /* particles with more contact candidates than a list holds
+                                             (32), or, in a scene of more than 32 colliders, more
+                                             colliders in reach than a slot's collider list holds
                                              (32), summed over the substeps; the surplus ones
                                              are dropped */
     uint64_t fluid_neighbor_total;        /* listed neighbors at iteration 0, summed over the

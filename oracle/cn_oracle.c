@@ -2272,8 +2272,10 @@ static void contact_velocity(const cno_scene *s, const float *cfg, uint32_t lo, 
  * velocity are exactly zero.
  * ------------------------------------------------------------------------------------------- */
 
-/* candidate colliders of particle i as a bitmask over the LBVH visit positions */
-static uint32_t rigid_mask(const cno_scene *s, const ocol *oc, const uint32_t *order, uint32_t nvis, uint32_t i)
+/* candidate colliders of particle i: their LBVH visit positions, ascending, into cand (at most
+ * nvis); returns the count */
+static uint32_t rigid_candidates(const cno_scene *s, const ocol *oc, const uint32_t *order, uint32_t nvis, uint32_t i,
+                                 uint32_t *cand)
 {
     if (s->kinds[i] != CN_PARTICLE_KIND_SOLID)
         return 0u;
@@ -2285,15 +2287,15 @@ static uint32_t rigid_mask(const cno_scene *s, const ocol *oc, const uint32_t *o
         qmin[a] = p[a] - r;
         qmax[a] = p[a] + r;
     }
-    uint32_t mask = 0u;
+    uint32_t n = 0;
     for (uint32_t k = 0; k < nvis; ++k) {
         const ocol *c = &oc[order[k]];
         const int overlap = qmin[0] <= c->mx[0] && qmax[0] >= c->mn[0] && qmin[1] <= c->mx[1] &&
                             qmax[1] >= c->mn[1] && qmin[2] <= c->mx[2] && qmax[2] >= c->mn[2];
         if (overlap && c->enabled && c->env == md[0] && (md[3] & c->layer) != 0u && (c->mask & md[2]) != 0u)
-            mask |= 1u << k;
+            cand[n++] = k;
     }
-    return mask;
+    return n;
 }
 
 /* generate_contacts :82-176 for one collider: the particle position p in the collider frame, the
@@ -2454,14 +2456,12 @@ static void rigid_velocity(const cno_scene *s, const float *cfg, uint32_t i, con
 /* the rigid contacts' corrections of particle i at the current position, added to acc after its
  * particle-particle ones; gen keeps the position they were generated at */
 static void rigid_iteration_particle(const cno_scene *s, const float *cfg, const ocol *oc, const uint32_t *order,
-                                     uint32_t i, uint32_t mask, float *gen, float *acc)
+                                     uint32_t i, uint32_t ncand, const uint32_t *cand, float *gen, float *acc)
 {
     const float *p = s->pos + 4 * (size_t)i;
     memcpy(gen + 4 * (size_t)i, p, 4 * sizeof(float));
-    for (uint32_t k = 0; k < 32u; ++k) {
-        if (!((mask >> k) & 1u))
-            continue;
-        const ocol *c = &oc[order[k]];
+    for (uint32_t q = 0; q < ncand; ++q) {
+        const ocol *c = &oc[order[cand[q]]];
         float nrm[3], pen, d[3];
         if (!rigid_contact(c, p, s->radii[i], nrm, &pen))
             continue;
@@ -2473,17 +2473,16 @@ static void rigid_iteration_particle(const cno_scene *s, const float *cfg, const
 
 /* one rigid velocity iteration over the contacts generated at gen, then the apply */
 static void rigid_velocity_iteration(cno_scene *s, const float *cfg, const ocol *oc, const uint32_t *order,
-                                     const uint32_t *rmask, const float *gen, float *corr)
+                                     const uint32_t *ncand, const uint32_t *cand, uint32_t stride, const float *gen,
+                                     float *corr)
 {
     const uint32_t n = s->n;
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
         float acc[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t k = 0; k < 32u; ++k) {
-            if (!((rmask[i] >> k) & 1u))
-                continue;
-            const ocol *c = &oc[order[k]];
+        for (uint32_t q = 0; q < ncand[i]; ++q) {
+            const ocol *c = &oc[order[cand[(size_t)i * stride + q]]];
             float nrm[3], pen, d[3];
             if (!rigid_contact(c, gen + 4 * (size_t)i, s->radii[i], nrm, &pen))
                 continue;
@@ -2505,7 +2504,9 @@ static void rigid_velocity_iteration(cno_scene *s, const float *cfg, const ocol 
 typedef struct rigid_state {
     const ocol *oc;        /* the scene's colliders (index order) */
     const uint32_t *order; /* LBVH visit order */
-    uint32_t *mask;        /* per particle: candidate visit positions */
+    uint32_t *ncand;       /* per particle: candidate count */
+    uint32_t *cand;        /* per particle: candidate visit positions, stride entries */
+    uint32_t stride;
     float *gen;            /* per particle: position of the last generation (float4) */
 } rigid_state;
 
@@ -2532,8 +2533,9 @@ static void contact_iteration(cno_scene *s, const float *cfg, uint32_t pcap, con
             for (int a = 0; a < 3; ++a)
                 acc[a] = acc[a] + d[a];
         }
-        if (rs && rs->mask[i])
-            rigid_iteration_particle(s, cfg, rs->oc, rs->order, i, rs->mask[i], rs->gen, acc);
+        if (rs && rs->ncand[i])
+            rigid_iteration_particle(s, cfg, rs->oc, rs->order, i, rs->ncand[i], rs->cand + (size_t)i * rs->stride,
+                                     rs->gen, acc);
         memcpy(corr + 4 * (size_t)i, acc, sizeof(acc));
     }
     for (uint32_t i = 0; i < n; ++i) {
@@ -2658,15 +2660,17 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         if (!pcount || !plist || !recs)
             return -1;
     }
-    rigid_state rs = {NULL, order, NULL, NULL};
+    rigid_state rs = {NULL, order, NULL, NULL, 0, NULL};
     ocol *roc = NULL;
     uint32_t rnvis = 0;
     if (has_rigid) {
         const uint32_t nc = n_cols(s);
         roc = (ocol *)malloc(sizeof(ocol) * nc);
-        rs.mask = (uint32_t *)calloc(n, sizeof(uint32_t));
+        rs.stride = nc;
+        rs.ncand = (uint32_t *)calloc(n, sizeof(uint32_t));
+        rs.cand = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n * nc);
         rs.gen = (float *)calloc(4 * (size_t)n, sizeof(float));
-        if (!roc || !rs.mask || !rs.gen)
+        if (!roc || !rs.ncand || !rs.cand || !rs.gen)
             return -1;
         for (uint32_t k = 0; k < nc; ++k)
             ocol_get(s, k, &roc[k]);
@@ -2719,7 +2723,7 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
             st.contact_overflows += cno_contact_partners(s, cells, idx, tbl, capacity, pcap, pcount, plist);
         if (has_rigid)
             for (uint32_t i = 0; i < n; ++i)
-                rs.mask[i] = rigid_mask(s, roc, order, rnvis, i);
+                rs.ncand[i] = rigid_candidates(s, roc, order, rnvis, i, rs.cand + (size_t)i * rs.stride);
         if (has_soft) {
             /* clearSoftConstraintState, physics_clear_soft_constraint_state.cs.hlsl:14-54 */
             memset(lam_e, 0, sizeof(float) * (s->n_edges + 1));
@@ -2783,7 +2787,7 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
         /* physics_solver.cpp:1301-1309, after the particle-particle velocity iterations */
         if (has_rigid)
             for (uint32_t v = 0; v < contact_iters; ++v)
-                rigid_velocity_iteration(s, cfg, roc, order, rs.mask, rs.gen, ccorr);
+                rigid_velocity_iteration(s, cfg, roc, order, rs.ncand, rs.cand, rs.stride, rs.gen, ccorr);
     }
     /* density-constraint residual of the last density pass of the step */
     if (has_fluid) {
@@ -2820,7 +2824,8 @@ int cno_step(cno_scene *s, const cn_step_params *prm, cno_stats *stats)
     free(recs);
     free(ccorr);
     free(roc);
-    free(rs.mask);
+    free(rs.ncand);
+    free(rs.cand);
     free(rs.gen);
     if (has_soft) {
         csr_free(&inc_e);

@@ -186,3 +186,41 @@ def test_fluid_rests_outside_a_sphere_and_a_capsule(oracle):
     for col in sc.shape_colliders:
         sds = np.array([oracle.collider_contact(col, q)[1] for q in osc.pos[:, :3]])
         assert sds.min() > 0.5 * r, (int(col["shape"]), sds.min())
+
+
+def test_out_of_reach_colliders_change_nothing(oracle):
+    """Forty pegs far above the fluid, listed after or around the scene's own colliders (49
+    colliders in all): the scene steps bit for bit as it does without them, so neither their
+    number nor their place in the LBVH visit order reaches the contacts of the colliders in reach."""
+    sc = scenes.shapes_scene(side=8)
+    far = scenes.pegboard_colliders(8, 5, y=1.6, origin=(2.2, 0.3))
+    after, around = sc.copy(), sc.copy()
+    after.shape_colliders = np.concatenate([sc.shape_colliders, far])
+    around.shape_colliders = np.concatenate([far[:20], sc.shape_colliders, far[20:]])
+    p = cnhip.default_params(dt=1 / 120, substeps=2, default_iterations=3, max_fluid_neighbors=64)
+    runs = [oracle.OracleScene(x) for x in (sc, after, around)]
+    for _ in range(20):
+        for o in runs:
+            oracle.step(o, p)
+    for o in runs[1:]:
+        np.testing.assert_array_equal(o.pos, runs[0].pos)
+        np.testing.assert_array_equal(o.vel, runs[0].vel)
+
+
+def test_fluid_and_cloth_rest_on_a_pegboard(oracle):
+    """A fluid block and a cloth dropped on 60 pegs (spheres, capsules, turned boxes; 66 colliders
+    with the walls): no particle ends inside a peg, and particles rest on them."""
+    sc = scenes.pegboard()
+    assert len(sc.shape_colliders) + len(sc.colliders) > 32
+    p = cnhip.default_params(dt=1 / 120, substeps=4, default_iterations=4, max_fluid_neighbors=64)
+    osc = oracle.OracleScene(sc)
+    for _ in range(40):
+        oracle.step(osc, p)
+    assert np.isfinite(osc.pos).all()
+    r = float(sc.radii[0])
+    closest = np.full(sc.n, np.inf)
+    for col in sc.shape_colliders:
+        sds = np.array([oracle.collider_contact(col, q)[1] for q in osc.pos[:, :3]])
+        assert sds.min() > 0.5 * r, (int(col["shape"]), sds.min())
+        closest = np.minimum(closest, sds)
+    assert (closest < 1.1 * r).sum() > 20  # resting on the pegs, not fallen past them
