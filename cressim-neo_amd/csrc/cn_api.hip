@@ -800,7 +800,7 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     if (!ctx) return CN_ERR_INVALID_ARGUMENT;
     if (n <= ctx->capacity && cap <= ctx->cap_alloc && ctx->s.pos[0]) return CN_OK;
     if ((uint64_t)list_slots(n ? n : 1) * (cap ? cap : 1) * sizeof(uint16_t) > kListBytesMax)
-        return fail(ctx, CN_ERR_CAPACITY, "neighbor lists of n x max_fluid_neighbors entries exceed 4 GiB");
+        return fail(ctx, CN_ERR_CAPACITY, "neighbor lists of n x max_fluid_neighbors entries exceed 4 GiB less 1 MiB");
     CN_HIP(ctx, hipSetDevice(ctx->device));
     CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_scene(ctx->s);

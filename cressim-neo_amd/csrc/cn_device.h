@@ -168,6 +168,20 @@ __device__ __forceinline__ float frcp_exact(float b)
     return y;
 }
 
+// 1 / b for b >= 1, +inf or NaN: frcp_exact while 1 / b stays normal (b < 2^126), else the IEEE
+// division.
+__device__ __forceinline__ float frcp_exact_ge1(float b)
+{
+    float y = __builtin_amdgcn_rcpf(b);
+    y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    if ((__float_as_uint(b) & 0x7fffffu) == 0x7fffffu || !(b < 0x1p126f)) {
+        __asm__ volatile("");
+        y = fdiv(1.0f, b);
+    }
+    return y;
+}
+
 // Square root by the hardware estimate (within one ulp) and the rounding test on its two
 // neighbours: s - ulp is the root when x - (s - ulp) * s <= 0, s + ulp when x - (s + ulp) * s > 0.
 // Equal to fsqrt for x == 0 and for 2^-96 <= x < inf, where fsqrt applies no scaling; for

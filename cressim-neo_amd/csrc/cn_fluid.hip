@@ -141,7 +141,7 @@ struct BlockQueue {
         while (tries < kXcds) {
             const uint32_t b0 = (uint32_t)(((uint64_t)nblk * die) / kXcds);
             const uint32_t b1 = (uint32_t)(((uint64_t)nblk * (die + 1)) / kXcds);
-            const uint32_t t = atomicAdd(q + die, 1u);
+            const uint32_t t = atomicAdd(q + die * kQueueStride, 1u);
             if (t < b1 - b0) return b0 + t;
             die = (die + 1) % kXcds;
             ++tries;
@@ -324,6 +324,9 @@ struct DensityTerm {
     float w, ngx, ngy, ngz;
 };
 
+// TENSION = false: the particle's material has no surface tension (x.tension false), so the normal
+// terms are left out at compile time.
+template <bool TENSION = true>
 __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const float4 pj)
 {
     DensityTerm t;
@@ -333,7 +336,7 @@ __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const 
     if (!x.fd) omq = __fsub_rn(1.0f, fdiv(dist, x.h));
     t.w = spiky_w_om(dist, x.h, omq, x.kw);
     t.ngx = t.ngy = t.ngz = 0.0f;
-    if (x.tension && !(dist <= kEps || dist >= x.h)) {
+    if (TENSION && x.tension && !(dist <= kEps || dist >= x.h)) {
         // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53, negated: der * -(d / dist)
         PairQuot p;
         pair_quot<true>(p, dx, dy, dz, dist, x.h, x.ih, x.kw);
@@ -345,10 +348,11 @@ __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const 
     return t;
 }
 
+template <bool TENSION = true>
 __device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x, const DensityTerm &t)
 {
     d.density = d.density + t.w;
-    if (x.tension) {
+    if (TENSION && x.tension) {
         d.snx = d.snx - t.ngx;
         d.sny = d.sny - t.ngy;
         d.snz = d.snz - t.ngz;
@@ -470,7 +474,7 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
             pdz = pdz + p.nnz * ck;
         }
         if (x.visc_on) {
-            const float vs = 1.0f - fdiv(1.0f, 1.0f + x.viscosity * spiky_w_om(dist, x.h, p.omq, x.kw));
+            const float vs = 1.0f - frcp_exact_ge1(1.0f + x.viscosity * spiky_w_om(dist, x.h, p.omq, x.kw));
             pdx = pdx - vs * rx;
             pdy = pdy - vs * ry;
             pdz = pdz - vs * rz;
@@ -840,24 +844,32 @@ __device__ __forceinline__ ListRef list_ref(uint16_t *buf, uint32_t i, uint32_t 
     return ListRef{reinterpret_cast<char *>(buf), (uint32_t)list_entry_base(i, rows)};
 }
 
-// Appends listed halo offsets to a particle's list, storing the first cap; n is the untruncated count.
+// Appends listed halo offsets to a particle's list, storing the first cap.  The write position is
+// the byte offset of the next entry and advances one entry stride per push, so a push is a compare,
+// a store and an add; n() is the untruncated count.  The position may run past entry cap by one
+// stride per walked candidate (at most the tile, 2^12 entries: 512 KiB), which the list buffer's
+// limit leaves room for below 4 GiB.
 struct ListOut {
-    ListRef r;
-    uint32_t n;
-    __device__ __forceinline__ void push(uint32_t f, uint32_t cap)
+    char *base;
+    uint32_t a0, a, lim;  // byte offsets of entry 0, the next entry and entry cap
+    __device__ __forceinline__ ListOut(const ListRef &r, uint32_t cap)
+        : base(r.base), a0(r.off), a(r.off), lim(r.off + cap * kEntryStride)
     {
-        if (n < cap) r.entry(n) = (uint16_t)f;
-        ++n;
     }
+    __device__ __forceinline__ void push(uint32_t f)
+    {
+        if (a < lim) *reinterpret_cast<uint16_t *>(base + a) = (uint16_t)f;
+        a += kEntryStride;
+    }
+    __device__ __forceinline__ uint32_t n() const { return (a - a0) / kEntryStride; }
 };
 
 // One contiguous range [k, f1) of the walk, four candidates per step: the loads read through a
 // pointer with immediate offsets and may run up to three entries past f1 (the staged arrays carry
-// kTilePad spare entries), which the k + u < f1 test then drops.
+// kTilePad spare entries), which the u < f1 - k test then drops.
 template <bool UNIFORM, bool SUP>
 __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4 *lpos, const uint32_t *lcls,
-                                           const DensitySelf &x, const FluidArgs &a, uint32_t cap, ListOut &lo,
-                                           ListOut &so)
+                                           const DensitySelf &x, const FluidArgs &a, ListOut &lo, ListOut &so)
 {
     const float4 *pp = lpos + k;
     const uint32_t *pc = lcls + k;
@@ -869,18 +881,19 @@ __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4
             pj[u] = pp[u];
             qc[u] = UNIFORM ? 0u : pc[u];
         }
+        const uint32_t rem = f1 - k;  // >= 1
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-            bool cand = (SUP ? d2 < x.ts2 : d2 < x.t2) & (u == 0 || k + u < f1);
+            bool cand = (SUP ? d2 < x.ts2 : d2 < x.t2) & (u == 0 || u < rem);
             if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
             if (cand) {
                 if (SUP) {
-                    so.push(k + u, kSuperSlots);
-                    if (d2 < x.t2) lo.push(k + u, cap);
+                    so.push(k + u);
+                    if (d2 < x.t2) lo.push(k + u);
                 } else {
-                    lo.push(k + u, cap);
+                    lo.push(k + u);
                 }
             }
         }
@@ -893,7 +906,7 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
                                               const FluidArgs &a, const SubstepConsts &c, const RowCull &rc,
                                               const ListRef &ls = ListRef{nullptr, 0u}, uint32_t *scount = nullptr)
 {
-    ListOut lo{li, 0u}, so{ls, 0u};
+    ListOut lo(li, c.cap), so(ls, kSuperSlots);
     const float gxl = gap_below(rc.ux, rc.cx), gxr = gap_above(rc.ux, rc.cx);
     // rows of three cells along x, z outer, y inner; the centre row (r = 4) is walked as the
     // ranges before and after the particle itself
@@ -908,14 +921,14 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
         const uint32_t f1 = hoff[h0 + (gxr * gxr + perp < rc.lim2 ? 3 : 2)];
         const uint32_t k0 = hoff[h0 + (gxl * gxl + perp < rc.lim2 ? 0 : 1)];
         if (r == 4) {
-            scan_range<UNIFORM, SUP>(k0, fs, lpos, lcls, x, a, c.cap, lo, so);
-            scan_range<UNIFORM, SUP>(fs + 1, f1, lpos, lcls, x, a, c.cap, lo, so);
+            scan_range<UNIFORM, SUP>(k0, fs, lpos, lcls, x, a, lo, so);
+            scan_range<UNIFORM, SUP>(fs + 1, f1, lpos, lcls, x, a, lo, so);
         } else {
-            scan_range<UNIFORM, SUP>(k0, f1, lpos, lcls, x, a, c.cap, lo, so);
+            scan_range<UNIFORM, SUP>(k0, f1, lpos, lcls, x, a, lo, so);
         }
     }
-    if (SUP) *scount = so.n;
-    return lo.n;
+    if (SUP) *scount = so.n();
+    return lo.n();
 }
 
 // List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past
@@ -937,7 +950,7 @@ __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t sc
                                                   const DensitySelf &x, const ListRef &li, uint32_t cap,
                                                   DensitySum &d)
 {
-    ListOut lo{li, 0u};
+    ListOut lo(li, cap);
     uint32_t en[kTermStep];
     list_group(ls, 0, scount, en);
     for (uint32_t k = 0; k < scount; k += kTermStep) {
@@ -955,15 +968,16 @@ __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t sc
             const DensityTerm t = density_term(x, pj[u]);
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
-                if (lo.n < cap) density_add(d, x, t);
-                lo.push(e[u], cap);
+                if (lo.a < lo.lim) density_add(d, x, t);
+                lo.push(e[u]);
             }
         }
     }
-    return lo.n;
+    return lo.n();
 }
 
 // Density sums over a compacted list, kTermStep entries per step (loads first, sums in list order).
+template <bool TENSION>
 __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, const float4 *lpos,
                                              const DensitySelf &x, DensitySum &d)
 {
@@ -977,10 +991,10 @@ __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, co
         list_group(li, k + kTermStep, cnt, en);
         DensityTerm t[kTermStep];
 #pragma unroll
-        for (uint32_t u = 0; u < kTermStep; ++u) t[u] = density_term(x, lpos[e[u]]);
+        for (uint32_t u = 0; u < kTermStep; ++u) t[u] = density_term<TENSION>(x, lpos[e[u]]);
 #pragma unroll
         for (uint32_t u = 0; u < kTermStep; ++u)
-            if (k + u < cnt) density_add(d, x, t[u]);
+            if (k + u < cnt) density_add<TENSION>(d, x, t[u]);
     }
 }
 
@@ -1014,7 +1028,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     // the iteration's first density launch resets the displacement word its delta pass raises
     if (MODE != kDensFallback && blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
-    uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds;
+    uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds * kQueueStride;
     uint32_t *fb_count = a.iter_state + kStateFbCount + a.it;
     uint32_t *fb_queue = a.iter_state + kStateFbQueue + a.it;
     const uint32_t nfb = MODE == kDensFallback ? *fb_count : 0u;
@@ -1117,7 +1131,12 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                                        : scan(std::false_type{}, x, ci, hc, fs, i, li);
                         }
                         d.inr = min(d.req, c.cap);
-                        if (!REUSE) density_list(li, d.inr, lpos, x, d);
+                        if (!REUSE) {
+                            if (x.tension)
+                                density_list<true>(li, d.inr, lpos, x, d);
+                            else
+                                density_list<false>(li, d.inr, lpos, x, d);
+                        }
                     }
                     cnt = d.inr;
                     density_end(i, d, x, ma, a, c);
@@ -1156,7 +1175,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     uint32_t *lslot = lcls + (UNIFORM ? 0u : a.tile);
     const uint32_t tid = threadIdx.x;
     const uint32_t nblk = a.nblk[1];
-    uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds;
+    uint32_t *queue = a.iter_state + kStateDelta + a.it * kXcds * kQueueStride;
     uint32_t *wqueue = a.iter_state + kStateWideQueue + a.it;
     const uint32_t nwide = WIDE ? a.iter_state[kStateWideCount] : 0u;
     BlockQueue bq(blockIdx.x);

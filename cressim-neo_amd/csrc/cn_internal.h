@@ -10,12 +10,15 @@ namespace cn {
 
 constexpr uint32_t kIterStateWords = 64;  // iterations with tiled fluid passes (later ones walk the run table)
 constexpr uint32_t kXcds = 8;             // accelerator dies; each has its own L2 and block-queue counter
-// iter_state words: block-queue counters of the density and delta passes (kXcds per iteration),
-// the queue counter of the wide-tile delta pass per iteration and the wide-block count (all
-// cleared once per substep), then the largest |delta|^2 bits of each iteration
+// iter_state words: block-queue counters of the density and delta passes (kXcds per iteration,
+// each on a 128-byte line of its own: the workgroups of every die pop their blocks with atomics
+// at the memory side, which serialise per line), the queue counter of the wide-tile delta pass per
+// iteration and the wide-block count (all cleared once per substep), then the largest |delta|^2
+// bits of each iteration
+constexpr uint32_t kQueueStride = 32;  // words between two block-queue counters
 constexpr uint32_t kStateDensity = 0;
-constexpr uint32_t kStateDelta = kIterStateWords * kXcds;
-constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds;
+constexpr uint32_t kStateDelta = kIterStateWords * kXcds * kQueueStride;
+constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds * kQueueStride;
 constexpr uint32_t kStateWideCount = kStateWideQueue + kIterStateWords;
 // per iteration: blocks the candidate-list pass handed to the walking pass, and their queue
 constexpr uint32_t kStateFbQueue = kStateWideCount + 1;
@@ -57,8 +60,9 @@ __host__ __device__ inline size_t list_entry_base(uint32_t i, uint32_t rows)
 }
 __host__ __device__ inline size_t list_slots(size_t n) { return (n + kListGroup - 1) / kListGroup * kListGroup; }
 // A list buffer is addressed with 32-bit byte offsets from its base (one 32-bit add per entry on the
-// device), so it stays within 4 GiB: n x max_fluid_neighbors < 2^31 entries.
-constexpr uint64_t kListBytesMax = 0xffffffffull;
+// device), so it stays within 4 GiB less 1 MiB: a list's write position may run up to 512 KiB past
+// its particle's last entry without wrapping.
+constexpr uint64_t kListBytesMax = 0xffffffffull - 0x100000ull;
 
 // ---- radix sort (cn_sort.hip) ----
 size_t sort_scratch_words(uint32_t n);

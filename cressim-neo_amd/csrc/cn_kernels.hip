@@ -417,10 +417,10 @@ __global__ __launch_bounds__(kBlock) void k_blk_emit(const uint32_t *__restrict_
             *nblk = idx;
         }
     }
-    if (blockIdx.x == 0) {
-        for (uint32_t w = threadIdx.x; w < nwork; w += kBlock) work[w] = 0;
-        if (threadIdx.x == 0) work[prev_word] = work[last_word];
-    }
+    // the substep's queue counters, cleared by every workgroup's share (the max-displacement words
+    // past nwork are kept)
+    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < nwork; w += gridDim.x * kBlock) work[w] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) work[prev_word] = work[last_word];
 }
 
 void launch_blocks(SceneBuffers &s, const SubstepConsts &c, hipStream_t st)

@@ -286,7 +286,7 @@ float cn_particle_grid_cell_size(uint32_t count, const float *radii,
 
 /* -------- scene state (PhysicsSceneGpuState analog) -------- */
 /* ensureCapacity (physics_scene_gpu_state.cpp:305).  CN_ERR_CAPACITY when the neighbor lists
-   (particle_capacity x max_fluid_neighbors 16-bit entries) would exceed 4 GiB. */
+   (particle_capacity x max_fluid_neighbors 16-bit entries) would exceed 4 GiB less 1 MiB. */
 cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t particle_capacity, uint32_t max_fluid_neighbors);
 /* uploadWorldState / uploadParticles / uploadSoftTopology (:2348, :3163, :3255). */
 cn_status cn_scene_upload(cn_ctx *ctx, const cn_particle_soa_view *particles,
