@@ -209,7 +209,6 @@ void free_scene(SceneBuffers &s)
     dfree(s.nblk);
     dfree(s.blk_halo);
     dfree(s.wide_blk);
-    dfree(s.fb_blk);
     dfree(s.iter_state);
     dfree(s.fluid_t2);
     dfree(s.list);
@@ -714,7 +713,7 @@ void cn_default_step_params(cn_step_params *p)
     std::memcpy(p->solver_config, cfg, sizeof(cfg));
     p->max_fluid_neighbors = 128;  // kDefaultFluidMaxNeighborhood, src/physics/rigid_body_common.h:304
     p->key_mode = CN_KEY_MORTON;
-    p->flags = 0u;  // candidate reuse measured no faster (see CN_STEP_CANDIDATE_REUSE)
+    p->flags = CN_STEP_CANDIDATE_REUSE;
 }
 
 // toFluidSolverMaterial and its helpers, src/physics/physics_world.cpp:99-263
@@ -844,7 +843,6 @@ cn_status cn_scene_reserve(cn_ctx *ctx, uint32_t n, uint32_t cap)
     s.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
     CN_HIP(ctx, dalloc(&s.blk_halo, (size_t)s.blk_cap * kHaloStride));
     CN_HIP(ctx, dalloc(&s.wide_blk, s.blk_cap));
-    CN_HIP(ctx, dalloc(&s.fb_blk, s.blk_cap));
     CN_HIP(ctx, dalloc(&s.list, list_slots(N) * (cap ? cap : 1)));
     CN_HIP(ctx, dalloc(&s.lcnt, N));
     CN_HIP(ctx, dalloc(&s.iter_state, kStateWords));
@@ -1675,7 +1673,6 @@ cn_status ensure_tick(cn_ctx *ctx, uint32_t m)
     t.blk_cap = (uint32_t)std::max<size_t>(256, N / 32);
     CN_HIP(ctx, dalloc(&t.blk_halo, (size_t)t.blk_cap * kHaloStride));
     CN_HIP(ctx, dalloc(&t.wide_blk, t.blk_cap));
-    CN_HIP(ctx, dalloc(&t.fb_blk, t.blk_cap));
     CN_HIP(ctx, dalloc(&t.list, list_slots(N) * std::max<uint32_t>(ctx->cap_alloc, 1u)));
     CN_HIP(ctx, dalloc(&t.lcnt, N));
     if (want_sl) {
@@ -1736,7 +1733,6 @@ SceneBuffers tick_view(const cn_ctx *ctx, uint32_t m, const cn_step_params *p)
     v.blk_halo = t.blk_halo;
     v.blk_cap = t.blk_cap;
     v.wide_blk = t.wide_blk;
-    v.fb_blk = t.fb_blk;
     v.list = t.list;
     v.lcnt = t.lcnt;
     v.slist = t.slist;

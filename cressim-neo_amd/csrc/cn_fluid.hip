@@ -62,7 +62,6 @@ struct FluidArgs {
     const uint32_t *nblk;
     uint32_t *blk_halo;
     uint32_t *wide_blk;
-    uint32_t *fb_blk;         // blocks the candidate-list pass leaves to the walking pass
     uint32_t *iter_state;
     const ParticleClass *classes;
     const cn_fluid_material *fm;
@@ -360,13 +359,13 @@ __device__ __forceinline__ void density_add(DensitySum &d, const DensitySelf &x,
 }
 
 // Boundary ghost density (physics_fluid_boundary_common.hlsli:112-138), the constraint and the record.
-__device__ __forceinline__ void density_end(uint32_t i, const DensitySum &d, const DensitySelf &x, MetaAcc &ma,
-                                            const FluidArgs &a, const SubstepConsts &c)
+__device__ __forceinline__ void density_end(uint32_t i, uint32_t bw, const DensitySum &d, const DensitySelf &x,
+                                            MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
 {
     float density = d.density;
     const float bscale = c.cfg[kCfgFluidBndDensity];
     const uint32_t col0 = a.scenes[x.pc.scene].col0;
-    each_candidate(a.bnd[i], a.bndl, i, [&](uint32_t k) {
+    each_candidate(bw, a.bndl, i, [&](uint32_t k) {
         float gx, gy, gz;
         if (!ghost_delta(a.col[col0 + k], x.s.x, x.s.y, x.s.z, x.h, gx, gy, gz)) return;
         const float gd = len3(gx, gy, gz);
@@ -408,7 +407,7 @@ __device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const Fl
             }
             return true;
         });
-    density_end(i, d, x, ma, a, c);
+    density_end(i, a.bnd[i], d, x, ma, a, c);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -514,21 +513,39 @@ __device__ __forceinline__ void delta_add(DeltaSum &d, const DeltaTerm &t)
     d.w = d.w + 1.0f;
 }
 
+// A particle's own inputs of the delta pass, loaded together at the top of its loop iteration so
+// their round trips to memory overlap (rather than one after another as the code reaches them).
+struct DeltaPre {
+    float4 p, di;
+    uint32_t cls, bnd;
+};
+
+template <bool UNIFORM>
+__device__ __forceinline__ DeltaPre delta_pre(uint32_t i, const FluidArgs &a)
+{
+    DeltaPre r;
+    r.p = a.pos[i];
+    r.di = a.it > 0 ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r.cls = UNIFORM ? 0u : a.cls[i];
+    r.bnd = a.bnd[i];
+    return r;
+}
+
 // sum(x, d) adds the neighbor terms in list order.  own addresses the particle's record in
 // rcs / rnc (LDS offset or slot).  Returns (|delta| + slack)^2 of the particle.
 template <bool UNIFORM, class Sum>
-__device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&sum, const float4 *rcs,
-                                                const float4 *rnc, const FluidArgs &a, const SubstepConsts &c)
+__device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, const DeltaPre &pre, Sum &&sum,
+                                                const float4 *rcs, const float4 *rnc, const FluidArgs &a,
+                                                const SubstepConsts &c)
 {
-    const ParticleClass pc = a.classes[UNIFORM ? 0u : a.cls[i]];
+    const ParticleClass pc = a.classes[pre.cls];
     if (pc.kind != CN_PARTICLE_KIND_FLUID) {
         a.dout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (a.disp) a.disp[i] = 0.0f;
         return 0.0f;
     }
-    const float4 p = a.pos[i];
-    const bool use_d = a.it > 0;
-    const float4 di = use_d ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 p = pre.p;
+    const float4 di = pre.di;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
     if (p.w > kEps) {
         DeltaSelf x;
@@ -560,7 +577,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
         float bx = 0.0f, by = 0.0f, bz = 0.0f;
         const float scn = fmaxd(x.Ci, 0.0f);
         const uint32_t col0 = a.scenes[pc.scene].col0;
-        each_candidate(a.bnd[i], a.bndl, i, [&](uint32_t k) {
+        each_candidate(pre.bnd, a.bndl, i, [&](uint32_t k) {
             float gx, gy, gz;
             if (!ghost_delta(a.col[col0 + k], x.sx, x.sy, x.sz, x.h, gx, gy, gz)) return;
             const float gd = len3(gx, gy, gz);
@@ -582,7 +599,7 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, Sum &&
     // clamp
     const float r = fmaxd(pc.radius, 0.0f);
     float cx = p.x + ax, cy = p.y + ay, cz = p.z + az;
-    const uint32_t bw = a.bnd[i];
+    const uint32_t bw = pre.bnd;
     const uint32_t bcol0 = a.scenes[pc.scene].col0;
     for (int sweep = 0; sweep < 2; ++sweep) {
         bool changed = false;
@@ -617,7 +634,7 @@ __device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, co
     const int4 ci = a.cellc[i];
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return delta_particle<UNIFORM>(
-        i, i,
+        i, i, delta_pre<UNIFORM>(i, a),
         [&](const DeltaSelf &x, DeltaSum &d) {
             uint32_t inr = 0;
             walk_runs(ci, run_table(a), c, [&](uint32_t q) {
@@ -789,13 +806,11 @@ __device__ __forceinline__ int halo_cell(const TileState &ts, const int4 ci)
 // LDS-path loops
 // ---------------------------------------------------------------------------------------------
 
-// The density walk over the staged halo: tests every candidate in walk order and compacts the
-// listed ones (their halo offsets) into the particle's list, four candidates per step with
-// their LDS loads issued first.  Returns the untruncated neighbor count.  Staged particles with
-// zero inverse mass carry an unreachable position, so the distance test also rejects them.
-//
-// SUP: also compacts the candidates within the wider radius sqrt(ts2) into ls (at most kSuperSlots,
-// scount = their untruncated number); later iterations filter those instead of walking.
+// The density walk over the staged halo: tests every candidate in walk order and compacts those
+// within the radius (squared distance below thr2: the neighbor threshold t2, or ts2 for the
+// candidate list of a substep) as halo offsets into a list, four candidates per step with their
+// LDS loads issued first.  Returns the untruncated count.  Staged particles with zero inverse mass
+// carry an unreachable position, so the distance test also rejects them.
 //
 // Cells that no candidate inside can reach are left out of the walk (the order of the others is
 // unchanged): a candidate's position at the substep start lies in its cell and has moved by at most
@@ -867,9 +882,9 @@ struct ListOut {
 // One contiguous range [k, f1) of the walk, four candidates per step: the loads read through a
 // pointer with immediate offsets and may run up to three entries past f1 (the staged arrays carry
 // kTilePad spare entries), which the u < f1 - k test then drops.
-template <bool UNIFORM, bool SUP>
+template <bool UNIFORM>
 __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4 *lpos, const uint32_t *lcls,
-                                           const DensitySelf &x, const FluidArgs &a, ListOut &lo, ListOut &so)
+                                           const DensitySelf &x, float thr2, const FluidArgs &a, ListOut &lo)
 {
     const float4 *pp = lpos + k;
     const uint32_t *pc = lcls + k;
@@ -886,27 +901,20 @@ __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4
         for (uint32_t u = 0; u < 4; ++u) {
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-            bool cand = (SUP ? d2 < x.ts2 : d2 < x.t2) & (u == 0 || u < rem);
+            bool cand = (d2 < thr2) & (u == 0 || u < rem);
             if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
-            if (cand) {
-                if (SUP) {
-                    so.push(k + u);
-                    if (d2 < x.t2) lo.push(k + u);
-                } else {
-                    lo.push(k + u);
-                }
-            }
+            if (cand) lo.push(k + u);
         }
     }
 }
 
-template <bool UNIFORM, bool SUP>
+template <bool UNIFORM>
 __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint32_t fs, const float4 *lpos,
-                                              const uint32_t *lcls, const DensitySelf &x, const ListRef &li,
-                                              const FluidArgs &a, const SubstepConsts &c, const RowCull &rc,
-                                              const ListRef &ls = ListRef{nullptr, 0u}, uint32_t *scount = nullptr)
+                                              const uint32_t *lcls, const DensitySelf &x, float thr2,
+                                              const ListRef &li, uint32_t cap, const FluidArgs &a,
+                                              const RowCull &rc)
 {
-    ListOut lo(li, c.cap), so(ls, kSuperSlots);
+    ListOut lo(li, cap);
     const float gxl = gap_below(rc.ux, rc.cx), gxr = gap_above(rc.ux, rc.cx);
     // rows of three cells along x, z outer, y inner; the centre row (r = 4) is walked as the
     // ranges before and after the particle itself
@@ -921,13 +929,12 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
         const uint32_t f1 = hoff[h0 + (gxr * gxr + perp < rc.lim2 ? 3 : 2)];
         const uint32_t k0 = hoff[h0 + (gxl * gxl + perp < rc.lim2 ? 0 : 1)];
         if (r == 4) {
-            scan_range<UNIFORM, SUP>(k0, fs, lpos, lcls, x, a, lo, so);
-            scan_range<UNIFORM, SUP>(fs + 1, f1, lpos, lcls, x, a, lo, so);
+            scan_range<UNIFORM>(k0, fs, lpos, lcls, x, thr2, a, lo);
+            scan_range<UNIFORM>(fs + 1, f1, lpos, lcls, x, thr2, a, lo);
         } else {
-            scan_range<UNIFORM, SUP>(k0, f1, lpos, lcls, x, a, lo, so);
+            scan_range<UNIFORM>(k0, f1, lpos, lcls, x, thr2, a, lo);
         }
     }
-    if (SUP) *scount = so.n();
     return lo.n();
 }
 
@@ -939,13 +946,14 @@ __device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32
     for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li.entry(min(k + u, cnt - 1));
 }
 
-// A later iteration from the iteration-0 candidates (complete while the displacements stay under
-// the skin, checked by the caller): the in-radius test at the current positions, the listed
-// neighbors written to the list in candidate order (= walk order), and the density terms of the
-// first cap summed in the same pass.  A step's four candidates are loaded first; each term is then
+// The neighbors from the substep's candidate list (complete at iteration 0, and later while the
+// displacements stay under the skin, checked by the caller): the in-radius test at the current
+// positions, the listed neighbors written to the list in candidate order (= walk order), and the
+// density terms of the first cap summed in the same pass.  A step's four candidates are loaded first; each term is then
 // evaluated before the test decides whether it is added, so every lane runs the same instructions.
 // (Filtering first and summing over the written list afterwards measured slower.)  Returns the
 // untruncated neighbor count.
+template <bool TENSION>
 __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t scount, const float4 *lpos,
                                                   const DensitySelf &x, const ListRef &li, uint32_t cap,
                                                   DensitySum &d)
@@ -965,10 +973,10 @@ __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t sc
         list_group(ls, k + kTermStep, scount, en);
 #pragma unroll
         for (uint32_t u = 0; u < kTermStep; ++u) {
-            const DensityTerm t = density_term(x, pj[u]);
+            const DensityTerm t = density_term<TENSION>(x, pj[u]);
             const float dx = x.s.x - pj[u].x, dy = x.s.y - pj[u].y, dz = x.s.z - pj[u].z;
             if ((k + u < scount) & (dot3(dx, dy, dz, dx, dy, dz) < x.t2)) {
-                if (lo.a < lo.lim) density_add(d, x, t);
+                if (lo.a < lo.lim) density_add<TENSION>(d, x, t);
                 lo.push(e[u]);
             }
         }
@@ -1002,11 +1010,10 @@ __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, co
 // kernels
 // ---------------------------------------------------------------------------------------------
 // MODE: kDensPlain walks every iteration; with candidate lists, kDensBuild (a substep's first
-// iteration) walks the wider radius and writes them, kDensReuse (a later iteration) filters them in
-// every block whose particles' displacements allow it and lists the other blocks, and
-// kDensFallback walks the listed blocks.  Separate instantiations, so each carries only its own
-// registers (the filter pass has no walk).
-constexpr int kDensPlain = 0, kDensBuild = 1, kDensReuse = 2, kDensFallback = 3;
+// iteration) walks the wider radius into the candidate lists and takes the neighbors from them,
+// and kDensReuse (a later iteration) takes them from the candidate lists in every block whose
+// particles' displacements allow it and walks the other blocks.
+constexpr int kDensPlain = 0, kDensBuild = 1, kDensReuse = 2;
 template <bool UNIFORM, int MODE>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_density_tiled(
     FluidArgs a, SubstepConsts c)
@@ -1025,22 +1032,15 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     // largest |delta|^2 of the previous iteration: bounds how far any fluid particle has moved
     // from its iteration-0 position
     const float md2 = a.it == 0 ? 0.0f : __uint_as_float(a.iter_state[kStateMaxDisp + a.it - 1]);
-    // the iteration's first density launch resets the displacement word its delta pass raises
-    if (MODE != kDensFallback && blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
+    // the density launch resets the displacement word its delta pass raises
+    if (blockIdx.x == 0 && tid == 0) a.iter_state[kStateMaxDisp + a.it] = 0u;
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds * kQueueStride;
-    uint32_t *fb_count = a.iter_state + kStateFbCount + a.it;
-    uint32_t *fb_queue = a.iter_state + kStateFbQueue + a.it;
-    const uint32_t nfb = MODE == kDensFallback ? *fb_count : 0u;
     if (tid < kWgMetaWords) s_meta[tid] = 0u;  // ordered before any use by tile_begin's barrier
     if (tid == 0) s_hmax = 0u;
     BlockQueue bq(blockIdx.x);
     TileState ts;
-    auto next = [&]() -> uint32_t {
-        if (MODE != kDensFallback) return bq.next(queue, nblk);
-        const uint32_t t = atomicAdd(fb_queue, 1u);
-        return t < nfb ? a.fb_blk[t] : kInvalid;
-    };
+    auto next = [&]() -> uint32_t { return bq.next(queue, nblk); };
     while (tile_begin(next, &s_blk, hoff, hstart, ts, 0u, a, c)) {
         if (ts.skip) {
             __syncthreads();
@@ -1073,11 +1073,13 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             }
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
+            bool reuse = false;
             if (REUSE) {
                 // The candidates of iteration 0 (within h + skin of x*) hold particle i's list while
                 // D_i + D_j < skin for every j in its 27 cells (D: displacement since iteration 0);
                 // D_j is bounded by the halo's largest.  A block where some particle fails, or has
-                // no complete candidate list, goes to the walking launch whole.
+                // no complete candidate list, is walked whole (one decision per block keeps the
+                // walk out of the waves of the others).
                 const float hmax = __uint_as_float(s_hmax);
                 bool bad = false;
                 for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
@@ -1089,32 +1091,27 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                     bad |= !a.scenes[pc.scene].fluid_only || a.scnt[i] == kSuperNone ||
                            !(a.disp[i] + hmax < 0.98f * (kSkinFrac * h));
                 }
-                if (__syncthreads_or(bad ? 1 : 0)) {
-                    if (tid == 0) a.fb_blk[atomicAdd(fb_count, 1u)] = ts.blk;
-                    __syncthreads();
-                    if (tid == 0) s_hmax = 0u;
-                    continue;
-                }
+                reuse = !__syncthreads_or(bad ? 1 : 0);
             }
-            auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, uint32_t i, const ListRef &li) {
+            // the walk at the neighbor radius (thr2 = t2, radius h) or the candidate radius (ts2) into l
+            auto scan = [&](auto bu_tag, const DensitySelf &x, int4 ci, int hc, uint32_t fs, float thr2,
+                            float radius, const ListRef &l, uint32_t cap) {
                 constexpr bool BU = decltype(bu_tag)::value;
                 const float inv = c.inv_cell > 0.0f ? c.inv_cell : a.scenes[x.pc.scene].inv_cell;
                 // displacement since the cells were assigned: none at iteration 0; later the largest
                 // |delta| of the previous iteration, where only the fluid passes move fluid particles
                 const float dmax = a.it == 0 ? 0.0f : (a.scenes[x.pc.scene].fluid_only ? fsqrt(md2) : 3.4e38f);
-                const RowCull rc = row_cull(x.s, ci, inv, x.h, dmax);
-                if (MODE == kDensBuild && x.sup) {
-                    uint32_t sreq;
-                    const RowCull rs = row_cull(x.s, ci, inv, fsqrt(x.ts2), dmax);
-                    const uint32_t req = scan_rows<BU, true>(hoff, hc, fs, lpos, lcls, x, li, a, c, rs,
-                                                             list_ref(a.slist, i, kSuperSlots), &sreq);
-                    a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
-                    return req;
-                }
-                return scan_rows<BU, false>(hoff, hc, fs, lpos, lcls, x, li, a, c, rc);
+                const RowCull rc = row_cull(x.s, ci, inv, radius, dmax);
+                return scan_rows<BU>(hoff, hc, fs, lpos, lcls, x, thr2, l, cap, a, rc);
+            };
+            auto super = [&](const DensitySelf &x, const ListRef &ls, uint32_t scount, const ListRef &li,
+                             DensitySum &d) {
+                return x.tension ? density_super<true>(ls, scount, lpos, x, li, c.cap, d)
+                                 : density_super<false>(ls, scount, lpos, x, li, c.cap, d);
             };
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                 const int4 ci = a.cellc[i];
+                const uint32_t bw = a.bnd[i];  // loaded beside the cell, long before its use
                 const int hc = halo_cell(ts, ci);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 DensitySelf x;
@@ -1123,23 +1120,38 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                     DensitySum d;
                     const ListRef li = list_ref(a.list, i, c.cap);
                     if (density_walks(x, bu, c)) {
-                        if (REUSE) {
+                        bool done = false;
+                        if (REUSE && reuse) {
                             ++ma.reuse;
-                            d.req = density_super(list_ref(a.slist, i, kSuperSlots), a.scnt[i], lpos, x, li, c.cap, d);
-                        } else {
-                            d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, i, li)
-                                       : scan(std::false_type{}, x, ci, hc, fs, i, li);
+                            d.req = super(x, list_ref(a.slist, i, kSuperSlots), a.scnt[i], li, d);
+                            done = true;
+                        } else if (MODE == kDensBuild && x.sup) {
+                            // the candidates within h + skin, then the neighbors among them; a
+                            // particle with more candidates than the list holds walks for its
+                            // neighbors (and never reuses: its count is marked)
+                            const ListRef ls = list_ref(a.slist, i, kSuperSlots);
+                            const float rs = fsqrt(x.ts2);
+                            const uint32_t sreq = bu ? scan(std::true_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots)
+                                                     : scan(std::false_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots);
+                            a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
+                            if (sreq <= kSuperSlots) {
+                                d.req = super(x, ls, sreq, li, d);
+                                done = true;
+                            }
                         }
-                        d.inr = min(d.req, c.cap);
-                        if (!REUSE) {
+                        if (!done) {
+                            d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap)
+                                       : scan(std::false_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap);
+                            d.inr = min(d.req, c.cap);
                             if (x.tension)
                                 density_list<true>(li, d.inr, lpos, x, d);
                             else
                                 density_list<false>(li, d.inr, lpos, x, d);
                         }
+                        d.inr = min(d.req, c.cap);
                     }
                     cnt = d.inr;
-                    density_end(i, d, x, ma, a, c);
+                    density_end(i, bw, d, x, ma, a, c);
                 }
                 a.lcnt[i] = cnt;
             }
@@ -1231,16 +1243,18 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
-                const int hc = halo_cell(ts, a.cellc[i]);
+                const int4 ci = a.cellc[i];
+                const DeltaPre pre = delta_pre<UNIFORM>(i, a);
+                const uint32_t cnt = a.lcnt[i];
+                const int hc = halo_cell(ts, ci);
                 const uint32_t fs = hoff[hc] + (i - hstart[hc]);
                 const ListRef li = list_ref(a.list, i, c.cap);
                 const float d2 = delta_particle<UNIFORM>(
-                    i, fs,
+                    i, fs, pre,
                     [&](const DeltaSelf &x, DeltaSum &d) {
                         // the listed neighbors, kTermStep per step: loads first (the next group's
                         // entries while this group's terms are evaluated), terms independent, sums
                         // in list order
-                        const uint32_t cnt = a.lcnt[i];
                         uint32_t en[kTermStep];
                         if (cnt > 0) list_group(li, 0, cnt, en);
                         for (uint32_t k = 0; k < cnt; k += kTermStep) {
@@ -1604,7 +1618,6 @@ static FluidArgs fluid_args(const SceneBuffers &s, const SubstepConsts &c, uint3
     a.nblk = s.nblk;
     a.blk_halo = s.blk_halo;
     a.wide_blk = s.wide_blk;
-    a.fb_blk = s.fb_blk;
     a.iter_state = s.iter_state;
     a.classes = s.classes;
     a.fm = s.fluid_mats;
@@ -1665,15 +1678,10 @@ void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_
             else
                 hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensBuild>), grid, block, lds, st, a, c);
         } else if (c.superset) {
-            // the candidate-list pass, then the walk over the blocks it could not take
-            const dim3 fgrid(std::min<uint32_t>(persistent_grid(c), 256u));
-            if (c.uniform_class) {
+            if (c.uniform_class)
                 hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensReuse>), grid, block, lds, st, a, c);
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensFallback>), fgrid, block, lds, st, a, c);
-            } else {
+            else
                 hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensReuse>), grid, block, lds, st, a, c);
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensFallback>), fgrid, block, lds, st, a, c);
-            }
         } else {
             if (c.uniform_class)
                 hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensPlain>), grid, block, lds, st, a, c);

@@ -20,10 +20,7 @@ constexpr uint32_t kStateDensity = 0;
 constexpr uint32_t kStateDelta = kIterStateWords * kXcds * kQueueStride;
 constexpr uint32_t kStateWideQueue = 2 * kIterStateWords * kXcds * kQueueStride;
 constexpr uint32_t kStateWideCount = kStateWideQueue + kIterStateWords;
-// per iteration: blocks the candidate-list pass handed to the walking pass, and their queue
-constexpr uint32_t kStateFbQueue = kStateWideCount + 1;
-constexpr uint32_t kStateFbCount = kStateFbQueue + kIterStateWords;
-constexpr uint32_t kStateMaxDisp = kStateFbCount + kIterStateWords;
+constexpr uint32_t kStateMaxDisp = kStateWideCount + 1;
 constexpr uint32_t kStatePrevDisp = kStateMaxDisp + kIterStateWords;  // the last iteration's word of the
                                                                         // previous substep
 constexpr uint32_t kStateWords = kStatePrevDisp + 1;
@@ -47,7 +44,7 @@ constexpr uint32_t kMetaWords = 16;  // device diagnostics words
 // (skin = kSkinFrac h), reused by a later iteration of the substep while its own displacement since
 // iteration 0 plus the largest one in its block's halo stays below the skin
 constexpr uint32_t kSuperSlots = 96;
-constexpr float kSkinFrac = 0.1f;
+constexpr float kSkinFrac = 0.05f;
 constexpr uint16_t kSuperNone = 0xffffu;  // count marker: the candidates did not fit, walk instead
 // Per-slot lists of 16-bit entries are stored in groups of kListGroup consecutive slots: entry k
 // of slot i sits at (i / kListGroup) * kListGroup * rows + k * kListGroup + i % kListGroup, so one
@@ -136,7 +133,6 @@ struct SceneBuffers {
     uint32_t *blk_halo = nullptr;   // kHaloStride words per block, for the first blk_cap blocks
     uint32_t blk_cap = 0;
     uint32_t *wide_blk = nullptr;   // blocks whose halo fits the wide delta tile only (blk_cap)
-    uint32_t *fb_blk = nullptr;     // blocks whose candidate lists may be incomplete (blk_cap)
     uint32_t *iter_state = nullptr; // kStateWords: block-queue counters and max |delta|^2 per iteration
     float *fluid_t2 = nullptr;      // per fluid material: smallest d^2 whose correctly rounded sqrt >= h
     uint16_t *list = nullptr;       // fluid neighbor lists as halo offsets, in slot groups (list_row)

@@ -214,17 +214,17 @@ typedef struct cn_step_params {
     float cell_size;                 /* particle grid cell size; 0 -> derived like the reference */
     uint32_t max_fluid_neighbors;    /* neighbor slots per fluid particle (reference: 128) */
     uint32_t key_mode;               /* CN_KEY_MORTON or CN_KEY_HASH */
-    uint32_t flags;                  /* CN_STEP_* bits (cn_default_step_params: none) */
+    uint32_t flags;                  /* CN_STEP_* bits (cn_default_step_params: CN_STEP_CANDIDATE_REUSE) */
 } cn_step_params;
 
-/* flags: in fluid-only scenes the fluid iterations after a substep's first filter the candidate
- * lists built by its first iteration (neighbors within 1.1 h of the predicted positions) instead of
- * walking the 27 cells again, for every particle whose displacement since the first iteration plus
- * the largest one in its cell block's halo stays below 0.1 h; the others walk.  The lists are the
- * same either way (same neighbors, same order), so the results are too.  The candidate lists take
- * 198 bytes per particle, allocated on the first step that asks (left off when they would exceed
- * 4 GiB).  Off in cn_default_step_params: it measured no faster on a 4M-particle fluid and slower
- * on pools of small scenes, where many blocks fail the displacement check and are walked anyway. */
+/* flags: in fluid-only scenes the fluid iterations after a substep's first take the neighbors from
+ * the candidate lists its first iteration builds (neighbors within 1.05 h of the positions at the
+ * first iteration) instead of walking the 27 cells again, in every cell block whose particles'
+ * displacement since the first iteration plus the largest one in the block's halo stays below
+ * 0.05 h; the other blocks walk.  The lists are the same either way (same neighbors, same order),
+ * so the results are too.  The candidate lists take 198 bytes per particle, allocated on the first
+ * step that asks (left off when they would exceed 4 GiB less 1 MiB).  On in
+ * cn_default_step_params. */
 #define CN_STEP_CANDIDATE_REUSE 1u
 
 /* Overflow and diagnostic counters of the last cn_step (validateGpuMetaBlocking analog,
