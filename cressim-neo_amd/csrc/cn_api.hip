@@ -601,6 +601,7 @@ SubstepConsts make_consts(const cn_ctx *ctx, const SceneBuffers &s, const cn_ste
     c.morton_bits = ctx->morton_bits;
     c.bucket_shift = key_bits_of(ctx, p) - s.bucket_bits;
     c.uniform_class = s.nclasses == 1 ? 1u : 0u;
+    c.fluid_uniform = s.fluid_uniform;
     c.tension = ctx->has_tension ? 1u : 0u;
     c.tile_max = ctx->tile_max;
     c.blk_cap = s.blk_cap;
@@ -1253,6 +1254,15 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             CN_HIP(ctx, hipMemcpy(s.adj_idx, adj_idx.data(), adj_idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     s.nclasses = (uint32_t)classes.size();
+    {
+        // the fluid passes take a cell block's particle class from its first particle when no
+        // scene with fluid mixes classes (a block holds cells of one scene)
+        std::vector<uint32_t> per_scene(count, 0u);
+        for (const ParticleClass &pc : classes) ++per_scene[pc.scene];
+        s.fluid_uniform = 1u;
+        for (uint32_t k = 0; k < count; ++k)
+            if (sinfo[k].has_fluid && per_scene[k] != 1u) s.fluid_uniform = 0u;
+    }
     CN_HIP(ctx, dalloc(&s.classes, classes.size()));
     CN_HIP(ctx, hipMemcpy(s.classes, classes.data(), classes.size() * sizeof(ParticleClass), hipMemcpyHostToDevice));
     CN_HIP(ctx, dalloc(&s.fluid_mats, fmh.size()));

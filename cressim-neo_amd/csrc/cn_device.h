@@ -121,6 +121,8 @@ struct SubstepConsts {
     uint32_t morton_bits; // bits per axis of the Morton key
     uint32_t bucket_shift;
     uint32_t uniform_class;  // 1: every particle has class 0
+    uint32_t fluid_uniform;  // 1: every scene with fluid has one particle class (a cell block's class
+                             // is its first particle's)
     uint32_t tension;        // 1: some fluid material has surface tension
     uint32_t tile_max;       // halo particles a fluid-pass workgroup may stage in LDS (upper bound)
     uint32_t tile_dens;      // halo capacity of the density pass

@@ -388,12 +388,14 @@ __device__ __forceinline__ void density_end(uint32_t i, uint32_t bw, const Densi
 }
 
 // Run-table walk with the sums taken in the walk.
+// UNIFORM: every particle the walk meets has class ucls.
 template <bool UNIFORM>
-__device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c)
+__device__ __forceinline__ void density_global(uint32_t i, MetaAcc &ma, const FluidArgs &a, const SubstepConsts &c,
+                                               uint32_t ucls = 0u)
 {
-    if (!a.active[a.classes[UNIFORM ? 0u : a.cls[i]].scene]) return;
+    if (!a.active[a.classes[UNIFORM ? ucls : a.cls[i]].scene]) return;
     DensitySelf x;
-    if (!density_begin(x, i, a.cur[i], UNIFORM ? 0u : a.cls[i], a)) return;
+    if (!density_begin(x, i, a.cur[i], UNIFORM ? ucls : a.cls[i], a)) return;
     DensitySum d;
     if (density_walks(x, UNIFORM, c))
         walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
@@ -521,12 +523,12 @@ struct DeltaPre {
 };
 
 template <bool UNIFORM>
-__device__ __forceinline__ DeltaPre delta_pre(uint32_t i, const FluidArgs &a)
+__device__ __forceinline__ DeltaPre delta_pre(uint32_t i, const FluidArgs &a, uint32_t ucls = 0u)
 {
     DeltaPre r;
     r.p = a.pos[i];
     r.di = a.it > 0 ? a.din[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r.cls = UNIFORM ? 0u : a.cls[i];
+    r.cls = UNIFORM ? ucls : a.cls[i];
     r.bnd = a.bnd[i];
     return r;
 }
@@ -628,13 +630,14 @@ __device__ __forceinline__ float delta_particle(uint32_t i, uint32_t own, const 
 
 // Run-table walk: the list filter and the cap applied in the walk.
 template <bool UNIFORM>
-__device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, const SubstepConsts &c)
+__device__ __forceinline__ float delta_global(uint32_t i, const FluidArgs &a, const SubstepConsts &c,
+                                              uint32_t ucls = 0u)
 {
-    if (!a.active[a.classes[UNIFORM ? 0u : a.cls[i]].scene]) return 0.0f;
+    if (!a.active[a.classes[UNIFORM ? ucls : a.cls[i]].scene]) return 0.0f;
     const int4 ci = a.cellc[i];
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return delta_particle<UNIFORM>(
-        i, i, delta_pre<UNIFORM>(i, a),
+        i, i, delta_pre<UNIFORM>(i, a, ucls),
         [&](const DeltaSelf &x, DeltaSum &d) {
             uint32_t inr = 0;
             walk_runs(ci, run_table(a), c, [&](uint32_t q) {
@@ -739,6 +742,7 @@ constexpr int kTileBlock = 512;  // threads of a tiled fluid workgroup: one pass
 // CU, which their LDS budget (kDynLdsBytes) also allows
 constexpr int kTileWaves = 6;
 constexpr uint32_t kTermStep = 4;  // listed neighbors whose loads are issued together in the term loops
+constexpr uint32_t kDeltaStep = 4;  // the same in the delta pass
 constexpr uint32_t kTilePad = 4;   // spare entries after each staged density array (walk loads past a range)
 
 // Common head of the tiled passes: claim a block (next() runs on thread 0), decide LDS or run
@@ -940,10 +944,11 @@ __device__ __forceinline__ uint32_t scan_rows(const uint32_t *hoff, int hc, uint
 
 // List entries k .. k + kTermStep - 1 of a list of cnt >= 1 entries (the last one repeated past
 // the end, so the LDS reads they drive stay in range).
-__device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32_t cnt, uint32_t (&e)[kTermStep])
+template <uint32_t N>
+__device__ __forceinline__ void list_group(const ListRef &li, uint32_t k, uint32_t cnt, uint32_t (&e)[N])
 {
 #pragma unroll
-    for (uint32_t u = 0; u < kTermStep; ++u) e[u] = li.entry(min(k + u, cnt - 1));
+    for (uint32_t u = 0; u < N; ++u) e[u] = li.entry(min(k + u, cnt - 1));
 }
 
 // The neighbors from the substep's candidate list (complete at iteration 0, and later while the
@@ -1014,10 +1019,14 @@ __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, co
 // and kDensReuse (a later iteration) takes them from the candidate lists in every block whose
 // particles' displacements allow it and walks the other blocks.
 constexpr int kDensPlain = 0, kDensBuild = 1, kDensReuse = 2;
-template <bool UNIFORM, int MODE>
+// UCLS: the particle classes of a tiled fluid launch -- kClsMixed per particle, kClsOne every
+// particle class 0, kClsBlock one class per scene with fluid, read from the block's first particle.
+constexpr int kClsMixed = 0, kClsOne = 1, kClsBlock = 2;
+template <int UCLS, int MODE>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_density_tiled(
     FluidArgs a, SubstepConsts c)
 {
+    constexpr bool UNIFORM = UCLS != kClsMixed;
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
     __shared__ uint32_t hstart[kHalo];
@@ -1047,11 +1056,14 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             continue;
         }
         MetaAcc ma;
+        // the block's class (wave-uniform: the class and what derives from it sit in scalar
+        // registers)
+        const uint32_t bcls = UCLS == kClsBlock ? __builtin_amdgcn_readfirstlane(a.cls[ts.p0]) : 0u;
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
             // a halo of one particle class (the common case in a pool of single-material scenes)
             // takes the class filter out of the walk, as a pool of one class does
-            const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
+            const uint32_t ref = UNIFORM ? bcls : a.cls[ts.p0];
             bool mixed = false;
             uint32_t dmb = 0;  // displacement bounds are non-negative floats: their bits order them
             for (uint32_t f = tid; f < total; f += kTileBlock) {
@@ -1085,7 +1097,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                 for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
                     const int hc = halo_cell(ts, a.cellc[i]);
                     const uint32_t fs = hoff[hc] + (i - hstart[hc]);
-                    const ParticleClass pc = a.classes[UNIFORM ? 0u : lcls[fs]];
+                    const ParticleClass pc = a.classes[UNIFORM ? bcls : lcls[fs]];
                     if (pc.kind != CN_PARTICLE_KIND_FLUID || !(lpos[fs].w > kEps) || c.cap == 0u) continue;
                     const float h = fmaxd(a.fm[pc.fluid_mat].smoothing_radius, 1.0e-4f);
                     bad |= !a.scenes[pc.scene].fluid_only || a.scnt[i] == kSuperNone ||
@@ -1109,55 +1121,67 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                 return x.tension ? density_super<true>(ls, scount, lpos, x, li, c.cap, d)
                                  : density_super<false>(ls, scount, lpos, x, li, c.cap, d);
             };
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
-                const int4 ci = a.cellc[i];
-                const uint32_t bw = a.bnd[i];  // loaded beside the cell, long before its use
-                const int hc = halo_cell(ts, ci);
-                const uint32_t fs = hoff[hc] + (i - hstart[hc]);
-                DensitySelf x;
-                uint32_t cnt = 0;
-                if (density_begin(x, i, lpos[fs], UNIFORM ? 0u : lcls[fs], a, SUP)) {
-                    DensitySum d;
-                    const ListRef li = list_ref(a.list, i, c.cap);
-                    if (density_walks(x, bu, c)) {
-                        bool done = false;
-                        if (REUSE && reuse) {
-                            ++ma.reuse;
-                            d.req = super(x, list_ref(a.slist, i, kSuperSlots), a.scnt[i], li, d);
-                            done = true;
-                        } else if (MODE == kDensBuild && x.sup) {
-                            // the candidates within h + skin, then the neighbors among them; a
-                            // particle with more candidates than the list holds walks for its
-                            // neighbors (and never reuses: its count is marked)
-                            const ListRef ls = list_ref(a.slist, i, kSuperSlots);
-                            const float rs = fsqrt(x.ts2);
-                            const uint32_t sreq = bu ? scan(std::true_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots)
-                                                     : scan(std::false_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots);
-                            a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
-                            if (sreq <= kSuperSlots) {
-                                d.req = super(x, ls, sreq, li, d);
+            // BLK: the block's halo holds one particle class (bref, wave-uniform), so the class and
+            // everything the particle derives from it (material, radius, thresholds) load into
+            // scalar registers; a pool of single-material scenes takes this path in its general
+            // instantiation
+            const uint32_t bref = __builtin_amdgcn_readfirstlane(ref);
+            auto particles = [&](auto blk_tag) {
+                constexpr bool BLK = decltype(blk_tag)::value;
+                for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
+                    const int4 ci = a.cellc[i];
+                    const uint32_t bw = a.bnd[i];  // loaded beside the cell, long before its use
+                    const int hc = halo_cell(ts, ci);
+                    const uint32_t fs = hoff[hc] + (i - hstart[hc]);
+                    DensitySelf x;
+                    uint32_t cnt = 0;
+                    if (density_begin(x, i, lpos[fs], UNIFORM ? bcls : (BLK ? bref : lcls[fs]), a, SUP)) {
+                        DensitySum d;
+                        const ListRef li = list_ref(a.list, i, c.cap);
+                        if (density_walks(x, BLK || bu, c)) {
+                            bool done = false;
+                            if (REUSE && reuse) {
+                                ++ma.reuse;
+                                d.req = super(x, list_ref(a.slist, i, kSuperSlots), a.scnt[i], li, d);
                                 done = true;
+                            } else if (MODE == kDensBuild && x.sup) {
+                                // the candidates within h + skin, then the neighbors among them; a
+                                // particle with more candidates than the list holds walks for its
+                                // neighbors (and never reuses: its count is marked)
+                                const ListRef ls = list_ref(a.slist, i, kSuperSlots);
+                                const float rs = fsqrt(x.ts2);
+                                const uint32_t sreq = BLK || bu ? scan(std::true_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots)
+                                                                : scan(std::false_type{}, x, ci, hc, fs, x.ts2, rs, ls, kSuperSlots);
+                                a.scnt[i] = sreq <= kSuperSlots ? (uint16_t)sreq : kSuperNone;
+                                if (sreq <= kSuperSlots) {
+                                    d.req = super(x, ls, sreq, li, d);
+                                    done = true;
+                                }
                             }
-                        }
-                        if (!done) {
-                            d.req = bu ? scan(std::true_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap)
-                                       : scan(std::false_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap);
+                            if (!done) {
+                                d.req = BLK || bu ? scan(std::true_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap)
+                                                  : scan(std::false_type{}, x, ci, hc, fs, x.t2, x.h, li, c.cap);
+                                d.inr = min(d.req, c.cap);
+                                if (x.tension)
+                                    density_list<true>(li, d.inr, lpos, x, d);
+                                else
+                                    density_list<false>(li, d.inr, lpos, x, d);
+                            }
                             d.inr = min(d.req, c.cap);
-                            if (x.tension)
-                                density_list<true>(li, d.inr, lpos, x, d);
-                            else
-                                density_list<false>(li, d.inr, lpos, x, d);
                         }
-                        d.inr = min(d.req, c.cap);
+                        cnt = d.inr;
+                        density_end(i, bw, d, x, ma, a, c);
                     }
-                    cnt = d.inr;
-                    density_end(i, bw, d, x, ma, a, c);
+                    a.lcnt[i] = cnt;
                 }
-                a.lcnt[i] = cnt;
-            }
+            };
+            if (!UNIFORM && bu)
+                particles(std::true_type{});
+            else
+                particles(std::false_type{});
             if (tid == 0) ++ma.lds;
         } else {
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) density_global<UNIFORM>(i, ma, a, c);
+            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) density_global<UNIFORM>(i, ma, a, c, bcls);
             if (tid == 0) ++ma.runs;
         }
         meta_to_wg(ma, s_meta);
@@ -1170,10 +1194,11 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
 // WIDE = false: every block, the previous delta staged in LDS beside the records; blocks whose halo
 // only fits without it are skipped and left to the WIDE = true launch, which takes the listed wide
 // blocks and reads the previous delta from HBM through a staged slot map.
-template <bool UNIFORM, bool WIDE>
+template <int UCLS, bool WIDE>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTileWaves))) void k_fluid_delta_tiled(
     FluidArgs a, SubstepConsts c)
 {
+    constexpr bool UNIFORM = UCLS != kClsMixed;
     extern __shared__ float4 dyn[];
     __shared__ uint32_t hoff[kHalo + 1];
     __shared__ uint32_t hstart[kHalo];
@@ -1204,15 +1229,17 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             __syncthreads();
             continue;
         }
+        // the block's class
+        const uint32_t bcls = UCLS == kClsBlock ? __builtin_amdgcn_readfirstlane(a.cls[ts.p0]) : 0u;
         if (ts.lds) {
             const uint32_t total = hoff[kHalo];
-            const uint32_t ref = UNIFORM ? 0u : a.cls[ts.p0];
+            const uint32_t ref = UNIFORM ? bcls : a.cls[ts.p0];
             // the neighbors' previous deltas are read only by the viscosity and CFL terms: staged
             // only when one of the block's fluid particles has one of them on
             bool rel = false;
             if (!WIDE && use_d) {
                 if (UNIFORM) {
-                    const ParticleClass pc0 = a.classes[0];
+                    const ParticleClass pc0 = a.classes[bcls];
                     rel = pc0.kind == CN_PARTICLE_KIND_FLUID &&
                           (cfl_term_on(a.fm[pc0.fluid_mat], a, c) || visc_term_on(a.fm[pc0.fluid_mat], a, c));
                 } else {
@@ -1242,47 +1269,59 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             // one particle class in the halo: every listed neighbor has the particle's material
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
-            for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
-                const int4 ci = a.cellc[i];
-                const DeltaPre pre = delta_pre<UNIFORM>(i, a);
-                const uint32_t cnt = a.lcnt[i];
-                const int hc = halo_cell(ts, ci);
-                const uint32_t fs = hoff[hc] + (i - hstart[hc]);
-                const ListRef li = list_ref(a.list, i, c.cap);
-                const float d2 = delta_particle<UNIFORM>(
-                    i, fs, pre,
-                    [&](const DeltaSelf &x, DeltaSum &d) {
-                        // the listed neighbors, kTermStep per step: loads first (the next group's
-                        // entries while this group's terms are evaluated), terms independent, sums
-                        // in list order
-                        uint32_t en[kTermStep];
-                        if (cnt > 0) list_group(li, 0, cnt, en);
-                        for (uint32_t k = 0; k < cnt; k += kTermStep) {
-                            uint32_t e[kTermStep];
-#pragma unroll
-                            for (uint32_t u = 0; u < kTermStep; ++u) e[u] = en[u];
-                            list_group(li, k + kTermStep, cnt, en);
-                            float4 dq[kTermStep];
-#pragma unroll
-                            for (uint32_t u = 0; u < kTermStep; ++u)
-                                dq[u] = !x.need_rel ? zero : (WIDE ? a.din[lslot[e[u]]] : ldin[e[u]]);
-                            DeltaTerm t[kTermStep];
-#pragma unroll
-                            for (uint32_t u = 0; u < kTermStep; ++u)
-                                t[u] = bu ? delta_term<true>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero, 0u, a)
-                                          : delta_term<false>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
-                                                              lcls[e[u]], a);
-#pragma unroll
-                            for (uint32_t u = 0; u < kTermStep; ++u)
-                                if (k + u < cnt) delta_add(d, t[u]);
-                        }
-                    },
-                    lcs, lnc, a, c);
-                d2max = fmaxd(d2max, d2);
-            }
+            // BLK: one particle class in the halo (bref, wave-uniform): the class-derived values of
+            // each particle load into scalar registers (the general instantiation in a pool of
+            // single-material scenes)
+            const uint32_t bref = __builtin_amdgcn_readfirstlane(ref);
+            auto particles = [&](auto blk_tag) {
+                constexpr bool BLK = decltype(blk_tag)::value;
+                for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock) {
+                    const int4 ci = a.cellc[i];
+                    const DeltaPre pre = delta_pre<UNIFORM || BLK>(i, a, UNIFORM ? bcls : bref);
+                    const uint32_t cnt = a.lcnt[i];
+                    const int hc = halo_cell(ts, ci);
+                    const uint32_t fs = hoff[hc] + (i - hstart[hc]);
+                    const ListRef li = list_ref(a.list, i, c.cap);
+                    const float d2 = delta_particle<UNIFORM>(
+                        i, fs, pre,
+                        [&](const DeltaSelf &x, DeltaSum &d) {
+                            // the listed neighbors, kDeltaStep per step: loads first (the next group's
+                            // entries while this group's terms are evaluated), terms independent, sums
+                            // in list order
+                            uint32_t en[kDeltaStep];
+                            if (cnt > 0) list_group(li, 0, cnt, en);
+                            for (uint32_t k = 0; k < cnt; k += kDeltaStep) {
+                                uint32_t e[kDeltaStep];
+    #pragma unroll
+                                for (uint32_t u = 0; u < kDeltaStep; ++u) e[u] = en[u];
+                                list_group(li, k + kDeltaStep, cnt, en);
+                                float4 dq[kDeltaStep];
+    #pragma unroll
+                                for (uint32_t u = 0; u < kDeltaStep; ++u)
+                                    dq[u] = !x.need_rel ? zero : (WIDE ? a.din[lslot[e[u]]] : ldin[e[u]]);
+                                DeltaTerm t[kDeltaStep];
+    #pragma unroll
+                                for (uint32_t u = 0; u < kDeltaStep; ++u)
+                                    t[u] = BLK || bu
+                                               ? delta_term<true>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero, 0u, a)
+                                               : delta_term<false>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
+                                                                   lcls[e[u]], a);
+    #pragma unroll
+                                for (uint32_t u = 0; u < kDeltaStep; ++u)
+                                    if (k + u < cnt) delta_add(d, t[u]);
+                            }
+                        },
+                        lcs, lnc, a, c);
+                    d2max = fmaxd(d2max, d2);
+                }
+            };
+            if (!UNIFORM && bu)
+                particles(std::true_type{});
+            else
+                particles(std::false_type{});
         } else {
             for (uint32_t i = ts.p0 + tid; i < ts.p1; i += kTileBlock)
-                d2max = fmaxd(d2max, delta_global<UNIFORM>(i, a, c));
+                d2max = fmaxd(d2max, delta_global<UNIFORM>(i, a, c, bcls));
         }
         __syncthreads();
     }
@@ -1651,7 +1690,7 @@ static uint32_t persistent_grid(const SubstepConsts &c)
 // passes stage was staged (and listed in halo offsets) by the density pass.
 void fluid_tiles(SubstepConsts &c)
 {
-    const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
+    const uint32_t cls = c.fluid_uniform ? 0u : 4u, nc = c.tension ? 16u : 0u;
     c.tile_dens = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (16u + cls) - kTilePad);
     c.tile_delta = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (32u + nc + cls));
     c.tile_wide = std::min<uint32_t>(c.tile_max, kDynLdsBytes / (20u + nc + cls));
@@ -1664,29 +1703,37 @@ void launch_blk_halo(const SceneBuffers &s, const SubstepConsts &c, hipStream_t 
     hipLaunchKernelGGL(k_blk_halo, dim3(g), dim3(kBlock), 0, st, a, c);
 }
 
+// One of a tiled pass's class instantiations: one class in the launch, one per scene with fluid,
+// or classes per particle.
+template <void (*ONE)(FluidArgs, SubstepConsts), void (*BLOCK)(FluidArgs, SubstepConsts),
+          void (*MIXED)(FluidArgs, SubstepConsts)>
+static void launch_tiled(const SubstepConsts &c, dim3 grid, dim3 block, size_t lds, hipStream_t st, const FluidArgs &a)
+{
+    if (c.uniform_class)
+        hipLaunchKernelGGL(ONE, grid, block, lds, st, a, c);
+    else if (c.fluid_uniform)
+        hipLaunchKernelGGL(BLOCK, grid, block, lds, st, a, c);
+    else
+        hipLaunchKernelGGL(MIXED, grid, block, lds, st, a, c);
+}
+
 void launch_fluid_density(const SceneBuffers &s, const SubstepConsts &c, uint32_t it, int din, hipStream_t st)
 {
     FluidArgs a = fluid_args(s, c, it, din);
     if (tiled(c, it)) {
-        const uint32_t per = 16u + (c.uniform_class ? 0u : 4u);
+        const uint32_t per = 16u + (c.fluid_uniform ? 0u : 4u);
         a.tile = c.tile_dens;
         const size_t lds = (size_t)(a.tile + kTilePad) * per;
         const dim3 grid(persistent_grid(c)), block(kTileBlock);
         if (c.superset && it == 0) {
-            if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensBuild>), grid, block, lds, st, a, c);
-            else
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensBuild>), grid, block, lds, st, a, c);
+            launch_tiled<k_fluid_density_tiled<kClsOne, kDensBuild>, k_fluid_density_tiled<kClsBlock, kDensBuild>,
+                         k_fluid_density_tiled<kClsMixed, kDensBuild>>(c, grid, block, lds, st, a);
         } else if (c.superset) {
-            if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensReuse>), grid, block, lds, st, a, c);
-            else
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensReuse>), grid, block, lds, st, a, c);
+            launch_tiled<k_fluid_density_tiled<kClsOne, kDensReuse>, k_fluid_density_tiled<kClsBlock, kDensReuse>,
+                         k_fluid_density_tiled<kClsMixed, kDensReuse>>(c, grid, block, lds, st, a);
         } else {
-            if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_density_tiled<true, kDensPlain>), grid, block, lds, st, a, c);
-            else
-                hipLaunchKernelGGL((k_fluid_density_tiled<false, kDensPlain>), grid, block, lds, st, a, c);
+            launch_tiled<k_fluid_density_tiled<kClsOne, kDensPlain>, k_fluid_density_tiled<kClsBlock, kDensPlain>,
+                         k_fluid_density_tiled<kClsMixed, kDensPlain>>(c, grid, block, lds, st, a);
         }
     } else {
         if (c.uniform_class)
@@ -1700,23 +1747,19 @@ void launch_fluid_delta(const SceneBuffers &s, const SubstepConsts &c, uint32_t 
 {
     FluidArgs a = fluid_args(s, c, it, din);
     if (tiled(c, it)) {
-        const uint32_t cls = c.uniform_class ? 0u : 4u, nc = c.tension ? 16u : 0u;
+        const uint32_t cls = c.fluid_uniform ? 0u : 4u, nc = c.tension ? 16u : 0u;
         // first iteration: no previous delta, so the delta pass stages as much as the wide one
         a.tile = it > 0 ? c.tile_delta : c.tile_wide;
         size_t lds = (size_t)a.tile * (16u + (it > 0 ? 16u : 0u) + nc + cls);
         const dim3 grid(persistent_grid(c)), block(kTileBlock);
-        if (c.uniform_class)
-            hipLaunchKernelGGL((k_fluid_delta_tiled<true, false>), grid, block, lds, st, a, c);
-        else
-            hipLaunchKernelGGL((k_fluid_delta_tiled<false, false>), grid, block, lds, st, a, c);
+        launch_tiled<k_fluid_delta_tiled<kClsOne, false>, k_fluid_delta_tiled<kClsBlock, false>,
+                     k_fluid_delta_tiled<kClsMixed, false>>(c, grid, block, lds, st, a);
         if (it > 0 && c.tile_wide > c.tile_delta) {
             a.tile = c.tile_wide;
             lds = (size_t)a.tile * (20u + nc + cls);
             const dim3 wgrid(persistent_grid(c));
-            if (c.uniform_class)
-                hipLaunchKernelGGL((k_fluid_delta_tiled<true, true>), wgrid, block, lds, st, a, c);
-            else
-                hipLaunchKernelGGL((k_fluid_delta_tiled<false, true>), wgrid, block, lds, st, a, c);
+            launch_tiled<k_fluid_delta_tiled<kClsOne, true>, k_fluid_delta_tiled<kClsBlock, true>,
+                         k_fluid_delta_tiled<kClsMixed, true>>(c, wgrid, block, lds, st, a);
         }
     } else {
         if (c.uniform_class)

@@ -166,6 +166,7 @@ struct SceneBuffers {
     // tables
     ParticleClass *classes = nullptr;
     uint32_t nclasses = 0;
+    uint32_t fluid_uniform = 0;     // every scene with fluid has one particle class
     cn_fluid_material *fluid_mats = nullptr;
     float4 *part_mats = nullptr;
     Collider *colliders = nullptr;
