@@ -111,6 +111,7 @@ struct cn_ctx {
     struct Graph {
         cn_step_params params;
         int cur0, cur1;                  // slot-order parity before and after the step
+        bool all_active;                 // recorded for a step of every scene
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -1483,6 +1484,14 @@ namespace {
 
 // Enqueues one cn_step (every substep) on the context stream over the buffer set s (the pool's,
 // or a compact tick's view of the listed scenes); advances s's slot-order parity.
+// every scene of the context takes part in the step (the active flags last written are all set)
+bool all_active(const cn_ctx *ctx)
+{
+    for (uint32_t f : ctx->active_host)
+        if (!f) return false;
+    return true;
+}
+
 void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
 {
     const uint32_t substeps = std::max<uint32_t>(p->substeps, 1u);
@@ -1544,6 +1553,7 @@ void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
         sa.inc_t_start = s.inc + s.inc_off[4];
         sa.active = s.scene_active;
         sa.pid_scene = s.pid_scene;
+        sa.all_live = all_active(ctx) ? 1u : 0u;
     }
     for (uint32_t sub = 0; sub < substeps; ++sub) {
         broadphase(ctx, s, c, key_bits);
@@ -1903,8 +1913,9 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
         ctx->graph_valid = true;
     }
     const int cur0 = ctx->s.cur;
+    const bool every = all_active(ctx);
     for (auto &g : ctx->graphs)
-        if (g.cur0 == cur0 && std::memcmp(&g.params, p, sizeof(*p)) == 0) {
+        if (g.cur0 == cur0 && g.all_active == every && std::memcmp(&g.params, p, sizeof(*p)) == 0) {
             CN_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
             ctx->s.cur = g.cur1;
             ctx->stepped = true;
@@ -1914,6 +1925,7 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
     cn_ctx::Graph g{};
     g.params = *p;
     g.cur0 = cur0;
+    g.all_active = every;
     CN_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     record_step(ctx, ctx->s, p);
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &g.graph);

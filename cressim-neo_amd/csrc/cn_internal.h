@@ -263,6 +263,7 @@ struct SoftArgs {
     const uint32_t *inc_t_start;
     const uint32_t *active;    // per scene
     const uint32_t *pid_scene; // particle index -> scene
+    uint32_t all_live;         // every scene takes part in the step (no per-particle check)
 };
 void launch_soft_clear(const SoftArgs &a, hipStream_t st);
 // gather / scatter: copy the constrained particles' positions from slot order before the solves and

@@ -29,8 +29,12 @@ __device__ __forceinline__ CorrRec rec3(float x, float y, float z) { return Corr
 
 __device__ __forceinline__ float4 ldp(const SoftArgs &a, uint32_t particle) { return a.posi[particle]; }
 
-// Constraints and particles of scenes that this step leaves untouched are skipped (records kept).
-__device__ __forceinline__ bool live(const SoftArgs &a, uint32_t particle) { return a.active[a.pid_scene[particle]] != 0u; }
+// Constraints and particles of scenes that this step leaves untouched are skipped (records kept);
+// a step of every scene skips the two dependent loads of the check.
+__device__ __forceinline__ bool live(const SoftArgs &a, uint32_t particle)
+{
+    return a.all_live || a.active[a.pid_scene[particle]] != 0u;
+}
 
 __device__ __forceinline__ void cross3d(float ax, float ay, float az, float bx, float by, float bz, float &ox, float &oy,
                                         float &oz)
