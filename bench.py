@@ -420,11 +420,13 @@ def main():
     achieved = bytes_pp[dom] * sc.n / (avg[dom] * 1e-3) / 1e9
     sub_ms = ms_per_step / SUBSTEPS
     sub_ach = bytes_pp["substep"] * sc.n / (sub_ms * 1e-3) / 1e9
+    # measured HBM bytes per launch of the dominant pass (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+    # the copy probe), from the committed summary of the same workload
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_fluid_density_large_pbf_4m.json")
+    tfile = os.path.join(ROOT, "profiles", "traffic_fluid_passes_large_pbf_4m.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
+            traffic = json.load(open(tfile))["passes"][dom]["bytes_per_launch"]
         except Exception:
             traffic = None
     out = {
