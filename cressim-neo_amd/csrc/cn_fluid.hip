@@ -1033,6 +1033,8 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     __shared__ uint32_t s_blk;
     __shared__ uint32_t s_meta[kWgMetaWords];
     __shared__ uint32_t s_hmax;  // largest displacement bound in the staged halo (float bits)
+    __shared__ uint32_t s_omax;  // the same over the block's own particles with mass (one class)
+    __shared__ uint32_t s_obad;  // one of those has no complete candidate list (one class)
     constexpr bool SUP = MODE == kDensBuild || MODE == kDensReuse;  // candidate lists apply
     constexpr bool REUSE = MODE == kDensReuse;
     float4 *lpos = dyn;
@@ -1046,7 +1048,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
     const uint32_t nblk = a.nblk[1];  // the blocks of scenes with fluid
     uint32_t *queue = a.iter_state + kStateDensity + a.it * kXcds * kQueueStride;
     if (tid < kWgMetaWords) s_meta[tid] = 0u;  // ordered before any use by tile_begin's barrier
-    if (tid == 0) s_hmax = 0u;
+    if (tid == 0) s_hmax = s_omax = s_obad = 0u;
     BlockQueue bq(blockIdx.x);
     TileState ts;
     auto next = [&]() -> uint32_t { return bq.next(queue, nblk); };
@@ -1064,15 +1066,24 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             // a halo of one particle class (the common case in a pool of single-material scenes)
             // takes the class filter out of the walk, as a pool of one class does
             const uint32_t ref = UNIFORM ? bcls : a.cls[ts.p0];
-            bool mixed = false;
-            uint32_t dmb = 0;  // displacement bounds are non-negative floats: their bits order them
+            bool mixed = false, obad = false;
+            uint32_t dmb = 0, dmo = 0;  // displacement bounds are non-negative floats: their bits order them
             for (uint32_t f = tid; f < total; f += kTileBlock) {
                 const uint32_t hc = halo_cell_of(hoff, f);
                 const uint32_t q = hstart[hc] + (f - hoff[hc]);
                 float4 v = a.cur[q];
-                if (!(v.w > kEps)) v.x = v.y = v.z = kFarAway;
+                const bool mass = v.w > kEps;
+                if (!mass) v.x = v.y = v.z = kFarAway;
                 lpos[f] = v;
-                if (REUSE) dmb = max(dmb, __float_as_uint(a.disp[q]));
+                if (REUSE) {
+                    const uint32_t dq = __float_as_uint(a.disp[q]);
+                    dmb = max(dmb, dq);
+                    // one class: the block's own particles' part of the reuse check rides along
+                    if (UNIFORM && mass && q >= ts.p0 && q < ts.p1) {
+                        dmo = max(dmo, dq);
+                        obad |= a.scnt[q] == kSuperNone;
+                    }
+                }
                 if (!UNIFORM) {
                     const uint32_t qc = a.cls[q];
                     lcls[f] = qc;
@@ -1082,11 +1093,25 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
             if (REUSE) {
                 dmb = wave_max(dmb);
                 if ((tid & 63u) == 0u) atomicMax(&s_hmax, dmb);
+                if (UNIFORM) {
+                    dmo = wave_max(dmo);
+                    if ((tid & 63u) == 0u) atomicMax(&s_omax, dmo);
+                    if (obad) s_obad = 1u;
+                }
             }
             const int any_mixed = __syncthreads_or(mixed ? 1 : 0);  // also the barrier after staging
             const bool bu = UNIFORM || !any_mixed;
             bool reuse = false;
-            if (REUSE) {
+            if (REUSE && UNIFORM) {
+                // One class: the check below for every particle at once, from the staging loop.
+                // D_i + hmax < allow holds for every particle with mass exactly when it holds for
+                // their largest D_i (rounded addition is monotone).
+                const ParticleClass pc = a.classes[bcls];
+                const float h = fmaxd(a.fm[pc.fluid_mat].smoothing_radius, 1.0e-4f);
+                reuse = pc.kind != CN_PARTICLE_KIND_FLUID || c.cap == 0u ||
+                        (a.scenes[pc.scene].fluid_only && s_obad == 0u &&
+                         __uint_as_float(s_omax) + __uint_as_float(s_hmax) < 0.98f * (kSkinFrac * h));
+            } else if (REUSE) {
                 // The candidates of iteration 0 (within h + skin of x*) hold particle i's list while
                 // D_i + D_j < skin for every j in its 27 cells (D: displacement since iteration 0);
                 // D_j is bounded by the halo's largest.  A block where some particle fails, or has
@@ -1186,7 +1211,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
         }
         meta_to_wg(ma, s_meta);
         __syncthreads();
-        if (tid == 0) s_hmax = 0u;  // read again only after the next block's staging barrier
+        if (tid == 0) s_hmax = s_omax = s_obad = 0u;  // read again only after the next block's staging barrier
     }
     flush_wg_meta(s_meta, a);
 }
