@@ -280,6 +280,9 @@ __device__ __forceinline__ float spiky_w_om(float d, float h, float omq, float k
     if (d >= h) return 0.0f;
     return __fmul_rn(__fmul_rn(k, omq), omq);
 }
+// The same for a pair known to lie within h (a listed neighbor: its squared distance is below the
+// threshold whose rounded root is h), where the range test is always false.
+__device__ __forceinline__ float spiky_w_om_in(float omq, float k) { return __fmul_rn(__fmul_rn(k, omq), omq); }
 __device__ __forceinline__ float cohesion_q(float d, float h, float q, float c1, float c2)
 {
     if (d <= kEps || d >= h) return 0.0f;

@@ -235,7 +235,9 @@ __device__ __forceinline__ bool quot_fast_ok(float h, float sx, float sy, float 
     return h <= 1024.0f && fminf(fminf(fabsf(sx), fabsf(sy)), fabsf(sz)) >= 0x1p-62f;
 }
 
-template <bool FAST>
+// LIVE: the caller adds the pair's terms only for eps < dist < h (a listed neighbor that is not
+// coincident), so the zero the derivative takes outside that range is never read.
+template <bool FAST, bool LIVE = false>
 __device__ __forceinline__ void pair_quot(PairQuot &p, float dx, float dy, float dz, float dist, float h, float ih,
                                           float k)
 {
@@ -255,7 +257,7 @@ __device__ __forceinline__ void pair_quot(PairQuot &p, float dx, float dy, float
     p.omq = __fadd_rn(1.0f, nq);
     const float num = __fmul_rn(__fmul_rn(2.0f, k), p.omq);
     const float der = FAST ? fdiv_rcp_neg(num, h, ih) : -fdiv(num, h);
-    p.der = (dist <= kEps || dist >= h) ? 0.0f : der;
+    p.der = LIVE ? der : ((dist <= kEps || dist >= h) ? 0.0f : der);
 }
 
 struct DensitySelf {
@@ -333,7 +335,8 @@ __device__ __forceinline__ DensityTerm density_term(const DensitySelf &x, const 
     const float dist = fsqrt_nearzero_free(dot3(dx, dy, dz, dx, dy, dz));
     float omq = __fadd_rn(1.0f, fdiv_rcp_neg(dist, x.h, x.ih));  // 1 - dist / h
     if (!x.fd) omq = __fsub_rn(1.0f, fdiv(dist, x.h));
-    t.w = spiky_w_om(dist, x.h, omq, x.kw);
+    // every term the callers add is a neighbor's (dist < h): the kernel's range test is left out
+    t.w = spiky_w_om_in(omq, x.kw);
     t.ngx = t.ngy = t.ngz = 0.0f;
     if (TENSION && x.tension && !(dist <= kEps || dist >= x.h)) {
         // FluidSpikyKernelGradient, physics_fluid_common.hlsli:44-53, negated: der * -(d / dist)
@@ -453,8 +456,9 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
     const float dist = fsqrt_nearzero_free(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
     t.live = !(dist <= kEps);
     PairQuot p;
-    pair_quot<true>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
-    if (!x.fd) pair_quot<false>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
+    // a listed neighbor (dist < h) whose term counts only when live (dist > eps)
+    pair_quot<true, true>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
+    if (!x.fd) pair_quot<false, true>(p, ddx, ddy, ddz, dist, x.h, x.ih, x.kw);
     // sc * n with sc = -0.5 (Ci + Cj) der, as (0.5 (Ci + Cj) der) * -n
     const float msc = 0.5f * (x.Ci + cq.w) * p.der;
     float pdx = msc * p.nnx, pdy = msc * p.nny, pdz = msc * p.nnz;
@@ -475,7 +479,7 @@ __device__ __forceinline__ DeltaTerm delta_term(const DeltaSelf &x, const float4
             pdz = pdz + p.nnz * ck;
         }
         if (x.visc_on) {
-            const float vs = 1.0f - frcp_exact_ge1(1.0f + x.viscosity * spiky_w_om(dist, x.h, p.omq, x.kw));
+            const float vs = 1.0f - frcp_exact_ge1(1.0f + x.viscosity * spiky_w_om_in(p.omq, x.kw));
             pdx = pdx - vs * rx;
             pdy = pdy - vs * ry;
             pdz = pdz - vs * rz;
