@@ -867,6 +867,22 @@ __device__ __forceinline__ ListRef list_ref(uint16_t *buf, uint32_t i, uint32_t 
     return ListRef{reinterpret_cast<char *>(buf), (uint32_t)list_entry_base(i, rows)};
 }
 
+// List entries are byte offsets into a staged array of 16-byte records (halo index x 16; the tile
+// holds at most 4,096 halo particles, so they fit 16 bits), so the LDS address of a listed
+// neighbor's record needs no scaling; staged arrays of 4-byte items take the entry / 4.
+constexpr uint32_t kEntryScale = 16;
+static_assert(4096u * kEntryScale <= 65536u, "a byte-offset entry of the largest tile fits 16 bits");
+template <class T>
+__device__ __forceinline__ const T &at16(const T *base, uint32_t e)
+{
+    static_assert(sizeof(T) == 16, "16-byte records");
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + e);
+}
+__device__ __forceinline__ uint32_t at4(const uint32_t *base, uint32_t e)
+{
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + (e >> 2));
+}
+
 // Appends listed halo offsets to a particle's list, storing the first cap.  The write position is
 // the byte offset of the next entry and advances one entry stride per push, so a push is a compare,
 // a store and an add; n() is the untruncated count.  The position may run past entry cap by one
@@ -896,7 +912,8 @@ __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4
 {
     const float4 *pp = lpos + k;
     const uint32_t *pc = lcls + k;
-    for (; k < f1; k += 4, pp += 4, pc += 4) {
+    uint32_t kb = k * kEntryScale;  // the entry of candidate k
+    for (; k < f1; k += 4, kb += 4 * kEntryScale, pp += 4, pc += 4) {
         float4 pj[4];
         uint32_t qc[4];
 #pragma unroll
@@ -911,7 +928,7 @@ __device__ __forceinline__ void scan_range(uint32_t k, uint32_t f1, const float4
             const float d2 = dot3(dx, dy, dz, dx, dy, dz);
             bool cand = (d2 < thr2) & (u == 0 || u < rem);
             if (!UNIFORM) cand = cand && neighbor_ok(x.pc, a.classes[qc[u]]);
-            if (cand) lo.push(k + u);
+            if (cand) lo.push(kb + u * kEntryScale);
         }
     }
 }
@@ -976,7 +993,7 @@ __device__ __forceinline__ uint32_t density_super(const ListRef &ls, uint32_t sc
 #pragma unroll
         for (uint32_t u = 0; u < kTermStep; ++u) {
             e[u] = en[u];
-            pj[u] = lpos[e[u]];
+            pj[u] = at16(lpos, e[u]);
         }
         // the next step's entries while this step's terms are evaluated
         list_group(ls, k + kTermStep, scount, en);
@@ -1008,7 +1025,7 @@ __device__ __forceinline__ void density_list(const ListRef &li, uint32_t cnt, co
         list_group(li, k + kTermStep, cnt, en);
         DensityTerm t[kTermStep];
 #pragma unroll
-        for (uint32_t u = 0; u < kTermStep; ++u) t[u] = density_term<TENSION>(x, lpos[e[u]]);
+        for (uint32_t u = 0; u < kTermStep; ++u) t[u] = density_term<TENSION>(x, at16(lpos, e[u]));
 #pragma unroll
         for (uint32_t u = 0; u < kTermStep; ++u)
             if (k + u < cnt) density_add<TENSION>(d, x, t[u]);
@@ -1327,14 +1344,15 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTil
                                 float4 dq[kDeltaStep];
     #pragma unroll
                                 for (uint32_t u = 0; u < kDeltaStep; ++u)
-                                    dq[u] = !x.need_rel ? zero : (WIDE ? a.din[lslot[e[u]]] : ldin[e[u]]);
+                                    dq[u] = !x.need_rel ? zero : (WIDE ? a.din[at4(lslot, e[u])] : at16(ldin, e[u]));
                                 DeltaTerm t[kDeltaStep];
     #pragma unroll
                                 for (uint32_t u = 0; u < kDeltaStep; ++u)
                                     t[u] = BLK || bu
-                                               ? delta_term<true>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero, 0u, a)
-                                               : delta_term<false>(x, lcs[e[u]], dq[u], x.tension ? lnc[e[u]] : zero,
-                                                                   lcls[e[u]], a);
+                                               ? delta_term<true>(x, at16(lcs, e[u]), dq[u],
+                                                                  x.tension ? at16(lnc, e[u]) : zero, 0u, a)
+                                               : delta_term<false>(x, at16(lcs, e[u]), dq[u],
+                                                                   x.tension ? at16(lnc, e[u]) : zero, at4(lcls, e[u]), a);
     #pragma unroll
                                 for (uint32_t u = 0; u < kDeltaStep; ++u)
                                     if (k + u < cnt) delta_add(d, t[u]);
