@@ -139,10 +139,17 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     for _ in range(warmup):
         pool.step(p)
     pool.synchronize()
-    # throughput: the steps issued back to back and waited for at the end, as the plain loops below
+    # throughput: the steps issued back to back and waited for at the end, as the plain loops below;
+    # one wait after the first loop_steps of them times the window the loops are timed on (the
+    # scenes grow costlier as their fluid collapses, so the comparison takes the same steps)
+    wsteps = min(loop_steps, steps)
+    t_win = None
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         pool.step(p)
+        if k + 1 == wsteps:
+            pool.synchronize()
+            t_win = time.perf_counter() - t0
     pool.synchronize()
     el = time.perf_counter() - t0
     trunc = pool.meta().fluid_neighbor_truncations
@@ -170,10 +177,19 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
     for sc in batch:
         s = cnhip.Solver(0)
         s.reserve(sc.n, CAP)
-        s.upload(sc)
         ctxs.append(s)
 
-    def loop(wait: bool):
+    def loop(wait: bool, graphs: bool):
+        # every loop from the scenes' initial state, warmed up as the pool was, then timed on the
+        # same steps as the pool's window
+        for s, sc in zip(ctxs, batch):
+            s.upload(sc)
+            s.graphs(graphs)
+        for _ in range(warmup):
+            for s in ctxs:
+                s.step(p)
+        for s in ctxs:
+            s.synchronize()
         t0 = time.perf_counter()
         for _ in range(loop_steps):
             for s in ctxs:
@@ -184,30 +200,23 @@ def batch_bench(count: int, steps: int, warmup: int, loop_steps: int, tune: bool
             s.synchronize()
         return count * loop_steps / (time.perf_counter() - t0)
 
-    for s in ctxs:
-        s.step(p)
-    for s in ctxs:
-        s.synchronize()
     # each scene's step issued in turn, the contexts' streams overlapping on the device
-    env_loop = loop(False)
+    env_loop = loop(False, False)
     # the same loop with each scene's step waited for before the next scene's (a blocking step call)
-    env_sync = loop(True)
-    # the overlapping loop with every context's step graph-captured (captured by the first step)
-    for s in ctxs:
-        s.graphs(True)
-        s.step(p)
-    for s in ctxs:
-        s.synchronize()
-    env_graph = loop(False)
+    env_sync = loop(True, False)
+    # the overlapping loop with every context's step graph-captured (captured in the warmup)
+    env_graph = loop(False, True)
     for s in ctxs:
         s.close()
     best = max(env_loop, env_graph)
+    env_win = count * wsteps / t_win
     print(f"bench: plain loops {env_loop:.0f} (overlapping) / {env_graph:.0f} (graphs) / {env_sync:.0f} (waiting) "
-          f"env-steps/s", file=sys.stderr, flush=True)
-    out.update(loop_plain_steps=loop_steps, loop_plain_env_steps_per_s=env_loop,
+          f"env-steps/s; the pool {env_win:.0f} on the same steps", file=sys.stderr, flush=True)
+    out.update(loop_plain_steps=loop_steps, loop_window=f"steps {warmup + 1}-{warmup + loop_steps}",
+               pool_window_env_steps_per_s=env_win, loop_plain_env_steps_per_s=env_loop,
                loop_plain_graph_env_steps_per_s=env_graph, loop_plain_sync_env_steps_per_s=env_sync,
-               speedup_vs_loop_plain=env / best, speedup_vs_loop_plain_overlap=env / env_loop,
-               speedup_vs_loop_plain_graph=env / env_graph, speedup_vs_loop_plain_sync=env / env_sync)
+               speedup_vs_loop_plain=env_win / best, speedup_vs_loop_plain_overlap=env_win / env_loop,
+               speedup_vs_loop_plain_graph=env_win / env_graph, speedup_vs_loop_plain_sync=env_win / env_sync)
     return out
 
 
@@ -293,7 +302,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
-    ap.add_argument("--loop-steps", type=int, default=10)
+    ap.add_argument("--loop-steps", type=int, default=20)
     ap.add_argument("--late-steps", type=int, default=20,
                     help="steps timed in the later window (steps 26-45 of the 4M workload); 0 skips it")
     ap.add_argument("--no-serve", action="store_true", help="leave the served leg out of the default line")
