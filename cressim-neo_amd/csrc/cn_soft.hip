@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_soft_clear(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 {
-    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t e = die_block() * kBlock + threadIdx.x;
     if (e >= a.ne) return;
     const EdgeDev E = a.edges[e];
     if (!live(a, E.a)) return;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 {
-    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t b = die_block() * kBlock + threadIdx.x;
     if (b >= a.nb) return;
     const BendDev B = a.bends[b];
     if (!live(a, B.p0)) return;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
 {
-    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t d = die_block() * kBlock + threadIdx.x;
     if (d >= a.nd) return;
     const uint32_t ci = a.nb + d;
     const DihedralDev D = a.dih[d];
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
 {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = die_block() * kBlock + threadIdx.x;
     if (t >= a.nt) return;
     const TetDev T = a.tets[t];
     if (!live(a, T.p[0])) return;
@@ -256,13 +256,14 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     out[3] = rec3(f3 * g3x * r, f3 * g3y * r, f3 * g3z * r);
 }
 
-// Per particle, in particle-index order: its incident records, stored contiguously in incidence
+// Per constrained particle, in particle-index order: its incident records, stored contiguously in incidence
 // order, summed in that order, then added to its position.
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
                                                               const CorrRec *__restrict__ rec)
 {
-    const uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
-    if (pid >= a.n) return;
+    const uint32_t j = die_block() * kBlock + threadIdx.x;
+    if (j >= a.nsoft) return;
+    const uint32_t pid = a.soft_ids[j];
     const uint32_t k0 = start[pid], k1 = start[pid + 1];
     if (k0 == k1 || !live(a, pid)) return;
     float4 p = a.posi[pid];
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
 // constraints' vertices are neighbors in memory: gathered from slot order before it, put back after.
 __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
 {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = die_block() * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     if (!live(a, pid)) return;  // a scene left out of the step may not be in the slots stepped
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_soft_scatter(SoftArgs a)
 {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = die_block() * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     if (!live(a, pid)) return;
@@ -320,16 +321,20 @@ void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, bool so
         if (a.nsoft && scatter) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
         return;
     }
-    if (a.ne) hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_e_start,
-                       a.corr_e);
+    // the applies run over the constrained particles; a group without constraints has nothing to apply
+    const uint32_t g = grid_n(a.nsoft);
+    if (a.ne) {
+        hipLaunchKernelGGL(k_solve_edges, dim3(grid_n(a.ne)), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL(k_apply_corrections, dim3(g), dim3(kBlock), 0, st, a, a.inc_e_start, a.corr_e);
+    }
     if (a.nb) hipLaunchKernelGGL(k_solve_bends, dim3(grid_n(a.nb)), dim3(kBlock), 0, st, a);
     if (a.nd) hipLaunchKernelGGL(k_solve_dihedrals, dim3(grid_n(a.nd)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_b_start,
-                       a.corr_b);
-    if (a.nt) hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
-    hipLaunchKernelGGL(k_apply_corrections, dim3(grid_n(a.n)), dim3(kBlock), 0, st, a, a.inc_t_start,
-                       a.corr_t);
+    if (a.nb + a.nd)
+        hipLaunchKernelGGL(k_apply_corrections, dim3(g), dim3(kBlock), 0, st, a, a.inc_b_start, a.corr_b);
+    if (a.nt) {
+        hipLaunchKernelGGL(k_solve_tets, dim3(grid_n(a.nt)), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL(k_apply_corrections, dim3(g), dim3(kBlock), 0, st, a, a.inc_t_start, a.corr_t);
+    }
     if (a.nsoft && scatter) hipLaunchKernelGGL(k_soft_scatter, dim3(grid_n(a.nsoft)), dim3(kBlock), 0, st, a);
 }
 
