@@ -539,7 +539,7 @@ struct RigidPidArgs {
 
 __global__ __launch_bounds__(kBlock) void k_rigid_pid(RigidPidArgs a, SubstepConsts c)
 {
-    const uint32_t j = die_block() * kBlock + threadIdx.x;
+    const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     const uint32_t scene = a.pid_scene[pid];

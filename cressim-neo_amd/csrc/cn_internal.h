@@ -10,15 +10,6 @@ namespace cn {
 
 constexpr uint32_t kIterStateWords = 64;  // iterations with tiled fluid passes (later ones walk the run table)
 constexpr uint32_t kXcds = 8;             // accelerator dies; each has its own L2 and block-queue counter
-// Workgroups are dispatched to the dies round robin (workgroup i to die i mod kXcds), each die with
-// its own L2.  The renumbered index gives each die one contiguous share of the grid, in order, so
-// that neighboring work items (whose scattered writes and gathers meet in the same lines) share an
-// L2.  A bijection of [0, gridDim.x).
-__device__ __forceinline__ uint32_t die_block()
-{
-    const uint32_t b = blockIdx.x, n = gridDim.x, per = n / kXcds, rem = n % kXcds, x = b % kXcds;
-    return x * per + min(x, rem) + b / kXcds;
-}
 // iter_state words: block-queue counters of the density and delta passes (kXcds per iteration,
 // each on a 128-byte line of its own: the workgroups of every die pop their blocks with atomics
 // at the memory side, which serialise per line), the queue counter of the wide-tile delta pass per

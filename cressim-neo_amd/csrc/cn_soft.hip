@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_soft_clear(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 {
-    const uint32_t e = die_block() * kBlock + threadIdx.x;
+    const uint32_t e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (e >= a.ne) return;
     const EdgeDev E = a.edges[e];
     if (!live(a, E.a)) return;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 {
-    const uint32_t b = die_block() * kBlock + threadIdx.x;
+    const uint32_t b = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (b >= a.nb) return;
     const BendDev B = a.bends[b];
     if (!live(a, B.p0)) return;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
 {
-    const uint32_t d = die_block() * kBlock + threadIdx.x;
+    const uint32_t d = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (d >= a.nd) return;
     const uint32_t ci = a.nb + d;
     const DihedralDev D = a.dih[d];
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
 {
-    const uint32_t t = die_block() * kBlock + threadIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= a.nt) return;
     const TetDev T = a.tets[t];
     if (!live(a, T.p[0])) return;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
 __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const uint32_t *__restrict__ start,
                                                               const CorrRec *__restrict__ rec)
 {
-    const uint32_t j = die_block() * kBlock + threadIdx.x;
+    const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     const uint32_t k0 = start[pid], k1 = start[pid + 1];
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_corrections(SoftArgs a, const 
 // constraints' vertices are neighbors in memory: gathered from slot order before it, put back after.
 __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
 {
-    const uint32_t j = die_block() * kBlock + threadIdx.x;
+    const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     if (!live(a, pid)) return;  // a scene left out of the step may not be in the slots stepped
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_soft_gather(SoftArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_soft_scatter(SoftArgs a)
 {
-    const uint32_t j = die_block() * kBlock + threadIdx.x;
+    const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     if (!live(a, pid)) return;
