@@ -241,6 +241,7 @@ void free_scene(SceneBuffers &s)
     dfree(s.slot_of);
     dfree(s.posi);
     dfree(s.soft_ids);
+    dfree(s.rigid_pick);
     dfree(s.edges);
     dfree(s.bends);
     dfree(s.dihedrals);
@@ -271,8 +272,10 @@ void free_constraints(SceneBuffers &s)
     dfree(s.inc);
     dfree(s.posi);
     dfree(s.soft_ids);
+    dfree(s.rigid_pick);
     s.posi = nullptr;
     s.soft_ids = nullptr;
+    s.rigid_pick = nullptr;
     s.nsoft = 0;
     s.edges = nullptr;
     s.bends = nullptr;
@@ -1432,6 +1435,7 @@ cn_status upload_pool(cn_ctx *ctx, uint32_t count, const cn_scene_desc *descs)
             }
             s.nsoft = (uint32_t)soft.size();
             CN_HIP(ctx, dalloc(&s.soft_ids, soft.size()));
+            CN_HIP(ctx, dalloc(&s.rigid_pick, soft.size()));
             if (!soft.empty())
                 CN_HIP(ctx, hipMemcpy(s.soft_ids, soft.data(), soft.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             CN_HIP(ctx, dalloc(&s.posi, n));
@@ -1590,7 +1594,7 @@ void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
                 const bool gather = it == 0 || has_contacts, scatter = it + 1 == pid_iters || has_contacts;
                 if (rigid_pid && it < contact_iters) {
                     launch_soft_iteration(sa, gather, false, it < soft_iters, soft_st);
-                    launch_rigid_pid(s, c, soft_st);
+                    launch_rigid_pid(s, c, it == 0, soft_st);
                     if (scatter) launch_soft_iteration(sa, false, true, false, soft_st);
                 } else {
                     launch_soft_iteration(sa, gather, scatter, it < soft_iters, soft_st);

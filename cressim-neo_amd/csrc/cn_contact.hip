@@ -534,25 +534,44 @@ struct RigidPidArgs {
     const ParticleClass *classes;
     const float4 *part_mats;
     float4 *rgen;             // slot order
+    uint32_t *pick;           // per constrained particle: kPickSkip, kPickNone or its slot
     uint32_t npid;
 };
 
-__global__ __launch_bounds__(kBlock) void k_rigid_pid(RigidPidArgs a, SubstepConsts c)
+// What the iterations of a substep do for a constrained particle, fixed for the substep (slot,
+// class, collider candidates): nothing (its scene is not stepped or not solved here), add a zero
+// sum (no candidates), or solve the candidates of its slot.
+constexpr uint32_t kPickSkip = 0xffffffffu, kPickNone = 0xfffffffeu;
+
+__global__ __launch_bounds__(kBlock) void k_rigid_pick(RigidPidArgs a, SubstepConsts c)
 {
     const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (j >= a.nsoft) return;
     const uint32_t pid = a.soft_ids[j];
     const uint32_t scene = a.pid_scene[pid];
-    if (!a.active[scene]) return;
-    const SceneInfo &sc = a.scenes[scene];
-    if (!sc.rigid_pid) return;
+    uint32_t v = kPickSkip;
+    if (a.active[scene] && a.scenes[scene].rigid_pid) {
+        const uint32_t t = a.slot_of[min(pid, a.npid - 1u)];
+        const ParticleClass &pc = a.classes[c.uniform_class ? 0u : a.cls[t]];
+        v = pc.kind == CN_PARTICLE_KIND_SOLID && a.bnd[t] != 0u ? t : kPickNone;
+    }
+    a.pick[j] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rigid_pid(RigidPidArgs a, SubstepConsts c)
+{
+    const uint32_t j = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (j >= a.nsoft) return;
+    const uint32_t v = a.pick[j];
+    if (v == kPickSkip) return;
+    const uint32_t pid = a.soft_ids[j];
     float4 p = a.posi[pid];
     if (p.w <= kEps) return;
-    const uint32_t t = a.slot_of[min(pid, a.npid - 1u)];
-    const ParticleClass &pc = a.classes[c.uniform_class ? 0u : a.cls[t]];
-    const uint32_t rm = pc.kind == CN_PARTICLE_KIND_SOLID ? a.bnd[t] : 0u;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    if (rm) {
+    if (v != kPickNone) {
+        const uint32_t t = v, rm = a.bnd[t];
+        const ParticleClass &pc = a.classes[c.uniform_class ? 0u : a.cls[t]];
+        const SceneInfo &sc = a.scenes[a.pid_scene[pid]];
         a.rgen[t] = p;
         const float4 pv = a.prev[t];
         const float4 pm = a.part_mats[pc.part_mat];
@@ -629,7 +648,7 @@ void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipS
     hipLaunchKernelGGL(k_contact_velocity_apply, dim3(grid_c(c.n)), dim3(kBlock), 0, st, a);
 }
 
-void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st)
+void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, bool pick, hipStream_t st)
 {
     if (!s.nsoft) return;
     RigidPidArgs a;
@@ -648,7 +667,9 @@ void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t
     a.classes = s.classes;
     a.part_mats = s.part_mats;
     a.rgen = s.ct_gen;
+    a.pick = s.rigid_pick;
     a.npid = s.npid;
+    if (pick) hipLaunchKernelGGL(k_rigid_pick, dim3(grid_c(s.nsoft)), dim3(kBlock), 0, st, a, c);
     hipLaunchKernelGGL(k_rigid_pid, dim3(grid_c(s.nsoft)), dim3(kBlock), 0, st, a, c);
 }
 

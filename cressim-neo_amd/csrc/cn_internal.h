@@ -185,6 +185,7 @@ struct SceneBuffers {
     float4 *posi = nullptr;         // positions of the constrained particles in particle-index order
                                     // during the soft phase of an iteration
     uint32_t *soft_ids = nullptr;   // the constrained particles, ascending
+    uint32_t *rigid_pick = nullptr; // per constrained particle: its particle-rigid work of the substep
     uint32_t nsoft = 0;
     EdgeDev *edges = nullptr;
     BendDev *bends = nullptr;
@@ -224,8 +225,9 @@ void launch_contact_partners(const SceneBuffers &s, const SubstepConsts &c, hipS
 void launch_contact_iteration(const SceneBuffers &s, const SubstepConsts &c, int din, hipStream_t st);
 void launch_contact_velocity(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
 void launch_rigid_velocity(const SceneBuffers &s, const SubstepConsts &c, uint32_t iters, hipStream_t st);
-// the particle-rigid contacts of scenes solved in particle-index order, on the soft phase's positions
-void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, hipStream_t st);
+// the particle-rigid contacts of scenes solved in particle-index order, on the soft phase's positions;
+// pick: first call of the substep (each particle's slot and candidates are looked up once)
+void launch_rigid_pid(const SceneBuffers &s, const SubstepConsts &c, bool pick, hipStream_t st);
 void launch_tick_gather(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
                         hipStream_t st);
 void launch_tick_scatter(const SceneBuffers &pool, const SceneBuffers &tick, const uint4 *seg, uint32_t nseg,
