@@ -1567,7 +1567,6 @@ void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
         if (has_soft) {
             sa.pos = s.pos[s.cur];
             sa.id = s.id[s.cur];
-            launch_soft_clear(sa, soft_st);
         }
         if (has_contacts) {
             ProfScope ps(ctx, "contact_partners");
@@ -1592,6 +1591,7 @@ void record_step(cn_ctx *ctx, SceneBuffers &s, const cn_step_params *p)
                 // positions stay in particle-index order from the first soft iteration to the last.
                 ProfScope ps(ctx, "soft");
                 const bool gather = it == 0 || has_contacts, scatter = it + 1 == pid_iters || has_contacts;
+                sa.first = it == 0 ? 1u : 0u;
                 if (rigid_pid && it < contact_iters) {
                     launch_soft_iteration(sa, gather, false, it < soft_iters, soft_st);
                     launch_rigid_pid(s, c, it == 0, soft_st);

@@ -266,8 +266,9 @@ struct SoftArgs {
     const uint32_t *active;    // per scene
     const uint32_t *pid_scene; // particle index -> scene
     uint32_t all_live;         // every scene takes part in the step (no per-particle check)
+    uint32_t first;            // first iteration of a substep: every lambda starts at zero (read as
+                               // zero, and written by every constraint, also where it does nothing)
 };
-void launch_soft_clear(const SoftArgs &a, hipStream_t st);
 // gather / scatter: copy the constrained particles' positions from slot order before the solves and
 // back after them (needed only around iterations where another pass moves those particles)
 // solve = false: only the gather and / or the scatter

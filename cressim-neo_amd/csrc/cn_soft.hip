@@ -44,14 +44,6 @@ __device__ __forceinline__ void cross3d(float ax, float ay, float az, float bx, 
     oz = ax * by - ay * bx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_soft_clear(SoftArgs a)
-{
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < a.ne) a.lam_e[i] = 0.0f;
-    if (i < a.nb + a.nd) a.lam_b[i] = 0.0f;
-    if (i < a.nt) a.lam_t[i] = 0.0f;
-}
-
 __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
 {
     const uint32_t e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -63,6 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     if (E.compliance != E.compliance) {  // disabled
         out[0] = z4;
         out[1] = z4;
+        if (a.first) a.lam_e[e] = 0.0f;
         return;
     }
     const float4 pa = ldp(a, E.a), pb = ldp(a, E.b);
@@ -70,6 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     if (wsum <= kEps) {
         out[0] = z4;
         out[1] = z4;
+        if (a.first) a.lam_e[e] = 0.0f;
         return;
     }
     const float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
@@ -77,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     if (l2 <= kEps) {
         out[0] = z4;
         out[1] = z4;
+        if (a.first) a.lam_e[e] = 0.0f;
         return;
     }
     const float len = fsqrt(l2);
@@ -84,11 +79,12 @@ __global__ __launch_bounds__(kBlock) void k_solve_edges(SoftArgs a)
     const float gbx = -gax, gby = -gay, gbz = -gaz;
     const float alpha = fdiv(E.compliance, fmaxd(a.dt * a.dt, kEps));
     const float C = len - E.rest_length;
-    const float lambda = a.lam_e[e];
+    const float lambda = a.first ? 0.0f : a.lam_e[e];
     const float den = wsum + alpha;
     if (den <= kEps) {
         out[0] = z4;
         out[1] = z4;
+        if (a.first) a.lam_e[e] = 0.0f;
         return;
     }
     const float dl = fdiv(-(C + alpha * lambda), den);
@@ -138,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
             zero = den + alpha <= kEps;
             if (!zero) {
                 const float C = theta - B.rest_angle;
-                const float lambda = a.lam_b[b];
+                const float lambda = a.first ? 0.0f : a.lam_b[b];
                 dl = fdiv(-(C + alpha * lambda), den + alpha);
                 a.lam_b[b] = lambda + dl;
             }
@@ -146,6 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_bends(SoftArgs a)
     }
     if (zero) {
         out[0] = out[1] = out[2] = out[3] = z4;
+        if (a.first) a.lam_b[b] = 0.0f;
         return;
     }
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, r = a.relax;
@@ -174,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const float n0s = dot3(n0x, n0y, n0z, n0x, n0y, n0z), n1s = dot3(n1x, n1y, n1z, n1x, n1y, n1z);
     if (el2 <= kEps || n0s <= kEps || n1s <= kEps) {
         out[0] = out[1] = out[2] = out[3] = z4;
+        if (a.first) a.lam_b[ci] = 0.0f;
         return;
     }
     const float el = fsqrt(el2);
@@ -204,9 +202,10 @@ __global__ __launch_bounds__(kBlock) void k_solve_dihedrals(SoftArgs a)
     const float alpha = fdiv(D.compliance, fmaxd(a.dt * a.dt, kEps));
     if (den + alpha <= kEps) {
         out[0] = out[1] = out[2] = out[3] = z4;
+        if (a.first) a.lam_b[ci] = 0.0f;
         return;
     }
-    const float lambda = a.lam_b[ci];
+    const float lambda = a.first ? 0.0f : a.lam_b[ci];
     const float dl = fdiv(-(C + alpha * lambda), den + alpha);
     a.lam_b[ci] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
@@ -224,7 +223,10 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     if (!live(a, T.p[0])) return;
     const float4 p0 = ldp(a, T.p[0]), p1 = ldp(a, T.p[1]), p2 = ldp(a, T.p[2]), p3 = ldp(a, T.p[3]);
     const float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
-    if ((w0 + w1 + w2 + w3) <= kEps) return;  // the record keeps its previous value
+    if ((w0 + w1 + w2 + w3) <= kEps) {  // the record keeps its previous value
+        if (a.first) a.lam_t[t] = 0.0f;
+        return;
+    }
     float cx, cy, cz;
     cross3d(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, cx, cy, cz);
     const float sv = fdiv(dot3(cx, cy, cz, p3.x - p0.x, p3.y - p0.y, p3.z - p0.z), 6.0f);
@@ -243,9 +245,12 @@ __global__ __launch_bounds__(kBlock) void k_solve_tets(SoftArgs a)
     const float sg = w0 * dot3(g0x, g0y, g0z, g0x, g0y, g0z) + w1 * dot3(g1x, g1y, g1z, g1x, g1y, g1z) +
                      w2 * dot3(g2x, g2y, g2z, g2x, g2y, g2z) + w3 * dot3(g3x, g3y, g3z, g3x, g3y, g3z);
     const float alpha = fdiv(T.compliance, fmaxd(a.dt * a.dt, kEps));
-    const float lambda = a.lam_t[t];
+    const float lambda = a.first ? 0.0f : a.lam_t[t];
     const float den = sg + alpha;
-    if (den <= kEps) return;
+    if (den <= kEps) {
+        if (a.first) a.lam_t[t] = 0.0f;
+        return;
+    }
     const float dl = fdiv(-(C + alpha * lambda), den);
     a.lam_t[t] = lambda + dl;
     const float f0 = w0 * dl, f1 = w1 * dl, f2 = w2 * dl, f3 = w3 * dl, r = a.relax;
@@ -305,12 +310,6 @@ __global__ __launch_bounds__(kBlock) void k_soft_scatter(SoftArgs a)
     const uint32_t pid = a.soft_ids[j];
     if (!live(a, pid)) return;
     a.pos[a.slot_of[pid]] = a.posi[pid];
-}
-
-void launch_soft_clear(const SoftArgs &a, hipStream_t st)
-{
-    const uint32_t m = std::max(std::max(a.ne, a.nb + a.nd), a.nt);
-    if (m) hipLaunchKernelGGL(k_soft_clear, dim3(grid_n(m)), dim3(kBlock), 0, st, a);
 }
 
 void launch_soft_iteration(const SoftArgs &a, bool gather, bool scatter, bool solve, hipStream_t st)
