@@ -1878,6 +1878,23 @@ static cn_status step_impl(cn_ctx *ctx, const cn_step_params *p)
         CN_HIP(ctx, dalloc(&ctx->s.disp, std::max<uint32_t>(ctx->capacity, 1u)));
         ctx->graph_valid = false;
     }
+    // the vorticity and anisotropy passes' neighbor lists on first use (the pass entry's list
+    // buffer, sized for every slot and the reserved neighbor count)
+    if (ctx->has_fluid && (ctx->has_vorticity || ctx->aniso_enabled) &&
+        ctx->s.nbr_alloc < (size_t)ctx->capacity * std::max<uint32_t>(ctx->cap_alloc, 1u)) {
+        CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SceneBuffers &s = ctx->s;
+        dfree(s.nbr);
+        dfree(s.nbr_cnt);
+        s.nbr = nullptr;
+        s.nbr_cnt = nullptr;
+        s.nbr_alloc = 0;
+        const size_t slots = (size_t)ctx->capacity * std::max<uint32_t>(ctx->cap_alloc, 1u);
+        CN_HIP(ctx, dalloc(&s.nbr, slots));
+        CN_HIP(ctx, dalloc(&s.nbr_cnt, std::max<uint32_t>(ctx->capacity, 1u)));
+        s.nbr_alloc = slots;
+        ctx->graph_valid = false;
+    }
     if (ctx->aniso_enabled && ctx->s.aniso_alloc < ctx->s.n) {  // output on first use
         CN_HIP(ctx, hipStreamSynchronize(ctx->stream));
         dfree(ctx->s.aniso);

@@ -1424,9 +1424,11 @@ __global__ __launch_bounds__(kBlock) void k_fluid_lists(FluidArgs a, SubstepCons
 // Vorticity confinement after the velocity update: physics_fluid_compute_vorticity.cs.hlsl:16-81
 // (APPLY = false: vort[i] = (curl, |curl|), zero for particles that do not confine) and
 // physics_fluid_apply_vorticity_confinement.cs.hlsl:16-94 (APPLY = true).  Both walk the
-// post-update neighbor list (physics_solver.cpp:1262) instead of storing it: the first cap
-// in-radius neighbors in walk order on the final positions, of which the same-material ones
-// contribute.  Inactive scenes are skipped whole (their neighbors are their own).
+// post-update neighbor list (physics_solver.cpp:1262): the first cap in-radius neighbors in walk
+// order on the final positions, of which the same-material ones contribute.  The curl pass walks
+// the run table and, where a list buffer is given (nbr, neighbor-major), stores the list; the
+// confinement pass then reads it instead of walking again.  Inactive scenes are skipped whole
+// (their neighbors are their own).
 template <bool UNIFORM, bool APPLY>
 __global__ __launch_bounds__(kBlock) void k_fluid_vorticity(FluidArgs a, SubstepConsts c, float4 *__restrict__ vel,
                                                             float4 *__restrict__ vort)
@@ -1452,38 +1454,53 @@ __global__ __launch_bounds__(kBlock) void k_fluid_vorticity(FluidArgs a, Substep
     const float h = fmaxd(m.smoothing_radius, 1.0e-4f);
     const float t2 = a.t2[pc.fluid_mat];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    uint32_t cnt = 0;
-    if (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc)))
-        walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
-            if (q == i) return true;
-            const uint32_t qmat = UNIFORM ? pc.fluid_mat : a.classes[a.cls[q]].fluid_mat;
-            if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
-            const float4 pj = a.pos[q];
-            if (pj.w <= kEps) return true;
-            const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
-            if (!(dot3(dx, dy, dz, dx, dy, dz) < t2)) return true;
-            if (qmat == pc.fluid_mat) {
-                const float dist = len3(dx, dy, dz);
-                if (!(dist <= kEps || dist >= h)) {
-                    // FluidSpikyKernelGradient, physics_fluid_common.hlsli:45-54
-                    const float dw = spiky_dw(dist, h);
-                    const float gx = dw * fdiv(dx, dist), gy = dw * fdiv(dy, dist), gz = dw * fdiv(dz, dist);
-                    if (APPLY) {
-                        const float mj = vort[q].w;
-                        ax = ax + mj * gx;
-                        ay = ay + mj * gy;
-                        az = az + mj * gz;
-                    } else {
-                        const float4 vj = vel[q];
-                        const float wx = vj.x - self.x, wy = vj.y - self.y, wz = vj.z - self.z;
-                        ax = ax + (wy * gz - wz * gy);
-                        ay = ay + (wz * gx - wx * gz);
-                        az = az + (wx * gy - wy * gx);
-                    }
-                }
+    // one listed neighbor's term (same material, inside the kernel's support)
+    auto term = [&](uint32_t q, uint32_t qmat, float dx, float dy, float dz) {
+        if (qmat != pc.fluid_mat) return;
+        const float dist = len3(dx, dy, dz);
+        if (dist <= kEps || dist >= h) return;
+        // FluidSpikyKernelGradient, physics_fluid_common.hlsli:45-54
+        const float dw = spiky_dw(dist, h);
+        const float gx = dw * fdiv(dx, dist), gy = dw * fdiv(dy, dist), gz = dw * fdiv(dz, dist);
+        if (APPLY) {
+            const float mj = vort[q].w;
+            ax = ax + mj * gx;
+            ay = ay + mj * gy;
+            az = az + mj * gz;
+        } else {
+            const float4 vj = vel[q];
+            const float wx = vj.x - self.x, wy = vj.y - self.y, wz = vj.z - self.z;
+            ax = ax + (wy * gz - wz * gy);
+            ay = ay + (wz * gx - wx * gz);
+            az = az + (wx * gy - wy * gx);
+        }
+    };
+    if (c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc))) {
+        if (APPLY && a.nbr) {
+            // the list the curl pass wrote: the same neighbors in the same order (positions unchanged)
+            const uint32_t cnt = a.nbr_cnt[i];
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const uint32_t q = a.nbr[(size_t)k * c.n + i];
+                const float4 pj = a.pos[q];
+                term(q, UNIFORM ? pc.fluid_mat : a.classes[a.cls[q]].fluid_mat, s.x - pj.x, s.y - pj.y, s.z - pj.z);
             }
-            return ++cnt < c.cap;
-        });
+        } else {
+            uint32_t cnt = 0;
+            walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
+                if (q == i) return true;
+                const uint32_t qmat = UNIFORM ? pc.fluid_mat : a.classes[a.cls[q]].fluid_mat;
+                if (!UNIFORM && !neighbor_ok(pc, a.classes[a.cls[q]])) return true;
+                const float4 pj = a.pos[q];
+                if (pj.w <= kEps) return true;
+                const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
+                if (!(dot3(dx, dy, dz, dx, dy, dz) < t2)) return true;
+                if (!APPLY && a.nbr) a.nbr[(size_t)cnt * c.n + i] = q;  // for the confinement pass
+                term(q, qmat, dx, dy, dz);
+                return ++cnt < c.cap;
+            });
+            if (!APPLY && a.nbr) a.nbr_cnt[i] = cnt;
+        }
+    }
     if (!APPLY) {
         vort[i] = make_float4(ax, ay, az, len3(ax, ay, az));
         return;
@@ -1576,7 +1593,27 @@ __global__ __launch_bounds__(kBlock) void k_fluid_render_aniso(FluidArgs a, Subs
         const float t2 = a.t2[pc.fluid_mat];
         const bool walk = s.w > kEps && c.cap > 0 && (!UNIFORM || neighbor_ok(pc, pc));
         // f(neighbor position, weight) per contributing entry of the list
-        auto each = [&](auto &&f) {
+        auto weigh = [&](const float4 &pj, float d2, auto &&f) {
+            if (d2 > 0.0f) {
+                const float dist = fsqrt(d2);
+                if (dist < support) {
+                    const float x = dist * inv;
+                    f(pj, 1.0f - x * x * x);
+                }
+            }
+        };
+        // first = true: walk the run table (storing the list where a list buffer is given);
+        // otherwise read that list back: the same neighbors in the same order
+        auto each = [&](bool first, auto &&f) {
+            if (!first && a.nbr) {
+                const uint32_t cnt = a.nbr_cnt[i];
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const float4 pj = a.pos[a.nbr[(size_t)k * c.n + i]];
+                    const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
+                    weigh(pj, dot3(dx, dy, dz, dx, dy, dz), f);
+                }
+                return;
+            }
             uint32_t cnt = 0;
             walk_runs(a.cellc[i], run_table(a), c, [&](uint32_t q) {
                 if (q == i) return true;
@@ -1586,19 +1623,15 @@ __global__ __launch_bounds__(kBlock) void k_fluid_render_aniso(FluidArgs a, Subs
                 const float dx = s.x - pj.x, dy = s.y - pj.y, dz = s.z - pj.z;
                 const float d2 = dot3(dx, dy, dz, dx, dy, dz);
                 if (!(d2 < t2)) return true;
-                if (d2 > 0.0f) {
-                    const float dist = fsqrt(d2);
-                    if (dist < support) {
-                        const float x = dist * inv;
-                        f(pj, 1.0f - x * x * x);
-                    }
-                }
+                if (first && a.nbr) a.nbr[(size_t)cnt * c.n + i] = q;
+                weigh(pj, d2, f);
                 return ++cnt < c.cap;
             });
+            if (first && a.nbr) a.nbr_cnt[i] = cnt;
         };
         float mx = 0.0f, my = 0.0f, mz = 0.0f, wsum = 0.0f;
         if (walk)
-            each([&](const float4 &pj, float w) {
+            each(true, [&](const float4 &pj, float w) {
                 wsum = wsum + w;
                 mx = mx + pj.x * w;
                 my = my + pj.y * w;
@@ -1609,7 +1642,7 @@ __global__ __launch_bounds__(kBlock) void k_fluid_render_aniso(FluidArgs a, Subs
             my = fdiv(my, wsum);
             mz = fdiv(mz, wsum);
             float C[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-            each([&](const float4 &pj, float w) {
+            each(false, [&](const float4 &pj, float w) {
                 const float cc[3] = {pj.x - mx, pj.y - my, pj.z - mz};
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
