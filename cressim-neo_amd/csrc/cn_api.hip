@@ -2115,7 +2115,10 @@ cn_status cn_autotune(cn_ctx *ctx, const cn_step_params *p, uint32_t steps, cn_t
             ms /= (float)steps;
             ++tried;
             if (tile == 4096 && grid == 1024) dflt = ms;
-            if (ms < best) {
+            // tiles are tried in ascending order; a larger one within 1 % of the best is kept: it
+            // sends no more dense blocks to the run-table walk, and a fluid that compresses after
+            // the timed steps grows such blocks
+            if (tile > best_tile && best < 3.4e38f ? ms < best * 1.01f : ms < best) {
                 best = ms;
                 best_tile = tile;
                 best_grid = grid;
